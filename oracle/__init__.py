@@ -1,0 +1,373 @@
+"""CPU oracle for the multilevel spectral bisection path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke entry point and by
+the CPU-baseline leg of bench.py, always as the checker.  The product package
+never imports it.
+
+``oracle.core`` wraps liboracle.so (the C restatement in sgpar_oracle.c);
+``oracle.ref`` wraps oracle/_ref/libsgpar_ref.so, the reference's own
+non-Kokkos sgpar.h compiled from the reference tree, when it was built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsgpar_ref.so")
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(with_ref: bool = True) -> None:
+    """Compile the oracle (and the reference wrapper when its sources exist)."""
+    target = "all" if with_ref else "oracle"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+def _load(path: str) -> C.CDLL:
+    if not os.path.exists(path):
+        build(with_ref=False)
+    return C.CDLL(path)
+
+
+class _Core:
+    def __init__(self) -> None:
+        L = _load(ORACLE_SO)
+        self.L = L
+        L.orc_pcg32.restype = C.c_uint32
+        L.orc_pcg32.argtypes = [C.POINTER(C.c_uint64), C.c_uint64]
+        L.orc_xs_step.restype = C.c_uint64
+        L.orc_xs_step.argtypes = [C.c_uint64]
+        L.orc_xs_urand64.restype = C.c_uint64
+        L.orc_xs_urand64.argtypes = [C.POINTER(C.c_uint64)]
+        L.orc_pool_state.restype = C.c_uint64
+        L.orc_pool_state.argtypes = [C.c_uint64]
+        L.orc_generate_permutation.restype = None
+        L.orc_generate_permutation.argtypes = [C.c_uint32, C.POINTER(C.c_uint64), u32p, u64p]
+        L.orc_heavy_neighbor.restype = None
+        L.orc_heavy_neighbor.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p,
+                                         C.POINTER(C.c_uint64), u32p]
+        L.orc_map_construct.restype = C.c_uint32
+        L.orc_map_construct.argtypes = [C.c_uint32, u32p, u32p, u32p, u32p, C.POINTER(C.c_int)]
+        L.orc_coarsen_hec.restype = C.c_uint32
+        L.orc_coarsen_hec.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64,
+                                      u32p, u32p, u32p]
+        L.orc_build_coarse.restype = C.c_uint32
+        L.orc_build_coarse.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, u32p, C.c_uint32,
+                                       C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_void_p), u32p]
+        L.orc_free.restype = None
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_hierarchy_build.restype = C.c_void_p
+        L.orc_hierarchy_build.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64]
+        L.orc_hier_nlevels.restype = C.c_int
+        L.orc_hier_nlevels.argtypes = [C.c_void_p]
+        L.orc_hier_level.restype = None
+        L.orc_hier_level.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 7
+        L.orc_hier_free.restype = None
+        L.orc_hier_free.argtypes = [C.c_void_p]
+        L.orc_power_iter.restype = C.c_uint64
+        L.orc_power_iter.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, f64p, C.c_int,
+                                     C.c_double, C.c_uint64, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
+        L.orc_eigensolve.restype = C.c_int
+        L.orc_eigensolve.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.c_int,
+                                     C.c_double, C.c_uint64, f64p, u64p]
+        L.orc_compute_partition.restype = C.c_int
+        L.orc_compute_partition.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_int, C.c_int,
+                                            u32p, C.POINTER(C.c_uint32)]
+        L.orc_partition_graph.restype = C.c_int
+        L.orc_partition_graph.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64,
+                                          C.POINTER(C.c_uint64), C.c_uint64, C.c_int, C.c_double,
+                                          u32p, C.POINTER(C.c_uint32), u32p, u64p, f64p]
+
+
+_core = None
+
+
+def core() -> _Core:
+    global _core
+    if _core is None:
+        _core = _Core()
+    return _core
+
+
+def _wptr(w):
+    if w is None:
+        return None, None
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    return w, w.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- RNGs
+def pcg32_stream(state: int, count: int, inc: int = 1):
+    """`count` PCG32 outputs from (state, inc); returns (values, new_state)."""
+    L = core().L
+    st = C.c_uint64(state)
+    out = np.empty(count, dtype=np.uint32)
+    for i in range(count):
+        out[i] = L.orc_pcg32(C.byref(st), inc)
+    return out, st.value
+
+
+def pool_state(seed: int) -> int:
+    return core().L.orc_pool_state(seed)
+
+
+def xs_step(s: int) -> int:
+    return core().L.orc_xs_step(s)
+
+
+def urand64_stream(state: int, count: int):
+    """`count` urand64 draws from a pool state; returns (values, new_state)."""
+    L = core().L
+    st = C.c_uint64(state)
+    out = np.empty(count, dtype=np.uint64)
+    for i in range(count):
+        out[i] = L.orc_xs_urand64(C.byref(st))
+    return out, st.value
+
+
+# ---------------------------------------------------------------- HEC
+def generate_permutation(n: int, seed: int):
+    """(vperm, keys, state_after) for the pool of `seed`."""
+    L = core().L
+    st = C.c_uint64(pool_state(seed))
+    vperm = np.empty(n, dtype=np.uint32)
+    keys = np.empty(n, dtype=np.uint64)
+    L.orc_generate_permutation(n, C.byref(st), vperm, keys)
+    return vperm, keys, st.value
+
+
+def heavy_neighbor(level: int, rowptr, col, w, state: int):
+    L = core().L
+    n = len(rowptr) - 1
+    st = C.c_uint64(state)
+    hn = np.empty(n, dtype=np.uint32)
+    w, wp = _wptr(w)
+    L.orc_heavy_neighbor(level, n, rowptr, col, wp, C.byref(st), hn)
+    return hn, st.value
+
+
+def map_construct(vperm, hn):
+    """Literal sequential model of the map construction; (vcmap, nc, passes)."""
+    L = core().L
+    vperm = np.ascontiguousarray(vperm, dtype=np.uint32)
+    hn = np.ascontiguousarray(hn, dtype=np.uint32)
+    n = len(vperm)
+    pos = np.empty(n, dtype=np.uint32)
+    pos[vperm] = np.arange(n, dtype=np.uint32)
+    vcmap = np.empty(n, dtype=np.uint32)
+    passes = C.c_int(0)
+    nc = L.orc_map_construct(n, vperm, hn, pos, vcmap, C.byref(passes))
+    return vcmap, int(nc), passes.value
+
+
+def coarsen_hec(level: int, rowptr, col, w, seed: int):
+    """One HEC level; returns (vcmap, nc, vperm, hn)."""
+    L = core().L
+    n = len(rowptr) - 1
+    vcmap = np.empty(n, dtype=np.uint32)
+    vperm = np.empty(n, dtype=np.uint32)
+    hn = np.empty(n, dtype=np.uint32)
+    w, wp = _wptr(w)
+    nc = L.orc_coarsen_hec(level, n, rowptr, col, wp, seed, vcmap, vperm, hn)
+    return vcmap, int(nc), vperm, hn
+
+
+def build_coarse(rowptr, col, w, vcmap, nc: int, vtx_w=None):
+    """Coarse CSR (rowptr_c, col_c, w_c, c_vtx_w) with sorted, merged rows."""
+    L = core().L
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vw, vwp = _wptr(vtx_w)
+    rp, cp, wc = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    cvw = np.zeros(nc + 1, dtype=np.uint32)
+    mc = L.orc_build_coarse(n, rowptr, col, wp, np.ascontiguousarray(vcmap, dtype=np.uint32),
+                            nc, vwp, C.byref(rp), C.byref(cp), C.byref(wc), cvw)
+    r = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint32)), shape=(nc + 1,)).copy()
+    c = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_uint32)), shape=(max(mc, 1),))[:mc].copy()
+    ww = np.ctypeslib.as_array(C.cast(wc, C.POINTER(C.c_uint32)), shape=(max(mc, 1),))[:mc].copy()
+    for p in (rp, cp, wc):
+        L.orc_free(p)
+    return r, c, ww, cvw[:nc]
+
+
+class Hierarchy:
+    """Multilevel hierarchy from the oracle (level 0 = finest)."""
+
+    def __init__(self, rowptr, col, w, seed: int):
+        L = core().L
+        self._L = L
+        n = len(rowptr) - 1
+        w, wp = _wptr(w)
+        self._h = L.orc_hierarchy_build(n, rowptr, col, wp, seed)
+        self.nlevels = L.orc_hier_nlevels(self._h)
+        self.levels = []
+        for l in range(self.nlevels):
+            nn, mm = C.c_uint32(), C.c_uint32()
+            rp, cp, wp2, vp, vwp = (C.c_void_p() for _ in range(5))
+            L.orc_hier_level(self._h, l, C.byref(nn), C.byref(mm), C.byref(rp), C.byref(cp),
+                             C.byref(wp2), C.byref(vp), C.byref(vwp))
+            n_, m_ = nn.value, mm.value
+
+            def arr(p, k):
+                if not p.value or k == 0:
+                    return np.zeros(0, dtype=np.uint32)
+                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(k,)).copy()
+
+            self.levels.append(dict(n=n_, m=m_, rowptr=arr(rp, n_ + 1), col=arr(cp, m_),
+                                    w=arr(wp2, m_), vcmap=arr(vp, n_) if vp.value else None,
+                                    vtx_w=arr(vwp, n_)))
+
+    def eigensolve(self, pcg_state: int, pcg_inc: int = 1, normLap: int = 0,
+                   tol: float = 1e-10, iter_cap: int = 0):
+        """(evec, iters per level (0 = finest), new pcg state)."""
+        st = C.c_uint64(pcg_state)
+        n0 = self.levels[0]["n"]
+        ev = np.empty(n0, dtype=np.float64)
+        iters = np.zeros(self.nlevels, dtype=np.uint64)
+        self._L.orc_eigensolve(self._h, C.byref(st), pcg_inc, normLap, tol, iter_cap, ev, iters)
+        return ev, iters, st.value
+
+    def __del__(self):
+        try:
+            self._L.orc_hier_free(self._h)
+        except Exception:
+            pass
+
+
+def power_iter(rowptr, col, w, u, normLap: int = 0, tol: float = 1e-10, iter_cap: int = 0):
+    """Runs the power iteration in place on a copy of u; (u, niter, max_reached, dot)."""
+    L = core().L
+    n = len(rowptr) - 1
+    u = np.array(u, dtype=np.float64, copy=True)
+    w, wp = _wptr(w)
+    mr = C.c_int(0)
+    d = C.c_double(0)
+    it = L.orc_power_iter(n, rowptr, col, wp, u, normLap, tol, iter_cap, C.byref(mr), C.byref(d))
+    return u, int(it), bool(mr.value), d.value
+
+
+def compute_partition(rowptr, col, evec, num_partitions: int = 2, imbalance: int = 0):
+    L = core().L
+    n = len(rowptr) - 1
+    part = np.empty(n, dtype=np.uint32)
+    ec = C.c_uint32(0)
+    L.orc_compute_partition(n, rowptr, col, np.ascontiguousarray(evec, dtype=np.float64),
+                            num_partitions, imbalance, part, C.byref(ec))
+    return part, ec.value
+
+
+def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
+                    normLap: int = 0, tol: float = 1e-10):
+    """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state)."""
+    L = core().L
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    st = C.c_uint64(pcg_state)
+    part = np.empty(n, dtype=np.uint32)
+    ec = C.c_uint32(0)
+    lv = np.zeros(210, dtype=np.uint32)
+    it = np.zeros(210, dtype=np.uint64)
+    ev = np.empty(n, dtype=np.float64)
+    nl = L.orc_partition_graph(n, rowptr, col, wp, hec_seed, C.byref(st), pcg_inc, normLap, tol,
+                               part, C.byref(ec), lv, it, ev)
+    return dict(part=part, edge_cut=ec.value, levels_n=lv[:nl].copy(), iters=it[:nl].copy(),
+                evec=ev, pcg_state=st.value)
+
+
+# ---------------------------------------------------------------- reference (_ref)
+class _Ref:
+    def __init__(self) -> None:
+        L = C.CDLL(REF_SO)
+        self.L = L
+        L.ref_pcg32.restype = C.c_uint32
+        L.ref_pcg32.argtypes = [C.POINTER(C.c_uint64), C.c_uint64]
+        for f in (L.ref_heap_dedupe, L.ref_hashmap_dedupe):
+            f.restype = None
+            f.argtypes = [u32p, u32p, u32p, u32p, C.POINTER(C.c_uint32)]
+        L.ref_power_iter.restype = C.c_long
+        L.ref_power_iter.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, f64p, C.c_int, C.c_int,
+                                     C.c_double]
+        L.ref_build_coarse.restype = C.c_long
+        L.ref_build_coarse.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, u32p, C.c_uint32,
+                                       u32p, u32p, u32p]
+        L.ref_compute_partition.restype = C.c_int
+        L.ref_compute_partition.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_int, C.c_int, u32p,
+                                            C.POINTER(C.c_uint32)]
+        L.ref_load_graph.restype = C.c_void_p
+        L.ref_load_graph.argtypes = [C.c_char_p]
+        L.ref_graph_info.restype = None
+        L.ref_graph_info.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.ref_graph_free.restype = None
+        L.ref_graph_free.argtypes = [C.c_void_p]
+        L.ref_load_config.restype = C.c_int
+        L.ref_load_config.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ref_use_partition.restype = C.c_int
+        L.ref_use_partition.argtypes = [C.c_uint32, u32p, u32p, u32p, C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p)]
+        L.ref_edit_distance.restype = C.c_int
+        L.ref_edit_distance.argtypes = [u32p, u32p, C.c_uint32, C.POINTER(C.c_uint32)]
+
+    def graph_arrays(self, h):
+        n, ne = C.c_uint32(), C.c_uint32()
+        rp, cp = C.c_void_p(), C.c_void_p()
+        self.L.ref_graph_info(h, C.byref(n), C.byref(ne), C.byref(rp), C.byref(cp))
+        nn = n.value
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint32)), shape=(nn + 1,)).copy()
+        m = int(rowptr[-1])
+        col = (np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_uint32)), shape=(m,)).copy()
+               if m else np.zeros(0, dtype=np.uint32))
+        return rowptr, col
+
+    def pcg32_stream(self, state: int, count: int, inc: int = 1):
+        st = C.c_uint64(state)
+        out = np.empty(count, dtype=np.uint32)
+        for i in range(count):
+            out[i] = self.L.ref_pcg32(C.byref(st), inc)
+        return out, st.value
+
+    def power_iter(self, rowptr, col, w, u, normLap=0, final_level=0, tol=1e-10):
+        u = np.array(u, dtype=np.float64, copy=True)
+        w, wp = _wptr(w)
+        it = self.L.ref_power_iter(len(rowptr) - 1, rowptr, col, wp, u, normLap, final_level, tol)
+        return u, int(it)
+
+    def build_coarse(self, rowptr, col, w, vcmap, nc):
+        m = int(rowptr[-1])
+        w, wp = _wptr(w)
+        orow = np.zeros(nc + 1, dtype=np.uint32)
+        ocol = np.zeros(max(m, 1), dtype=np.uint32)
+        ow = np.zeros(max(m, 1), dtype=np.uint32)
+        mc = self.L.ref_build_coarse(len(rowptr) - 1, rowptr, col, wp,
+                                     np.array(vcmap, dtype=np.uint32), nc, orow, ocol, ow)
+        return orow, ocol[:mc], ow[:mc]
+
+    def compute_partition(self, rowptr, col, evec, num_partitions=2, imbalance=0):
+        n = len(rowptr) - 1
+        part = np.empty(n, dtype=np.uint32)
+        ec = C.c_uint32(0)
+        rc = self.L.ref_compute_partition(n, rowptr, col, np.ascontiguousarray(evec, np.float64),
+                                          num_partitions, imbalance, part, C.byref(ec))
+        return rc, part, ec.value
+
+
+_ref = None
+
+
+def ref():
+    """The reference wrapper, or None when oracle/_ref was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        _ref = _Ref()
+    return _ref

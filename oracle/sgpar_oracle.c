@@ -1,0 +1,675 @@
+/*
+ * sgpar_oracle.c -- CPU restatement of the reference multilevel spectral
+ * bisection path (HEC coarsening, coarse-graph build, Fiedler power
+ * iteration, sort-and-split bisection) with the semantics of the reference's
+ * Kokkos *Serial* backend run.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py load it.  The product path
+ * (ipdps_2021_graph_coarsening_amd/csrc) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Integer work is
+ * restated exactly; floating-point sums are restated in the reference's
+ * sequential (Serial backend) order so iteration counts and eigenvectors
+ * match that run bit for bit.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   - pcg32, compute_partition, power iteration, coarse build (as a per-row
+ *     multiset), dedupe sums: against the reference's own non-Kokkos
+ *     sgpar.h compiled from /root/reference into oracle/_ref.
+ *   - XorShift64 pool stream, BinSort permutation, heavy neighbour, map
+ *     construction, level loop: against the recorded reference Serial run
+ *     on the 256x256 grid with time() pinned to 12345 (level sizes,
+ *     per-level iteration counts and cut), plus mathematical checks
+ *     (full period of the xorshift transition, sortedness of the argsort,
+ *     brute-force heavy neighbour, validity of the cluster map).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef uint32_t vid_t;
+typedef uint32_t eid_t;
+typedef uint32_t wgt_t;
+#define ORC_INF 0xFFFFFFFFu
+#define ORC_VTX_CUTOFF 50u      /* definitions_kokkos.h:77-79 */
+#define ORC_MAX_GRAPHS 200u     /* coarsen_kokkos.h:1218 */
+#define ORC_POWERITER_ITER 100000000000000ULL /* definitions_kokkos.h:92 */
+
+/* ------------------------------------------------------------------ */
+/* PCG32 XSH-RR, definitions_kokkos.h:46-55                            */
+/* ------------------------------------------------------------------ */
+uint32_t orc_pcg32(uint64_t *state, uint64_t inc)
+{
+    uint64_t old = *state;
+    *state = old * 6364136223846793005ULL + (inc | 1u);
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
+/* ------------------------------------------------------------------ */
+/* Kokkos Random_XorShift64, Kokkos_Random.hpp:683-747                 */
+/* ------------------------------------------------------------------ */
+static const uint64_t XS_MULT = 2685821657736338717ULL;
+
+uint64_t orc_xs_step(uint64_t s)
+{
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return s;
+}
+
+/* urand64(), Kokkos_Random.hpp:712-718 */
+uint64_t orc_xs_urand64(uint64_t *s)
+{
+    *s = orc_xs_step(*s);
+    return (*s * XS_MULT) - 1u;
+}
+
+/* urand(), Kokkos_Random.hpp:702-710 */
+static uint32_t xs_urand(uint64_t *s)
+{
+    *s = orc_xs_step(*s);
+    return (uint32_t)(((*s) * XS_MULT) >> 16);
+}
+
+/* rand() = urand()/2, Kokkos_Random.hpp:746-747 */
+static int xs_rand(uint64_t *s) { return (int)(xs_urand(s) / 2u); }
+
+/*
+ * State of the single generator of Random_XorShift64_Pool(seed) on a backend
+ * whose concurrency() is 1 (Serial): Kokkos_Random.hpp:863-890 (init) and
+ * :894-898 (get_state builds a generator, whose constructor maps 0 to
+ * 1318319, :694-696).
+ */
+uint64_t orc_pool_state(uint64_t seed)
+{
+    if (seed == 0) seed = 1318319ULL;
+    uint64_t g = seed; /* generator ctor: nonzero already */
+    for (int i = 0; i < 17; i++) (void)xs_rand(&g);
+    int n1 = xs_rand(&g);
+    int n2 = xs_rand(&g);
+    int n3 = xs_rand(&g);
+    int n4 = xs_rand(&g);
+    uint64_t st = (((uint64_t)n1) & 0xffffu) | ((((uint64_t)n2) & 0xffffu) << 16) |
+                  ((((uint64_t)n3) & 0xffffu) << 32) | ((((uint64_t)n4) & 0xffffu) << 48);
+    return st == 0 ? 1318319ULL : st;
+}
+
+/* ------------------------------------------------------------------ */
+/* Permutation: heuristics.h:614-631 with Kokkos::BinSort             */
+/* (Kokkos_Sort.hpp:241-262, 362-400) and BinOp1D (:404-440).          */
+/* `state` is the pool state; it is advanced by n draws.               */
+/* ------------------------------------------------------------------ */
+void orc_generate_permutation(vid_t n, uint64_t *state, vid_t *vperm, uint64_t *keys_out)
+{
+    uint64_t *keys = (uint64_t *)malloc((size_t)n * sizeof(uint64_t) + 8);
+    for (vid_t i = 0; i < n; i++) keys[i] = orc_xs_urand64(state);
+
+    /* BinOp1D(n, 0, UINT64_MAX): max_bins = n+1, mul = 1.0*n/(max-min) */
+    int max_bins = (int)n + 1;
+    double mul = 1.0 * (int)n / (double)(UINT64_MAX - 0);
+    uint32_t *count = (uint32_t *)calloc((size_t)max_bins, sizeof(uint32_t));
+    uint32_t *offs = (uint32_t *)calloc((size_t)max_bins, sizeof(uint32_t));
+    for (vid_t i = 0; i < n; i++) count[(int)(mul * (double)(keys[i] - 0))]++;
+    uint32_t run = 0;
+    for (int b = 0; b < max_bins; b++) {
+        offs[b] = run;
+        run += count[b];
+    }
+    memset(count, 0, (size_t)max_bins * sizeof(uint32_t));
+    for (vid_t i = 0; i < n; i++) {
+        int b = (int)(mul * (double)(keys[i] - 0));
+        vperm[offs[b] + count[b]++] = i;
+    }
+    /* in-bin bubble sort, Kokkos_Sort.hpp:376-400 */
+    for (int b = 0; b < max_bins; b++) {
+        uint32_t bin_size = count[b];
+        if (bin_size <= 1) continue;
+        int upper = (int)(offs[b] + bin_size);
+        int sorted = 0;
+        while (!sorted) {
+            sorted = 1;
+            vid_t old_idx = vperm[offs[b]];
+            for (int k = (int)offs[b] + 1; k < upper; k++) {
+                vid_t new_idx = vperm[k];
+                if (!(keys[old_idx] < keys[new_idx])) {
+                    vperm[k - 1] = new_idx;
+                    vperm[k] = old_idx;
+                    sorted = 0;
+                } else {
+                    old_idx = new_idx;
+                }
+            }
+            upper--;
+        }
+    }
+    if (keys_out) memcpy(keys_out, keys, (size_t)n * sizeof(uint64_t));
+    free(count);
+    free(offs);
+    free(keys);
+}
+
+/* ------------------------------------------------------------------ */
+/* Heavy neighbour, heuristics.h:959-998.                              */
+/* level 1: uniform random neighbour (pool draw per vertex, index      */
+/* order); level > 1: max weight, ties to the LAST CSR index (`>=` at  */
+/* :982, MaxLoc on one thread); isolated vertex: urand64 % n.          */
+/* ------------------------------------------------------------------ */
+void orc_heavy_neighbor(int level, vid_t n, const eid_t *rowptr, const vid_t *col,
+                        const wgt_t *w, uint64_t *state, vid_t *hn)
+{
+    for (vid_t i = 0; i < n; i++) {
+        vid_t deg = rowptr[i + 1] - rowptr[i];
+        if (level == 1) {
+            if (deg > 0) {
+                vid_t off = (vid_t)(rowptr[i] + (orc_xs_urand64(state) % deg));
+                hn[i] = col[off];
+            } else {
+                hn[i] = (vid_t)(orc_xs_urand64(state) % n);
+            }
+        } else {
+            if (deg > 0) {
+                wgt_t best = 0;
+                eid_t loc = rowptr[i];
+                for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+                    wgt_t we = w ? w[e] : 1u;
+                    if (we >= best) {
+                        best = we;
+                        loc = e;
+                    }
+                }
+                hn[i] = col[loc];
+            } else {
+                hn[i] = (vid_t)(orc_xs_urand64(state) % n);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* parallel_map_construct on a one-thread backend, heuristics.h:633-693 */
+/* (literal sequential model: passes in vperm order, alternating the    */
+/* ordering condition, requeue in order).                               */
+/* ------------------------------------------------------------------ */
+vid_t orc_map_construct(vid_t n, const vid_t *vperm, const vid_t *hn, const vid_t *pos,
+                        vid_t *vcmap, int *passes_out)
+{
+    vid_t *match = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *curr = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *next = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    for (vid_t i = 0; i < n; i++) {
+        match[i] = ORC_INF;
+        vcmap[i] = ORC_INF;
+        curr[i] = vperm[i];
+    }
+    vid_t len = n, nc = 0;
+    int swap = 1, passes = 0;
+    while (len > 0) {
+        passes++;
+        for (vid_t i = 0; i < len; i++) {
+            vid_t u = curr[i];
+            vid_t v = hn[u];
+            int condition = pos[u] < pos[v];
+            if (condition ^ swap) {
+                if (match[u] == ORC_INF) {
+                    match[u] = v;
+                    if (u == v || match[v] == ORC_INF) {
+                        if (u != v) match[v] = u;
+                        vid_t cv = nc++;
+                        vcmap[u] = cv;
+                        vcmap[v] = cv;
+                    } else {
+                        if (vcmap[v] < n) {
+                            vcmap[u] = vcmap[v];
+                        } else {
+                            match[u] = ORC_INF;
+                        }
+                    }
+                }
+            }
+        }
+        vid_t nl = 0;
+        for (vid_t i = 0; i < len; i++) {
+            vid_t u = curr[i];
+            if (vcmap[u] >= n) {
+                next[nl++] = u;
+                match[u] = ORC_INF;
+            }
+        }
+        swap ^= 1;
+        len = nl;
+        vid_t *t = curr;
+        curr = next;
+        next = t;
+        if (passes > 64) break; /* cannot happen: the serial model ends in <= 2 passes */
+    }
+    if (passes_out) *passes_out = passes;
+    free(match);
+    free(curr);
+    free(next);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
+/* One HEC level, heuristics.h:930-1031: pool re-created from `seed`   */
+/* (:947), permutation, reverse map (:952-955), heavy neighbour, map.  */
+/* ------------------------------------------------------------------ */
+vid_t orc_coarsen_hec(int level, vid_t n, const eid_t *rowptr, const vid_t *col,
+                      const wgt_t *w, uint64_t seed, vid_t *vcmap, vid_t *vperm_out,
+                      vid_t *hn_out)
+{
+    uint64_t st = orc_pool_state(seed);
+    vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *pos = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    orc_generate_permutation(n, &st, vperm, NULL);
+    for (vid_t i = 0; i < n; i++) pos[vperm[i]] = i;
+    orc_heavy_neighbor(level, n, rowptr, col, w, &st, hn);
+    vid_t nc = orc_map_construct(n, vperm, hn, pos, vcmap, NULL);
+    if (vperm_out) memcpy(vperm_out, vperm, (size_t)n * sizeof(vid_t));
+    if (hn_out) memcpy(hn_out, hn, (size_t)n * sizeof(vid_t));
+    free(vperm);
+    free(pos);
+    free(hn);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Coarse graph, coarsen_kokkos.h:1061-1127 (map/count) with the       */
+/* non-skew builder :799-900 and the RADIX dedupe :750-762, :445-510:  */
+/* rows sorted by column, equal columns merged with summed weights,    */
+/* self-loops dropped.  Returns m_c; *rowptr_c (nc+1), *col_c, *w_c    */
+/* are malloc'd.  c_vtx_w[c] = sum of f_vtx_w over the cluster.        */
+/* ------------------------------------------------------------------ */
+typedef struct { vid_t c; wgt_t w; } orc_cw_t;
+
+static int cmp_cw(const void *a, const void *b)
+{
+    const orc_cw_t *x = (const orc_cw_t *)a, *y = (const orc_cw_t *)b;
+    if (x->c != y->c) return x->c < y->c ? -1 : 1;
+    return 0;
+}
+
+eid_t orc_build_coarse(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                       const vid_t *vcmap, vid_t nc, const wgt_t *f_vtx_w,
+                       eid_t **rowptr_c, vid_t **col_c, wgt_t **w_c, wgt_t *c_vtx_w)
+{
+    eid_t *cnt = (eid_t *)calloc((size_t)nc + 1, sizeof(eid_t));
+    for (vid_t i = 0; i < n; i++) {
+        vid_t u = vcmap[i];
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+            if (vcmap[col[e]] != u) cnt[u]++;
+        if (c_vtx_w) c_vtx_w[u] += f_vtx_w ? f_vtx_w[i] : 1u;
+    }
+    eid_t *bucket = (eid_t *)calloc((size_t)nc + 1, sizeof(eid_t));
+    for (vid_t c = 0; c < nc; c++) bucket[c + 1] = bucket[c] + cnt[c];
+    eid_t total = bucket[nc];
+    orc_cw_t *buf = (orc_cw_t *)malloc((size_t)total * sizeof(orc_cw_t) + 8);
+    memset(cnt, 0, (size_t)nc * sizeof(eid_t));
+    for (vid_t i = 0; i < n; i++) {
+        vid_t u = vcmap[i];
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            vid_t v = vcmap[col[e]];
+            if (v != u) {
+                orc_cw_t t = {v, w ? w[e] : 1u};
+                buf[bucket[u] + cnt[u]++] = t;
+            }
+        }
+    }
+    eid_t *rc = (eid_t *)calloc((size_t)nc + 1, sizeof(eid_t));
+    eid_t mc = 0;
+    for (vid_t c = 0; c < nc; c++) {
+        orc_cw_t *row = buf + bucket[c];
+        eid_t len = bucket[c + 1] - bucket[c];
+        qsort(row, len, sizeof(orc_cw_t), cmp_cw);
+        eid_t out = 0;
+        vid_t last = ORC_INF;
+        for (eid_t k = 0; k < len; k++) {
+            if (row[k].c != last) {
+                row[out] = row[k];
+                last = row[k].c;
+                out++;
+            } else {
+                row[out - 1].w += row[k].w;
+            }
+        }
+        cnt[c] = out;
+        mc += out;
+    }
+    for (vid_t c = 0; c < nc; c++) rc[c + 1] = rc[c] + cnt[c];
+    vid_t *cc = (vid_t *)malloc((size_t)mc * sizeof(vid_t) + 4);
+    wgt_t *wc = (wgt_t *)malloc((size_t)mc * sizeof(wgt_t) + 4);
+    for (vid_t c = 0; c < nc; c++)
+        for (eid_t k = 0; k < cnt[c]; k++) {
+            cc[rc[c] + k] = buf[bucket[c] + k].c;
+            wc[rc[c] + k] = buf[bucket[c] + k].w;
+        }
+    free(buf);
+    free(bucket);
+    free(cnt);
+    *rowptr_c = rc;
+    *col_c = cc;
+    *w_c = wc;
+    return mc;
+}
+
+void orc_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------ */
+/* Multilevel hierarchy, coarsen_kokkos.h:1165-1234.                   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    vid_t n;
+    eid_t m;
+    eid_t *rowptr;
+    vid_t *col;
+    wgt_t *w;       /* NULL = unit weights */
+    wgt_t *vtx_w;
+    vid_t *vcmap;   /* map from this level to the next coarser one (NULL at coarsest) */
+} orc_level_t;
+
+typedef struct {
+    int nlevels;
+    orc_level_t lv[ORC_MAX_GRAPHS + 2];
+} orc_hier_t;
+
+orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
+                                const wgt_t *w, uint64_t seed)
+{
+    orc_hier_t *h = (orc_hier_t *)calloc(1, sizeof(orc_hier_t));
+    orc_level_t *f = &h->lv[0];
+    f->n = n;
+    f->m = rowptr[n];
+    f->rowptr = (eid_t *)malloc(((size_t)n + 1) * sizeof(eid_t));
+    memcpy(f->rowptr, rowptr, ((size_t)n + 1) * sizeof(eid_t));
+    f->col = (vid_t *)malloc((size_t)f->m * sizeof(vid_t) + 4);
+    memcpy(f->col, col, (size_t)f->m * sizeof(vid_t));
+    f->w = (wgt_t *)malloc((size_t)f->m * sizeof(wgt_t) + 4);
+    for (eid_t e = 0; e < f->m; e++) f->w[e] = w ? w[e] : 1u;
+    f->vtx_w = (wgt_t *)malloc((size_t)n * sizeof(wgt_t) + 4);
+    for (vid_t i = 0; i < n; i++) f->vtx_w[i] = 1u;
+    h->nlevels = 1;
+    int level = 0;
+    while (h->lv[h->nlevels - 1].n > ORC_VTX_CUTOFF) {
+        orc_level_t *a = &h->lv[h->nlevels - 1];
+        orc_level_t *c = &h->lv[h->nlevels];
+        level++;
+        a->vcmap = (vid_t *)malloc((size_t)a->n * sizeof(vid_t) + 4);
+        vid_t nc = orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
+                                   NULL, NULL);
+        c->n = nc;
+        c->vtx_w = (wgt_t *)calloc((size_t)nc + 1, sizeof(wgt_t));
+        c->m = orc_build_coarse(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
+                                &c->rowptr, &c->col, &c->w, c->vtx_w);
+        h->nlevels++;
+        if ((unsigned)h->nlevels > ORC_MAX_GRAPHS) break;
+    }
+    if (h->nlevels > 1 && h->lv[h->nlevels - 1].n < 10) {
+        orc_level_t *c = &h->lv[h->nlevels - 1];
+        free(c->rowptr);
+        free(c->col);
+        free(c->w);
+        free(c->vtx_w);
+        memset(c, 0, sizeof(*c));
+        h->nlevels--;
+        free(h->lv[h->nlevels - 1].vcmap);
+        h->lv[h->nlevels - 1].vcmap = NULL;
+    }
+    return h;
+}
+
+int orc_hier_nlevels(const orc_hier_t *h) { return h->nlevels; }
+
+/* Expose one level's arrays (pointers stay owned by the hierarchy). */
+void orc_hier_level(const orc_hier_t *h, int l, vid_t *n, eid_t *m, eid_t **rowptr,
+                    vid_t **col, wgt_t **w, vid_t **vcmap, wgt_t **vtx_w)
+{
+    const orc_level_t *a = &h->lv[l];
+    *n = a->n;
+    *m = a->m;
+    *rowptr = a->rowptr;
+    *col = a->col;
+    *w = a->w;
+    *vcmap = a->vcmap;
+    *vtx_w = a->vtx_w;
+}
+
+void orc_hier_free(orc_hier_t *h)
+{
+    if (!h) return;
+    for (int l = 0; l < h->nlevels; l++) {
+        free(h->lv[l].rowptr);
+        free(h->lv[l].col);
+        free(h->lv[l].w);
+        free(h->lv[l].vtx_w);
+        free(h->lv[l].vcmap);
+    }
+    free(h);
+}
+
+/* ------------------------------------------------------------------ */
+/* Power iteration, eigensolve_kokkos.h:102-206 (vector helpers :14-58,*/
+/* KokkosBlas::dot = sequential `sum += x[i]*y[i]` on Serial,          */
+/* KokkosSparse::spmv row sum `lsum += w*x[col]` in CSR order,         */
+/* spmv_impl.hpp:187-218).                                             */
+/* ------------------------------------------------------------------ */
+static double dot_seq(const double *x, const double *y, vid_t n)
+{
+    double s = 0;
+    for (vid_t i = 0; i < n; i++) s += x[i] * y[i];
+    return s;
+}
+
+static void normalize_seq(double *u, vid_t n)
+{
+    double ss = dot_seq(u, u, n);
+    double inv = 1 / sqrt(ss);
+    for (vid_t i = 0; i < n; i++) u[i] = u[i] * inv;
+}
+
+static void orthogonalize_seq(double *u1, const double *u2, vid_t n)
+{
+    double mult1 = dot_seq(u1, u2, n);
+    for (vid_t i = 0; i < n; i++) u1[i] -= mult1 * u2[i];
+}
+
+static void d_orthogonalize_seq(double *u1, const double *u2, const wgt_t *D, vid_t n)
+{
+    double numer = 0.0, denom = 0.0;
+    for (vid_t i = 0; i < n; i++) numer += u1[i] * D[i] * u2[i];
+    for (vid_t i = 0; i < n; i++) denom += u2[i] * D[i] * u2[i];
+    for (vid_t i = 0; i < n; i++) u1[i] -= numer * u2[i] / denom;
+}
+
+/*
+ * Runs the reference power iteration on (rowptr, col, w) in place on u.
+ * Returns the iteration count; *max_reached set when the 1e14/n cap hit.
+ * iter_cap (0 = none) additionally stops after that many iterations, used
+ * only to time a bounded CPU sample.
+ */
+uint64_t orc_power_iter(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                        double *u, int normLap, double tol, uint64_t iter_cap,
+                        int *max_reached, double *last_dot)
+{
+    double *vec1 = (double *)malloc((size_t)n * sizeof(double) + 8);
+    double *v = (double *)malloc((size_t)n * sizeof(double) + 8);
+    wgt_t *wdeg = (wgt_t *)malloc((size_t)n * sizeof(wgt_t) + 4);
+    for (vid_t i = 0; i < n; i++) vec1[i] = 1.0;
+    for (vid_t i = 0; i < n; i++) {
+        wgt_t d = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) d += w ? w[e] : 1u;
+        wdeg[i] = d;
+    }
+    wgt_t gb = 2;
+    if (!normLap) {
+        wgt_t mx = 0;
+        for (vid_t i = 0; i < n; i++) {
+            wgt_t s = (wgt_t)(2.0 * wdeg[i]);
+            if (mx < s) mx = s;
+        }
+        gb = mx;
+        if (gb < 2.0) gb = 2;
+    }
+    normalize_seq(vec1, n);
+    if (!normLap)
+        orthogonalize_seq(u, vec1, n);
+    else
+        d_orthogonalize_seq(u, vec1, wdeg, n);
+    normalize_seq(u, n);
+    memcpy(v, u, (size_t)n * sizeof(double));
+
+    uint64_t niter = 0;
+    uint64_t iter_max = ORC_POWERITER_ITER / (uint64_t)n;
+    double dotprod = 0, lastDotprod = 1;
+    while (fabs(dotprod - lastDotprod) > tol && niter < iter_max) {
+        if (iter_cap && niter >= iter_cap) break;
+        memcpy(u, v, (size_t)n * sizeof(double));
+        for (vid_t i = 0; i < n; i++) {
+            double s = 0;
+            for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+                s += (double)(w ? w[e] : 1u) * u[col[e]];
+            v[i] = s;
+        }
+        for (vid_t i = 0; i < n; i++) {
+            double vi;
+            if (!normLap) {
+                vi = (double)(wgt_t)(gb - wdeg[i]) * u[i];
+                vi += v[i];
+            } else {
+                double winv = 1.0 / wdeg[i];
+                vi = 0.5 * u[i];
+                vi += 0.5 * v[i] * winv;
+            }
+            v[i] = vi;
+        }
+        if (!normLap) orthogonalize_seq(v, vec1, n);
+        normalize_seq(v, n);
+        lastDotprod = dotprod;
+        dotprod = dot_seq(u, v, n);
+        niter++;
+    }
+    if (max_reached) *max_reached = niter >= iter_max;
+    if (last_dot) *last_dot = dotprod;
+    free(vec1);
+    free(v);
+    free(wdeg);
+    return niter;
+}
+
+/* ------------------------------------------------------------------ */
+/* Eigensolve driver, eigensolve_kokkos.h:1493-1595 (non-FM branch).   */
+/* Coarsest guess pcg32()/UINT32_MAX (:1500-1505), power iteration per */
+/* level coarse->fine, prolongation fine[i] = coarse[vcmap[i]]         */
+/* (:1556-1558), final solve at the finest level.  iters[l] receives   */
+/* the count of level l (0 = finest).                                  */
+/* ------------------------------------------------------------------ */
+int orc_eigensolve(const orc_hier_t *h, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                   double tol, uint64_t iter_cap, double *evec_out, uint64_t *iters)
+{
+    int L = h->nlevels;
+    vid_t gn = h->lv[L - 1].n;
+    double *cur = (double *)malloc((size_t)gn * sizeof(double) + 8);
+    for (vid_t i = 0; i < gn; i++) cur[i] = ((double)orc_pcg32(pcg_state, pcg_inc)) / UINT32_MAX;
+    normalize_seq(cur, gn);
+    for (int l = L - 1; l >= 0; l--) {
+        const orc_level_t *a = &h->lv[l];
+        uint64_t it = orc_power_iter(a->n, a->rowptr, a->col, a->w, cur, normLap, tol,
+                                     iter_cap, NULL, NULL);
+        if (iters) iters[l] = it;
+        if (l > 0) {
+            const orc_level_t *f = &h->lv[l - 1];
+            double *fine = (double *)malloc((size_t)f->n * sizeof(double) + 8);
+            for (vid_t i = 0; i < f->n; i++) fine[i] = cur[f->vcmap[i]];
+            free(cur);
+            cur = fine;
+        }
+    }
+    memcpy(evec_out, cur, (size_t)h->lv[0].n * sizeof(double));
+    free(cur);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* sgp_compute_partition, sgpar.h:1969-2087 with local_search_alg = 0   */
+/* (returns at :2080-2081): sort (ev, u) ascending (vu_cmpfn_inc,       */
+/* sgpar.h:756-764), split, optional odd-vertex move, cut / 2.          */
+/* ------------------------------------------------------------------ */
+typedef struct { double ev; vid_t u; } orc_vu_t;
+
+static int cmp_vu(const void *a, const void *b)
+{
+    const orc_vu_t *x = (const orc_vu_t *)a, *y = (const orc_vu_t *)b;
+    if (x->ev != y->ev) return x->ev < y->ev ? -1 : 1;
+    return x->u < y->u ? -1 : (x->u > y->u ? 1 : 0);
+}
+
+int orc_compute_partition(vid_t n, const eid_t *rowptr, const vid_t *col, const double *evec,
+                          int num_partitions, int perc_imbalance_allowed, vid_t *part,
+                          eid_t *edgecut)
+{
+    orc_vu_t *vu = (orc_vu_t *)malloc((size_t)n * sizeof(orc_vu_t) + 16);
+    for (vid_t i = 0; i < n; i++) {
+        vu[i].ev = evec[i];
+        vu[i].u = i;
+    }
+    qsort(vu, n, sizeof(orc_vu_t), cmp_vu);
+    if (num_partitions == 4) num_partitions = 2;
+    long max_part_size = (long)ceil(n / ((double)num_partitions));
+    vid_t imbr = (vid_t)floor(max_part_size * (1.0 + perc_imbalance_allowed / 100.0));
+    vid_t imbl = n - imbr;
+    for (vid_t i = 0; i < imbl; i++) part[vu[i].u] = 0;
+    for (vid_t i = imbl; i < n; i++) part[vu[i].u] = 1;
+    long cur = 0;
+    for (vid_t i = 0; i < n; i++)
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+            if (part[col[e]] != part[i]) cur++;
+    long best = cur;
+    for (vid_t i = imbl; i < imbr; i++) {
+        vid_t u = vu[i].u;
+        long upd = 0;
+        for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++) {
+            if (part[col[e]] == 1)
+                upd++;
+            else
+                upd--;
+        }
+        cur = cur + 2 * upd;
+        if (cur <= best) {
+            part[u] = 0;
+            best = cur;
+        }
+    }
+    free(vu);
+    *edgecut = (eid_t)(best / 2);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Whole bisection, sgpar.h:2363-2431 (Kokkos branch).                 */
+/* hec_seed plays the role of std::time(nullptr) at every level        */
+/* (heuristics.h:947); pcg state/inc is the caller's rng (advanced).   */
+/* levels_n[l], iters[l] (room for 202) receive per-level sizes and    */
+/* iteration counts (0 = finest); returns the number of levels.        */
+/* ------------------------------------------------------------------ */
+int orc_partition_graph(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                        uint64_t hec_seed, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                        double tol, vid_t *part, eid_t *edgecut, vid_t *levels_n,
+                        uint64_t *iters, double *evec_out)
+{
+    orc_hier_t *h = orc_hierarchy_build(n, rowptr, col, w, hec_seed);
+    int L = h->nlevels;
+    double *ev = (double *)malloc((size_t)n * sizeof(double) + 8);
+    orc_eigensolve(h, pcg_state, pcg_inc, normLap, tol, 0, ev, iters);
+    if (levels_n)
+        for (int l = 0; l < L; l++) levels_n[l] = h->lv[l].n;
+    orc_compute_partition(n, rowptr, col, ev, 2, 0, part, edgecut);
+    if (evec_out) memcpy(evec_out, ev, (size_t)n * sizeof(double));
+    free(ev);
+    orc_hier_free(h);
+    return L;
+}
