@@ -1,0 +1,190 @@
+/*
+ * sgpar_mi.h -- C ABI of the MI355X (gfx950) multilevel spectral bisection
+ * library libsgpar_mi.so.
+ *
+ * The library replaces the hot path of the reference's sgpar partitioner:
+ * heavy-edge coarsening, the coarse-graph build P^T A P, the multilevel
+ * Fiedler power iteration and the sort-and-split bisection.  Plain pointers
+ * and sizes only; every call returns an int status (0 = success, see
+ * MI_STATUS_*) and never throws.  mi_last_error() describes the last failure
+ * of a context.
+ *
+ * Types follow the reference's default build (definitions_kokkos.h:61-75):
+ * vertex ids, edge offsets and weights are uint32_t, vectors are double.
+ *
+ * Ownership: host arrays are caller-owned and only read/written during the
+ * call.  Device arrays passed with the *_device entry points are
+ * caller-owned device memory on the context's device.  Objects (mi_ctx,
+ * mi_graph, mi_hier) are owned by the caller and released with the matching
+ * *_destroy call.  One context per host thread; all work of a context runs
+ * in order on its own HIP stream.
+ */
+#ifndef SGPAR_MI_H_
+#define SGPAR_MI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_STATUS_OK 0
+#define MI_STATUS_ARG 1      /* invalid argument or malformed CSR */
+#define MI_STATUS_HIP 2      /* HIP runtime failure (no GPU, launch error) */
+#define MI_STATUS_ALLOC 3    /* device memory exhausted */
+#define MI_STATUS_STATE 4    /* object used in the wrong state */
+#define MI_STATUS_INTERNAL 5 /* internal consistency check failed */
+
+#define MI_MAX_LEVELS 202 /* the reference stops after 200 graphs, coarsen_kokkos.h:1218 */
+
+typedef struct mi_ctx mi_ctx;
+typedef struct mi_graph mi_graph;
+typedef struct mi_hier mi_hier;
+
+/* Mirrors config_t, sgpar.h:186-193 (same fields, same order). */
+typedef struct {
+    int coarsening_alg;   /* ignored on this path, as in the reference's Kokkos branch */
+    int refine_alg;       /* 0: Laplacian, 1: normalised Laplacian (normLap) */
+    int local_search_alg; /* 0 only (the reference returns before local search) */
+    int num_iter;         /* trials; the library runs one trial per call */
+    double tol;           /* power-iteration tolerance, default 1e-10 */
+    int num_partitions;   /* 2 or 4 (4 splits in two, like sgp_compute_partition) */
+} mi_config;
+
+/* Per-call measurements, levels indexed 0 = finest. */
+typedef struct {
+    int nlevels;
+    uint32_t level_n[MI_MAX_LEVELS];
+    uint32_t level_m[MI_MAX_LEVELS];          /* stored nonzeros */
+    uint64_t level_iters[MI_MAX_LEVELS];      /* power iterations */
+    int level_max_reached[MI_MAX_LEVELS];     /* 1e14/n cap hit */
+    double level_eigen_ms[MI_MAX_LEVELS];
+    double coarsen_ms, eigen_ms, partition_ms, total_ms;
+    double permute_ms, heavy_ms, map_ms, build_ms;
+    uint32_t edge_cut;
+} mi_stats;
+
+/* ---------------------------------------------------------------- context */
+/* Creates a context on HIP device `device` (its stream, pool, workspace). */
+int mi_ctx_create(int device, mi_ctx** out);
+void mi_ctx_destroy(mi_ctx* ctx);
+/* Text of the last failure on this context ("" if none). */
+const char* mi_last_error(const mi_ctx* ctx);
+/* Returns cached pool blocks to the runtime. */
+int mi_ctx_trim(mi_ctx* ctx);
+
+/* ---------------------------------------------------------------- graphs */
+/*
+ * Uploads a CSR graph (host arrays) to HBM.  Replaces the host->device copy
+ * of sgp_generate_coarse_graphs, coarsen_kokkos.h:1169-1195.  w == NULL means
+ * unit weights, which is what the reference's sgp_partition_graph always uses
+ * (coarsen_kokkos.h:1182); a non-NULL w gives the weighted form.  The CSR must
+ * be symmetric with no self-loops (the reference input format, mtx2csr.cpp).
+ */
+int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                    const uint32_t* w, mi_graph** out);
+/* Wraps device-resident CSR arrays (not copied; must outlive the graph). */
+int mi_graph_create_device(mi_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* d_rowptr,
+                           const uint32_t* d_col, const uint32_t* d_w, mi_graph** out);
+void mi_graph_destroy(mi_graph* g);
+int mi_graph_size(const mi_graph* g, uint32_t* n, uint32_t* m);
+/* Copies a graph to host arrays (rowptr n+1, col m, w m; any may be NULL;
+   w receives 1s for a unit-weight graph). */
+int mi_graph_copy(mi_ctx* ctx, const mi_graph* g, uint32_t* rowptr, uint32_t* col, uint32_t* w);
+
+/* ---------------------------------------------------------------- coarsening */
+/*
+ * Builds the multilevel hierarchy: replaces sgpar_kokkos::sgp_generate_coarse_graphs,
+ * coarsen_kokkos.h:1165-1234 (HEC map heuristics.h:930-1031 and coarse build
+ * coarsen_kokkos.h:1061-1127 per level; coarsen while n > 50, at most 200
+ * graphs, drop a last level under 10 vertices).  `seed` plays the role of
+ * std::time(nullptr), which the reference uses to seed the XorShift pool at
+ * every level (heuristics.h:947): with one seed for all levels the result
+ * equals the reference's Serial-backend run with time() fixed to `seed`.
+ */
+int mi_hierarchy_build(mi_ctx* ctx, const mi_graph* g, uint64_t seed, mi_hier** out,
+                       mi_stats* stats);
+/* Wraps a hierarchy given as host CSR levels (level 0 finest; vcmap[l] maps
+   level l to l+1, vcmap[nlevels-1] unused).  w[l] == NULL means unit. */
+int mi_hier_create_from_host(mi_ctx* ctx, int nlevels, const uint32_t* n,
+                             const uint32_t* const* rowptr, const uint32_t* const* col,
+                             const uint32_t* const* w, const uint32_t* const* vcmap,
+                             mi_hier** out);
+void mi_hier_destroy(mi_hier* h);
+int mi_hier_nlevels(const mi_hier* h);
+int mi_hier_level_size(const mi_hier* h, int level, uint32_t* n, uint32_t* m);
+/* Copies level `level` to host; any pointer may be NULL.  w receives 1s for a
+   unit-weight level; vcmap (n entries) is only defined below the coarsest. */
+int mi_hier_level_copy(mi_ctx* ctx, const mi_hier* h, int level, uint32_t* rowptr,
+                       uint32_t* col, uint32_t* w, uint32_t* vcmap, uint32_t* vtx_w);
+
+/* ---------------------------------------------------------------- eigensolve */
+/*
+ * Multilevel Fiedler vector: replaces sgpar_kokkos::sgp_eigensolve,
+ * eigensolve_kokkos.h:1493-1595 (non-FM branch).  The coarsest guess is
+ * pcg32()/UINT32_MAX drawn from (*pcg_state, pcg_inc), which is advanced like
+ * the reference's rng (:1500-1505); each level runs sgp_power_iter
+ * (:102-206) then prolongs (:1556-1558).  normLap = config.refine_alg.
+ * evec (host, n0 doubles) may be NULL.  stats may be NULL.
+ */
+int mi_fiedler(mi_ctx* ctx, const mi_hier* h, uint64_t* pcg_state, uint64_t pcg_inc,
+               int normLap, double tol, double* evec, mi_stats* stats);
+/* One reference power iteration (eigensolve_kokkos.h:102-206) on graph g,
+   in place on host vector u (n); *iters receives the iteration count. */
+int mi_power_iter(mi_ctx* ctx, const mi_graph* g, double* u, int normLap, double tol,
+                  uint64_t* iters, int* max_reached);
+
+/* ---------------------------------------------------------------- partition */
+/*
+ * Sort-and-split bisection of g by the host eigenvector evec: replaces
+ * sgp_compute_partition, sgpar.h:1969-2087 with local_search_alg = 0.  part
+ * (host, n) receives 0/1, *edge_cut the number of cut edges.
+ */
+int mi_compute_partition(mi_ctx* ctx, const mi_graph* g, const double* evec,
+                         int num_partitions, int perc_imbalance, uint32_t* part,
+                         uint32_t* edge_cut);
+
+/*
+ * The whole bisection: replaces sgp_partition_graph, sgpar.h:2363-2431
+ * (Kokkos branch): coarsen, eigensolve, split.  part is a host array of n, or
+ * a device array when part_on_device != 0.  hec_seed as in
+ * mi_hierarchy_build; (*pcg_state, pcg_inc) as in mi_fiedler.
+ */
+int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config,
+                       int perc_imbalance, uint64_t hec_seed, uint64_t* pcg_state,
+                       uint64_t pcg_inc, uint32_t* part, int part_on_device,
+                       uint32_t* edge_cut, mi_stats* stats);
+
+/* ---------------------------------------------------------------- building blocks
+ * Single steps of the path, exposed for parity tests and measurement. */
+
+/* urand64 draws of the XorShift64 pool of `seed` at stream positions
+   [first, first+count) (Kokkos_Random.hpp:702-720, 863-898). */
+int mi_xs_draws(mi_ctx* ctx, uint64_t seed, uint64_t first, uint64_t count, uint64_t* out);
+/* One HEC level (heuristics.h:930-1031) on g; host outputs, any may be NULL
+   except vcmap. */
+int mi_hec_level(mi_ctx* ctx, const mi_graph* g, int level, uint64_t seed, uint32_t* vcmap,
+                 uint32_t* vperm, uint32_t* hn, uint32_t* nc);
+/* Coarse graph of g under vcmap (coarsen_kokkos.h:1061-1127); returns a new
+   graph (weighted) and the coarse vertex weights (host, nc; may be NULL). */
+int mi_build_coarse(mi_ctx* ctx, const mi_graph* g, const uint32_t* vcmap, uint32_t nc,
+                    mi_graph** out, uint32_t* c_vtx_w);
+/* y = A x + (gb - wdeg) .* x with gb = max(2 wdeg, 2) (eigensolve_kokkos.h:161-179);
+   host x, y (n). */
+int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y);
+/* Times `reps` launches of the FP64 shifted SpMV on the finest level of g on
+   the context stream with HIP events; returns the mean launch time and the
+   algorithmic bytes of one launch. */
+int mi_bench_spmv(mi_ctx* ctx, const mi_graph* g, int reps, double* avg_ms, double* alg_bytes);
+/* Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), host arrays. */
+int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
+                        int end_bit);
+/* Exclusive prefix sum of u32, host arrays; *total receives the sum. */
+int mi_scan_u32(mi_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGPAR_MI_H_ */

@@ -1,0 +1,731 @@
+// mi_api.cpp -- the C ABI of libsgpar_mi.so (declared in include/sgpar_mi.h)
+// plus the host runtime under it: context, caching pool, graph and
+// hierarchy objects, and the multilevel driver that mirrors the reference's
+// sgp_partition_graph (sgpar.h:2363-2431), sgp_generate_coarse_graphs
+// (coarsen_kokkos.h:1165-1234) and sgp_eigensolve (eigensolve_kokkos.h:1493-1595).
+#include "../../include/sgpar_mi.h"
+#include "mi_kernels.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <sstream>
+
+namespace mi {
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line)
+{
+    std::ostringstream os;
+    os << "HIP error " << (int)e << " (" << hipGetErrorString(e) << ") in " << what << " at " << file << ":"
+       << line;
+    throw Error(e == hipErrorOutOfMemory ? ST_ALLOC : ST_HIP, os.str());
+}
+
+DevicePool::~DevicePool()
+{
+    trim();
+    for (auto& kv : size_of_) (void)hipFree(kv.first);
+}
+
+static size_t size_class(size_t bytes)
+{
+    size_t s = 256;
+    while (s < bytes) s <<= 1;
+    return s;
+}
+
+void* DevicePool::alloc(size_t bytes)
+{
+    const size_t sc = size_class(bytes);
+    auto it = free_.find(sc);
+    if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        cached_ -= sc;
+        in_use_ += sc;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sc);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, sc);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(ST_ALLOC, "device allocation of " + std::to_string(sc) + " bytes failed");
+        }
+    }
+    size_of_[p] = sc;
+    in_use_ += sc;
+    return p;
+}
+
+void DevicePool::release(void* p)
+{
+    auto it = size_of_.find(p);
+    if (it == size_of_.end()) return;
+    free_[it->second].push_back(p);
+    in_use_ -= it->second;
+    cached_ += it->second;
+}
+
+void DevicePool::trim()
+{
+    if (cached_ == 0) return;
+    (void)hipDeviceSynchronize();
+    for (auto& kv : free_) {
+        for (void* p : kv.second) {
+            (void)hipFree(p);
+            size_of_.erase(p);
+        }
+        kv.second.clear();
+    }
+    cached_ = 0;
+}
+
+Ctx::~Ctx()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (partials) (void)hipFree(partials);
+    if (scal) (void)hipFree(scal);
+    if (cnt) (void)hipFree(cnt);
+    if (niter) (void)hipFree(niter);
+    if (xs_jump) (void)hipFree(xs_jump);
+    if (host_pinned) (void)hipHostFree(host_pinned);
+    pool.trim();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ctx_init(Ctx& c, int device)
+{
+    int ndev = 0;
+    MI_HIP(hipGetDeviceCount(&ndev));
+    MI_CHECK(device >= 0 && device < ndev, ST_ARG, "no such HIP device");
+    c.device = device;
+    MI_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MI_HIP(hipGetDeviceProperties(&prop, device));
+    c.num_cu = prop.multiProcessorCount;
+    MI_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    MI_HIP(hipMalloc(&c.partials, sizeof(double) * MAX_RED_BLOCKS * 4));
+    MI_HIP(hipMalloc(&c.scal, sizeof(double) * SC_COUNT));
+    MI_HIP(hipMalloc(&c.cnt, sizeof(uint32_t) * CT_COUNT));
+    MI_HIP(hipMalloc(&c.niter, sizeof(unsigned long long)));
+    MI_HIP(hipMemset(c.scal, 0, sizeof(double) * SC_COUNT));
+    MI_HIP(hipMemset(c.cnt, 0, sizeof(uint32_t) * CT_COUNT));
+    MI_HIP(hipMemset(c.niter, 0, sizeof(unsigned long long)));
+    MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_pinned), 64 * sizeof(uint32_t), hipHostMallocDefault));
+}
+
+void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes)
+{
+    MI_CHECK(bytes <= 64 * sizeof(uint32_t), ST_INTERNAL, "read_words too large");
+    MI_HIP(hipMemcpyAsync(c.host_pinned, dsrc, bytes, hipMemcpyDeviceToHost, c.stream));
+    MI_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(hdst, c.host_pinned, bytes);
+}
+
+}  // namespace mi
+
+using namespace mi;
+
+struct mi_ctx {
+    Ctx c;
+};
+
+struct mi_graph {
+    mi_ctx* ctx = nullptr;
+    DGraph g;
+};
+
+struct mi_hier {
+    mi_ctx* ctx = nullptr;
+    std::vector<DLevel> lv;
+};
+
+namespace {
+
+template <class F>
+int guard(mi_ctx* ctx, F&& f)
+{
+    try {
+        if (ctx) {
+            ctx->c.last_error.clear();
+            MI_HIP(hipSetDevice(ctx->c.device));
+        }
+        f();
+        return ST_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->c.last_error = "host allocation failed";
+        return ST_ALLOC;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return ST_INTERNAL;
+    }
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void h2d(Ctx& c, void* d, const void* h, size_t bytes)
+{
+    if (bytes) MI_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.stream));
+}
+
+void d2h(Ctx& c, void* h, const void* d, size_t bytes)
+{
+    if (bytes) MI_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c.stream));
+}
+
+void check_csr(uint32_t n, const uint32_t* rowptr, const uint32_t* col)
+{
+    MI_CHECK(rowptr && (rowptr[n] == 0 || col), ST_ARG, "null CSR array");
+    MI_CHECK(rowptr[0] == 0, ST_ARG, "rowptr[0] must be 0");
+    for (uint32_t i = 0; i < n; i++) MI_CHECK(rowptr[i] <= rowptr[i + 1], ST_ARG, "rowptr not monotone");
+    const uint32_t m = rowptr[n];
+    for (uint32_t e = 0; e < m; e++) MI_CHECK(col[e] < n, ST_ARG, "column index out of range");
+}
+
+// Fine level 0 of a hierarchy borrows the graph's device arrays.
+DLevel borrow_level(Ctx& c, const DGraph& g)
+{
+    DLevel L;
+    L.g.n = g.n;
+    L.g.m = g.m;
+    L.g.unit = g.unit;
+    L.g.b_rowptr = g.R();
+    L.g.b_col = g.Cc();
+    L.g.b_w = g.W();
+    L.vtx_w = DBuf<uint32_t>(&c.pool, g.n);
+    std::vector<uint32_t> ones(g.n, 1u);
+    h2d(c, L.vtx_w.get(), ones.data(), (size_t)g.n * 4);
+    MI_HIP(hipStreamSynchronize(c.stream));
+    return L;
+}
+
+void build_hierarchy(Ctx& c, const DGraph& g, uint64_t seed, std::vector<DLevel>& lv, mi_stats* st)
+{
+    lv.clear();
+    lv.reserve(MI_MAX_LEVELS + 1);
+    lv.push_back(borrow_level(c, g));
+    HecTimes ht;
+    double build_ms = 0;
+    int level = 0;
+    while (lv.back().g.n > 50u) {  // SGPAR_COARSENING_VTX_CUTOFF, definitions_kokkos.h:77-79
+        level++;
+        DLevel& a = lv.back();
+        a.vcmap = DBuf<uint32_t>(&c.pool, a.g.n);
+        const uint32_t nc = hec_map(c, a.g, level, seed, a.vcmap.get(), nullptr, nullptr, st ? &ht : nullptr);
+        a.nc_next = nc;
+        DLevel nl;
+        auto tb = std::chrono::steady_clock::now();
+        build_coarse(c, a.g, a.vcmap.get(), nc, a.vtx_w.get(), nl.g, nl.vtx_w);
+        if (st) {
+            MI_HIP(hipStreamSynchronize(c.stream));
+            build_ms += ms_since(tb);
+        }
+        lv.push_back(std::move(nl));
+        if (lv.size() > 200) break;  // coarsen_kokkos.h:1218
+    }
+    // coarsen_kokkos.h:1226-1231; the fine level itself is never dropped
+    if (lv.size() > 1 && lv.back().g.n < 10u) {
+        lv.pop_back();
+        lv.back().vcmap.reset();
+        lv.back().nc_next = 0;
+    }
+    if (st) {
+        st->permute_ms = ht.permute_ms;
+        st->heavy_ms = ht.heavy_ms;
+        st->map_ms = ht.map_ms;
+        st->build_ms = build_ms;
+    }
+}
+
+void fill_level_stats(const std::vector<DLevel>& lv, mi_stats* st)
+{
+    if (!st) return;
+    st->nlevels = (int)lv.size();
+    for (size_t l = 0; l < lv.size() && l < MI_MAX_LEVELS; l++) {
+        st->level_n[l] = lv[l].g.n;
+        st->level_m[l] = lv[l].g.m;
+    }
+}
+
+// Returns the finest-level eigenvector in a device buffer.
+DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state, uint64_t pcg_inc, int normLap,
+                     double tol, mi_stats* st)
+{
+    const int L = (int)lv.size();
+    MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
+    const uint32_t gn = lv[L - 1].g.n;
+    std::vector<double> cg(gn);
+    // eigensolve_kokkos.h:1500-1505: pcg32()/UINT32_MAX, then normalise
+    for (uint32_t i = 0; i < gn; i++) {
+        uint64_t old = *pcg_state;
+        *pcg_state = old * 6364136223846793005ULL + (pcg_inc | 1u);
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        uint32_t r = (xs >> rot) | (xs << ((-rot) & 31u));
+        cg[i] = ((double)r) / UINT32_MAX;
+    }
+    DBuf<double> cur(&c.pool, gn);
+    h2d(c, cur.get(), cg.data(), (size_t)gn * 8);
+    normalize(c, cur.get(), gn);
+    for (int l = L - 1; l >= 0; l--) {
+        PowerStats ps;
+        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps);
+        if (st && l < MI_MAX_LEVELS) {
+            st->level_iters[l] = ps.iters;
+            st->level_max_reached[l] = ps.max_reached;
+            st->level_eigen_ms[l] = ps.ms;
+        }
+        if (l > 0) {
+            const uint32_t nf = lv[l - 1].g.n;
+            DBuf<double> fine(&c.pool, nf);
+            prolong(c, lv[l - 1].vcmap.get(), nf, cur.get(), fine.get());
+            cur = std::move(fine);
+        }
+    }
+    return cur;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi_ctx_create(int device, mi_ctx** out)
+{
+    if (!out) return ST_ARG;
+    *out = nullptr;
+    mi_ctx* ctx = new (std::nothrow) mi_ctx;
+    if (!ctx) return ST_ALLOC;
+    int rc = guard(nullptr, [&] { ctx_init(ctx->c, device); });
+    if (rc != ST_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return ST_OK;
+}
+
+void mi_ctx_destroy(mi_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    delete ctx;
+}
+
+const char* mi_last_error(const mi_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+int mi_ctx_trim(mi_ctx* ctx)
+{
+    if (!ctx) return ST_ARG;
+    return guard(ctx, [&] { ctx->c.pool.trim(); });
+}
+
+int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint32_t* w,
+                    mi_graph** out)
+{
+    if (!ctx || !out) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        MI_CHECK(n > 0, ST_ARG, "graph must have at least one vertex");
+        check_csr(n, rowptr, col);
+        Ctx& c = ctx->c;
+        auto G = std::make_unique<mi_graph>();
+        G->ctx = ctx;
+        G->g.n = n;
+        G->g.m = rowptr[n];
+        G->g.unit = (w == nullptr);
+        G->g.rowptr = DBuf<uint32_t>(&c.pool, (size_t)n + 1);
+        G->g.col = DBuf<uint32_t>(&c.pool, G->g.m);
+        h2d(c, G->g.rowptr.get(), rowptr, ((size_t)n + 1) * 4);
+        h2d(c, G->g.col.get(), col, (size_t)G->g.m * 4);
+        if (w) {
+            G->g.w = DBuf<uint32_t>(&c.pool, G->g.m);
+            h2d(c, G->g.w.get(), w, (size_t)G->g.m * 4);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        *out = G.release();
+    });
+}
+
+int mi_graph_create_device(mi_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* d_rowptr, const uint32_t* d_col,
+                           const uint32_t* d_w, mi_graph** out)
+{
+    if (!ctx || !out) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        MI_CHECK(n > 0 && d_rowptr && (m == 0 || d_col), ST_ARG, "bad device CSR");
+        auto G = std::make_unique<mi_graph>();
+        G->ctx = ctx;
+        G->g.n = n;
+        G->g.m = m;
+        G->g.unit = (d_w == nullptr);
+        G->g.b_rowptr = d_rowptr;
+        G->g.b_col = d_col;
+        G->g.b_w = d_w;
+        *out = G.release();
+    });
+}
+
+void mi_graph_destroy(mi_graph* g)
+{
+    if (!g) return;
+    if (g->ctx) (void)hipSetDevice(g->ctx->c.device);
+    delete g;
+}
+
+int mi_graph_size(const mi_graph* g, uint32_t* n, uint32_t* m)
+{
+    if (!g) return ST_ARG;
+    if (n) *n = g->g.n;
+    if (m) *m = g->g.m;
+    return ST_OK;
+}
+
+int mi_graph_copy(mi_ctx* ctx, const mi_graph* g, uint32_t* rowptr, uint32_t* col, uint32_t* w)
+{
+    if (!ctx || !g) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const DGraph& G = g->g;
+        if (rowptr) d2h(c, rowptr, G.R(), ((size_t)G.n + 1) * 4);
+        if (col) d2h(c, col, G.Cc(), (size_t)G.m * 4);
+        if (w) {
+            if (G.unit)
+                for (uint32_t e = 0; e < G.m; e++) w[e] = 1u;
+            else
+                d2h(c, w, G.W(), (size_t)G.m * 4);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_hierarchy_build(mi_ctx* ctx, const mi_graph* g, uint64_t seed, mi_hier** out, mi_stats* stats)
+{
+    if (!ctx || !g || !out) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        auto H = std::make_unique<mi_hier>();
+        H->ctx = ctx;
+        auto t0 = std::chrono::steady_clock::now();
+        build_hierarchy(ctx->c, g->g, seed, H->lv, stats);
+        MI_HIP(hipStreamSynchronize(ctx->c.stream));
+        if (stats) {
+            stats->coarsen_ms = ms_since(t0);
+            fill_level_stats(H->lv, stats);
+        }
+        *out = H.release();
+    });
+}
+
+int mi_hier_create_from_host(mi_ctx* ctx, int nlevels, const uint32_t* n, const uint32_t* const* rowptr,
+                             const uint32_t* const* col, const uint32_t* const* w, const uint32_t* const* vcmap,
+                             mi_hier** out)
+{
+    if (!ctx || !out || nlevels < 1 || !n || !rowptr || !col) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        auto H = std::make_unique<mi_hier>();
+        H->ctx = ctx;
+        for (int l = 0; l < nlevels; l++) {
+            MI_CHECK(n[l] > 0, ST_ARG, "empty level");
+            check_csr(n[l], rowptr[l], col[l]);
+            DLevel L;
+            L.g.n = n[l];
+            L.g.m = rowptr[l][n[l]];
+            L.g.unit = !(w && w[l]);
+            L.g.rowptr = DBuf<uint32_t>(&c.pool, (size_t)n[l] + 1);
+            L.g.col = DBuf<uint32_t>(&c.pool, L.g.m);
+            h2d(c, L.g.rowptr.get(), rowptr[l], ((size_t)n[l] + 1) * 4);
+            h2d(c, L.g.col.get(), col[l], (size_t)L.g.m * 4);
+            if (!L.g.unit) {
+                L.g.w = DBuf<uint32_t>(&c.pool, L.g.m);
+                h2d(c, L.g.w.get(), w[l], (size_t)L.g.m * 4);
+            }
+            if (l + 1 < nlevels) {
+                MI_CHECK(vcmap && vcmap[l], ST_ARG, "missing vcmap");
+                for (uint32_t i = 0; i < n[l]; i++)
+                    MI_CHECK(vcmap[l][i] < n[l + 1], ST_ARG, "vcmap entry out of range");
+                L.vcmap = DBuf<uint32_t>(&c.pool, n[l]);
+                h2d(c, L.vcmap.get(), vcmap[l], (size_t)n[l] * 4);
+                L.nc_next = n[l + 1];
+            }
+            MI_HIP(hipStreamSynchronize(c.stream));
+            H->lv.push_back(std::move(L));
+        }
+        *out = H.release();
+    });
+}
+
+void mi_hier_destroy(mi_hier* h)
+{
+    if (!h) return;
+    if (h->ctx) (void)hipSetDevice(h->ctx->c.device);
+    delete h;
+}
+
+int mi_hier_nlevels(const mi_hier* h) { return h ? (int)h->lv.size() : -1; }
+
+int mi_hier_level_size(const mi_hier* h, int level, uint32_t* n, uint32_t* m)
+{
+    if (!h || level < 0 || level >= (int)h->lv.size()) return ST_ARG;
+    if (n) *n = h->lv[level].g.n;
+    if (m) *m = h->lv[level].g.m;
+    return ST_OK;
+}
+
+int mi_hier_level_copy(mi_ctx* ctx, const mi_hier* h, int level, uint32_t* rowptr, uint32_t* col, uint32_t* w,
+                       uint32_t* vcmap, uint32_t* vtx_w)
+{
+    if (!ctx || !h || level < 0 || level >= (int)h->lv.size()) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const DLevel& L = h->lv[level];
+        if (rowptr) d2h(c, rowptr, L.g.R(), ((size_t)L.g.n + 1) * 4);
+        if (col) d2h(c, col, L.g.Cc(), (size_t)L.g.m * 4);
+        if (w) {
+            if (L.g.unit)
+                for (uint32_t e = 0; e < L.g.m; e++) w[e] = 1u;
+            else
+                d2h(c, w, L.g.W(), (size_t)L.g.m * 4);
+        }
+        if (vcmap) {
+            MI_CHECK(L.vcmap, ST_ARG, "coarsest level has no vcmap");
+            d2h(c, vcmap, L.vcmap.get(), (size_t)L.g.n * 4);
+        }
+        if (vtx_w) {
+            MI_CHECK(L.vtx_w, ST_ARG, "level has no vertex weights");
+            d2h(c, vtx_w, L.vtx_w.get(), (size_t)L.g.n * 4);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_fiedler(mi_ctx* ctx, const mi_hier* h, uint64_t* pcg_state, uint64_t pcg_inc, int normLap, double tol,
+               double* evec, mi_stats* stats)
+{
+    if (!ctx || !h || !pcg_state) return ST_ARG;
+    return guard(ctx, [&] {
+        auto t0 = std::chrono::steady_clock::now();
+        DBuf<double> ev = fiedler(ctx->c, h->lv, pcg_state, pcg_inc, normLap, tol, stats);
+        if (evec) d2h(ctx->c, evec, ev.get(), (size_t)h->lv[0].g.n * 8);
+        MI_HIP(hipStreamSynchronize(ctx->c.stream));
+        if (stats) {
+            stats->eigen_ms = ms_since(t0);
+            fill_level_stats(h->lv, stats);
+        }
+    });
+}
+
+int mi_power_iter(mi_ctx* ctx, const mi_graph* g, double* u, int normLap, double tol, uint64_t* iters,
+                  int* max_reached)
+{
+    if (!ctx || !g || !u) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        DBuf<double> d(&c.pool, g->g.n);
+        h2d(c, d.get(), u, (size_t)g->g.n * 8);
+        PowerStats ps;
+        power_iter(c, g->g, d.get(), normLap, tol, &ps);
+        d2h(c, u, d.get(), (size_t)g->g.n * 8);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        if (iters) *iters = ps.iters;
+        if (max_reached) *max_reached = ps.max_reached;
+    });
+}
+
+int mi_compute_partition(mi_ctx* ctx, const mi_graph* g, const double* evec, int num_partitions,
+                         int perc_imbalance, uint32_t* part, uint32_t* edge_cut)
+{
+    if (!ctx || !g || !evec || !part || !edge_cut) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        DBuf<double> ev(&c.pool, n);
+        DBuf<uint32_t> dp(&c.pool, n);
+        h2d(c, ev.get(), evec, (size_t)n * 8);
+        *edge_cut = compute_partition(c, g->g, ev.get(), num_partitions, perc_imbalance, dp.get());
+        d2h(c, part, dp.get(), (size_t)n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config, int perc_imbalance,
+                       uint64_t hec_seed, uint64_t* pcg_state, uint64_t pcg_inc, uint32_t* part,
+                       int part_on_device, uint32_t* edge_cut, mi_stats* stats)
+{
+    if (!ctx || !g || !config || !pcg_state || !part || !edge_cut) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        MI_CHECK(config->local_search_alg == 0, ST_ARG, "only local_search_alg = 0 is on this path");
+        MI_CHECK(config->refine_alg == 0 || config->refine_alg == 1, ST_ARG, "refine_alg must be 0 or 1");
+        MI_CHECK(config->tol >= 0, ST_ARG, "tol must be >= 0");
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<DLevel> lv;
+        build_hierarchy(c, g->g, hec_seed, lv, stats);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        auto t1 = std::chrono::steady_clock::now();
+        DBuf<double> ev = fiedler(c, lv, pcg_state, pcg_inc, config->refine_alg, config->tol, stats);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        auto t2 = std::chrono::steady_clock::now();
+        fill_level_stats(lv, stats);
+        lv.clear();
+        const uint32_t n = g->g.n;
+        if (part_on_device) {
+            *edge_cut = compute_partition(c, g->g, ev.get(), config->num_partitions, perc_imbalance, part);
+        } else {
+            DBuf<uint32_t> dp(&c.pool, n);
+            *edge_cut = compute_partition(c, g->g, ev.get(), config->num_partitions, perc_imbalance, dp.get());
+            d2h(c, part, dp.get(), (size_t)n * 4);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        if (stats) {
+            stats->coarsen_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            stats->eigen_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            stats->partition_ms = ms_since(t2);
+            stats->total_ms = ms_since(t0);
+            stats->edge_cut = *edge_cut;
+        }
+    });
+}
+
+int mi_xs_draws(mi_ctx* ctx, uint64_t seed, uint64_t first, uint64_t count, uint64_t* out)
+{
+    if (!ctx || (!out && count)) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        DBuf<uint64_t> d(&c.pool, count);
+        xs_draws(c, xs_pool_state(seed), first, count, d.get());
+        d2h(c, out, d.get(), count * 8);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_hec_level(mi_ctx* ctx, const mi_graph* g, int level, uint64_t seed, uint32_t* vcmap, uint32_t* vperm,
+                 uint32_t* hn, uint32_t* nc)
+{
+    if (!ctx || !g || !vcmap || !nc || level < 1) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        DBuf<uint32_t> dv(&c.pool, n), dp(&c.pool, n), dh(&c.pool, n);
+        *nc = hec_map(c, g->g, level, seed, dv.get(), dp.get(), dh.get(), nullptr);
+        d2h(c, vcmap, dv.get(), (size_t)n * 4);
+        if (vperm) d2h(c, vperm, dp.get(), (size_t)n * 4);
+        if (hn) d2h(c, hn, dh.get(), (size_t)n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_build_coarse(mi_ctx* ctx, const mi_graph* g, const uint32_t* vcmap, uint32_t nc, mi_graph** out,
+                    uint32_t* c_vtx_w)
+{
+    if (!ctx || !g || !vcmap || !out) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        MI_CHECK(nc >= 1 && nc <= n, ST_ARG, "nc out of range");
+        for (uint32_t i = 0; i < n; i++) MI_CHECK(vcmap[i] < nc, ST_ARG, "vcmap entry out of range");
+        DBuf<uint32_t> dv(&c.pool, n);
+        h2d(c, dv.get(), vcmap, (size_t)n * 4);
+        auto G = std::make_unique<mi_graph>();
+        G->ctx = ctx;
+        DBuf<uint32_t> cvw;
+        build_coarse(c, g->g, dv.get(), nc, nullptr, G->g, cvw);
+        if (c_vtx_w) d2h(c, c_vtx_w, cvw.get(), (size_t)nc * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        *out = G.release();
+    });
+}
+
+int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y)
+{
+    if (!ctx || !g || !x || !y) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
+        DBuf<double> dx(&c.pool, n), dy(&c.pool, n);
+        weighted_degree(c, g->g, wdeg.get(), gb.get());
+        h2d(c, dx.get(), x, (size_t)n * 8);
+        spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());
+        d2h(c, y, dy.get(), (size_t)n * 8);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_bench_spmv(mi_ctx* ctx, const mi_graph* g, int reps, double* avg_ms, double* alg_bytes)
+{
+    if (!ctx || !g || reps < 1 || !avg_ms || !alg_bytes) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        const uint64_t m = g->g.m;
+        DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
+        DBuf<double> dx(&c.pool, n), dy(&c.pool, n);
+        weighted_degree(c, g->g, wdeg.get(), gb.get());
+        std::vector<double> ones(n, 1.0);
+        h2d(c, dx.get(), ones.data(), (size_t)n * 8);
+        spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());  // warm-up
+        MI_HIP(hipStreamSynchronize(c.stream));
+        hipEvent_t e0, e1;
+        MI_HIP(hipEventCreate(&e0));
+        MI_HIP(hipEventCreate(&e1));
+        MI_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < reps; r++) spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());
+        MI_HIP(hipEventRecord(e1, c.stream));
+        MI_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        MI_HIP(hipEventDestroy(e0));
+        MI_HIP(hipEventDestroy(e1));
+        *avg_ms = ms / reps;
+        // rowptr + col (+ w) + wdeg + x gather (n distinct, counted once) + y
+        *alg_bytes = 4.0 * (n + 1) + 4.0 * m + (g->g.unit ? 0.0 : 4.0 * m) + 4.0 * n + 8.0 * n + 8.0 * n;
+    });
+}
+
+int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit)
+{
+    if (!ctx || (n && (!keys || !vals))) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        DBuf<uint64_t> k(&c.pool, n), kt(&c.pool, n);
+        DBuf<uint32_t> v(&c.pool, n), vt(&c.pool, n);
+        h2d(c, k.get(), keys, n * 8);
+        h2d(c, v.get(), vals, n * 4);
+        radix_sort_pairs(c, k.get(), v.get(), n, begin_bit, end_bit, kt.get(), vt.get());
+        d2h(c, keys, k.get(), n * 8);
+        d2h(c, vals, v.get(), n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_scan_u32(mi_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total)
+{
+    if (!ctx || (n && (!in || !out))) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        DBuf<uint32_t> a(&c.pool, n), t(&c.pool, 1);
+        h2d(c, a.get(), in, n * 4);
+        scan_exclusive_u32(c, a.get(), a.get(), n, t.get());
+        d2h(c, out, a.get(), n * 4);
+        uint32_t tot = 0;
+        read_words(c, t.get(), &tot, 4);
+        if (total) *total = tot;
+    });
+}
+
+}  // extern "C"

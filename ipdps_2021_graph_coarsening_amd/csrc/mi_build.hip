@@ -1,0 +1,148 @@
+// mi_build.hip -- coarse graph P^T A P (diagonal dropped, duplicate edges
+// merged with summed u32 weights, rows sorted by column), the output of the
+// reference's non-skew builder with RADIX dedupe on its Serial run:
+// coarsen_kokkos.h:1061-1127 (map/count), :799-900 (bucket, dedupe, compact),
+// :750-762 + :445-510 (per-row sort, merge of equal columns).
+//
+// gfx950 form: every fine edge (i, j) emits the key (vcmap[i], vcmap[j])
+// packed in 2B bits (B = bits of nc); intra-cluster edges get the sentinel
+// row nc and sort to the end.  One LSD radix sort over 2B bits groups each
+// coarse row with its columns ascending; heads of equal-key runs are counted
+// with a scan, weights are merged with u32 atomics (exact and order-free) and
+// row lengths fall out of the head counts.
+#include "mi_kernels.hpp"
+
+namespace mi {
+
+namespace {
+
+inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
+
+template <int G>
+__global__ void __launch_bounds__(BLOCK) k_emit_edges(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                      const uint32_t* __restrict__ col,
+                                                      const uint32_t* __restrict__ w,
+                                                      const uint32_t* __restrict__ vcmap,
+                                                      const uint32_t* __restrict__ vtx_w, uint32_t nc,
+                                                      int B, uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals,
+                                                      uint32_t* __restrict__ c_vtx_w)
+{
+    constexpr int RPB = BLOCK / G;
+    const int sub = threadIdx.x % G;
+    const uint64_t sentinel = (uint64_t)nc << B;
+    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
+         row += (uint64_t)gridDim.x * RPB) {
+        const uint32_t b = rowptr[row], e = rowptr[row + 1];
+        const uint32_t cu = vcmap[row];
+        if (sub == 0) atomicAdd(&c_vtx_w[cu], vtx_w ? vtx_w[row] : 1u);
+        for (uint32_t k = b + sub; k < e; k += G) {
+            const uint32_t cv = vcmap[col[k]];
+            keys[k] = (cu == cv) ? sentinel : (((uint64_t)cu << B) | cv);
+            vals[k] = w ? w[k] : 1u;
+        }
+    }
+}
+
+__global__ void k_heads(uint64_t m, const uint64_t* __restrict__ keys, uint64_t sentinel,
+                        uint32_t* __restrict__ head)
+{
+    uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= m) return;
+    uint64_t key = keys[k];
+    head[k] = (key < sentinel && (k == 0 || keys[k - 1] != key)) ? 1u : 0u;
+}
+
+__global__ void k_write_coarse(uint64_t m, const uint64_t* __restrict__ keys,
+                               const uint32_t* __restrict__ vals, const uint32_t* __restrict__ uidx,
+                               uint64_t sentinel, int B, uint32_t* __restrict__ col_c,
+                               uint32_t* __restrict__ w_c, uint32_t* __restrict__ rowcnt)
+{
+    uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= m) return;
+    const uint64_t key = keys[k];
+    if (key >= sentinel) return;
+    const bool head = (k == 0 || keys[k - 1] != key);
+    const uint32_t grp = head ? uidx[k] : uidx[k] - 1u;
+    atomicAdd(&w_c[grp], vals[k]);
+    if (head) {
+        col_c[grp] = (uint32_t)(key & ((1ull << B) - 1ull));
+        atomicAdd(&rowcnt[(uint32_t)(key >> B)], 1u);
+    }
+}
+
+int bits_for(uint32_t nc)
+{
+    int B = 1;
+    while (B < 32 && (1ull << B) <= nc) B++;
+    return B;
+}
+
+}  // namespace
+
+void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
+                  const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w)
+{
+    const uint32_t n = g.n;
+    const uint64_t m = g.m;
+    MI_CHECK(nc > 0 && nc <= n, ST_INTERNAL, "coarse size out of range");
+    const int B = bits_for(nc);  // nc < 2^B, so row field nc (sentinel) fits too
+    const uint64_t sentinel = (uint64_t)nc << B;
+
+    c_vtx_w = DBuf<uint32_t>(&c.pool, nc);
+    MI_HIP(hipMemsetAsync(c_vtx_w.get(), 0, (size_t)nc * 4, c.stream));
+
+    DBuf<uint64_t> keys(&c.pool, m), ktmp(&c.pool, m);
+    DBuf<uint32_t> vals(&c.pool, m), vtmp(&c.pool, m);
+    const uint32_t avg = n ? (uint32_t)(m / n) : 0;
+    int G = 1;
+    while (G < 64 && G * 2 <= (int)avg) G *= 2;
+    const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * G, BLOCK), 65535ull * 4);
+    const uint32_t* W = g.W();
+#define EMIT(GG)                                                                                  \
+    k_emit_edges<GG><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, vcmap, vtx_w, nc, B,   \
+                                                   keys.get(), vals.get(), c_vtx_w.get())
+    switch (G) {
+        case 1: EMIT(1); break;
+        case 2: EMIT(2); break;
+        case 4: EMIT(4); break;
+        case 8: EMIT(8); break;
+        case 16: EMIT(16); break;
+        case 32: EMIT(32); break;
+        default: EMIT(64); break;
+    }
+#undef EMIT
+    MI_HIP(hipGetLastError());
+    radix_sort_pairs(c, keys.get(), vals.get(), m, 0, 2 * B, ktmp.get(), vtmp.get());
+    ktmp.reset();
+    vtmp.reset();
+
+    DBuf<uint32_t> uidx(&c.pool, m);
+    DBuf<uint32_t> total(&c.pool, 2);
+    if (m) {
+        k_heads<<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), sentinel, uidx.get());
+        MI_HIP(hipGetLastError());
+    }
+    scan_exclusive_u32(c, uidx.get(), uidx.get(), m, total.get());
+    uint32_t mc = 0;
+    read_words(c, total.get(), &mc, 4);
+
+    out = DGraph();
+    out.n = nc;
+    out.m = mc;
+    out.unit = false;
+    out.rowptr = DBuf<uint32_t>(&c.pool, (size_t)nc + 1);
+    out.col = DBuf<uint32_t>(&c.pool, mc);
+    out.w = DBuf<uint32_t>(&c.pool, mc);
+    DBuf<uint32_t> rowcnt(&c.pool, (size_t)nc + 1);
+    MI_HIP(hipMemsetAsync(rowcnt.get(), 0, ((size_t)nc + 1) * 4, c.stream));
+    if (mc) MI_HIP(hipMemsetAsync(out.w.get(), 0, (size_t)mc * 4, c.stream));
+    if (m) {
+        k_write_coarse<<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), vals.get(), uidx.get(), sentinel, B,
+                                                         out.col.get(), out.w.get(), rowcnt.get());
+        MI_HIP(hipGetLastError());
+    }
+    scan_exclusive_u32(c, rowcnt.get(), out.rowptr.get(), (uint64_t)nc + 1, nullptr);
+}
+
+}  // namespace mi

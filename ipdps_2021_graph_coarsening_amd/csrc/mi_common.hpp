@@ -1,0 +1,266 @@
+// mi_common.hpp -- shared host/device plumbing for the MI355X (gfx950)
+// multilevel spectral bisection library: error handling, the caching device
+// pool, the per-context state and wave/block reduction helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace mi {
+
+constexpr uint32_t INF = 0xFFFFFFFFu;
+constexpr int WAVE = 64;     // CDNA wavefront width
+constexpr int BLOCK = 256;   // default workgroup: 4 waves
+constexpr int MAX_RED_BLOCKS = 4096;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// status codes returned across the C ABI
+enum Status : int {
+    ST_OK = 0,
+    ST_ARG = 1,      // bad argument (sizes, null pointers, malformed CSR)
+    ST_HIP = 2,      // HIP runtime failure
+    ST_ALLOC = 3,    // device allocation failure
+    ST_STATE = 4,    // object used in the wrong state
+    ST_INTERNAL = 5  // internal consistency check failed
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define MI_HIP(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) ::mi::throw_hip(e_, #x, __FILE__, __LINE__); \
+    } while (0)
+
+#define MI_CHECK(cond, code, msg)                                     \
+    do {                                                              \
+        if (!(cond)) throw ::mi::Error((code), (msg));                \
+    } while (0)
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Caching device allocator.  Blocks are rounded up to a power of two
+// (minimum 256 B) and recycled per size class; everything is stream-ordered
+// on the owning context's single stream, so a block freed by host code can be
+// handed to the next kernel without a synchronisation.
+class DevicePool {
+public:
+    ~DevicePool();
+    void* alloc(size_t bytes);
+    void release(void* p);
+    void trim();  // return cached blocks to the runtime
+    size_t bytes_in_use() const { return in_use_; }
+    size_t bytes_cached() const { return cached_; }
+
+private:
+    std::unordered_map<size_t, std::vector<void*>> free_;
+    std::unordered_map<void*, size_t> size_of_;
+    size_t in_use_ = 0, cached_ = 0;
+};
+
+struct Ctx;
+
+// Owning device buffer of T drawn from a context's pool.
+template <class T>
+class DBuf {
+public:
+    DBuf() = default;
+    DBuf(DevicePool* pool, size_t n) : pool_(pool), n_(n)
+    {
+        p_ = static_cast<T*>(pool_->alloc((n ? n : 1) * sizeof(T)));
+    }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept { swap(o); }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if (this != &o) {
+            reset();
+            swap(o);
+        }
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p_ && pool_) pool_->release(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void swap(DBuf& o) noexcept
+    {
+        std::swap(pool_, o.pool_);
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    explicit operator bool() const { return p_ != nullptr; }
+
+private:
+    DevicePool* pool_ = nullptr;
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Slots of the context's device scalar array (FP64).
+enum Scalar : int {
+    SC_MULT = 0,    // deflation coefficient  dot(v, vec1)
+    SC_SS = 1,      // squared norm
+    SC_DOT = 2,     // dot(u, v) of the current iteration
+    SC_LASTDOT = 3, // dot(u, v) of the previous iteration
+    SC_NUMER = 4,   // D-orthogonalisation numerator
+    SC_DENOM = 5,   // D-orthogonalisation denominator
+    SC_TMP = 6,
+    SC_COUNT = 16
+};
+
+// Slots of the context's device counter array (u32 / u64).
+enum Counter : int {
+    CT_RED = 0,      // last-block counter of the reductions
+    CT_RED2 = 1,
+    CT_DONE = 2,     // power iteration finished flag
+    CT_MAXR = 3,     // iteration cap reached
+    CT_A = 4,        // general counters
+    CT_B = 5,
+    CT_C = 6,
+    CT_D = 7,
+    CT_COUNT = 16
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevicePool pool;
+    std::string last_error;
+    int num_cu = 0;
+    // reduction workspace
+    double* partials = nullptr;     // MAX_RED_BLOCKS * 4
+    double* scal = nullptr;         // SC_COUNT
+    uint32_t* cnt = nullptr;        // CT_COUNT
+    unsigned long long* niter = nullptr;  // iteration counter of the power iteration
+    uint32_t* host_pinned = nullptr;      // small pinned staging area (64 words)
+    uint64_t* xs_jump = nullptr;          // T^(2^k) of the xorshift transition, 48 x 64 words
+    ~Ctx();
+};
+
+void ctx_init(Ctx& c, int device);
+
+// Copy a few words device -> host through pinned staging and wait.
+void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes);
+
+// ------------------------------------------------------------------------
+// Device helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        uint32_t t = __shfl_xor(v, o, WAVE);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// Sum over a block of BLOCK threads; result valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double* lds /* >= BLOCK/WAVE */)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / WAVE; k++) s += lds[k];
+    return s;
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* lds)
+{
+    v = wave_sum_u32(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / WAVE; k++) s += lds[k];
+    return s;
+}
+
+__device__ __forceinline__ double load_acquire_f64(const double* p)
+{
+    unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+// "Last block" finisher of a grid-wide reduction: block b publishes its
+// partial value(s) in partials[b*K + k]; the block that arrives last sums all
+// partials in block order (deterministic for a fixed grid) and returns true
+// with the totals in out[0..K).  The counter is re-armed by that block.
+template <int K>
+__device__ bool grid_reduce_last(const double (&mine)[K], double* partials, uint32_t* counter,
+                                 double (&out)[K])
+{
+    __shared__ double red_lds[K][BLOCK / WAVE];
+    __shared__ bool am_last;
+    double tot[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = block_sum(mine[k], red_lds[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = tot[k];
+        __threadfence();
+        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += load_acquire_f64(&partials[b * K + k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = block_sum(acc[k], red_lds[k]);
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+inline unsigned red_grid(uint64_t work_items, unsigned per_block = BLOCK)
+{
+    uint64_t g = ceil_div(work_items ? work_items : 1, per_block);
+    return static_cast<unsigned>(g > MAX_RED_BLOCKS ? MAX_RED_BLOCKS : g);
+}
+
+}  // namespace mi

@@ -1,0 +1,511 @@
+// mi_hec.hip -- heavy-edge coarsening (HEC) on gfx950 with the reference's
+// one-thread (Serial) semantics, heuristics.h:930-1031.
+//
+// Random stream.  The reference draws from Kokkos' Random_XorShift64_Pool
+// re-created from the same seed at every level (heuristics.h:947); with one
+// pool state the draws form one sequential stream: positions 0..n-1 are the
+// permutation keys (:617-621), position n+i is vertex i's random neighbour at
+// level 1 (:961-971), and at deeper levels the k-th isolated vertex (index
+// order) draws position n+k (:992-996).  The xorshift transition is linear
+// over GF(2)^64, so each thread jumps to its stream position with
+// precomputed powers T^(2^k) and then steps sequentially.
+//
+// Permutation.  BinSort with BinOp1D over [0, 2^64) and in-bin bubble sort
+// (Kokkos_Sort.hpp:241-262, 376-440) orders distinct keys ascending, so vperm
+// is exactly the argsort of the keys: a 64-bit LSD radix sort.
+//
+// Map construction.  The reference's two alternating CAS passes
+// (heuristics.h:633-693), run by one thread, have a closed form that is
+// evaluated here with data-parallel kernels (see hec_map below).
+#include "mi_kernels.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace mi {
+
+namespace {
+
+constexpr uint64_t XS_MULT = 2685821657736338717ULL;
+constexpr int XS_JUMPS = 48;  // stream positions < 2^48
+constexpr int DRAW_CH = 16;
+
+uint64_t host_xs_step(uint64_t s)
+{
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return s;
+}
+
+uint64_t host_apply(const uint64_t* M, uint64_t s)
+{
+    uint64_t r = 0;
+    for (int j = 0; j < 64; j++)
+        if ((s >> j) & 1u) r ^= M[j];
+    return r;
+}
+
+__device__ __forceinline__ uint64_t xs_step(uint64_t s)
+{
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return s;
+}
+
+__device__ __forceinline__ uint64_t apply_jump(const uint64_t* __restrict__ M, uint64_t s)
+{
+    uint64_t r = 0;
+#pragma unroll 16
+    for (int j = 0; j < 64; j++) r ^= M[j] & (0ull - ((s >> j) & 1ull));
+    return r;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_xs_draws(uint64_t s0, uint64_t first, uint64_t count,
+                                                    const uint64_t* __restrict__ jumps,
+                                                    uint64_t* __restrict__ out)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t idx0 = g * DRAW_CH;
+    if (idx0 >= count) return;
+    uint64_t p = first + idx0;  // state after p steps
+    uint64_t s = s0;
+    for (int k = 0; k < XS_JUMPS && p; k++, p >>= 1)
+        if (p & 1ull) s = apply_jump(jumps + (size_t)k * 64, s);
+#pragma unroll
+    for (int k = 0; k < DRAW_CH; k++) {
+        s = xs_step(s);
+        if (idx0 + k < count) out[idx0 + k] = s * XS_MULT - 1ull;
+    }
+}
+
+__global__ void k_scatter_pos(const uint32_t* __restrict__ vperm, uint32_t n, uint32_t* __restrict__ pos)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) pos[vperm[i]] = i;
+}
+
+// level 1: hn(i) = col[rowptr[i] + draw % deg], isolated: draw % n
+__global__ void k_hn_random(uint32_t n, const uint32_t* __restrict__ rowptr,
+                            const uint32_t* __restrict__ col, const uint64_t* __restrict__ draws,
+                            uint32_t* __restrict__ hn)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = rowptr[i], d = rowptr[i + 1] - b;
+    uint64_t r = draws[(uint64_t)n + i];
+    hn[i] = d ? col[b + (uint32_t)(r % d)] : (uint32_t)(r % n);
+}
+
+// level > 1: heaviest neighbour, ties to the last CSR index.  G lanes per
+// row; the lexicographic max of (weight, edge index) packed in 64 bits.
+template <int G>
+__global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col,
+                                                    const uint32_t* __restrict__ w,
+                                                    uint32_t* __restrict__ hn,
+                                                    uint32_t* __restrict__ iso)
+{
+    constexpr int RPB = BLOCK / G;
+    const int sub = threadIdx.x % G;
+    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
+         row += (uint64_t)gridDim.x * RPB) {
+        const uint32_t b = rowptr[row], e = rowptr[row + 1];
+        unsigned long long best = 0;
+        for (uint32_t k = b + sub; k < e; k += G) {
+            unsigned long long key = ((unsigned long long)(w ? w[k] : 1u) << 32) | k;
+            best = key > best ? key : best;
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            unsigned long long t = __shfl_xor(best, o, G);
+            best = t > best ? t : best;
+        }
+        if (sub == 0) {
+            if (e > b) {
+                hn[row] = col[(uint32_t)best];
+                iso[row] = 0;
+            } else {
+                iso[row] = 1;
+            }
+        }
+    }
+}
+
+__global__ void k_hn_iso(uint32_t n, const uint32_t* __restrict__ iso, const uint32_t* __restrict__ rank,
+                         const uint64_t* __restrict__ draws, uint32_t* __restrict__ hn)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && iso[i]) hn[i] = (uint32_t)(draws[(uint64_t)n + rank[i]] % n);
+}
+
+// ------------------------------------------------------------------ map
+enum : uint8_t {
+    FL_E = 1,      // pass 1 acts: pos(u) >= pos(hn(u))
+    FL_CR = 2,     // pass-1 creator
+    FL_R = 4,      // unassigned after pass 1
+    FL_ACT = 8,    // pass 2 acts
+    FL_NACT = 16,  // pass 2 claimed before acting
+    FL_PROP = 32   // pass-2 state already pushed to hn(u)
+};
+
+__global__ void k_map_mark(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
+                           uint8_t* __restrict__ fl, uint32_t* __restrict__ minpos,
+                           uint32_t* __restrict__ vcmap)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint32_t v = hn[u], pu = pos[u];
+    bool e = pu >= pos[v];
+    fl[u] = e ? FL_E : 0;
+    vcmap[u] = INF;
+    if (e && v != u) atomicMin(&minpos[v], pu);
+}
+
+__global__ void k_map_creators1(uint32_t n, const uint32_t* __restrict__ vperm,
+                                const uint32_t* __restrict__ hn, uint8_t* fl,
+                                const uint32_t* __restrict__ minpos, uint32_t* __restrict__ cflag)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    uint32_t u = vperm[i], v = hn[u];
+    uint8_t fu = fl[u];
+    bool cr = (fu & FL_E) && (v == u || (!(fl[v] & FL_E) && minpos[v] == i));
+    cflag[i] = cr ? 1u : 0u;
+    if (cr) fl[u] = fu | FL_CR;
+}
+
+__global__ void k_map_assign1(uint32_t n, const uint32_t* __restrict__ vperm,
+                              const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
+                              const uint8_t* __restrict__ fl, const uint32_t* __restrict__ minpos,
+                              const uint32_t* __restrict__ id, uint32_t* __restrict__ vcmap)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint8_t fu = fl[u];
+    if (fu & FL_CR) {
+        uint32_t c = id[pos[u]];
+        vcmap[u] = c;
+        if (hn[u] != u) vcmap[hn[u]] = c;
+    } else if (fu & FL_E) {
+        // follow hn while it stays inside E; positions strictly decrease
+        uint32_t x = u, root = u;
+        for (uint32_t step = 0; step <= n; step++) {
+            if (fl[x] & FL_CR) {
+                root = x;
+                break;
+            }
+            uint32_t v = hn[x];
+            if (fl[v] & FL_E) {
+                x = v;
+            } else {
+                root = vperm[minpos[v]];
+                break;
+            }
+        }
+        vcmap[u] = id[pos[root]];
+    }
+}
+
+__global__ void k_map_r(uint32_t n, const uint32_t* __restrict__ vcmap, uint8_t* __restrict__ fl)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u < n && vcmap[u] == INF) fl[u] |= FL_R;
+}
+
+__global__ void k_map_indeg(uint32_t n, const uint32_t* __restrict__ hn, const uint8_t* __restrict__ fl,
+                            uint32_t* __restrict__ indeg)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    if ((fl[u] & FL_R) && (fl[hn[u]] & FL_R)) atomicAdd(&indeg[hn[u]], 1u);
+}
+
+// pass 2: u acts iff no child w (hn(w) = u, both unassigned) acts
+__global__ void k_map_resolve(uint32_t n, uint8_t* fl, const uint32_t* __restrict__ indeg,
+                              const uint32_t* __restrict__ nonact, const uint32_t* __restrict__ hasact,
+                              uint32_t* __restrict__ unknown)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t left = 0;
+    if (u < n) {
+        uint8_t f = fl[u];
+        if ((f & FL_R) && !(f & (FL_ACT | FL_NACT))) {
+            if (hasact[u])
+                fl[u] = f | FL_NACT;
+            else if (nonact[u] == indeg[u])
+                fl[u] = f | FL_ACT;
+            else
+                left = 1;
+        }
+    }
+    left = wave_sum_u32(left);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && left) atomicAdd(unknown, left);
+}
+
+__global__ void k_map_propagate(uint32_t n, const uint32_t* __restrict__ hn, uint8_t* fl,
+                                uint32_t* __restrict__ nonact, uint32_t* __restrict__ hasact)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint8_t f = fl[u];
+    if (!(f & FL_R) || !(f & (FL_ACT | FL_NACT)) || (f & FL_PROP)) return;
+    uint32_t v = hn[u];
+    if (fl[v] & FL_R) {
+        if (f & FL_ACT)
+            hasact[v] = 1u;
+        else
+            atomicAdd(&nonact[v], 1u);
+    }
+    fl[u] = f | FL_PROP;
+}
+
+__global__ void k_map_min2(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
+                           const uint8_t* __restrict__ fl, uint32_t* __restrict__ minpos2)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint8_t f = fl[u];
+    if ((f & FL_R) && (f & FL_ACT) && (fl[hn[u]] & FL_R)) atomicMin(&minpos2[hn[u]], pos[u]);
+}
+
+__global__ void k_map_creators2(uint32_t n, const uint32_t* __restrict__ vperm,
+                                const uint32_t* __restrict__ hn, const uint8_t* __restrict__ fl,
+                                const uint32_t* __restrict__ minpos2, uint32_t* __restrict__ cflag)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    uint32_t u = vperm[i];
+    uint8_t f = fl[u];
+    bool cr = (f & FL_R) && (f & FL_ACT) && (fl[hn[u]] & FL_R) && minpos2[hn[u]] == i;
+    cflag[i] = cr ? 1u : 0u;
+}
+
+__global__ void k_map_assign2(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
+                              const uint8_t* __restrict__ fl, const uint32_t* __restrict__ minpos2,
+                              const uint32_t* __restrict__ id2, const uint32_t* __restrict__ nc1p,
+                              uint32_t* vcmap)
+{
+    uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint8_t f = fl[u];
+    if (!(f & FL_R)) return;
+    const uint32_t nc1 = *nc1p;
+    if (f & FL_ACT) {
+        uint32_t v = hn[u];
+        if (!(fl[v] & FL_R))
+            vcmap[u] = vcmap[v];
+        else
+            vcmap[u] = nc1 + id2[minpos2[v]];  // == own id when u is the creator
+    } else {
+        vcmap[u] = nc1 + id2[minpos2[u]];
+    }
+}
+
+// max over vcmap, so the host can verify every vertex got a cluster id < nc
+__global__ void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t v = i < n ? a[i] : 0u;
+    v = wave_max_u32(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, v);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
+
+void fill_u32(Ctx& c, uint32_t* p, uint64_t n, uint32_t v)
+{
+    if (!n) return;
+    k_fill_u32<<<grid1(n), BLOCK, 0, c.stream>>>(p, n, v);
+    MI_HIP(hipGetLastError());
+}
+
+const uint64_t* jump_table(Ctx& c)
+{
+    if (c.xs_jump) return c.xs_jump;
+    std::vector<uint64_t> h((size_t)XS_JUMPS * 64);
+    for (int j = 0; j < 64; j++) h[j] = host_xs_step(1ull << j);
+    for (int k = 1; k < XS_JUMPS; k++)
+        for (int j = 0; j < 64; j++)
+            h[(size_t)k * 64 + j] = host_apply(&h[(size_t)(k - 1) * 64], h[(size_t)(k - 1) * 64 + j]);
+    uint64_t* d = nullptr;
+    MI_HIP(hipMalloc(&d, h.size() * sizeof(uint64_t)));
+    MI_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    c.xs_jump = d;
+    return d;
+}
+
+
+
+}  // namespace
+
+uint64_t xs_pool_state(uint64_t seed)
+{
+    if (seed == 0) seed = 1318319ull;
+    uint64_t g = seed;
+    auto rnd = [&g]() -> int {
+        g = host_xs_step(g);
+        uint32_t u = (uint32_t)((g * XS_MULT) >> 16);
+        return (int)(u / 2u);
+    };
+    for (int i = 0; i < 17; i++) (void)rnd();
+    uint64_t n1 = (uint64_t)rnd(), n2 = (uint64_t)rnd(), n3 = (uint64_t)rnd(), n4 = (uint64_t)rnd();
+    uint64_t st = (n1 & 0xffff) | ((n2 & 0xffff) << 16) | ((n3 & 0xffff) << 32) | ((n4 & 0xffff) << 48);
+    return st ? st : 1318319ull;
+}
+
+void xs_draws(Ctx& c, uint64_t state0, uint64_t first, uint64_t count, uint64_t* draws)
+{
+    if (!count) return;
+    MI_CHECK(first + count < (1ull << XS_JUMPS), ST_ARG, "stream position too large");
+    const uint64_t* J = jump_table(c);
+    uint64_t threads = ceil_div(count, DRAW_CH);
+    k_xs_draws<<<(unsigned)ceil_div(threads, BLOCK), BLOCK, 0, c.stream>>>(state0, first, count, J, draws);
+    MI_HIP(hipGetLastError());
+}
+
+uint32_t hec_map(Ctx& c, const DGraph& g, int level, uint64_t seed, uint32_t* vcmap,
+                 uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
+{
+    const uint32_t n = g.n;
+    MI_CHECK(n > 0, ST_ARG, "empty level");
+    hipEvent_t ev[4];
+    if (t)
+        for (auto& e : ev) MI_HIP(hipEventCreate(&e));
+    if (t) MI_HIP(hipEventRecord(ev[0], c.stream));
+
+    // ---- random stream: positions [0, 2n)
+    DBuf<uint64_t> draws(&c.pool, 2ull * n);
+    xs_draws(c, xs_pool_state(seed), 0, 2ull * n, draws.get());
+
+    // ---- permutation: argsort of the first n draws
+    DBuf<uint64_t> keys(&c.pool, n), ktmp(&c.pool, n);
+    DBuf<uint32_t> vperm(&c.pool, n), vtmp(&c.pool, n), pos(&c.pool, n);
+    MI_HIP(hipMemcpyAsync(keys.get(), draws.get(), (size_t)n * 8, hipMemcpyDeviceToDevice, c.stream));
+    iota_u32(c, vperm.get(), n, 0);
+    radix_sort_pairs(c, keys.get(), vperm.get(), n, 0, 64, ktmp.get(), vtmp.get());
+    k_scatter_pos<<<grid1(n), BLOCK, 0, c.stream>>>(vperm.get(), n, pos.get());
+    MI_HIP(hipGetLastError());
+    keys.reset();
+    ktmp.reset();
+    vtmp.reset();
+    if (t) MI_HIP(hipEventRecord(ev[1], c.stream));
+
+    // ---- heavy neighbour
+    DBuf<uint32_t> hn(&c.pool, n);
+    if (level == 1) {
+        k_hn_random<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), draws.get(), hn.get());
+        MI_HIP(hipGetLastError());
+    } else {
+        DBuf<uint32_t> iso(&c.pool, n), rank(&c.pool, n);
+        const uint32_t avg = g.m / n;
+        int G = 1;
+        while (G < 64 && G * 2 <= (int)avg) G *= 2;
+        const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * G, BLOCK), 65535u * 4);
+        const uint32_t* W = g.W();
+        switch (G) {
+            case 1: k_hn_heavy<1><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            case 2: k_hn_heavy<2><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            case 4: k_hn_heavy<4><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            case 8: k_hn_heavy<8><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            case 16: k_hn_heavy<16><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            case 32: k_hn_heavy<32><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+            default: k_hn_heavy<64><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+        }
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, iso.get(), rank.get(), n, nullptr);
+        k_hn_iso<<<grid1(n), BLOCK, 0, c.stream>>>(n, iso.get(), rank.get(), draws.get(), hn.get());
+        MI_HIP(hipGetLastError());
+    }
+    draws.reset();
+    if (t) MI_HIP(hipEventRecord(ev[2], c.stream));
+
+    // ---- map construction, pass 1 (closed form)
+    DBuf<uint8_t> fl(&c.pool, n);
+    DBuf<uint32_t> minpos(&c.pool, n), id(&c.pool, n);
+    DBuf<uint32_t> totals(&c.pool, 4);
+    fill_u32(c, minpos.get(), n, INF);
+    k_map_mark<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(), vcmap);
+    MI_HIP(hipGetLastError());
+    k_map_creators1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
+                                                      id.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals.get() + 0);
+    k_map_assign1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), pos.get(), hn.get(), fl.get(),
+                                                    minpos.get(), id.get(), vcmap);
+    MI_HIP(hipGetLastError());
+
+    // ---- pass 2
+    DBuf<uint32_t> indeg(&c.pool, n), nonact(&c.pool, n), hasact(&c.pool, n);
+    MI_HIP(hipMemsetAsync(indeg.get(), 0, (size_t)n * 4, c.stream));
+    MI_HIP(hipMemsetAsync(nonact.get(), 0, (size_t)n * 4, c.stream));
+    MI_HIP(hipMemsetAsync(hasact.get(), 0, (size_t)n * 4, c.stream));
+    k_map_r<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, fl.get());
+    MI_HIP(hipGetLastError());
+    k_map_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), indeg.get());
+    MI_HIP(hipGetLastError());
+    // Rounds until every unassigned vertex is resolved; the forest formed by
+    // u -> hn(u) has strictly increasing positions, so the lowest unresolved
+    // vertex always resolves and at most n rounds are needed.
+    uint32_t rounds = 0;
+    for (;;) {
+        const int batch = 4;
+        for (int b = 0; b < batch; b++) {
+            MI_HIP(hipMemsetAsync(totals.get() + 2, 0, 4, c.stream));
+            k_map_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, fl.get(), indeg.get(), nonact.get(),
+                                                            hasact.get(), totals.get() + 2);
+            MI_HIP(hipGetLastError());
+            k_map_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), nonact.get(),
+                                                              hasact.get());
+            MI_HIP(hipGetLastError());
+        }
+        rounds += batch;
+        uint32_t left = 0;
+        read_words(c, totals.get() + 2, &left, 4);
+        if (left == 0) break;
+        MI_CHECK(rounds <= n + 8, ST_INTERNAL, "map construction did not converge");
+    }
+    fill_u32(c, minpos.get(), n, INF);
+    k_map_min2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get());
+    MI_HIP(hipGetLastError());
+    k_map_creators2<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
+                                                      id.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals.get() + 1);
+    k_map_assign2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(),
+                                                    id.get(), totals.get() + 0, vcmap);
+    MI_HIP(hipGetLastError());
+    if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
+
+    MI_HIP(hipMemsetAsync(totals.get() + 3, 0, 4, c.stream));
+    k_max_u32<<<grid1(n), BLOCK, 0, c.stream>>>(vcmap, n, totals.get() + 3);
+    MI_HIP(hipGetLastError());
+    uint32_t tot[4] = {0, 0, 0, 0};
+    read_words(c, totals.get(), tot, 16);
+    MI_CHECK(tot[0] + tot[1] >= 1 && tot[3] < tot[0] + tot[1], ST_INTERNAL,
+             "map construction left a vertex without a cluster id");
+    if (vperm_out)
+        MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    if (t) {
+        float a = 0, b = 0, d = 0;
+        MI_HIP(hipEventSynchronize(ev[3]));
+        MI_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        MI_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        MI_HIP(hipEventElapsedTime(&d, ev[2], ev[3]));
+        t->permute_ms += a;
+        t->heavy_ms += b;
+        t->map_ms += d;
+        for (auto& e : ev) MI_HIP(hipEventDestroy(e));
+    }
+    return tot[0] + tot[1];
+}
+
+}  // namespace mi

@@ -1,0 +1,88 @@
+// mi_kernels.hpp -- host-side launchers of the gfx950 kernels and the
+// device-resident graph / hierarchy objects they work on.
+#pragma once
+
+#include "mi_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mi {
+
+// A CSR graph resident in HBM.  Index and weight arrays are u32 like the
+// reference's sgp_vid_t / sgp_eid_t / sgp_wgt_t (definitions_kokkos.h:69-75).
+// unit == true means every edge weight is 1 and `w` is not stored.
+struct DGraph {
+    uint32_t n = 0;
+    uint32_t m = 0;
+    bool unit = true;
+    DBuf<uint32_t> rowptr, col, w;
+    // borrowed device arrays (caller-owned) when the graph wraps device memory
+    const uint32_t* b_rowptr = nullptr;
+    const uint32_t* b_col = nullptr;
+    const uint32_t* b_w = nullptr;
+    const uint32_t* R() const { return b_rowptr ? b_rowptr : rowptr.get(); }
+    const uint32_t* Cc() const { return b_col ? b_col : col.get(); }
+    const uint32_t* W() const { return unit ? nullptr : (b_rowptr ? b_w : w.get()); }
+};
+
+// One level of the multilevel hierarchy.
+struct DLevel {
+    DGraph g;
+    DBuf<uint32_t> vtx_w;  // vertex weights (cluster sizes)
+    DBuf<uint32_t> vcmap;  // map to the next coarser level (empty at the coarsest)
+    uint32_t nc_next = 0;
+};
+
+struct HecTimes {
+    double permute_ms = 0, heavy_ms = 0, map_ms = 0, build_ms = 0;
+};
+
+// ---------------------------------------------------------------- primitives
+void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n,
+                        uint32_t* total_dev);
+void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start);
+void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
+                      int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
+
+// ---------------------------------------------------------------- RNG / HEC
+// Host: state of the single generator of Kokkos' Random_XorShift64_Pool(seed)
+// on a one-thread backend (Kokkos_Random.hpp:863-898).
+uint64_t xs_pool_state(uint64_t seed);
+// Device: draws[p] = urand64 at stream position first+p, p < count.
+void xs_draws(Ctx& c, uint64_t state0, uint64_t first, uint64_t count, uint64_t* draws);
+
+// Deterministic HEC (reference Serial semantics). Writes vcmap (n) and
+// returns nc.  level >= 1 as in heuristics.h:930 (level 1 = random neighbour).
+uint32_t hec_map(Ctx& c, const DGraph& g, int level, uint64_t seed, uint32_t* vcmap,
+                 uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t);
+
+// Coarse graph P^T A P without diagonal, rows sorted, duplicates merged.
+void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
+                  const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
+
+// ---------------------------------------------------------------- eigen
+struct PowerStats {
+    uint64_t iters = 0;
+    int max_reached = 0;
+    double ms = 0;
+};
+// Power iteration on level graph g, in place on d_u (n doubles).
+void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st);
+// out[i] = in[vcmap[i]]
+void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, double* out);
+// u /= ||u||
+void normalize(Ctx& c, double* d_u, uint32_t n);
+// y = A x + (gb - wdeg) x : the shifted Laplacian product of one iteration
+// (asynchronous; gb_dev = device u32 from weighted_degree)
+void spmv_shift(Ctx& c, const DGraph& g, const uint32_t* wdeg, const uint32_t* gb_dev, const double* x,
+                double* y);
+void weighted_degree(Ctx& c, const DGraph& g, uint32_t* wdeg, uint32_t* gb_dev);
+
+// ---------------------------------------------------------------- partition
+// Sort-and-split of level-0 graph g by device eigenvector ev into device
+// array part; returns the edge cut.
+uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
+                           int perc_imbalance, uint32_t* part);
+
+}  // namespace mi

@@ -1,0 +1,122 @@
+// mi_partition.hip -- sort-and-split bisection on gfx950, sgpar.h:1969-2087
+// (sgp_compute_partition, local_search_alg = 0).
+//
+// The reference sorts (ev, u) pairs ascending with ties broken by vertex id
+// (vu_cmpfn_inc, sgpar.h:756-764), puts the first n - imbr sorted vertices in
+// part 0, then walks positions [imbl, imbr) once, moving a vertex to part 0
+// whenever the cut does not grow.  Here the FP64 keys are mapped to
+// order-preserving u64 (negatives bit-inverted, positives with the sign bit
+// set, -0.0 folded onto +0.0 because the reference compares them equal) and
+// radix-sorted with the vertex id as a stable value, which reproduces the
+// tie order.  The [imbl, imbr) walk is sequential by definition and runs in
+// one workgroup.
+#include "mi_kernels.hpp"
+
+namespace mi {
+
+namespace {
+
+inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
+
+__global__ void k_ev_keys(uint32_t n, const double* __restrict__ ev, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double x = ev[i];
+    uint64_t b = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
+    keys[i] = (b >> 63) ? ~b : (b | (1ull << 63));
+    vals[i] = i;
+}
+
+__global__ void k_assign(uint32_t n, const uint32_t* __restrict__ order, uint32_t imbl,
+                         uint32_t* __restrict__ part)
+{
+    uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) part[order[k]] = k < imbl ? 0u : 1u;
+}
+
+template <int G>
+__global__ void __launch_bounds__(BLOCK) k_cut(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                               const uint32_t* __restrict__ col,
+                                               const uint32_t* __restrict__ part,
+                                               unsigned long long* __restrict__ cut)
+{
+    constexpr int RPB = BLOCK / G;
+    const int sub = threadIdx.x % G;
+    uint32_t local = 0;
+    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
+         row += (uint64_t)gridDim.x * RPB) {
+        const uint32_t pr = part[row];
+        for (uint32_t k = rowptr[row] + sub; k < rowptr[row + 1]; k += G) local += part[col[k]] != pr;
+    }
+    local = wave_sum_u32(local);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && local) atomicAdd(cut, (unsigned long long)local);
+}
+
+// Sequential walk over sorted positions [imbl, imbr), sgpar.h:2041-2065.
+__global__ void __launch_bounds__(BLOCK) k_walk(uint32_t imbl, uint32_t imbr, const uint32_t* __restrict__ order,
+                                                const uint32_t* __restrict__ rowptr,
+                                                const uint32_t* __restrict__ col, uint32_t* part,
+                                                const unsigned long long* __restrict__ cut0,
+                                                unsigned long long* __restrict__ best_out)
+{
+    __shared__ int32_t lds[BLOCK / WAVE];
+    long long cur = (long long)*cut0, best = cur;
+    for (uint32_t i = imbl; i < imbr; i++) {
+        const uint32_t u = order[i];
+        int32_t upd = 0;
+        for (uint32_t k = rowptr[u] + threadIdx.x; k < rowptr[u + 1]; k += BLOCK)
+            upd += (part[col[k]] == 1u) ? 1 : -1;
+        for (int o = WAVE / 2; o > 0; o >>= 1) upd += __shfl_xor(upd, o, WAVE);
+        __syncthreads();
+        if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = upd;
+        __syncthreads();
+        long long tot = 0;
+        for (int k = 0; k < BLOCK / WAVE; k++) tot += lds[k];
+        cur += 2 * tot;
+        if (cur <= best) {
+            best = cur;
+            if (threadIdx.x == 0) part[u] = 0u;
+        }
+        __syncthreads();
+        __threadfence_block();
+    }
+    if (threadIdx.x == 0) *best_out = (unsigned long long)best;
+}
+
+}  // namespace
+
+// Device partition of level-0 graph g from device eigenvector ev.  part is a
+// device array of n.  Returns the edge cut (number of cut undirected edges).
+uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
+                           int perc_imbalance, uint32_t* part)
+{
+    const uint32_t n = g.n;
+    MI_CHECK(num_partitions == 2 || num_partitions == 4, ST_ARG, "num_partitions must be 2 or 4");
+    MI_CHECK(perc_imbalance >= 0 && perc_imbalance <= 100, ST_ARG, "imbalance must be in [0, 100]");
+    if (num_partitions == 4) num_partitions = 2;
+    const long max_part_size = (long)std::ceil(n / (double)num_partitions);
+    const uint32_t imbr = (uint32_t)std::floor(max_part_size * (1.0 + perc_imbalance / 100.0));
+    const uint32_t imbl = n - imbr;
+
+    DBuf<uint64_t> keys(&c.pool, n), ktmp(&c.pool, n);
+    DBuf<uint32_t> order(&c.pool, n), vtmp(&c.pool, n);
+    k_ev_keys<<<grid1(n), BLOCK, 0, c.stream>>>(n, ev, keys.get(), order.get());
+    MI_HIP(hipGetLastError());
+    radix_sort_pairs(c, keys.get(), order.get(), n, 0, 64, ktmp.get(), vtmp.get());
+    k_assign<<<grid1(n), BLOCK, 0, c.stream>>>(n, order.get(), imbl, part);
+    MI_HIP(hipGetLastError());
+    DBuf<unsigned long long> cut(&c.pool, 2);
+    MI_HIP(hipMemsetAsync(cut.get(), 0, 16, c.stream));
+    const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * 8, BLOCK), 65535ull * 4);
+    k_cut<8><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), part, cut.get());
+    MI_HIP(hipGetLastError());
+    k_walk<<<1, BLOCK, 0, c.stream>>>(imbl, imbr, order.get(), g.R(), g.Cc(), part, cut.get(), cut.get() + 1);
+    MI_HIP(hipGetLastError());
+    unsigned long long best = 0;
+    read_words(c, cut.get() + 1, &best, 8);
+    return (uint32_t)(best / 2);
+}
+
+}  // namespace mi

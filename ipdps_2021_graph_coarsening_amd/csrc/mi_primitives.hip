@@ -1,0 +1,252 @@
+// mi_primitives.hip -- device-wide exclusive scan and LSD radix sort for
+// gfx950.  Both are HBM-bound streaming kernels: the scan moves 8 B per item
+// (read + write), one radix pass moves 8 B (histogram read) + 2 x 12 B
+// (key/value read and scatter) per item.
+//
+// Radix ranking: each 256-thread tile walks its keys in 16 rounds of 256
+// (striped, so loads are coalesced and ranks follow input order); inside a
+// round, a wavefront finds the lanes sharing its 8-bit digit with 8 ballots
+// (64-bit lane masks), ranks itself by popcount below its lane, and the four
+// waves combine through LDS.  Per-tile digit counts are stored digit-major so
+// one exclusive scan over them yields every tile's scatter base.
+#include "mi_kernels.hpp"
+
+namespace mi {
+
+namespace {
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 2048
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds_w, uint32_t* total)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) lds_w[wid] = incl;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / WAVE; k++) {
+        uint32_t s = lds_w[k];
+        if (k < wid) wbase += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wbase + incl - v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                       uint32_t* __restrict__ tile_sums)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint64_t i = base + (uint64_t)k * BLOCK + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t t = block_sum_u32(s, lds);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                     const uint32_t* __restrict__ tile_offs,
+                                                     uint32_t* total_out)
+{
+    __shared__ uint32_t tile[SCAN_TILE];
+    __shared__ uint32_t lds_w[BLOCK / WAVE];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint64_t i = base + (uint64_t)k * BLOCK + threadIdx.x;
+        tile[k * BLOCK + threadIdx.x] = (i < n) ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = tile[threadIdx.x * SCAN_ITEMS + k];
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(s, lds_w, &tot) + (tile_offs ? tile_offs[blockIdx.x] : 0u);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        tile[threadIdx.x * SCAN_ITEMS + k] = run;
+        run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint64_t i = base + (uint64_t)k * BLOCK + threadIdx.x;
+        if (i < n) out[i] = tile[k * BLOCK + threadIdx.x];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == BLOCK - 1) {
+        // last thread of the last tile: its running value is the grand total
+        *total_out = run;
+    }
+}
+
+// ---------------------------------------------------------------- radix
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int R_ITEMS = 16;
+constexpr int R_TILE = BLOCK * R_ITEMS;  // 4096
+static_assert(RADIX == BLOCK, "one digit per thread in the tile bookkeeping");
+
+__global__ void __launch_bounds__(BLOCK) k_radix_hist(const uint64_t* __restrict__ keys, uint64_t n,
+                                                      int shift, uint32_t mask, uint32_t num_tiles,
+                                                      uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * R_TILE;
+#pragma unroll 4
+    for (int k = 0; k < R_ITEMS; k++) {
+        uint64_t i = base + (uint64_t)k * BLOCK + threadIdx.x;
+        if (i < n) {
+            uint32_t d = (uint32_t)(keys[i] >> shift) & mask;
+            atomicAdd(&h[d], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_radix_scatter(const uint64_t* __restrict__ keys_in,
+                                                         const uint32_t* __restrict__ vals_in,
+                                                         uint64_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ vals_out, uint64_t n,
+                                                         int shift, uint32_t mask, uint32_t num_tiles,
+                                                         const uint32_t* __restrict__ offs)
+{
+    __shared__ uint32_t gofs[RADIX];
+    __shared__ uint32_t run[RADIX];
+    __shared__ uint32_t wcnt[BLOCK / WAVE][RADIX];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    gofs[threadIdx.x] = offs[(uint64_t)threadIdx.x * num_tiles + blockIdx.x];
+    run[threadIdx.x] = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / WAVE; w++) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * R_TILE;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (WAVE - lane));
+    for (int r = 0; r < R_ITEMS; r++) {
+        uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
+        const bool active = i < n;
+        uint64_t key = 0;
+        uint32_t val = 0, d = 0;
+        if (active) {
+            key = keys_in[i];
+            val = vals_in[i];
+            d = (uint32_t)(key >> shift) & mask;
+        }
+        unsigned long long peers = __ballot(active);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; b++) {
+            const bool bit = (d >> b) & 1u;
+            unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (active && rank == 0) wcnt[wid][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (active) {
+            uint32_t p = gofs[d] + run[d] + rank;
+            for (int w = 0; w < wid; w++) p += wcnt[w][d];
+            keys_out[p] = key;
+            vals_out[p] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+#pragma unroll
+            for (int w = 0; w < BLOCK / WAVE; w++) {
+                add += wcnt[w][threadIdx.x];
+                wcnt[w][threadIdx.x] = 0;
+            }
+            run[threadIdx.x] += add;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_iota(uint32_t* out, uint64_t n, uint32_t start)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = start + (uint32_t)i;
+}
+
+}  // namespace
+
+void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev)
+{
+    if (n == 0) {
+        if (total_dev) MI_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), c.stream));
+        return;
+    }
+    const uint64_t tiles = ceil_div(n, SCAN_TILE);
+    MI_CHECK(tiles < (1ull << 31), ST_ARG, "scan too large");
+    if (tiles == 1) {
+        k_scan_down<<<1, BLOCK, 0, c.stream>>>(in, out, n, nullptr, total_dev);
+        MI_HIP(hipGetLastError());
+        return;
+    }
+    DBuf<uint32_t> sums(&c.pool, tiles);
+    k_scan_reduce<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, n, sums.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, sums.get(), sums.get(), tiles, nullptr);
+    k_scan_down<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, out, n, sums.get(), total_dev);
+    MI_HIP(hipGetLastError());
+}
+
+void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start)
+{
+    if (!n) return;
+    k_iota<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, c.stream>>>(out, n, start);
+    MI_HIP(hipGetLastError());
+}
+
+void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
+                      int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp)
+{
+    MI_CHECK(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, ST_ARG, "radix bit range");
+    if (n <= 1 || begin_bit == end_bit) return;
+    const uint64_t tiles = ceil_div(n, R_TILE);
+    MI_CHECK(tiles * RADIX < (1ull << 32), ST_ARG, "radix sort too large");
+    DBuf<uint32_t> hist(&c.pool, tiles * RADIX);
+    uint64_t* ksrc = keys;
+    uint32_t* vsrc = vals;
+    uint64_t* kdst = keys_tmp;
+    uint32_t* vdst = vals_tmp;
+    for (int shift = begin_bit; shift < end_bit; shift += RADIX_BITS) {
+        const int bits = (end_bit - shift) < RADIX_BITS ? (end_bit - shift) : RADIX_BITS;
+        const uint32_t mask = (1u << bits) - 1u;
+        k_radix_hist<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles,
+                                                             hist.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, hist.get(), hist.get(), tiles * RADIX, nullptr);
+        k_radix_scatter<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
+                                                                (uint32_t)tiles, hist.get());
+        MI_HIP(hipGetLastError());
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+    }
+    if (ksrc != keys) {
+        MI_HIP(hipMemcpyAsync(keys, ksrc, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, c.stream));
+        MI_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
+    }
+}
+
+}  // namespace mi
