@@ -669,32 +669,7 @@ int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y)
 int mi_bench_spmv(mi_ctx* ctx, const mi_graph* g, int reps, double* avg_ms, double* alg_bytes)
 {
     if (!ctx || !g || reps < 1 || !avg_ms || !alg_bytes) return ST_ARG;
-    return guard(ctx, [&] {
-        Ctx& c = ctx->c;
-        const uint32_t n = g->g.n;
-        const uint64_t m = g->g.m;
-        DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
-        DBuf<double> dx(&c.pool, n), dy(&c.pool, n);
-        weighted_degree(c, g->g, wdeg.get(), gb.get());
-        std::vector<double> ones(n, 1.0);
-        h2d(c, dx.get(), ones.data(), (size_t)n * 8);
-        spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());  // warm-up
-        MI_HIP(hipStreamSynchronize(c.stream));
-        hipEvent_t e0, e1;
-        MI_HIP(hipEventCreate(&e0));
-        MI_HIP(hipEventCreate(&e1));
-        MI_HIP(hipEventRecord(e0, c.stream));
-        for (int r = 0; r < reps; r++) spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());
-        MI_HIP(hipEventRecord(e1, c.stream));
-        MI_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        MI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        MI_HIP(hipEventDestroy(e0));
-        MI_HIP(hipEventDestroy(e1));
-        *avg_ms = ms / reps;
-        // rowptr + col (+ w) + wdeg + x gather (n distinct, counted once) + y
-        *alg_bytes = 4.0 * (n + 1) + 4.0 * m + (g->g.unit ? 0.0 : 4.0 * m) + 4.0 * n + 8.0 * n + 8.0 * n;
-    });
+    return guard(ctx, [&] { *avg_ms = bench_power_step(ctx->c, g->g, reps, alg_bytes); });
 }
 
 int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit)

@@ -47,7 +47,7 @@ enum Status : int {
         if (!(cond)) throw ::mi::Error((code), (msg));                \
     } while (0)
 
-inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Caching device allocator.  Blocks are rounded up to a power of two
 // (minimum 256 B) and recycled per size class; everything is stream-ordered
@@ -122,6 +122,10 @@ enum Scalar : int {
     SC_NUMER = 4,   // D-orthogonalisation numerator
     SC_DENOM = 5,   // D-orthogonalisation denominator
     SC_TMP = 6,
+    SC_A = 7,       // input shift of the next iteration: x = (y - a) * s
+    SC_S = 8,       // input scale of the next iteration
+    SC_AP = 9,      // shift used by the last iteration
+    SC_SP = 10,     // scale used by the last iteration
     SC_COUNT = 16
 };
 
@@ -131,10 +135,7 @@ enum Counter : int {
     CT_RED2 = 1,
     CT_DONE = 2,     // power iteration finished flag
     CT_MAXR = 3,     // iteration cap reached
-    CT_A = 4,        // general counters
-    CT_B = 5,
-    CT_C = 6,
-    CT_D = 7,
+    CT_ERR = 4,      // a kernel found a violated size invariant
     CT_COUNT = 16
 };
 

@@ -62,6 +62,19 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
 
 // ---------------------------------------------------------------- eigen
+// Load-balanced work list of one level for the fused power-iteration step:
+// item i is a tile of short rows [it_a, it_b) or, with the top bit of it_b
+// set, chunk (it_b & ~top) of the long row it_a.
+struct LevelPlan {
+    DBuf<uint32_t> it_a, it_b, long_rows, lcnt;
+    DBuf<double> lpart;
+    uint32_t n_items = 0, n_long = 0;
+};
+void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
+// Mean time (ms) of `reps` launches of the fused step on g; *alg_bytes gets
+// the algorithmic bytes of one launch.
+double bench_power_step(Ctx& c, const DGraph& g, int reps, double* alg_bytes);
+
 struct PowerStats {
     uint64_t iters = 0;
     int max_reached = 0;

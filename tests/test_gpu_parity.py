@@ -137,6 +137,50 @@ def test_spmv_shift(ctx):
     assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
+def _hub_graph(seed=0):
+    """Random graph plus hubs of degree 12000 (3 chunks of 4096), 4096
+    (exactly one chunk), 1500 (one chunk) and 1025 (just above the tile
+    limit) -- every long-row path of the fused step."""
+    rng = np.random.default_rng(seed)
+    n = 30000
+    m = 90000
+    u = [rng.integers(0, n, m)]
+    v = [rng.integers(0, n, m)]
+    for hub, deg in ((0, 12000), (1, 4096), (2, 1500), (3, 1025)):
+        nb = rng.choice(np.arange(4, n), deg, replace=False)
+        u.append(np.full(deg, hub))
+        v.append(nb)
+    u = np.concatenate(u)
+    v = np.concatenate(v)
+    w = rng.integers(1, 9, len(u))
+    return graphs.from_edges(n, u, v, w, lcc=True)
+
+
+def test_spmv_shift_hubs(ctx):
+    rp, col, w = _hub_graph()
+    deg = np.diff(rp)
+    assert deg.max() > 8192 and np.count_nonzero(deg > 1024) >= 4
+    x = np.random.default_rng(5).standard_normal(len(rp) - 1)
+    y = ctx.graph(rp, col, w).spmv_shift(x)
+    src = np.repeat(np.arange(len(rp) - 1), deg)
+    wdeg = np.bincount(src, weights=w, minlength=len(rp) - 1)
+    gb = max(int(2 * wdeg.max()), 2)
+    ref = (gb - wdeg) * x + np.bincount(src, weights=w * x[col], minlength=len(rp) - 1)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_power_iter_hubs_vs_oracle_and_deterministic(ctx):
+    rp, col, w = _hub_graph(1)
+    u0 = np.random.default_rng(2).random(len(rp) - 1)
+    g = ctx.graph(rp, col, w)
+    u, it, _ = g.power_iter(u0, tol=1e-10)
+    u2, it2, _ = g.power_iter(u0, tol=1e-10)
+    assert it == it2 and np.array_equal(u, u2)  # fixed-order reductions
+    ou, oit, _, _ = oracle.power_iter(rp, col, w, u0, tol=1e-10)
+    assert abs(it - oit) <= max(2, ITER_SLACK * oit)
+    assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
+
+
 @pytest.mark.parametrize("normLap", [0, 1])
 def test_power_iter_vs_oracle(ctx, normLap):
     rng = np.random.default_rng(4 + normLap)
