@@ -73,6 +73,12 @@ void mi_ctx_destroy(mi_ctx* ctx);
 const char* mi_last_error(const mi_ctx* ctx);
 /* Returns cached pool blocks to the runtime. */
 int mi_ctx_trim(mi_ctx* ctx);
+/* Tuning of the power-iteration step: consecutive tiles of short rows per
+   workgroup and consecutive long-row items per wavefront (0 = automatic:
+   about 8 workgroups and 64 wavefronts per CU).  Applies to levels planned
+   after the call.  No counterpart in the reference (it has no GPU work
+   decomposition of its own). */
+int mi_ctx_set_tuning(mi_ctx* ctx, uint32_t tiles_per_block, uint32_t items_per_wave);
 
 /* ---------------------------------------------------------------- graphs */
 /*

@@ -73,6 +73,7 @@ _SIGS = [
     ("mi_ctx_destroy", None, [vp]),
     ("mi_last_error", C.c_char_p, [vp]),
     ("mi_ctx_trim", C.c_int, [vp]),
+    ("mi_ctx_set_tuning", C.c_int, [vp, C.c_uint32, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -161,6 +162,11 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_tuning(self, tiles_per_block: int = 0, items_per_wave: int = 0) -> None:
+        """Power-step work per workgroup / wavefront (0 = automatic)."""
+        self.check(self._L.mi_ctx_set_tuning(self.h, tiles_per_block, items_per_wave),
+                   "mi_ctx_set_tuning")
 
     # ------------------------------------------------------------ graphs
     def graph(self, rowptr, col, w=None) -> "Graph":

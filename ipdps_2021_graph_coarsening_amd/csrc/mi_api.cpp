@@ -222,11 +222,14 @@ void build_hierarchy(Ctx& c, const DGraph& g, uint64_t seed, std::vector<DLevel>
         level++;
         DLevel& a = lv.back();
         a.vcmap = DBuf<uint32_t>(&c.pool, a.g.n);
-        const uint32_t nc = hec_map(c, a.g, level, seed, a.vcmap.get(), nullptr, nullptr, st ? &ht : nullptr);
+        DBuf<uint32_t> roc;
+        build_row_index(c, a.g, roc);
+        const uint32_t nc =
+            hec_map(c, a.g, roc.get(), level, seed, a.vcmap.get(), nullptr, nullptr, st ? &ht : nullptr);
         a.nc_next = nc;
         DLevel nl;
         auto tb = std::chrono::steady_clock::now();
-        build_coarse(c, a.g, a.vcmap.get(), nc, a.vtx_w.get(), nl.g, nl.vtx_w);
+        build_coarse(c, a.g, roc.get(), a.vcmap.get(), nc, a.vtx_w.get(), nl.g, nl.vtx_w);
         if (st) {
             MI_HIP(hipStreamSynchronize(c.stream));
             build_ms += ms_since(tb);
@@ -328,6 +331,14 @@ int mi_ctx_trim(mi_ctx* ctx)
 {
     if (!ctx) return ST_ARG;
     return guard(ctx, [&] { ctx->c.pool.trim(); });
+}
+
+int mi_ctx_set_tuning(mi_ctx* ctx, uint32_t tiles_per_block, uint32_t items_per_wave)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.tune_tiles_per_block = tiles_per_block;
+    ctx->c.tune_items_per_wave = items_per_wave;
+    return ST_OK;
 }
 
 int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint32_t* w,
@@ -620,7 +631,9 @@ int mi_hec_level(mi_ctx* ctx, const mi_graph* g, int level, uint64_t seed, uint3
         Ctx& c = ctx->c;
         const uint32_t n = g->g.n;
         DBuf<uint32_t> dv(&c.pool, n), dp(&c.pool, n), dh(&c.pool, n);
-        *nc = hec_map(c, g->g, level, seed, dv.get(), dp.get(), dh.get(), nullptr);
+        DBuf<uint32_t> roc;
+        build_row_index(c, g->g, roc);
+        *nc = hec_map(c, g->g, roc.get(), level, seed, dv.get(), dp.get(), dh.get(), nullptr);
         d2h(c, vcmap, dv.get(), (size_t)n * 4);
         if (vperm) d2h(c, vperm, dp.get(), (size_t)n * 4);
         if (hn) d2h(c, hn, dh.get(), (size_t)n * 4);
@@ -643,7 +656,9 @@ int mi_build_coarse(mi_ctx* ctx, const mi_graph* g, const uint32_t* vcmap, uint3
         auto G = std::make_unique<mi_graph>();
         G->ctx = ctx;
         DBuf<uint32_t> cvw;
-        build_coarse(c, g->g, dv.get(), nc, nullptr, G->g, cvw);
+        DBuf<uint32_t> roc;
+        build_row_index(c, g->g, roc);
+        build_coarse(c, g->g, roc.get(), dv.get(), nc, nullptr, G->g, cvw);
         if (c_vtx_w) d2h(c, c_vtx_w, cvw.get(), (size_t)nc * 4);
         MI_HIP(hipStreamSynchronize(c.stream));
         *out = G.release();
@@ -658,9 +673,11 @@ int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y)
         const uint32_t n = g->g.n;
         DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
         DBuf<double> dx(&c.pool, n), dy(&c.pool, n);
-        weighted_degree(c, g->g, wdeg.get(), gb.get());
+        LevelPlan plan;
+        plan_level(c, g->g, plan);
+        weighted_degree(c, g->g, plan, wdeg.get(), gb.get());
         h2d(c, dx.get(), x, (size_t)n * 8);
-        spmv_shift(c, g->g, wdeg.get(), gb.get(), dx.get(), dy.get());
+        spmv_shift(c, g->g, plan, wdeg.get(), gb.get(), dx.get(), dy.get());
         d2h(c, y, dy.get(), (size_t)n * 8);
         MI_HIP(hipStreamSynchronize(c.stream));
     });

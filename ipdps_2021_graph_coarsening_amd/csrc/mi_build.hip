@@ -7,9 +7,10 @@
 // gfx950 form: every fine edge (i, j) emits the key (vcmap[i], vcmap[j])
 // packed in 2B bits (B = bits of nc); intra-cluster edges get the sentinel
 // row nc and sort to the end.  One LSD radix sort over 2B bits groups each
-// coarse row with its columns ascending; heads of equal-key runs are counted
-// with a scan, weights are merged with u32 atomics (exact and order-free) and
-// row lengths fall out of the head counts.
+// coarse row with its columns ascending.  Heads of equal-key runs are
+// numbered with a scan; run weights and per-row counts are summed inside a
+// wavefront first (runs are contiguous after the sort) and merged with one
+// u32 atomic per run per wavefront -- exact and order-free.
 #include "mi_kernels.hpp"
 
 namespace mi {
@@ -18,30 +19,27 @@ namespace {
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
-template <int G>
-__global__ void __launch_bounds__(BLOCK) k_emit_edges(uint32_t n, const uint32_t* __restrict__ rowptr,
+// one lane per fine entry; the row comes from the row index
+__global__ void __launch_bounds__(BLOCK) k_emit_edges(uint64_t m, const uint32_t* __restrict__ rowptr,
+                                                      const uint32_t* __restrict__ roc,
                                                       const uint32_t* __restrict__ col,
                                                       const uint32_t* __restrict__ w,
-                                                      const uint32_t* __restrict__ vcmap,
-                                                      const uint32_t* __restrict__ vtx_w, uint32_t nc,
-                                                      int B, uint64_t* __restrict__ keys,
-                                                      uint32_t* __restrict__ vals,
-                                                      uint32_t* __restrict__ c_vtx_w)
+                                                      const uint32_t* __restrict__ vcmap, uint32_t nc, int B,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
-    constexpr int RPB = BLOCK / G;
-    const int sub = threadIdx.x % G;
-    const uint64_t sentinel = (uint64_t)nc << B;
-    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
-         row += (uint64_t)gridDim.x * RPB) {
-        const uint32_t b = rowptr[row], e = rowptr[row + 1];
-        const uint32_t cu = vcmap[row];
-        if (sub == 0) atomicAdd(&c_vtx_w[cu], vtx_w ? vtx_w[row] : 1u);
-        for (uint32_t k = b + sub; k < e; k += G) {
-            const uint32_t cv = vcmap[col[k]];
-            keys[k] = (cu == cv) ? sentinel : (((uint64_t)cu << B) | cv);
-            vals[k] = w ? w[k] : 1u;
-        }
-    }
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t r = entry_row((uint32_t)e, roc, rowptr);
+    const uint32_t cu = vcmap[r], cv = vcmap[col[e]];
+    keys[e] = (cu == cv) ? ((uint64_t)nc << B) : (((uint64_t)cu << B) | cv);
+    vals[e] = w ? w[e] : 1u;
+}
+
+__global__ void k_cluster_weights(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ vtx_w,
+                                  uint32_t* __restrict__ c_vtx_w)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&c_vtx_w[vcmap[i]], vtx_w ? vtx_w[i] : 1u);
 }
 
 __global__ void k_heads(uint64_t m, const uint64_t* __restrict__ keys, uint64_t sentinel,
@@ -53,22 +51,32 @@ __global__ void k_heads(uint64_t m, const uint64_t* __restrict__ keys, uint64_t 
     head[k] = (key < sentinel && (k == 0 || keys[k - 1] != key)) ? 1u : 0u;
 }
 
-__global__ void k_write_coarse(uint64_t m, const uint64_t* __restrict__ keys,
-                               const uint32_t* __restrict__ vals, const uint32_t* __restrict__ uidx,
-                               uint64_t sentinel, int B, uint32_t* __restrict__ col_c,
-                               uint32_t* __restrict__ w_c, uint32_t* __restrict__ rowcnt)
+__global__ void __launch_bounds__(BLOCK) k_write_coarse(uint64_t m, const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const uint32_t* __restrict__ uidx, uint64_t sentinel, int B,
+                                                        uint32_t* __restrict__ col_c, uint32_t* __restrict__ w_c,
+                                                        uint32_t* __restrict__ rowcnt)
 {
-    uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= m) return;
-    const uint64_t key = keys[k];
-    if (key >= sentinel) return;
-    const bool head = (k == 0 || keys[k - 1] != key);
-    const uint32_t grp = head ? uidx[k] : uidx[k] - 1u;
-    atomicAdd(&w_c[grp], vals[k]);
-    if (head) {
-        col_c[grp] = (uint32_t)(key & ((1ull << B) - 1ull));
-        atomicAdd(&rowcnt[(uint32_t)(key >> B)], 1u);
+    const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t grp = INF, row = INF, v = 0, one = 0;
+    bool head = false;
+    if (k < m) {
+        const uint64_t key = keys[k];
+        if (key < sentinel) {
+            head = (k == 0 || keys[k - 1] != key);
+            grp = head ? uidx[k] : uidx[k] - 1u;
+            row = (uint32_t)(key >> B);
+            v = vals[k];
+            one = head ? 1u : 0u;
+            if (head) col_c[grp] = (uint32_t)(key & ((1ull << B) - 1ull));
+        }
     }
+    auto plus = [](uint32_t x, uint32_t y) { return x + y; };
+    bool first;
+    const uint32_t ws = wave_seg_reduce(grp, v, plus, &first);
+    if (first && grp != INF) atomicAdd(&w_c[grp], ws);
+    const uint32_t rc = wave_seg_reduce(row, one, plus, &first);
+    if (first && row != INF && rc) atomicAdd(&rowcnt[row], rc);
 }
 
 int bits_for(uint32_t nc)
@@ -80,7 +88,7 @@ int bits_for(uint32_t nc)
 
 }  // namespace
 
-void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
+void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w)
 {
     const uint32_t n = g.n;
@@ -91,28 +99,16 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
 
     c_vtx_w = DBuf<uint32_t>(&c.pool, nc);
     MI_HIP(hipMemsetAsync(c_vtx_w.get(), 0, (size_t)nc * 4, c.stream));
+    k_cluster_weights<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, vtx_w, c_vtx_w.get());
+    MI_HIP(hipGetLastError());
 
     DBuf<uint64_t> keys(&c.pool, m), ktmp(&c.pool, m);
     DBuf<uint32_t> vals(&c.pool, m), vtmp(&c.pool, m);
-    const uint32_t avg = n ? (uint32_t)(m / n) : 0;
-    int G = 1;
-    while (G < 64 && G * 2 <= (int)avg) G *= 2;
-    const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * G, BLOCK), 65535ull * 4);
-    const uint32_t* W = g.W();
-#define EMIT(GG)                                                                                  \
-    k_emit_edges<GG><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, vcmap, vtx_w, nc, B,   \
-                                                   keys.get(), vals.get(), c_vtx_w.get())
-    switch (G) {
-        case 1: EMIT(1); break;
-        case 2: EMIT(2); break;
-        case 4: EMIT(4); break;
-        case 8: EMIT(8); break;
-        case 16: EMIT(16); break;
-        case 32: EMIT(32); break;
-        default: EMIT(64); break;
+    if (m) {
+        k_emit_edges<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, keys.get(),
+                                                       vals.get());
+        MI_HIP(hipGetLastError());
     }
-#undef EMIT
-    MI_HIP(hipGetLastError());
     radix_sort_pairs(c, keys.get(), vals.get(), m, 0, 2 * B, ktmp.get(), vtmp.get());
     ktmp.reset();
     vtmp.reset();

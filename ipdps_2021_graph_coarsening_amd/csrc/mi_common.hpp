@@ -126,6 +126,10 @@ enum Scalar : int {
     SC_S = 8,       // input scale of the next iteration
     SC_AP = 9,      // shift used by the last iteration
     SC_SP = 10,     // scale used by the last iteration
+    SC_T1 = 11,     // sums S1..S4 of the tile kernel of the current iteration
+    SC_T2 = 12,
+    SC_T3 = 13,
+    SC_T4 = 14,
     SC_COUNT = 16
 };
 
@@ -152,6 +156,8 @@ struct Ctx {
     unsigned long long* niter = nullptr;  // iteration counter of the power iteration
     uint32_t* host_pinned = nullptr;      // small pinned staging area (64 words)
     uint64_t* xs_jump = nullptr;          // T^(2^k) of the xorshift transition, 48 x 64 words
+    uint32_t tune_items_per_wave = 0;     // power-step row items per wavefront (0 = automatic)
+    uint32_t tune_tiles_per_block = 0;    // power-step tiles per workgroup (0 = automatic)
     ~Ctx();
 };
 
@@ -214,11 +220,43 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* lds)
     return s;
 }
 
-__device__ __forceinline__ double load_acquire_f64(const double* p)
+// Cross-workgroup communication inside one launch.  Values that another
+// workgroup reads in the same launch are written and read with device-scope
+// atomic accesses, which are performed where every XCD sees them.  Before a
+// wavefront signals (arrival counter) it waits until its own such stores are
+// acknowledged (wait_stores: s_waitcnt with every counter at zero); a
+// wavefront whose stores another wavefront of the workgroup signals for
+// waits likewise before the workgroup barrier.  No agent-scope fence is
+// used: on a GPU with one L2 per XCD such a fence writes back the whole L2,
+// and doing that once per workgroup costs far more than the kernel's own
+// traffic, while the device-scope stores need no write-back.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ __forceinline__ double ld_dev_f64(const double* p)
 {
     unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return __longlong_as_double(static_cast<long long>(b));
+}
+
+__device__ __forceinline__ void st_dev_f64(double* p, double v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       static_cast<unsigned long long>(__double_as_longlong(v)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits for this wavefront's device-scope writes, then counts its arrival.
+__device__ __forceinline__ uint32_t arrive(uint32_t* counter)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    wait_stores();
+    return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void rearm(uint32_t* counter)
+{
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // "Last block" finisher of a grid-wide reduction: block b publishes its
@@ -236,25 +274,99 @@ __device__ bool grid_reduce_last(const double (&mine)[K], double* partials, uint
     for (int k = 0; k < K; k++) tot[k] = block_sum(mine[k], red_lds[k]);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < K; k++) partials[blockIdx.x * K + k] = tot[k];
-        __threadfence();
-        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == gridDim.x - 1);
+        for (int k = 0; k < K; k++) st_dev_f64(&partials[blockIdx.x * K + k], tot[k]);
+        am_last = (arrive(counter) == gridDim.x - 1);
     }
     __syncthreads();
     if (!am_last) return false;
-    __threadfence();
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] = 0;
     for (uint32_t b = threadIdx.x; b < gridDim.x; b += BLOCK) {
 #pragma unroll
-        for (int k = 0; k < K; k++) acc[k] += load_acquire_f64(&partials[b * K + k]);
+        for (int k = 0; k < K; k++) acc[k] += ld_dev_f64(&partials[b * K + k]);
     }
 #pragma unroll
     for (int k = 0; k < K; k++) out[k] = block_sum(acc[k], red_lds[k]);
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) rearm(counter);
+    return true;
+}
+
+// Two-level form for large grids.  Block blk (of nblk) belongs to group
+// blk / RED_GROUP; the last block of a group to arrive sums that group's
+// partials in block order into gpart[group]; the last group to finish sums
+// the group totals in group order and returns true with them in out.
+// Deterministic for a fixed grid.  gcnt[] (one per group) and *fcnt start at
+// zero and are re-armed by the blocks that consume them.
+constexpr uint32_t RED_GROUP = 64;
+
+// K sums over a block with one barrier pair: wavefront shuffles, then the
+// four wave totals combined in wave order.  Result valid in every thread.
+template <int K>
+__device__ __forceinline__ void block_sum_k(const double (&v)[K], double (&out)[K], double (*lds)[BLOCK / WAVE])
+{
+    double w[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) w[k] = wave_sum(v[k]);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k][wid] = w[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < BLOCK / WAVE; j++) s += lds[k][j];
+        out[k] = s;
+    }
+}
+
+template <int K>
+__device__ bool grid_reduce2(const double (&mine)[K], uint32_t blk, uint32_t nblk, double* partials,
+                             double* gpart, uint32_t* gcnt, uint32_t* fcnt, double (&out)[K])
+{
+    __shared__ double red2[K][BLOCK / WAVE];
+    __shared__ int stage;
+    double tot[K];
+    block_sum_k<K>(mine, tot, red2);
+    const uint32_t g = blk / RED_GROUP;
+    const uint32_t gbase = g * RED_GROUP;
+    const uint32_t gsize = (nblk - gbase) < RED_GROUP ? (nblk - gbase) : RED_GROUP;
+    const uint32_t ngroups = (nblk + RED_GROUP - 1) / RED_GROUP;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) st_dev_f64(&partials[(size_t)blk * K + k], tot[k]);
+        stage = (arrive(&gcnt[g]) == gsize - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (stage == 0) return false;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    if (threadIdx.x < gsize) {
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = ld_dev_f64(&partials[(size_t)(gbase + threadIdx.x) * K + k]);
+    }
+    block_sum_k<K>(acc, tot, red2);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) st_dev_f64(&gpart[(size_t)g * K + k], tot[k]);
+        rearm(&gcnt[g]);
+        stage = (arrive(fcnt) == ngroups - 1) ? 2 : 0;
+    }
+    __syncthreads();
+    if (stage != 2) return false;
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    for (uint32_t j = threadIdx.x; j < ngroups; j += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += ld_dev_f64(&gpart[(size_t)j * K + k]);
+    }
+    block_sum_k<K>(acc, out, red2);
+    if (threadIdx.x == 0) rearm(fcnt);
     return true;
 }
 

@@ -98,38 +98,39 @@ __global__ void k_hn_random(uint32_t n, const uint32_t* __restrict__ rowptr,
     hn[i] = d ? col[b + (uint32_t)(r % d)] : (uint32_t)(r % n);
 }
 
-// level > 1: heaviest neighbour, ties to the last CSR index.  G lanes per
-// row; the lexicographic max of (weight, edge index) packed in 64 bits.
-template <int G>
-__global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint32_t n, const uint32_t* __restrict__ rowptr,
-                                                    const uint32_t* __restrict__ col,
+// level > 1: heaviest neighbour, ties to the last CSR index: per row, the max
+// of (weight, edge index) packed in 64 bits.  One lane per entry; a
+// wavefront reduces each run of equal rows and its first lane merges the
+// run's max with one atomicMax, so hub rows cost no serial walk.
+__global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint64_t m, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ roc,
                                                     const uint32_t* __restrict__ w,
-                                                    uint32_t* __restrict__ hn,
-                                                    uint32_t* __restrict__ iso)
+                                                    unsigned long long* __restrict__ best)
 {
-    constexpr int RPB = BLOCK / G;
-    const int sub = threadIdx.x % G;
-    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
-         row += (uint64_t)gridDim.x * RPB) {
-        const uint32_t b = rowptr[row], e = rowptr[row + 1];
-        unsigned long long best = 0;
-        for (uint32_t k = b + sub; k < e; k += G) {
-            unsigned long long key = ((unsigned long long)(w ? w[k] : 1u) << 32) | k;
-            best = key > best ? key : best;
-        }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            unsigned long long t = __shfl_xor(best, o, G);
-            best = t > best ? t : best;
-        }
-        if (sub == 0) {
-            if (e > b) {
-                hn[row] = col[(uint32_t)best];
-                iso[row] = 0;
-            } else {
-                iso[row] = 1;
-            }
-        }
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t row = INF;
+    unsigned long long key = 0;
+    if (e < m) {
+        row = entry_row((uint32_t)e, roc, rowptr);
+        key = ((unsigned long long)(w ? w[e] : 1u) << 32) | e;
+    }
+    bool head;
+    key = wave_seg_reduce(row, key, [](unsigned long long x, unsigned long long y) { return x > y ? x : y; },
+                          &head);
+    if (head && row != INF) atomicMax(&best[row], key);
+}
+
+__global__ void k_hn_from_best(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                               const unsigned long long* __restrict__ best, uint32_t* __restrict__ hn,
+                               uint32_t* __restrict__ iso)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (rowptr[i + 1] > rowptr[i]) {
+        hn[i] = col[(uint32_t)best[i]];
+        iso[i] = 0;
+    } else {
+        iso[i] = 1;
     }
 }
 
@@ -371,7 +372,7 @@ void xs_draws(Ctx& c, uint64_t state0, uint64_t first, uint64_t count, uint64_t*
     MI_HIP(hipGetLastError());
 }
 
-uint32_t hec_map(Ctx& c, const DGraph& g, int level, uint64_t seed, uint32_t* vcmap,
+uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed, uint32_t* vcmap,
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
 {
     const uint32_t n = g.n;
@@ -405,20 +406,13 @@ uint32_t hec_map(Ctx& c, const DGraph& g, int level, uint64_t seed, uint32_t* vc
         MI_HIP(hipGetLastError());
     } else {
         DBuf<uint32_t> iso(&c.pool, n), rank(&c.pool, n);
-        const uint32_t avg = g.m / n;
-        int G = 1;
-        while (G < 64 && G * 2 <= (int)avg) G *= 2;
-        const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * G, BLOCK), 65535u * 4);
-        const uint32_t* W = g.W();
-        switch (G) {
-            case 1: k_hn_heavy<1><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            case 2: k_hn_heavy<2><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            case 4: k_hn_heavy<4><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            case 8: k_hn_heavy<8><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            case 16: k_hn_heavy<16><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            case 32: k_hn_heavy<32><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
-            default: k_hn_heavy<64><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), W, hn.get(), iso.get()); break;
+        DBuf<unsigned long long> best(&c.pool, n);
+        MI_HIP(hipMemsetAsync(best.get(), 0, (size_t)n * 8, c.stream));
+        if (g.m) {
+            k_hn_heavy<<<(unsigned)ceil_div(g.m, BLOCK), BLOCK, 0, c.stream>>>(g.m, g.R(), roc, g.W(), best.get());
+            MI_HIP(hipGetLastError());
         }
+        k_hn_from_best<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), best.get(), hn.get(), iso.get());
         MI_HIP(hipGetLastError());
         scan_exclusive_u32(c, iso.get(), rank.get(), n, nullptr);
         k_hn_iso<<<grid1(n), BLOCK, 0, c.stream>>>(n, iso.get(), rank.get(), draws.get(), hn.get());

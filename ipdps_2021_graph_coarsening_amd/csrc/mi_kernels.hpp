@@ -38,6 +38,44 @@ struct HecTimes {
     double permute_ms = 0, heavy_ms = 0, map_ms = 0, build_ms = 0;
 };
 
+// ---------------------------------------------------------------- row index
+// roc[k] = row holding entry 64 k (k < ceil(m / 64)), roc[ceil(m/64)] = n - 1.
+// Lets a lane find the row of any entry with a search over a short range.
+void build_row_index(Ctx& c, const DGraph& g, DBuf<uint32_t>& roc);
+
+__device__ __forceinline__ uint32_t entry_row(uint32_t e, const uint32_t* __restrict__ roc,
+                                              const uint32_t* __restrict__ rowptr)
+{
+    uint32_t lo = roc[e >> 6], hi = roc[(e >> 6) + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (rowptr[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Wavefront segmented reduction: lanes carry (key, v) with keys
+// non-decreasing by lane (INF for idle lanes).  Returns in every lane the
+// reduction of v over itself and the following lanes with the same key, so
+// the first lane of each run holds the run's total; *head marks those lanes.
+template <class T, class Op>
+__device__ __forceinline__ T wave_seg_reduce(uint32_t key, T v, Op op, bool* head)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const T ov = __shfl_down(v, o, WAVE);
+        const uint32_t ok = __shfl_down(key, o, WAVE);
+        if (lane + o < WAVE && ok == key) v = op(v, ov);
+    }
+    const uint32_t pk = __shfl_up(key, 1, WAVE);
+    *head = (lane == 0) || pk != key;
+    return v;
+}
+
 // ---------------------------------------------------------------- primitives
 void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n,
                         uint32_t* total_dev);
@@ -54,21 +92,26 @@ void xs_draws(Ctx& c, uint64_t state0, uint64_t first, uint64_t count, uint64_t*
 
 // Deterministic HEC (reference Serial semantics). Writes vcmap (n) and
 // returns nc.  level >= 1 as in heuristics.h:930 (level 1 = random neighbour).
-uint32_t hec_map(Ctx& c, const DGraph& g, int level, uint64_t seed, uint32_t* vcmap,
+// roc is the row index of g (build_row_index).
+uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed, uint32_t* vcmap,
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t);
 
 // Coarse graph P^T A P without diagonal, rows sorted, duplicates merged.
-void build_coarse(Ctx& c, const DGraph& g, const uint32_t* vcmap, uint32_t nc,
+void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
 
 // ---------------------------------------------------------------- eigen
-// Load-balanced work list of one level for the fused power-iteration step:
-// item i is a tile of short rows [it_a, it_b) or, with the top bit of it_b
-// set, chunk (it_b & ~top) of the long row it_a.
+// Load-balanced work lists of one level for the power-iteration step.
+// Tile list: tile i is the run of short rows [t_a[i], t_b[i]).  Row list:
+// item i is row r_a[i] whole (r_b[i] == 0) or chunk (r_b[i] & 0x7fffffff) of
+// the chunked row r_a[i] (bit 31 of r_b), whose index among the chunked rows
+// is r_c[i].  The rest is the per-launch reduction workspace of the plan.
 struct LevelPlan {
-    DBuf<uint32_t> it_a, it_b, long_rows, lcnt;
-    DBuf<double> lpart;
-    uint32_t n_items = 0, n_long = 0;
+    DBuf<uint32_t> t_a, t_b, r_a, r_b, r_c, long_rows, lcnt, gcnt, fcnt;
+    DBuf<double> lpart, lS, partials, gpart;
+    uint32_t n_tiles = 0, n_ritems = 0, n_long = 0;
+    uint32_t tiles_per_block = 1;  // consecutive tiles one workgroup processes
+    uint32_t ritems_per_wave = 1;  // consecutive row items one wavefront processes
 };
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
 // Mean time (ms) of `reps` launches of the fused step on g; *alg_bytes gets
@@ -88,9 +131,10 @@ void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, d
 void normalize(Ctx& c, double* d_u, uint32_t n);
 // y = A x + (gb - wdeg) x : the shifted Laplacian product of one iteration
 // (asynchronous; gb_dev = device u32 from weighted_degree)
-void spmv_shift(Ctx& c, const DGraph& g, const uint32_t* wdeg, const uint32_t* gb_dev, const double* x,
-                double* y);
-void weighted_degree(Ctx& c, const DGraph& g, uint32_t* wdeg, uint32_t* gb_dev);
+void spmv_shift(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wdeg, const uint32_t* gb_dev,
+                const double* x, double* y);
+// wdeg = row sums of the weights, *gb_dev = max(2 wdeg, 2)
+void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg, uint32_t* gb_dev);
 
 // ---------------------------------------------------------------- partition
 // Sort-and-split of level-0 graph g by device eigenvector ev into device

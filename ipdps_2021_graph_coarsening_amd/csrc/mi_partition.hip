@@ -36,22 +36,17 @@ __global__ void k_assign(uint32_t n, const uint32_t* __restrict__ order, uint32_
     if (k < n) part[order[k]] = k < imbl ? 0u : 1u;
 }
 
-template <int G>
-__global__ void __launch_bounds__(BLOCK) k_cut(uint32_t n, const uint32_t* __restrict__ rowptr,
-                                               const uint32_t* __restrict__ col,
+// one lane per directed edge: counts edges whose ends lie in different parts
+__global__ void __launch_bounds__(BLOCK) k_cut(uint64_t m, const uint32_t* __restrict__ rowptr,
+                                               const uint32_t* __restrict__ roc, const uint32_t* __restrict__ col,
                                                const uint32_t* __restrict__ part,
                                                unsigned long long* __restrict__ cut)
 {
-    constexpr int RPB = BLOCK / G;
-    const int sub = threadIdx.x % G;
-    uint32_t local = 0;
-    for (uint64_t row = (uint64_t)blockIdx.x * RPB + threadIdx.x / G; row < n;
-         row += (uint64_t)gridDim.x * RPB) {
-        const uint32_t pr = part[row];
-        for (uint32_t k = rowptr[row] + sub; k < rowptr[row + 1]; k += G) local += part[col[k]] != pr;
-    }
-    local = wave_sum_u32(local);
-    if ((threadIdx.x & (WAVE - 1)) == 0 && local) atomicAdd(cut, (unsigned long long)local);
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t x = 0;
+    if (e < m) x = part[col[e]] != part[entry_row((uint32_t)e, roc, rowptr)] ? 1u : 0u;
+    x = wave_sum_u32(x);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && x) atomicAdd(cut, (unsigned long long)x);
 }
 
 // Sequential walk over sorted positions [imbl, imbr), sgpar.h:2041-2065.
@@ -109,9 +104,12 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     MI_HIP(hipGetLastError());
     DBuf<unsigned long long> cut(&c.pool, 2);
     MI_HIP(hipMemsetAsync(cut.get(), 0, 16, c.stream));
-    const unsigned grid = (unsigned)std::min<uint64_t>(ceil_div((uint64_t)n * 8, BLOCK), 65535ull * 4);
-    k_cut<8><<<grid, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), part, cut.get());
-    MI_HIP(hipGetLastError());
+    if (g.m) {
+        DBuf<uint32_t> roc;
+        build_row_index(c, g, roc);
+        k_cut<<<grid1(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), part, cut.get());
+        MI_HIP(hipGetLastError());
+    }
     k_walk<<<1, BLOCK, 0, c.stream>>>(imbl, imbr, order.get(), g.R(), g.Cc(), part, cut.get(), cut.get() + 1);
     MI_HIP(hipGetLastError());
     unsigned long long best = 0;

@@ -182,6 +182,18 @@ __global__ void __launch_bounds__(BLOCK) k_radix_scatter(const uint64_t* __restr
     }
 }
 
+// roc[k] = r for every multiple 64 k of an entry index inside row r
+__global__ void k_row_index(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ roc,
+                            uint32_t nchunks)
+{
+    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r < n) {
+        const uint32_t b = rowptr[r], e = rowptr[r + 1];
+        for (uint32_t k = (b + 63u) >> 6; (uint64_t)k << 6 < e; k++) roc[k] = r;
+    }
+    if (r == 0) roc[nchunks] = n - 1u;
+}
+
 __global__ void k_iota(uint32_t* out, uint64_t n, uint32_t start)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,6 +220,14 @@ void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, u
     MI_HIP(hipGetLastError());
     scan_exclusive_u32(c, sums.get(), sums.get(), tiles, nullptr);
     k_scan_down<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, out, n, sums.get(), total_dev);
+    MI_HIP(hipGetLastError());
+}
+
+void build_row_index(Ctx& c, const DGraph& g, DBuf<uint32_t>& roc)
+{
+    const uint32_t nchunks = (uint32_t)ceil_div(g.m, 64);
+    roc = DBuf<uint32_t>(&c.pool, (size_t)nchunks + 1);
+    k_row_index<<<(unsigned)ceil_div(g.n, BLOCK), BLOCK, 0, c.stream>>>(g.n, g.R(), roc.get(), nchunks);
     MI_HIP(hipGetLastError());
 }
 
