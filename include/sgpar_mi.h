@@ -73,12 +73,6 @@ void mi_ctx_destroy(mi_ctx* ctx);
 const char* mi_last_error(const mi_ctx* ctx);
 /* Returns cached pool blocks to the runtime. */
 int mi_ctx_trim(mi_ctx* ctx);
-/* Tuning of the power-iteration step: consecutive tiles of short rows per
-   workgroup and consecutive long-row items per wavefront (0 = automatic:
-   about 8 workgroups and 64 wavefronts per CU).  Applies to levels planned
-   after the call.  No counterpart in the reference (it has no GPU work
-   decomposition of its own). */
-int mi_ctx_set_tuning(mi_ctx* ctx, uint32_t tiles_per_block, uint32_t items_per_wave);
 
 /* ---------------------------------------------------------------- graphs */
 /*
@@ -179,10 +173,17 @@ int mi_build_coarse(mi_ctx* ctx, const mi_graph* g, const uint32_t* vcmap, uint3
 /* y = A x + (gb - wdeg) .* x with gb = max(2 wdeg, 2) (eigensolve_kokkos.h:161-179);
    host x, y (n). */
 int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y);
-/* Times `reps` launches of the FP64 shifted SpMV on the finest level of g on
-   the context stream with HIP events; returns the mean launch time and the
-   algorithmic bytes of one launch. */
-int mi_bench_spmv(mi_ctx* ctx, const mi_graph* g, int reps, double* avg_ms, double* alg_bytes);
+/* Measurement of one power-iteration step (the FP64 shifted product with its
+   reductions, eigensolve_kokkos.h:161-189) on g, timed with HIP events on the
+   context stream over `reps` steps, and of its streaming kernel alone over
+   `reps` launches.  Algorithmic bytes count col (+ w), the vectors x and y,
+   wdeg (and rowptr for the step); impl_bytes adds the per-entry row ids the
+   streaming kernel reads.  No counterpart in the reference. */
+typedef struct {
+    double step_ms, stream_ms;
+    double step_bytes, stream_bytes, impl_bytes;
+} mi_step_bench;
+int mi_bench_step(mi_ctx* ctx, const mi_graph* g, int reps, mi_step_bench* out);
 /* Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), host arrays. */
 int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                         int end_bit);

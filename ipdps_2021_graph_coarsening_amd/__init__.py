@@ -40,6 +40,12 @@ class Config(C.Structure):
                 ("num_iter", C.c_int), ("tol", C.c_double), ("num_partitions", C.c_int)]
 
 
+class StepBench(C.Structure):
+    """mi_step_bench"""
+    _fields_ = [("step_ms", C.c_double), ("stream_ms", C.c_double), ("step_bytes", C.c_double),
+                ("stream_bytes", C.c_double), ("impl_bytes", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("nlevels", C.c_int),
                 ("level_n", C.c_uint32 * MAX_LEVELS),
@@ -73,7 +79,6 @@ _SIGS = [
     ("mi_ctx_destroy", None, [vp]),
     ("mi_last_error", C.c_char_p, [vp]),
     ("mi_ctx_trim", C.c_int, [vp]),
-    ("mi_ctx_set_tuning", C.c_int, [vp, C.c_uint32, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -98,7 +103,7 @@ _SIGS = [
     ("mi_hec_level", C.c_int, [vp, vp, C.c_int, C.c_uint64, u32p, vp, vp, C.POINTER(C.c_uint32)]),
     ("mi_build_coarse", C.c_int, [vp, vp, u32p, C.c_uint32, C.POINTER(vp), vp]),
     ("mi_spmv_shift", C.c_int, [vp, vp, f64p, f64p]),
-    ("mi_bench_spmv", C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("mi_bench_step", C.c_int, [vp, vp, C.c_int, C.c_void_p]),
     ("mi_radix_sort_pairs", C.c_int, [vp, u64p, u32p, C.c_uint64, C.c_int, C.c_int]),
     ("mi_scan_u32", C.c_int, [vp, u32p, u32p, C.c_uint64, C.POINTER(C.c_uint32)]),
 ]
@@ -162,11 +167,6 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
-
-    def set_tuning(self, tiles_per_block: int = 0, items_per_wave: int = 0) -> None:
-        """Power-step work per workgroup / wavefront (0 = automatic)."""
-        self.check(self._L.mi_ctx_set_tuning(self.h, tiles_per_block, items_per_wave),
-                   "mi_ctx_set_tuning")
 
     # ------------------------------------------------------------ graphs
     def graph(self, rowptr, col, w=None) -> "Graph":
@@ -256,11 +256,17 @@ class Graph:
         c.check(c._L.mi_spmv_shift(c.h, self.h, x, y), "mi_spmv_shift")
         return y
 
-    def bench_spmv(self, reps: int = 20):
+    def bench_step(self, reps: int = 20) -> dict:
+        """Times the power-iteration step on this graph (mi_bench_step)."""
         c = self.ctx
-        ms, by = C.c_double(), C.c_double()
-        c.check(c._L.mi_bench_spmv(c.h, self.h, reps, C.byref(ms), C.byref(by)), "mi_bench_spmv")
-        return ms.value, by.value
+        b = StepBench()
+        c.check(c._L.mi_bench_step(c.h, self.h, reps, C.byref(b)), "mi_bench_step")
+        return {f: getattr(b, f) for f, _ in StepBench._fields_}
+
+    def bench_spmv(self, reps: int = 20):
+        """(ms, algorithmic bytes) of one whole power-iteration step."""
+        b = self.bench_step(reps)
+        return b["step_ms"], b["step_bytes"]
 
     def power_iter(self, u, normLap: int = 0, tol: float = 1e-10):
         c = self.ctx

@@ -333,14 +333,6 @@ int mi_ctx_trim(mi_ctx* ctx)
     return guard(ctx, [&] { ctx->c.pool.trim(); });
 }
 
-int mi_ctx_set_tuning(mi_ctx* ctx, uint32_t tiles_per_block, uint32_t items_per_wave)
-{
-    if (!ctx) return ST_ARG;
-    ctx->c.tune_tiles_per_block = tiles_per_block;
-    ctx->c.tune_items_per_wave = items_per_wave;
-    return ST_OK;
-}
-
 int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint32_t* w,
                     mi_graph** out)
 {
@@ -683,10 +675,18 @@ int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y)
     });
 }
 
-int mi_bench_spmv(mi_ctx* ctx, const mi_graph* g, int reps, double* avg_ms, double* alg_bytes)
+int mi_bench_step(mi_ctx* ctx, const mi_graph* g, int reps, mi_step_bench* out)
 {
-    if (!ctx || !g || reps < 1 || !avg_ms || !alg_bytes) return ST_ARG;
-    return guard(ctx, [&] { *avg_ms = bench_power_step(ctx->c, g->g, reps, alg_bytes); });
+    if (!ctx || !g || reps < 1 || !out) return ST_ARG;
+    return guard(ctx, [&] {
+        StepBench b;
+        bench_power_step(ctx->c, g->g, reps, &b);
+        out->step_ms = b.step_ms;
+        out->stream_ms = b.stream_ms;
+        out->step_bytes = b.step_bytes;
+        out->stream_bytes = b.stream_bytes;
+        out->impl_bytes = b.impl_bytes;
+    });
 }
 
 int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit)

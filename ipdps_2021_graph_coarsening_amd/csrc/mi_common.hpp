@@ -126,10 +126,6 @@ enum Scalar : int {
     SC_S = 8,       // input scale of the next iteration
     SC_AP = 9,      // shift used by the last iteration
     SC_SP = 10,     // scale used by the last iteration
-    SC_T1 = 11,     // sums S1..S4 of the tile kernel of the current iteration
-    SC_T2 = 12,
-    SC_T3 = 13,
-    SC_T4 = 14,
     SC_COUNT = 16
 };
 
@@ -156,8 +152,6 @@ struct Ctx {
     unsigned long long* niter = nullptr;  // iteration counter of the power iteration
     uint32_t* host_pinned = nullptr;      // small pinned staging area (64 words)
     uint64_t* xs_jump = nullptr;          // T^(2^k) of the xorshift transition, 48 x 64 words
-    uint32_t tune_items_per_wave = 0;     // power-step row items per wavefront (0 = automatic)
-    uint32_t tune_tiles_per_block = 0;    // power-step tiles per workgroup (0 = automatic)
     ~Ctx();
 };
 

@@ -17,22 +17,20 @@
 // norm formula has no cancellation.)  Each iteration streams the CSR once and
 // reads/writes one vector.
 //
-// Work decomposition, built once per level (plan_level):
-//  * rows of at most 64 entries are grouped into tiles: runs of rows whose
-//    merge-path coordinate rowptr[r] + r falls in one 1024-wide bin, so a tile
-//    holds at most 1024 rows + nonzeros plus one row of <= 64 entries.
-//    k_power_tiles gives each tile to a workgroup, which stages its products
-//    in LDS with coalesced loads and sums the rows with a balanced blocked
-//    segmented reduction;
-//  * longer rows go to k_power_rows: one wavefront per row of 65..2048
-//    entries, and 2048-entry chunks of longer rows whose last-arriving chunk
-//    adds the chunk partials in chunk order.  No LDS, so occupancy is set by
-//    registers alone and one wavefront's memory latency is covered by others.
-// The tile kernel reduces its sums to four scalars; the row kernel adds its
-// own and those of the chunked rows and does the convergence bookkeeping.
-// Workgroups are numbered so that each XCD (workgroups are dealt round-robin
-// over 8 XCDs) gets one contiguous range of items.  All reductions use fixed
-// orders, so results are bit-identical run to run.
+// Work decomposition: flat over the nonzeros, independent of row lengths.
+// Wavefront w owns entries [w PW, (w+1) PW); each lane takes 8 consecutive
+// entries per pass (16-byte loads of col, w and a per-entry row id), gathers
+// x, and sums its entries by row in registers.  A segmented scan over the 64
+// lanes joins rows that cross lanes; a carry joins rows that cross passes.
+// Rows that begin and end inside the wavefront's range are finished at once
+// (shift, store y, the four sums).  A row entering the range from the
+// previous wavefront leaves its partial in head[w]; a row leaving it leaves
+// its partial in tail[w].  The second kernel adds, for every row that starts
+// in wavefront w and leaves it, tail[w] + head[w+1] + ... in wavefront order,
+// finishes those rows and the empty ones, reduces the four sums of all
+// workgroups and does the convergence bookkeeping.  Every sum has a fixed
+// order, so results are bit-identical run to run; the streaming kernel uses
+// no atomics.
 #include "mi_kernels.hpp"
 
 #include <chrono>
@@ -43,19 +41,17 @@ namespace mi {
 
 namespace {
 
-constexpr uint32_t TILE_NB = 1024;  // merge-path width of a tile bin
-constexpr uint32_t LONG_DEG = 64;   // longer rows go to the row kernel
-constexpr uint32_t CHUNK = 2048;    // rows longer than this are chunked
-constexpr uint32_t MULTIF = 0x80000000u;  // row item is a chunk of a multi-chunk row
-constexpr int LDS_CAP = TILE_NB + LONG_DEG;
+constexpr int SEG = 8;                              // consecutive entries per lane and pass
+constexpr uint32_t PASS = WAVE * SEG;               // entries per wavefront pass
+constexpr uint32_t RINF = 0xFFFFFFFFu;              // row id of the lanes past the last entry
 constexpr int NXCD = 8;
-constexpr int NW = BLOCK / WAVE;  // wavefronts per workgroup
-constexpr int UNR = 8;
+constexpr int NW = BLOCK / WAVE;                    // wavefronts per workgroup
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
 // XCD-aware renumbering of workgroups: workgroup b runs on XCD b % 8; the
-// logical indices of one XCD are consecutive.
+// logical indices of one XCD are consecutive, so each XCD streams one
+// contiguous part of the matrix.
 __device__ __forceinline__ uint32_t logical_block()
 {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
@@ -64,116 +60,52 @@ __device__ __forceinline__ uint32_t logical_block()
 }
 
 // ------------------------------------------------------------------ plan
-// kind: 0 = short row (tiles), 1 = row item, 2 = chunked row item
-__global__ void k_tile_flags(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ start,
-                             uint32_t* __restrict__ kind)
+// rid[e] = row holding entry e
+__global__ void k_entry_rows(uint32_t m, const uint32_t* __restrict__ roc, const uint32_t* __restrict__ rowptr,
+                             uint32_t* __restrict__ rid)
 {
-    uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t b = rowptr[r], e = rowptr[r + 1];
-    const uint32_t d = e - b;
-    kind[r] = d > CHUNK ? 2u : (d > LONG_DEG ? 1u : 0u);
-    bool st = (r == 0) || d > LONG_DEG;
-    if (!st) {
-        const uint32_t pb = rowptr[r - 1];
-        const uint64_t bin = ((uint64_t)b + r) / TILE_NB, pbin = ((uint64_t)pb + r - 1) / TILE_NB;
-        st = (b - pb) > LONG_DEG || bin != pbin;
-    }
-    start[r] = st ? 1u : 0u;
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e < m) rid[e] = entry_row((uint32_t)e, roc, rowptr);
 }
 
-__global__ void k_tile_starts(uint32_t n, const uint32_t* __restrict__ start, const uint32_t* __restrict__ tid,
-                              uint32_t* __restrict__ starts)
+__global__ void k_empty_flag(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ flag)
 {
-    uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < n && start[r]) starts[tid[r]] = r;
+    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r < n) flag[r] = rowptr[r + 1] == rowptr[r] ? 1u : 0u;
 }
 
-// per group (tile or long row): tile-list and row-list item counts
-__global__ void k_group_count(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ kind,
-                              const uint32_t* __restrict__ starts, const uint32_t* __restrict__ ngroups_p,
-                              uint32_t* __restrict__ tcnt, uint32_t* __restrict__ rcnt)
+__global__ void k_compact_empty(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                uint32_t* __restrict__ out)
 {
-    uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t > n) return;
-    const uint32_t ng = *ngroups_p;
-    if (t >= ng) {
-        tcnt[t] = 0;
-        rcnt[t] = 0;
-        return;
-    }
-    const uint32_t r0 = starts[t];
-    const uint32_t kd = kind[r0];
-    tcnt[t] = kd == 0u ? 1u : 0u;
-    rcnt[t] = kd == 0u ? 0u : (kd == 1u ? 1u : (uint32_t)ceil_div(rowptr[r0 + 1] - rowptr[r0], CHUNK));
+    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r < n && flag[r]) out[pos[r]] = r;
 }
 
-__global__ void k_group_write(uint32_t n, const uint32_t* __restrict__ kind, const uint32_t* __restrict__ starts,
-                              const uint32_t* __restrict__ ngroups_p, const uint32_t* __restrict__ rcnt,
-                              const uint32_t* __restrict__ toff, const uint32_t* __restrict__ roff,
-                              const uint32_t* __restrict__ mid, uint32_t* __restrict__ t_a,
-                              uint32_t* __restrict__ t_b, uint32_t* __restrict__ r_a, uint32_t* __restrict__ r_b,
-                              uint32_t* __restrict__ r_c)
+__global__ void k_copy_u32(uint64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out)
 {
-    uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-    const uint32_t ng = *ngroups_p;
-    if (t >= ng) return;
-    const uint32_t r0 = starts[t];
-    const uint32_t kd = kind[r0];
-    if (kd == 0u) {
-        t_a[toff[t]] = r0;
-        t_b[toff[t]] = (t + 1 < ng) ? starts[t + 1] : n;
-    } else if (kd == 1u) {
-        r_a[roff[t]] = r0;
-        r_b[roff[t]] = 0u;
-        r_c[roff[t]] = 0u;
-    } else {
-        for (uint32_t k = 0; k < rcnt[t]; k++) {
-            r_a[roff[t] + k] = r0;
-            r_b[roff[t] + k] = MULTIF | k;
-            r_c[roff[t] + k] = mid[r0];
-        }
-    }
-}
-
-__global__ void k_multi_flag(uint32_t n, const uint32_t* __restrict__ kind, uint32_t* __restrict__ flag)
-{
-    uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < n) flag[r] = kind[r] == 2u ? 1u : 0u;
-}
-
-__global__ void k_compact_multi(uint32_t n, const uint32_t* __restrict__ kind, const uint32_t* __restrict__ mid,
-                                uint32_t* __restrict__ long_rows)
-{
-    uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < n && kind[r] == 2u) long_rows[mid[r]] = r;
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = in[i];
 }
 
 // ------------------------------------------------------------------ step
 struct StepArgs {
     uint32_t n;
+    uint32_t m;
     const uint32_t* rowptr;
-    const uint32_t* col;
-    const uint32_t* w;
+    const uint32_t* col;   // 16-byte aligned
+    const uint32_t* w;     // 16-byte aligned (unused for unit weights)
+    const uint32_t* rid;   // row of every entry
     const uint32_t* wdeg;
     const uint32_t* gbp;
-    // tiles of short rows
-    const uint32_t* t_a;
-    const uint32_t* t_b;
-    uint32_t n_tiles;
-    uint32_t tiles_per_block;
-    // rows and row chunks
-    const uint32_t* r_a;
-    const uint32_t* r_b;
-    const uint32_t* r_c;
-    uint32_t n_ritems;
-    uint32_t ritems_per_wave;
-    uint32_t n_long;  // chunked rows
-    double* lpart;    // chunk partial per row item
-    uint32_t* lcnt;   // chunk arrival counter per chunked row
-    double* lS;       // S1..S4 terms of each chunked row
-    // reduction workspace (shared by the two kernels, which run in sequence)
-    double* partials;
+    uint32_t pw;           // entries per wavefront, a multiple of PASS
+    uint32_t nwaves;
+    const uint32_t* empty_rows;
+    uint32_t n_empty;
+    double* head;          // per wavefront: partial of the row entering its range
+    double* tail;          // per wavefront: partial of the row leaving its range
+    double* bpart;         // per workgroup of the streaming kernel: S1..S4
+    uint32_t nblk_a;
+    double* partials;      // reduction workspace of the second kernel
     double* gpart;
     uint32_t* gcnt;
     uint32_t* fcnt;
@@ -188,252 +120,201 @@ struct StepArgs {
     unsigned long long* niter;
 };
 
-// Products w_e * x_col(e) for entries base + u*STRIDE (u < UNR) below e1,
-// with x_j = (y_j - a) * s.  All index and weight loads are issued before the
-// gathers, so each lane keeps UNR requests in flight; the streamed CSR is
-// read non-temporally so that L2 keeps the gathered vector.
-template <bool UNIT, int STRIDE>
-__device__ __forceinline__ void gather_products(const StepArgs& A, uint32_t base, uint32_t e1, double a,
-                                                double s, double (&v)[UNR])
+__device__ __forceinline__ void load8(const uint32_t* __restrict__ p, uint64_t e, uint32_t (&o)[SEG])
 {
-    uint32_t ci[UNR];
-    uint32_t wv[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; u++) {
-        const uint32_t e = base + u * STRIDE;
-        const bool in = e < e1;
-        ci[u] = in ? __builtin_nontemporal_load(&A.col[e]) : 0u;
-        wv[u] = (UNIT || !in) ? 1u : __builtin_nontemporal_load(&A.w[e]);
-    }
-#pragma unroll
-    for (int u = 0; u < UNR; u++) {
-        const bool in = base + u * STRIDE < e1;
-        const double xv = in ? (A.yprev[ci[u]] - a) * s : 0.0;
-        v[u] = UNIT ? xv : (double)wv[u] * xv;
-    }
+    const uint4 v0 = *reinterpret_cast<const uint4*>(p + e);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(p + e + 4);
+    o[0] = v0.x; o[1] = v0.y; o[2] = v0.z; o[3] = v0.w;
+    o[4] = v1.x; o[5] = v1.y; o[6] = v1.z; o[7] = v1.w;
 }
 
+// y_r from the row sum (the shifted product of the reference, or A x in mode 2)
 template <bool NORMLAP>
-__device__ __forceinline__ double finish_row(const StepArgs& A, uint32_t r, double spmv, double xr, uint32_t gb)
+__device__ __forceinline__ double shifted(const StepArgs& A, uint32_t r, double spmv, double xr, uint32_t wd,
+                                          uint32_t gb)
 {
-    double y;
-    if (A.mode == 2) {
-        y = spmv;
-    } else if (!NORMLAP) {
-        y = (double)(uint32_t)(gb - A.wdeg[r]) * xr + spmv;
-    } else {
-        const double winv = 1.0 / (double)A.wdeg[r];
-        y = 0.5 * xr + 0.5 * spmv * winv;
+    if (A.mode == 2) return spmv;
+    if (!NORMLAP) return (double)(uint32_t)(gb - wd) * xr + spmv;
+    return 0.5 * xr + 0.5 * spmv * (1.0 / (double)wd);
+}
+
+// Finishes row r with row sum v: stores y_r and adds the four sums.  The row
+// that entered the wavefront's range from the previous one (hrow) is not
+// complete here: its partial goes to head[wv] instead.
+template <bool NORMLAP>
+__device__ __forceinline__ void finish(const StepArgs& A, uint32_t r, double v, uint32_t hrow, uint32_t wv,
+                                       double a, double s, uint32_t gb, double (&S)[4])
+{
+    if (r == RINF) return;
+    if (r == hrow) {
+        A.head[wv] = v;
+        return;
     }
+    const double xr = (A.yprev[r] - a) * s;
+    const double y = shifted<NORMLAP>(A, r, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
     A.ynew[r] = y;
-    return y;
+    S[0] += y;
+    S[1] += y * y;
+    S[2] += xr * y;
+    S[3] += xr;
 }
 
-// ---- tiles of short rows: one workgroup per tile
 template <bool UNIT, bool NORMLAP>
-__global__ void __launch_bounds__(BLOCK) k_power_tiles(StepArgs A)
+__global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
 {
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
-    __shared__ double prod[LDS_CAP];
-    __shared__ double rowsum[TILE_NB];
-    __shared__ uint32_t rps[TILE_NB + 1];
-    __shared__ double tail_val[BLOCK];
-    __shared__ uint32_t tail_row[BLOCK];
-    const double a = A.mode ? 0.0 : A.scal[SC_A];
-    const double s = A.mode ? 1.0 : A.scal[SC_S];
-    const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
-    const uint32_t tid = threadIdx.x;
-    double S1 = 0, S2 = 0, S3 = 0, S4 = 0;
-
-    const uint64_t i0 = (uint64_t)logical_block() * A.tiles_per_block;
-    const uint64_t i1 = (i0 + A.tiles_per_block) < A.n_tiles ? i0 + A.tiles_per_block : A.n_tiles;
-    for (uint64_t item = i0; item < i1; item++) {
-        const uint32_t r0 = A.t_a[item], r1 = A.t_b[item];
-        const uint32_t nr = r1 - r0;
-        const uint32_t e0 = A.rowptr[r0], e1 = A.rowptr[r1];
-        if (e1 - e0 > (uint32_t)LDS_CAP || nr > TILE_NB) {  // excluded by the tile rule
-            if (tid == 0) A.cnt[CT_ERR] = 1u;
-            continue;
-        }
-        for (uint32_t k = tid; k <= nr; k += BLOCK) rps[k] = A.rowptr[r0 + k] - e0;
-        for (uint32_t base = e0 + tid; base < e1; base += BLOCK * UNR) {
-            double v[UNR];
-            gather_products<UNIT, BLOCK>(A, base, e1, a, s, v);
-#pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const uint32_t e = base + u * BLOCK;
-                if (e < e1) prod[e - e0] = v[u];
-            }
-        }
-        __syncthreads();
-        // Blocked segmented sum: thread t owns entries [t K, t K + K) (K odd,
-        // so the strided LDS reads spread over the banks).  A row that ends
-        // inside a thread's range gets its sum there, adding the partial sums
-        // ("tails") earlier threads left for it, earliest first.  Sums go to
-        // rowsum[]; the rows are finished afterwards with coalesced accesses.
-        const uint32_t E = e1 - e0;
-        uint32_t K = (E + BLOCK - 1) / BLOCK;
-        K |= 1u;
-        const uint32_t p0 = tid * K, p1 = (p0 + K < E) ? p0 + K : E;
-        uint32_t hrow = INF;  // local row closed here that began in an earlier thread
-        double hv = 0;
-        tail_row[tid] = INF;
-        if (p0 < p1) {
-            // the non-empty local row holding entry p0: last j with rps[j] <= p0
-            uint32_t lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const uint32_t md = (lo + hi + 1) >> 1;
-                if (rps[md] <= p0)
-                    lo = md;
-                else
-                    hi = md - 1;
-            }
-            uint32_t j = lo;
-            uint32_t rs = rps[j], re = rps[j + 1];
-            double acc = 0;
-            for (uint32_t p = p0; p < p1; p++) {
-                acc += prod[p];
-                if (p + 1 == re) {
-                    if (rs < p0) {
-                        hrow = j;
-                        hv = acc;
-                    } else {
-                        rowsum[j] = acc;
-                    }
-                    acc = 0;
-                    if (p + 1 < p1) {
-                        do {
-                            j++;
-                        } while (rps[j + 1] == rps[j]);
-                        rs = rps[j];
-                        re = rps[j + 1];
-                    }
-                }
-            }
-            if (p1 < re) {
-                tail_row[tid] = j;
-                tail_val[tid] = acc;
-            }
-        }
-        __syncthreads();
-        if (hrow != INF) {
-            uint32_t t0 = tid;
-            while (t0 > 0 && tail_row[t0 - 1] == hrow) t0--;
-            double tot = 0;
-            for (uint32_t t = t0; t < tid; t++) tot += tail_val[t];
-            rowsum[hrow] = tot + hv;
-        }
-        __syncthreads();
-        for (uint32_t j = tid; j < nr; j += BLOCK) {
-            const uint32_t r = r0 + j;
-            const double v = (rps[j + 1] == rps[j]) ? 0.0 : rowsum[j];  // isolated rows sum to 0
-            const double xr = (A.yprev[r] - a) * s;
-            const double y = finish_row<NORMLAP>(A, r, v, xr, gb);
-            S1 += y;
-            S2 += y * y;
-            S3 += xr * y;
-            S4 += xr;
-        }
-        __syncthreads();
-    }
-
-    double mine[4] = {S1, S2, S3, S4}, out[4];
-    if (!grid_reduce2<4>(mine, blockIdx.x, gridDim.x, A.partials, A.gpart, A.gcnt, A.fcnt, out)) return;
-    if (tid == 0 && A.mode != 2) {
-        A.scal[SC_T1] = out[0];
-        A.scal[SC_T2] = out[1];
-        A.scal[SC_T3] = out[2];
-        A.scal[SC_T4] = out[3];
-    }
-}
-
-// Sum of w_e x_col(e) over entries [e0, e1) by one wavefront; valid in every lane.
-template <bool UNIT>
-__device__ __forceinline__ double wave_row_sum(const StepArgs& A, uint32_t e0, uint32_t e1, double a, double s)
-{
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    double p = 0;
-    for (uint32_t base = e0 + lane; base < e1; base += WAVE * UNR) {
-        double v[UNR];
-        gather_products<UNIT, WAVE>(A, base, e1, a, s, v);
-#pragma unroll
-        for (int u = 0; u < UNR; u++) p += v[u];
-    }
-    return wave_sum(p);
-}
-
-// ---- long rows: one wavefront per row or row chunk; final bookkeeping
-template <bool UNIT, bool NORMLAP>
-__global__ void __launch_bounds__(BLOCK) k_power_rows(StepArgs A, int with_tiles)
-{
-    if (A.mode == 0 && A.cnt[CT_DONE]) return;
-    __shared__ double lsum[4][NW];
+    __shared__ double red[4][NW];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const double a = A.mode ? 0.0 : A.scal[SC_A];
     const double s = A.mode ? 1.0 : A.scal[SC_S];
     const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
-    double S1 = 0, S2 = 0, S3 = 0, S4 = 0;
-
-    // each wavefront takes its own contiguous run of items
-    const uint64_t i0 = ((uint64_t)logical_block() * NW + wid) * A.ritems_per_wave;
-    const uint64_t i1 = (i0 + A.ritems_per_wave) < A.n_ritems ? i0 + A.ritems_per_wave : A.n_ritems;
-    for (uint64_t item = i0; item < i1; item++) {
-        const uint32_t r = A.r_a[item], ib = A.r_b[item];
-        const uint32_t rb = A.rowptr[r], re = A.rowptr[r + 1];
-        if (ib & MULTIF) {
-            const uint32_t chunk = ib & ~MULTIF, k = A.r_c[item];
-            const uint32_t e0 = rb + chunk * CHUNK;
-            const uint32_t e1 = (re - e0) > CHUNK ? e0 + CHUNK : re;
-            const double p = wave_row_sum<UNIT>(A, e0, e1, a, s);
-            if (lane == 0) {
-                const uint32_t nch = (uint32_t)ceil_div(re - rb, CHUNK);
-                st_dev_f64(&A.lpart[item], p);
-                if (arrive(&A.lcnt[k]) == nch - 1) {
-                    // the last chunk adds the chunk partials in chunk order
-                    const uint64_t first = item - chunk;
-                    double tot = 0;
-                    for (uint32_t j = 0; j < nch; j++) tot += ld_dev_f64(&A.lpart[first + j]);
-                    const double xr = (A.yprev[r] - a) * s;
-                    const double y = finish_row<NORMLAP>(A, r, tot, xr, gb);
-                    st_dev_f64(&A.lS[(size_t)k * 4 + 0], y);
-                    st_dev_f64(&A.lS[(size_t)k * 4 + 1], y * y);
-                    st_dev_f64(&A.lS[(size_t)k * 4 + 2], xr * y);
-                    st_dev_f64(&A.lS[(size_t)k * 4 + 3], xr);
-                    rearm(&A.lcnt[k]);
-                    wait_stores();  // thread 0 signals for these in the final reduction
+    double S[4] = {0, 0, 0, 0};
+    const uint32_t wv = logical_block() * NW + wid;
+    if (wv < A.nwaves) {
+        const uint64_t E0 = (uint64_t)wv * A.pw;
+        const uint64_t E1 = (E0 + A.pw < A.m) ? E0 + A.pw : A.m;
+        // the row at E0 entered from the previous wavefront if entry E0 - 1 is in it
+        const uint32_t r_a = A.rid[E0];
+        const uint32_t hrow = (E0 > 0 && A.rid[E0 - 1] == r_a) ? r_a : RINF;
+        uint32_t cr = hrow;  // row carried into the pass (RINF: none)
+        double cs = 0.0;     // its partial sum so far
+        for (uint64_t base = E0; base < E1; base += PASS) {
+            const uint64_t e0 = base + (uint64_t)lane * SEG;
+            uint32_t rr[SEG], cc[SEG], ww[SEG];
+            if (e0 + SEG <= E1) {
+                load8(A.rid, e0, rr);
+                load8(A.col, e0, cc);
+                if (!UNIT) load8(A.w, e0, ww);
+            } else {
+#pragma unroll
+                for (int u = 0; u < SEG; u++) {
+                    const bool in = e0 + u < E1;
+                    rr[u] = in ? A.rid[e0 + u] : RINF;
+                    cc[u] = in ? A.col[e0 + u] : 0u;
+                    if (!UNIT) ww[u] = in ? A.w[e0 + u] : 0u;
                 }
             }
-        } else {
-            const double p = wave_row_sum<UNIT>(A, rb, re, a, s);
-            if (lane == 0) {
-                const double xr = (A.yprev[r] - a) * s;
-                const double y = finish_row<NORMLAP>(A, r, p, xr, gb);
-                S1 += y;
-                S2 += y * y;
-                S3 += xr * y;
-                S4 += xr;
+            // row of the entry after lane 63's last one (next pass or next wavefront)
+            const uint32_t nx63 = (base + PASS < A.m) ? A.rid[base + PASS] : RINF;
+            double pv[SEG];
+#pragma unroll
+            for (int u = 0; u < SEG; u++) {
+                const double xv = rr[u] != RINF ? (A.yprev[cc[u]] - a) * s : 0.0;
+                pv[u] = UNIT ? xv : (double)ww[u] * xv;
             }
+            // lane-local segments
+            const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
+            uint32_t mid_r[SEG - 1];
+            double mid_s[SEG - 1];
+            double fs = 0.0, acc = pv[0];
+            uint32_t cur = rr[0];
+            bool multi = false;
+#pragma unroll
+            for (int u = 1; u < SEG; u++) {
+                mid_r[u - 1] = RINF;
+                mid_s[u - 1] = 0.0;
+                if (rr[u] != cur) {
+                    if (!multi) {
+                        fs = acc;  // first segment: may continue from earlier lanes
+                        multi = true;
+                    } else {
+                        mid_r[u - 1] = cur;  // begins and ends in this lane
+                        mid_s[u - 1] = acc;
+                    }
+                    cur = rr[u];
+                    acc = pv[u];
+                } else {
+                    acc += pv[u];
+                }
+            }
+            // segmented inclusive scan of the last segments over the lanes
+            // (row ids never decrease along the entries, so equal keys are
+            // contiguous lanes)
+            double sc = acc;
+            const uint32_t key = last_row;
+            if (lane == 0 && !multi && key == cr) sc = cs + sc;
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) {
+                const double ov = __shfl_up(sc, o, WAVE);
+                const uint32_t ok = __shfl_up(key, o, WAVE);
+                if (lane >= (uint32_t)o && ok == key) sc = ov + sc;
+            }
+            const double psc = __shfl_up(sc, 1, WAVE);
+            const uint32_t pkey = __shfl_up(key, 1, WAVE);
+            uint32_t nxt = __shfl_down(first_row, 1, WAVE);
+            if (lane == WAVE - 1) nxt = nx63;
+            // a lane's first segment closes inside the lane when it has several
+            uint32_t xr_row = RINF;
+            double xr_sum = 0.0;
+            if (multi) {
+                const bool crosses = lane == 0 ? (cr == first_row) : (pkey == first_row);
+                const double before = lane == 0 ? cs : psc;
+                xr_row = first_row;
+                xr_sum = crosses ? before + fs : fs;
+            }
+            // the last segment closes at the lane's end when the next entry is another row
+            const uint32_t lr_row = (nxt != last_row) ? last_row : RINF;
+            // carry into the next pass
+            const uint32_t k63 = __shfl(key, WAVE - 1, WAVE);
+            const double s63 = __shfl(sc, WAVE - 1, WAVE);
+            const bool open = (k63 != RINF) && (nx63 == k63);
+            finish<NORMLAP>(A, xr_row, xr_sum, hrow, wv, a, s, gb, S);
+#pragma unroll
+            for (int u = 0; u < SEG - 1; u++) finish<NORMLAP>(A, mid_r[u], mid_s[u], hrow, wv, a, s, gb, S);
+            finish<NORMLAP>(A, lr_row, sc, hrow, wv, a, s, gb, S);
+            cr = open ? k63 : RINF;
+            cs = open ? s63 : 0.0;
+        }
+        // a row still open continues past this wavefront's range
+        if (lane == 0 && cr != RINF) {
+            if (cr == hrow)
+                A.head[wv] = cs;
+            else
+                A.tail[wv] = cs;
         }
     }
-
-    double mine[4] = {S1, S2, S3, S4}, out[4];
-    if (!grid_reduce2<4>(mine, blockIdx.x, gridDim.x, A.partials, A.gpart, A.gcnt, A.fcnt, out)) return;
     if (A.mode == 2) return;
-    // terms of the chunked rows, in list order
-    double L[4] = {0, 0, 0, 0}, Lt[4];
-    for (uint32_t k = threadIdx.x; k < A.n_long; k += BLOCK) {
+    double tot[4];
+    block_sum_k<4>(S, tot, red);
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q4 = 0; q4 < 4; q4++) L[q4] += ld_dev_f64(&A.lS[(size_t)k * 4 + q4]);
+        for (int k = 0; k < 4; k++) A.bpart[(size_t)blockIdx.x * 4 + k] = tot[k];
     }
-    block_sum_k<4>(L, Lt, lsum);
+}
+
+// Rows crossing wavefront boundaries, empty rows, the global sums and the
+// convergence bookkeeping.
+template <bool NORMLAP>
+__global__ void __launch_bounds__(BLOCK) k_power_join(StepArgs A)
+{
+    if (A.mode == 0 && A.cnt[CT_DONE]) return;
+    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const double a = A.mode ? 0.0 : A.scal[SC_A];
+    const double s = A.mode ? 1.0 : A.scal[SC_S];
+    const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
+    double S[4] = {0, 0, 0, 0};
+    if (t < A.nwaves) {
+        const uint64_t E0 = t * A.pw;
+        const uint64_t E1 = (E0 + A.pw < A.m) ? E0 + A.pw : A.m;
+        const uint32_t r = A.rid[E1 - 1];
+        if (E1 < A.m && A.rid[E1] == r && A.rowptr[r] >= E0) {
+            // row r starts in wavefront t and ends in wavefront klast
+            const uint64_t klast = ((uint64_t)A.rowptr[r + 1] - 1) / A.pw;
+            double tot = A.tail[t];
+            for (uint64_t k = t + 1; k <= klast; k++) tot += A.head[k];
+            finish<NORMLAP>(A, r, tot, RINF, 0, a, s, gb, S);
+        }
+    }
+    if (t < A.n_empty) finish<NORMLAP>(A, A.empty_rows[t], 0.0, RINF, 0, a, s, gb, S);
+    if (A.mode == 2) return;
+    for (uint64_t j = t; j < A.nblk_a; j += (uint64_t)gridDim.x * BLOCK) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) S[k] += A.bpart[j * 4 + k];
+    }
+    double out[4];
+    if (!grid_reduce2<4>(S, blockIdx.x, gridDim.x, A.partials, A.gpart, A.gcnt, A.fcnt, out)) return;
     if (threadIdx.x != 0) return;
-    double T1 = out[0] + Lt[0], T2 = out[1] + Lt[1], T3 = out[2] + Lt[2], T4 = out[3] + Lt[3];
-    if (with_tiles) {
-        T1 += A.scal[SC_T1];
-        T2 += A.scal[SC_T2];
-        T3 += A.scal[SC_T3];
-        T4 += A.scal[SC_T4];
-    }
+    const double T1 = out[0], T2 = out[1], T3 = out[2], T4 = out[3];
     const double mult = NORMLAP ? 0.0 : T1 * A.c;  // dot(v, vec1)
     const double an = mult * A.c;                   // subtracted from every entry
     const double nrm2 = T2 - 2.0 * an * T1 + an * an * (double)A.n;
@@ -575,29 +456,27 @@ __global__ void k_loop_init(double* scal, uint32_t* cnt, unsigned long long* nit
     }
 }
 
+
 StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wdeg, const uint32_t* gb,
                    const double* yprev, double* ynew, double tol, int mode)
 {
     StepArgs A;
     A.n = g.n;
+    A.m = g.m;
     A.rowptr = g.R();
-    A.col = g.Cc();
-    A.w = g.W();
+    A.col = p.col_al.get() ? p.col_al.get() : g.Cc();
+    A.w = p.w_al.get() ? p.w_al.get() : g.W();
+    A.rid = p.rid.get();
     A.wdeg = wdeg;
     A.gbp = gb;
-    A.t_a = p.t_a.get();
-    A.t_b = p.t_b.get();
-    A.n_tiles = p.n_tiles;
-    A.tiles_per_block = p.tiles_per_block;
-    A.r_a = p.r_a.get();
-    A.r_b = p.r_b.get();
-    A.r_c = p.r_c.get();
-    A.n_ritems = p.n_ritems;
-    A.ritems_per_wave = p.ritems_per_wave;
-    A.n_long = p.n_long;
-    A.lpart = p.lpart.get();
-    A.lcnt = p.lcnt.get();
-    A.lS = p.lS.get();
+    A.pw = p.pw;
+    A.nwaves = p.nwaves;
+    A.empty_rows = p.empty_rows.get();
+    A.n_empty = p.n_empty;
+    A.head = p.head.get();
+    A.tail = p.tail.get();
+    A.bpart = p.bpart.get();
+    A.nblk_a = p.nblk_a;
     A.partials = p.partials.get();
     A.gpart = p.gpart.get();
     A.gcnt = p.gcnt.get();
@@ -614,103 +493,94 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     return A;
 }
 
-// One step = the tile kernel (if the level has short rows) then the row kernel.
-void launch_step(Ctx& c, const DGraph& g, int normLap, const StepArgs& A)
+void launch_stream(Ctx& c, const DGraph& g, int normLap, const StepArgs& A)
 {
-    const int with_tiles = A.n_tiles > 0 ? 1 : 0;
-    if (with_tiles) {
-        const unsigned grid = (unsigned)ceil_div(A.n_tiles, A.tiles_per_block);
-        if (g.unit) {
-            if (normLap)
-                k_power_tiles<true, true><<<grid, BLOCK, 0, c.stream>>>(A);
-            else
-                k_power_tiles<true, false><<<grid, BLOCK, 0, c.stream>>>(A);
-        } else {
-            if (normLap)
-                k_power_tiles<false, true><<<grid, BLOCK, 0, c.stream>>>(A);
-            else
-                k_power_tiles<false, false><<<grid, BLOCK, 0, c.stream>>>(A);
-        }
-        MI_HIP(hipGetLastError());
-    }
-    unsigned grid = (unsigned)ceil_div(A.n_ritems, (uint64_t)A.ritems_per_wave * NW);
-    if (grid == 0) grid = 1;  // the bookkeeping runs even without long rows
-    if (g.unit) {
+    if (A.nblk_a == 0) return;
+    const bool unit = A.w == nullptr;
+    if (unit) {
         if (normLap)
-            k_power_rows<true, true><<<grid, BLOCK, 0, c.stream>>>(A, with_tiles);
+            k_power_stream<true, true><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
         else
-            k_power_rows<true, false><<<grid, BLOCK, 0, c.stream>>>(A, with_tiles);
+            k_power_stream<true, false><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
     } else {
         if (normLap)
-            k_power_rows<false, true><<<grid, BLOCK, 0, c.stream>>>(A, with_tiles);
+            k_power_stream<false, true><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
         else
-            k_power_rows<false, false><<<grid, BLOCK, 0, c.stream>>>(A, with_tiles);
+            k_power_stream<false, false><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
     }
     MI_HIP(hipGetLastError());
+}
+
+void launch_join(Ctx& c, int normLap, const StepArgs& A, uint32_t nblk_b)
+{
+    if (normLap)
+        k_power_join<true><<<nblk_b, BLOCK, 0, c.stream>>>(A);
+    else
+        k_power_join<false><<<nblk_b, BLOCK, 0, c.stream>>>(A);
+    MI_HIP(hipGetLastError());
+}
+
+// One step = the streaming kernel then the joining kernel.
+void launch_step(Ctx& c, const DGraph& g, const LevelPlan& p, int normLap, const StepArgs& A)
+{
+    launch_stream(c, g, normLap, A);
+    launch_join(c, normLap, A, p.nblk_b);
 }
 
 }  // namespace
 
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
 {
-    const uint32_t n = g.n;
-    const uint64_t max_items = (uint64_t)n + g.m / CHUNK + 2;
-    MI_CHECK(max_items < (1ull << 31) && n < MULTIF, ST_ARG, "level too large for the tile plan");
-    DBuf<uint32_t> start(&c.pool, n), kind(&c.pool, n), gid(&c.pool, n), starts(&c.pool, (size_t)n + 1);
-    DBuf<uint32_t> tcnt(&c.pool, (size_t)n + 1), rcnt(&c.pool, (size_t)n + 1);
-    DBuf<uint32_t> toff(&c.pool, (size_t)n + 1), roff(&c.pool, (size_t)n + 1);
-    DBuf<uint32_t> mid(&c.pool, n), ismulti(&c.pool, n);
-    DBuf<uint32_t> tot(&c.pool, 4);
-    p.t_a = DBuf<uint32_t>(&c.pool, n);
-    p.t_b = DBuf<uint32_t>(&c.pool, n);
-    p.r_a = DBuf<uint32_t>(&c.pool, max_items);
-    p.r_b = DBuf<uint32_t>(&c.pool, max_items);
-    p.r_c = DBuf<uint32_t>(&c.pool, max_items);
-    k_tile_flags<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), start.get(), kind.get());
-    MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, start.get(), gid.get(), n, tot.get() + 0);
-    k_tile_starts<<<grid1(n), BLOCK, 0, c.stream>>>(n, start.get(), gid.get(), starts.get());
-    MI_HIP(hipGetLastError());
-    k_group_count<<<grid1((uint64_t)n + 1), BLOCK, 0, c.stream>>>(n, g.R(), kind.get(), starts.get(), tot.get(),
-                                                                  tcnt.get(), rcnt.get());
-    MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, tcnt.get(), toff.get(), (uint64_t)n + 1, tot.get() + 1);
-    scan_exclusive_u32(c, rcnt.get(), roff.get(), (uint64_t)n + 1, tot.get() + 2);
-    k_multi_flag<<<grid1(n), BLOCK, 0, c.stream>>>(n, kind.get(), ismulti.get());
-    MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, ismulti.get(), mid.get(), n, tot.get() + 3);
-    k_group_write<<<grid1(n), BLOCK, 0, c.stream>>>(n, kind.get(), starts.get(), tot.get(), rcnt.get(), toff.get(),
-                                                    roff.get(), mid.get(), p.t_a.get(), p.t_b.get(), p.r_a.get(),
-                                                    p.r_b.get(), p.r_c.get());
-    MI_HIP(hipGetLastError());
-    uint32_t t4[4];
-    read_words(c, tot.get(), t4, 16);
-    p.n_tiles = t4[1];
-    p.n_ritems = t4[2];
-    p.n_long = t4[3];
-    MI_CHECK(p.n_tiles <= n && p.n_ritems <= max_items && p.n_tiles + p.n_ritems >= 1, ST_INTERNAL,
-             "tile plan out of range");
-    p.long_rows = DBuf<uint32_t>(&c.pool, p.n_long ? p.n_long : 1);
-    p.lcnt = DBuf<uint32_t>(&c.pool, p.n_long ? p.n_long : 1);
-    p.lS = DBuf<double>(&c.pool, 4ull * (p.n_long ? p.n_long : 1));
-    if (p.n_long) {
-        k_compact_multi<<<grid1(n), BLOCK, 0, c.stream>>>(n, kind.get(), mid.get(), p.long_rows.get());
+    const uint32_t n = g.n, m = g.m;
+    // Entries per wavefront: two passes on large levels, one on small ones
+    // (more wavefronts to spread over the CUs).  Fixed by m alone, so the
+    // summation order does not depend on the device.
+    p.pw = m >= (1u << 22) ? 2 * PASS : PASS;
+    p.nwaves = (uint32_t)ceil_div(m, p.pw);
+    p.nblk_a = (uint32_t)ceil_div(p.nwaves, NW);
+    MI_CHECK((uint64_t)p.nwaves * p.pw < (1ull << 32) + p.pw, ST_ARG, "level too large for the step plan");
+    // row of every entry
+    p.rid = DBuf<uint32_t>(&c.pool, m ? m : 1);
+    if (m) {
+        DBuf<uint32_t> roc;
+        build_row_index(c, g, roc);
+        k_entry_rows<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc.get(), g.R(), p.rid.get());
         MI_HIP(hipGetLastError());
-        MI_HIP(hipMemsetAsync(p.lcnt.get(), 0, (size_t)p.n_long * 4, c.stream));
     }
-    // A workgroup / wavefront takes a run of consecutive items so the
-    // reduction tail is paid once per run: about 8 tile workgroups per CU and
-    // 64 row wavefronts per CU (or the tuned values).
-    const uint64_t ncu = (uint64_t)(c.num_cu > 0 ? c.num_cu : 256);
-    p.tiles_per_block = c.tune_tiles_per_block ? c.tune_tiles_per_block
-                                               : (uint32_t)std::max<uint64_t>(1, p.n_tiles / (ncu * 8));
-    p.ritems_per_wave = c.tune_items_per_wave ? c.tune_items_per_wave
-                                              : (uint32_t)std::max<uint64_t>(1, p.n_ritems / (ncu * 64));
-    const uint64_t nblk = std::max<uint64_t>(
-        {1, ceil_div(p.n_tiles, p.tiles_per_block), ceil_div(p.n_ritems, (uint64_t)p.ritems_per_wave * NW)});
-    const uint32_t ngroups = (uint32_t)ceil_div(nblk, RED_GROUP);
-    p.lpart = DBuf<double>(&c.pool, p.n_ritems ? p.n_ritems : 1);
-    p.partials = DBuf<double>(&c.pool, 4ull * nblk);
+    // the streaming kernel reads col / w with 16-byte loads
+    const auto misaligned = [](const void* q) { return q && (reinterpret_cast<uintptr_t>(q) & 15u) != 0; };
+    if (m && misaligned(g.Cc())) {
+        p.col_al = DBuf<uint32_t>(&c.pool, m);
+        k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.Cc(), p.col_al.get());
+        MI_HIP(hipGetLastError());
+    }
+    if (m && misaligned(g.W())) {
+        p.w_al = DBuf<uint32_t>(&c.pool, m);
+        k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.W(), p.w_al.get());
+        MI_HIP(hipGetLastError());
+    }
+    // empty rows
+    {
+        DBuf<uint32_t> flag(&c.pool, n), pos(&c.pool, n), tot(&c.pool, 1);
+        k_empty_flag<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), flag.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, flag.get(), pos.get(), n, tot.get());
+        uint32_t ne = 0;
+        read_words(c, tot.get(), &ne, 4);
+        p.n_empty = ne;
+        p.empty_rows = DBuf<uint32_t>(&c.pool, ne ? ne : 1);
+        if (ne) {
+            k_compact_empty<<<grid1(n), BLOCK, 0, c.stream>>>(n, flag.get(), pos.get(), p.empty_rows.get());
+            MI_HIP(hipGetLastError());
+        }
+    }
+    p.head = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
+    p.tail = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
+    p.bpart = DBuf<double>(&c.pool, 4ull * (p.nblk_a ? p.nblk_a : 1));
+    const uint64_t nthr = std::max<uint64_t>({1, p.nwaves, p.n_empty, p.nblk_a});
+    p.nblk_b = (uint32_t)ceil_div(nthr, BLOCK);
+    const uint32_t ngroups = (uint32_t)ceil_div(p.nblk_b, RED_GROUP);
+    p.partials = DBuf<double>(&c.pool, 4ull * p.nblk_b);
     p.gpart = DBuf<double>(&c.pool, 4ull * ngroups);
     p.gcnt = DBuf<uint32_t>(&c.pool, ngroups);
     p.fcnt = DBuf<uint32_t>(&c.pool, 1);
@@ -728,7 +598,7 @@ void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg
         k_fill_f64<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, ones.get(), 1.0);
         MI_HIP(hipGetLastError());
         StepArgs A = make_args(c, g, p, nullptr, nullptr, ones.get(), y.get(), 0.0, 2);
-        launch_step(c, g, 0, A);
+        launch_step(c, g, p, 0, A);
         k_wdeg_finish<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), y.get(), wdeg, gb_dev);
     }
     MI_HIP(hipGetLastError());
@@ -740,10 +610,10 @@ void spmv_shift(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wde
                 const double* x, double* y)
 {
     StepArgs A = make_args(c, g, p, wdeg, gb_dev, x, y, 0.0, 1);
-    launch_step(c, g, 0, A);
+    launch_step(c, g, p, 0, A);
 }
 
-double bench_power_step(Ctx& c, const DGraph& g, int reps, double* alg_bytes)
+void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
 {
     const uint32_t n = g.n;
     DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
@@ -755,23 +625,32 @@ double bench_power_step(Ctx& c, const DGraph& g, int reps, double* alg_bytes)
     for (uint32_t i = 0; i < n; i++) h[i] = 1.0 / (1.0 + (i % 97));
     MI_HIP(hipMemcpyAsync(x.get(), h.data(), (size_t)n * 8, hipMemcpyHostToDevice, c.stream));
     StepArgs A = make_args(c, g, p, wdeg.get(), gb.get(), x.get(), y.get(), 0.0, 1);
-    launch_step(c, g, 0, A);  // warm-up
+    launch_step(c, g, p, 0, A);  // warm-up
     MI_HIP(hipStreamSynchronize(c.stream));
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, e2;
     MI_HIP(hipEventCreate(&e0));
     MI_HIP(hipEventCreate(&e1));
+    MI_HIP(hipEventCreate(&e2));
+    // whole steps, then the streaming kernel alone, on the context stream
     MI_HIP(hipEventRecord(e0, c.stream));
-    for (int r = 0; r < reps; r++) launch_step(c, g, 0, A);
+    for (int r = 0; r < reps; r++) launch_step(c, g, p, 0, A);
     MI_HIP(hipEventRecord(e1, c.stream));
-    MI_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    for (int r = 0; r < reps; r++) launch_stream(c, g, 0, A);
+    MI_HIP(hipEventRecord(e2, c.stream));
+    MI_HIP(hipEventSynchronize(e2));
+    float ms_step = 0, ms_stream = 0;
+    MI_HIP(hipEventElapsedTime(&ms_step, e0, e1));
+    MI_HIP(hipEventElapsedTime(&ms_stream, e1, e2));
     MI_HIP(hipEventDestroy(e0));
     MI_HIP(hipEventDestroy(e1));
-    // rowptr + col (+ w) + wdeg + x (own entries; gathers of the same vector
-    // are cache hits and not counted again) + y
-    *alg_bytes = 4.0 * (n + 1) + 4.0 * g.m + (g.unit ? 0.0 : 4.0 * g.m) + 4.0 * n + 8.0 * n + 8.0 * n;
-    return (double)ms / reps;
+    MI_HIP(hipEventDestroy(e2));
+    out->step_ms = (double)ms_step / reps;
+    out->stream_ms = (double)ms_stream / reps;
+    // algorithmic bytes: col (+ w) + x (each entry once; repeated gathers are
+    // cache hits) + wdeg + y; the step adds rowptr
+    out->stream_bytes = 4.0 * g.m + (g.unit ? 0.0 : 4.0 * g.m) + 8.0 * n + 4.0 * n + 8.0 * n;
+    out->step_bytes = out->stream_bytes + 4.0 * (n + 1);
+    out->impl_bytes = out->stream_bytes + 4.0 * g.m;  // plus the per-entry row ids
 }
 
 void normalize(Ctx& c, double* d_u, uint32_t n)
@@ -822,16 +701,13 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     uint32_t chunk = 8;
     uint32_t flags[2] = {0, 0};
     for (;;) {
-        for (uint32_t k = 0; k < chunk; k++, issued++) launch_step(c, g, normLap, (issued & 1) ? A1 : A0);
+        for (uint32_t k = 0; k < chunk; k++, issued++) launch_step(c, g, plan, normLap, (issued & 1) ? A1 : A0);
         read_words(c, c.cnt + CT_DONE, flags, 8);
         if (flags[0]) break;
         if (chunk < 256) chunk *= 2;
     }
     unsigned long long done_iters = 0;
     read_words(c, c.niter, &done_iters, 8);
-    uint32_t err = 0;
-    read_words(c, c.cnt + CT_ERR, &err, 4);
-    MI_CHECK(err == 0, ST_INTERNAL, "power iteration: tile larger than its LDS stage");
     // the reference returns u = input of the last iteration (eigensolve_kokkos.h:157)
     if (done_iters >= 1) {
         k_materialize<<<grid1(n), BLOCK, 0, c.stream>>>(n, buf[(done_iters - 1) & 1], d_u, c.scal);

@@ -101,22 +101,30 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
 
 // ---------------------------------------------------------------- eigen
-// Load-balanced work lists of one level for the power-iteration step.
-// Tile list: tile i is the run of short rows [t_a[i], t_b[i]).  Row list:
-// item i is row r_a[i] whole (r_b[i] == 0) or chunk (r_b[i] & 0x7fffffff) of
-// the chunked row r_a[i] (bit 31 of r_b), whose index among the chunked rows
-// is r_c[i].  The rest is the per-launch reduction workspace of the plan.
+// Per-level plan of the power-iteration step: the row of every entry, the
+// empty rows, the per-wavefront boundary partials and the reduction
+// workspace.  Wavefront w of the streaming kernel owns entries
+// [w pw, (w+1) pw).
 struct LevelPlan {
-    DBuf<uint32_t> t_a, t_b, r_a, r_b, r_c, long_rows, lcnt, gcnt, fcnt;
-    DBuf<double> lpart, lS, partials, gpart;
-    uint32_t n_tiles = 0, n_ritems = 0, n_long = 0;
-    uint32_t tiles_per_block = 1;  // consecutive tiles one workgroup processes
-    uint32_t ritems_per_wave = 1;  // consecutive row items one wavefront processes
+    DBuf<uint32_t> rid, empty_rows, gcnt, fcnt;
+    DBuf<uint32_t> col_al, w_al;  // 16-byte aligned copies when the graph's arrays are not
+    DBuf<double> head, tail, bpart, partials, gpart;
+    uint32_t pw = 0, nwaves = 0, n_empty = 0;
+    uint32_t nblk_a = 0;  // workgroups of the streaming kernel
+    uint32_t nblk_b = 0;  // workgroups of the joining kernel
 };
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
-// Mean time (ms) of `reps` launches of the fused step on g; *alg_bytes gets
-// the algorithmic bytes of one launch.
-double bench_power_step(Ctx& c, const DGraph& g, int reps, double* alg_bytes);
+
+struct StepBench {
+    double step_ms = 0;       // one whole step (both kernels)
+    double stream_ms = 0;     // the streaming kernel alone
+    double step_bytes = 0;    // algorithmic bytes of a step
+    double stream_bytes = 0;  // algorithmic bytes of the streaming kernel
+    double impl_bytes = 0;    // bytes the streaming kernel reads and writes, row ids included
+};
+// Times `reps` shifted-product steps on g (and `reps` launches of the
+// streaming kernel alone) with HIP events on the context stream.
+void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out);
 
 struct PowerStats {
     uint64_t iters = 0;
