@@ -9,10 +9,20 @@ vertices, P^T A P builds, multilevel FP64 power iteration, sort-and-split)
 with the graph already resident in HBM and the partition left in HBM.
 
 Prints one JSON line.  `value` is milliseconds per bisection (lower is
-better); `roofline` is the finest-level FP64 shifted SpMV, timed with HIP
-events on the library stream; `cpu_baseline` times the CPU oracle (a scalar
+better).  `roofline` is the kernel that dominates the bisection: the
+streaming kernel of the FP64 power-iteration step (k_power_stream) on the
+level with the most eigensolve time (level 1 of R-MAT-20, which takes about
+400 iterations), timed with HIP events on the library stream over 50
+launches; its algorithmic bytes are col + w + x + wdeg + y of that level.
+`traffic` is the HBM traffic per launch of that kernel measured with
+rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a copy of
+known size (profiles/k_power_stream_rmat20_L1_traffic.txt), or null when
+that summary is absent.  `cpu_baseline` times the CPU oracle (a scalar
 restatement of the reference's Serial path) on a bounded sample and scales
 it to one whole bisection.
+
+    python bench.py [--steps K] [--warmup W]
+    python bench.py --roofline-only      # only the roofline measurement (for rocprofv3 runs)
 """
 from __future__ import annotations
 
@@ -29,6 +39,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 SEED = 1
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_power_stream_rmat20_L1_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of k_power_stream on R-MAT-20 level 1 from the
+    committed rocprofv3 counter summary (line 'traffic_bytes_per_launch: X')."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg: str) -> None:
@@ -51,7 +75,7 @@ def cpu_baseline(rp, col, w, gpu_stats, budget_s: float = 20.0) -> dict:
     spent = t_coarsen
     for l in range(H.nlevels):
         L = H.levels[l]
-        cap = 20 if L["n"] > 100000 else 200
+        cap = 50 if L["n"] > 100000 else min(400, max(1, int(iters_gpu[l]) if l < len(iters_gpu) else 400))
         u0 = rng.random(L["n"])
         t = time.perf_counter()
         _, it, _, _ = oracle.power_iter(L["rowptr"], L["col"], L["w"], u0, 0, 1e-10, cap)
@@ -80,6 +104,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline measurement of the dominant kernel, plus a 256 MiB device copy "
+                         "that calibrates rocprofv3's byte counters")
+    ap.add_argument("--roofline-level", type=int, default=-1,
+                    help="level for the roofline (default: the level with the most eigensolve time; "
+                         "1 with --roofline-only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -99,39 +129,80 @@ def main() -> int:
     g = ctx.graph(rp, col, w)
     L = ctx._L
     cfg = mi.Config(4, 0, 0, 1, 1e-10, 2)
-    d_part = C.c_void_p()
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     hip.hipFree.argtypes = [ctypes.c_void_p]
     hip.hipDeviceSynchronize.argtypes = []
-    assert hip.hipMalloc(ctypes.byref(d_part), 4 * n) == 0
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
 
-    def step():
-        st = mi.Stats()
-        pcg = C.c_uint64(SEED)
-        ec = C.c_uint32()
-        ctx.check(L.mi_partition_graph(ctx.h, g.h, C.byref(cfg), 0, SEED, C.byref(pcg), 1, d_part, 1,
-                                       C.byref(ec), C.byref(st)), "mi_partition_graph")
-        return st
+    ms = None
+    stats = None
+    if not args.roofline_only:
+        d_part = C.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(d_part), 4 * n) == 0
 
-    for i in range(args.warmup):
-        st = step()
-        log(f"warmup {i}: {st.total_ms:.2f} ms, cut {st.edge_cut}, levels {st.nlevels}")
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = step()
-    hip.hipDeviceSynchronize()
-    el = time.perf_counter() - t0
-    ms = el * 1000.0 / args.steps
-    stats = st.as_dict()
-    log(f"timed {args.steps} steps: {ms:.2f} ms/bisection, cut {stats['edge_cut']}")
+        def step():
+            st = mi.Stats()
+            pcg = C.c_uint64(SEED)
+            ec = C.c_uint32()
+            ctx.check(L.mi_partition_graph(ctx.h, g.h, C.byref(cfg), 0, SEED, C.byref(pcg), 1, d_part, 1,
+                                           C.byref(ec), C.byref(st)), "mi_partition_graph")
+            return st
 
-    # roofline of the dominant kernel: finest-level FP64 shifted SpMV
-    spmv_ms, spmv_bytes = g.bench_spmv(50)
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    log(f"SpMV: {spmv_ms * 1000:.1f} us/launch, {spmv_bytes / 1e6:.1f} MB algorithmic, {achieved:.0f} GB/s")
+        for i in range(args.warmup):
+            st = step()
+            log(f"warmup {i}: {st.total_ms:.2f} ms, cut {st.edge_cut}, levels {st.nlevels}")
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            st = step()
+        hip.hipDeviceSynchronize()
+        el = time.perf_counter() - t0
+        ms = el * 1000.0 / args.steps
+        stats = st.as_dict()
+        hip.hipFree(d_part)
+        log(f"timed {args.steps} steps: {ms:.2f} ms/bisection, cut {stats['edge_cut']}, "
+            f"iterations per level {stats['level_iters']}")
+
+    # roofline of the dominant kernel: the streaming kernel of the power step
+    # on the level with the most eigensolve time
+    lvl = args.roofline_level
+    if lvl < 0:
+        lvl = 1 if stats is None else int(np.argmax(stats["level_eigen_ms"]))
+    if lvl == 0:
+        gl = g
+    else:
+        H = g.hierarchy(SEED)
+        lv = H.level(lvl)
+        H.close()
+        gl = ctx.graph(lv["rowptr"], lv["col"], lv["w"])
+    b = gl.bench_step(50)
+    achieved = b["stream_bytes"] / (b["stream_ms"] * 1e-3) / 1e9
+    log(f"level {lvl} (n={gl.n}, m={gl.m}): step {b['step_ms'] * 1000:.1f} us, "
+        f"k_power_stream {b['stream_ms'] * 1000:.1f} us/launch, {b['stream_bytes'] / 1e6:.1f} MB algorithmic, "
+        f"{achieved:.0f} GB/s")
+    traffic = measured_traffic() if (lvl == 1 and args.scale == 20) else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "kernel": f"k_power_stream (FP64 power-iteration step, streaming part), R-MAT-{args.scale} level {lvl}",
+                "level_n": gl.n, "level_m": gl.m,
+                "alg_bytes_per_launch": b["stream_bytes"], "impl_bytes_per_launch": b["impl_bytes"],
+                "launch_us": round(b["stream_ms"] * 1000, 2),
+                "step_us": round(b["step_ms"] * 1000, 2),
+                "step_gbps": round(b["step_bytes"] / (b["step_ms"] * 1e-3) / 1e9, 1)}
+    if args.roofline_only:
+        # calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE: a device copy of known size
+        nb = 256 << 20
+        a_, b_ = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(a_), nb) == 0 and hip.hipMalloc(ctypes.byref(b_), nb) == 0
+        for _ in range(3):
+            hip.hipMemcpy(b_, a_, nb, 3)
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a_)
+        hip.hipFree(b_)
+        print(json.dumps({"roofline": roofline, "calibration_copy_bytes": nb}), flush=True)
+        return 0
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -158,16 +229,12 @@ def main() -> int:
                    "edge_cut": stats["edge_cut"],
                    "phases_ms": {k: round(stats[k], 3) for k in
                                  ("coarsen_ms", "eigen_ms", "partition_ms", "permute_ms", "heavy_ms",
-                                  "map_ms", "build_ms")}},
-        "spmv_gbps": round(achieved, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "k_spmv_shift (finest level, FP64)", "alg_bytes_per_launch": spmv_bytes,
-                     "launch_us": round(spmv_ms * 1000, 2)},
+                                  "map_ms", "build_ms")},
+                   "level_eigen_ms": [round(x, 3) for x in stats["level_eigen_ms"]]},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    hip.hipFree(d_part)
     g.close()
     ctx.close()
     return 0

@@ -177,11 +177,14 @@ int mi_spmv_shift(mi_ctx* ctx, const mi_graph* g, const double* x, double* y);
    reductions, eigensolve_kokkos.h:161-189) on g, timed with HIP events on the
    context stream over `reps` steps, and of its streaming kernel alone over
    `reps` launches.  Algorithmic bytes count col (+ w), the vectors x and y,
-   wdeg (and rowptr for the step); impl_bytes adds the per-entry row ids the
-   streaming kernel reads.  No counterpart in the reference. */
+   wdeg (and rowptr for the step); impl_bytes adds the per-entry row ids and
+   the row sums the streaming kernel moves.  column_blocks: 0 when the level
+   gathers x from global memory, else the number of LDS column blocks.  No
+   counterpart in the reference. */
 typedef struct {
     double step_ms, stream_ms;
     double step_bytes, stream_bytes, impl_bytes;
+    uint32_t column_blocks;
 } mi_step_bench;
 int mi_bench_step(mi_ctx* ctx, const mi_graph* g, int reps, mi_step_bench* out);
 /* Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), host arrays. */

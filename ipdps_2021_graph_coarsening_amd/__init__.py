@@ -43,7 +43,7 @@ class Config(C.Structure):
 class StepBench(C.Structure):
     """mi_step_bench"""
     _fields_ = [("step_ms", C.c_double), ("stream_ms", C.c_double), ("step_bytes", C.c_double),
-                ("stream_bytes", C.c_double), ("impl_bytes", C.c_double)]
+                ("stream_bytes", C.c_double), ("impl_bytes", C.c_double), ("column_blocks", C.c_uint32)]
 
 
 class Stats(C.Structure):

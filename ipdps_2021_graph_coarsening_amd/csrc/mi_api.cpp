@@ -686,6 +686,7 @@ int mi_bench_step(mi_ctx* ctx, const mi_graph* g, int reps, mi_step_bench* out)
         out->step_bytes = b.step_bytes;
         out->stream_bytes = b.stream_bytes;
         out->impl_bytes = b.impl_bytes;
+        out->column_blocks = b.ncb;
     });
 }
 

@@ -187,6 +187,20 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
     return v;
 }
 
+// Max over a block of BLOCK threads; result valid in every thread.
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* lds /* >= BLOCK/WAVE */)
+{
+    v = wave_max_u32(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / WAVE; k++) m = lds[k] > m ? lds[k] : m;
+    return m;
+}
+
 // Sum over a block of BLOCK threads; result valid in every thread.
 __device__ __forceinline__ double block_sum(double v, double* lds /* >= BLOCK/WAVE */)
 {

@@ -8,32 +8,46 @@
 // with 1n = 1/sqrt(n) everywhere and gb = max(2 wdeg), both u32 like
 // sgp_wgt_t.
 //
-// gfx950 form.  The deflation and normalisation of iteration k are never
-// written back: iteration k+1 reads its input as x_j = (y_j - a) * s, where
-// a and s follow from four sums reduced while y is produced,
+// gfx950 form.  The deflation and normalisation of iteration i are never
+// written back: iteration i+1 reads its input as x_j = (y_j - a) * s, where
+// a and s follow from four sums taken while y is produced,
 //   S1 = sum y,  S2 = sum y^2,  S3 = sum x y,  S4 = sum x,
 //   a = (S1 c) c,  ||y - a||^2 = S2 - 2 a S1 + a^2 n,  dot = (S3 - a S4) s.
 // (y is orthogonal to 1 up to rounding because x is and M 1 = gb 1, so the
-// norm formula has no cancellation.)  Each iteration streams the CSR once and
-// reads/writes one vector.
+// norm formula has no cancellation.)
 //
-// Work decomposition: flat over the nonzeros, independent of row lengths.
-// Wavefront w owns entries [w PW, (w+1) PW); each lane takes 8 consecutive
-// entries per pass (16-byte loads of col, w and a per-entry row id), gathers
-// x, and sums its entries by row in registers.  A segmented scan over the 64
-// lanes joins rows that cross lanes; a carry joins rows that cross passes.
-// Rows that begin and end inside the wavefront's range are finished at once
-// (shift, store y, the four sums).  A row entering the range from the
-// previous wavefront leaves its partial in head[w]; a row leaving it leaves
-// its partial in tail[w].  The second kernel adds, for every row that starts
-// in wavefront w and leaves it, tail[w] + head[w+1] + ... in wavefront order,
-// finishes those rows and the empty ones, reduces the four sums of all
-// workgroups and does the convergence bookkeeping.  Every sum has a fixed
-// order, so results are bit-identical run to run; the streaming kernel uses
-// no atomics.
+// An iteration is two kernels and uses no atomics.
+//
+// k_power_stream / k_power_stream_lds: the row sums of A x, flat over the
+// nonzeros and independent of row lengths.  Every workgroup first adds the
+// finish kernel's per-workgroup sums of the previous iteration, in one fixed
+// order, and so obtains the same a, s and dot as every other; workgroup 0
+// records them and decides convergence (one launch after the iteration it
+// concerns).  Wavefront w then owns the entries of one fixed-size range;
+// each lane takes 8 consecutive entries per pass (16-byte loads of the
+// column, weight and row id arrays), gathers x and sums its entries by row
+// in registers.  A segmented scan over the 64 lanes joins rows that cross
+// lanes; a carry joins rows that cross passes.  A row that begins and ends
+// inside the range stores its sum in part[row]; a row entering the range
+// from the previous one leaves its partial in head[w], one leaving it in
+// tail[w].  The _lds form serves levels whose x fits in LDS in at most 8
+// column blocks: the entries are laid out block by block (row r of block b
+// is "virtual row" b n + r), each 1024-thread workgroup stages its block's
+// slice of x, already shifted and scaled, in LDS and gathers from there, and
+// columns are stored as 16-bit offsets into the block.  Row ids are stored
+// as 16-bit offsets from the first row of their range when every range
+// spans fewer than 65536 rows.
+//
+// k_power_finish: one thread per row adds the row's sums over the column
+// blocks in block order, each sum being part[] or, for a row crossing
+// ranges, tail[first] + head[first+1] + ... in range order; then the shift,
+// y, and the four sums, reduced per workgroup.  Every sum has a fixed order,
+// so results are bit-identical run to run.
 #include "mi_kernels.hpp"
 
+#include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -41,11 +55,17 @@ namespace mi {
 
 namespace {
 
-constexpr int SEG = 8;                              // consecutive entries per lane and pass
-constexpr uint32_t PASS = WAVE * SEG;               // entries per wavefront pass
-constexpr uint32_t RINF = 0xFFFFFFFFu;              // row id of the lanes past the last entry
+constexpr int SEG = 8;                  // consecutive entries per lane and pass
+constexpr uint32_t PASS = WAVE * SEG;   // entries per wavefront pass
+constexpr uint32_t RINF = 0xFFFFFFFFu;  // row id of the lanes past the last entry
+constexpr uint32_t HEADF = 0x80000000u; // range metadata: the first row began earlier
 constexpr int NXCD = 8;
-constexpr int NW = BLOCK / WAVE;                    // wavefronts per workgroup
+constexpr int NW = BLOCK / WAVE;        // wavefronts per workgroup, global-gather form
+constexpr int LBLOCK = 1024;            // workgroup of the LDS form
+constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
+constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
+constexpr int MAXCB = 8;                // column blocks at most
+constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
@@ -59,6 +79,28 @@ __device__ __forceinline__ uint32_t logical_block()
     return xq * q + (xq < rem ? xq : rem) + b / NXCD;
 }
 
+// Block layout of a level: block q holds entries [boff[q], boff[q+1]) and its
+// ranges start at range index bwave[q].
+struct Blocks {
+    uint32_t ncb, pw, m;
+    uint32_t boff[MAXCB + 1];
+    uint32_t bwave[MAXCB + 1];
+};
+
+// range holding entry e (static indices into the block arrays)
+__device__ __forceinline__ uint32_t range_of(const Blocks& B, uint32_t e)
+{
+    uint32_t off = B.boff[0], wb = B.bwave[0];
+#pragma unroll
+    for (int q = 1; q < MAXCB; q++) {
+        if ((uint32_t)q < B.ncb && e >= B.boff[q]) {
+            off = B.boff[q];
+            wb = B.bwave[q];
+        }
+    }
+    return wb + (e - off) / B.pw;
+}
+
 // ------------------------------------------------------------------ plan
 // rid[e] = row holding entry e
 __global__ void k_entry_rows(uint32_t m, const uint32_t* __restrict__ roc, const uint32_t* __restrict__ rowptr,
@@ -68,47 +110,135 @@ __global__ void k_entry_rows(uint32_t m, const uint32_t* __restrict__ roc, const
     if (e < m) rid[e] = entry_row((uint32_t)e, roc, rowptr);
 }
 
-__global__ void k_empty_flag(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ flag)
-{
-    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < n) flag[r] = rowptr[r + 1] == rowptr[r] ? 1u : 0u;
-}
-
-__global__ void k_compact_empty(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
-                                uint32_t* __restrict__ out)
-{
-    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < n && flag[r]) out[pos[r]] = r;
-}
-
 __global__ void k_copy_u32(uint64_t n, const uint32_t* __restrict__ in, uint32_t* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) out[i] = in[i];
 }
 
+// Column-block layout, one wavefront per row: vcnt[q n + r] = entries of row
+// r whose column lies in block q (columns [q bw, (q+1) bw)).
+__global__ void __launch_bounds__(BLOCK) k_vcount(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                  const uint32_t* __restrict__ col, uint32_t bw, uint32_t ncb,
+                                                  uint32_t* __restrict__ vcnt)
+{
+    const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (r >= n) return;
+    uint32_t cnt[MAXCB];
+#pragma unroll
+    for (int q = 0; q < MAXCB; q++) cnt[q] = 0;
+    for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) {
+        const uint32_t b = col[e] / bw;
+#pragma unroll
+        for (int q = 0; q < MAXCB; q++) cnt[q] += (b == (uint32_t)q) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < MAXCB; q++) {
+        const uint32_t t = wave_sum_u32(cnt[q]);
+        if (lane == 0 && (uint32_t)q < ncb) vcnt[(size_t)q * n + r] = t;
+    }
+}
+
+// Scatters row r's entries to their blocks, keeping their order within each
+// (block, row) (ballot ranks the lanes of one chunk); columns become 16-bit
+// offsets into their block.
+__global__ void __launch_bounds__(BLOCK) k_vscatter(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col,
+                                                    const uint32_t* __restrict__ w, uint32_t bw, uint32_t ncb,
+                                                    const uint32_t* __restrict__ vrowptr,
+                                                    uint16_t* __restrict__ vcol, uint32_t* __restrict__ vw)
+{
+    const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (r >= n) return;
+    uint32_t pos[MAXCB];
+#pragma unroll
+    for (int q = 0; q < MAXCB; q++) pos[q] = (uint32_t)q < ncb ? vrowptr[(size_t)q * n + r] : 0u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t e0 = rowptr[r], e1 = rowptr[r + 1];
+    for (uint32_t base = e0; base < e1; base += WAVE) {
+        const uint32_t e = base + lane;
+        const bool in = e < e1;
+        const uint32_t c = in ? col[e] : 0u;
+        const uint32_t b = in ? c / bw : (uint32_t)MAXCB;
+        uint32_t dst = 0;
+#pragma unroll
+        for (int q = 0; q < MAXCB; q++) {
+            const unsigned long long mask = __ballot(b == (uint32_t)q);
+            if (b == (uint32_t)q) dst = pos[q] + (uint32_t)__popcll(mask & below);
+            pos[q] += (uint32_t)__popcll(mask);
+        }
+        if (in) {
+            vcol[dst] = (uint16_t)(c - b * bw);
+            if (w) vw[dst] = w[e];
+        }
+    }
+}
+
+// one block (x fits in LDS whole): columns as 16-bit values
+__global__ void k_col16(uint64_t m, const uint32_t* __restrict__ col, uint16_t* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < m) out[i] = (uint16_t)col[i];
+}
+
+// range metadata: first row of range w, with HEADF when that row began in
+// an earlier range
+__global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ wmeta)
+{
+    const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
+    if (w >= nwaves) return;
+    uint32_t e0 = 0, bstart = 0;
+#pragma unroll
+    for (int q = 0; q < MAXCB; q++) {
+        if ((uint32_t)q < (B.ncb ? B.ncb : 1u) && w >= B.bwave[q]) {
+            e0 = B.boff[q] + (w - B.bwave[q]) * B.pw;
+            bstart = B.boff[q];
+        }
+    }
+    const uint32_t r = rid[e0];
+    wmeta[w] = r | ((e0 > bstart && rid[e0 - 1] == r) ? HEADF : 0u);
+}
+
+// 16-bit row ids relative to the first row of each range; *maxd receives the
+// largest offset (the plan keeps the 32-bit ids when it does not fit)
+__global__ void __launch_bounds__(BLOCK) k_rid16(Blocks B, const uint32_t* __restrict__ rid,
+                                                 const uint32_t* __restrict__ wmeta, uint16_t* __restrict__ rid16,
+                                                 uint32_t* __restrict__ maxd)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    uint32_t mx = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < B.m; e += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t d = rid[e] - (wmeta[range_of(B, (uint32_t)e)] & ~HEADF);
+        rid16[e] = (uint16_t)d;
+        mx = d > mx ? d : mx;
+    }
+    mx = block_max_u32(mx, lds);
+    if (threadIdx.x == 0) atomicMax(maxd, mx);
+}
+
 // ------------------------------------------------------------------ step
 struct StepArgs {
     uint32_t n;
     uint32_t m;
-    const uint32_t* rowptr;
-    const uint32_t* col;   // 16-byte aligned
-    const uint32_t* w;     // 16-byte aligned (unused for unit weights)
-    const uint32_t* rid;   // row of every entry
+    uint32_t iter;           // 1-based iteration index of this launch (mode 0)
+    const uint32_t* rowptr;  // CSR of the (virtual) rows: block-major when ncb >= 2
+    const void* col;         // u16 block offsets (LDS form) or u32 columns, 16-byte aligned
+    const uint32_t* w;       // 16-byte aligned, same order (null for unit weights)
+    const void* rid;         // u16 offsets from the range's first row, or u32 rows
+    uint32_t r16;            // 1 when rid holds u16 offsets
+    const uint32_t* wmeta;   // per range: first row | HEADF
     const uint32_t* wdeg;
     const uint32_t* gbp;
-    uint32_t pw;           // entries per wavefront, a multiple of PASS
+    Blocks B;
+    uint32_t bw;             // vertices per column block
     uint32_t nwaves;
-    const uint32_t* empty_rows;
-    uint32_t n_empty;
-    double* head;          // per wavefront: partial of the row entering its range
-    double* tail;          // per wavefront: partial of the row leaving its range
-    double* bpart;         // per workgroup of the streaming kernel: S1..S4
-    uint32_t nblk_a;
-    double* partials;      // reduction workspace of the second kernel
-    double* gpart;
-    uint32_t* gcnt;
-    uint32_t* fcnt;
+    double* part;            // sum of each (virtual) row that lies in one range
+    double* head;            // per range: partial of the row entering it
+    double* tail;            // per range: partial of the row leaving it
+    uint32_t nblk_a;         // workgroups of the streaming kernel
+    uint32_t nblk_f;         // workgroups of the finish kernel
+    const double* fprev;     // finish sums S1..S4 per workgroup, previous iteration
+    double* fcur;            // ... this iteration
     const double* yprev;
     double* ynew;
     double c;
@@ -120,232 +250,400 @@ struct StepArgs {
     unsigned long long* niter;
 };
 
-__device__ __forceinline__ void load8(const uint32_t* __restrict__ p, uint64_t e, uint32_t (&o)[SEG])
+// Input transform of this iteration from the previous one's four sums, and
+// the convergence decision for the previous iteration (eigensolve_kokkos.h:
+// 155-189).  Every workgroup computes the same values in the same order;
+// workgroup 0 records them.  Returns false when the loop has ended.
+template <int NWV>
+__device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, double& s, double (*lds)[NWV])
 {
-    const uint4 v0 = *reinterpret_cast<const uint4*>(p + e);
-    const uint4 v1 = *reinterpret_cast<const uint4*>(p + e + 4);
+    a = 0.0;
+    s = 1.0;
+    if (A.mode != 0) return true;
+    if (A.cnt[CT_DONE]) return false;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (A.iter < 2) {
+        if (lead) {
+            A.scal[SC_A] = 0.0;
+            A.scal[SC_S] = 1.0;
+        }
+        return true;
+    }
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    double T[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double v = threadIdx.x < A.nblk_f ? A.fprev[(size_t)threadIdx.x * 4 + k] : 0.0;
+        v = wave_sum(v);
+        if (lane == 0) lds[k][wid] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < NWV; j++) t += lds[k][j];
+        T[k] = t;
+    }
+    const double mult = T[0] * A.c;  // dot(v, vec1); 0 for normLap (the finish kernel zeroes S1)
+    const double an = mult * A.c;    // subtracted from every entry
+    const double nrm2 = T[1] - 2.0 * an * T[0] + an * an * (double)A.n;
+    const double sn = 1 / sqrt(nrm2);
+    const double dot = (T[2] - an * T[3]) * sn;
+    const unsigned long long itn = A.iter - 1;                 // iterations done
+    const double lastd = A.scal[SC_DOT + (A.iter & 1u)];        // dot of iteration iter - 2
+    const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
+    if (lead) {
+        A.scal[SC_DOT + ((A.iter - 1) & 1u)] = dot;
+        if (go_on) {
+            A.scal[SC_A] = an;
+            A.scal[SC_S] = sn;
+        } else {
+            A.cnt[CT_DONE] = 1u;
+            A.cnt[CT_MAXR] = itn >= A.iter_max ? 1u : 0u;
+            *A.niter = itn;
+        }
+    }
+    a = an;
+    s = sn;
+    return go_on;
+}
+
+// One lane's 8 entries of a pass as loaded: row ids, columns, weights.
+struct Raw {
+    uint4 r0, r1, c0, c1, w0, w1;
+};
+
+// Issues the 16-byte loads of the 8 entries at e0 when they all lie below E1
+// (returns false otherwise: the caller loads that lane entry by entry).
+template <bool UNIT, bool LDSX, bool R16, class ColT, class RidT>
+__device__ __forceinline__ bool fetch(const StepArgs& A, const ColT* col, const RidT* rid, uint64_t e0, uint64_t E1,
+                                      Raw& R)
+{
+    if (e0 + SEG > E1) return false;
+    R.r0 = *reinterpret_cast<const uint4*>(rid + e0);
+    if (!R16) R.r1 = *reinterpret_cast<const uint4*>(rid + e0 + 4);
+    R.c0 = *reinterpret_cast<const uint4*>(col + e0);
+    if (!LDSX) R.c1 = *reinterpret_cast<const uint4*>(col + e0 + 4);
+    if (!UNIT) {
+        R.w0 = *reinterpret_cast<const uint4*>(A.w + e0);
+        R.w1 = *reinterpret_cast<const uint4*>(A.w + e0 + 4);
+    }
+    return true;
+}
+
+__device__ __forceinline__ void unpack16(uint4 v, uint32_t (&o)[SEG])
+{
+    o[0] = v.x & 0xffffu; o[1] = v.x >> 16; o[2] = v.y & 0xffffu; o[3] = v.y >> 16;
+    o[4] = v.z & 0xffffu; o[5] = v.z >> 16; o[6] = v.w & 0xffffu; o[7] = v.w >> 16;
+}
+
+__device__ __forceinline__ void unpack32(uint4 v0, uint4 v1, uint32_t (&o)[SEG])
+{
     o[0] = v0.x; o[1] = v0.y; o[2] = v0.z; o[3] = v0.w;
     o[4] = v1.x; o[5] = v1.y; o[6] = v1.z; o[7] = v1.w;
 }
 
+template <bool LDSX, bool R16>
+__device__ __forceinline__ void unpack(const Raw& R, uint32_t (&rr)[SEG], uint32_t (&cc)[SEG], uint32_t (&ww)[SEG])
+{
+    if (R16)
+        unpack16(R.r0, rr);
+    else
+        unpack32(R.r0, R.r1, rr);
+    if (LDSX)
+        unpack16(R.c0, cc);
+    else
+        unpack32(R.c0, R.c1, cc);
+    unpack32(R.w0, R.w1, ww);
+}
+
+// row id (or offset) of a lane's first entry in a loaded pass
+template <bool R16>
+__device__ __forceinline__ uint32_t first_id(const Raw& R)
+{
+    return R16 ? (R.r0.x & 0xffffu) : R.r0.x;
+}
+
+// Stores a (virtual) row sum: part[r], or head[wv] for the row that entered
+// the range from the previous one.
+__device__ __forceinline__ void put(const StepArgs& A, uint32_t r, double v, uint32_t hrow, uint32_t wv)
+{
+    if (r == RINF) return;
+    if (r == hrow)
+        A.head[wv] = v;
+    else
+        A.part[r] = v;
+}
+
+// Row sums of the entries [E0, E1) by one wavefront (range index wv; nrow =
+// row of entry E1 when it continues this block, else RINF).  x_j is read as
+// (yprev[j] - a) s, or from LDS (already shifted and scaled) in the LDS form.
+template <bool UNIT, bool LDSX, bool R16>
+__device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint64_t E0, uint64_t E1,
+                                             uint32_t nrow, double a, double s, const double* xl)
+{
+    using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
+    using RidT = typename std::conditional<R16, uint16_t, uint32_t>::type;
+    const ColT* col = static_cast<const ColT*>(A.col);
+    const RidT* rid = static_cast<const RidT*>(A.rid);
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t meta = A.wmeta[wv];
+    const uint32_t row0 = meta & ~HEADF;
+    const uint32_t rbase = R16 ? row0 : 0u;
+    const uint32_t hrow = (meta & HEADF) ? row0 : RINF;
+    uint32_t cr = hrow;  // row carried into the pass (RINF: none)
+    double cs = 0.0;     // its partial sum so far
+    // The 16-byte loads of pass p+1 are issued before pass p is processed.
+    Raw nx;
+    bool nfull = fetch<UNIT, LDSX, R16>(A, col, rid, E0 + (uint64_t)lane * SEG, E1, nx);
+    for (uint64_t base = E0; base < E1; base += PASS) {
+        const uint64_t e0 = base + (uint64_t)lane * SEG;
+        uint32_t rr[SEG], cc[SEG], ww[SEG];
+        if (nfull) {
+            unpack<LDSX, R16>(nx, rr, cc, ww);
+#pragma unroll
+            for (int u = 0; u < SEG; u++) rr[u] += rbase;
+        } else {
+#pragma unroll
+            for (int u = 0; u < SEG; u++) {
+                const bool in = e0 + u < E1;
+                rr[u] = in ? (uint32_t)rid[e0 + u] + rbase : RINF;
+                cc[u] = in ? (uint32_t)col[e0 + u] : 0u;
+                if (!UNIT) ww[u] = in ? A.w[e0 + u] : 0u;
+            }
+        }
+        const uint64_t nbase = base + PASS;
+        if (nbase < E1) nfull = fetch<UNIT, LDSX, R16>(A, col, rid, nbase + (uint64_t)lane * SEG, E1, nx);
+        // row of the entry after lane 63's last one: lane 0's first entry of
+        // the next pass, or the next range's first row (in a partial pass
+        // lane 63 holds no entries)
+        uint32_t nx63 = nrow;
+        if (nbase < E1) {
+            if (nbase + SEG <= E1)
+                nx63 = __shfl(first_id<R16>(nx), 0, WAVE) + rbase;
+            else
+                nx63 = (uint32_t)rid[nbase] + rbase;
+        }
+        double pv[SEG];
+#pragma unroll
+        for (int u = 0; u < SEG; u++) {
+            double xv;
+            if (LDSX)
+                xv = rr[u] != RINF ? xl[cc[u]] : 0.0;
+            else
+                xv = rr[u] != RINF ? (A.yprev[cc[u]] - a) * s : 0.0;
+            pv[u] = UNIT ? xv : (double)ww[u] * xv;
+        }
+        // lane-local segments
+        const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
+        uint32_t mid_r[SEG - 1];
+        double mid_s[SEG - 1];
+        double fs = 0.0, acc = pv[0];
+        uint32_t cur = rr[0];
+        bool multi = false;
+#pragma unroll
+        for (int u = 1; u < SEG; u++) {
+            mid_r[u - 1] = RINF;
+            mid_s[u - 1] = 0.0;
+            if (rr[u] != cur) {
+                if (!multi) {
+                    fs = acc;  // first segment: may continue from earlier lanes
+                    multi = true;
+                } else {
+                    mid_r[u - 1] = cur;  // begins and ends in this lane
+                    mid_s[u - 1] = acc;
+                }
+                cur = rr[u];
+                acc = pv[u];
+            } else {
+                acc += pv[u];
+            }
+        }
+        // segmented inclusive scan of the last segments over the lanes (row
+        // ids never decrease along a range, so equal keys are contiguous lanes)
+        double sc = acc;
+        const uint32_t key = last_row;
+        if (lane == 0 && !multi && key == cr) sc = cs + sc;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const double ov = __shfl_up(sc, o, WAVE);
+            const uint32_t ok = __shfl_up(key, o, WAVE);
+            if (lane >= (uint32_t)o && ok == key) sc = ov + sc;
+        }
+        const double psc = __shfl_up(sc, 1, WAVE);
+        const uint32_t pkey = __shfl_up(key, 1, WAVE);
+        uint32_t nxt = __shfl_down(first_row, 1, WAVE);
+        if (lane == WAVE - 1) nxt = nx63;
+        // a lane's first segment closes inside the lane when it has several
+        if (multi) {
+            const bool crosses = lane == 0 ? (cr == first_row) : (pkey == first_row);
+            const double before = lane == 0 ? cs : psc;
+            put(A, first_row, crosses ? before + fs : fs, hrow, wv);
+        }
+#pragma unroll
+        for (int u = 0; u < SEG - 1; u++) put(A, mid_r[u], mid_s[u], hrow, wv);
+        // the last segment closes at the lane's end when the next entry is another row
+        if (nxt != last_row) put(A, last_row, sc, hrow, wv);
+        // carry into the next pass
+        const uint32_t k63 = __shfl(key, WAVE - 1, WAVE);
+        const double s63 = __shfl(sc, WAVE - 1, WAVE);
+        const bool open = (k63 != RINF) && (nx63 == k63);
+        cr = open ? k63 : RINF;
+        cs = open ? s63 : 0.0;
+    }
+    // a row still open continues past this range
+    if (lane == 0 && cr != RINF) {
+        if (cr == hrow)
+            A.head[wv] = cs;
+        else
+            A.tail[wv] = cs;
+    }
+}
+
+template <bool UNIT, bool R16>
+__global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
+{
+    __shared__ double red[4][NW];
+    double a, s;
+    if (!iteration_start<NW>(A, a, s, red)) return;
+    const uint32_t wv = logical_block() * NW + threadIdx.x / WAVE;
+    if (wv >= A.nwaves) return;
+    const uint64_t E0 = (uint64_t)wv * A.B.pw;
+    const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
+    const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+    stream_range<UNIT, false, R16>(A, wv, E0, E1, nrow, a, s, nullptr);
+}
+
+// Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
+// [g W / G, (g+1) W / G) in order and stages the slice of x of each column
+// block its share touches.  Which workgroup streams a range does not change
+// what is summed, so the result does not depend on the grid.
+template <bool UNIT, bool R16>
+__global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
+{
+    __shared__ double xl[XB];
+    __shared__ double red[4][LNW];
+    double a, s;
+    if (!iteration_start<LNW>(A, a, s, red)) return;
+    const uint32_t G = gridDim.x, g = logical_block(), wid = threadIdx.x / WAVE;
+    const uint32_t R1 = (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
+    uint32_t cur = (uint32_t)((uint64_t)g * A.nwaves / G);
+    while (cur < R1) {
+        // the column block holding range cur (static indices into the argument arrays)
+        uint32_t b = 0, off0 = A.B.boff[0], off1 = A.B.boff[1], wbase = A.B.bwave[0], wend = A.B.bwave[1];
+#pragma unroll
+        for (int q = 1; q < MAXCB; q++) {
+            if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) {
+                b = q;
+                off0 = A.B.boff[q];
+                off1 = A.B.boff[q + 1];
+                wbase = A.B.bwave[q];
+                wend = A.B.bwave[q + 1];
+            }
+        }
+        const uint32_t seg_end = R1 < wend ? R1 : wend;
+        const uint32_t x0 = b * A.bw;
+        const uint32_t xn = (A.n - x0) < A.bw ? (A.n - x0) : A.bw;
+        __syncthreads();  // the previous block's gathers are done
+        for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = (A.yprev[x0 + i] - a) * s;
+        __syncthreads();
+        for (uint32_t wv = cur + wid; wv < seg_end; wv += LNW) {
+            const uint64_t E0 = (uint64_t)off0 + (uint64_t)(wv - wbase) * A.B.pw;
+            const uint64_t E1 = (E0 + A.B.pw < off1) ? E0 + A.B.pw : off1;
+            const uint32_t nrow = (wv + 1 < wend) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+            stream_range<UNIT, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
+        }
+        cur = seg_end;
+    }
+}
+
 // y_r from the row sum (the shifted product of the reference, or A x in mode 2)
 template <bool NORMLAP>
-__device__ __forceinline__ double shifted(const StepArgs& A, uint32_t r, double spmv, double xr, uint32_t wd,
-                                          uint32_t gb)
+__device__ __forceinline__ double shifted(const StepArgs& A, double spmv, double xr, uint32_t wd, uint32_t gb)
 {
     if (A.mode == 2) return spmv;
     if (!NORMLAP) return (double)(uint32_t)(gb - wd) * xr + spmv;
     return 0.5 * xr + 0.5 * spmv * (1.0 / (double)wd);
 }
 
-// Finishes row r with row sum v: stores y_r and adds the four sums.  The row
-// that entered the wavefront's range from the previous one (hrow) is not
-// complete here: its partial goes to head[wv] instead.
+// Row sums over the column blocks, the shift, y and the four sums per
+// workgroup (fixed grid of at most MAXF workgroups, rows strided).
 template <bool NORMLAP>
-__device__ __forceinline__ void finish(const StepArgs& A, uint32_t r, double v, uint32_t hrow, uint32_t wv,
-                                       double a, double s, uint32_t gb, double (&S)[4])
-{
-    if (r == RINF) return;
-    if (r == hrow) {
-        A.head[wv] = v;
-        return;
-    }
-    const double xr = (A.yprev[r] - a) * s;
-    const double y = shifted<NORMLAP>(A, r, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
-    A.ynew[r] = y;
-    S[0] += y;
-    S[1] += y * y;
-    S[2] += xr * y;
-    S[3] += xr;
-}
-
-template <bool UNIT, bool NORMLAP>
-__global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
+__global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
 {
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
     __shared__ double red[4][NW];
-    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const double a = A.mode ? 0.0 : A.scal[SC_A];
     const double s = A.mode ? 1.0 : A.scal[SC_S];
     const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
+    const uint32_t nb = A.B.ncb > 1 ? A.B.ncb : 1u;
     double S[4] = {0, 0, 0, 0};
-    const uint32_t wv = logical_block() * NW + wid;
-    if (wv < A.nwaves) {
-        const uint64_t E0 = (uint64_t)wv * A.pw;
-        const uint64_t E1 = (E0 + A.pw < A.m) ? E0 + A.pw : A.m;
-        // the row at E0 entered from the previous wavefront if entry E0 - 1 is in it
-        const uint32_t r_a = A.rid[E0];
-        const uint32_t hrow = (E0 > 0 && A.rid[E0 - 1] == r_a) ? r_a : RINF;
-        uint32_t cr = hrow;  // row carried into the pass (RINF: none)
-        double cs = 0.0;     // its partial sum so far
-        for (uint64_t base = E0; base < E1; base += PASS) {
-            const uint64_t e0 = base + (uint64_t)lane * SEG;
-            uint32_t rr[SEG], cc[SEG], ww[SEG];
-            if (e0 + SEG <= E1) {
-                load8(A.rid, e0, rr);
-                load8(A.col, e0, cc);
-                if (!UNIT) load8(A.w, e0, ww);
-            } else {
+    for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
+        // all blocks' bounds, then all first partials, as independent loads
+        uint32_t s0[MAXCB], s1[MAXCB], w0[MAXCB], w1[MAXCB];
+        double first[MAXCB];
 #pragma unroll
-                for (int u = 0; u < SEG; u++) {
-                    const bool in = e0 + u < E1;
-                    rr[u] = in ? A.rid[e0 + u] : RINF;
-                    cc[u] = in ? A.col[e0 + u] : 0u;
-                    if (!UNIT) ww[u] = in ? A.w[e0 + u] : 0u;
-                }
+        for (int q = 0; q < MAXCB; q++) {
+            s0[q] = s1[q] = 0;
+            if ((uint32_t)q < nb) {
+                const uint32_t vr = (uint32_t)q * A.n + r;
+                s0[q] = A.rowptr[vr];
+                s1[q] = A.rowptr[vr + 1];
             }
-            // row of the entry after lane 63's last one (next pass or next wavefront)
-            const uint32_t nx63 = (base + PASS < A.m) ? A.rid[base + PASS] : RINF;
-            double pv[SEG];
+        }
 #pragma unroll
-            for (int u = 0; u < SEG; u++) {
-                const double xv = rr[u] != RINF ? (A.yprev[cc[u]] - a) * s : 0.0;
-                pv[u] = UNIT ? xv : (double)ww[u] * xv;
+        for (int q = 0; q < MAXCB; q++) {
+            first[q] = 0.0;
+            w0[q] = w1[q] = 0;
+            if (s0[q] != s1[q]) {
+                w0[q] = A.B.bwave[q] + (s0[q] - A.B.boff[q]) / A.B.pw;
+                w1[q] = A.B.bwave[q] + (s1[q] - 1 - A.B.boff[q]) / A.B.pw;
+                first[q] = w0[q] == w1[q] ? A.part[(uint32_t)q * A.n + r] : A.tail[w0[q]];
             }
-            // lane-local segments
-            const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
-            uint32_t mid_r[SEG - 1];
-            double mid_s[SEG - 1];
-            double fs = 0.0, acc = pv[0];
-            uint32_t cur = rr[0];
-            bool multi = false;
+        }
+        double v = 0.0;
 #pragma unroll
-            for (int u = 1; u < SEG; u++) {
-                mid_r[u - 1] = RINF;
-                mid_s[u - 1] = 0.0;
-                if (rr[u] != cur) {
-                    if (!multi) {
-                        fs = acc;  // first segment: may continue from earlier lanes
-                        multi = true;
-                    } else {
-                        mid_r[u - 1] = cur;  // begins and ends in this lane
-                        mid_s[u - 1] = acc;
+        for (int q = 0; q < MAXCB; q++) {
+            if (s0[q] != s1[q]) {
+                double vb = first[q];
+                if (w1[q] > w0[q]) {
+                    // a row crossing ranges: in range order, eight loads at a time
+                    uint32_t k = w0[q] + 1;
+                    for (; k + 8 <= w1[q] + 1; k += 8) {
+                        double h[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) h[j] = A.head[k + j];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) vb += h[j];
                     }
-                    cur = rr[u];
-                    acc = pv[u];
-                } else {
-                    acc += pv[u];
+                    for (; k <= w1[q]; k++) vb += A.head[k];
                 }
+                v = q == 0 ? vb : v + vb;
             }
-            // segmented inclusive scan of the last segments over the lanes
-            // (row ids never decrease along the entries, so equal keys are
-            // contiguous lanes)
-            double sc = acc;
-            const uint32_t key = last_row;
-            if (lane == 0 && !multi && key == cr) sc = cs + sc;
-#pragma unroll
-            for (int o = 1; o < WAVE; o <<= 1) {
-                const double ov = __shfl_up(sc, o, WAVE);
-                const uint32_t ok = __shfl_up(key, o, WAVE);
-                if (lane >= (uint32_t)o && ok == key) sc = ov + sc;
-            }
-            const double psc = __shfl_up(sc, 1, WAVE);
-            const uint32_t pkey = __shfl_up(key, 1, WAVE);
-            uint32_t nxt = __shfl_down(first_row, 1, WAVE);
-            if (lane == WAVE - 1) nxt = nx63;
-            // a lane's first segment closes inside the lane when it has several
-            uint32_t xr_row = RINF;
-            double xr_sum = 0.0;
-            if (multi) {
-                const bool crosses = lane == 0 ? (cr == first_row) : (pkey == first_row);
-                const double before = lane == 0 ? cs : psc;
-                xr_row = first_row;
-                xr_sum = crosses ? before + fs : fs;
-            }
-            // the last segment closes at the lane's end when the next entry is another row
-            const uint32_t lr_row = (nxt != last_row) ? last_row : RINF;
-            // carry into the next pass
-            const uint32_t k63 = __shfl(key, WAVE - 1, WAVE);
-            const double s63 = __shfl(sc, WAVE - 1, WAVE);
-            const bool open = (k63 != RINF) && (nx63 == k63);
-            finish<NORMLAP>(A, xr_row, xr_sum, hrow, wv, a, s, gb, S);
-#pragma unroll
-            for (int u = 0; u < SEG - 1; u++) finish<NORMLAP>(A, mid_r[u], mid_s[u], hrow, wv, a, s, gb, S);
-            finish<NORMLAP>(A, lr_row, sc, hrow, wv, a, s, gb, S);
-            cr = open ? k63 : RINF;
-            cs = open ? s63 : 0.0;
         }
-        // a row still open continues past this wavefront's range
-        if (lane == 0 && cr != RINF) {
-            if (cr == hrow)
-                A.head[wv] = cs;
-            else
-                A.tail[wv] = cs;
-        }
+        const double xr = (A.yprev[r] - a) * s;
+        const double y = shifted<NORMLAP>(A, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
+        A.ynew[r] = y;
+        S[0] += NORMLAP ? 0.0 : y;
+        S[1] += y * y;
+        S[2] += xr * y;
+        S[3] += xr;
     }
-    if (A.mode == 2) return;
+    if (A.mode != 0) return;
     double tot[4];
     block_sum_k<4>(S, tot, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) A.bpart[(size_t)blockIdx.x * 4 + k] = tot[k];
+        for (int k = 0; k < 4; k++) A.fcur[(size_t)blockIdx.x * 4 + k] = tot[k];
     }
 }
 
-// Rows crossing wavefront boundaries, empty rows, the global sums and the
-// convergence bookkeeping.
-template <bool NORMLAP>
-__global__ void __launch_bounds__(BLOCK) k_power_join(StepArgs A)
-{
-    if (A.mode == 0 && A.cnt[CT_DONE]) return;
-    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const double a = A.mode ? 0.0 : A.scal[SC_A];
-    const double s = A.mode ? 1.0 : A.scal[SC_S];
-    const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
-    double S[4] = {0, 0, 0, 0};
-    if (t < A.nwaves) {
-        const uint64_t E0 = t * A.pw;
-        const uint64_t E1 = (E0 + A.pw < A.m) ? E0 + A.pw : A.m;
-        const uint32_t r = A.rid[E1 - 1];
-        if (E1 < A.m && A.rid[E1] == r && A.rowptr[r] >= E0) {
-            // row r starts in wavefront t and ends in wavefront klast
-            const uint64_t klast = ((uint64_t)A.rowptr[r + 1] - 1) / A.pw;
-            double tot = A.tail[t];
-            for (uint64_t k = t + 1; k <= klast; k++) tot += A.head[k];
-            finish<NORMLAP>(A, r, tot, RINF, 0, a, s, gb, S);
-        }
-    }
-    if (t < A.n_empty) finish<NORMLAP>(A, A.empty_rows[t], 0.0, RINF, 0, a, s, gb, S);
-    if (A.mode == 2) return;
-    for (uint64_t j = t; j < A.nblk_a; j += (uint64_t)gridDim.x * BLOCK) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) S[k] += A.bpart[j * 4 + k];
-    }
-    double out[4];
-    if (!grid_reduce2<4>(S, blockIdx.x, gridDim.x, A.partials, A.gpart, A.gcnt, A.fcnt, out)) return;
-    if (threadIdx.x != 0) return;
-    const double T1 = out[0], T2 = out[1], T3 = out[2], T4 = out[3];
-    const double mult = NORMLAP ? 0.0 : T1 * A.c;  // dot(v, vec1)
-    const double an = mult * A.c;                   // subtracted from every entry
-    const double nrm2 = T2 - 2.0 * an * T1 + an * an * (double)A.n;
-    const double sn = 1 / sqrt(nrm2);
-    const double dot = (T3 - an * T4) * sn;
-    if (A.mode) {
-        A.scal[SC_MULT] = mult;
-        A.scal[SC_SS] = nrm2;
-        return;
-    }
-    A.scal[SC_AP] = a;
-    A.scal[SC_SP] = s;
-    A.scal[SC_A] = an;
-    A.scal[SC_S] = sn;
-    const double lastd = A.scal[SC_DOT];
-    A.scal[SC_LASTDOT] = lastd;
-    A.scal[SC_DOT] = dot;
-    const unsigned long long itn = *A.niter + 1ull;
-    *A.niter = itn;
-    const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
-    if (!go_on) {
-        A.cnt[CT_DONE] = 1u;
-        A.cnt[CT_MAXR] = itn >= A.iter_max ? 1u : 0u;
-    }
-}
-
-// u = (y - a_prev) * s_prev : the input of the last iteration
+// u = (y - a) * s : the input of the last iteration
 __global__ void k_materialize(uint32_t n, const double* __restrict__ y, double* __restrict__ u, const double* scal)
 {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) u[i] = (y[i] - scal[SC_AP]) * scal[SC_SP];
+    if (i < n) u[i] = (y[i] - scal[SC_A]) * scal[SC_S];
 }
 
 // ------------------------------------------------------------------ setup helpers
@@ -415,18 +713,21 @@ __global__ void k_fill_f64(uint32_t n, double* __restrict__ x, double v)
 
 // weighted degree from y = A 1 (exact: integer sums below 2^53) or from the
 // row lengths of a unit-weight graph; gb = max(2 wdeg) (eigensolve_kokkos.h:119-136)
-__global__ void k_wdeg_finish(uint32_t n, const uint32_t* __restrict__ rowptr, const double* __restrict__ y,
-                              uint32_t* __restrict__ wdeg, uint32_t* __restrict__ gb)
+// (fixed grid, one atomic per workgroup)
+__global__ void __launch_bounds__(BLOCK) k_wdeg_finish(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                       const double* __restrict__ y, uint32_t* __restrict__ wdeg,
+                                                       uint32_t* __restrict__ gb)
 {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ uint32_t lds[BLOCK / WAVE];
     uint32_t t = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
         const uint32_t s = y ? (uint32_t)y[i] : rowptr[i + 1] - rowptr[i];
         wdeg[i] = s;
-        t = (uint32_t)(2.0 * (double)s);
+        const uint32_t t2 = (uint32_t)(2.0 * (double)s);
+        t = t2 > t ? t2 : t;
     }
-    t = wave_max_u32(t);
-    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(gb, t);
+    t = block_max_u32(t, lds);
+    if (threadIdx.x == 0) atomicMax(gb, t);
 }
 
 __global__ void k_gb_floor(uint32_t* gb)
@@ -441,20 +742,21 @@ __global__ void k_prolong(uint32_t n, const uint32_t* __restrict__ vcmap, const 
     if (i < n) out[i] = in[vcmap[i]];
 }
 
+// dot_j of iteration j lives in scal[SC_DOT + (j & 1)]; dot_0 = 0
 __global__ void k_loop_init(double* scal, uint32_t* cnt, unsigned long long* niter)
 {
     if (threadIdx.x == 0) {
         scal[SC_DOT] = 0.0;
-        scal[SC_LASTDOT] = 1.0;
+        scal[SC_DOT + 1] = 0.0;
         scal[SC_A] = 0.0;
         scal[SC_S] = 1.0;
-        scal[SC_AP] = 0.0;
-        scal[SC_SP] = 1.0;
         cnt[CT_DONE] = 0u;
         cnt[CT_MAXR] = 0u;
         *niter = 0ull;
     }
 }
+
+
 
 
 StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wdeg, const uint32_t* gb,
@@ -463,24 +765,34 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     StepArgs A;
     A.n = g.n;
     A.m = g.m;
-    A.rowptr = g.R();
-    A.col = p.col_al.get() ? p.col_al.get() : g.Cc();
-    A.w = p.w_al.get() ? p.w_al.get() : g.W();
-    A.rid = p.rid.get();
+    A.iter = 0;
+    A.rowptr = p.vrowptr.get() ? p.vrowptr.get() : g.R();
+    if (p.ncb)
+        A.col = p.col16.get();
+    else
+        A.col = p.col_al.get() ? p.col_al.get() : g.Cc();
+    A.w = g.unit ? nullptr : (p.w_al.get() ? p.w_al.get() : g.W());
+    A.rid = p.rid16.get() ? static_cast<const void*>(p.rid16.get()) : static_cast<const void*>(p.rid.get());
+    A.r16 = p.rid16.get() ? 1u : 0u;
+    A.wmeta = p.wmeta.get();
     A.wdeg = wdeg;
     A.gbp = gb;
-    A.pw = p.pw;
+    A.B.ncb = p.ncb;
+    A.B.pw = p.pw;
+    A.B.m = g.m;
+    for (int q = 0; q <= MAXCB; q++) {
+        A.B.boff[q] = p.boff[q];
+        A.B.bwave[q] = p.bwave[q];
+    }
+    A.bw = p.bw;
     A.nwaves = p.nwaves;
-    A.empty_rows = p.empty_rows.get();
-    A.n_empty = p.n_empty;
+    A.part = p.part.get();
     A.head = p.head.get();
     A.tail = p.tail.get();
-    A.bpart = p.bpart.get();
     A.nblk_a = p.nblk_a;
-    A.partials = p.partials.get();
-    A.gpart = p.gpart.get();
-    A.gcnt = p.gcnt.get();
-    A.fcnt = p.fcnt.get();
+    A.nblk_f = p.nblk_f;
+    A.fprev = p.fsum.get();
+    A.fcur = p.fsum.get();
     A.yprev = yprev;
     A.ynew = ynew;
     A.c = 1 / std::sqrt((double)g.n);
@@ -493,38 +805,50 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     return A;
 }
 
-void launch_stream(Ctx& c, const DGraph& g, int normLap, const StepArgs& A)
+void launch_stream(Ctx& c, const StepArgs& A)
 {
-    if (A.nblk_a == 0) return;
-    const bool unit = A.w == nullptr;
-    if (unit) {
-        if (normLap)
-            k_power_stream<true, true><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
-        else
-            k_power_stream<true, false><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
-    } else {
-        if (normLap)
-            k_power_stream<false, true><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
-        else
-            k_power_stream<false, false><<<A.nblk_a, BLOCK, 0, c.stream>>>(A);
+    if (A.nblk_a == 0) {
+        // no entries: nothing to stream, but the iteration bookkeeping runs
+        if (A.mode == 0) k_power_stream<true, false><<<1, BLOCK, 0, c.stream>>>(A);
+        MI_HIP(hipGetLastError());
+        return;
     }
+    const bool unit = A.w == nullptr, r16 = A.r16 != 0;
+#define MI_STREAM(K, NB, NT, U, R) K<U, R><<<NB, NT, 0, c.stream>>>(A)
+    if (A.B.ncb) {
+        if (unit)
+            r16 ? MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, true, true)
+                : MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, true, false);
+        else
+            r16 ? MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, false, true)
+                : MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, false, false);
+        // (nblk_a = persistent workgroups of the LDS form)
+    } else {
+        if (unit)
+            r16 ? MI_STREAM(k_power_stream, A.nblk_a, BLOCK, true, true)
+                : MI_STREAM(k_power_stream, A.nblk_a, BLOCK, true, false);
+        else
+            r16 ? MI_STREAM(k_power_stream, A.nblk_a, BLOCK, false, true)
+                : MI_STREAM(k_power_stream, A.nblk_a, BLOCK, false, false);
+    }
+#undef MI_STREAM
     MI_HIP(hipGetLastError());
 }
 
-void launch_join(Ctx& c, int normLap, const StepArgs& A, uint32_t nblk_b)
+void launch_finish(Ctx& c, int normLap, const StepArgs& A)
 {
     if (normLap)
-        k_power_join<true><<<nblk_b, BLOCK, 0, c.stream>>>(A);
+        k_power_finish<true><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
     else
-        k_power_join<false><<<nblk_b, BLOCK, 0, c.stream>>>(A);
+        k_power_finish<false><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
     MI_HIP(hipGetLastError());
 }
 
-// One step = the streaming kernel then the joining kernel.
-void launch_step(Ctx& c, const DGraph& g, const LevelPlan& p, int normLap, const StepArgs& A)
+// One step = the streaming kernel then the finish kernel.
+void launch_step(Ctx& c, int normLap, const StepArgs& A)
 {
-    launch_stream(c, g, normLap, A);
-    launch_join(c, normLap, A, p.nblk_b);
+    launch_stream(c, A);
+    launch_finish(c, normLap, A);
 }
 
 }  // namespace
@@ -532,74 +856,136 @@ void launch_step(Ctx& c, const DGraph& g, const LevelPlan& p, int normLap, const
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
 {
     const uint32_t n = g.n, m = g.m;
-    // Entries per wavefront: two passes on large levels, one on small ones
-    // (more wavefronts to spread over the CUs).  Fixed by m alone, so the
-    // summation order does not depend on the device.
+    MI_CHECK(n > 0, ST_ARG, "empty level");
+    // LDS form when x fits in LDS in at most MAXCB column blocks; beyond one
+    // block only for dense levels, whose gathers dominate (the block layout
+    // costs a pass over the level).
+    const uint32_t need = (uint32_t)ceil_div(n, XB);
+    uint32_t ncb = 0;
+    if (m && need == 1)
+        ncb = 1;
+    else if (m && need <= (uint32_t)MAXCB && (uint64_t)m >= 64ull * n)
+        ncb = need;
+    p.ncb = ncb;
+    p.bw = ncb ? (uint32_t)ceil_div(n, ncb) : 0u;
+    for (int q = 0; q <= MAXCB; q++) p.boff[q] = p.bwave[q] = 0;
+    // Entries per range, fixed by the level alone (so the summation order does
+    // not depend on the device): two passes on large levels, one on small ones.
     p.pw = m >= (1u << 22) ? 2 * PASS : PASS;
-    p.nwaves = (uint32_t)ceil_div(m, p.pw);
-    p.nblk_a = (uint32_t)ceil_div(p.nwaves, NW);
-    MI_CHECK((uint64_t)p.nwaves * p.pw < (1ull << 32) + p.pw, ST_ARG, "level too large for the step plan");
-    // row of every entry
-    p.rid = DBuf<uint32_t>(&c.pool, m ? m : 1);
-    if (m) {
-        DBuf<uint32_t> roc;
-        build_row_index(c, g, roc);
-        k_entry_rows<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc.get(), g.R(), p.rid.get());
+    const uint32_t* rows = g.R();  // CSR the row ids and the finish kernel follow
+    uint32_t nrows = n;
+    if (ncb >= 2) {
+        // block-major layout: virtual row q n + r = the entries of row r in block q
+        const uint64_t nv = (uint64_t)ncb * n;
+        DBuf<uint32_t> vcnt(&c.pool, nv);
+        p.vrowptr = DBuf<uint32_t>(&c.pool, nv + 1);
+        p.col16 = DBuf<uint16_t>(&c.pool, m);
+        if (!g.unit) p.w_al = DBuf<uint32_t>(&c.pool, m);
+        const unsigned gw = (unsigned)ceil_div(n, NW);
+        k_vcount<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), p.bw, ncb, vcnt.get());
         MI_HIP(hipGetLastError());
-    }
-    // the streaming kernel reads col / w with 16-byte loads
-    const auto misaligned = [](const void* q) { return q && (reinterpret_cast<uintptr_t>(q) & 15u) != 0; };
-    if (m && misaligned(g.Cc())) {
-        p.col_al = DBuf<uint32_t>(&c.pool, m);
-        k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.Cc(), p.col_al.get());
+        scan_exclusive_u32(c, vcnt.get(), p.vrowptr.get(), nv, p.vrowptr.get() + nv);
+        k_vscatter<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), p.bw, ncb, p.vrowptr.get(),
+                                               p.col16.get(), g.unit ? nullptr : p.w_al.get());
         MI_HIP(hipGetLastError());
-    }
-    if (m && misaligned(g.W())) {
-        p.w_al = DBuf<uint32_t>(&c.pool, m);
-        k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.W(), p.w_al.get());
-        MI_HIP(hipGetLastError());
-    }
-    // empty rows
-    {
-        DBuf<uint32_t> flag(&c.pool, n), pos(&c.pool, n), tot(&c.pool, 1);
-        k_empty_flag<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), flag.get());
-        MI_HIP(hipGetLastError());
-        scan_exclusive_u32(c, flag.get(), pos.get(), n, tot.get());
-        uint32_t ne = 0;
-        read_words(c, tot.get(), &ne, 4);
-        p.n_empty = ne;
-        p.empty_rows = DBuf<uint32_t>(&c.pool, ne ? ne : 1);
-        if (ne) {
-            k_compact_empty<<<grid1(n), BLOCK, 0, c.stream>>>(n, flag.get(), pos.get(), p.empty_rows.get());
+        rows = p.vrowptr.get();
+        nrows = (uint32_t)nv;
+        std::vector<uint32_t> hv((size_t)nv + 1);
+        MI_HIP(hipMemcpyAsync(hv.data(), p.vrowptr.get(), (nv + 1) * 4, hipMemcpyDeviceToHost, c.stream));
+        MI_HIP(hipStreamSynchronize(c.stream));
+        for (uint32_t q = 0; q <= ncb; q++) p.boff[q] = hv[(size_t)q * n];
+        MI_CHECK(p.boff[ncb] == m, ST_INTERNAL, "column-block layout lost entries");
+    } else {
+        // the streaming kernel reads its arrays with 16-byte loads
+        const auto misaligned = [](const void* q) { return q && (reinterpret_cast<uintptr_t>(q) & 15u) != 0; };
+        if (ncb == 1 && m) {
+            p.col16 = DBuf<uint16_t>(&c.pool, m);
+            k_col16<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.Cc(), p.col16.get());
+            MI_HIP(hipGetLastError());
+        } else if (m && misaligned(g.Cc())) {
+            p.col_al = DBuf<uint32_t>(&c.pool, m);
+            k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.Cc(), p.col_al.get());
             MI_HIP(hipGetLastError());
         }
+        if (m && misaligned(g.W())) {
+            p.w_al = DBuf<uint32_t>(&c.pool, m);
+            k_copy_u32<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.W(), p.w_al.get());
+            MI_HIP(hipGetLastError());
+        }
+        p.boff[1] = m;
     }
+    // ranges and workgroups
+    if (ncb == 0) {
+        p.nwaves = (uint32_t)ceil_div(m, p.pw);
+        p.nblk_a = (uint32_t)ceil_div(p.nwaves, NW);
+        p.bwave[1] = p.nwaves;
+    } else {
+        for (uint32_t q = 0; q < ncb; q++) {
+            const uint32_t nw = (uint32_t)ceil_div(p.boff[q + 1] - p.boff[q], p.pw);
+            p.bwave[q + 1] = p.bwave[q] + nw;
+        }
+        p.nwaves = p.bwave[ncb];
+        // persistent: one workgroup per CU (LDS holds one), at most one per
+        // LNW ranges
+        const uint32_t ncu = c.num_cu > 0 ? (uint32_t)c.num_cu : 256u;
+        p.nblk_a = std::max<uint32_t>(1, std::min<uint32_t>(ncu, (uint32_t)ceil_div(p.nwaves, LNW)));
+    }
+    for (uint32_t q = std::max<uint32_t>(ncb, 1) + 1; q <= (uint32_t)MAXCB; q++) {
+        p.boff[q] = p.boff[q - 1];
+        p.bwave[q] = p.bwave[q - 1];
+    }
+    Blocks B;
+    B.ncb = ncb;
+    B.pw = p.pw;
+    B.m = m;
+    for (int q = 0; q <= MAXCB; q++) {
+        B.boff[q] = p.boff[q];
+        B.bwave[q] = p.bwave[q];
+    }
+    // row ids: per range the first row, per entry its offset from it (16 bits
+    // when every range fits, else the 32-bit rows)
+    p.wmeta = DBuf<uint32_t>(&c.pool, p.nwaves ? p.nwaves : 1);
+    if (m) {
+        DBuf<uint32_t> roc, rid32(&c.pool, m), maxd(&c.pool, 1);
+        DGraph rg;  // the (virtual) row structure, borrowed
+        rg.n = nrows;
+        rg.m = m;
+        rg.b_rowptr = rows;
+        build_row_index(c, rg, roc);
+        k_entry_rows<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc.get(), rows, rid32.get());
+        MI_HIP(hipGetLastError());
+        k_wmeta<<<grid1(p.nwaves), BLOCK, 0, c.stream>>>(B, p.nwaves, rid32.get(), p.wmeta.get());
+        MI_HIP(hipGetLastError());
+        p.rid16 = DBuf<uint16_t>(&c.pool, m);
+        MI_HIP(hipMemsetAsync(maxd.get(), 0, 4, c.stream));
+        k_rid16<<<red_grid(m), BLOCK, 0, c.stream>>>(B, rid32.get(), p.wmeta.get(), p.rid16.get(), maxd.get());
+        MI_HIP(hipGetLastError());
+        uint32_t mx = 0;
+        read_words(c, maxd.get(), &mx, 4);
+        if (mx > 0xffffu) {
+            p.rid16.reset();
+            p.rid = std::move(rid32);
+        }
+    }
+    p.part = DBuf<double>(&c.pool, (size_t)nrows);
     p.head = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     p.tail = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
-    p.bpart = DBuf<double>(&c.pool, 4ull * (p.nblk_a ? p.nblk_a : 1));
-    const uint64_t nthr = std::max<uint64_t>({1, p.nwaves, p.n_empty, p.nblk_a});
-    p.nblk_b = (uint32_t)ceil_div(nthr, BLOCK);
-    const uint32_t ngroups = (uint32_t)ceil_div(p.nblk_b, RED_GROUP);
-    p.partials = DBuf<double>(&c.pool, 4ull * p.nblk_b);
-    p.gpart = DBuf<double>(&c.pool, 4ull * ngroups);
-    p.gcnt = DBuf<uint32_t>(&c.pool, ngroups);
-    p.fcnt = DBuf<uint32_t>(&c.pool, 1);
-    MI_HIP(hipMemsetAsync(p.gcnt.get(), 0, (size_t)ngroups * 4, c.stream));
-    MI_HIP(hipMemsetAsync(p.fcnt.get(), 0, 4, c.stream));
+    p.nblk_f = (uint32_t)std::min<uint64_t>(MAXF, ceil_div(n, BLOCK));
+    p.fsum = DBuf<double>(&c.pool, 2ull * 4 * MAXF);
 }
 
 void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg, uint32_t* gb_dev)
 {
     MI_HIP(hipMemsetAsync(gb_dev, 0, 4, c.stream));
     if (g.unit) {
-        k_wdeg_finish<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
+        k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
     } else {
         DBuf<double> ones(&c.pool, g.n), y(&c.pool, g.n);
         k_fill_f64<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, ones.get(), 1.0);
         MI_HIP(hipGetLastError());
         StepArgs A = make_args(c, g, p, nullptr, nullptr, ones.get(), y.get(), 0.0, 2);
-        launch_step(c, g, p, 0, A);
-        k_wdeg_finish<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), y.get(), wdeg, gb_dev);
+        launch_step(c, 0, A);
+        k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), y.get(), wdeg, gb_dev);
     }
     MI_HIP(hipGetLastError());
     k_gb_floor<<<1, WAVE, 0, c.stream>>>(gb_dev);
@@ -610,7 +996,7 @@ void spmv_shift(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wde
                 const double* x, double* y)
 {
     StepArgs A = make_args(c, g, p, wdeg, gb_dev, x, y, 0.0, 1);
-    launch_step(c, g, p, 0, A);
+    launch_step(c, 0, A);
 }
 
 void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
@@ -625,7 +1011,7 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
     for (uint32_t i = 0; i < n; i++) h[i] = 1.0 / (1.0 + (i % 97));
     MI_HIP(hipMemcpyAsync(x.get(), h.data(), (size_t)n * 8, hipMemcpyHostToDevice, c.stream));
     StepArgs A = make_args(c, g, p, wdeg.get(), gb.get(), x.get(), y.get(), 0.0, 1);
-    launch_step(c, g, p, 0, A);  // warm-up
+    launch_step(c, 0, A);  // warm-up
     MI_HIP(hipStreamSynchronize(c.stream));
     hipEvent_t e0, e1, e2;
     MI_HIP(hipEventCreate(&e0));
@@ -633,9 +1019,9 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
     MI_HIP(hipEventCreate(&e2));
     // whole steps, then the streaming kernel alone, on the context stream
     MI_HIP(hipEventRecord(e0, c.stream));
-    for (int r = 0; r < reps; r++) launch_step(c, g, p, 0, A);
+    for (int r = 0; r < reps; r++) launch_step(c, 0, A);
     MI_HIP(hipEventRecord(e1, c.stream));
-    for (int r = 0; r < reps; r++) launch_stream(c, g, 0, A);
+    for (int r = 0; r < reps; r++) launch_stream(c, A);
     MI_HIP(hipEventRecord(e2, c.stream));
     MI_HIP(hipEventSynchronize(e2));
     float ms_step = 0, ms_stream = 0;
@@ -646,11 +1032,15 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
     MI_HIP(hipEventDestroy(e2));
     out->step_ms = (double)ms_step / reps;
     out->stream_ms = (double)ms_stream / reps;
-    // algorithmic bytes: col (+ w) + x (each entry once; repeated gathers are
-    // cache hits) + wdeg + y; the step adds rowptr
-    out->stream_bytes = 4.0 * g.m + (g.unit ? 0.0 : 4.0 * g.m) + 8.0 * n + 4.0 * n + 8.0 * n;
-    out->step_bytes = out->stream_bytes + 4.0 * (n + 1);
-    out->impl_bytes = out->stream_bytes + 4.0 * g.m;  // plus the per-entry row ids
+    out->ncb = p.ncb;
+    // algorithmic bytes of the streaming kernel: col (+ w) and x once (the
+    // repeated gathers are cache or LDS hits); the step adds rowptr, wdeg and y
+    out->stream_bytes = 4.0 * g.m + (g.unit ? 0.0 : 4.0 * g.m) + 8.0 * n;
+    out->step_bytes = out->stream_bytes + 4.0 * (n + 1) + 4.0 * n + 8.0 * n;
+    // what the streaming kernel moves: its column, weight and row-id arrays,
+    // x and the row sums
+    const double colb = p.ncb ? 2.0 : 4.0, ridb = p.rid16.get() ? 2.0 : 4.0;
+    out->impl_bytes = (colb + ridb + (g.unit ? 0.0 : 4.0)) * g.m + 8.0 * n + 8.0 * (p.ncb > 1 ? p.ncb : 1) * n;
 }
 
 void normalize(Ctx& c, double* d_u, uint32_t n)
@@ -666,6 +1056,7 @@ void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, d
     k_prolong<<<grid1(n_fine), BLOCK, 0, c.stream>>>(n_fine, vcmap, in, out);
     MI_HIP(hipGetLastError());
 }
+
 
 void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st)
 {
@@ -694,25 +1085,36 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     k_loop_init<<<1, WAVE, 0, c.stream>>>(c.scal, c.cnt, c.niter);
     MI_HIP(hipGetLastError());
 
+    // Launch i (1-based) reads y_{i-1} from buf[(i-1) & 1] and writes y_i to
+    // the other buffer; its streaming kernel first decides whether iteration
+    // i-1 was the last, so the loop ends one launch after the last iteration.
     double* buf[2] = {d_u, other.get()};
-    StepArgs A0 = make_args(c, g, plan, wdeg.get(), gb.get(), buf[0], buf[1], tol, 0);
-    StepArgs A1 = make_args(c, g, plan, wdeg.get(), gb.get(), buf[1], buf[0], tol, 0);
+    StepArgs A[2];
+    for (int k = 0; k < 2; k++) {
+        A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
+        A[k].fprev = plan.fsum.get() + 4 * MAXF * (k ^ 1);  // sums of the previous launch
+        A[k].fcur = plan.fsum.get() + 4 * MAXF * k;
+    }
     unsigned long long issued = 0;
     uint32_t chunk = 8;
     uint32_t flags[2] = {0, 0};
     for (;;) {
-        for (uint32_t k = 0; k < chunk; k++, issued++) launch_step(c, g, plan, normLap, (issued & 1) ? A1 : A0);
+        for (uint32_t k = 0; k < chunk; k++) {
+            issued++;
+            StepArgs Ai = A[(issued - 1) & 1];
+            Ai.iter = (uint32_t)std::min<unsigned long long>(issued, 0xffffffffull);
+            launch_step(c, normLap, Ai);
+        }
         read_words(c, c.cnt + CT_DONE, flags, 8);
         if (flags[0]) break;
         if (chunk < 256) chunk *= 2;
     }
     unsigned long long done_iters = 0;
     read_words(c, c.niter, &done_iters, 8);
+    MI_CHECK(done_iters >= 1, ST_INTERNAL, "power iteration ended without an iteration");
     // the reference returns u = input of the last iteration (eigensolve_kokkos.h:157)
-    if (done_iters >= 1) {
-        k_materialize<<<grid1(n), BLOCK, 0, c.stream>>>(n, buf[(done_iters - 1) & 1], d_u, c.scal);
-        MI_HIP(hipGetLastError());
-    }
+    k_materialize<<<grid1(n), BLOCK, 0, c.stream>>>(n, buf[(done_iters - 1) & 1], d_u, c.scal);
+    MI_HIP(hipGetLastError());
     if (st) {
         MI_HIP(hipStreamSynchronize(c.stream));
         st->iters = done_iters;

@@ -305,12 +305,16 @@ __global__ void k_map_assign2(uint32_t n, const uint32_t* __restrict__ pos, cons
 }
 
 // max over vcmap, so the host can verify every vertex got a cluster id < nc
-__global__ void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out)
+// (fixed grid, one atomic per workgroup)
+__global__ void __launch_bounds__(BLOCK) k_max_u32(const uint32_t* __restrict__ a, uint32_t n,
+                                                   uint32_t* __restrict__ out)
 {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t v = i < n ? a[i] : 0u;
-    v = wave_max_u32(v);
-    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, v);
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    uint32_t v = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+        v = a[i] > v ? a[i] : v;
+    v = block_max_u32(v, lds);
+    if (threadIdx.x == 0) atomicMax(out, v);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
@@ -479,7 +483,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
 
     MI_HIP(hipMemsetAsync(totals.get() + 3, 0, 4, c.stream));
-    k_max_u32<<<grid1(n), BLOCK, 0, c.stream>>>(vcmap, n, totals.get() + 3);
+    k_max_u32<<<red_grid(n), BLOCK, 0, c.stream>>>(vcmap, n, totals.get() + 3);
     MI_HIP(hipGetLastError());
     uint32_t tot[4] = {0, 0, 0, 0};
     read_words(c, totals.get(), tot, 16);

@@ -101,17 +101,22 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
 
 // ---------------------------------------------------------------- eigen
-// Per-level plan of the power-iteration step: the row of every entry, the
-// empty rows, the per-wavefront boundary partials and the reduction
-// workspace.  Wavefront w of the streaming kernel owns entries
-// [w pw, (w+1) pw).
+// Per-level plan of the power-iteration step.  ncb = 0: the streaming kernel
+// gathers x from global memory; ncb >= 1: it stages x in LDS in ncb column
+// blocks of bw vertices, and for ncb >= 2 the entries are laid out block by
+// block (vrowptr: virtual row q n + r = row r's entries in block q; col_al /
+// w_al hold that order).  Wavefront ranges of pw entries never cross a block:
+// block q holds entries [boff[q], boff[q+1]) and ranges from bwave[q].
 struct LevelPlan {
-    DBuf<uint32_t> rid, empty_rows, gcnt, fcnt;
-    DBuf<uint32_t> col_al, w_al;  // 16-byte aligned copies when the graph's arrays are not
-    DBuf<double> head, tail, bpart, partials, gpart;
-    uint32_t pw = 0, nwaves = 0, n_empty = 0;
+    DBuf<uint32_t> rid, wmeta, vrowptr;
+    DBuf<uint16_t> rid16, col16;  // 16-bit row offsets; 16-bit columns within a block
+    DBuf<uint32_t> col_al, w_al;  // block layout, or 16-byte aligned copies
+    DBuf<double> part, head, tail, fsum;
+    uint32_t ncb = 0, bw = 0;
+    uint32_t boff[9] = {}, bwave[9] = {};
+    uint32_t pw = 0, nwaves = 0;
     uint32_t nblk_a = 0;  // workgroups of the streaming kernel
-    uint32_t nblk_b = 0;  // workgroups of the joining kernel
+    uint32_t nblk_f = 0;  // workgroups of the finish kernel
 };
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
 
@@ -121,6 +126,7 @@ struct StepBench {
     double step_bytes = 0;    // algorithmic bytes of a step
     double stream_bytes = 0;  // algorithmic bytes of the streaming kernel
     double impl_bytes = 0;    // bytes the streaming kernel reads and writes, row ids included
+    uint32_t ncb = 0;         // column blocks of the level's plan (0: gathers from global memory)
 };
 // Times `reps` shifted-product steps on g (and `reps` launches of the
 // streaming kernel alone) with HIP events on the context stream.

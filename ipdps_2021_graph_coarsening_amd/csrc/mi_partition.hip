@@ -36,17 +36,21 @@ __global__ void k_assign(uint32_t n, const uint32_t* __restrict__ order, uint32_
     if (k < n) part[order[k]] = k < imbl ? 0u : 1u;
 }
 
-// one lane per directed edge: counts edges whose ends lie in different parts
+// Counts directed edges whose ends lie in different parts.  A fixed grid
+// walks the entries with a stride and each workgroup adds its count with one
+// atomic (integer sums: the order does not matter), so the counter sees at
+// most MAX_RED_BLOCKS atomics whatever m is.
 __global__ void __launch_bounds__(BLOCK) k_cut(uint64_t m, const uint32_t* __restrict__ rowptr,
                                                const uint32_t* __restrict__ roc, const uint32_t* __restrict__ col,
                                                const uint32_t* __restrict__ part,
                                                unsigned long long* __restrict__ cut)
 {
-    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ uint32_t lds[BLOCK / WAVE];
     uint32_t x = 0;
-    if (e < m) x = part[col[e]] != part[entry_row((uint32_t)e, roc, rowptr)] ? 1u : 0u;
-    x = wave_sum_u32(x);
-    if ((threadIdx.x & (WAVE - 1)) == 0 && x) atomicAdd(cut, (unsigned long long)x);
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < m; e += (uint64_t)gridDim.x * BLOCK)
+        x += part[col[e]] != part[entry_row((uint32_t)e, roc, rowptr)] ? 1u : 0u;
+    x = block_sum_u32(x, lds);
+    if (threadIdx.x == 0 && x) atomicAdd(cut, (unsigned long long)x);
 }
 
 // Sequential walk over sorted positions [imbl, imbr), sgpar.h:2041-2065.
@@ -107,7 +111,7 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     if (g.m) {
         DBuf<uint32_t> roc;
         build_row_index(c, g, roc);
-        k_cut<<<grid1(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), part, cut.get());
+        k_cut<<<red_grid(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), part, cut.get());
         MI_HIP(hipGetLastError());
     }
     k_walk<<<1, BLOCK, 0, c.stream>>>(imbl, imbr, order.get(), g.R(), g.Cc(), part, cut.get(), cut.get() + 1);

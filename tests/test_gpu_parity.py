@@ -181,6 +181,48 @@ def test_power_iter_hubs_vs_oracle_and_deterministic(ctx):
     assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
 
 
+def _dense_graph(n, avg_deg, seed, weighted):
+    """Random graph dense enough (m >= 64 n) for the column-block LDS layout,
+    with a few hubs so rows cross wavefront ranges and blocks unevenly."""
+    rng = np.random.default_rng(seed)
+    m = n * avg_deg // 2
+    u = np.concatenate([rng.integers(0, n, m), np.full(3 * n // 4, 7), np.full(n // 3, n - 2)])
+    v = np.concatenate([rng.integers(0, n, m), rng.integers(0, n, 3 * n // 4), rng.integers(0, n, n // 3)])
+    w = rng.integers(1, 9, len(u)) if weighted else None
+    from ipdps_2021_graph_coarsening_amd import graphs
+    return graphs.from_edges(n, u, v, w, lcc=True)
+
+
+@pytest.mark.parametrize("n,weighted", [(40000, True), (70000, False), (130000, True)])
+def test_spmv_shift_column_blocks(ctx, n, weighted):
+    rp, col, w = _dense_graph(n, 72, n, weighted)
+    nn = len(rp) - 1
+    assert rp[-1] >= 64 * nn
+    g = ctx.graph(rp, col, w)
+    assert g.bench_step(1)["column_blocks"] == -(-nn // 16384) >= 3  # the block layout is in use
+    x = np.random.default_rng(n).standard_normal(nn)
+    y = g.spmv_shift(x)
+    ww = np.ones(len(col)) if w is None else w
+    src = np.repeat(np.arange(nn), np.diff(rp))
+    wdeg = np.bincount(src, weights=ww, minlength=nn)
+    gb = max(int(2 * wdeg.max()), 2)
+    ref = (gb - wdeg) * x + np.bincount(src, weights=ww * x[col], minlength=nn)
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_power_iter_column_blocks_vs_oracle(ctx):
+    rp, col, w = _dense_graph(36000, 70, 11, True)
+    g = ctx.graph(rp, col, w)
+    assert g.bench_step(1)["column_blocks"] == 3
+    u0 = np.random.default_rng(12).random(len(rp) - 1)
+    u, it, _ = g.power_iter(u0, tol=1e-10)
+    u2, it2, _ = g.power_iter(u0, tol=1e-10)
+    assert it == it2 and np.array_equal(u, u2)  # fixed-order reductions
+    ou, oit, _, _ = oracle.power_iter(rp, col, w, u0, tol=1e-10)
+    assert abs(it - oit) <= max(2, ITER_SLACK * oit)
+    assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
+
+
 @pytest.mark.parametrize("normLap", [0, 1])
 def test_power_iter_vs_oracle(ctx, normLap):
     rng = np.random.default_rng(4 + normLap)
