@@ -66,6 +66,7 @@ constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
 constexpr int MAXCB = 8;                // column blocks at most
 constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most
+constexpr int W_UNIT = 0, W_U16 = 1, W_U32 = 2;  // weight formats of the streamed arrays
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
@@ -181,6 +182,22 @@ __global__ void k_col16(uint64_t m, const uint32_t* __restrict__ col, uint16_t* 
     if (i < m) out[i] = (uint16_t)col[i];
 }
 
+// 16-bit copy of the weights; *maxw receives the largest weight (the plan
+// keeps the 32-bit weights when it does not fit)
+__global__ void __launch_bounds__(BLOCK) k_w16(uint64_t m, const uint32_t* __restrict__ w, uint16_t* __restrict__ w16,
+                                               uint32_t* __restrict__ maxw)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    uint32_t mx = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < m; e += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t v = w[e];
+        w16[e] = (uint16_t)v;
+        mx = v > mx ? v : mx;
+    }
+    mx = block_max_u32(mx, lds);
+    if (threadIdx.x == 0) atomicMax(maxw, mx);
+}
+
 // range metadata: first row of range w, with HEADF when that row began in
 // an earlier range
 __global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ wmeta)
@@ -223,7 +240,8 @@ struct StepArgs {
     uint32_t iter;           // 1-based iteration index of this launch (mode 0)
     const uint32_t* rowptr;  // CSR of the (virtual) rows: block-major when ncb >= 2
     const void* col;         // u16 block offsets (LDS form) or u32 columns, 16-byte aligned
-    const uint32_t* w;       // 16-byte aligned, same order (null for unit weights)
+    const void* w;           // weights, same order: u16 or u32 (wfmt), null for unit weights
+    uint32_t wfmt;           // W_UNIT, W_U16 or W_U32
     const void* rid;         // u16 offsets from the range's first row, or u32 rows
     uint32_t r16;            // 1 when rid holds u16 offsets
     const uint32_t* wmeta;   // per range: first row | HEADF
@@ -316,19 +334,16 @@ struct Raw {
 
 // Issues the 16-byte loads of the 8 entries at e0 when they all lie below E1
 // (returns false otherwise: the caller loads that lane entry by entry).
-template <bool UNIT, bool LDSX, bool R16, class ColT, class RidT>
-__device__ __forceinline__ bool fetch(const StepArgs& A, const ColT* col, const RidT* rid, uint64_t e0, uint64_t E1,
-                                      Raw& R)
+template <int WF, bool LDSX, bool R16, class ColT, class RidT, class WT>
+__device__ __forceinline__ bool fetch(const ColT* col, const RidT* rid, const WT* w, uint64_t e0, uint64_t E1, Raw& R)
 {
     if (e0 + SEG > E1) return false;
     R.r0 = *reinterpret_cast<const uint4*>(rid + e0);
     if (!R16) R.r1 = *reinterpret_cast<const uint4*>(rid + e0 + 4);
     R.c0 = *reinterpret_cast<const uint4*>(col + e0);
     if (!LDSX) R.c1 = *reinterpret_cast<const uint4*>(col + e0 + 4);
-    if (!UNIT) {
-        R.w0 = *reinterpret_cast<const uint4*>(A.w + e0);
-        R.w1 = *reinterpret_cast<const uint4*>(A.w + e0 + 4);
-    }
+    if (WF != W_UNIT) R.w0 = *reinterpret_cast<const uint4*>(w + e0);
+    if (WF == W_U32) R.w1 = *reinterpret_cast<const uint4*>(w + e0 + 4);
     return true;
 }
 
@@ -344,7 +359,7 @@ __device__ __forceinline__ void unpack32(uint4 v0, uint4 v1, uint32_t (&o)[SEG])
     o[4] = v1.x; o[5] = v1.y; o[6] = v1.z; o[7] = v1.w;
 }
 
-template <bool LDSX, bool R16>
+template <int WF, bool LDSX, bool R16>
 __device__ __forceinline__ void unpack(const Raw& R, uint32_t (&rr)[SEG], uint32_t (&cc)[SEG], uint32_t (&ww)[SEG])
 {
     if (R16)
@@ -355,7 +370,8 @@ __device__ __forceinline__ void unpack(const Raw& R, uint32_t (&rr)[SEG], uint32
         unpack16(R.c0, cc);
     else
         unpack32(R.c0, R.c1, cc);
-    unpack32(R.w0, R.w1, ww);
+    if (WF == W_U16) unpack16(R.w0, ww);
+    if (WF == W_U32) unpack32(R.w0, R.w1, ww);
 }
 
 // row id (or offset) of a lane's first entry in a loaded pass
@@ -379,14 +395,16 @@ __device__ __forceinline__ void put(const StepArgs& A, uint32_t r, double v, uin
 // Row sums of the entries [E0, E1) by one wavefront (range index wv; nrow =
 // row of entry E1 when it continues this block, else RINF).  x_j is read as
 // (yprev[j] - a) s, or from LDS (already shifted and scaled) in the LDS form.
-template <bool UNIT, bool LDSX, bool R16>
+template <int WF, bool LDSX, bool R16>
 __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint64_t E0, uint64_t E1,
                                              uint32_t nrow, double a, double s, const double* xl)
 {
     using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
     using RidT = typename std::conditional<R16, uint16_t, uint32_t>::type;
+    using WT = typename std::conditional<WF == W_U16, uint16_t, uint32_t>::type;
     const ColT* col = static_cast<const ColT*>(A.col);
     const RidT* rid = static_cast<const RidT*>(A.rid);
+    const WT* w = static_cast<const WT*>(A.w);
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t meta = A.wmeta[wv];
     const uint32_t row0 = meta & ~HEADF;
@@ -396,12 +414,12 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
     double cs = 0.0;     // its partial sum so far
     // The 16-byte loads of pass p+1 are issued before pass p is processed.
     Raw nx;
-    bool nfull = fetch<UNIT, LDSX, R16>(A, col, rid, E0 + (uint64_t)lane * SEG, E1, nx);
+    bool nfull = fetch<WF, LDSX, R16>(col, rid, w, E0 + (uint64_t)lane * SEG, E1, nx);
     for (uint64_t base = E0; base < E1; base += PASS) {
         const uint64_t e0 = base + (uint64_t)lane * SEG;
         uint32_t rr[SEG], cc[SEG], ww[SEG];
         if (nfull) {
-            unpack<LDSX, R16>(nx, rr, cc, ww);
+            unpack<WF, LDSX, R16>(nx, rr, cc, ww);
 #pragma unroll
             for (int u = 0; u < SEG; u++) rr[u] += rbase;
         } else {
@@ -410,11 +428,11 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
                 const bool in = e0 + u < E1;
                 rr[u] = in ? (uint32_t)rid[e0 + u] + rbase : RINF;
                 cc[u] = in ? (uint32_t)col[e0 + u] : 0u;
-                if (!UNIT) ww[u] = in ? A.w[e0 + u] : 0u;
+                if (WF != W_UNIT) ww[u] = in ? (uint32_t)w[e0 + u] : 0u;
             }
         }
         const uint64_t nbase = base + PASS;
-        if (nbase < E1) nfull = fetch<UNIT, LDSX, R16>(A, col, rid, nbase + (uint64_t)lane * SEG, E1, nx);
+        if (nbase < E1) nfull = fetch<WF, LDSX, R16>(col, rid, w, nbase + (uint64_t)lane * SEG, E1, nx);
         // row of the entry after lane 63's last one: lane 0's first entry of
         // the next pass, or the next range's first row (in a partial pass
         // lane 63 holds no entries)
@@ -433,7 +451,7 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
                 xv = rr[u] != RINF ? xl[cc[u]] : 0.0;
             else
                 xv = rr[u] != RINF ? (A.yprev[cc[u]] - a) * s : 0.0;
-            pv[u] = UNIT ? xv : (double)ww[u] * xv;
+            pv[u] = WF == W_UNIT ? xv : (double)ww[u] * xv;
         }
         // lane-local segments
         const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
@@ -501,7 +519,7 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
     }
 }
 
-template <bool UNIT, bool R16>
+template <int WF, bool R16>
 __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
 {
     __shared__ double red[4][NW];
@@ -512,14 +530,14 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
     const uint64_t E0 = (uint64_t)wv * A.B.pw;
     const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
     const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-    stream_range<UNIT, false, R16>(A, wv, E0, E1, nrow, a, s, nullptr);
+    stream_range<WF, false, R16>(A, wv, E0, E1, nrow, a, s, nullptr);
 }
 
 // Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
 // [g W / G, (g+1) W / G) in order and stages the slice of x of each column
 // block its share touches.  Which workgroup streams a range does not change
 // what is summed, so the result does not depend on the grid.
-template <bool UNIT, bool R16>
+template <int WF, bool R16>
 __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 {
     __shared__ double xl[XB];
@@ -552,7 +570,7 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
             const uint64_t E0 = (uint64_t)off0 + (uint64_t)(wv - wbase) * A.B.pw;
             const uint64_t E1 = (E0 + A.B.pw < off1) ? E0 + A.B.pw : off1;
             const uint32_t nrow = (wv + 1 < wend) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-            stream_range<UNIT, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
+            stream_range<WF, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
         }
         cur = seg_end;
     }
@@ -771,7 +789,16 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
         A.col = p.col16.get();
     else
         A.col = p.col_al.get() ? p.col_al.get() : g.Cc();
-    A.w = g.unit ? nullptr : (p.w_al.get() ? p.w_al.get() : g.W());
+    if (g.unit) {
+        A.w = nullptr;
+        A.wfmt = W_UNIT;
+    } else if (p.w16.get()) {
+        A.w = p.w16.get();
+        A.wfmt = W_U16;
+    } else {
+        A.w = p.w_al.get() ? p.w_al.get() : g.W();
+        A.wfmt = W_U32;
+    }
     A.rid = p.rid16.get() ? static_cast<const void*>(p.rid16.get()) : static_cast<const void*>(p.rid.get());
     A.r16 = p.rid16.get() ? 1u : 0u;
     A.wmeta = p.wmeta.get();
@@ -809,27 +836,33 @@ void launch_stream(Ctx& c, const StepArgs& A)
 {
     if (A.nblk_a == 0) {
         // no entries: nothing to stream, but the iteration bookkeeping runs
-        if (A.mode == 0) k_power_stream<true, false><<<1, BLOCK, 0, c.stream>>>(A);
+        if (A.mode == 0) k_power_stream<W_UNIT, false><<<1, BLOCK, 0, c.stream>>>(A);
         MI_HIP(hipGetLastError());
         return;
     }
-    const bool unit = A.w == nullptr, r16 = A.r16 != 0;
-#define MI_STREAM(K, NB, NT, U, R) K<U, R><<<NB, NT, 0, c.stream>>>(A)
+    const bool r16 = A.r16 != 0;
+#define MI_STREAM(K, NT, WF)                                                   \
+    do {                                                                       \
+        if (r16)                                                               \
+            K<WF, true><<<A.nblk_a, NT, 0, c.stream>>>(A);                     \
+        else                                                                   \
+            K<WF, false><<<A.nblk_a, NT, 0, c.stream>>>(A);                    \
+    } while (0)
     if (A.B.ncb) {
-        if (unit)
-            r16 ? MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, true, true)
-                : MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, true, false);
-        else
-            r16 ? MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, false, true)
-                : MI_STREAM(k_power_stream_lds, A.nblk_a, LBLOCK, false, false);
         // (nblk_a = persistent workgroups of the LDS form)
-    } else {
-        if (unit)
-            r16 ? MI_STREAM(k_power_stream, A.nblk_a, BLOCK, true, true)
-                : MI_STREAM(k_power_stream, A.nblk_a, BLOCK, true, false);
+        if (A.wfmt == W_UNIT)
+            MI_STREAM(k_power_stream_lds, LBLOCK, W_UNIT);
+        else if (A.wfmt == W_U16)
+            MI_STREAM(k_power_stream_lds, LBLOCK, W_U16);
         else
-            r16 ? MI_STREAM(k_power_stream, A.nblk_a, BLOCK, false, true)
-                : MI_STREAM(k_power_stream, A.nblk_a, BLOCK, false, false);
+            MI_STREAM(k_power_stream_lds, LBLOCK, W_U32);
+    } else {
+        if (A.wfmt == W_UNIT)
+            MI_STREAM(k_power_stream, BLOCK, W_UNIT);
+        else if (A.wfmt == W_U16)
+            MI_STREAM(k_power_stream, BLOCK, W_U16);
+        else
+            MI_STREAM(k_power_stream, BLOCK, W_U32);
     }
 #undef MI_STREAM
     MI_HIP(hipGetLastError());
@@ -913,6 +946,21 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
             MI_HIP(hipGetLastError());
         }
         p.boff[1] = m;
+    }
+    // weights as 16 bits when they all fit (exact: they are integers)
+    if (!g.unit && m) {
+        const uint32_t* wsrc = p.w_al.get() ? p.w_al.get() : g.W();
+        DBuf<uint32_t> maxw(&c.pool, 1);
+        MI_HIP(hipMemsetAsync(maxw.get(), 0, 4, c.stream));
+        p.w16 = DBuf<uint16_t>(&c.pool, m);
+        k_w16<<<red_grid(m), BLOCK, 0, c.stream>>>(m, wsrc, p.w16.get(), maxw.get());
+        MI_HIP(hipGetLastError());
+        uint32_t mw = 0;
+        read_words(c, maxw.get(), &mw, 4);
+        if (mw > 0xffffu)
+            p.w16.reset();
+        else
+            p.w_al.reset();
     }
     // ranges and workgroups
     if (ncb == 0) {
@@ -1040,7 +1088,8 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
     // what the streaming kernel moves: its column, weight and row-id arrays,
     // x and the row sums
     const double colb = p.ncb ? 2.0 : 4.0, ridb = p.rid16.get() ? 2.0 : 4.0;
-    out->impl_bytes = (colb + ridb + (g.unit ? 0.0 : 4.0)) * g.m + 8.0 * n + 8.0 * (p.ncb > 1 ? p.ncb : 1) * n;
+    const double wb = g.unit ? 0.0 : (p.w16.get() ? 2.0 : 4.0);
+    out->impl_bytes = (colb + ridb + wb) * g.m + 8.0 * n + 8.0 * (p.ncb > 1 ? p.ncb : 1) * n;
 }
 
 void normalize(Ctx& c, double* d_u, uint32_t n)

@@ -110,6 +110,7 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
 struct LevelPlan {
     DBuf<uint32_t> rid, wmeta, vrowptr;
     DBuf<uint16_t> rid16, col16;  // 16-bit row offsets; 16-bit columns within a block
+    DBuf<uint16_t> w16;           // 16-bit weights when every weight fits
     DBuf<uint32_t> col_al, w_al;  // block layout, or 16-byte aligned copies
     DBuf<double> part, head, tail, fsum;
     uint32_t ncb = 0, bw = 0;

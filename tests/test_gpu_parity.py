@@ -124,9 +124,13 @@ def test_hierarchy_bitexact_rmat_weighted(ctx):
 
 
 # ------------------------------------------------------------ eigensolve
-def test_spmv_shift(ctx):
+@pytest.mark.parametrize("n,wscale", [(20000, 1), (20000, 40000), (9000, 1), (9000, 40000)])
+def test_spmv_shift(ctx, n, wscale):
+    # n = 20000: gathers from global memory; n = 9000: x staged in LDS whole.
+    # wscale 40000 puts weights above 65535 (32-bit weight stream).
     rng = np.random.default_rng(3)
-    rp, col, w = random_graph(rng, 20000, 0.0005, weighted=True)
+    rp, col, w = random_graph(rng, n, 10.0 / n, weighted=True)
+    w = w * wscale
     x = rng.standard_normal(len(rp) - 1)
     y = ctx.graph(rp, col, w).spmv_shift(x)
     src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
@@ -193,9 +197,12 @@ def _dense_graph(n, avg_deg, seed, weighted):
     return graphs.from_edges(n, u, v, w, lcc=True)
 
 
-@pytest.mark.parametrize("n,weighted", [(40000, True), (70000, False), (130000, True)])
-def test_spmv_shift_column_blocks(ctx, n, weighted):
+@pytest.mark.parametrize("n,weighted,wscale", [(40000, True, 1), (70000, False, 1), (130000, True, 1),
+                                               (40000, True, 9000)])
+def test_spmv_shift_column_blocks(ctx, n, weighted, wscale):
     rp, col, w = _dense_graph(n, 72, n, weighted)
+    if w is not None:
+        w = w * wscale
     nn = len(rp) - 1
     assert rp[-1] >= 64 * nn
     g = ctx.graph(rp, col, w)
