@@ -175,6 +175,56 @@ __global__ void __launch_bounds__(BLOCK) k_vscatter(uint32_t n, const uint32_t* 
     }
 }
 
+// Column-block layout for rows sorted by column (coarse levels always are):
+// the entries of (block q, row r) are then contiguous, so each entry finds
+// its place from where its (q, r) segment starts.  key = q n + r.
+__device__ __forceinline__ uint32_t vkey(uint32_t e, const uint32_t* __restrict__ roc,
+                                         const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                         uint32_t bw, uint32_t n)
+{
+    return (col[e] / bw) * n + entry_row(e, roc, rowptr);
+}
+
+// 1 when some row is not strictly increasing by column
+__global__ void k_unsorted(uint32_t m, const uint32_t* __restrict__ roc, const uint32_t* __restrict__ rowptr,
+                           const uint32_t* __restrict__ col, uint32_t* __restrict__ flag)
+{
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e == 0 || e >= m) return;
+    const uint32_t r = entry_row(e, roc, rowptr);
+    if (e > rowptr[r] && col[e] <= col[e - 1]) *flag = 1u;
+}
+
+// segment starts and counts (one atomic per run of a segment in a wavefront)
+__global__ void __launch_bounds__(BLOCK) k_vcount_sorted(uint32_t m, const uint32_t* __restrict__ roc,
+                                                         const uint32_t* __restrict__ rowptr,
+                                                         const uint32_t* __restrict__ col, uint32_t bw, uint32_t n,
+                                                         uint32_t* __restrict__ vcnt, uint32_t* __restrict__ vstart)
+{
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t key = e < m ? vkey(e, roc, rowptr, col, bw, n) : INF;
+    if (e < m && (e == 0 || vkey(e - 1, roc, rowptr, col, bw, n) != key)) vstart[key] = e;
+    auto plus = [](uint32_t x, uint32_t y) { return x + y; };
+    bool first;
+    const uint32_t c = wave_seg_reduce(key, e < m ? 1u : 0u, plus, &first);
+    if (first && key != INF) atomicAdd(&vcnt[key], c);
+}
+
+__global__ void k_vscatter_sorted(uint32_t m, const uint32_t* __restrict__ roc, const uint32_t* __restrict__ rowptr,
+                                  const uint32_t* __restrict__ col, const uint32_t* __restrict__ w, uint32_t bw,
+                                  uint32_t n, const uint32_t* __restrict__ vrowptr,
+                                  const uint32_t* __restrict__ vstart, uint16_t* __restrict__ vcol,
+                                  uint32_t* __restrict__ vw)
+{
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t c = col[e];
+    const uint32_t key = vkey(e, roc, rowptr, col, bw, n);
+    const uint32_t dst = vrowptr[key] + (e - vstart[key]);
+    vcol[dst] = (uint16_t)(c - (c / bw) * bw);
+    if (w) vw[dst] = w[e];
+}
+
 // one block (x fits in LDS whole): columns as 16-bit values
 __global__ void k_col16(uint64_t m, const uint32_t* __restrict__ col, uint16_t* __restrict__ out)
 {
@@ -598,6 +648,8 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     const uint32_t nb = A.B.ncb > 1 ? A.B.ncb : 1u;
     double S[4] = {0, 0, 0, 0};
     for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
+        const double yr = A.yprev[r];
+        const uint32_t wd = A.wdeg ? A.wdeg[r] : 0u;
         // all blocks' bounds, then all first partials, as independent loads
         uint32_t s0[MAXCB], s1[MAXCB], w0[MAXCB], w1[MAXCB];
         double first[MAXCB];
@@ -640,8 +692,8 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
                 v = q == 0 ? vb : v + vb;
             }
         }
-        const double xr = (A.yprev[r] - a) * s;
-        const double y = shifted<NORMLAP>(A, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
+        const double xr = (yr - a) * s;
+        const double y = shifted<NORMLAP>(A, v, xr, wd, gb);
         A.ynew[r] = y;
         S[0] += NORMLAP ? 0.0 : y;
         S[1] += y * y;
@@ -914,13 +966,35 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
         p.vrowptr = DBuf<uint32_t>(&c.pool, nv + 1);
         p.col16 = DBuf<uint16_t>(&c.pool, m);
         if (!g.unit) p.w_al = DBuf<uint32_t>(&c.pool, m);
-        const unsigned gw = (unsigned)ceil_div(n, NW);
-        k_vcount<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), p.bw, ncb, vcnt.get());
+        DBuf<uint32_t> roc0, flag(&c.pool, 1);
+        build_row_index(c, g, roc0);
+        MI_HIP(hipMemsetAsync(flag.get(), 0, 4, c.stream));
+        k_unsorted<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc0.get(), g.R(), g.Cc(), flag.get());
         MI_HIP(hipGetLastError());
-        scan_exclusive_u32(c, vcnt.get(), p.vrowptr.get(), nv, p.vrowptr.get() + nv);
-        k_vscatter<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), p.bw, ncb, p.vrowptr.get(),
-                                               p.col16.get(), g.unit ? nullptr : p.w_al.get());
-        MI_HIP(hipGetLastError());
+        uint32_t unsorted = 0;
+        read_words(c, flag.get(), &unsorted, 4);
+        if (!unsorted) {
+            // flat over the entries
+            DBuf<uint32_t> vstart(&c.pool, nv);
+            MI_HIP(hipMemsetAsync(vcnt.get(), 0, nv * 4, c.stream));
+            k_vcount_sorted<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc0.get(), g.R(), g.Cc(), p.bw, n, vcnt.get(),
+                                                              vstart.get());
+            MI_HIP(hipGetLastError());
+            scan_exclusive_u32(c, vcnt.get(), p.vrowptr.get(), nv, p.vrowptr.get() + nv);
+            k_vscatter_sorted<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc0.get(), g.R(), g.Cc(), g.W(), p.bw, n,
+                                                                p.vrowptr.get(), vstart.get(), p.col16.get(),
+                                                                g.unit ? nullptr : p.w_al.get());
+            MI_HIP(hipGetLastError());
+        } else {
+            // one wavefront per row, order kept by ballot ranking
+            const unsigned gw = (unsigned)ceil_div(n, NW);
+            k_vcount<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), p.bw, ncb, vcnt.get());
+            MI_HIP(hipGetLastError());
+            scan_exclusive_u32(c, vcnt.get(), p.vrowptr.get(), nv, p.vrowptr.get() + nv);
+            k_vscatter<<<gw, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), p.bw, ncb, p.vrowptr.get(),
+                                                   p.col16.get(), g.unit ? nullptr : p.w_al.get());
+            MI_HIP(hipGetLastError());
+        }
         rows = p.vrowptr.get();
         nrows = (uint32_t)nv;
         std::vector<uint32_t> hv((size_t)nv + 1);
@@ -1144,20 +1218,49 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         A[k].fprev = plan.fsum.get() + 4 * MAXF * (k ^ 1);  // sums of the previous launch
         A[k].fcur = plan.fsum.get() + 4 * MAXF * k;
     }
+    // Chunks of launches; each chunk ends with an asynchronous copy of the
+    // done flags into pinned memory.  The next chunk is queued before the
+    // host waits for the previous chunk's flags, so the device never drains;
+    // launches after the end are no-ops.
     unsigned long long issued = 0;
-    uint32_t chunk = 8;
-    uint32_t flags[2] = {0, 0};
-    for (;;) {
-        for (uint32_t k = 0; k < chunk; k++) {
+    auto issue = [&](uint32_t count) {
+        for (uint32_t k = 0; k < count; k++) {
             issued++;
             StepArgs Ai = A[(issued - 1) & 1];
             Ai.iter = (uint32_t)std::min<unsigned long long>(issued, 0xffffffffull);
             launch_step(c, normLap, Ai);
         }
-        read_words(c, c.cnt + CT_DONE, flags, 8);
+    };
+    uint32_t* slot[2] = {c.host_pinned + 48, c.host_pinned + 52};
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t* ev = evs.e;
+    MI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    MI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    auto mark = [&](int k) {
+        MI_HIP(hipMemcpyAsync(slot[k], c.cnt + CT_DONE, 8, hipMemcpyDeviceToHost, c.stream));
+        MI_HIP(hipEventRecord(ev[k], c.stream));
+    };
+    uint32_t flags[2] = {0, 0};
+    int k = 0;
+    issue(8);
+    mark(0);
+    for (uint32_t round = 0;; round++) {
+        issue(round < 2 ? 8 : 16);
+        mark(k ^ 1);
+        MI_HIP(hipEventSynchronize(ev[k]));
+        flags[0] = slot[k][0];
+        flags[1] = slot[k][1];
         if (flags[0]) break;
-        if (chunk < 256) chunk *= 2;
+        k ^= 1;
     }
+    MI_HIP(hipEventSynchronize(ev[k ^ 1]));
     unsigned long long done_iters = 0;
     read_words(c, c.niter, &done_iters, 8);
     MI_CHECK(done_iters >= 1, ST_INTERNAL, "power iteration ended without an iteration");

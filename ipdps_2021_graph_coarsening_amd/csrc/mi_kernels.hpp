@@ -80,8 +80,11 @@ __device__ __forceinline__ T wave_seg_reduce(uint32_t key, T v, Op op, bool* hea
 void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n,
                         uint32_t* total_dev);
 void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start);
+// Stable LSD radix sort of (key, value) pairs on key bits [begin_bit, end_bit).
 void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                       int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
+void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
+                      int end_bit, uint32_t* keys_tmp, uint32_t* vals_tmp);
 
 // ---------------------------------------------------------------- RNG / HEC
 // Host: state of the single generator of Kokkos' Random_XorShift64_Pool(seed)

@@ -98,15 +98,22 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down(const uint32_t* in, uint32_
 }
 
 // ---------------------------------------------------------------- radix
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, tiles of 4096.
+// Per pass: a per-tile digit histogram, one scan over (digit, tile), then
+// the scatter.  In the scatter each wavefront ranks its own contiguous
+// quarter of the tile with ballots (no barrier per round, stable by
+// construction), the tile is placed in LDS in digit order, and written out
+// so that each digit's run goes to consecutive addresses.
 constexpr int RADIX_BITS = 8;
 constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int R_ITEMS = 16;
 constexpr int R_TILE = BLOCK * R_ITEMS;  // 4096
+constexpr int R_WAVES = BLOCK / WAVE;
 static_assert(RADIX == BLOCK, "one digit per thread in the tile bookkeeping");
 
-__global__ void __launch_bounds__(BLOCK) k_radix_hist(const uint64_t* __restrict__ keys, uint64_t n,
-                                                      int shift, uint32_t mask, uint32_t num_tiles,
-                                                      uint32_t* __restrict__ hist)
+template <class K>
+__global__ void __launch_bounds__(BLOCK) k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
+                                                      uint32_t mask, uint32_t num_tiles, uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[RADIX];
     h[threadIdx.x] = 0;
@@ -124,61 +131,93 @@ __global__ void __launch_bounds__(BLOCK) k_radix_hist(const uint64_t* __restrict
     hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_radix_scatter(const uint64_t* __restrict__ keys_in,
+template <class K>
+__global__ void __launch_bounds__(BLOCK) k_radix_scatter(const K* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
-                                                         uint64_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ vals_out, uint64_t n,
-                                                         int shift, uint32_t mask, uint32_t num_tiles,
+                                                         K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                         uint64_t n, int shift, uint32_t mask, uint32_t num_tiles,
                                                          const uint32_t* __restrict__ offs)
 {
-    __shared__ uint32_t gofs[RADIX];
-    __shared__ uint32_t run[RADIX];
-    __shared__ uint32_t wcnt[BLOCK / WAVE][RADIX];
+    __shared__ K skey[R_TILE];
+    __shared__ uint32_t sval[R_TILE];
+    __shared__ uint32_t wrun[R_WAVES][RADIX];  // per wavefront: items of each digit so far
+    __shared__ uint32_t lstart[RADIX];         // first LDS slot of each digit
+    __shared__ uint32_t gofs[RADIX];           // first output position of each digit
+    __shared__ uint32_t wsum[R_WAVES];
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     gofs[threadIdx.x] = offs[(uint64_t)threadIdx.x * num_tiles + blockIdx.x];
-    run[threadIdx.x] = 0;
 #pragma unroll
-    for (int w = 0; w < BLOCK / WAVE; w++) wcnt[w][threadIdx.x] = 0;
+    for (int w = 0; w < R_WAVES; w++) wrun[w][threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * R_TILE;
+    const uint64_t wbase = base + (uint64_t)wid * (R_TILE / R_WAVES);  // this wavefront's quarter
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (WAVE - lane));
+    K kk[R_ITEMS];
+    uint32_t vv[R_ITEMS], rk[R_ITEMS], dd[R_ITEMS];
+#pragma unroll
     for (int r = 0; r < R_ITEMS; r++) {
-        uint64_t i = base + (uint64_t)r * BLOCK + threadIdx.x;
+        const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
         const bool active = i < n;
-        uint64_t key = 0;
-        uint32_t val = 0, d = 0;
-        if (active) {
-            key = keys_in[i];
-            val = vals_in[i];
-            d = (uint32_t)(key >> shift) & mask;
-        }
+        kk[r] = active ? keys_in[i] : (K)0;
+        vv[r] = active ? vals_in[i] : 0u;
+        dd[r] = active ? ((uint32_t)(kk[r] >> shift) & mask) : (uint32_t)RADIX;
+    }
+#pragma unroll
+    for (int r = 0; r < R_ITEMS; r++) {
+        const uint32_t d = dd[r];
+        const bool active = d < (uint32_t)RADIX;
         unsigned long long peers = __ballot(active);
 #pragma unroll
         for (int b = 0; b < RADIX_BITS; b++) {
             const bool bit = (d >> b) & 1u;
-            unsigned long long bb = __ballot(bit);
+            const unsigned long long bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (active && rank == 0) wcnt[wid][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (active) {
-            uint32_t p = gofs[d] + run[d] + rank;
-            for (int w = 0; w < wid; w++) p += wcnt[w][d];
-            keys_out[p] = key;
-            vals_out[p] = val;
-        }
-        __syncthreads();
-        {
-            uint32_t add = 0;
+        // rank within this wavefront's items of digit d so far
+        const uint32_t prior = active ? wrun[wid][d] : 0u;
+        rk[r] = prior + (uint32_t)__popcll(peers & lt_mask);
+        if (active && (peers & lt_mask) == 0ull) wrun[wid][d] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // digit d: the count over the tile, and each wavefront's offset inside it
+    uint32_t tot = 0;
 #pragma unroll
-            for (int w = 0; w < BLOCK / WAVE; w++) {
-                add += wcnt[w][threadIdx.x];
-                wcnt[w][threadIdx.x] = 0;
-            }
-            run[threadIdx.x] += add;
+    for (int w = 0; w < R_WAVES; w++) {
+        const uint32_t c = wrun[w][threadIdx.x];
+        wrun[w][threadIdx.x] = tot;
+        tot += c;
+    }
+    // exclusive scan of the tile's digit counts (one digit per thread)
+    uint32_t incl = tot;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wid; w++) before += wsum[w];
+    lstart[threadIdx.x] = before + incl - tot;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R_ITEMS; r++) {
+        const uint32_t d = dd[r];
+        if (d < (uint32_t)RADIX) {
+            const uint32_t p = lstart[d] + wrun[wid][d] + rk[r];
+            skey[p] = kk[r];
+            sval[p] = vv[r];
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    const uint32_t cnt = (n - base) < (uint64_t)R_TILE ? (uint32_t)(n - base) : (uint32_t)R_TILE;
+    for (uint32_t p = threadIdx.x; p < cnt; p += BLOCK) {
+        const K key = skey[p];
+        const uint32_t d = (uint32_t)(key >> shift) & mask;
+        const uint32_t gp = gofs[d] + (p - lstart[d]);
+        keys_out[gp] = key;
+        vals_out[gp] = sval[p];
     }
 }
 
@@ -238,35 +277,47 @@ void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start)
     MI_HIP(hipGetLastError());
 }
 
-void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
-                      int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp)
+template <class K>
+static void radix_sort_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit, K* keys_tmp,
+                            uint32_t* vals_tmp)
 {
-    MI_CHECK(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, ST_ARG, "radix bit range");
+    MI_CHECK(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit, ST_ARG, "radix bit range");
     if (n <= 1 || begin_bit == end_bit) return;
     const uint64_t tiles = ceil_div(n, R_TILE);
     MI_CHECK(tiles * RADIX < (1ull << 32), ST_ARG, "radix sort too large");
     DBuf<uint32_t> hist(&c.pool, tiles * RADIX);
-    uint64_t* ksrc = keys;
+    K* ksrc = keys;
     uint32_t* vsrc = vals;
-    uint64_t* kdst = keys_tmp;
+    K* kdst = keys_tmp;
     uint32_t* vdst = vals_tmp;
     for (int shift = begin_bit; shift < end_bit; shift += RADIX_BITS) {
         const int bits = (end_bit - shift) < RADIX_BITS ? (end_bit - shift) : RADIX_BITS;
         const uint32_t mask = (1u << bits) - 1u;
-        k_radix_hist<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles,
-                                                             hist.get());
+        k_radix_hist<K><<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles, hist.get());
         MI_HIP(hipGetLastError());
         scan_exclusive_u32(c, hist.get(), hist.get(), tiles * RADIX, nullptr);
-        k_radix_scatter<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
-                                                                (uint32_t)tiles, hist.get());
+        k_radix_scatter<K><<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
+                                                                   (uint32_t)tiles, hist.get());
         MI_HIP(hipGetLastError());
         std::swap(ksrc, kdst);
         std::swap(vsrc, vdst);
     }
     if (ksrc != keys) {
-        MI_HIP(hipMemcpyAsync(keys, ksrc, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, c.stream));
+        MI_HIP(hipMemcpyAsync(keys, ksrc, n * sizeof(K), hipMemcpyDeviceToDevice, c.stream));
         MI_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
     }
+}
+
+void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
+                      uint64_t* keys_tmp, uint32_t* vals_tmp)
+{
+    radix_sort_impl<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
+                      uint32_t* keys_tmp, uint32_t* vals_tmp)
+{
+    radix_sort_impl<uint32_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
 }
 
 }  // namespace mi

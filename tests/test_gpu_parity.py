@@ -197,12 +197,22 @@ def _dense_graph(n, avg_deg, seed, weighted):
     return graphs.from_edges(n, u, v, w, lcc=True)
 
 
-@pytest.mark.parametrize("n,weighted,wscale", [(40000, True, 1), (70000, False, 1), (130000, True, 1),
-                                               (40000, True, 9000)])
-def test_spmv_shift_column_blocks(ctx, n, weighted, wscale):
+@pytest.mark.parametrize("n,weighted,wscale,shuffle", [(40000, True, 1, False), (70000, False, 1, False),
+                                                       (130000, True, 1, False), (40000, True, 9000, False),
+                                                       (40000, True, 1, True)])
+def test_spmv_shift_column_blocks(ctx, n, weighted, wscale, shuffle):
     rp, col, w = _dense_graph(n, 72, n, weighted)
     if w is not None:
         w = w * wscale
+    if shuffle:
+        # columns out of order inside rows: the layout's per-row path
+        rng = np.random.default_rng(1)
+        col = col.copy()
+        w = w.copy()
+        for r in range(0, len(rp) - 1, 3):
+            p = rng.permutation(rp[r + 1] - rp[r]) + rp[r]
+            col[rp[r]:rp[r + 1]] = col[p]
+            w[rp[r]:rp[r + 1]] = w[p]
     nn = len(rp) - 1
     assert rp[-1] >= 64 * nn
     g = ctx.graph(rp, col, w)
