@@ -79,6 +79,7 @@ _SIGS = [
     ("mi_ctx_destroy", None, [vp]),
     ("mi_last_error", C.c_char_p, [vp]),
     ("mi_ctx_trim", C.c_int, [vp]),
+    ("mi_ctx_set_graphs", C.c_int, [vp, C.c_int]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -167,6 +168,10 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_graphs(self, enable: bool) -> None:
+        """Replay power iterations from a captured HIP graph (default) or launch eagerly."""
+        self.check(self._L.mi_ctx_set_graphs(self.h, 1 if enable else 0), "mi_ctx_set_graphs")
 
     # ------------------------------------------------------------ graphs
     def graph(self, rowptr, col, w=None) -> "Graph":

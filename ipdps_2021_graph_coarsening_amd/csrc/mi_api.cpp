@@ -94,6 +94,7 @@ Ctx::~Ctx()
     if (niter) (void)hipFree(niter);
     if (xs_jump) (void)hipFree(xs_jump);
     if (host_pinned) (void)hipHostFree(host_pinned);
+    if (host_mapped) (void)hipHostFree(host_mapped);
     pool.trim();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -117,14 +118,29 @@ void ctx_init(Ctx& c, int device)
     MI_HIP(hipMemset(c.cnt, 0, sizeof(uint32_t) * CT_COUNT));
     MI_HIP(hipMemset(c.niter, 0, sizeof(unsigned long long)));
     MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_pinned), 64 * sizeof(uint32_t), hipHostMallocDefault));
+    MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_mapped), 128 * sizeof(uint32_t),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c.host_mapped, 0, 128 * sizeof(uint32_t));
+    MI_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.host_mapped_dev), c.host_mapped, 0));
 }
+
+namespace {
+__global__ void k_read_words(const uint32_t* __restrict__ src, uint32_t* dst, uint32_t nwords)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < nwords) __hip_atomic_store(&dst[i], src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+}  // namespace
 
 void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes)
 {
-    MI_CHECK(bytes <= 64 * sizeof(uint32_t), ST_INTERNAL, "read_words too large");
-    MI_HIP(hipMemcpyAsync(c.host_pinned, dsrc, bytes, hipMemcpyDeviceToHost, c.stream));
+    MI_CHECK(bytes <= 64 * sizeof(uint32_t) && bytes % 4 == 0 && (reinterpret_cast<uintptr_t>(dsrc) & 3u) == 0,
+             ST_INTERNAL, "read_words: at most 64 aligned words");
+    k_read_words<<<1, 64, 0, c.stream>>>(static_cast<const uint32_t*>(dsrc), c.host_mapped_dev,
+                                         (uint32_t)(bytes / 4));
+    MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(c.stream));
-    std::memcpy(hdst, c.host_pinned, bytes);
+    std::memcpy(hdst, c.host_mapped, bytes);
 }
 
 }  // namespace mi
@@ -331,6 +347,13 @@ int mi_ctx_trim(mi_ctx* ctx)
 {
     if (!ctx) return ST_ARG;
     return guard(ctx, [&] { ctx->c.pool.trim(); });
+}
+
+int mi_ctx_set_graphs(mi_ctx* ctx, int enable)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.use_graphs = enable != 0;
+    return ST_OK;
 }
 
 int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint32_t* w,

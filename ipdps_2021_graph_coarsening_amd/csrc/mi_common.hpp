@@ -136,6 +136,7 @@ enum Counter : int {
     CT_DONE = 2,     // power iteration finished flag
     CT_MAXR = 3,     // iteration cap reached
     CT_ERR = 4,      // a kernel found a violated size invariant
+    CT_ITER = 5,     // power iteration: launches whose finish kernel has run
     CT_COUNT = 16
 };
 
@@ -151,13 +152,19 @@ struct Ctx {
     uint32_t* cnt = nullptr;        // CT_COUNT
     unsigned long long* niter = nullptr;  // iteration counter of the power iteration
     uint32_t* host_pinned = nullptr;      // small pinned staging area (64 words)
+    // mapped, coherent host memory the device writes directly: words 0..63
+    // for read_words, word 64 the power iteration's done flag
+    uint32_t* host_mapped = nullptr;
+    uint32_t* host_mapped_dev = nullptr;  // its device address
     uint64_t* xs_jump = nullptr;          // T^(2^k) of the xorshift transition, 48 x 64 words
+    bool use_graphs = true;               // replay power iterations from a captured HIP graph
     ~Ctx();
 };
 
 void ctx_init(Ctx& c, int device);
 
-// Copy a few words device -> host through pinned staging and wait.
+// Copy a few words device -> host and wait: a one-wavefront kernel stores
+// them into mapped host memory (cheaper in the stream than a copy operation).
 void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes);
 
 // ------------------------------------------------------------------------

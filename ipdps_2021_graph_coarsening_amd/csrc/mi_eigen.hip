@@ -287,7 +287,6 @@ __global__ void __launch_bounds__(BLOCK) k_rid16(Blocks B, const uint32_t* __res
 struct StepArgs {
     uint32_t n;
     uint32_t m;
-    uint32_t iter;           // 1-based iteration index of this launch (mode 0)
     const uint32_t* rowptr;  // CSR of the (virtual) rows: block-major when ncb >= 2
     const void* col;         // u16 block offsets (LDS form) or u32 columns, 16-byte aligned
     const void* w;           // weights, same order: u16 or u32 (wfmt), null for unit weights
@@ -316,6 +315,7 @@ struct StepArgs {
     uint32_t* cnt;
     double* scal;
     unsigned long long* niter;
+    uint32_t* hdone;         // done flag in mapped host memory (mode 0 loop), or null
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -330,7 +330,8 @@ __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, do
     if (A.mode != 0) return true;
     if (A.cnt[CT_DONE]) return false;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (A.iter < 2) {
+    const uint32_t iter = A.cnt[CT_ITER] + 1u;  // 1-based index of this launch
+    if (iter < 2) {
         if (lead) {
             A.scal[SC_A] = 0.0;
             A.scal[SC_S] = 1.0;
@@ -358,11 +359,11 @@ __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, do
     const double nrm2 = T[1] - 2.0 * an * T[0] + an * an * (double)A.n;
     const double sn = 1 / sqrt(nrm2);
     const double dot = (T[2] - an * T[3]) * sn;
-    const unsigned long long itn = A.iter - 1;                 // iterations done
-    const double lastd = A.scal[SC_DOT + (A.iter & 1u)];        // dot of iteration iter - 2
+    const unsigned long long itn = iter - 1;                  // iterations done
+    const double lastd = A.scal[SC_DOT + (iter & 1u)];         // dot of iteration iter - 2
     const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
     if (lead) {
-        A.scal[SC_DOT + ((A.iter - 1) & 1u)] = dot;
+        A.scal[SC_DOT + ((iter - 1) & 1u)] = dot;
         if (go_on) {
             A.scal[SC_A] = an;
             A.scal[SC_S] = sn;
@@ -370,6 +371,7 @@ __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, do
             A.cnt[CT_DONE] = 1u;
             A.cnt[CT_MAXR] = itn >= A.iter_max ? 1u : 0u;
             *A.niter = itn;
+            if (A.hdone) __hip_atomic_store(A.hdone, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     a = an;
@@ -592,12 +594,33 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 {
     __shared__ double xl[XB];
     __shared__ double red[4][LNW];
-    double a, s;
-    if (!iteration_start<LNW>(A, a, s, red)) return;
+    constexpr int XPT = XB / LBLOCK;  // staged values per thread
+    if (A.mode == 0 && A.cnt[CT_DONE]) return;
     const uint32_t G = gridDim.x, g = logical_block(), wid = threadIdx.x / WAVE;
     const uint32_t R1 = (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
     uint32_t cur = (uint32_t)((uint64_t)g * A.nwaves / G);
-    while (cur < R1) {
+    // The slice of y for the first block is loaded before the previous
+    // iteration's sums are reduced, so the two loads overlap.
+    double raw[XPT];
+    {
+        uint32_t b = 0;
+#pragma unroll
+        for (int q = 1; q < MAXCB; q++)
+            if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) b = q;
+        const uint32_t x0 = b * A.bw;
+        const uint32_t xn = cur < R1 ? ((A.n - x0) < A.bw ? (A.n - x0) : A.bw) : 0u;
+#pragma unroll
+        for (int j = 0; j < XPT; j++) {
+            const uint32_t i = threadIdx.x + (uint32_t)j * LBLOCK;
+            raw[j] = i < xn ? A.yprev[x0 + i] : 0.0;
+        }
+    }
+    double a, s;
+    if (!iteration_start<LNW>(A, a, s, red)) return;
+    // one segment of ranges inside one column block; the first one stages
+    // from the values loaded above (their registers end there), later ones
+    // load again
+    auto segment = [&](bool from_raw) {
         // the column block holding range cur (static indices into the argument arrays)
         uint32_t b = 0, off0 = A.B.boff[0], off1 = A.B.boff[1], wbase = A.B.bwave[0], wend = A.B.bwave[1];
 #pragma unroll
@@ -613,8 +636,16 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
         const uint32_t seg_end = R1 < wend ? R1 : wend;
         const uint32_t x0 = b * A.bw;
         const uint32_t xn = (A.n - x0) < A.bw ? (A.n - x0) : A.bw;
-        __syncthreads();  // the previous block's gathers are done
-        for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = (A.yprev[x0 + i] - a) * s;
+        if (from_raw) {
+#pragma unroll
+            for (int j = 0; j < XPT; j++) {
+                const uint32_t i = threadIdx.x + (uint32_t)j * LBLOCK;
+                if (i < xn) xl[i] = (raw[j] - a) * s;
+            }
+        } else {
+            __syncthreads();  // the previous block's gathers are done
+            for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = (A.yprev[x0 + i] - a) * s;
+        }
         __syncthreads();
         for (uint32_t wv = cur + wid; wv < seg_end; wv += LNW) {
             const uint64_t E0 = (uint64_t)off0 + (uint64_t)(wv - wbase) * A.B.pw;
@@ -623,7 +654,9 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
             stream_range<WF, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
         }
         cur = seg_end;
-    }
+    };
+    if (cur < R1) segment(true);
+    while (cur < R1) segment(false);
 }
 
 // y_r from the row sum (the shifted product of the reference, or A x in mode 2)
@@ -706,6 +739,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 4; k++) A.fcur[(size_t)blockIdx.x * 4 + k] = tot[k];
+        if (blockIdx.x == 0) A.cnt[CT_ITER] += 1u;  // read by the next launch only
     }
 }
 
@@ -822,6 +856,7 @@ __global__ void k_loop_init(double* scal, uint32_t* cnt, unsigned long long* nit
         scal[SC_S] = 1.0;
         cnt[CT_DONE] = 0u;
         cnt[CT_MAXR] = 0u;
+        cnt[CT_ITER] = 0u;
         *niter = 0ull;
     }
 }
@@ -835,7 +870,6 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     StepArgs A;
     A.n = g.n;
     A.m = g.m;
-    A.iter = 0;
     A.rowptr = p.vrowptr.get() ? p.vrowptr.get() : g.R();
     if (p.ncb)
         A.col = p.col16.get();
@@ -881,6 +915,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.cnt = c.cnt;
     A.scal = c.scal;
     A.niter = c.niter;
+    A.hdone = nullptr;
     return A;
 }
 
@@ -1213,25 +1248,52 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     // i-1 was the last, so the loop ends one launch after the last iteration.
     double* buf[2] = {d_u, other.get()};
     StepArgs A[2];
+    c.host_mapped[64] = 0u;  // no kernel of this level has run yet
     for (int k = 0; k < 2; k++) {
         A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
+        A[k].hdone = c.host_mapped_dev + 64;
         A[k].fprev = plan.fsum.get() + 4 * MAXF * (k ^ 1);  // sums of the previous launch
         A[k].fcur = plan.fsum.get() + 4 * MAXF * k;
     }
-    // Chunks of launches; each chunk ends with an asynchronous copy of the
-    // done flags into pinned memory.  The next chunk is queued before the
-    // host waits for the previous chunk's flags, so the device never drains;
-    // launches after the end are no-ops.
-    unsigned long long issued = 0;
-    auto issue = [&](uint32_t count) {
-        for (uint32_t k = 0; k < count; k++) {
-            issued++;
-            StepArgs Ai = A[(issued - 1) & 1];
-            Ai.iter = (uint32_t)std::min<unsigned long long>(issued, 0xffffffffull);
-            launch_step(c, normLap, Ai);
+    // Every launch of this level has the same arguments but for the ping-pong
+    // parity (the iteration index lives on the device), so a chunk of
+    // GRAPH_ITERS iterations is captured once as a HIP graph (kernel nodes
+    // only) and replayed; without a graph the launches are issued one by one.
+    constexpr uint32_t GRAPH_ITERS = 8;
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        ~Graph()
+        {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
         }
+    } graph;
+    if (c.use_graphs && hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        bool ok = true;
+        try {
+            for (uint32_t k = 0; k < GRAPH_ITERS; k++) launch_step(c, normLap, A[k & 1]);
+        } catch (const Error&) {
+            ok = false;
+        }
+        const hipError_t e = hipStreamEndCapture(c.stream, &graph.g);
+        if (!ok || e != hipSuccess || !graph.g || hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0) != hipSuccess)
+            graph.x = nullptr;
+        (void)hipGetLastError();
+    }
+    unsigned long long issued = 0;
+    auto issue = [&](uint32_t count) {  // count: a multiple of GRAPH_ITERS, issued even
+        if (graph.x) {
+            for (uint32_t k = 0; k < count; k += GRAPH_ITERS) MI_HIP(hipGraphLaunch(graph.x, c.stream));
+            issued += count;
+            return;
+        }
+        for (uint32_t k = 0; k < count; k++, issued++) launch_step(c, normLap, A[issued & 1]);
     };
-    uint32_t* slot[2] = {c.host_pinned + 48, c.host_pinned + 52};
+    // Chunks of launches, one queued ahead; after each chunk the host reads
+    // the done flag the streaming kernel writes into mapped host memory (no
+    // copy operation in the stream).  Launches after the end are no-ops.
+    volatile uint32_t* hdone = c.host_mapped + 64;
     struct Events {
         hipEvent_t e[2] = {nullptr, nullptr};
         ~Events()
@@ -1243,24 +1305,27 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     hipEvent_t* ev = evs.e;
     MI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     MI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    auto mark = [&](int k) {
-        MI_HIP(hipMemcpyAsync(slot[k], c.cnt + CT_DONE, 8, hipMemcpyDeviceToHost, c.stream));
-        MI_HIP(hipEventRecord(ev[k], c.stream));
-    };
-    uint32_t flags[2] = {0, 0};
+    const unsigned long long cap = A[0].iter_max + 4ull * GRAPH_ITERS;  // the device stops at iter_max
     int k = 0;
-    issue(8);
-    mark(0);
-    for (uint32_t round = 0;; round++) {
-        issue(round < 2 ? 8 : 16);
-        mark(k ^ 1);
-        MI_HIP(hipEventSynchronize(ev[k]));
-        flags[0] = slot[k][0];
-        flags[1] = slot[k][1];
-        if (flags[0]) break;
+    issue(GRAPH_ITERS);
+    MI_HIP(hipEventRecord(ev[0], c.stream));
+    for (;;) {
+        if (!*hdone && issued < cap) {
+            issue(GRAPH_ITERS);
+            MI_HIP(hipEventRecord(ev[k ^ 1], c.stream));
+        }
+        // wait for the older chunk (or the flag)
+        hipError_t q;
+        while ((q = hipEventQuery(ev[k])) == hipErrorNotReady && !*hdone) {
+        }
+        if (q != hipSuccess && q != hipErrorNotReady) MI_HIP(q);
+        if (*hdone) break;
+        MI_CHECK(issued < cap, ST_INTERNAL, "power iteration did not stop at its iteration cap");
         k ^= 1;
     }
-    MI_HIP(hipEventSynchronize(ev[k ^ 1]));
+    MI_HIP(hipStreamSynchronize(c.stream));
+    uint32_t flags[2] = {0, 0};
+    read_words(c, c.cnt + CT_DONE, flags, 8);
     unsigned long long done_iters = 0;
     read_words(c, c.niter, &done_iters, 8);
     MI_CHECK(done_iters >= 1, ST_INTERNAL, "power iteration ended without an iteration");

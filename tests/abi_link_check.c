@@ -18,6 +18,7 @@ int main(void)
     void (*fp_mi_ctx_destroy)(mi_ctx*);
     const char* (*fp_mi_last_error)(const mi_ctx*);
     int (*fp_mi_ctx_trim)(mi_ctx*);
+    int (*fp_mi_ctx_set_graphs)(mi_ctx*, int);
     int (*fp_mi_graph_create)(mi_ctx*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, mi_graph**);
     int (*fp_mi_graph_create_device)(mi_ctx*, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
                                      const uint32_t*, mi_graph**);
@@ -50,6 +51,7 @@ int main(void)
     BIND(mi_ctx_destroy);
     BIND(mi_last_error);
     BIND(mi_ctx_trim);
+    BIND(mi_ctx_set_graphs);
     BIND(mi_graph_create);
     BIND(mi_graph_create_device);
     BIND(mi_graph_destroy);

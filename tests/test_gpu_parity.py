@@ -241,6 +241,31 @@ def test_power_iter_column_blocks_vs_oracle(ctx):
 
 
 @pytest.mark.parametrize("normLap", [0, 1])
+def test_power_iter_graph_replay_equals_eager(ctx, normLap):
+    """Iterations replayed from the captured HIP graph give exactly what
+    launching every kernel from the host gives, on several levels in a row
+    (each level captures its own graph; eager work runs between them)."""
+    rng = np.random.default_rng(21 + normLap)
+    cases = [random_graph(rng, 3000, 0.002, weighted=True, lcc=True), _hub_graph(3),
+             _dense_graph(20000, 70, 5, True)]
+    try:
+        for rp, col, w in cases:
+            g = ctx.graph(rp, col, w)
+            u0 = rng.random(len(rp) - 1)
+            ctx.set_graphs(True)
+            ug, itg, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+            y = g.spmv_shift(u0)  # eager work between replays
+            ug2, itg2, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+            ctx.set_graphs(False)
+            ue, ite, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+            assert itg == itg2 == ite and itg > 8
+            assert np.array_equal(ug, ue) and np.array_equal(ug2, ue)
+            assert np.array_equal(y, g.spmv_shift(u0))
+    finally:
+        ctx.set_graphs(True)
+
+
+@pytest.mark.parametrize("normLap", [0, 1])
 def test_power_iter_vs_oracle(ctx, normLap):
     rng = np.random.default_rng(4 + normLap)
     rp, col, w = random_graph(rng, 3000, 0.002, weighted=True, lcc=True)
