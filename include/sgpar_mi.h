@@ -160,6 +160,34 @@ int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config,
                        uint64_t pcg_inc, uint32_t* part, int part_on_device,
                        uint32_t* edge_cut, mi_stats* stats);
 
+/*
+ * Induced subgraphs of the two sides of part (n entries, 0 = first side,
+ * anything else = second; host or device memory): sgp_use_partition,
+ * sgpar.h:2666-2752.  Vertices are relabelled in order within their side,
+ * edges between the sides are dropped, the rest keep their CSR order, and
+ * every edge weight is 1 whatever g's weights are.  *g1 and *g2 are new
+ * graphs on ctx (mi_graph_destroy them); a side may have no vertices.
+ */
+int mi_use_partition(mi_ctx* ctx, const mi_graph* g, const uint32_t* part, int part_on_device,
+                     mi_graph** g1, mi_graph** g2);
+
+/*
+ * Recursive 4-way partition, the driver of cmg/sgpar.cpp:53-89 run on the
+ * device: bisect g (as mi_partition_graph), take the induced subgraphs of
+ * the two sides (sgp_use_partition, sgpar.h:2666-2752: each side's vertices
+ * relabelled in order, edges between the sides dropped, every edge weight 1
+ * whatever g's weights are), then bisect each
+ * side with the same config, hec_seed and the continuing pcg state.  part
+ * (n entries, host or device) receives 2*side + sub-side in {0,1,2,3};
+ * cuts[0..2] the three bisection cuts; *edge_cut_4way their sum (what the
+ * reference logs as the 4-way cut, ExperimentLoggerUtil::setEdgeCut4Way).
+ * A side with fewer than 2 vertices is not split (cut 0).
+ */
+int mi_partition_4way(mi_ctx* ctx, const mi_graph* g, const mi_config* config,
+                      int perc_imbalance, uint64_t hec_seed, uint64_t* pcg_state,
+                      uint64_t pcg_inc, uint32_t* part, int part_on_device, uint32_t* cuts,
+                      uint32_t* edge_cut_4way);
+
 /* ---------------------------------------------------------------- building blocks
  * Single steps of the path, exposed for parity tests and measurement. */
 

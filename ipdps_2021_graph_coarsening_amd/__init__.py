@@ -100,6 +100,9 @@ _SIGS = [
     ("mi_partition_graph", C.c_int, [vp, vp, C.POINTER(Config), C.c_int, C.c_uint64,
                                      C.POINTER(C.c_uint64), C.c_uint64, vp, C.c_int,
                                      C.POINTER(C.c_uint32), C.POINTER(Stats)]),
+    ("mi_use_partition", C.c_int, [vp, vp, u32p, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+    ("mi_partition_4way", C.c_int, [vp, vp, C.POINTER(Config), C.c_int, C.c_uint64, C.POINTER(C.c_uint64),
+                                    C.c_uint64, vp, C.c_int, u32p, C.POINTER(C.c_uint32)]),
     ("mi_xs_draws", C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64, u64p]),
     ("mi_hec_level", C.c_int, [vp, vp, C.c_int, C.c_uint64, u32p, vp, vp, C.POINTER(C.c_uint32)]),
     ("mi_build_coarse", C.c_int, [vp, vp, u32p, C.c_uint32, C.POINTER(vp), vp]),
@@ -291,6 +294,14 @@ class Graph:
                 "mi_compute_partition")
         return part, ec.value
 
+    def use_partition(self, part):
+        """Induced subgraphs of the sides of part (sgp_use_partition, sgpar.h:2666-2752),
+        built on the device: (Graph, Graph), unit weights."""
+        c = self.ctx
+        h1, h2 = vp(), vp()
+        c.check(c._L.mi_use_partition(c.h, self.h, _u32(part), 0, C.byref(h1), C.byref(h2)), "mi_use_partition")
+        return Graph(c, None, None, _handle=h1), Graph(c, None, None, _handle=h2)
+
     def hierarchy(self, seed: int) -> "Hierarchy":
         return Hierarchy.build(self, seed)
 
@@ -416,3 +427,19 @@ def partition_graph(ctx: Context, g: Graph, config: Config | None = None, perc_i
                                         pcg_inc, _ptr(part), 0, C.byref(ec), C.byref(stats)),
               "mi_partition_graph")
     return PartitionResult(part=part, edge_cut=ec.value, stats=stats.as_dict(), pcg_state=st.value)
+
+
+def partition_4way(ctx: Context, g: Graph, config: Config | None = None, perc_imbalance: int = 0,
+                   hec_seed: int = 0, pcg_state: int = 0, pcg_inc: int = 1) -> PartitionResult:
+    """Recursive 4-way partition (the driver of sgpar.cpp:53-89) on the device.
+    part in {0..3}; edge_cut = sum of the three bisection cuts; extra['cuts']."""
+    if config is None:
+        config = Config(4, 0, 0, 1, 1e-10, 4)
+    part = np.empty(g.n, dtype=np.uint32)
+    st = C.c_uint64(pcg_state)
+    cuts = np.zeros(3, dtype=np.uint32)
+    ec = C.c_uint32()
+    ctx.check(ctx._L.mi_partition_4way(ctx.h, g.h, C.byref(config), perc_imbalance, hec_seed, C.byref(st), pcg_inc,
+                                       part.ctypes.data_as(vp), 0, cuts, C.byref(ec)), "mi_partition_4way")
+    return PartitionResult(part=part, edge_cut=ec.value, stats={}, pcg_state=st.value,
+                           extra={"cuts": [int(x) for x in cuts]})

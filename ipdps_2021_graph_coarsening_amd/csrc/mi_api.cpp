@@ -586,6 +586,46 @@ int mi_compute_partition(mi_ctx* ctx, const mi_graph* g, const double* evec, int
     });
 }
 
+namespace {
+
+void check_config(const mi_config* config)
+{
+    MI_CHECK(config->local_search_alg == 0, ST_ARG, "only local_search_alg = 0 is on this path");
+    MI_CHECK(config->refine_alg == 0 || config->refine_alg == 1, ST_ARG, "refine_alg must be 0 or 1");
+    MI_CHECK(config->tol >= 0, ST_ARG, "tol must be >= 0");
+    MI_CHECK(config->num_partitions == 2 || config->num_partitions == 4, ST_ARG, "num_partitions must be 2 or 4");
+}
+
+// One bisection of g into device array d_part (sgp_partition_graph,
+// sgpar.h:2363-2431); returns the edge cut.
+uint32_t bisect(Ctx& c, const DGraph& g, const mi_config* config, int perc_imbalance, uint64_t hec_seed,
+                uint64_t* pcg_state, uint64_t pcg_inc, uint32_t* d_part, mi_stats* stats)
+{
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<DLevel> lv;
+    build_hierarchy(c, g, hec_seed, lv, stats);
+    MI_HIP(hipStreamSynchronize(c.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    DBuf<double> ev = fiedler(c, lv, pcg_state, pcg_inc, config->refine_alg, config->tol, stats);
+    MI_HIP(hipStreamSynchronize(c.stream));
+    auto t2 = std::chrono::steady_clock::now();
+    fill_level_stats(lv, stats);
+    lv.clear();
+    const uint32_t cut = compute_partition(c, g, ev.get(), config->num_partitions, perc_imbalance, d_part);
+    MI_HIP(hipStreamSynchronize(c.stream));
+    if (stats) {
+        stats->coarsen_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->eigen_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        stats->partition_ms = ms_since(t2);
+        stats->total_ms = ms_since(t0);
+        stats->edge_cut = cut;
+    }
+    return cut;
+}
+
+}  // namespace
+
 int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config, int perc_imbalance,
                        uint64_t hec_seed, uint64_t* pcg_state, uint64_t pcg_inc, uint32_t* part,
                        int part_on_device, uint32_t* edge_cut, mi_stats* stats)
@@ -593,36 +633,75 @@ int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config, 
     if (!ctx || !g || !config || !pcg_state || !part || !edge_cut) return ST_ARG;
     return guard(ctx, [&] {
         Ctx& c = ctx->c;
-        MI_CHECK(config->local_search_alg == 0, ST_ARG, "only local_search_alg = 0 is on this path");
-        MI_CHECK(config->refine_alg == 0 || config->refine_alg == 1, ST_ARG, "refine_alg must be 0 or 1");
-        MI_CHECK(config->tol >= 0, ST_ARG, "tol must be >= 0");
-        if (stats) std::memset(stats, 0, sizeof(*stats));
-        auto t0 = std::chrono::steady_clock::now();
-        std::vector<DLevel> lv;
-        build_hierarchy(c, g->g, hec_seed, lv, stats);
-        MI_HIP(hipStreamSynchronize(c.stream));
-        auto t1 = std::chrono::steady_clock::now();
-        DBuf<double> ev = fiedler(c, lv, pcg_state, pcg_inc, config->refine_alg, config->tol, stats);
-        MI_HIP(hipStreamSynchronize(c.stream));
-        auto t2 = std::chrono::steady_clock::now();
-        fill_level_stats(lv, stats);
-        lv.clear();
+        check_config(config);
         const uint32_t n = g->g.n;
         if (part_on_device) {
-            *edge_cut = compute_partition(c, g->g, ev.get(), config->num_partitions, perc_imbalance, part);
+            *edge_cut = bisect(c, g->g, config, perc_imbalance, hec_seed, pcg_state, pcg_inc, part, stats);
         } else {
             DBuf<uint32_t> dp(&c.pool, n);
-            *edge_cut = compute_partition(c, g->g, ev.get(), config->num_partitions, perc_imbalance, dp.get());
+            *edge_cut = bisect(c, g->g, config, perc_imbalance, hec_seed, pcg_state, pcg_inc, dp.get(), stats);
             d2h(c, part, dp.get(), (size_t)n * 4);
+            MI_HIP(hipStreamSynchronize(c.stream));
         }
+    });
+}
+
+int mi_use_partition(mi_ctx* ctx, const mi_graph* g, const uint32_t* part, int part_on_device, mi_graph** g1,
+                     mi_graph** g2)
+{
+    if (!ctx || !g || !part || !g1 || !g2) return ST_ARG;
+    *g1 = *g2 = nullptr;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        DBuf<uint32_t> dp;
+        const uint32_t* d_part = part;
+        if (!part_on_device) {
+            dp = DBuf<uint32_t>(&c.pool, n);
+            h2d(c, dp.get(), part, (size_t)n * 4);
+            d_part = dp.get();
+        }
+        auto A = std::make_unique<mi_graph>(), B = std::make_unique<mi_graph>();
+        A->ctx = B->ctx = ctx;
+        DBuf<uint32_t> mapping;
+        use_partition(c, g->g, d_part, A->g, B->g, mapping);
+        *g1 = A.release();
+        *g2 = B.release();
+    });
+}
+
+int mi_partition_4way(mi_ctx* ctx, const mi_graph* g, const mi_config* config, int perc_imbalance,
+                      uint64_t hec_seed, uint64_t* pcg_state, uint64_t pcg_inc, uint32_t* part, int part_on_device,
+                      uint32_t* cuts, uint32_t* edge_cut_4way)
+{
+    if (!ctx || !g || !config || !pcg_state || !part || !cuts || !edge_cut_4way) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        check_config(config);
+        const uint32_t n = g->g.n;
+        DBuf<uint32_t> top(&c.pool, n);
+        cuts[0] = bisect(c, g->g, config, perc_imbalance, hec_seed, pcg_state, pcg_inc, top.get(), nullptr);
+        DGraph side[2];
+        DBuf<uint32_t> mapping;
+        use_partition(c, g->g, top.get(), side[0], side[1], mapping);
+        DBuf<uint32_t> sub[2];
+        for (int s = 0; s < 2; s++) {
+            cuts[1 + s] = 0;
+            if (side[s].n < 2) continue;  // nothing to split
+            sub[s] = DBuf<uint32_t>(&c.pool, side[s].n);
+            cuts[1 + s] =
+                bisect(c, side[s], config, perc_imbalance, hec_seed, pcg_state, pcg_inc, sub[s].get(), nullptr);
+        }
+        *edge_cut_4way = cuts[0] + cuts[1] + cuts[2];
+        DBuf<uint32_t> out;
+        uint32_t* dst = part;
+        if (!part_on_device) {
+            out = DBuf<uint32_t>(&c.pool, n);
+            dst = out.get();
+        }
+        compose_4way(c, n, top.get(), mapping.get(), sub[0].get(), sub[1].get(), dst);
+        if (!part_on_device) d2h(c, part, dst, (size_t)n * 4);
         MI_HIP(hipStreamSynchronize(c.stream));
-        if (stats) {
-            stats->coarsen_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-            stats->eigen_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-            stats->partition_ms = ms_since(t2);
-            stats->total_ms = ms_since(t0);
-            stats->edge_cut = *edge_cut;
-        }
     });
 }
 

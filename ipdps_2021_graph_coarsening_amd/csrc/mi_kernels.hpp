@@ -159,5 +159,11 @@ void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg
 // array part; returns the edge cut.
 uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
                            int perc_imbalance, uint32_t* part);
+// Induced subgraphs of the sides 0 and nonzero of device array part
+// (sgp_use_partition); mapping[i] = i's index in its side's graph.
+void use_partition(Ctx& c, const DGraph& g, const uint32_t* part, DGraph& g0, DGraph& g1, DBuf<uint32_t>& mapping);
+// out[i] = 2 side(i) + sub-part of i in its side's graph (p0 / p1 may be null: 0)
+void compose_4way(Ctx& c, uint32_t n, const uint32_t* part, const uint32_t* mapping, const uint32_t* p0,
+                  const uint32_t* p1, uint32_t* out);
 
 }  // namespace mi

@@ -121,4 +121,123 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     return (uint32_t)(best / 2);
 }
 
+// ---------------------------------------------------------------- use_partition
+// Induced subgraphs of the two sides of a bisection, sgp_use_partition
+// (sgpar.h:2666-2752): the vertices of each side keep their order
+// (mapping[i] = rank of i among its side), an edge is kept when both ends
+// are on the same side, and each kept row keeps its entries' order.  One
+// scan per side over the entries gives every kept entry its place.
+namespace {
+
+__global__ void k_side0(uint32_t n, const uint32_t* __restrict__ part, uint32_t* __restrict__ f0)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) f0[i] = part[i] == 0u ? 1u : 0u;
+}
+
+__global__ void k_keep(uint64_t m, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ roc,
+                       const uint32_t* __restrict__ col, const uint32_t* __restrict__ part, uint32_t* __restrict__ k0,
+                       uint32_t* __restrict__ k1)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t pr = part[entry_row((uint32_t)e, roc, rowptr)], pc = part[col[e]];
+    k0[e] = (pr == 0u && pc == 0u) ? 1u : 0u;
+    k1[e] = (pr != 0u && pc != 0u) ? 1u : 0u;
+}
+
+__global__ void k_side_rows(uint32_t n, uint64_t m, const uint32_t* __restrict__ rowptr,
+                            const uint32_t* __restrict__ part, const uint32_t* __restrict__ rank0,
+                            const uint32_t* __restrict__ s0, const uint32_t* __restrict__ s1,
+                            const uint32_t* __restrict__ tot, uint32_t* __restrict__ mapping,
+                            uint32_t* __restrict__ rp0, uint32_t* __restrict__ rp1)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = rowptr[i];
+    if (part[i] == 0u) {
+        mapping[i] = rank0[i];
+        rp0[rank0[i]] = e < m ? s0[e] : tot[0];
+    } else {
+        mapping[i] = i - rank0[i];
+        rp1[i - rank0[i]] = e < m ? s1[e] : tot[1];
+    }
+}
+
+__global__ void k_side_entries(uint64_t m, const uint32_t* __restrict__ col, const uint32_t* __restrict__ k0,
+                               const uint32_t* __restrict__ k1, const uint32_t* __restrict__ s0,
+                               const uint32_t* __restrict__ s1, const uint32_t* __restrict__ mapping,
+                               uint32_t* __restrict__ c0, uint32_t* __restrict__ c1)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    if (k0[e])
+        c0[s0[e]] = mapping[col[e]];
+    else if (k1[e])
+        c1[s1[e]] = mapping[col[e]];
+}
+
+__global__ void k_compose4(uint32_t n, const uint32_t* __restrict__ part, const uint32_t* __restrict__ mapping,
+                           const uint32_t* __restrict__ p0, const uint32_t* __restrict__ p1, uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t side = part[i] == 0u ? 0u : 1u;
+    const uint32_t* sub = side ? p1 : p0;
+    out[i] = 2u * side + (sub ? sub[mapping[i]] : 0u);
+}
+
+}  // namespace
+
+void use_partition(Ctx& c, const DGraph& g, const uint32_t* part, DGraph& g0, DGraph& g1, DBuf<uint32_t>& mapping)
+{
+    const uint32_t n = g.n;
+    const uint64_t m = g.m;
+    DBuf<uint32_t> f0(&c.pool, n), rank0(&c.pool, n), tot(&c.pool, 4);
+    k_side0<<<grid1(n), BLOCK, 0, c.stream>>>(n, part, f0.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, f0.get(), rank0.get(), n, tot.get() + 2);
+    DBuf<uint32_t> k0(&c.pool, m), k1(&c.pool, m), s0(&c.pool, m), s1(&c.pool, m);
+    if (m) {
+        DBuf<uint32_t> roc;
+        build_row_index(c, g, roc);
+        k_keep<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc.get(), g.Cc(), part, k0.get(), k1.get());
+        MI_HIP(hipGetLastError());
+    }
+    scan_exclusive_u32(c, k0.get(), s0.get(), m, tot.get() + 0);
+    scan_exclusive_u32(c, k1.get(), s1.get(), m, tot.get() + 1);
+    uint32_t t[4];
+    read_words(c, tot.get(), t, 16);
+    const uint32_t n0 = t[2], n1 = n - t[2], m0 = t[0], m1 = t[1];
+    // topology only: the sides carry unit weights whatever g's are
+    auto alloc = [&](DGraph& h, uint32_t nn, uint32_t mm) {
+        h = DGraph();
+        h.n = nn;
+        h.m = mm;
+        h.unit = true;
+        h.rowptr = DBuf<uint32_t>(&c.pool, (size_t)nn + 1);
+        h.col = DBuf<uint32_t>(&c.pool, mm);
+        MI_HIP(hipMemcpyAsync(h.rowptr.get() + nn, &t[&h == &g0 ? 0 : 1], 4, hipMemcpyHostToDevice, c.stream));
+    };
+    alloc(g0, n0, m0);
+    alloc(g1, n1, m1);
+    mapping = DBuf<uint32_t>(&c.pool, n);
+    k_side_rows<<<grid1(n), BLOCK, 0, c.stream>>>(n, m, g.R(), part, rank0.get(), s0.get(), s1.get(), tot.get(),
+                                                  mapping.get(), g0.rowptr.get(), g1.rowptr.get());
+    MI_HIP(hipGetLastError());
+    if (m) {
+        k_side_entries<<<grid1(m), BLOCK, 0, c.stream>>>(m, g.Cc(), k0.get(), k1.get(), s0.get(), s1.get(),
+                                                         mapping.get(), g0.col.get(), g1.col.get());
+        MI_HIP(hipGetLastError());
+    }
+    MI_HIP(hipStreamSynchronize(c.stream));  // t[] is read by the copies above
+}
+
+void compose_4way(Ctx& c, uint32_t n, const uint32_t* part, const uint32_t* mapping, const uint32_t* p0,
+                  const uint32_t* p1, uint32_t* out)
+{
+    k_compose4<<<grid1(n), BLOCK, 0, c.stream>>>(n, part, mapping, p0, p1, out);
+    MI_HIP(hipGetLastError());
+}
+
 }  // namespace mi

@@ -39,6 +39,9 @@ int main(void)
     int (*fp_mi_compute_partition)(mi_ctx*, const mi_graph*, const double*, int, int, uint32_t*, uint32_t*);
     int (*fp_mi_partition_graph)(mi_ctx*, const mi_graph*, const mi_config*, int, uint64_t, uint64_t*, uint64_t,
                                  uint32_t*, int, uint32_t*, mi_stats*);
+    int (*fp_mi_use_partition)(mi_ctx*, const mi_graph*, const uint32_t*, int, mi_graph**, mi_graph**);
+    int (*fp_mi_partition_4way)(mi_ctx*, const mi_graph*, const mi_config*, int, uint64_t, uint64_t*, uint64_t,
+                                uint32_t*, int, uint32_t*, uint32_t*);
     int (*fp_mi_xs_draws)(mi_ctx*, uint64_t, uint64_t, uint64_t, uint64_t*);
     int (*fp_mi_hec_level)(mi_ctx*, const mi_graph*, int, uint64_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*);
     int (*fp_mi_build_coarse)(mi_ctx*, const mi_graph*, const uint32_t*, uint32_t, mi_graph**, uint32_t*);
@@ -67,6 +70,8 @@ int main(void)
     BIND(mi_power_iter);
     BIND(mi_compute_partition);
     BIND(mi_partition_graph);
+    BIND(mi_use_partition);
+    BIND(mi_partition_4way);
     BIND(mi_xs_draws);
     BIND(mi_hec_level);
     BIND(mi_build_coarse);

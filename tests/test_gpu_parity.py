@@ -336,6 +336,101 @@ def test_partition_graph_edge_cases(ctx, name):
     assert abs(r.edge_cut - o["edge_cut"]) <= max(2, CUT_REL * o["edge_cut"])
 
 
+@pytest.mark.parametrize("case", ["rmat_weighted", "rgg", "one_side_empty", "isolated"])
+def test_use_partition_bitexact(ctx, case):
+    """Device sgp_use_partition (sgpar.h:2666-2752) against the host
+    restatement, itself checked against the compiled reference."""
+    rng = np.random.default_rng(8)
+    if case == "rmat_weighted":
+        rp, col, w = graphs.rmat(12, 16, seed=2)
+    elif case == "rgg":
+        rp, col, w = graphs.rgg(20000, seed=5)
+    else:
+        rp, col, w = _graph_with_isolated(rng)
+    n = len(rp) - 1
+    part = rng.integers(0, 2, n).astype(np.uint32)
+    if case == "one_side_empty":
+        part[:] = 1
+    (r0, c0), (r1, c1) = graphs.use_partition(rp, col, part)
+    g0, g1 = ctx.graph(rp, col, w).use_partition(part)
+    for g, r, c in ((g0, r0, c0), (g1, r1, c1)):
+        hr, hc, hw = g.to_host()
+        assert np.array_equal(hr, r) and np.array_equal(hc, c)
+        assert np.all(hw == 1)
+
+
+def _cut4(rp, col, part4):
+    src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return int(np.count_nonzero(part4[src] != part4[col])) // 2
+
+
+@pytest.mark.parametrize("name", ["grid128", "rgg", "rmat_weighted"])
+def test_partition_4way_equals_host_composed_flow(ctx, name):
+    """The device driver (sgpar.cpp:53-89) against the same steps composed on
+    the host: bisect, sgp_use_partition on the host, bisect each side with
+    the continuing pcg state.  Same kernels on the same inputs: bit-exact."""
+    if name == "grid128":
+        rp, col, w = graphs.grid2d(128)
+    elif name == "rgg":
+        rp, col, w = graphs.rgg(20000, seed=5)
+    else:
+        rp, col, w = graphs.rmat(12, 16, seed=2)
+    r4 = mi.partition_4way(ctx, ctx.graph(rp, col, w), hec_seed=7, pcg_state=11)
+    top = mi.partition_graph(ctx, ctx.graph(rp, col, w), hec_seed=7, pcg_state=11)
+    (r0, c0), (r1, c1) = graphs.use_partition(rp, col, top.part)
+    s0 = mi.partition_graph(ctx, ctx.graph(r0, c0), hec_seed=7, pcg_state=top.pcg_state)
+    s1 = mi.partition_graph(ctx, ctx.graph(r1, c1), hec_seed=7, pcg_state=s0.pcg_state)
+    assert r4.extra["cuts"] == [top.edge_cut, s0.edge_cut, s1.edge_cut]
+    assert r4.edge_cut == top.edge_cut + s0.edge_cut + s1.edge_cut
+    assert r4.pcg_state == s1.pcg_state
+    side = top.part.astype(bool)
+    exp = np.empty(len(rp) - 1, np.uint32)
+    exp[~side] = s0.part
+    exp[side] = 2 + s1.part
+    assert np.array_equal(r4.part, exp)
+    # the summed cut is the number of edges between different quarters
+    assert r4.edge_cut == _cut4(rp, col, r4.part)
+    assert np.bincount(r4.part, minlength=4).min() >= (len(rp) - 1) // 4 - 2
+
+
+def test_partition_4way_vs_oracle(ctx):
+    rp, col, _ = graphs.rgg(8000, seed=9)
+    r4 = mi.partition_4way(ctx, ctx.graph(rp, col), hec_seed=4, pcg_state=4)
+    o = oracle.partition_graph(rp, col, None, 4, 4)
+    (r0, c0), (r1, c1) = graphs.use_partition(rp, col, o["part"])
+    o0 = oracle.partition_graph(r0, c0, None, 4, o["pcg_state"])
+    o1 = oracle.partition_graph(r1, c1, None, 4, o0["pcg_state"])
+    ocut = o["edge_cut"] + o0["edge_cut"] + o1["edge_cut"]
+    assert abs(r4.extra["cuts"][0] - o["edge_cut"]) <= max(2, CUT_REL * o["edge_cut"])
+    assert abs(r4.edge_cut - ocut) <= max(3, 0.02 * ocut)
+
+
+def test_partition_4way_tiny_sides(ctx):
+    # a 3-vertex path: one side holds a single vertex and is not split
+    rp, col, _ = graphs.path(3)
+    r4 = mi.partition_4way(ctx, ctx.graph(rp, col), hec_seed=1, pcg_state=1)
+    assert r4.edge_cut == _cut4(rp, col, r4.part)
+    assert sorted(set(r4.part.tolist())) <= [0, 1, 2, 3]
+
+
+def test_partition_4way_star_edgeless_side(ctx):
+    # the leaves split off from the centre form a side without edges, which
+    # is still coarsened and bisected (cut 0), as the oracle does
+    k = 500
+    rp, col, _ = graphs.from_edges(k + 1, np.zeros(k, np.int64), np.arange(1, k + 1), None, lcc=False)
+    r4 = mi.partition_4way(ctx, ctx.graph(rp, col), hec_seed=3, pcg_state=3)
+    o = oracle.partition_graph(rp, col, None, 3, 3)
+    (r0, c0), (r1, c1) = graphs.use_partition(rp, col, o["part"])
+    o0 = oracle.partition_graph(r0, c0, None, 3, o["pcg_state"])
+    o1 = oracle.partition_graph(r1, c1, None, 3, o0["pcg_state"])
+    oc = [o["edge_cut"], o0["edge_cut"], o1["edge_cut"]]
+    assert 0 in oc[1:] and 0 in r4.extra["cuts"][1:]
+    for a, b in zip(r4.extra["cuts"], oc):
+        assert abs(a - b) <= max(2, CUT_REL * b)
+    assert r4.edge_cut == _cut4(rp, col, r4.part)
+    assert np.bincount(r4.part, minlength=4).min() >= k // 4 - 2
+
+
 def test_bad_input_is_rejected(ctx):
     rp = np.array([0, 2, 1], dtype=np.uint32)  # not monotone
     col = np.array([1, 0], dtype=np.uint32)
