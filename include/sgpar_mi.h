@@ -63,6 +63,15 @@ typedef struct {
     double coarsen_ms, eigen_ms, partition_ms, total_ms;
     double permute_ms, heavy_ms, map_ms, build_ms;
     uint32_t edge_cut;
+    /* Final eigenvalue log of the finest level, refine_alg 0 only
+       (sgp_power_iter_eigenvalue_log, eigensolve_kokkos.h:60-100):
+       eigenvalue = sum of t_i^2 with t = len .* u - A u (len = row lengths),
+       eigen_min / eigen_max = min / max of t_i / u_i.  The reference prints
+       these with "edge cut lb" = eigenvalue * n / 4 and
+       "gap ratio" = ceil(1 / (1 - eigenvalue)).  has_eigen_log = 0 when not
+       computed (refine_alg 1, or a call without a final level). */
+    int has_eigen_log;
+    double eigenvalue, eigen_min, eigen_max;
 } mi_stats;
 
 /* ---------------------------------------------------------------- context */

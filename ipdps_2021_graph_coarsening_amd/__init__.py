@@ -57,7 +57,9 @@ class Stats(C.Structure):
                 ("partition_ms", C.c_double), ("total_ms", C.c_double),
                 ("permute_ms", C.c_double), ("heavy_ms", C.c_double),
                 ("map_ms", C.c_double), ("build_ms", C.c_double),
-                ("edge_cut", C.c_uint32)]
+                ("edge_cut", C.c_uint32),
+                ("has_eigen_log", C.c_int),
+                ("eigenvalue", C.c_double), ("eigen_min", C.c_double), ("eigen_max", C.c_double)]
 
     def as_dict(self) -> dict:
         L = self.nlevels
@@ -68,7 +70,9 @@ class Stats(C.Structure):
                     coarsen_ms=self.coarsen_ms, eigen_ms=self.eigen_ms,
                     partition_ms=self.partition_ms, total_ms=self.total_ms,
                     permute_ms=self.permute_ms, heavy_ms=self.heavy_ms, map_ms=self.map_ms,
-                    build_ms=self.build_ms, edge_cut=self.edge_cut)
+                    build_ms=self.build_ms, edge_cut=self.edge_cut,
+                    eigen_log=(dict(eigenvalue=self.eigenvalue, min=self.eigen_min, max=self.eigen_max)
+                               if self.has_eigen_log else None))
 
 
 _lib = None

@@ -220,9 +220,7 @@ DLevel borrow_level(Ctx& c, const DGraph& g)
     L.g.b_col = g.Cc();
     L.g.b_w = g.W();
     L.vtx_w = DBuf<uint32_t>(&c.pool, g.n);
-    std::vector<uint32_t> ones(g.n, 1u);
-    h2d(c, L.vtx_w.get(), ones.data(), (size_t)g.n * 4);
-    MI_HIP(hipStreamSynchronize(c.stream));
+    fill_u32(c, L.vtx_w.get(), g.n, 1u);
     return L;
 }
 
@@ -283,6 +281,7 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
 {
     const int L = (int)lv.size();
     MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
+    if (st) st->has_eigen_log = 0;
     const uint32_t gn = lv[L - 1].g.n;
     std::vector<double> cg(gn);
     // eigensolve_kokkos.h:1500-1505: pcg32()/UINT32_MAX, then normalise
@@ -299,7 +298,17 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
     normalize(c, cur.get(), gn);
     for (int l = L - 1; l >= 0; l--) {
         PowerStats ps;
-        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps);
+        // the reference logs the eigenvalue after the final (finest) solve
+        // with the Laplacian (eigensolve_kokkos.h:201-203)
+        double elog[3];
+        const bool log_here = st && l == 0 && !normLap;
+        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps, log_here ? elog : nullptr);
+        if (log_here) {
+            st->has_eigen_log = 1;
+            st->eigenvalue = elog[0];
+            st->eigen_min = elog[1];
+            st->eigen_max = elog[2];
+        }
         if (st && l < MI_MAX_LEVELS) {
             st->level_iters[l] = ps.iters;
             st->level_max_reached[l] = ps.max_reached;

@@ -46,6 +46,7 @@
 #include "mi_kernels.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <type_traits>
 #include <cmath>
@@ -809,6 +810,78 @@ __global__ void k_scale_inv_sqrt(uint32_t n, double* __restrict__ x, const doubl
     x[i] = x[i] * inv;
 }
 
+// Eigenvalue diagnostic of the finest level (eigensolve_kokkos.h:60-100):
+// t_i = len_i u_i - (A u)_i with len_i the row length; per block the sum of
+// t_i^2 * 1e9 (the reference's scaling), and the max / min of t_i / u_i
+// (a NaN quotient never wins a comparison, as in the reference's loops).
+__device__ __forceinline__ void block_sum_max_min(double& s, double& mx, double& mn, double (*lds)[BLOCK / WAVE])
+{
+    s = wave_sum(s);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const double omx = __shfl_xor(mx, o, WAVE), omn = __shfl_xor(mn, o, WAVE);
+        mx = mx < omx ? omx : mx;
+        mn = mn > omn ? omn : mn;
+    }
+    const uint32_t wid = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        lds[0][wid] = s;
+        lds[1][wid] = mx;
+        lds[2][wid] = mn;
+    }
+    __syncthreads();
+    s = 0.0;
+    mx = -DBL_MAX;
+    mn = DBL_MAX;
+    for (int j = 0; j < BLOCK / WAVE; j++) {
+        s += lds[0][j];
+        mx = mx < lds[1][j] ? lds[1][j] : mx;
+        mn = mn > lds[2][j] ? lds[2][j] : mn;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_eig_log_part(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                        const double* __restrict__ u, const double* __restrict__ au,
+                                                        double* __restrict__ part)
+{
+    __shared__ double lds[3][BLOCK / WAVE];
+    double s = 0.0, mx = -DBL_MAX, mn = DBL_MAX;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+        const double ui = u[i];
+        double t = (double)(rowptr[i + 1] - rowptr[i]) * ui;
+        t -= au[i];
+        s += (t * t) * 1e9;
+        const double est = t / ui;
+        if (mx < est) mx = est;
+        if (mn > est) mn = est;
+    }
+    block_sum_max_min(s, mx, mn, lds);
+    if (threadIdx.x == 0) {
+        part[(size_t)blockIdx.x * 3 + 0] = s;
+        part[(size_t)blockIdx.x * 3 + 1] = mx;
+        part[(size_t)blockIdx.x * 3 + 2] = mn;
+    }
+}
+
+// one block: the partials of nblk blocks in a fixed order
+__global__ void __launch_bounds__(BLOCK) k_eig_log_final(uint32_t nblk, const double* __restrict__ part,
+                                                         double* __restrict__ out)
+{
+    __shared__ double lds[3][BLOCK / WAVE];
+    double s = 0.0, mx = -DBL_MAX, mn = DBL_MAX;
+    for (uint32_t b = threadIdx.x; b < nblk; b += BLOCK) {
+        s += part[(size_t)b * 3 + 0];
+        mx = mx < part[(size_t)b * 3 + 1] ? part[(size_t)b * 3 + 1] : mx;
+        mn = mn > part[(size_t)b * 3 + 2] ? part[(size_t)b * 3 + 2] : mn;
+    }
+    block_sum_max_min(s, mx, mn, lds);
+    if (threadIdx.x == 0) {
+        out[0] = s * 1e-9;
+        out[1] = mn;
+        out[2] = mx;
+    }
+}
+
 __global__ void k_fill_f64(uint32_t n, double* __restrict__ x, double v)
 {
     uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -1216,7 +1289,22 @@ void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, d
 }
 
 
-void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st)
+// eigenvalue, min, max of the diagnostic above into out[0..3) (host)
+static void eigenvalue_log(Ctx& c, const DGraph& g, const LevelPlan& p, const double* d_u, double* tmp,
+                           double* out)
+{
+    StepArgs A = make_args(c, g, p, nullptr, nullptr, d_u, tmp, 0.0, 2);
+    launch_step(c, 0, A);
+    const unsigned nb = red_grid(g.n);
+    DBuf<double> part(&c.pool, (size_t)nb * 3), res(&c.pool, 3);
+    k_eig_log_part<<<nb, BLOCK, 0, c.stream>>>(g.n, g.R(), d_u, tmp, part.get());
+    MI_HIP(hipGetLastError());
+    k_eig_log_final<<<1, BLOCK, 0, c.stream>>>(nb, part.get(), res.get());
+    MI_HIP(hipGetLastError());
+    read_words(c, res.get(), out, 3 * sizeof(double));
+}
+
+void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st, double* eig_log)
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
@@ -1332,6 +1420,7 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     // the reference returns u = input of the last iteration (eigensolve_kokkos.h:157)
     k_materialize<<<grid1(n), BLOCK, 0, c.stream>>>(n, buf[(done_iters - 1) & 1], d_u, c.scal);
     MI_HIP(hipGetLastError());
+    if (eig_log && !normLap) eigenvalue_log(c, g, plan, d_u, other.get(), eig_log);
     if (st) {
         MI_HIP(hipStreamSynchronize(c.stream));
         st->iters = done_iters;

@@ -317,20 +317,7 @@ __global__ void __launch_bounds__(BLOCK) k_max_u32(const uint32_t* __restrict__ 
     if (threadIdx.x == 0) atomicMax(out, v);
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
-{
-    uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
-
-void fill_u32(Ctx& c, uint32_t* p, uint64_t n, uint32_t v)
-{
-    if (!n) return;
-    k_fill_u32<<<grid1(n), BLOCK, 0, c.stream>>>(p, n, v);
-    MI_HIP(hipGetLastError());
-}
 
 const uint64_t* jump_table(Ctx& c)
 {

@@ -80,6 +80,7 @@ __device__ __forceinline__ T wave_seg_reduce(uint32_t key, T v, Op op, bool* hea
 void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n,
                         uint32_t* total_dev);
 void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start);
+void fill_u32(Ctx& c, uint32_t* p, uint64_t n, uint32_t v);
 // Stable LSD radix sort of (key, value) pairs on key bits [begin_bit, end_bit).
 void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                       int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
@@ -141,8 +142,11 @@ struct PowerStats {
     int max_reached = 0;
     double ms = 0;
 };
-// Power iteration on level graph g, in place on d_u (n doubles).
-void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st);
+// Power iteration on level graph g, in place on d_u (n doubles).  eig_log
+// (host, 3 doubles, may be null; normLap 0 only) receives the final
+// eigenvalue diagnostic of eigensolve_kokkos.h:60-100: eigenvalue, min, max.
+void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st,
+                double* eig_log = nullptr);
 // out[i] = in[vcmap[i]]
 void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, double* out);
 // u /= ||u||

@@ -233,6 +233,12 @@ __global__ void k_row_index(uint32_t n, const uint32_t* __restrict__ rowptr, uin
     if (r == 0) roc[nchunks] = n - 1u;
 }
 
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_iota(uint32_t* out, uint64_t n, uint32_t start)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,6 +273,13 @@ void build_row_index(Ctx& c, const DGraph& g, DBuf<uint32_t>& roc)
     const uint32_t nchunks = (uint32_t)ceil_div(g.m, 64);
     roc = DBuf<uint32_t>(&c.pool, (size_t)nchunks + 1);
     k_row_index<<<(unsigned)ceil_div(g.n, BLOCK), BLOCK, 0, c.stream>>>(g.n, g.R(), roc.get(), nchunks);
+    MI_HIP(hipGetLastError());
+}
+
+void fill_u32(Ctx& c, uint32_t* p, uint64_t n, uint32_t v)
+{
+    if (!n) return;
+    k_fill_u32<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, c.stream>>>(p, n, v);
     MI_HIP(hipGetLastError());
 }
 

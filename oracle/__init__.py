@@ -80,6 +80,8 @@ class _Core:
         L.orc_eigensolve.restype = C.c_int
         L.orc_eigensolve.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.c_int,
                                      C.c_double, C.c_uint64, f64p, u64p]
+        L.orc_eigenvalue_log.restype = None
+        L.orc_eigenvalue_log.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, f64p, f64p]
         L.orc_compute_partition.restype = C.c_int
         L.orc_compute_partition.argtypes = [C.c_uint32, u32p, u32p, f64p, C.c_int, C.c_int,
                                             u32p, C.POINTER(C.c_uint32)]
@@ -256,6 +258,17 @@ def power_iter(rowptr, col, w, u, normLap: int = 0, tol: float = 1e-10, iter_cap
     return u, int(it), bool(mr.value), d.value
 
 
+def eigenvalue_log(rowptr, col, w, u):
+    """Final eigenvalue log (eigensolve_kokkos.h:60-100): (eigenvalue, min, max)."""
+    L = core().L
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    out = np.zeros(3, dtype=np.float64)
+    L.orc_eigenvalue_log(n, np.ascontiguousarray(rowptr, dtype=np.uint32), np.ascontiguousarray(col, dtype=np.uint32),
+                         wp, np.ascontiguousarray(u, dtype=np.float64), out)
+    return float(out[0]), float(out[1]), float(out[2])
+
+
 def compute_partition(rowptr, col, evec, num_partitions: int = 2, imbalance: int = 0):
     L = core().L
     n = len(rowptr) - 1
@@ -314,6 +327,8 @@ class _Ref:
         L.ref_use_partition.restype = C.c_int
         L.ref_use_partition.argtypes = [C.c_uint32, u32p, u32p, u32p, C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p)]
+        L.ref_eigenvalue_log.restype = C.c_int
+        L.ref_eigenvalue_log.argtypes = [C.c_uint32, u32p, u32p, f64p, f64p]
         L.ref_edit_distance.restype = C.c_int
         L.ref_edit_distance.argtypes = [u32p, u32p, C.c_uint32, C.POINTER(C.c_uint32)]
 

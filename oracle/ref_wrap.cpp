@@ -10,6 +10,7 @@
 //   hashmap_deduplicate          sgpar.h:911-938
 //   sgp_build_coarse_graph_msd   sgpar.h:1223-1345 (serial)
 //   sgp_power_iter               sgpar.h:1790-1948 (serial)
+//   sgp_power_iter_eigenvalue_log sgpar.h:1758-1788 (serial, prints only)
 //   sgp_compute_partition        sgpar.h:1969-2087 (shared with the Kokkos build)
 //   sgp_load_graph / sgp_free_graph / sgp_load_config / sgp_use_partition /
 //   compute_partition_edit_distance   sgpar.h:2262-2361, 2666-2787
@@ -108,6 +109,34 @@ long ref_power_iter(uint32_t n, const uint32_t* rowptr, const uint32_t* col,
     size_t at = text.rfind(key);
     if (at == std::string::npos) return -1;
     return std::strtol(text.c_str() + at + key.size(), nullptr, 10);
+}
+
+// Serial reference eigenvalue log (sgpar.h:1758-1788, topology only); it
+// only prints, so stdout is captured and out = {eigenvalue, min, max} is
+// parsed from its line (9 decimals).  Returns 0 on success.
+int ref_eigenvalue_log(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* u, double* out)
+{
+    std::vector<uint32_t> eps, wdeg, ew;
+    sgp_graph_t g = make_graph(n, rowptr, col, nullptr, eps, wdeg, ew);
+    std::vector<double> uu(u, u + n);
+    std::fflush(stdout);
+    FILE* tmp = std::tmpfile();
+    if (!tmp) return -1;
+    int saved = dup(1);
+    dup2(fileno(tmp), 1);
+    sgp_power_iter_eigenvalue_log(uu.data(), g);
+    std::fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    std::rewind(tmp);
+    std::string text;
+    char buf[4096];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof(buf), tmp)) > 0) text.append(buf, k);
+    std::fclose(tmp);
+    const size_t at = text.rfind("eigenvalue = ");
+    if (at == std::string::npos) return -1;
+    return std::sscanf(text.c_str() + at, "eigenvalue = %lf (%lf %lf)", &out[0], &out[1], &out[2]) == 3 ? 0 : -1;
 }
 
 // Serial reference coarse build for a level > 1 (weighted) with hashmap

@@ -24,6 +24,7 @@
  *     (full period of the xorshift transition, sortedness of the argsort,
  *     brute-force heavy neighbour, validity of the cluster map).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -559,6 +560,35 @@ uint64_t orc_power_iter(vid_t n, const eid_t *rowptr, const vid_t *col, const wg
     free(v);
     free(wdeg);
     return niter;
+}
+
+/* ------------------------------------------------------------------ */
+/* Final eigenvalue log, sgp_power_iter_eigenvalue_log,                 */
+/* eigensolve_kokkos.h:60-100 (called at :201-203 after the finest      */
+/* Laplacian solve).  v = A u (spmv with the edge values, :68); with    */
+/* len_i the row length (:71), t_i = len_i u_i - v_i (:72-75); the sum  */
+/* of t_i^2 * 1e9 (:77) in row order (Serial), times 1e-9 when printed; */
+/* max / min of t_i / u_i through Kokkos::Max / Min reducers (:80-92),  */
+/* whose start values are the reducers' identities (-DBL_MAX, DBL_MAX), */
+/* not the 0 and 2 the variables held.  out = {eigenvalue, min, max}.   */
+/* ------------------------------------------------------------------ */
+void orc_eigenvalue_log(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                        const double *u, double *out)
+{
+    double sum = 0.0, mx = -DBL_MAX, mn = DBL_MAX;
+    for (vid_t i = 0; i < n; i++) {
+        double av = 0.0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) av += (double)(w ? w[e] : 1u) * u[col[e]];
+        double t = (double)(rowptr[i + 1] - rowptr[i]) * u[i];
+        t -= av;
+        sum += (t * t) * 1e9;
+        const double est = t / u[i];
+        if (mx < est) mx = est;
+        if (mn > est) mn = est;
+    }
+    out[0] = sum * 1e-9;
+    out[1] = mn;
+    out[2] = mx;
 }
 
 /* ------------------------------------------------------------------ */

@@ -315,6 +315,26 @@ def test_fiedler_vs_oracle_rmat_weighted(ctx):
         assert abs(int(a) - int(b)) <= max(2, ITER_SLACK * int(b))
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_eigenvalue_log_vs_oracle(ctx, weighted):
+    """Final eigenvalue log (eigensolve_kokkos.h:60-100) of the device
+    eigenvector against the oracle's on that same vector; FP64 sums in
+    another order: relative 1e-9 on the sum, 1e-9 on min / max."""
+    rp, col, w = graphs.rmat(13, 16, seed=3)
+    if not weighted:
+        w = None
+    ev, st, _ = ctx.graph(rp, col, w).hierarchy(2).fiedler(55)
+    el = st["eigen_log"]
+    assert el is not None
+    oe, omn, omx = oracle.eigenvalue_log(rp, col, w, ev)
+    assert abs(el["eigenvalue"] - oe) <= 1e-9 * abs(oe)
+    assert abs(el["min"] - omn) <= 1e-9 * max(1.0, abs(omn))
+    assert abs(el["max"] - omx) <= 1e-9 * max(1.0, abs(omx))
+    # normLap solves do not log (eigensolve_kokkos.h:201)
+    _, st1, _ = ctx.graph(rp, col, w).hierarchy(2).fiedler(55, normLap=1)
+    assert st1["eigen_log"] is None
+
+
 @pytest.mark.parametrize("name", ["path40", "path3000", "star", "disconnected", "rgg"])
 def test_partition_graph_edge_cases(ctx, name):
     rng = np.random.default_rng(1)

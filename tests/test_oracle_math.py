@@ -217,6 +217,35 @@ def test_power_iteration_grid_lambda2_closed_form(k):
     assert abs(u.sum()) < 1e-9 and abs(np.linalg.norm(u) - 1) < 1e-12
 
 
+@pytest.mark.parametrize("k", [10, 64])
+def test_eigenvalue_log_closed_form_on_path(k):
+    """For an exact Laplacian eigenvector u (unit norm) of the k-path,
+    (D - A) u = lam u, so the log's sum is lam^2 and every t_i / u_i is lam
+    (lam = 2 - 2 cos(pi / k), u_i = cos(pi (i + 1/2) / k); even k keeps
+    every u_i away from 0)."""
+    rp, col, _ = graphs.path(k)
+    lam = 2 - 2 * np.cos(np.pi / k)
+    u = np.cos(np.pi * (np.arange(k) + 0.5) / k)
+    u /= np.linalg.norm(u)
+    ev, mn, mx = oracle.eigenvalue_log(rp, col, None, u)
+    assert abs(ev - lam * lam) < 1e-13
+    assert abs(mn - lam) < 1e-9 and abs(mx - lam) < 1e-9
+
+
+def test_eigenvalue_log_uses_row_length_and_edge_values():
+    """The degree term is the row length while the product carries the edge
+    values (eigensolve_kokkos.h:68-72); with weights 3 on a 4-cycle and
+    u = e_0: t = (2, -3, 0, -3), so the sum is 4 + 9 + 9 and t_0 / u_0 = 2."""
+    rp = np.array([0, 2, 4, 6, 8], np.uint32)
+    col = np.array([1, 3, 0, 2, 1, 3, 0, 2], np.uint32)
+    w = np.full(8, 3, np.uint32)
+    u = np.array([1.0, 0.0, 0.0, 0.0])
+    ev, mn, mx = oracle.eigenvalue_log(rp, col, w, u)
+    assert abs(ev - 22.0) < 1e-12
+    # -3/0 = -inf, 0/0 = NaN (skipped), 2/1 = 2
+    assert mx == 2.0 and mn == -np.inf
+
+
 def test_power_iteration_matches_eigh_fiedler_vector():
     rng = np.random.default_rng(7)
     rp, col, w = random_graph(rng, 120, 0.05, weighted=True, lcc=True)

@@ -178,3 +178,22 @@ def test_use_partition_and_edit_distance_match_reference():
     d = C.c_uint32()
     REF.L.ref_edit_distance(part, other, 200, C.byref(d))
     assert graphs.partition_edit_distance(part, other) == d.value
+
+
+def test_eigenvalue_log_matches_reference_print():
+    """The reference's serial eigenvalue log (sgpar.h:1758-1788) prints with
+    9 decimals; the oracle follows the Kokkos form (eigensolve_kokkos.h:60-100),
+    the same arithmetic on a unit-weight graph."""
+    import ctypes as C  # noqa: F401
+    rng = np.random.default_rng(4)
+    for n in (50, 300):
+        rp, col, _ = random_graph(rng, n, 0.06)
+        u = rng.standard_normal(n)
+        u /= np.linalg.norm(u) * 40  # keeps the sum printable at 9 decimals
+        out = np.zeros(3)
+        assert REF.L.ref_eigenvalue_log(n, rp, col, u, out) == 0
+        ev, mn, mx = oracle.eigenvalue_log(rp, col, None, u)
+        # the serial form starts max at 0 and min at 2; random u spans both
+        assert mn < 0 and mx > 2
+        for a, b in zip((ev, mn, mx), out):
+            assert abs(a - b) <= 6e-10 + 1e-12 * abs(b)
