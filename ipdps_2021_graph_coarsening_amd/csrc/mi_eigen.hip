@@ -434,15 +434,66 @@ __device__ __forceinline__ uint32_t first_id(const Raw& R)
     return R16 ? (R.r0.x & 0xffffu) : R.r0.x;
 }
 
+// Cross-lane moves of the pass scan as DPP operations (no LDS traffic):
+// row_shr:k shifts within rows of 16 lanes, row_bcast:15 / :31 pass lane 15
+// (47) / lane 31 on to the following row(s), wave_shr:1 / wave_shl:1 shift the
+// whole wavefront by one lane.  A lane whose source is outside the pattern
+// (or whose row is masked off) receives `old`.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143, DPP_WAVE_SHR1 = 0x138, DPP_WAVE_SHL1 = 0x130;
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, 0xf, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v)  // old = 0.0
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const uint32_t lo = dpp_u32<CTRL, ROWMASK>(0u, (uint32_t)b);
+    const uint32_t hi = dpp_u32<CTRL, ROWMASK>(0u, (uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Kogge-Stone step of the segmented scan: add the partial of the lane the
+// pattern names when it carries the same key.  Keys never decrease along
+// the lanes, so an equal key means every lane in between has it too.
+constexpr uint32_t KEY_NONE = 0xfffffffeu;  // never a row id, never RINF
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_scan_step(double& sc, uint32_t key)
+{
+    const double ov = dpp_f64<CTRL, ROWMASK>(sc);
+    const uint32_t ok = dpp_u32<CTRL, ROWMASK>(KEY_NONE, key);
+    if (ok == key) sc = ov + sc;
+}
+
+// Inclusive segmented sum over the 64 lanes (all lanes active).
+__device__ __forceinline__ double seg_scan_wave(double sc, uint32_t key)
+{
+    seg_scan_step<DPP_ROW_SHR1, 0xf>(sc, key);
+    seg_scan_step<DPP_ROW_SHR2, 0xf>(sc, key);
+    seg_scan_step<DPP_ROW_SHR4, 0xf>(sc, key);
+    seg_scan_step<DPP_ROW_SHR8, 0xf>(sc, key);
+    seg_scan_step<DPP_ROW_BCAST15, 0xa>(sc, key);
+    seg_scan_step<DPP_ROW_BCAST31, 0xc>(sc, key);
+    return sc;
+}
+
 // Stores a (virtual) row sum: part[r], or head[wv] for the row that entered
 // the range from the previous one.
 __device__ __forceinline__ void put(const StepArgs& A, uint32_t r, double v, uint32_t hrow, uint32_t wv)
 {
-    if (r == RINF) return;
-    if (r == hrow)
-        A.head[wv] = v;
-    else
-        A.part[r] = v;
+    if (r != RINF) *(r == hrow ? A.head + wv : A.part + r) = v;
 }
 
 // Row sums of the entries [E0, E1) by one wavefront (range index wv; nrow =
@@ -492,73 +543,63 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
         uint32_t nx63 = nrow;
         if (nbase < E1) {
             if (nbase + SEG <= E1)
-                nx63 = __shfl(first_id<R16>(nx), 0, WAVE) + rbase;
+                nx63 = (uint32_t)__builtin_amdgcn_readlane((int)first_id<R16>(nx), 0) + rbase;
             else
                 nx63 = (uint32_t)rid[nbase] + rbase;
         }
+        // gathers for every entry (an idle entry has column 0, in bounds) and
+        // a select, so no branch surrounds the loads
         double pv[SEG];
 #pragma unroll
         for (int u = 0; u < SEG; u++) {
-            double xv;
-            if (LDSX)
-                xv = rr[u] != RINF ? xl[cc[u]] : 0.0;
-            else
-                xv = rr[u] != RINF ? (A.yprev[cc[u]] - a) * s : 0.0;
-            pv[u] = WF == W_UNIT ? xv : (double)ww[u] * xv;
+            const double xv = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
+            const double p = WF == W_UNIT ? xv : (double)ww[u] * xv;
+            pv[u] = rr[u] != RINF ? p : 0.0;
         }
-        // lane-local segments
+        // lane-local segments, as selects: fs = the first segment (it may
+        // continue from earlier lanes), mid_* = segments that begin and end in
+        // this lane, acc = the last segment
         const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
         uint32_t mid_r[SEG - 1];
         double mid_s[SEG - 1];
         double fs = 0.0, acc = pv[0];
-        uint32_t cur = rr[0];
-        bool multi = false;
+        bool multi = false, mids = false;
 #pragma unroll
         for (int u = 1; u < SEG; u++) {
-            mid_r[u - 1] = RINF;
-            mid_s[u - 1] = 0.0;
-            if (rr[u] != cur) {
-                if (!multi) {
-                    fs = acc;  // first segment: may continue from earlier lanes
-                    multi = true;
-                } else {
-                    mid_r[u - 1] = cur;  // begins and ends in this lane
-                    mid_s[u - 1] = acc;
-                }
-                cur = rr[u];
-                acc = pv[u];
-            } else {
-                acc += pv[u];
-            }
+            const bool br = rr[u] != rr[u - 1];
+            const bool mid = br && multi;
+            mid_r[u - 1] = mid ? rr[u - 1] : RINF;
+            mid_s[u - 1] = acc;
+            fs = (br && !multi) ? acc : fs;
+            mids = mids || mid;
+            multi = multi || br;
+            acc = br ? pv[u] : acc + pv[u];
         }
         // segmented inclusive scan of the last segments over the lanes (row
         // ids never decrease along a range, so equal keys are contiguous lanes)
         double sc = acc;
         const uint32_t key = last_row;
         if (lane == 0 && !multi && key == cr) sc = cs + sc;
-#pragma unroll
-        for (int o = 1; o < WAVE; o <<= 1) {
-            const double ov = __shfl_up(sc, o, WAVE);
-            const uint32_t ok = __shfl_up(key, o, WAVE);
-            if (lane >= (uint32_t)o && ok == key) sc = ov + sc;
-        }
-        const double psc = __shfl_up(sc, 1, WAVE);
-        const uint32_t pkey = __shfl_up(key, 1, WAVE);
-        uint32_t nxt = __shfl_down(first_row, 1, WAVE);
-        if (lane == WAVE - 1) nxt = nx63;
+        sc = seg_scan_wave(sc, key);
+        const double psc = dpp_f64<DPP_WAVE_SHR1, 0xf>(sc);
+        const uint32_t pkey = dpp_u32<DPP_WAVE_SHR1, 0xf>(KEY_NONE, key);
+        const uint32_t nxt = dpp_u32<DPP_WAVE_SHL1, 0xf>(nx63, first_row);  // lane 63: nx63
         // a lane's first segment closes inside the lane when it has several
         if (multi) {
             const bool crosses = lane == 0 ? (cr == first_row) : (pkey == first_row);
             const double before = lane == 0 ? cs : psc;
             put(A, first_row, crosses ? before + fs : fs, hrow, wv);
         }
+        // rows wholly inside one lane's 8 entries are rare on long-row levels
+        if (__ballot(mids)) {
 #pragma unroll
-        for (int u = 0; u < SEG - 1; u++) put(A, mid_r[u], mid_s[u], hrow, wv);
+            for (int u = 0; u < SEG - 1; u++) put(A, mid_r[u], mid_s[u], hrow, wv);
+        }
         // the last segment closes at the lane's end when the next entry is another row
         if (nxt != last_row) put(A, last_row, sc, hrow, wv);
         // carry into the next pass
-        const uint32_t k63 = __shfl(key, WAVE - 1, WAVE);
-        const double s63 = __shfl(sc, WAVE - 1, WAVE);
+        const uint32_t k63 = (uint32_t)__builtin_amdgcn_readlane((int)key, WAVE - 1);
+        const double s63 = readlane_f64(sc, WAVE - 1);
         const bool open = (k63 != RINF) && (nx63 == k63);
         cr = open ? k63 : RINF;
         cs = open ? s63 : 0.0;
