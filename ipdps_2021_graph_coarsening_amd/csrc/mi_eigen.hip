@@ -387,8 +387,10 @@ struct Raw {
 
 // Issues the 16-byte loads of the 8 entries at e0 when they all lie below E1
 // (returns false otherwise: the caller loads that lane entry by entry).
+// Positions are 32-bit (the plan keeps m below 2^32 - 2^16, so no position
+// arithmetic of a range wraps).
 template <int WF, bool LDSX, bool R16, class ColT, class RidT, class WT>
-__device__ __forceinline__ bool fetch(const ColT* col, const RidT* rid, const WT* w, uint64_t e0, uint64_t E1, Raw& R)
+__device__ __forceinline__ bool fetch(const ColT* col, const RidT* rid, const WT* w, uint32_t e0, uint32_t E1, Raw& R)
 {
     if (e0 + SEG > E1) return false;
     R.r0 = *reinterpret_cast<const uint4*>(rid + e0);
@@ -457,6 +459,23 @@ __device__ __forceinline__ double dpp_f64(double v)  // old = 0.0
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// All rows enabled, and a lane whose source lies outside its row reads 0
+// (bound_ctrl), so no value has to be set up beforehand.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp0_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp0_f64(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const uint32_t lo = dpp0_u32<CTRL>((uint32_t)b);
+    const uint32_t hi = dpp0_u32<CTRL>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
     const unsigned long long b = __double_as_longlong(v);
@@ -477,13 +496,23 @@ __device__ __forceinline__ void seg_scan_step(double& sc, uint32_t key)
     if (ok == key) sc = ov + sc;
 }
 
+// Row shifts: a lane with no source reads value 0.0 and key 0; should that
+// key match, adding 0.0 changes nothing.
+template <int CTRL>
+__device__ __forceinline__ void seg_scan_shift(double& sc, uint32_t key)
+{
+    const double ov = dpp0_f64<CTRL>(sc);
+    const uint32_t ok = dpp0_u32<CTRL>(key);
+    if (ok == key) sc = ov + sc;
+}
+
 // Inclusive segmented sum over the 64 lanes (all lanes active).
 __device__ __forceinline__ double seg_scan_wave(double sc, uint32_t key)
 {
-    seg_scan_step<DPP_ROW_SHR1, 0xf>(sc, key);
-    seg_scan_step<DPP_ROW_SHR2, 0xf>(sc, key);
-    seg_scan_step<DPP_ROW_SHR4, 0xf>(sc, key);
-    seg_scan_step<DPP_ROW_SHR8, 0xf>(sc, key);
+    seg_scan_shift<DPP_ROW_SHR1>(sc, key);
+    seg_scan_shift<DPP_ROW_SHR2>(sc, key);
+    seg_scan_shift<DPP_ROW_SHR4>(sc, key);
+    seg_scan_shift<DPP_ROW_SHR8>(sc, key);
     seg_scan_step<DPP_ROW_BCAST15, 0xa>(sc, key);
     seg_scan_step<DPP_ROW_BCAST31, 0xc>(sc, key);
     return sc;
@@ -500,7 +529,7 @@ __device__ __forceinline__ void put(const StepArgs& A, uint32_t r, double v, uin
 // row of entry E1 when it continues this block, else RINF).  x_j is read as
 // (yprev[j] - a) s, or from LDS (already shifted and scaled) in the LDS form.
 template <int WF, bool LDSX, bool R16>
-__device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint64_t E0, uint64_t E1,
+__device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint32_t E0, uint32_t E1,
                                              uint32_t nrow, double a, double s, const double* xl)
 {
     using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
@@ -518,9 +547,11 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
     double cs = 0.0;     // its partial sum so far
     // The 16-byte loads of pass p+1 are issued before pass p is processed.
     Raw nx;
-    bool nfull = fetch<WF, LDSX, R16>(col, rid, w, E0 + (uint64_t)lane * SEG, E1, nx);
-    for (uint64_t base = E0; base < E1; base += PASS) {
-        const uint64_t e0 = base + (uint64_t)lane * SEG;
+    bool nfull = fetch<WF, LDSX, R16>(col, rid, w, E0 + lane * SEG, E1, nx);
+    for (uint32_t base = E0; base < E1; base += PASS) {
+        const uint32_t e0 = base + lane * SEG;
+        // a full pass: every lane holds 8 entries (wavefront-uniform)
+        const bool full = base + PASS <= E1;
         uint32_t rr[SEG], cc[SEG], ww[SEG];
         if (nfull) {
             unpack<WF, LDSX, R16>(nx, rr, cc, ww);
@@ -535,8 +566,8 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
                 if (WF != W_UNIT) ww[u] = in ? (uint32_t)w[e0 + u] : 0u;
             }
         }
-        const uint64_t nbase = base + PASS;
-        if (nbase < E1) nfull = fetch<WF, LDSX, R16>(col, rid, w, nbase + (uint64_t)lane * SEG, E1, nx);
+        const uint32_t nbase = base + PASS;
+        if (nbase < E1) nfull = fetch<WF, LDSX, R16>(col, rid, w, nbase + lane * SEG, E1, nx);
         // row of the entry after lane 63's last one: lane 0's first entry of
         // the next pass, or the next range's first row (in a partial pass
         // lane 63 holds no entries)
@@ -549,20 +580,26 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
         }
         // gathers for every entry (an idle entry has column 0, in bounds) and
         // a select, so no branch surrounds the loads
-        double pv[SEG];
+        double xv[SEG];
+        if (full) {
 #pragma unroll
-        for (int u = 0; u < SEG; u++) {
-            const double xv = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
-            const double p = WF == W_UNIT ? xv : (double)ww[u] * xv;
-            pv[u] = rr[u] != RINF ? p : 0.0;
+            for (int u = 0; u < SEG; u++) xv[u] = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
+        } else {
+            // idle entries (row RINF) contribute 0 whatever x holds
+#pragma unroll
+            for (int u = 0; u < SEG; u++) {
+                const double v = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
+                xv[u] = rr[u] != RINF ? v : 0.0;
+            }
         }
         // lane-local segments, as selects: fs = the first segment (it may
         // continue from earlier lanes), mid_* = segments that begin and end in
-        // this lane, acc = the last segment
+        // this lane, acc = the last segment; a boundary restarts the sum
         const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
         uint32_t mid_r[SEG - 1];
         double mid_s[SEG - 1];
-        double fs = 0.0, acc = pv[0];
+        double fs = 0.0;
+        double acc = WF == W_UNIT ? xv[0] : (double)ww[0] * xv[0];
         bool multi = false, mids = false;
 #pragma unroll
         for (int u = 1; u < SEG; u++) {
@@ -573,7 +610,8 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
             fs = (br && !multi) ? acc : fs;
             mids = mids || mid;
             multi = multi || br;
-            acc = br ? pv[u] : acc + pv[u];
+            const double base_acc = br ? 0.0 : acc;
+            acc = WF == W_UNIT ? base_acc + xv[u] : fma((double)ww[u], xv[u], base_acc);
         }
         // segmented inclusive scan of the last segments over the lanes (row
         // ids never decrease along a range, so equal keys are contiguous lanes)
@@ -624,7 +662,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
     const uint64_t E0 = (uint64_t)wv * A.B.pw;
     const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
     const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-    stream_range<WF, false, R16>(A, wv, E0, E1, nrow, a, s, nullptr);
+    stream_range<WF, false, R16>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr);
 }
 
 // Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
@@ -690,8 +728,8 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
         }
         __syncthreads();
         for (uint32_t wv = cur + wid; wv < seg_end; wv += LNW) {
-            const uint64_t E0 = (uint64_t)off0 + (uint64_t)(wv - wbase) * A.B.pw;
-            const uint64_t E1 = (E0 + A.B.pw < off1) ? E0 + A.B.pw : off1;
+            const uint32_t E0 = off0 + (wv - wbase) * A.B.pw;
+            const uint32_t E1 = (off1 - E0 > A.B.pw) ? E0 + A.B.pw : off1;
             const uint32_t nrow = (wv + 1 < wend) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
             stream_range<WF, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
         }
@@ -1105,6 +1143,8 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
     for (int q = 0; q <= MAXCB; q++) p.boff[q] = p.bwave[q] = 0;
     // Entries per range, fixed by the level alone (so the summation order does
     // not depend on the device): two passes on large levels, one on small ones.
+    // the streaming kernels keep entry positions in 32 bits
+    MI_CHECK(m <= 0xffff0000u, ST_ARG, "level too large for 32-bit entry positions");
     p.pw = m >= (1u << 22) ? 2 * PASS : PASS;
     const uint32_t* rows = g.R();  // CSR the row ids and the finish kernel follow
     uint32_t nrows = n;
