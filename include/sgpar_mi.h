@@ -231,6 +231,11 @@ int mi_bench_step(mi_ctx* ctx, const mi_graph* g, int reps, mi_step_bench* out);
 /* Stable LSD radix sort of (key, value) pairs on bits [begin_bit, end_bit), host arrays. */
 int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                         int end_bit);
+/* The permutation sort of generate_permutation (heuristics.h:614-631: a
+   BinSort of 64-bit random draws): stable order of all 64 key bits, taken as
+   four radix passes on the high halves plus the ordering of runs of equal
+   high halves.  Exact for any keys; fast when the keys are random. */
+int mi_sort_random_keys(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n);
 /* Exclusive prefix sum of u32, host arrays; *total receives the sum. */
 int mi_scan_u32(mi_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total);
 

@@ -113,6 +113,7 @@ _SIGS = [
     ("mi_spmv_shift", C.c_int, [vp, vp, f64p, f64p]),
     ("mi_bench_step", C.c_int, [vp, vp, C.c_int, C.c_void_p]),
     ("mi_radix_sort_pairs", C.c_int, [vp, u64p, u32p, C.c_uint64, C.c_int, C.c_int]),
+    ("mi_sort_random_keys", C.c_int, [vp, u64p, u32p, C.c_uint64]),
     ("mi_scan_u32", C.c_int, [vp, u32p, u32p, C.c_uint64, C.POINTER(C.c_uint32)]),
 ]
 
@@ -195,6 +196,13 @@ class Context:
         v = np.array(vals, dtype=np.uint32, copy=True)
         self.check(self._L.mi_radix_sort_pairs(self.h, k, v, len(k), begin_bit, end_bit),
                    "mi_radix_sort_pairs")
+        return k, v
+
+    def sort_random_keys(self, keys, vals):
+        """Stable 64-bit sort as used for the HEC permutation (mi_sort_random_keys)."""
+        k = np.array(keys, dtype=np.uint64, copy=True)
+        v = np.array(vals, dtype=np.uint32, copy=True)
+        self.check(self._L.mi_sort_random_keys(self.h, k, v, len(k)), "mi_sort_random_keys")
         return k, v
 
     def scan_u32(self, a):

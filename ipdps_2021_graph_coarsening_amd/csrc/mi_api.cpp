@@ -817,6 +817,22 @@ int mi_radix_sort_pairs(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n,
     });
 }
 
+int mi_sort_random_keys(mi_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n)
+{
+    if (!ctx || (n && (!keys || !vals))) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        DBuf<uint64_t> k(&c.pool, n), kt(&c.pool, n);
+        DBuf<uint32_t> v(&c.pool, n), vt(&c.pool, n);
+        h2d(c, k.get(), keys, n * 8);
+        h2d(c, v.get(), vals, n * 4);
+        sort_random_keys(c, k.get(), v.get(), n, kt.get(), vt.get());
+        d2h(c, keys, k.get(), n * 8);
+        d2h(c, vals, v.get(), n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
 int mi_scan_u32(mi_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total)
 {
     if (!ctx || (n && (!in || !out))) return ST_ARG;

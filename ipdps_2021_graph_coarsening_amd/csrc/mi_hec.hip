@@ -382,7 +382,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     DBuf<uint32_t> vperm(&c.pool, n), vtmp(&c.pool, n), pos(&c.pool, n);
     MI_HIP(hipMemcpyAsync(keys.get(), draws.get(), (size_t)n * 8, hipMemcpyDeviceToDevice, c.stream));
     iota_u32(c, vperm.get(), n, 0);
-    radix_sort_pairs(c, keys.get(), vperm.get(), n, 0, 64, ktmp.get(), vtmp.get());
+    sort_random_keys(c, keys.get(), vperm.get(), n, ktmp.get(), vtmp.get());
     k_scatter_pos<<<grid1(n), BLOCK, 0, c.stream>>>(vperm.get(), n, pos.get());
     MI_HIP(hipGetLastError());
     keys.reset();

@@ -86,6 +86,11 @@ void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int be
                       int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
 void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                       int end_bit, uint32_t* keys_tmp, uint32_t* vals_tmp);
+// Stable sort of (key, value) pairs on all 64 key bits, for random keys:
+// four radix passes on the high halves, then runs of equal high halves
+// ordered by the whole key (the same result as eight passes; slow only when
+// many keys share their high half, which random keys do not).
+void sort_random_keys(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, uint64_t* keys_tmp, uint32_t* vals_tmp);
 
 // ---------------------------------------------------------------- RNG / HEC
 // Host: state of the single generator of Kokkos' Random_XorShift64_Pool(seed)

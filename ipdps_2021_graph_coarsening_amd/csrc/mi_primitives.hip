@@ -321,10 +321,50 @@ static void radix_sort_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int beg
     }
 }
 
+namespace {
+
+// After a stable sort on the high 32 bits, runs of keys equal there are put
+// in order of the whole key with an insertion sort (stable: equal keys keep
+// their order), which completes the stable 64-bit order.  One thread per
+// run head.  For random 64-bit keys such runs are rare and short; the result
+// is exact for any keys (long runs only cost time).
+__global__ void k_order_hi_runs(uint64_t n, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint32_t hi = (uint32_t)(keys[i] >> 32);
+    if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == hi) return;  // not the head of its run
+    if ((uint32_t)(keys[i + 1] >> 32) != hi) return;            // a run of one
+    uint64_t e = i + 2;
+    while (e < n && (uint32_t)(keys[e] >> 32) == hi) e++;
+    for (uint64_t a = i + 1; a < e; a++) {
+        const uint64_t k = keys[a];
+        const uint32_t v = vals[a];
+        uint64_t b = a;
+        while (b > i && keys[b - 1] > k) {
+            keys[b] = keys[b - 1];
+            vals[b] = vals[b - 1];
+            b--;
+        }
+        keys[b] = k;
+        vals[b] = v;
+    }
+}
+
+}  // namespace
+
 void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
                       uint64_t* keys_tmp, uint32_t* vals_tmp)
 {
     radix_sort_impl<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void sort_random_keys(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, uint64_t* keys_tmp, uint32_t* vals_tmp)
+{
+    if (n < 2) return;
+    radix_sort_impl<uint64_t>(c, keys, vals, n, 32, 64, keys_tmp, vals_tmp);
+    k_order_hi_runs<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, c.stream>>>(n, keys, vals);
+    MI_HIP(hipGetLastError());
 }
 
 void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,

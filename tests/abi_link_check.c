@@ -48,6 +48,7 @@ int main(void)
     int (*fp_mi_spmv_shift)(mi_ctx*, const mi_graph*, const double*, double*);
     int (*fp_mi_bench_step)(mi_ctx*, const mi_graph*, int, mi_step_bench*);
     int (*fp_mi_radix_sort_pairs)(mi_ctx*, uint64_t*, uint32_t*, uint64_t, int, int);
+    int (*fp_mi_sort_random_keys)(mi_ctx*, uint64_t*, uint32_t*, uint64_t);
     int (*fp_mi_scan_u32)(mi_ctx*, const uint32_t*, uint32_t*, uint64_t, uint32_t*);
 
     BIND(mi_ctx_create);
@@ -78,6 +79,7 @@ int main(void)
     BIND(mi_spmv_shift);
     BIND(mi_bench_step);
     BIND(mi_radix_sort_pairs);
+    BIND(mi_sort_random_keys);
     BIND(mi_scan_u32);
 
     int bad = 0;

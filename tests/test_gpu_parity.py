@@ -51,6 +51,23 @@ def test_radix_sort(ctx, n, b, e):
     assert np.array_equal(v, order.astype(np.uint32)) and np.array_equal(k, keys[order])
 
 
+@pytest.mark.parametrize("n,hi_values", [(5000, 40), (20000, 3000), (600, 1)])
+def test_sort_random_keys_equal_high_halves(ctx, n, hi_values):
+    """The permutation sort runs on the high halves and then orders the runs
+    of equal high halves by the whole key; many and long such runs, ties on
+    the whole key included, must give the stable 64-bit order."""
+    rng = np.random.default_rng(n + hi_values)
+    hi = rng.integers(0, hi_values, n).astype(np.uint64)
+    lo = rng.integers(0, 2**32, n, dtype=np.int64).astype(np.uint64)
+    keys = (hi << np.uint64(32)) | lo
+    dup = np.arange(0, n - 1, 7)
+    keys[dup] = keys[dup + 1]
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.sort_random_keys(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, order.astype(np.uint32)) and np.array_equal(k, keys[order])
+
+
 @pytest.mark.parametrize("seed,first,count", [(12345, 0, 5000), (0, 0, 1000), (7, 123457, 3000)])
 def test_xs_draws(ctx, seed, first, count):
     d = ctx.xs_draws(seed, first, count)
