@@ -13,6 +13,8 @@
 // u32 atomic per run per wavefront -- exact and order-free.
 #include "mi_kernels.hpp"
 
+#include <utility>
+
 namespace mi {
 
 namespace {
@@ -124,7 +126,17 @@ static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, cons
                                                           vals.get());
         MI_HIP(hipGetLastError());
     }
-    radix_sort_pairs(c, keys.get(), vals.get(), m, 0, 2 * B, ktmp.get(), vtmp.get());
+    {
+        K* kp = keys.get();
+        uint32_t* vp = vals.get();
+        K* kt = ktmp.get();
+        uint32_t* vt = vtmp.get();
+        radix_sort_pairs_ping(c, kp, vp, m, 0, 2 * B, kt, vt);
+        if (kp != keys.get()) {  // the sorted pairs are in the second buffers
+            std::swap(keys, ktmp);
+            std::swap(vals, vtmp);
+        }
+    }
     ktmp.reset();
     vtmp.reset();
 

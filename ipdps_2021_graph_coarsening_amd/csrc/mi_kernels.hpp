@@ -86,6 +86,12 @@ void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int be
                       int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
 void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                       int end_bit, uint32_t* keys_tmp, uint32_t* vals_tmp);
+// The same without the final copy: on return keys / vals point to whichever
+// buffer pair holds the sorted pairs and keys_tmp / vals_tmp to the other.
+void radix_sort_pairs_ping(Ctx& c, uint64_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                           uint64_t*& keys_tmp, uint32_t*& vals_tmp);
+void radix_sort_pairs_ping(Ctx& c, uint32_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                           uint32_t*& keys_tmp, uint32_t*& vals_tmp);
 // Stable sort of (key, value) pairs on all 64 key bits, for random keys:
 // four radix passes on the high halves, then runs of equal high halves
 // ordered by the whole key (the same result as eight passes; slow only when

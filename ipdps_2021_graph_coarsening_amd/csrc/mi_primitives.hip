@@ -98,30 +98,37 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down(const uint32_t* in, uint32_
 }
 
 // ---------------------------------------------------------------- radix
-// Stable LSD radix sort of (key, value) pairs, 8-bit digits, tiles of 4096.
-// Per pass: a per-tile digit histogram, one scan over (digit, tile), then
-// the scatter.  In the scatter each wavefront ranks its own contiguous
-// quarter of the tile with ballots (no barrier per round, stable by
-// construction), the tile is placed in LDS in digit order, and written out
-// so that each digit's run goes to consecutive addresses.
-constexpr int RADIX_BITS = 8;
-constexpr int RADIX = 1 << RADIX_BITS;
+// Stable LSD radix sort of (key, value) pairs.  A pass sorts on a digit of
+// up to DB bits (DB = 8: 256 threads, or 9: 512 threads; one digit per
+// thread in the tile bookkeeping) over tiles of 16 items per thread.  Per
+// pass: a per-tile digit histogram, one scan over (digit, tile), then the
+// scatter.  In the scatter each wavefront ranks its own contiguous share of
+// the tile with ballots (no barrier per round, stable by construction), the
+// tile is placed in LDS in digit order, and written out so that each digit's
+// run goes to consecutive addresses.
 constexpr int R_ITEMS = 16;
-constexpr int R_TILE = BLOCK * R_ITEMS;  // 4096
-constexpr int R_WAVES = BLOCK / WAVE;
-static_assert(RADIX == BLOCK, "one digit per thread in the tile bookkeeping");
 
-template <class K>
-__global__ void __launch_bounds__(BLOCK) k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
-                                                      uint32_t mask, uint32_t num_tiles, uint32_t* __restrict__ hist)
+template <int DB>
+struct RadixShape {
+    static constexpr int RADIX = 1 << DB;
+    static constexpr int THREADS = RADIX;  // one digit per thread
+    static constexpr int TILE = THREADS * R_ITEMS;
+    static constexpr int WAVES = THREADS / WAVE;
+};
+
+template <class K, int DB>
+__global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_hist(const K* __restrict__ keys, uint64_t n,
+                                                                        int shift, uint32_t mask, uint32_t num_tiles,
+                                                                        uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t h[RADIX];
+    using S = RadixShape<DB>;
+    __shared__ uint32_t h[S::RADIX];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * R_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * S::TILE;
 #pragma unroll 4
     for (int k = 0; k < R_ITEMS; k++) {
-        uint64_t i = base + (uint64_t)k * BLOCK + threadIdx.x;
+        uint64_t i = base + (uint64_t)k * S::THREADS + threadIdx.x;
         if (i < n) {
             uint32_t d = (uint32_t)(keys[i] >> shift) & mask;
             atomicAdd(&h[d], 1u);
@@ -131,26 +138,27 @@ __global__ void __launch_bounds__(BLOCK) k_radix_hist(const K* __restrict__ keys
     hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-template <class K>
-__global__ void __launch_bounds__(BLOCK) k_radix_scatter(const K* __restrict__ keys_in,
-                                                         const uint32_t* __restrict__ vals_in,
-                                                         K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
-                                                         uint64_t n, int shift, uint32_t mask, uint32_t num_tiles,
-                                                         const uint32_t* __restrict__ offs)
+template <class K, int DB>
+__global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, uint64_t n, int shift, uint32_t mask, uint32_t num_tiles,
+    const uint32_t* __restrict__ offs)
 {
-    __shared__ K skey[R_TILE];
-    __shared__ uint32_t sval[R_TILE];
-    __shared__ uint32_t wrun[R_WAVES][RADIX];  // per wavefront: items of each digit so far
-    __shared__ uint32_t lstart[RADIX];         // first LDS slot of each digit
-    __shared__ uint32_t gofs[RADIX];           // first output position of each digit
-    __shared__ uint32_t wsum[R_WAVES];
+    using S = RadixShape<DB>;
+    constexpr int RADIX = S::RADIX, WAVES = S::WAVES, TILE = S::TILE;
+    __shared__ K skey[TILE];
+    __shared__ uint32_t sval[TILE];
+    __shared__ uint32_t wrun[WAVES][RADIX];  // per wavefront: items of each digit so far
+    __shared__ uint32_t lstart[RADIX];       // first LDS slot of each digit
+    __shared__ uint32_t gofs[RADIX];         // first output position of each digit
+    __shared__ uint32_t wsum[WAVES];
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     gofs[threadIdx.x] = offs[(uint64_t)threadIdx.x * num_tiles + blockIdx.x];
 #pragma unroll
-    for (int w = 0; w < R_WAVES; w++) wrun[w][threadIdx.x] = 0;
+    for (int w = 0; w < WAVES; w++) wrun[w][threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * R_TILE;
-    const uint64_t wbase = base + (uint64_t)wid * (R_TILE / R_WAVES);  // this wavefront's quarter
+    const uint64_t base = (uint64_t)blockIdx.x * TILE;
+    const uint64_t wbase = base + (uint64_t)wid * (TILE / WAVES);  // this wavefront's share
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (WAVE - lane));
     K kk[R_ITEMS];
     uint32_t vv[R_ITEMS], rk[R_ITEMS], dd[R_ITEMS];
@@ -168,7 +176,7 @@ __global__ void __launch_bounds__(BLOCK) k_radix_scatter(const K* __restrict__ k
         const bool active = d < (uint32_t)RADIX;
         unsigned long long peers = __ballot(active);
 #pragma unroll
-        for (int b = 0; b < RADIX_BITS; b++) {
+        for (int b = 0; b < DB; b++) {
             const bool bit = (d >> b) & 1u;
             const unsigned long long bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
@@ -183,7 +191,7 @@ __global__ void __launch_bounds__(BLOCK) k_radix_scatter(const K* __restrict__ k
     // digit d: the count over the tile, and each wavefront's offset inside it
     uint32_t tot = 0;
 #pragma unroll
-    for (int w = 0; w < R_WAVES; w++) {
+    for (int w = 0; w < WAVES; w++) {
         const uint32_t c = wrun[w][threadIdx.x];
         wrun[w][threadIdx.x] = tot;
         tot += c;
@@ -211,8 +219,8 @@ __global__ void __launch_bounds__(BLOCK) k_radix_scatter(const K* __restrict__ k
         }
     }
     __syncthreads();
-    const uint32_t cnt = (n - base) < (uint64_t)R_TILE ? (uint32_t)(n - base) : (uint32_t)R_TILE;
-    for (uint32_t p = threadIdx.x; p < cnt; p += BLOCK) {
+    const uint32_t cnt = (n - base) < (uint64_t)TILE ? (uint32_t)(n - base) : (uint32_t)TILE;
+    for (uint32_t p = threadIdx.x; p < cnt; p += S::THREADS) {
         const K key = skey[p];
         const uint32_t d = (uint32_t)(key >> shift) & mask;
         const uint32_t gp = gofs[d] + (p - lstart[d]);
@@ -290,35 +298,87 @@ void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start)
     MI_HIP(hipGetLastError());
 }
 
+template <class K, int DB>
+static void radix_pass(Ctx& c, const K* ksrc, const uint32_t* vsrc, K* kdst, uint32_t* vdst, uint64_t n, int shift,
+                       int bits, DBuf<uint32_t>& hist)
+{
+    using S = RadixShape<DB>;
+    const uint64_t tiles = ceil_div(n, S::TILE);
+    MI_CHECK(tiles * S::RADIX < (1ull << 32), ST_ARG, "radix sort too large");
+    if (hist.size() < tiles * S::RADIX) hist = DBuf<uint32_t>(&c.pool, tiles * S::RADIX);
+    const uint32_t mask = (1u << bits) - 1u;
+    k_radix_hist<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles,
+                                                                     hist.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, hist.get(), hist.get(), tiles * S::RADIX, nullptr);
+    k_radix_scatter<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
+                                                                        (uint32_t)tiles, hist.get());
+    MI_HIP(hipGetLastError());
+}
+
+// Sorts on bits [begin_bit, end_bit) in the fewest passes of at most 9 bits
+// (8-bit passes when that takes no more of them), the widths split evenly.
+// The result ends in whichever buffer pair the last pass wrote: on return
+// keys / vals point to it and keys_tmp / vals_tmp to the other pair.
+template <class K>
+static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit, K*& keys_tmp,
+                            uint32_t*& vals_tmp)
+{
+    MI_CHECK(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit, ST_ARG, "radix bit range");
+    if (n <= 1 || begin_bit == end_bit) return;
+    const int total = end_bit - begin_bit;
+    const int p8 = (total + 7) / 8, p9 = (total + 8) / 9;
+    const bool wide = p9 < p8;
+    const int passes = wide ? p9 : p8;
+    DBuf<uint32_t> hist;
+    int shift = begin_bit;
+    for (int q = 0; q < passes; q++) {
+        const int bits = (total - (shift - begin_bit) + (passes - q) - 1) / (passes - q);  // even split
+        if (wide)
+            radix_pass<K, 9>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist);
+        else
+            radix_pass<K, 8>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist);
+        std::swap(keys, keys_tmp);
+        std::swap(vals, vals_tmp);
+        shift += bits;
+    }
+}
+
 template <class K>
 static void radix_sort_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit, K* keys_tmp,
                             uint32_t* vals_tmp)
 {
-    MI_CHECK(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit, ST_ARG, "radix bit range");
-    if (n <= 1 || begin_bit == end_bit) return;
-    const uint64_t tiles = ceil_div(n, R_TILE);
-    MI_CHECK(tiles * RADIX < (1ull << 32), ST_ARG, "radix sort too large");
-    DBuf<uint32_t> hist(&c.pool, tiles * RADIX);
-    K* ksrc = keys;
-    uint32_t* vsrc = vals;
-    K* kdst = keys_tmp;
-    uint32_t* vdst = vals_tmp;
-    for (int shift = begin_bit; shift < end_bit; shift += RADIX_BITS) {
-        const int bits = (end_bit - shift) < RADIX_BITS ? (end_bit - shift) : RADIX_BITS;
-        const uint32_t mask = (1u << bits) - 1u;
-        k_radix_hist<K><<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles, hist.get());
-        MI_HIP(hipGetLastError());
-        scan_exclusive_u32(c, hist.get(), hist.get(), tiles * RADIX, nullptr);
-        k_radix_scatter<K><<<(unsigned)tiles, BLOCK, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
-                                                                   (uint32_t)tiles, hist.get());
-        MI_HIP(hipGetLastError());
-        std::swap(ksrc, kdst);
-        std::swap(vsrc, vdst);
+    K* k = keys;
+    uint32_t* v = vals;
+    radix_sort_ping<K>(c, k, v, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+    if (k != keys) {
+        MI_HIP(hipMemcpyAsync(keys, k, n * sizeof(K), hipMemcpyDeviceToDevice, c.stream));
+        MI_HIP(hipMemcpyAsync(vals, v, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
     }
-    if (ksrc != keys) {
-        MI_HIP(hipMemcpyAsync(keys, ksrc, n * sizeof(K), hipMemcpyDeviceToDevice, c.stream));
-        MI_HIP(hipMemcpyAsync(vals, vsrc, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
-    }
+}
+
+void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
+                      uint64_t* keys_tmp, uint32_t* vals_tmp)
+{
+    radix_sort_impl<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
+                      uint32_t* keys_tmp, uint32_t* vals_tmp)
+{
+    radix_sort_impl<uint32_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void radix_sort_pairs_ping(Ctx& c, uint64_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                           uint64_t*& keys_tmp, uint32_t*& vals_tmp)
+{
+    radix_sort_ping<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void radix_sort_pairs_ping(Ctx& c, uint32_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                           uint32_t*& keys_tmp, uint32_t*& vals_tmp)
+{
+    radix_sort_ping<uint32_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
 }
 
 namespace {
@@ -353,12 +413,6 @@ __global__ void k_order_hi_runs(uint64_t n, uint64_t* __restrict__ keys, uint32_
 
 }  // namespace
 
-void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
-                      uint64_t* keys_tmp, uint32_t* vals_tmp)
-{
-    radix_sort_impl<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
-}
-
 void sort_random_keys(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, uint64_t* keys_tmp, uint32_t* vals_tmp)
 {
     if (n < 2) return;
@@ -367,10 +421,6 @@ void sort_random_keys(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, uint64
     MI_HIP(hipGetLastError());
 }
 
-void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit, int end_bit,
-                      uint32_t* keys_tmp, uint32_t* vals_tmp)
-{
-    radix_sort_impl<uint32_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
-}
+
 
 }  // namespace mi

@@ -37,7 +37,8 @@ def test_scan(ctx, n):
     assert np.array_equal(out, ref) and tot == int(a.sum())
 
 
-@pytest.mark.parametrize("n,b,e", [(1, 0, 64), (5000, 0, 64), (100003, 0, 38), (70000, 8, 40), (4096 * 3, 0, 8)])
+@pytest.mark.parametrize("n,b,e", [(1, 0, 64), (5000, 0, 64), (100003, 0, 38), (70000, 8, 40), (4096 * 3, 0, 8),
+                                   (100003, 0, 34), (50000, 3, 29), (8193, 0, 9), (20000, 5, 22)])
 def test_radix_sort(ctx, n, b, e):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2)
