@@ -66,7 +66,8 @@ constexpr int LBLOCK = 1024;            // workgroup of the LDS form
 constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
 constexpr int MAXCB = 8;                // column blocks at most
-constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most
+constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most (global form)
+constexpr uint32_t MAXF_LDS = LBLOCK;   // LDS form: its 1024-thread workgroups reduce up to 1024 sums
 constexpr int W_UNIT = 0, W_U16 = 1, W_U32 = 2;  // weight formats of the streamed arrays
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
@@ -1281,8 +1282,10 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
     p.part = DBuf<double>(&c.pool, (size_t)nrows);
     p.head = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     p.tail = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
-    p.nblk_f = (uint32_t)std::min<uint64_t>(MAXF, ceil_div(n, BLOCK));
-    p.fsum = DBuf<double>(&c.pool, 2ull * 4 * MAXF);
+    // one row per finish thread where the streaming kernel can reduce that
+    // many workgroup sums (each thread's rows form a dependent load chain)
+    p.nblk_f = (uint32_t)std::min<uint64_t>(ncb ? MAXF_LDS : MAXF, ceil_div(n, BLOCK));
+    p.fsum = DBuf<double>(&c.pool, 2ull * 4 * MAXF_LDS);
 }
 
 void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg, uint32_t* gb_dev)
@@ -1421,8 +1424,8 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     for (int k = 0; k < 2; k++) {
         A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
         A[k].hdone = c.host_mapped_dev + 64;
-        A[k].fprev = plan.fsum.get() + 4 * MAXF * (k ^ 1);  // sums of the previous launch
-        A[k].fcur = plan.fsum.get() + 4 * MAXF * k;
+        A[k].fprev = plan.fsum.get() + 4 * MAXF_LDS * (k ^ 1);  // sums of the previous launch
+        A[k].fcur = plan.fsum.get() + 4 * MAXF_LDS * k;
     }
     // Every launch of this level has the same arguments but for the ping-pong
     // parity (the iteration index lives on the device), so a chunk of
