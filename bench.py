@@ -19,7 +19,11 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a copy of
 known size (profiles/k_power_stream_rmat20_L1_traffic.txt), or null when
 that summary is absent.  `cpu_baseline` times the CPU oracle (a scalar
 restatement of the reference's Serial path) on a bounded sample and scales
-it to one whole bisection.
+it to one whole bisection.  `mixed` is the same bisection in mixed precision
+(FP32 vectors on the levels above the finest with at least 2^20 entries,
+FP64 sums, deflation, stopping test and finest level) beside the FP64 one;
+`mixed_grid3d` the same comparison on the 128^3 7-point grid with weights
+U{1..16}.
 
     python bench.py [--steps K] [--warmup W]
     python bench.py --roofline-only      # only the roofline measurement (for rocprofv3 runs)
@@ -97,6 +101,37 @@ def cpu_baseline(rp, col, w, gpu_stats, budget_s: float = 20.0) -> dict:
                        f"(per-iteration time x the GPU run's iteration counts) + split {t_split:.2f} s")}
 
 
+def mixed_summary(ms64: float, st64: dict, ms32: float, st32: dict) -> dict:
+    """Mixed-precision bisection beside the plain FP64 one on the same graph."""
+    return {"ms": round(ms32, 3), "fp64_ms": round(ms64, 3), "speedup": round(ms64 / ms32, 4),
+            "edge_cut": st32["edge_cut"], "fp64_edge_cut": st64["edge_cut"],
+            "cut_change_pct": round(100.0 * (st32["edge_cut"] - st64["edge_cut"]) / max(st64["edge_cut"], 1), 4),
+            "fp32_levels": st32["mixed_levels"], "level_iters": st32["level_iters"],
+            "fp64_level_iters": st64["level_iters"],
+            "level_eigen_ms": [round(x, 3) for x in st32["level_eigen_ms"]]}
+
+
+def grid3d_mixed(mi, ctx, steps: int) -> dict:
+    """The 128^3 7-point grid with weights U{1..16} (seed 2): FP64 and
+    mixed-precision bisections, timed on the host clock around `steps` calls."""
+    from ipdps_2021_graph_coarsening_amd import graphs
+    rp, col, w = graphs.grid3d(128, seed=2)
+    g = ctx.graph(rp, col, w)
+    res = {}
+    for mixed in (False, True):
+        ctx.set_precision(mixed)
+        r = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
+        res[mixed] = ((time.perf_counter() - t0) * 1000.0 / steps, r.stats)
+    ctx.set_precision(False)
+    g.close()
+    out = mixed_summary(res[False][0], res[False][1], res[True][0], res[True][1])
+    out["workload"] = f"grid3d-128 (n={len(rp) - 1}, m={int(rp[-1])}, weights U1..16, seed 2)"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,20 +189,29 @@ def main() -> int:
                                            C.byref(ec), C.byref(st)), "mi_partition_graph")
             return st
 
-        for i in range(args.warmup):
-            st = step()
-            log(f"warmup {i}: {st.total_ms:.2f} ms, cut {st.edge_cut}, levels {st.nlevels}")
-        hip.hipDeviceSynchronize()
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            st = step()
-        hip.hipDeviceSynchronize()
-        el = time.perf_counter() - t0
-        ms = el * 1000.0 / args.steps
-        stats = st.as_dict()
-        hip.hipFree(d_part)
+        def timed():
+            for i in range(args.warmup):
+                st = step()
+                log(f"warmup {i}: {st.total_ms:.2f} ms, cut {st.edge_cut}, levels {st.nlevels}")
+            hip.hipDeviceSynchronize()
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                st = step()
+            hip.hipDeviceSynchronize()
+            return (time.perf_counter() - t0) * 1000.0 / args.steps, st.as_dict()
+
+        ms, stats = timed()
         log(f"timed {args.steps} steps: {ms:.2f} ms/bisection, cut {stats['edge_cut']}, "
             f"iterations per level {stats['level_iters']}")
+        # the same bisection in mixed precision (FP32 on the levels above the
+        # finest with at least 2^20 entries, FP64 finest level)
+        ctx.set_precision(True)
+        ms_mixed, st_mixed = timed()
+        ctx.set_precision(False)
+        hip.hipFree(d_part)
+        mixed = mixed_summary(ms, stats, ms_mixed, st_mixed)
+        log(f"mixed precision: {ms_mixed:.2f} ms/bisection, cut {st_mixed['edge_cut']}, "
+            f"iterations per level {st_mixed['level_iters']}")
 
     # roofline of the dominant kernel: the streaming kernel of the power step
     # on the level with the most eigensolve time
@@ -210,6 +254,13 @@ def main() -> int:
         print(json.dumps({"roofline": roofline, "calibration_copy_bytes": nb}), flush=True)
         return 0
 
+    g3 = None
+    if not args.roofline_only:
+        t = time.perf_counter()
+        g3 = grid3d_mixed(mi, ctx, max(1, min(args.steps, 5)))
+        log(f"grid3d-128: FP64 {g3['fp64_ms']:.2f} ms, mixed {g3['ms']:.2f} ms, cut change "
+            f"{g3['cut_change_pct']}% ({time.perf_counter() - t:.1f} s)")
+
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
@@ -240,6 +291,8 @@ def main() -> int:
                    "level_eigen_ms": [round(x, 3) for x in stats["level_eigen_ms"]]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "mixed_grid3d": g3,
     }
     print(json.dumps(out), flush=True)
     g.close()

@@ -72,11 +72,15 @@ typedef struct {
        computed (refine_alg 1, or a call without a final level). */
     int has_eigen_log;
     double eigenvalue, eigen_min, eigen_max;
+    int mixed;  /* levels whose power iteration ran with FP32 vectors */
 } mi_stats;
 
 /* ---------------------------------------------------------------- context */
 /* Creates a context on HIP device `device` (its stream, pool, workspace). */
 int mi_ctx_create(int device, mi_ctx** out);
+/* Ends the caller's use of ctx.  Graphs and hierarchies made on it stay
+   valid for mi_graph_destroy / mi_hier_destroy, in any order; the context's
+   memory goes when the last of them does. */
 void mi_ctx_destroy(mi_ctx* ctx);
 /* Text of the last failure on this context ("" if none). */
 const char* mi_last_error(const mi_ctx* ctx);
@@ -86,6 +90,19 @@ int mi_ctx_trim(mi_ctx* ctx);
    captured once per level; 0: every kernel is launched from the host.
    Results are identical either way.  No counterpart in the reference. */
 int mi_ctx_set_graphs(mi_ctx* ctx, int enable);
+/* mixed != 0: mixed-precision eigensolve.  On every level above the finest
+   with at least min_entries stored entries (0 selects 2^20: smaller levels
+   are bound by launch and reduction latency, not by bytes, and stay FP64)
+   the power iteration keeps
+   its vectors, row partials and products in FP32 while the weighted degrees
+   (integers), the shift, the five sums behind the deflation, the norm and
+   the dot product (the dot normalised by the norm of the rounded x), and the
+   stopping test stay FP64; it stops at the configured tolerance or after
+   20000 iterations.  The finest level is always solved in FP64: the final
+   refinement.  mi_power_iter (one level) runs in FP32 under this setting
+   whatever the level's size.  mixed == 0 (the default): FP64 throughout, as
+   the reference.  No counterpart in the reference. */
+int mi_ctx_set_precision(mi_ctx* ctx, int mixed, uint32_t min_entries);
 
 /* ---------------------------------------------------------------- graphs */
 /*

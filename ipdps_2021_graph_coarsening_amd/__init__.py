@@ -59,7 +59,8 @@ class Stats(C.Structure):
                 ("map_ms", C.c_double), ("build_ms", C.c_double),
                 ("edge_cut", C.c_uint32),
                 ("has_eigen_log", C.c_int),
-                ("eigenvalue", C.c_double), ("eigen_min", C.c_double), ("eigen_max", C.c_double)]
+                ("eigenvalue", C.c_double), ("eigen_min", C.c_double), ("eigen_max", C.c_double),
+                ("mixed", C.c_int)]
 
     def as_dict(self) -> dict:
         L = self.nlevels
@@ -72,7 +73,8 @@ class Stats(C.Structure):
                     permute_ms=self.permute_ms, heavy_ms=self.heavy_ms, map_ms=self.map_ms,
                     build_ms=self.build_ms, edge_cut=self.edge_cut,
                     eigen_log=(dict(eigenvalue=self.eigenvalue, min=self.eigen_min, max=self.eigen_max)
-                               if self.has_eigen_log else None))
+                               if self.has_eigen_log else None),
+                    mixed_levels=int(self.mixed))
 
 
 _lib = None
@@ -84,6 +86,7 @@ _SIGS = [
     ("mi_last_error", C.c_char_p, [vp]),
     ("mi_ctx_trim", C.c_int, [vp]),
     ("mi_ctx_set_graphs", C.c_int, [vp, C.c_int]),
+    ("mi_ctx_set_precision", C.c_int, [vp, C.c_int, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -176,6 +179,11 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_precision(self, mixed: bool, min_entries: int = 0) -> None:
+        """Mixed-precision eigensolve (FP32 on levels above the finest with at
+        least min_entries entries, 0: 2^20; FP64 finest level) or plain FP64."""
+        self.check(self._L.mi_ctx_set_precision(self.h, 1 if mixed else 0, min_entries), "mi_ctx_set_precision")
 
     def set_graphs(self, enable: bool) -> None:
         """Replay power iterations from a captured HIP graph (default) or launch eagerly."""

@@ -6,6 +6,9 @@
 #include "../../include/sgpar_mi.h"
 #include "mi_kernels.hpp"
 
+#include <algorithm>
+#include <atomic>
+
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -147,17 +150,52 @@ void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes)
 
 using namespace mi;
 
+// A context lives until mi_ctx_destroy has been called and every graph and
+// hierarchy made on it is destroyed too: their device buffers return to its
+// pool, so they may be destroyed in any order.
 struct mi_ctx {
     Ctx c;
+    std::atomic<int> refs{1};
 };
 
+namespace {
+
+void ctx_release(mi_ctx* ctx)
+{
+    if (ctx && ctx->refs.fetch_sub(1) == 1) {
+        (void)hipSetDevice(ctx->c.device);
+        delete ctx;
+    }
+}
+
+// The owning context of a graph or hierarchy.  Declared first in those
+// objects, so it is released after their buffers.
+struct CtxRef {
+    mi_ctx* p = nullptr;
+    CtxRef() = default;
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    CtxRef& operator=(mi_ctx* c)
+    {
+        if (c) c->refs.fetch_add(1);
+        ctx_release(p);
+        p = c;
+        return *this;
+    }
+    ~CtxRef() { ctx_release(p); }
+    mi_ctx* operator->() const { return p; }
+    explicit operator bool() const { return p != nullptr; }
+};
+
+}  // namespace
+
 struct mi_graph {
-    mi_ctx* ctx = nullptr;
+    CtxRef ctx;
     DGraph g;
 };
 
 struct mi_hier {
-    mi_ctx* ctx = nullptr;
+    CtxRef ctx;
     std::vector<DLevel> lv;
 };
 
@@ -281,7 +319,10 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
 {
     const int L = (int)lv.size();
     MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
-    if (st) st->has_eigen_log = 0;
+    if (st) {
+        st->has_eigen_log = 0;
+        st->mixed = 0;
+    }
     const uint32_t gn = lv[L - 1].g.n;
     std::vector<double> cg(gn);
     // eigensolve_kokkos.h:1500-1505: pcg32()/UINT32_MAX, then normalise
@@ -302,7 +343,11 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
         // with the Laplacian (eigensolve_kokkos.h:201-203)
         double elog[3];
         const bool log_here = st && l == 0 && !normLap;
-        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps, log_here ? elog : nullptr);
+        // mixed precision: FP32 iterations on the large levels above the
+        // finest, whose FP64 solve is the final refinement
+        const bool mixed = c.mixed && l > 0 && lv[l].g.m >= c.mixed_min_m;
+        if (st && mixed) st->mixed++;
+        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps, log_here ? elog : nullptr, mixed);
         if (log_here) {
             st->has_eigen_log = 1;
             st->eigenvalue = elog[0];
@@ -345,9 +390,7 @@ int mi_ctx_create(int device, mi_ctx** out)
 
 void mi_ctx_destroy(mi_ctx* ctx)
 {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->c.device);
-    delete ctx;
+    ctx_release(ctx);
 }
 
 const char* mi_last_error(const mi_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
@@ -362,6 +405,14 @@ int mi_ctx_set_graphs(mi_ctx* ctx, int enable)
 {
     if (!ctx) return ST_ARG;
     ctx->c.use_graphs = enable != 0;
+    return ST_OK;
+}
+
+int mi_ctx_set_precision(mi_ctx* ctx, int mixed, uint32_t min_entries)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.mixed = mixed != 0;
+    ctx->c.mixed_min_m = min_entries ? min_entries : MIXED_MIN_M;
     return ST_OK;
 }
 
@@ -571,7 +622,7 @@ int mi_power_iter(mi_ctx* ctx, const mi_graph* g, double* u, int normLap, double
         DBuf<double> d(&c.pool, g->g.n);
         h2d(c, d.get(), u, (size_t)g->g.n * 8);
         PowerStats ps;
-        power_iter(c, g->g, d.get(), normLap, tol, &ps);
+        power_iter(c, g->g, d.get(), normLap, tol, &ps, nullptr, c.mixed);
         d2h(c, u, d.get(), (size_t)g->g.n * 8);
         MI_HIP(hipStreamSynchronize(c.stream));
         if (iters) *iters = ps.iters;
@@ -671,7 +722,8 @@ int mi_use_partition(mi_ctx* ctx, const mi_graph* g, const uint32_t* part, int p
             d_part = dp.get();
         }
         auto A = std::make_unique<mi_graph>(), B = std::make_unique<mi_graph>();
-        A->ctx = B->ctx = ctx;
+        A->ctx = ctx;
+        B->ctx = ctx;
         DBuf<uint32_t> mapping;
         use_partition(c, g->g, d_part, A->g, B->g, mapping);
         *g1 = A.release();

@@ -14,7 +14,9 @@
 //   S1 = sum y,  S2 = sum y^2,  S3 = sum x y,  S4 = sum x,
 //   a = (S1 c) c,  ||y - a||^2 = S2 - 2 a S1 + a^2 n,  dot = (S3 - a S4) s.
 // (y is orthogonal to 1 up to rounding because x is and M 1 = gb 1, so the
-// norm formula has no cancellation.)
+// norm formula has no cancellation.)  With FP32 vectors (the mixed-precision
+// form) a fifth sum, S5 = sum x^2, normalises the dot by the norm of the
+// rounded x.
 //
 // An iteration is two kernels and uses no atomics.
 //
@@ -67,7 +69,9 @@ constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
 constexpr int MAXCB = 8;                // column blocks at most
 constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most (global form)
+constexpr unsigned long long MIXED_ITER_CAP = 20000;  // iterations of one mixed-precision level at most
 constexpr uint32_t MAXF_LDS = LBLOCK;   // LDS form: its 1024-thread workgroups reduce up to 1024 sums
+constexpr int NSUM = 5;                 // sums per finish workgroup (the fifth with FP32 vectors only)
 constexpr int W_UNIT = 0, W_U16 = 1, W_U32 = 2;  // weight formats of the streamed arrays
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
@@ -301,15 +305,16 @@ struct StepArgs {
     Blocks B;
     uint32_t bw;             // vertices per column block
     uint32_t nwaves;
-    double* part;            // sum of each (virtual) row that lies in one range
-    double* head;            // per range: partial of the row entering it
-    double* tail;            // per range: partial of the row leaving it
+    void* part;              // sum of each (virtual) row that lies in one range (V)
+    void* head;              // per range: partial of the row entering it (V)
+    void* tail;              // per range: partial of the row leaving it (V)
     uint32_t nblk_a;         // workgroups of the streaming kernel
     uint32_t nblk_f;         // workgroups of the finish kernel
     const double* fprev;     // finish sums S1..S4 per workgroup, previous iteration
     double* fcur;            // ... this iteration
-    const double* yprev;
-    double* ynew;
+    const void* yprev;       // vectors of the value type V: double, or float in the mixed form
+    void* ynew;
+    uint32_t f32;            // 1: V = float (mixed precision), 0: V = double
     double c;
     double tol;
     unsigned long long iter_max;
@@ -324,7 +329,7 @@ struct StepArgs {
 // the convergence decision for the previous iteration (eigensolve_kokkos.h:
 // 155-189).  Every workgroup computes the same values in the same order;
 // workgroup 0 records them.  Returns false when the loop has ended.
-template <int NWV>
+template <int NWV, int NS>
 __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, double& s, double (*lds)[NWV])
 {
     a = 0.0;
@@ -341,16 +346,16 @@ __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, do
         return true;
     }
     const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    double T[4];
+    double T[NS];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double v = threadIdx.x < A.nblk_f ? A.fprev[(size_t)threadIdx.x * 4 + k] : 0.0;
+    for (int k = 0; k < NS; k++) {
+        double v = threadIdx.x < A.nblk_f ? A.fprev[(size_t)threadIdx.x * NSUM + k] : 0.0;
         v = wave_sum(v);
         if (lane == 0) lds[k][wid] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NS; k++) {
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < NWV; j++) t += lds[k][j];
@@ -360,7 +365,11 @@ __device__ __forceinline__ bool iteration_start(const StepArgs& A, double& a, do
     const double an = mult * A.c;    // subtracted from every entry
     const double nrm2 = T[1] - 2.0 * an * T[0] + an * an * (double)A.n;
     const double sn = 1 / sqrt(nrm2);
-    const double dot = (T[2] - an * T[3]) * sn;
+    double dot = (T[2] - an * T[3]) * sn;
+    // FP32 vectors: x was rounded entry by entry, so its norm is 1 only to
+    // about 1e-7 / sqrt(n), a jitter the test of a change below 1e-10 cannot
+    // tell from movement; the dot is taken with x / ||x||
+    if (NS == 5) dot /= sqrt(T[NS - 1]);
     const unsigned long long itn = iter - 1;                  // iterations done
     const double lastd = A.scal[SC_DOT + (iter & 1u)];         // dot of iteration iter - 2
     const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
@@ -477,6 +486,45 @@ __device__ __forceinline__ double dpp0_f64(double v)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// float versions (the mixed form): one move per value
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v)  // old = 0.0f
+{
+    return __uint_as_float(dpp_u32<CTRL, ROWMASK>(0u, __float_as_uint(v)));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp0_f32(float v)
+{
+    return __uint_as_float(dpp0_u32<CTRL>(__float_as_uint(v)));
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_v(double v)
+{
+    return dpp_f64<CTRL, ROWMASK>(v);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_v(float v)
+{
+    return dpp_f32<CTRL, ROWMASK>(v);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp0_v(double v)
+{
+    return dpp0_f64<CTRL>(v);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp0_v(float v)
+{
+    return dpp0_f32<CTRL>(v);
+}
+
+__device__ __forceinline__ float readlane_v(float v, int lane)
+{
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
     const unsigned long long b = __double_as_longlong(v);
@@ -485,30 +533,46 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+__device__ __forceinline__ double readlane_v(double v, int lane)
+{
+    return readlane_f64(v, lane);
+}
+
+// fused multiply-add in the value type
+__device__ __forceinline__ double fma_v(double a, double b, double c)
+{
+    return fma(a, b, c);
+}
+__device__ __forceinline__ float fma_v(float a, float b, float c)
+{
+    return fmaf(a, b, c);
+}
+
 // Kogge-Stone step of the segmented scan: add the partial of the lane the
 // pattern names when it carries the same key.  Keys never decrease along
 // the lanes, so an equal key means every lane in between has it too.
 constexpr uint32_t KEY_NONE = 0xfffffffeu;  // never a row id, never RINF
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void seg_scan_step(double& sc, uint32_t key)
+template <int CTRL, int ROWMASK, class V>
+__device__ __forceinline__ void seg_scan_step(V& sc, uint32_t key)
 {
-    const double ov = dpp_f64<CTRL, ROWMASK>(sc);
+    const V ov = dpp_v<CTRL, ROWMASK>(sc);
     const uint32_t ok = dpp_u32<CTRL, ROWMASK>(KEY_NONE, key);
     if (ok == key) sc = ov + sc;
 }
 
 // Row shifts: a lane with no source reads value 0.0 and key 0; should that
 // key match, adding 0.0 changes nothing.
-template <int CTRL>
-__device__ __forceinline__ void seg_scan_shift(double& sc, uint32_t key)
+template <int CTRL, class V>
+__device__ __forceinline__ void seg_scan_shift(V& sc, uint32_t key)
 {
-    const double ov = dpp0_f64<CTRL>(sc);
+    const V ov = dpp0_v<CTRL>(sc);
     const uint32_t ok = dpp0_u32<CTRL>(key);
     if (ok == key) sc = ov + sc;
 }
 
 // Inclusive segmented sum over the 64 lanes (all lanes active).
-__device__ __forceinline__ double seg_scan_wave(double sc, uint32_t key)
+template <class V>
+__device__ __forceinline__ V seg_scan_wave(V sc, uint32_t key)
 {
     seg_scan_shift<DPP_ROW_SHR1>(sc, key);
     seg_scan_shift<DPP_ROW_SHR2>(sc, key);
@@ -521,18 +585,29 @@ __device__ __forceinline__ double seg_scan_wave(double sc, uint32_t key)
 
 // Stores a (virtual) row sum: part[r], or head[wv] for the row that entered
 // the range from the previous one.
-__device__ __forceinline__ void put(const StepArgs& A, uint32_t r, double v, uint32_t hrow, uint32_t wv)
+// x_j = (y_j - a) s in the value type.  In FP32 the rounded s rescales x
+// by up to 6e-8 from one iteration to the next; the fifth finish sum (sum
+// x^2) takes that out of the dot product.
+template <class V>
+__device__ __forceinline__ V xform(V y, double a, double s)
 {
-    if (r != RINF) *(r == hrow ? A.head + wv : A.part + r) = v;
+    return (y - (V)a) * (V)s;
+}
+
+template <class V>
+__device__ __forceinline__ void put(const StepArgs& A, uint32_t r, V v, uint32_t hrow, uint32_t wv)
+{
+    if (r != RINF) *(r == hrow ? static_cast<V*>(A.head) + wv : static_cast<V*>(A.part) + r) = v;
 }
 
 // Row sums of the entries [E0, E1) by one wavefront (range index wv; nrow =
 // row of entry E1 when it continues this block, else RINF).  x_j is read as
 // (yprev[j] - a) s, or from LDS (already shifted and scaled) in the LDS form.
-template <int WF, bool LDSX, bool R16>
+template <int WF, bool LDSX, bool R16, class V>
 __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint32_t E0, uint32_t E1,
-                                             uint32_t nrow, double a, double s, const double* xl)
+                                             uint32_t nrow, double a, double s, const V* xl)
 {
+    const V* yp = static_cast<const V*>(A.yprev);
     using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
     using RidT = typename std::conditional<R16, uint16_t, uint32_t>::type;
     using WT = typename std::conditional<WF == W_U16, uint16_t, uint32_t>::type;
@@ -545,7 +620,7 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
     const uint32_t rbase = R16 ? row0 : 0u;
     const uint32_t hrow = (meta & HEADF) ? row0 : RINF;
     uint32_t cr = hrow;  // row carried into the pass (RINF: none)
-    double cs = 0.0;     // its partial sum so far
+    V cs = 0;            // its partial sum so far
     // The 16-byte loads of pass p+1 are issued before pass p is processed.
     Raw nx;
     bool nfull = fetch<WF, LDSX, R16>(col, rid, w, E0 + lane * SEG, E1, nx);
@@ -581,16 +656,16 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
         }
         // gathers for every entry (an idle entry has column 0, in bounds) and
         // a select, so no branch surrounds the loads
-        double xv[SEG];
+        V xv[SEG];
         if (full) {
 #pragma unroll
-            for (int u = 0; u < SEG; u++) xv[u] = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
+            for (int u = 0; u < SEG; u++) xv[u] = LDSX ? xl[cc[u]] : xform(yp[cc[u]], a, s);
         } else {
             // idle entries (row RINF) contribute 0 whatever x holds
 #pragma unroll
             for (int u = 0; u < SEG; u++) {
-                const double v = LDSX ? xl[cc[u]] : (A.yprev[cc[u]] - a) * s;
-                xv[u] = rr[u] != RINF ? v : 0.0;
+                const V v = LDSX ? xl[cc[u]] : xform(yp[cc[u]], a, s);
+                xv[u] = rr[u] != RINF ? v : (V)0;
             }
         }
         // lane-local segments, as selects: fs = the first segment (it may
@@ -598,9 +673,9 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
         // this lane, acc = the last segment; a boundary restarts the sum
         const uint32_t first_row = rr[0], last_row = rr[SEG - 1];
         uint32_t mid_r[SEG - 1];
-        double mid_s[SEG - 1];
-        double fs = 0.0;
-        double acc = WF == W_UNIT ? xv[0] : (double)ww[0] * xv[0];
+        V mid_s[SEG - 1];
+        V fs = 0;
+        V acc = WF == W_UNIT ? xv[0] : (V)ww[0] * xv[0];
         bool multi = false, mids = false;
 #pragma unroll
         for (int u = 1; u < SEG; u++) {
@@ -611,22 +686,22 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
             fs = (br && !multi) ? acc : fs;
             mids = mids || mid;
             multi = multi || br;
-            const double base_acc = br ? 0.0 : acc;
-            acc = WF == W_UNIT ? base_acc + xv[u] : fma((double)ww[u], xv[u], base_acc);
+            const V base_acc = br ? (V)0 : acc;
+            acc = WF == W_UNIT ? base_acc + xv[u] : fma_v((V)ww[u], xv[u], base_acc);
         }
         // segmented inclusive scan of the last segments over the lanes (row
         // ids never decrease along a range, so equal keys are contiguous lanes)
-        double sc = acc;
+        V sc = acc;
         const uint32_t key = last_row;
         if (lane == 0 && !multi && key == cr) sc = cs + sc;
         sc = seg_scan_wave(sc, key);
-        const double psc = dpp_f64<DPP_WAVE_SHR1, 0xf>(sc);
+        const V psc = dpp_v<DPP_WAVE_SHR1, 0xf>(sc);
         const uint32_t pkey = dpp_u32<DPP_WAVE_SHR1, 0xf>(KEY_NONE, key);
         const uint32_t nxt = dpp_u32<DPP_WAVE_SHL1, 0xf>(nx63, first_row);  // lane 63: nx63
         // a lane's first segment closes inside the lane when it has several
         if (multi) {
             const bool crosses = lane == 0 ? (cr == first_row) : (pkey == first_row);
-            const double before = lane == 0 ? cs : psc;
+            const V before = lane == 0 ? cs : psc;
             put(A, first_row, crosses ? before + fs : fs, hrow, wv);
         }
         // rows wholly inside one lane's 8 entries are rare on long-row levels
@@ -638,51 +713,50 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
         if (nxt != last_row) put(A, last_row, sc, hrow, wv);
         // carry into the next pass
         const uint32_t k63 = (uint32_t)__builtin_amdgcn_readlane((int)key, WAVE - 1);
-        const double s63 = readlane_f64(sc, WAVE - 1);
+        const V s63 = readlane_v(sc, WAVE - 1);
         const bool open = (k63 != RINF) && (nx63 == k63);
         cr = open ? k63 : RINF;
-        cs = open ? s63 : 0.0;
+        cs = open ? s63 : (V)0;
     }
     // a row still open continues past this range
-    if (lane == 0 && cr != RINF) {
-        if (cr == hrow)
-            A.head[wv] = cs;
-        else
-            A.tail[wv] = cs;
-    }
+    if (lane == 0 && cr != RINF) *(cr == hrow ? static_cast<V*>(A.head) + wv : static_cast<V*>(A.tail) + wv) = cs;
 }
 
-template <int WF, bool R16>
+template <int WF, bool R16, class V>
 __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
 {
-    __shared__ double red[4][NW];
+    constexpr int NS = sizeof(V) == 4 ? 5 : 4;
+    __shared__ double red[NS][NW];
     double a, s;
-    if (!iteration_start<NW>(A, a, s, red)) return;
+    if (!iteration_start<NW, NS>(A, a, s, red)) return;
     const uint32_t wv = logical_block() * NW + threadIdx.x / WAVE;
     if (wv >= A.nwaves) return;
     const uint64_t E0 = (uint64_t)wv * A.B.pw;
     const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
     const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-    stream_range<WF, false, R16>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr);
+    stream_range<WF, false, R16, V>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr);
 }
 
 // Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
 // [g W / G, (g+1) W / G) in order and stages the slice of x of each column
 // block its share touches.  Which workgroup streams a range does not change
 // what is summed, so the result does not depend on the grid.
-template <int WF, bool R16>
+template <int WF, bool R16, class V>
 __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 {
-    __shared__ double xl[XB];
-    __shared__ double red[4][LNW];
-    constexpr int XPT = XB / LBLOCK;  // staged values per thread
+    constexpr uint32_t XN = XB;
+    __shared__ V xl[XN];
+    constexpr int NS = sizeof(V) == 4 ? 5 : 4;
+    __shared__ double red[NS][LNW];
+    constexpr int XPT = XN / LBLOCK;  // staged values per thread
+    const V* yp = static_cast<const V*>(A.yprev);
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
     const uint32_t G = gridDim.x, g = logical_block(), wid = threadIdx.x / WAVE;
     const uint32_t R1 = (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
     uint32_t cur = (uint32_t)((uint64_t)g * A.nwaves / G);
     // The slice of y for the first block is loaded before the previous
     // iteration's sums are reduced, so the two loads overlap.
-    double raw[XPT];
+    V raw[XPT];
     {
         uint32_t b = 0;
 #pragma unroll
@@ -693,11 +767,11 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 #pragma unroll
         for (int j = 0; j < XPT; j++) {
             const uint32_t i = threadIdx.x + (uint32_t)j * LBLOCK;
-            raw[j] = i < xn ? A.yprev[x0 + i] : 0.0;
+            raw[j] = i < xn ? yp[x0 + i] : (V)0;
         }
     }
-    double a, s;
-    if (!iteration_start<LNW>(A, a, s, red)) return;
+    double ad, sd;
+    if (!iteration_start<LNW, NS>(A, ad, sd, red)) return;
     // one segment of ranges inside one column block; the first one stages
     // from the values loaded above (their registers end there), later ones
     // load again
@@ -721,18 +795,18 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 #pragma unroll
             for (int j = 0; j < XPT; j++) {
                 const uint32_t i = threadIdx.x + (uint32_t)j * LBLOCK;
-                if (i < xn) xl[i] = (raw[j] - a) * s;
+                if (i < xn) xl[i] = xform(raw[j], ad, sd);
             }
         } else {
             __syncthreads();  // the previous block's gathers are done
-            for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = (A.yprev[x0 + i] - a) * s;
+            for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = xform(yp[x0 + i], ad, sd);
         }
         __syncthreads();
         for (uint32_t wv = cur + wid; wv < seg_end; wv += LNW) {
             const uint32_t E0 = off0 + (wv - wbase) * A.B.pw;
             const uint32_t E1 = (off1 - E0 > A.B.pw) ? E0 + A.B.pw : off1;
             const uint32_t nrow = (wv + 1 < wend) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-            stream_range<WF, true, R16>(A, wv, E0, E1, nrow, a, s, xl);
+            stream_range<WF, true, R16, V>(A, wv, E0, E1, nrow, ad, sd, xl);
         }
         cur = seg_end;
     };
@@ -751,22 +825,34 @@ __device__ __forceinline__ double shifted(const StepArgs& A, double spmv, double
 
 // Row sums over the column blocks, the shift, y and the four sums per
 // workgroup (fixed grid of at most MAXF workgroups, rows strided).
-template <bool NORMLAP>
+// V: the value type of the vectors and row partials.  The partials of a
+// row are added in V; the shift and y are formed in FP64 and y is stored in
+// V; the four sums are FP64 over the stored values (a fifth, sum x^2, when
+// V is float).
+template <bool NORMLAP, class V>
 __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
 {
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
-    __shared__ double red[4][NW];
+    constexpr int NS = sizeof(V) == 4 ? 5 : 4;
+    __shared__ double red[NS][NW];
     const double a = A.mode ? 0.0 : A.scal[SC_A];
     const double s = A.mode ? 1.0 : A.scal[SC_S];
+    const V* yp = static_cast<const V*>(A.yprev);
+    V* yn = static_cast<V*>(A.ynew);
+    const V* part = static_cast<const V*>(A.part);
+    const V* head = static_cast<const V*>(A.head);
+    const V* tail = static_cast<const V*>(A.tail);
     const uint32_t gb = (NORMLAP || A.mode == 2) ? 0u : *A.gbp;
     const uint32_t nb = A.B.ncb > 1 ? A.B.ncb : 1u;
-    double S[4] = {0, 0, 0, 0};
+    double S[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) S[k] = 0.0;
     for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
-        const double yr = A.yprev[r];
+        const V yr = yp[r];
         const uint32_t wd = A.wdeg ? A.wdeg[r] : 0u;
         // all blocks' bounds, then all first partials, as independent loads
         uint32_t s0[MAXCB], s1[MAXCB], w0[MAXCB], w1[MAXCB];
-        double first[MAXCB];
+        V first[MAXCB];
 #pragma unroll
         for (int q = 0; q < MAXCB; q++) {
             s0[q] = s1[q] = 0;
@@ -778,57 +864,95 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
         }
 #pragma unroll
         for (int q = 0; q < MAXCB; q++) {
-            first[q] = 0.0;
+            first[q] = 0;
             w0[q] = w1[q] = 0;
             if (s0[q] != s1[q]) {
                 w0[q] = A.B.bwave[q] + (s0[q] - A.B.boff[q]) / A.B.pw;
                 w1[q] = A.B.bwave[q] + (s1[q] - 1 - A.B.boff[q]) / A.B.pw;
-                first[q] = w0[q] == w1[q] ? A.part[(uint32_t)q * A.n + r] : A.tail[w0[q]];
+                first[q] = w0[q] == w1[q] ? part[(uint32_t)q * A.n + r] : tail[w0[q]];
             }
         }
-        double v = 0.0;
+        V v = 0;
 #pragma unroll
         for (int q = 0; q < MAXCB; q++) {
             if (s0[q] != s1[q]) {
-                double vb = first[q];
+                V vb = first[q];
                 if (w1[q] > w0[q]) {
                     // a row crossing ranges: in range order, eight loads at a time
                     uint32_t k = w0[q] + 1;
                     for (; k + 8 <= w1[q] + 1; k += 8) {
-                        double h[8];
+                        V h[8];
 #pragma unroll
-                        for (int j = 0; j < 8; j++) h[j] = A.head[k + j];
+                        for (int j = 0; j < 8; j++) h[j] = head[k + j];
 #pragma unroll
                         for (int j = 0; j < 8; j++) vb += h[j];
                     }
-                    for (; k <= w1[q]; k++) vb += A.head[k];
+                    for (; k <= w1[q]; k++) vb += head[k];
                 }
                 v = q == 0 ? vb : v + vb;
             }
         }
-        const double xr = (yr - a) * s;
-        const double y = shifted<NORMLAP>(A, v, xr, wd, gb);
-        A.ynew[r] = y;
+        const V xv = xform(yr, a, s);  // as the streaming kernel formed x
+        const double xr = (double)xv;
+        const V yv = (V)shifted<NORMLAP>(A, (double)v, xr, wd, gb);
+        yn[r] = yv;
+        const double y = (double)yv;
         S[0] += NORMLAP ? 0.0 : y;
         S[1] += y * y;
         S[2] += xr * y;
         S[3] += xr;
+        if (NS == 5) S[NS - 1] += xr * xr;
     }
     if (A.mode != 0) return;
-    double tot[4];
-    block_sum_k<4>(S, tot, red);
+    double tot[NS];
+    block_sum_k<NS>(S, tot, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) A.fcur[(size_t)blockIdx.x * 4 + k] = tot[k];
+        for (int k = 0; k < NS; k++) A.fcur[(size_t)blockIdx.x * NSUM + k] = tot[k];
         if (blockIdx.x == 0) A.cnt[CT_ITER] += 1u;  // read by the next launch only
     }
 }
 
 // u = (y - a) * s : the input of the last iteration
-__global__ void k_materialize(uint32_t n, const double* __restrict__ y, double* __restrict__ u, const double* scal)
+template <class V>
+__global__ void k_materialize(uint32_t n, const V* __restrict__ y, double* __restrict__ u, const double* scal)
 {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) u[i] = (y[i] - scal[SC_A]) * scal[SC_S];
+    if (i < n) u[i] = ((double)y[i] - scal[SC_A]) * scal[SC_S];
+}
+
+__global__ void k_to_f32(uint32_t n, const double* __restrict__ x, float* __restrict__ y)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+
+// weighted degrees as u32 sums (wrapping like the reference's sgp_wgt_t),
+// one wavefront per row: exact whatever the vector precision
+__global__ void __launch_bounds__(BLOCK) k_wdeg_int(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg)
+{
+    const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE;
+    if (r >= n) return;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint32_t t = 0;
+    for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) t += w[e];
+    t = wave_sum_u32(t);
+    if (lane == 0) wdeg[r] = t;
+}
+
+// gb = max(2 wdeg) over given degrees (fixed grid, one atomic per workgroup)
+__global__ void __launch_bounds__(BLOCK) k_gb_from(uint32_t n, const uint32_t* __restrict__ wdeg,
+                                                   uint32_t* __restrict__ gb)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    uint32_t t = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t t2 = (uint32_t)(2.0 * (double)wdeg[i]);
+        t = t2 > t ? t2 : t;
+    }
+    t = block_max_u32(t, lds);
+    if (threadIdx.x == 0) atomicMax(gb, t);
 }
 
 // ------------------------------------------------------------------ setup helpers
@@ -1018,7 +1142,7 @@ __global__ void k_loop_init(double* scal, uint32_t* cnt, unsigned long long* nit
 
 
 StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* wdeg, const uint32_t* gb,
-                   const double* yprev, double* ynew, double tol, int mode)
+                   const void* yprev, void* ynew, double tol, int mode)
 {
     StepArgs A;
     A.n = g.n;
@@ -1061,6 +1185,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.fcur = p.fsum.get();
     A.yprev = yprev;
     A.ynew = ynew;
+    A.f32 = p.f32 ? 1u : 0u;
     A.c = 1 / std::sqrt((double)g.n);
     A.tol = tol;
     A.iter_max = 100000000000000ull / (unsigned long long)g.n;
@@ -1072,11 +1197,12 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     return A;
 }
 
-void launch_stream(Ctx& c, const StepArgs& A)
+template <class V>
+void launch_stream_t(Ctx& c, const StepArgs& A)
 {
     if (A.nblk_a == 0) {
         // no entries: nothing to stream, but the iteration bookkeeping runs
-        if (A.mode == 0) k_power_stream<W_UNIT, false><<<1, BLOCK, 0, c.stream>>>(A);
+        if (A.mode == 0) k_power_stream<W_UNIT, false, V><<<1, BLOCK, 0, c.stream>>>(A);
         MI_HIP(hipGetLastError());
         return;
     }
@@ -1084,9 +1210,9 @@ void launch_stream(Ctx& c, const StepArgs& A)
 #define MI_STREAM(K, NT, WF)                                                   \
     do {                                                                       \
         if (r16)                                                               \
-            K<WF, true><<<A.nblk_a, NT, 0, c.stream>>>(A);                     \
+            K<WF, true, V><<<A.nblk_a, NT, 0, c.stream>>>(A);                  \
         else                                                                   \
-            K<WF, false><<<A.nblk_a, NT, 0, c.stream>>>(A);                    \
+            K<WF, false, V><<<A.nblk_a, NT, 0, c.stream>>>(A);                 \
     } while (0)
     if (A.B.ncb) {
         // (nblk_a = persistent workgroups of the LDS form)
@@ -1108,12 +1234,27 @@ void launch_stream(Ctx& c, const StepArgs& A)
     MI_HIP(hipGetLastError());
 }
 
+void launch_stream(Ctx& c, const StepArgs& A)
+{
+    if (A.f32)
+        launch_stream_t<float>(c, A);
+    else
+        launch_stream_t<double>(c, A);
+}
+
 void launch_finish(Ctx& c, int normLap, const StepArgs& A)
 {
-    if (normLap)
-        k_power_finish<true><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
-    else
-        k_power_finish<false><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+    if (A.f32) {
+        if (normLap)
+            k_power_finish<true, float><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+        else
+            k_power_finish<false, float><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+    } else {
+        if (normLap)
+            k_power_finish<true, double><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+        else
+            k_power_finish<false, double><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+    }
     MI_HIP(hipGetLastError());
 }
 
@@ -1126,8 +1267,9 @@ void launch_step(Ctx& c, int normLap, const StepArgs& A)
 
 }  // namespace
 
-void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
+void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
 {
+    p.f32 = f32;
     const uint32_t n = g.n, m = g.m;
     MI_CHECK(n > 0, ST_ARG, "empty level");
     // LDS form when x fits in LDS in at most MAXCB column blocks; beyond one
@@ -1285,13 +1427,18 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p)
     // one row per finish thread where the streaming kernel can reduce that
     // many workgroup sums (each thread's rows form a dependent load chain)
     p.nblk_f = (uint32_t)std::min<uint64_t>(ncb ? MAXF_LDS : MAXF, ceil_div(n, BLOCK));
-    p.fsum = DBuf<double>(&c.pool, 2ull * 4 * MAXF_LDS);
+    p.fsum = DBuf<double>(&c.pool, 2ull * NSUM * MAXF_LDS);
 }
 
 void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg, uint32_t* gb_dev)
 {
     MI_HIP(hipMemsetAsync(gb_dev, 0, 4, c.stream));
-    if (g.unit) {
+    if (!g.unit && p.f32) {
+        // integer row sums: a float product of ones would not be exact
+        k_wdeg_int<<<(unsigned)ceil_div(g.n, NW), BLOCK, 0, c.stream>>>(g.n, g.R(), g.W(), wdeg);
+        MI_HIP(hipGetLastError());
+        k_gb_from<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, wdeg, gb_dev);
+    } else if (g.unit) {
         k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
     } else {
         DBuf<double> ones(&c.pool, g.n), y(&c.pool, g.n);
@@ -1317,13 +1464,18 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
 {
     const uint32_t n = g.n;
     DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
-    DBuf<double> x(&c.pool, n), y(&c.pool, n);
+    DBuf<double> x(&c.pool, n), y(&c.pool, n);  // (float vectors use their first halves)
     LevelPlan p;
-    plan_level(c, g, p);
+    plan_level(c, g, p, c.mixed);  // the precision the context runs in
     weighted_degree(c, g, p, wdeg.get(), gb.get());
     std::vector<double> h(n);
     for (uint32_t i = 0; i < n; i++) h[i] = 1.0 / (1.0 + (i % 97));
     MI_HIP(hipMemcpyAsync(x.get(), h.data(), (size_t)n * 8, hipMemcpyHostToDevice, c.stream));
+    if (c.mixed) {
+        k_to_f32<<<grid1(n), BLOCK, 0, c.stream>>>(n, x.get(), reinterpret_cast<float*>(y.get()));
+        MI_HIP(hipGetLastError());
+        MI_HIP(hipMemcpyAsync(x.get(), y.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    }
     StepArgs A = make_args(c, g, p, wdeg.get(), gb.get(), x.get(), y.get(), 0.0, 1);
     launch_step(c, 0, A);  // warm-up
     MI_HIP(hipStreamSynchronize(c.stream));
@@ -1355,7 +1507,12 @@ void bench_power_step(Ctx& c, const DGraph& g, int reps, StepBench* out)
     // x and the row sums
     const double colb = p.ncb ? 2.0 : 4.0, ridb = p.rid16.get() ? 2.0 : 4.0;
     const double wb = g.unit ? 0.0 : (p.w16.get() ? 2.0 : 4.0);
-    out->impl_bytes = (colb + ridb + wb) * g.m + 8.0 * n + 8.0 * (p.ncb > 1 ? p.ncb : 1) * n;
+    const double vb = p.f32 ? 4.0 : 8.0;
+    out->impl_bytes = (colb + ridb + wb) * g.m + vb * n + vb * (p.ncb > 1 ? p.ncb : 1) * n;
+    if (p.f32) {  // x and the row partials in FP32
+        out->stream_bytes -= 4.0 * n;
+        out->step_bytes -= 12.0 * n;
+    }
 }
 
 void normalize(Ctx& c, double* d_u, uint32_t n)
@@ -1388,15 +1545,17 @@ static void eigenvalue_log(Ctx& c, const DGraph& g, const LevelPlan& p, const do
     read_words(c, res.get(), out, 3 * sizeof(double));
 }
 
-void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st, double* eig_log)
+void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st, double* eig_log,
+                bool mixed)
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
     auto t0 = std::chrono::steady_clock::now();
     DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
     DBuf<double> other(&c.pool, n);
+    DBuf<float> f0, f1;  // the mixed form's FP32 vectors
     LevelPlan plan;
-    plan_level(c, g, plan);
+    plan_level(c, g, plan, mixed);
     weighted_degree(c, g, plan, wdeg.get(), gb.get());
 
     // vec1 = 1, normalised: every entry is 1/sqrt(n) (n is exact in FP64)
@@ -1418,14 +1577,27 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     // Launch i (1-based) reads y_{i-1} from buf[(i-1) & 1] and writes y_i to
     // the other buffer; its streaming kernel first decides whether iteration
     // i-1 was the last, so the loop ends one launch after the last iteration.
-    double* buf[2] = {d_u, other.get()};
+    void* buf[2] = {d_u, other.get()};
+    if (mixed) {
+        // FP32 vectors from the FP64 start vector (set up in FP64 above)
+        f0 = DBuf<float>(&c.pool, n);
+        f1 = DBuf<float>(&c.pool, n);
+        k_to_f32<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, f0.get());
+        MI_HIP(hipGetLastError());
+        buf[0] = f0.get();
+        buf[1] = f1.get();
+    }
     StepArgs A[2];
     c.host_mapped[64] = 0u;  // no kernel of this level has run yet
     for (int k = 0; k < 2; k++) {
         A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
+        // FP32 noise can hold the change of the dot product above a small
+        // tolerance: the mixed form stops after MIXED_ITER_CAP iterations
+        // whatever (the FP64 solve of the finest level follows it)
+        if (mixed && A[k].iter_max > MIXED_ITER_CAP) A[k].iter_max = MIXED_ITER_CAP;
         A[k].hdone = c.host_mapped_dev + 64;
-        A[k].fprev = plan.fsum.get() + 4 * MAXF_LDS * (k ^ 1);  // sums of the previous launch
-        A[k].fcur = plan.fsum.get() + 4 * MAXF_LDS * k;
+        A[k].fprev = plan.fsum.get() + NSUM * MAXF_LDS * (k ^ 1);  // sums of the previous launch
+        A[k].fcur = plan.fsum.get() + NSUM * MAXF_LDS * k;
     }
     // Every launch of this level has the same arguments but for the ping-pong
     // parity (the iteration index lives on the device), so a chunk of
@@ -1502,9 +1674,14 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     read_words(c, c.niter, &done_iters, 8);
     MI_CHECK(done_iters >= 1, ST_INTERNAL, "power iteration ended without an iteration");
     // the reference returns u = input of the last iteration (eigensolve_kokkos.h:157)
-    k_materialize<<<grid1(n), BLOCK, 0, c.stream>>>(n, buf[(done_iters - 1) & 1], d_u, c.scal);
+    if (mixed)
+        k_materialize<float><<<grid1(n), BLOCK, 0, c.stream>>>(n, static_cast<const float*>(buf[(done_iters - 1) & 1]),
+                                                              d_u, c.scal);
+    else
+        k_materialize<double><<<grid1(n), BLOCK, 0, c.stream>>>(
+            n, static_cast<const double*>(buf[(done_iters - 1) & 1]), d_u, c.scal);
     MI_HIP(hipGetLastError());
-    if (eig_log && !normLap) eigenvalue_log(c, g, plan, d_u, other.get(), eig_log);
+    if (eig_log && !normLap && !mixed) eigenvalue_log(c, g, plan, d_u, other.get(), eig_log);
     if (st) {
         MI_HIP(hipStreamSynchronize(c.stream));
         st->iters = done_iters;

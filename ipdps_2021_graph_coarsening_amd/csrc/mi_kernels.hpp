@@ -133,8 +133,13 @@ struct LevelPlan {
     uint32_t pw = 0, nwaves = 0;
     uint32_t nblk_a = 0;  // workgroups of the streaming kernel
     uint32_t nblk_f = 0;  // workgroups of the finish kernel
+    bool f32 = false;     // FP32 vectors and row partials (mixed precision)
 };
-void plan_level(Ctx& c, const DGraph& g, LevelPlan& p);
+void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32 = false);
+// Mixed precision applies by default to levels above the finest with at
+// least this many stored entries; smaller ones are bound by launch and
+// reduction latency.
+constexpr uint32_t MIXED_MIN_M = 1u << 20;
 
 struct StepBench {
     double step_ms = 0;       // one whole step (both kernels)
@@ -156,8 +161,10 @@ struct PowerStats {
 // Power iteration on level graph g, in place on d_u (n doubles).  eig_log
 // (host, 3 doubles, may be null; normLap 0 only) receives the final
 // eigenvalue diagnostic of eigensolve_kokkos.h:60-100: eigenvalue, min, max.
+// mixed: FP32 vectors, row partials and products (mixed precision); the
+// deflation, sums, normalisation and stopping test stay FP64.
 void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st,
-                double* eig_log = nullptr);
+                double* eig_log = nullptr, bool mixed = false);
 // out[i] = in[vcmap[i]]
 void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, double* out);
 // u /= ||u||

@@ -19,6 +19,7 @@ int main(void)
     const char* (*fp_mi_last_error)(const mi_ctx*);
     int (*fp_mi_ctx_trim)(mi_ctx*);
     int (*fp_mi_ctx_set_graphs)(mi_ctx*, int);
+    int (*fp_mi_ctx_set_precision)(mi_ctx*, int, uint32_t);
     int (*fp_mi_graph_create)(mi_ctx*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, mi_graph**);
     int (*fp_mi_graph_create_device)(mi_ctx*, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
                                      const uint32_t*, mi_graph**);
@@ -56,6 +57,7 @@ int main(void)
     BIND(mi_last_error);
     BIND(mi_ctx_trim);
     BIND(mi_ctx_set_graphs);
+    BIND(mi_ctx_set_precision);
     BIND(mi_graph_create);
     BIND(mi_graph_create_device);
     BIND(mi_graph_destroy);

@@ -477,3 +477,16 @@ def test_bad_input_is_rejected(ctx):
     assert e.value.status == 1
     with pytest.raises(mi.MiError):
         ctx.graph(np.array([0, 1, 2], np.uint32), np.array([5, 0], np.uint32))  # column out of range
+
+
+def test_context_destroyed_before_its_objects():
+    """mi_ctx_destroy first, then the graph and the hierarchy made on it."""
+    c2 = mi.Context(0)
+    rp, col, w = graphs.rmat(10, 8, seed=5)
+    g = c2.graph(rp, col, w)
+    H = g.hierarchy(3)
+    lib, gh, hh = c2._L, g.h, H.h
+    g.h = H.h = None  # the handles are released by hand below
+    c2.close()
+    lib.mi_hier_destroy(hh)
+    lib.mi_graph_destroy(gh)
