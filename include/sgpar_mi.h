@@ -103,6 +103,13 @@ int mi_ctx_set_graphs(mi_ctx* ctx, int enable);
    whatever the level's size.  mixed == 0 (the default): FP64 throughout, as
    the reference.  No counterpart in the reference. */
 int mi_ctx_set_precision(mi_ctx* ctx, int mixed, uint32_t min_entries);
+/* Levels of at most 16384 vertices and at most max_entries stored entries
+   (default 131072; 0: none) run their whole FP64 power iteration in one
+   1024-thread workgroup, chunk iterations per launch at most (0: 8192),
+   instead of two kernel launches per iteration.  Same stop rule and
+   result bookkeeping; the sums are added in another order.  No counterpart
+   in the reference. */
+int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk);
 
 /* ---------------------------------------------------------------- graphs */
 /*

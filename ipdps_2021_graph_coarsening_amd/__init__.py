@@ -87,6 +87,7 @@ _SIGS = [
     ("mi_ctx_trim", C.c_int, [vp]),
     ("mi_ctx_set_graphs", C.c_int, [vp, C.c_int]),
     ("mi_ctx_set_precision", C.c_int, [vp, C.c_int, C.c_uint32]),
+    ("mi_ctx_set_small_levels", C.c_int, [vp, C.c_uint32, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -179,6 +180,11 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:
+        """Levels of <= 16384 vertices and <= max_entries entries iterate in one
+        workgroup (0: never), at most chunk iterations per launch (0: 8192)."""
+        self.check(self._L.mi_ctx_set_small_levels(self.h, max_entries, chunk), "mi_ctx_set_small_levels")
 
     def set_precision(self, mixed: bool, min_entries: int = 0) -> None:
         """Mixed-precision eigensolve (FP32 on levels above the finest with at

@@ -408,6 +408,14 @@ int mi_ctx_set_graphs(mi_ctx* ctx, int enable)
     return ST_OK;
 }
 
+int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.small_max_m = max_entries;
+    ctx->c.small_chunk = chunk ? chunk : 8192u;
+    return ST_OK;
+}
+
 int mi_ctx_set_precision(mi_ctx* ctx, int mixed, uint32_t min_entries)
 {
     if (!ctx) return ST_ARG;

@@ -160,6 +160,8 @@ struct Ctx {
     bool use_graphs = true;               // replay power iterations from a captured HIP graph
     bool mixed = false;                   // FP32 power iterations on the levels above the finest
     uint32_t mixed_min_m = 1u << 20;      // ... that hold at least this many entries
+    uint32_t small_max_m = 1u << 17;      // levels of <= 16384 vertices and this many entries: one workgroup
+    uint32_t small_chunk = 8192;          // iterations per launch of that workgroup at most
     ~Ctx();
 };
 

@@ -823,6 +823,57 @@ __device__ __forceinline__ double shifted(const StepArgs& A, double spmv, double
     return 0.5 * xr + 0.5 * spmv * (1.0 / (double)wd);
 }
 
+// A x at row r: the row's partials over its NB <= CB column blocks and the
+// ranges it crosses, added in range order.
+template <int CB, class V>
+__device__ __forceinline__ V row_total(const StepArgs& A, uint32_t r, uint32_t nb, const V* part, const V* head,
+                                       const V* tail)
+{
+    // all blocks' bounds, then all first partials, as independent loads
+    uint32_t s0[CB], s1[CB], w0[CB], w1[CB];
+    V first[CB];
+#pragma unroll
+    for (int q = 0; q < CB; q++) {
+        s0[q] = s1[q] = 0;
+        if ((uint32_t)q < nb) {
+            const uint32_t vr = (uint32_t)q * A.n + r;
+            s0[q] = A.rowptr[vr];
+            s1[q] = A.rowptr[vr + 1];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CB; q++) {
+        first[q] = 0;
+        w0[q] = w1[q] = 0;
+        if (s0[q] != s1[q]) {
+            w0[q] = A.B.bwave[q] + (s0[q] - A.B.boff[q]) / A.B.pw;
+            w1[q] = A.B.bwave[q] + (s1[q] - 1 - A.B.boff[q]) / A.B.pw;
+            first[q] = w0[q] == w1[q] ? part[(uint32_t)q * A.n + r] : tail[w0[q]];
+        }
+    }
+    V v = 0;
+#pragma unroll
+    for (int q = 0; q < CB; q++) {
+        if (s0[q] != s1[q]) {
+            V vb = first[q];
+            if (w1[q] > w0[q]) {
+                // a row crossing ranges: in range order, eight loads at a time
+                uint32_t k = w0[q] + 1;
+                for (; k + 8 <= w1[q] + 1; k += 8) {
+                    V h[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) h[j] = head[k + j];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) vb += h[j];
+                }
+                for (; k <= w1[q]; k++) vb += head[k];
+            }
+            v = q == 0 ? vb : v + vb;
+        }
+    }
+    return v;
+}
+
 // Row sums over the column blocks, the shift, y and the four sums per
 // workgroup (fixed grid of at most MAXF workgroups, rows strided).
 // V: the value type of the vectors and row partials.  The partials of a
@@ -850,48 +901,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
         const V yr = yp[r];
         const uint32_t wd = A.wdeg ? A.wdeg[r] : 0u;
-        // all blocks' bounds, then all first partials, as independent loads
-        uint32_t s0[MAXCB], s1[MAXCB], w0[MAXCB], w1[MAXCB];
-        V first[MAXCB];
-#pragma unroll
-        for (int q = 0; q < MAXCB; q++) {
-            s0[q] = s1[q] = 0;
-            if ((uint32_t)q < nb) {
-                const uint32_t vr = (uint32_t)q * A.n + r;
-                s0[q] = A.rowptr[vr];
-                s1[q] = A.rowptr[vr + 1];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < MAXCB; q++) {
-            first[q] = 0;
-            w0[q] = w1[q] = 0;
-            if (s0[q] != s1[q]) {
-                w0[q] = A.B.bwave[q] + (s0[q] - A.B.boff[q]) / A.B.pw;
-                w1[q] = A.B.bwave[q] + (s1[q] - 1 - A.B.boff[q]) / A.B.pw;
-                first[q] = w0[q] == w1[q] ? part[(uint32_t)q * A.n + r] : tail[w0[q]];
-            }
-        }
-        V v = 0;
-#pragma unroll
-        for (int q = 0; q < MAXCB; q++) {
-            if (s0[q] != s1[q]) {
-                V vb = first[q];
-                if (w1[q] > w0[q]) {
-                    // a row crossing ranges: in range order, eight loads at a time
-                    uint32_t k = w0[q] + 1;
-                    for (; k + 8 <= w1[q] + 1; k += 8) {
-                        V h[8];
-#pragma unroll
-                        for (int j = 0; j < 8; j++) h[j] = head[k + j];
-#pragma unroll
-                        for (int j = 0; j < 8; j++) vb += h[j];
-                    }
-                    for (; k <= w1[q]; k++) vb += head[k];
-                }
-                v = q == 0 ? vb : v + vb;
-            }
-        }
+        const V v = row_total<MAXCB>(A, r, nb, part, head, tail);
         const V xv = xform(yr, a, s);  // as the streaming kernel formed x
         const double xr = (double)xv;
         const V yv = (V)shifted<NORMLAP>(A, (double)v, xr, wd, gb);
@@ -910,6 +920,118 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
 #pragma unroll
         for (int k = 0; k < NS; k++) A.fcur[(size_t)blockIdx.x * NSUM + k] = tot[k];
         if (blockIdx.x == 0) A.cnt[CT_ITER] += 1u;  // read by the next launch only
+    }
+}
+
+// Small levels (n <= XB, few entries): the whole power iteration in one
+// 1024-thread workgroup, with no kernel boundary between iterations.  Every
+// iteration is the two-kernel step above in one place: the stop test on the
+// previous iteration's sums, x = (y - a) s staged in LDS, the ranges of
+// entries streamed by the 16 wavefronts in turn (stream_range, LDS form,
+// one column block), then one thread per row adds its partials, forms y and
+// the four sums, which the workgroup reduces in a fixed order.  The
+// workgroup's own global stores (partials, y) are read back after a
+// workgroup barrier, which orders them within the CU.  A launch runs at
+// most `chunk` iterations and leaves its state (iteration count, the last
+// two dot products, the last sums, a and s) where the next launch resumes
+// it; the launch that reaches the stop leaves the bookkeeping of the
+// two-kernel loop (done flag, iteration count, a and s of the last
+// iteration), so the result is read the same way.
+template <int WF, bool R16, bool NORMLAP>
+__global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0, double* buf1, uint32_t chunk)
+{
+    __shared__ double xl[XB];
+    __shared__ double red[4][LNW];
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const bool lead = tid == 0;
+    if (A.cnt[CT_DONE]) return;
+    const double* part = static_cast<const double*>(A.part);
+    const double* head = static_cast<const double*>(A.head);
+    const double* tail = static_cast<const double*>(A.tail);
+    const uint32_t gb = NORMLAP ? 0u : *A.gbp;
+    uint32_t iter = A.cnt[CT_ITER] + 1u;  // 1-based index of the next iteration
+    double T[4], dot0 = A.scal[SC_DOT], dot1 = A.scal[SC_DOT + 1];
+    double last_a = A.scal[SC_A], last_s = A.scal[SC_S];
+#pragma unroll
+    for (int k = 0; k < 4; k++) T[k] = A.fcur[k];  // (fcur[0..4) holds the sums between launches)
+    for (uint32_t it = 0; it < chunk; it++, iter++) {
+        double a = 0.0, s = 1.0;
+        if (iter >= 2) {
+            // the stop test of iteration iter - 1 (iteration_start's formulas)
+            const double an = T[0] * A.c * A.c;
+            const double nrm2 = T[1] - 2.0 * an * T[0] + an * an * (double)A.n;
+            const double sn = 1 / sqrt(nrm2);
+            const double dot = (T[2] - an * T[3]) * sn;
+            const unsigned long long itn = iter - 1;
+            const double lastd = (iter & 1u) ? dot1 : dot0;
+            const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
+            if ((iter - 1) & 1u)
+                dot1 = dot;
+            else
+                dot0 = dot;
+            if (!go_on) {
+                if (lead) {
+                    A.scal[SC_DOT] = dot0;
+                    A.scal[SC_DOT + 1] = dot1;
+                    A.scal[SC_A] = last_a;
+                    A.scal[SC_S] = last_s;
+                    A.cnt[CT_ITER] = iter - 1;
+                    A.cnt[CT_DONE] = 1u;
+                    A.cnt[CT_MAXR] = itn >= A.iter_max ? 1u : 0u;
+                    *A.niter = itn;
+                }
+                return;
+            }
+            a = an;
+            s = sn;
+        }
+        last_a = a;
+        last_s = s;
+        const double* yp = ((iter - 1) & 1u) ? buf1 : buf0;
+        double* yn = (iter & 1u) ? buf1 : buf0;
+        for (uint32_t i = tid; i < A.n; i += LBLOCK) xl[i] = (yp[i] - a) * s;
+        __syncthreads();
+        for (uint32_t wv = wid; wv < A.nwaves; wv += LNW) {
+            const uint32_t E0 = wv * A.B.pw;
+            const uint32_t E1 = (A.m - E0 > A.B.pw) ? E0 + A.B.pw : A.m;
+            const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+            stream_range<WF, true, R16, double>(A, wv, E0, E1, nrow, a, s, xl);
+        }
+        __syncthreads();
+        double S[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t r = tid; r < A.n; r += LBLOCK) {
+            const double v = row_total<1>(A, r, 1u, part, head, tail);
+            const double xr = xl[r];
+            const double y = shifted<NORMLAP>(A, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
+            yn[r] = y;
+            S[0] += NORMLAP ? 0.0 : y;
+            S[1] += y * y;
+            S[2] += xr * y;
+            S[3] += xr;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double w = wave_sum(S[k]);
+            if (lane == 0) red[k][wid] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < LNW; j++) t += red[k][j];
+            T[k] = t;
+        }
+        __syncthreads();  // red and xl are rewritten by the next iteration
+    }
+    if (lead) {
+        A.cnt[CT_ITER] = iter - 1;
+        A.scal[SC_DOT] = dot0;
+        A.scal[SC_DOT + 1] = dot1;
+        A.scal[SC_A] = last_a;
+        A.scal[SC_S] = last_s;
+#pragma unroll
+        for (int k = 0; k < 4; k++) A.fcur[k] = T[k];
     }
 }
 
@@ -1545,60 +1667,9 @@ static void eigenvalue_log(Ctx& c, const DGraph& g, const LevelPlan& p, const do
     read_words(c, res.get(), out, 3 * sizeof(double));
 }
 
-void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st, double* eig_log,
-                bool mixed)
+// The iteration loop of a level as launches of the two-kernel step.
+static void run_steps(Ctx& c, int normLap, const StepArgs (&A)[2])
 {
-    const uint32_t n = g.n;
-    MI_CHECK(n > 0, ST_ARG, "empty level");
-    auto t0 = std::chrono::steady_clock::now();
-    DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
-    DBuf<double> other(&c.pool, n);
-    DBuf<float> f0, f1;  // the mixed form's FP32 vectors
-    LevelPlan plan;
-    plan_level(c, g, plan, mixed);
-    weighted_degree(c, g, plan, wdeg.get(), gb.get());
-
-    // vec1 = 1, normalised: every entry is 1/sqrt(n) (n is exact in FP64)
-    const double cval = 1 / std::sqrt((double)n);
-    if (!normLap) {
-        k_sum_c<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.partials, c.cnt, c.scal, SC_MULT);
-        MI_HIP(hipGetLastError());
-        k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 0);
-    } else {
-        k_d_sums<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, wdeg.get(), cval, c.partials, c.cnt, c.scal);
-        MI_HIP(hipGetLastError());
-        k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 1);
-    }
-    MI_HIP(hipGetLastError());
-    normalize(c, d_u, n);
-    k_loop_init<<<1, WAVE, 0, c.stream>>>(c.scal, c.cnt, c.niter);
-    MI_HIP(hipGetLastError());
-
-    // Launch i (1-based) reads y_{i-1} from buf[(i-1) & 1] and writes y_i to
-    // the other buffer; its streaming kernel first decides whether iteration
-    // i-1 was the last, so the loop ends one launch after the last iteration.
-    void* buf[2] = {d_u, other.get()};
-    if (mixed) {
-        // FP32 vectors from the FP64 start vector (set up in FP64 above)
-        f0 = DBuf<float>(&c.pool, n);
-        f1 = DBuf<float>(&c.pool, n);
-        k_to_f32<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, f0.get());
-        MI_HIP(hipGetLastError());
-        buf[0] = f0.get();
-        buf[1] = f1.get();
-    }
-    StepArgs A[2];
-    c.host_mapped[64] = 0u;  // no kernel of this level has run yet
-    for (int k = 0; k < 2; k++) {
-        A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
-        // FP32 noise can hold the change of the dot product above a small
-        // tolerance: the mixed form stops after MIXED_ITER_CAP iterations
-        // whatever (the FP64 solve of the finest level follows it)
-        if (mixed && A[k].iter_max > MIXED_ITER_CAP) A[k].iter_max = MIXED_ITER_CAP;
-        A[k].hdone = c.host_mapped_dev + 64;
-        A[k].fprev = plan.fsum.get() + NSUM * MAXF_LDS * (k ^ 1);  // sums of the previous launch
-        A[k].fcur = plan.fsum.get() + NSUM * MAXF_LDS * k;
-    }
     // Every launch of this level has the same arguments but for the ping-pong
     // parity (the iteration index lives on the device), so a chunk of
     // GRAPH_ITERS iterations is captured once as a HIP graph (kernel nodes
@@ -1667,6 +1738,107 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         MI_CHECK(issued < cap, ST_INTERNAL, "power iteration did not stop at its iteration cap");
         k ^= 1;
     }
+}
+
+// The iteration loop of a small level: k_power_small in launches of at most
+// c.small_chunk iterations until one reaches the stop.
+static void run_small(Ctx& c, int normLap, const StepArgs& A, double* b0, double* b1)
+{
+    const uint32_t SMALL_CHUNK = c.small_chunk;
+    const bool r16 = A.r16 != 0;
+#define MI_SMALL(WF, NL)                                                                            \
+    do {                                                                                            \
+        if (r16)                                                                                    \
+            k_power_small<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);         \
+        else                                                                                        \
+            k_power_small<WF, false, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);        \
+    } while (0)
+    for (unsigned long long launched = 0;; launched += SMALL_CHUNK) {
+        MI_CHECK(launched <= A.iter_max + SMALL_CHUNK, ST_INTERNAL,
+                 "power iteration did not stop at its iteration cap");
+        if (normLap) {
+            if (A.wfmt == W_UNIT)
+                MI_SMALL(W_UNIT, true);
+            else if (A.wfmt == W_U16)
+                MI_SMALL(W_U16, true);
+            else
+                MI_SMALL(W_U32, true);
+        } else {
+            if (A.wfmt == W_UNIT)
+                MI_SMALL(W_UNIT, false);
+            else if (A.wfmt == W_U16)
+                MI_SMALL(W_U16, false);
+            else
+                MI_SMALL(W_U32, false);
+        }
+        MI_HIP(hipGetLastError());
+        uint32_t done = 0;
+        read_words(c, c.cnt + CT_DONE, &done, 4);
+        if (done) break;
+    }
+#undef MI_SMALL
+}
+
+void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st, double* eig_log,
+                bool mixed)
+{
+    const uint32_t n = g.n;
+    MI_CHECK(n > 0, ST_ARG, "empty level");
+    auto t0 = std::chrono::steady_clock::now();
+    DBuf<uint32_t> wdeg(&c.pool, n), gb(&c.pool, 1);
+    DBuf<double> other(&c.pool, n);
+    DBuf<float> f0, f1;  // the mixed form's FP32 vectors
+    LevelPlan plan;
+    plan_level(c, g, plan, mixed);
+    weighted_degree(c, g, plan, wdeg.get(), gb.get());
+
+    // vec1 = 1, normalised: every entry is 1/sqrt(n) (n is exact in FP64)
+    const double cval = 1 / std::sqrt((double)n);
+    if (!normLap) {
+        k_sum_c<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.partials, c.cnt, c.scal, SC_MULT);
+        MI_HIP(hipGetLastError());
+        k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 0);
+    } else {
+        k_d_sums<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, wdeg.get(), cval, c.partials, c.cnt, c.scal);
+        MI_HIP(hipGetLastError());
+        k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 1);
+    }
+    MI_HIP(hipGetLastError());
+    normalize(c, d_u, n);
+    k_loop_init<<<1, WAVE, 0, c.stream>>>(c.scal, c.cnt, c.niter);
+    MI_HIP(hipGetLastError());
+
+    // Launch i (1-based) reads y_{i-1} from buf[(i-1) & 1] and writes y_i to
+    // the other buffer; its streaming kernel first decides whether iteration
+    // i-1 was the last, so the loop ends one launch after the last iteration.
+    void* buf[2] = {d_u, other.get()};
+    if (mixed) {
+        // FP32 vectors from the FP64 start vector (set up in FP64 above)
+        f0 = DBuf<float>(&c.pool, n);
+        f1 = DBuf<float>(&c.pool, n);
+        k_to_f32<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, f0.get());
+        MI_HIP(hipGetLastError());
+        buf[0] = f0.get();
+        buf[1] = f1.get();
+    }
+    StepArgs A[2];
+    c.host_mapped[64] = 0u;  // no kernel of this level has run yet
+    for (int k = 0; k < 2; k++) {
+        A[k] = make_args(c, g, plan, wdeg.get(), gb.get(), buf[k], buf[k ^ 1], tol, 0);
+        // FP32 noise can hold the change of the dot product above a small
+        // tolerance: the mixed form stops after MIXED_ITER_CAP iterations
+        // whatever (the FP64 solve of the finest level follows it)
+        if (mixed && A[k].iter_max > MIXED_ITER_CAP) A[k].iter_max = MIXED_ITER_CAP;
+        A[k].hdone = c.host_mapped_dev + 64;
+        A[k].fprev = plan.fsum.get() + NSUM * MAXF_LDS * (k ^ 1);  // sums of the previous launch
+        A[k].fcur = plan.fsum.get() + NSUM * MAXF_LDS * k;
+    }
+    // Small levels run in one workgroup (no mixed precision there: they are
+    // latency-bound); the others in launches of the two-kernel step.
+    if (!mixed && plan.ncb == 1 && g.m > 0 && g.m <= c.small_max_m)
+        run_small(c, normLap, A[0], static_cast<double*>(buf[0]), static_cast<double*>(buf[1]));
+    else
+        run_steps(c, normLap, A);
     MI_HIP(hipStreamSynchronize(c.stream));
     uint32_t flags[2] = {0, 0};
     read_words(c, c.cnt + CT_DONE, flags, 8);

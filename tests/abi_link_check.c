@@ -20,6 +20,7 @@ int main(void)
     int (*fp_mi_ctx_trim)(mi_ctx*);
     int (*fp_mi_ctx_set_graphs)(mi_ctx*, int);
     int (*fp_mi_ctx_set_precision)(mi_ctx*, int, uint32_t);
+    int (*fp_mi_ctx_set_small_levels)(mi_ctx*, uint32_t, uint32_t);
     int (*fp_mi_graph_create)(mi_ctx*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, mi_graph**);
     int (*fp_mi_graph_create_device)(mi_ctx*, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
                                      const uint32_t*, mi_graph**);
@@ -58,6 +59,7 @@ int main(void)
     BIND(mi_ctx_trim);
     BIND(mi_ctx_set_graphs);
     BIND(mi_ctx_set_precision);
+    BIND(mi_ctx_set_small_levels);
     BIND(mi_graph_create);
     BIND(mi_graph_create_device);
     BIND(mi_graph_destroy);

@@ -490,3 +490,42 @@ def test_context_destroyed_before_its_objects():
     c2.close()
     lib.mi_hier_destroy(hh)
     lib.mi_graph_destroy(gh)
+
+
+@pytest.mark.parametrize("normLap", [0, 1])
+@pytest.mark.parametrize("case", ["er3000w", "grid64", "rmat11w_hubs"])
+def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
+    """The one-workgroup loop of small levels against the two-kernel step
+    (same stop rule, sums added in another order: iteration count within
+    2, eigenvectors within 1e-9 in angle) and against the oracle; resumed
+    launches (7 iterations each) equal one launch bit for bit."""
+    rng = np.random.default_rng(21)
+    if case == "er3000w":
+        rp, col, w = random_graph(rng, 3000, 0.002, weighted=True, lcc=True)
+    elif case == "grid64":
+        rp, col, w = graphs.grid2d(64)
+    else:
+        rp, col, w = graphs.rmat(11, 16, seed=4)
+    u0 = rng.random(len(rp) - 1)
+    g = ctx.graph(rp, col, w)
+    try:
+        ctx.set_small_levels(1 << 17)
+        us, its, mrs = g.power_iter(u0, normLap=normLap, tol=1e-10)
+        ctx.set_small_levels(1 << 17, 7)
+        uc, itc, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+        ctx.set_small_levels(0)
+        um, itm, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+    finally:
+        ctx.set_small_levels()
+    assert itc == its and np.array_equal(uc, us)
+    assert abs(its - itm) <= 2 and not mrs
+    assert abs(us @ um) / (np.linalg.norm(us) * np.linalg.norm(um)) >= 1 - 1e-9
+    if normLap == 0:
+        # (with normLap on the ER graph u . v peaks near iteration 300, so
+        # |d(dot)| dips through 1e-10 there and whether it stops depends on
+        # the last bits of the sums: numpy's FP64 run stops at 301 like both
+        # device loops, the oracle's sequential sums at 439)
+        ou, oit, _, _ = oracle.power_iter(rp, col, w, u0, normLap=normLap, tol=1e-10)
+        assert abs(its - oit) <= max(2, ITER_SLACK * oit)
+        assert abs(us @ ou) / (np.linalg.norm(us) * np.linalg.norm(ou)) >= EV_COS
+    g.close()
