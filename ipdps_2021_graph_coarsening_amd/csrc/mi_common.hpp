@@ -174,11 +174,33 @@ void read_words(Ctx& c, const void* dsrc, void* hdst, size_t bytes);
 // ------------------------------------------------------------------------
 // Device helpers
 // ------------------------------------------------------------------------
+// v from another lane by a DPP pattern, both halves; lanes that the pattern
+// or ROWMASK leaves without a source read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64_or0(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWMASK, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWMASK, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the wavefront (every lane active), returned in every lane:
+// prefix sums inside each row of 16 lanes (row_shr 1, 2, 4, 8), the rows
+// joined by the broadcasts of lanes 15 and 31, the total read from lane 63.
+// A fixed order with no LDS traffic.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+    v += dpp_f64_or0<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_f64_or0<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_f64_or0<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_f64_or0<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_f64_or0<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64_or0<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const unsigned long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, WAVE - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), WAVE - 1);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)

@@ -1035,6 +1035,168 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
     }
 }
 
+// The same loop with every vector in LDS, for levels where x, y, the row
+// partials and the range partials fit there together (3 n + 2 ranges <=
+// SMALL_LDS doubles, rows <= 4 per thread): only the matrix is read from
+// global memory (and stays in the caches), and each thread keeps its rows'
+// range indices and weighted degrees in registers for the whole loop.
+// stream_range writes the partials through generic pointers into LDS.  Each
+// thread stages x and forms y for the same indices (its rows), so x and y
+// need no barrier between those two steps.  A launch starts from y in
+// global memory; when its chunk ends it leaves y of its last iteration in
+// that iteration's output buffer and x of that iteration in the other one.
+// At the stop, x of the last iteration is in the buffer the result is read
+// from, with a = 0 and s = 1.
+constexpr uint32_t SMALL_LDS = 18432;  // doubles (144 KiB)
+constexpr uint32_t SMALL_RPT = 4;      // rows per thread at most
+template <int WF, bool R16, bool NORMLAP>
+__global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* buf0, double* buf1, uint32_t chunk)
+{
+    __shared__ double sm[SMALL_LDS];
+    __shared__ double red[4][LNW];
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const bool lead = tid == 0;
+    if (A.cnt[CT_DONE]) return;
+    const uint32_t n = A.n, nw = A.nwaves;
+    double* xl = sm;
+    double* yl = sm + n;
+    double* pl = sm + 2 * n;
+    double* hl = sm + 3 * n;
+    double* tl = hl + nw;
+    StepArgs B = A;  // the partials go to LDS
+    B.part = pl;
+    B.head = hl;
+    B.tail = tl;
+    const uint32_t gb = NORMLAP ? 0u : *A.gbp;
+    // this thread's rows: range of the first and last entry, weighted degree
+    uint32_t w0[SMALL_RPT], w1[SMALL_RPT], wd[SMALL_RPT];
+    bool has[SMALL_RPT];
+#pragma unroll
+    for (uint32_t k = 0; k < SMALL_RPT; k++) {
+        const uint32_t r = tid + k * LBLOCK;
+        has[k] = false;
+        w0[k] = w1[k] = wd[k] = 0;
+        if (r < n) {
+            const uint32_t s0 = A.rowptr[r], s1 = A.rowptr[r + 1];
+            has[k] = s0 != s1;
+            if (has[k]) {
+                w0[k] = s0 / A.B.pw;
+                w1[k] = (s1 - 1) / A.B.pw;
+            }
+            wd[k] = A.wdeg ? A.wdeg[r] : 0u;
+        }
+    }
+    uint32_t iter = A.cnt[CT_ITER] + 1u;  // 1-based index of the next iteration
+    double T[4], dot0 = A.scal[SC_DOT], dot1 = A.scal[SC_DOT + 1];
+    double last_a = A.scal[SC_A], last_s = A.scal[SC_S];
+#pragma unroll
+    for (int k = 0; k < 4; k++) T[k] = A.fcur[k];  // (fcur[0..4) holds the sums between launches)
+    {
+        const double* y0 = ((iter - 1) & 1u) ? buf1 : buf0;
+        for (uint32_t i = tid; i < n; i += LBLOCK) yl[i] = y0[i];
+    }
+    for (uint32_t it = 0; it < chunk; it++, iter++) {
+        double a = 0.0, s = 1.0;
+        if (iter >= 2) {
+            const double an = T[0] * A.c * A.c;
+            const double nrm2 = T[1] - 2.0 * an * T[0] + an * an * (double)n;
+            const double sn = 1 / sqrt(nrm2);
+            const double dot = (T[2] - an * T[3]) * sn;
+            const unsigned long long itn = iter - 1;
+            const double lastd = (iter & 1u) ? dot1 : dot0;
+            const bool go_on = (fabs(dot - lastd) > A.tol) && (itn < A.iter_max);
+            if ((iter - 1) & 1u)
+                dot1 = dot;
+            else
+                dot0 = dot;
+            if (!go_on) {
+                // x of iteration itn where the result is read: still in LDS,
+                // or left there by the previous launch when this one ran none
+                if (it > 0) {
+                    double* out = ((itn - 1) & 1u) ? buf1 : buf0;
+                    for (uint32_t i = tid; i < n; i += LBLOCK) out[i] = xl[i];
+                }
+                if (lead) {
+                    A.scal[SC_DOT] = dot0;
+                    A.scal[SC_DOT + 1] = dot1;
+                    A.scal[SC_A] = 0.0;
+                    A.scal[SC_S] = 1.0;
+                    A.cnt[CT_ITER] = iter - 1;
+                    A.cnt[CT_DONE] = 1u;
+                    A.cnt[CT_MAXR] = itn >= A.iter_max ? 1u : 0u;
+                    *A.niter = itn;
+                }
+                return;
+            }
+            a = an;
+            s = sn;
+        }
+        last_a = a;
+        last_s = s;
+        // (every read of x by the previous iteration's gathers ended before
+        // its final barrier)
+        for (uint32_t i = tid; i < n; i += LBLOCK) xl[i] = (yl[i] - a) * s;
+        __syncthreads();
+        for (uint32_t wv = wid; wv < nw; wv += LNW) {
+            const uint32_t E0 = wv * A.B.pw;
+            const uint32_t E1 = (A.m - E0 > A.B.pw) ? E0 + A.B.pw : A.m;
+            const uint32_t nrow = (wv + 1 < nw) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+            stream_range<WF, true, R16, double>(B, wv, E0, E1, nrow, a, s, xl);
+        }
+        __syncthreads();
+        double S[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (uint32_t k = 0; k < SMALL_RPT; k++) {
+            const uint32_t r = tid + k * LBLOCK;
+            if (r < n) {
+                // row_total's order: the first partial, then the heads
+                double v = 0.0;
+                if (has[k]) {
+                    v = w0[k] == w1[k] ? pl[r] : tl[w0[k]];
+                    for (uint32_t q = w0[k] + 1; q <= w1[k]; q++) v += hl[q];
+                }
+                const double xr = xl[r];
+                const double y = shifted<NORMLAP>(A, v, xr, wd[k], gb);
+                yl[r] = y;
+                S[0] += NORMLAP ? 0.0 : y;
+                S[1] += y * y;
+                S[2] += xr * y;
+                S[3] += xr;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double w = wave_sum(S[k]);
+            if (lane == 0) red[k][wid] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < LNW; j++) t += red[k][j];
+            T[k] = t;
+        }
+    }
+    // chunk end (iteration iter - 1 >= 1 ran here): its y and x to the
+    // buffers, and the state
+    double* yo = ((iter - 1) & 1u) ? buf1 : buf0;
+    double* xo = ((iter - 1) & 1u) ? buf0 : buf1;
+    for (uint32_t i = tid; i < n; i += LBLOCK) {
+        yo[i] = yl[i];
+        xo[i] = xl[i];
+    }
+    if (lead) {
+        A.cnt[CT_ITER] = iter - 1;
+        A.scal[SC_DOT] = dot0;
+        A.scal[SC_DOT + 1] = dot1;
+        A.scal[SC_A] = last_a;
+        A.scal[SC_S] = last_s;
+#pragma unroll
+        for (int k = 0; k < 4; k++) A.fcur[k] = T[k];
+    }
+}
+
 // u = (y - a) * s : the input of the last iteration
 template <class V>
 __global__ void k_materialize(uint32_t n, const V* __restrict__ y, double* __restrict__ u, const double* scal)
@@ -1746,12 +1908,20 @@ static void run_small(Ctx& c, int normLap, const StepArgs& A, double* b0, double
 {
     const uint32_t SMALL_CHUNK = c.small_chunk;
     const bool r16 = A.r16 != 0;
-#define MI_SMALL(WF, NL)                                                                            \
-    do {                                                                                            \
-        if (r16)                                                                                    \
-            k_power_small<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);         \
-        else                                                                                        \
-            k_power_small<WF, false, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);        \
+    // every vector in LDS when it fits
+    const bool lds = A.n <= SMALL_RPT * LBLOCK && 3ull * A.n + 2ull * A.nwaves <= SMALL_LDS;
+#define MI_SMALL(WF, NL)                                                                               \
+    do {                                                                                               \
+        if (lds) {                                                                                     \
+            if (r16)                                                                                   \
+                k_power_small_lds<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);    \
+            else                                                                                       \
+                k_power_small_lds<WF, false, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);   \
+        } else if (r16) {                                                                              \
+            k_power_small<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);            \
+        } else {                                                                                       \
+            k_power_small<WF, false, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);           \
+        }                                                                                              \
     } while (0)
     for (unsigned long long launched = 0;; launched += SMALL_CHUNK) {
         MI_CHECK(launched <= A.iter_max + SMALL_CHUNK, ST_INTERNAL,

@@ -493,17 +493,21 @@ def test_context_destroyed_before_its_objects():
 
 
 @pytest.mark.parametrize("normLap", [0, 1])
-@pytest.mark.parametrize("case", ["er3000w", "grid64", "rmat11w_hubs"])
+@pytest.mark.parametrize("case", ["er3000w", "grid64", "rmat11w_hubs", "grid100"])
 def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
-    """The one-workgroup loop of small levels against the two-kernel step
-    (same stop rule, sums added in another order: iteration count within
-    2, eigenvectors within 1e-9 in angle) and against the oracle; resumed
-    launches (7 iterations each) equal one launch bit for bit."""
+    """The one-workgroup loop of small levels (vectors in LDS up to about
+    6000 vertices: grid100 takes the form with vectors in global memory)
+    against the two-kernel step (same stop rule, sums added in another
+    order: iteration count within 2, eigenvectors within 1e-9 in angle) and
+    against the oracle; launches resumed every 7 iterations, and every
+    iteration, equal one launch bit for bit."""
     rng = np.random.default_rng(21)
     if case == "er3000w":
         rp, col, w = random_graph(rng, 3000, 0.002, weighted=True, lcc=True)
     elif case == "grid64":
         rp, col, w = graphs.grid2d(64)
+    elif case == "grid100":
+        rp, col, w = graphs.grid2d(100)
     else:
         rp, col, w = graphs.rmat(11, 16, seed=4)
     u0 = rng.random(len(rp) - 1)
@@ -513,11 +517,14 @@ def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
         us, its, mrs = g.power_iter(u0, normLap=normLap, tol=1e-10)
         ctx.set_small_levels(1 << 17, 7)
         uc, itc, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+        ctx.set_small_levels(1 << 17, 1)
+        u1, it1, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
         ctx.set_small_levels(0)
         um, itm, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
     finally:
         ctx.set_small_levels()
     assert itc == its and np.array_equal(uc, us)
+    assert it1 == its and np.array_equal(u1, us)
     assert abs(its - itm) <= 2 and not mrs
     assert abs(us @ um) / (np.linalg.norm(us) * np.linalg.norm(um)) >= 1 - 1e-9
     if normLap == 0:
