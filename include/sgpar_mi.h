@@ -194,6 +194,28 @@ int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config,
                        uint32_t* edge_cut, mi_stats* stats);
 
 /*
+ * Bisections of count independent graphs: the same results, graph by graph
+ * and bit for bit, as mi_partition_graph(ctx, graphs[k], config,
+ * perc_imbalance, hec_seeds[k], &pcg_states[k], pcg_inc, parts[k], ...) for
+ * k = 0 .. count-1.  The reference has no batched form; it is the
+ * sub-bisection work list of the survey's design (mi_bisect_batch).  The
+ * hierarchies are built graph by graph; then the power iterations of the
+ * small levels (those that run in one workgroup with every vector in LDS),
+ * from each graph's coarsest level down, run for all graphs together, one
+ * workgroup per graph and level, level by level; the larger levels and the
+ * splits follow graph by graph.  parts[k] are host arrays of graphs[k]'s n
+ * (device arrays when parts_on_device != 0).  stats (count entries, may be
+ * NULL) receive the sizes, iteration counts, cuts and eigenvalue logs but
+ * no times; phase_ms (3 doubles, may be NULL) receives the times of the
+ * coarsening, the eigensolves and the splits of the whole batch.
+ */
+int mi_bisect_batch(mi_ctx* ctx, int count, const mi_graph* const* graphs,
+                    const mi_config* config, int perc_imbalance, const uint64_t* hec_seeds,
+                    uint64_t* pcg_states, uint64_t pcg_inc, uint32_t* const* parts,
+                    int parts_on_device, uint32_t* edge_cuts, mi_stats* stats,
+                    double* phase_ms);
+
+/*
  * Induced subgraphs of the two sides of part (n entries, 0 = first side,
  * anything else = second; host or device memory): sgp_use_partition,
  * sgpar.h:2666-2752.  Vertices are relabelled in order within their side,

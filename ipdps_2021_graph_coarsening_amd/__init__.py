@@ -108,6 +108,9 @@ _SIGS = [
     ("mi_partition_graph", C.c_int, [vp, vp, C.POINTER(Config), C.c_int, C.c_uint64,
                                      C.POINTER(C.c_uint64), C.c_uint64, vp, C.c_int,
                                      C.POINTER(C.c_uint32), C.POINTER(Stats)]),
+    ("mi_bisect_batch", C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(Config), C.c_int, C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(vp), C.c_int,
+                                  C.POINTER(C.c_uint32), C.POINTER(Stats), C.POINTER(C.c_double)]),
     ("mi_use_partition", C.c_int, [vp, vp, u32p, C.c_int, C.POINTER(vp), C.POINTER(vp)]),
     ("mi_partition_4way", C.c_int, [vp, vp, C.POINTER(Config), C.c_int, C.c_uint64, C.POINTER(C.c_uint64),
                                     C.c_uint64, vp, C.c_int, u32p, C.POINTER(C.c_uint32)]),
@@ -453,6 +456,29 @@ def partition_graph(ctx: Context, g: Graph, config: Config | None = None, perc_i
                                         pcg_inc, _ptr(part), 0, C.byref(ec), C.byref(stats)),
               "mi_partition_graph")
     return PartitionResult(part=part, edge_cut=ec.value, stats=stats.as_dict(), pcg_state=st.value)
+
+
+def bisect_batch(ctx: Context, graphs: list, config: Config | None = None, perc_imbalance: int = 0,
+                 hec_seeds=None, pcg_states=None, pcg_inc: int = 1):
+    """Bisections of many graphs with the small levels' power iterations run
+    together (mi_bisect_batch); per graph the same result as partition_graph.
+    Returns (list of PartitionResult, phase times in ms: coarsen, eigen, split)."""
+    if config is None:
+        config = Config(4, 0, 0, 1, 1e-10, 2)
+    K = len(graphs)
+    seeds = (C.c_uint64 * max(K, 1))(*(hec_seeds if hec_seeds is not None else [0] * K))
+    states = (C.c_uint64 * max(K, 1))(*(pcg_states if pcg_states is not None else [0] * K))
+    parts = [np.empty(g.n, dtype=np.uint32) for g in graphs]
+    hs = (C.c_void_p * max(K, 1))(*[g.h for g in graphs])
+    pp = (C.c_void_p * max(K, 1))(*[p.ctypes.data for p in parts])
+    cuts = (C.c_uint32 * max(K, 1))()
+    stats = (Stats * max(K, 1))()
+    ph = (C.c_double * 3)()
+    ctx.check(ctx._L.mi_bisect_batch(ctx.h, K, hs, C.byref(config), perc_imbalance, seeds, states, pcg_inc, pp, 0,
+                                     cuts, stats, ph), "mi_bisect_batch")
+    res = [PartitionResult(part=parts[k], edge_cut=cuts[k], stats=stats[k].as_dict(), pcg_state=states[k])
+           for k in range(K)]
+    return res, list(ph)
 
 
 def partition_4way(ctx: Context, g: Graph, config: Config | None = None, perc_imbalance: int = 0,

@@ -313,17 +313,9 @@ void fill_level_stats(const std::vector<DLevel>& lv, mi_stats* st)
     }
 }
 
-// Returns the finest-level eigenvector in a device buffer.
-DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state, uint64_t pcg_inc, int normLap,
-                     double tol, mi_stats* st)
+// The coarsest level's start vector, normalised.
+DBuf<double> start_vector(Ctx& c, uint32_t gn, uint64_t* pcg_state, uint64_t pcg_inc)
 {
-    const int L = (int)lv.size();
-    MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
-    if (st) {
-        st->has_eigen_log = 0;
-        st->mixed = 0;
-    }
-    const uint32_t gn = lv[L - 1].g.n;
     std::vector<double> cg(gn);
     // eigensolve_kokkos.h:1500-1505: pcg32()/UINT32_MAX, then normalise
     for (uint32_t i = 0; i < gn; i++) {
@@ -337,7 +329,15 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
     DBuf<double> cur(&c.pool, gn);
     h2d(c, cur.get(), cg.data(), (size_t)gn * 8);
     normalize(c, cur.get(), gn);
-    for (int l = L - 1; l >= 0; l--) {
+    return cur;
+}
+
+// The levels `from` .. 0 of the eigensolve, cur holding the start vector of
+// level `from`; returns the finest-level eigenvector.
+DBuf<double> fiedler_levels(Ctx& c, const std::vector<DLevel>& lv, int from, DBuf<double> cur, int normLap,
+                            double tol, mi_stats* st)
+{
+    for (int l = from; l >= 0; l--) {
         PowerStats ps;
         // the reference logs the eigenvalue after the final (finest) solve
         // with the Laplacian (eigensolve_kokkos.h:201-203)
@@ -367,6 +367,20 @@ DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state,
         }
     }
     return cur;
+}
+
+// Returns the finest-level eigenvector in a device buffer.
+DBuf<double> fiedler(Ctx& c, const std::vector<DLevel>& lv, uint64_t* pcg_state, uint64_t pcg_inc, int normLap,
+                     double tol, mi_stats* st)
+{
+    const int L = (int)lv.size();
+    MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
+    if (st) {
+        st->has_eigen_log = 0;
+        st->mixed = 0;
+    }
+    DBuf<double> cur = start_vector(c, lv[L - 1].g.n, pcg_state, pcg_inc);
+    return fiedler_levels(c, lv, L - 1, std::move(cur), normLap, tol, st);
 }
 
 }  // namespace
@@ -710,6 +724,136 @@ int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config, 
             *edge_cut = bisect(c, g->g, config, perc_imbalance, hec_seed, pcg_state, pcg_inc, dp.get(), stats);
             d2h(c, part, dp.get(), (size_t)n * 4);
             MI_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int mi_bisect_batch(mi_ctx* ctx, int count, const mi_graph* const* graphs, const mi_config* config,
+                    int perc_imbalance, const uint64_t* hec_seeds, uint64_t* pcg_states, uint64_t pcg_inc,
+                    uint32_t* const* parts, int parts_on_device, uint32_t* edge_cuts, mi_stats* stats,
+                    double* phase_ms)
+{
+    if (!ctx || count < 0 || (count > 0 && (!graphs || !config || !hec_seeds || !pcg_states || !parts || !edge_cuts)))
+        return ST_ARG;
+    for (int k = 0; k < count; k++)
+        if (!graphs[k] || !parts[k]) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (count == 0) return;
+        check_config(config);
+        const int normLap = config->refine_alg;
+        const double tol = config->tol;
+        const size_t K = (size_t)count;
+        auto t0 = std::chrono::steady_clock::now();
+        // hierarchies, graph by graph
+        std::vector<std::vector<DLevel>> H(K);
+        for (size_t k = 0; k < K; k++) {
+            mi_stats* st = stats ? &stats[k] : nullptr;
+            if (st) std::memset(st, 0, sizeof(*st));
+            build_hierarchy(c, graphs[k]->g, hec_seeds[k], H[k], st);
+            fill_level_stats(H[k], st);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        auto t1 = std::chrono::steady_clock::now();
+        // start vectors (consuming each graph's pcg state, as one call would)
+        // and the chains of small levels from the coarsest: level l of graph
+        // k is batched when every coarser level is
+        std::vector<DBuf<double>> cur(K);
+        std::vector<int> lo(K);
+        int depth = 0;
+        for (size_t k = 0; k < K; k++) {
+            const int L = (int)H[k].size();
+            MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
+            cur[k] = start_vector(c, H[k][L - 1].g.n, &pcg_states[k], pcg_inc);
+            int l = L - 1;
+            while (l >= 0) {
+                const DGraph& g = H[k][l].g;
+                const bool mixed = c.mixed && l > 0 && g.m >= c.mixed_min_m;
+                if (!small_lds_level(c, g, mixed)) break;
+                l--;
+            }
+            lo[k] = l + 1;
+            depth = std::max(depth, L - lo[k]);
+            if (stats) {
+                stats[k].has_eigen_log = 0;
+                stats[k].mixed = 0;
+            }
+        }
+        std::vector<std::vector<DBuf<double>>> out(K);
+        for (size_t k = 0; k < K; k++) out[k].resize(H[k].size());
+        for (int d = 0; d < depth; d++) {
+            std::vector<BatchLevel> jobs;
+            std::vector<std::pair<size_t, int>> who;
+            for (size_t k = 0; k < K; k++) {
+                const int L = (int)H[k].size(), l = L - 1 - d;
+                if (l < lo[k]) continue;
+                BatchLevel b;
+                b.g = &H[k][l].g;
+                b.start = d == 0 ? cur[k].get() : out[k][l + 1].get();
+                b.vcmap = d == 0 ? nullptr : H[k][l].vcmap.get();
+                out[k][l] = DBuf<double>(&c.pool, H[k][l].g.n);
+                b.out = out[k][l].get();
+                jobs.push_back(b);
+                who.emplace_back(k, l);
+            }
+            power_iter_small_batch(c, jobs, normLap, tol);
+            for (size_t j = 0; j < jobs.size(); j++) {
+                const size_t k = who[j].first;
+                const int l = who[j].second;
+                if (stats && l < MI_MAX_LEVELS) {
+                    stats[k].level_iters[l] = jobs[j].iters;
+                    stats[k].level_max_reached[l] = jobs[j].max_reached;
+                }
+            }
+        }
+        // the remaining (larger) levels, graph by graph
+        std::vector<DBuf<double>> ev(K);
+        for (size_t k = 0; k < K; k++) {
+            mi_stats* st = stats ? &stats[k] : nullptr;
+            const int L = (int)H[k].size();
+            if (lo[k] == L) {
+                ev[k] = fiedler_levels(c, H[k], L - 1, std::move(cur[k]), normLap, tol, st);
+            } else if (lo[k] == 0) {
+                ev[k] = std::move(out[k][0]);
+                if (st && !normLap) {
+                    double elog[3];
+                    eigen_log_level(c, H[k][0].g, ev[k].get(), elog);
+                    st->has_eigen_log = 1;
+                    st->eigenvalue = elog[0];
+                    st->eigen_min = elog[1];
+                    st->eigen_max = elog[2];
+                }
+            } else {
+                const int l = lo[k] - 1;
+                DBuf<double> fine(&c.pool, H[k][l].g.n);
+                prolong(c, H[k][l].vcmap.get(), H[k][l].g.n, out[k][l + 1].get(), fine.get());
+                ev[k] = fiedler_levels(c, H[k], l, std::move(fine), normLap, tol, st);
+            }
+            out[k].clear();
+            cur[k] = DBuf<double>();
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        auto t2 = std::chrono::steady_clock::now();
+        H.clear();
+        for (size_t k = 0; k < K; k++) {
+            const DGraph& g = graphs[k]->g;
+            DBuf<uint32_t> dp;
+            uint32_t* d_part = parts[k];
+            if (!parts_on_device) {
+                dp = DBuf<uint32_t>(&c.pool, g.n);
+                d_part = dp.get();
+            }
+            edge_cuts[k] = compute_partition(c, g, ev[k].get(), config->num_partitions, perc_imbalance, d_part);
+            if (!parts_on_device) d2h(c, parts[k], d_part, (size_t)g.n * 4);
+            if (stats) stats[k].edge_cut = edge_cuts[k];
+            ev[k] = DBuf<double>();
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        auto t3 = std::chrono::steady_clock::now();
+        if (phase_ms) {
+            phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            phase_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            phase_ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
         }
     });
 }

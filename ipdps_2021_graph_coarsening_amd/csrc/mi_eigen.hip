@@ -1046,14 +1046,38 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
 // global memory; when its chunk ends it leaves y of its last iteration in
 // that iteration's output buffer and x of that iteration in the other one.
 // At the stop, x of the last iteration is in the buffer the result is read
-// from, with a = 0 and s = 1.
+// from, with a = 0 and s = 1 (and in `out` when given).
+//
+// The first launch also prepares the start vector, as the two-kernel loop's
+// setup kernels do (eigensolve_kokkos.h:118-150): u = start (or
+// start[vcmap[i]], the prolongation from the coarser level), orthogonalised
+// against 1n (normLap 0: u -= (u . 1n) 1n; normLap 1: u -= (sum u D 1n /
+// sum 1n D 1n) 1n) and normalised, with the sums taken in this workgroup.
+// The batched form runs this same function, one workgroup per level of a
+// graph, so its results equal the one-graph calls bit for bit.
 constexpr uint32_t SMALL_LDS = 18432;  // doubles (144 KiB)
 constexpr uint32_t SMALL_RPT = 4;      // rows per thread at most
-template <int WF, bool R16, bool NORMLAP>
-__global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* buf0, double* buf1, uint32_t chunk)
+
+// Sum of v over the workgroup, in every thread (a fixed order); red[0] is
+// reused, behind barriers on both sides.
+__device__ __forceinline__ double small_block_sum(double v, double (*red)[LNW])
 {
-    __shared__ double sm[SMALL_LDS];
-    __shared__ double red[4][LNW];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[0][wid] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < LNW; j++) t += red[0][j];
+    return t;
+}
+
+template <int WF, bool R16, bool NORMLAP>
+__device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, double* buf1, uint32_t chunk,
+                                           const double* start, const uint32_t* vcmap, double* out, double* sm,
+                                           double (*red)[LNW])
+{
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const bool lead = tid == 0;
     if (A.cnt[CT_DONE]) return;
@@ -1091,7 +1115,32 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* 
     double last_a = A.scal[SC_A], last_s = A.scal[SC_S];
 #pragma unroll
     for (int k = 0; k < 4; k++) T[k] = A.fcur[k];  // (fcur[0..4) holds the sums between launches)
-    {
+    if (iter == 1) {
+        // the start vector, orthogonalised and normalised
+        const double c = A.c;
+        double p0 = 0.0, p1 = 0.0;
+        for (uint32_t i = tid; i < n; i += LBLOCK) {
+            const double u = vcmap ? start[vcmap[i]] : start[i];
+            yl[i] = u;
+            if (!NORMLAP) {
+                p0 += u * c;
+            } else {
+                const double d = (double)A.wdeg[i];
+                p0 += u * d * c;
+                p1 += c * d * c;
+            }
+        }
+        const double num = small_block_sum(p0, red);
+        const double shift = NORMLAP ? num * c / small_block_sum(p1, red) : num * c;
+        double q = 0.0;
+        for (uint32_t i = tid; i < n; i += LBLOCK) {
+            const double u = yl[i] - shift;
+            yl[i] = u;
+            q += u * u;
+        }
+        const double inv = 1 / sqrt(small_block_sum(q, red));
+        for (uint32_t i = tid; i < n; i += LBLOCK) yl[i] = yl[i] * inv;
+    } else {
         const double* y0 = ((iter - 1) & 1u) ? buf1 : buf0;
         for (uint32_t i = tid; i < n; i += LBLOCK) yl[i] = y0[i];
     }
@@ -1112,10 +1161,11 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* 
             if (!go_on) {
                 // x of iteration itn where the result is read: still in LDS,
                 // or left there by the previous launch when this one ran none
-                if (it > 0) {
-                    double* out = ((itn - 1) & 1u) ? buf1 : buf0;
-                    for (uint32_t i = tid; i < n; i += LBLOCK) out[i] = xl[i];
-                }
+                double* res = ((itn - 1) & 1u) ? buf1 : buf0;
+                if (it > 0)
+                    for (uint32_t i = tid; i < n; i += LBLOCK) res[i] = xl[i];
+                if (out)
+                    for (uint32_t i = tid; i < n; i += LBLOCK) out[i] = it > 0 ? xl[i] : res[i];
                 if (lead) {
                     A.scal[SC_DOT] = dot0;
                     A.scal[SC_DOT + 1] = dot1;
@@ -1195,6 +1245,62 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* 
 #pragma unroll
         for (int k = 0; k < 4; k++) A.fcur[k] = T[k];
     }
+}
+
+// Whether a small level's vectors fit in LDS together (k_power_small_lds).
+bool small_lds_fits(uint32_t n, uint32_t nwaves)
+{
+    return n <= SMALL_RPT * LBLOCK && 3ull * n + 2ull * nwaves <= SMALL_LDS;
+}
+
+template <int WF, bool R16, bool NORMLAP>
+__global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* buf0, double* buf1, uint32_t chunk)
+{
+    __shared__ double sm[SMALL_LDS];
+    __shared__ double red[4][LNW];
+    small_loop<WF, R16, NORMLAP>(A, buf0, buf1, chunk, buf0, nullptr, nullptr, sm, red);
+}
+
+// One level of the batched form: its step arguments (with its own loop
+// state), its two work vectors, where its start vector comes from and where
+// its result goes.
+struct SmallJob {
+    StepArgs A;
+    double* b0;
+    double* b1;
+    const double* start;
+    const uint32_t* vcmap;
+    double* out;
+};
+
+// Independent small levels (of different graphs) in one launch, one
+// workgroup each; the same loop as k_power_small_lds.
+template <bool NORMLAP>
+__global__ void __launch_bounds__(LBLOCK) k_power_small_batch(const SmallJob* __restrict__ jobs, uint32_t chunk)
+{
+    __shared__ double sm[SMALL_LDS];
+    __shared__ double red[4][LNW];
+    const SmallJob& J = jobs[blockIdx.x];
+    const int wf = J.A.wfmt;
+    const bool r16 = J.A.r16 != 0;
+#define MI_SB(WFV, R) small_loop<WFV, R, NORMLAP>(J.A, J.b0, J.b1, chunk, J.start, J.vcmap, J.out, sm, red)
+    if (wf == W_UNIT) {
+        if (r16)
+            MI_SB(W_UNIT, true);
+        else
+            MI_SB(W_UNIT, false);
+    } else if (wf == W_U16) {
+        if (r16)
+            MI_SB(W_U16, true);
+        else
+            MI_SB(W_U16, false);
+    } else {
+        if (r16)
+            MI_SB(W_U32, true);
+        else
+            MI_SB(W_U32, false);
+    }
+#undef MI_SB
 }
 
 // u = (y - a) * s : the input of the last iteration
@@ -1909,7 +2015,7 @@ static void run_small(Ctx& c, int normLap, const StepArgs& A, double* b0, double
     const uint32_t SMALL_CHUNK = c.small_chunk;
     const bool r16 = A.r16 != 0;
     // every vector in LDS when it fits
-    const bool lds = A.n <= SMALL_RPT * LBLOCK && 3ull * A.n + 2ull * A.nwaves <= SMALL_LDS;
+    const bool lds = small_lds_fits(A.n, A.nwaves);
 #define MI_SMALL(WF, NL)                                                                               \
     do {                                                                                               \
         if (lds) {                                                                                     \
@@ -1962,9 +2068,15 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     plan_level(c, g, plan, mixed);
     weighted_degree(c, g, plan, wdeg.get(), gb.get());
 
+    // small levels iterate in one workgroup; with every vector in LDS that
+    // workgroup also prepares the start vector
+    const bool small = !mixed && plan.ncb == 1 && g.m > 0 && g.m <= c.small_max_m;
+    const bool small_lds = small && small_lds_fits(n, plan.nwaves);
     // vec1 = 1, normalised: every entry is 1/sqrt(n) (n is exact in FP64)
     const double cval = 1 / std::sqrt((double)n);
-    if (!normLap) {
+    if (small_lds) {
+        // (k_power_small_lds orthogonalises and normalises d_u itself)
+    } else if (!normLap) {
         k_sum_c<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.partials, c.cnt, c.scal, SC_MULT);
         MI_HIP(hipGetLastError());
         k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 0);
@@ -1974,7 +2086,7 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         k_orth_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, cval, c.scal, 1);
     }
     MI_HIP(hipGetLastError());
-    normalize(c, d_u, n);
+    if (!small_lds) normalize(c, d_u, n);
     k_loop_init<<<1, WAVE, 0, c.stream>>>(c.scal, c.cnt, c.niter);
     MI_HIP(hipGetLastError());
 
@@ -2005,7 +2117,7 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
     }
     // Small levels run in one workgroup (no mixed precision there: they are
     // latency-bound); the others in launches of the two-kernel step.
-    if (!mixed && plan.ncb == 1 && g.m > 0 && g.m <= c.small_max_m)
+    if (small)
         run_small(c, normLap, A[0], static_cast<double*>(buf[0]), static_cast<double*>(buf[1]));
     else
         run_steps(c, normLap, A);
@@ -2030,6 +2142,91 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         st->max_reached = flags[1] ? 1 : 0;
         st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+}
+
+bool small_lds_level(const Ctx& c, const DGraph& g, bool mixed)
+{
+    if (mixed || g.m == 0 || g.m > c.small_max_m || g.n > XB) return false;
+    const uint32_t pw = g.m >= (1u << 22) ? 2 * PASS : PASS;  // plan_level's range size
+    return small_lds_fits(g.n, (uint32_t)ceil_div(g.m, pw));
+}
+
+void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, double tol)
+{
+    const size_t J = lv.size();
+    if (!J) return;
+    struct Own {
+        LevelPlan plan;
+        DBuf<uint32_t> wdeg, gb;
+        DBuf<double> b0, b1;
+    };
+    std::vector<Own> own(J);
+    // per job: 8 counters, 16 scalars, the iteration count (zero = a fresh loop)
+    DBuf<uint32_t> cnt(&c.pool, 8 * J);
+    DBuf<double> scal(&c.pool, 16 * J);
+    DBuf<unsigned long long> nit(&c.pool, J);
+    MI_HIP(hipMemsetAsync(cnt.get(), 0, 32 * J, c.stream));
+    MI_HIP(hipMemsetAsync(scal.get(), 0, 128 * J, c.stream));
+    MI_HIP(hipMemsetAsync(nit.get(), 0, 8 * J, c.stream));
+    std::vector<SmallJob> host(J);
+    unsigned long long cap = 0;
+    for (size_t j = 0; j < J; j++) {
+        const DGraph& g = *lv[j].g;
+        Own& o = own[j];
+        plan_level(c, g, o.plan, false);
+        MI_CHECK(o.plan.ncb == 1 && small_lds_fits(g.n, o.plan.nwaves), ST_INTERNAL,
+                 "batched level does not fit the small-level form");
+        o.wdeg = DBuf<uint32_t>(&c.pool, g.n);
+        o.gb = DBuf<uint32_t>(&c.pool, 1);
+        weighted_degree(c, g, o.plan, o.wdeg.get(), o.gb.get());
+        o.b0 = DBuf<double>(&c.pool, g.n);
+        o.b1 = DBuf<double>(&c.pool, g.n);
+        SmallJob& h = host[j];
+        h.A = make_args(c, g, o.plan, o.wdeg.get(), o.gb.get(), o.b0.get(), o.b1.get(), tol, 0);
+        h.A.cnt = cnt.get() + 8 * j;
+        h.A.scal = scal.get() + 16 * j;
+        h.A.niter = nit.get() + j;
+        h.A.fprev = o.plan.fsum.get();
+        h.A.fcur = o.plan.fsum.get();
+        h.b0 = o.b0.get();
+        h.b1 = o.b1.get();
+        h.start = lv[j].start;
+        h.vcmap = lv[j].vcmap;
+        h.out = lv[j].out;
+        cap = std::max(cap, h.A.iter_max);
+    }
+    DBuf<uint64_t> jobs(&c.pool, (J * sizeof(SmallJob) + 7) / 8);
+    MI_HIP(hipMemcpyAsync(jobs.get(), host.data(), J * sizeof(SmallJob), hipMemcpyHostToDevice, c.stream));
+    const SmallJob* dj = reinterpret_cast<const SmallJob*>(jobs.get());
+    std::vector<uint32_t> hc(8 * J);
+    for (unsigned long long launched = 0;; launched += c.small_chunk) {
+        MI_CHECK(launched <= cap + c.small_chunk, ST_INTERNAL, "power iteration did not stop at its iteration cap");
+        if (normLap)
+            k_power_small_batch<true><<<(unsigned)J, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
+        else
+            k_power_small_batch<false><<<(unsigned)J, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
+        MI_HIP(hipGetLastError());
+        MI_HIP(hipMemcpyAsync(hc.data(), cnt.get(), 32 * J, hipMemcpyDeviceToHost, c.stream));
+        MI_HIP(hipStreamSynchronize(c.stream));
+        bool all = true;
+        for (size_t j = 0; j < J && all; j++) all = hc[8 * j + CT_DONE] != 0;
+        if (all) break;
+    }
+    std::vector<unsigned long long> hn(J);
+    MI_HIP(hipMemcpyAsync(hn.data(), nit.get(), 8 * J, hipMemcpyDeviceToHost, c.stream));
+    MI_HIP(hipStreamSynchronize(c.stream));
+    for (size_t j = 0; j < J; j++) {
+        lv[j].iters = hn[j];
+        lv[j].max_reached = hc[8 * j + CT_MAXR] ? 1 : 0;
+    }
+}
+
+void eigen_log_level(Ctx& c, const DGraph& g, const double* d_u, double* out3)
+{
+    LevelPlan plan;
+    plan_level(c, g, plan);
+    DBuf<double> tmp(&c.pool, g.n);
+    eigenvalue_log(c, g, plan, d_u, tmp.get(), out3);
 }
 
 }  // namespace mi

@@ -165,6 +165,25 @@ struct PowerStats {
 // deflation, sums, normalisation and stopping test stay FP64.
 void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, PowerStats* st,
                 double* eig_log = nullptr, bool mixed = false);
+// A small level (small_lds_level) solved by power_iter_small_batch: its start
+// vector is start[vcmap[i]] (start[i] when vcmap is null), orthogonalised
+// and normalised as power_iter does; its result goes to out (n doubles).
+struct BatchLevel {
+    const DGraph* g = nullptr;
+    const double* start = nullptr;
+    const uint32_t* vcmap = nullptr;
+    double* out = nullptr;
+    uint64_t iters = 0;  // results
+    int max_reached = 0;
+};
+// Whether power_iter runs level g in one workgroup with every vector in LDS
+// (so that it can be batched); mixed: the level would run in mixed precision.
+bool small_lds_level(const Ctx& c, const DGraph& g, bool mixed);
+// The FP64 power iterations of independent small levels in one launch (one
+// workgroup each): the same results, bit for bit, as power_iter on each.
+void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, double tol);
+// The final eigenvalue log (eigenvalue, min, max) of level g for vector d_u.
+void eigen_log_level(Ctx& c, const DGraph& g, const double* d_u, double* out3);
 // out[i] = in[vcmap[i]]
 void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, double* out);
 // u /= ||u||

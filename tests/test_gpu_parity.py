@@ -498,9 +498,9 @@ def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
     """The one-workgroup loop of small levels (vectors in LDS up to about
     6000 vertices: grid100 takes the form with vectors in global memory)
     against the two-kernel step (same stop rule, sums added in another
-    order: iteration count within 2, eigenvectors within 1e-9 in angle) and
-    against the oracle; launches resumed every 7 iterations, and every
-    iteration, equal one launch bit for bit."""
+    order, the start vector's too: the FP64 tolerances used against the
+    oracle) and against the oracle; launches resumed every 7 iterations, and
+    every iteration, equal one launch bit for bit."""
     rng = np.random.default_rng(21)
     if case == "er3000w":
         rp, col, w = random_graph(rng, 3000, 0.002, weighted=True, lcc=True)
@@ -525,8 +525,8 @@ def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
         ctx.set_small_levels()
     assert itc == its and np.array_equal(uc, us)
     assert it1 == its and np.array_equal(u1, us)
-    assert abs(its - itm) <= 2 and not mrs
-    assert abs(us @ um) / (np.linalg.norm(us) * np.linalg.norm(um)) >= 1 - 1e-9
+    assert abs(its - itm) <= max(2, ITER_SLACK * itm) and not mrs
+    assert abs(us @ um) / (np.linalg.norm(us) * np.linalg.norm(um)) >= EV_COS
     if normLap == 0:
         # (with normLap on the ER graph u . v peaks near iteration 300, so
         # |d(dot)| dips through 1e-10 there and whether it stops depends on
