@@ -23,7 +23,8 @@ it to one whole bisection.  `mixed` is the same bisection in mixed precision
 (FP32 vectors on the levels above the finest with at least 2^20 entries,
 FP64 sums, deflation, stopping test and finest level) beside the FP64 one;
 `mixed_grid3d` the same comparison on the 128^3 7-point grid with weights
-U{1..16}.
+U{1..16}.  `batched_rgg512` times the batched bisection of 512 random
+geometric graphs (mi_bisect_batch) against one call per graph.
 
     python bench.py [--steps K] [--warmup W]
     python bench.py --roofline-only      # only the roofline measurement (for rocprofv3 runs)
@@ -132,6 +133,38 @@ def grid3d_mixed(mi, ctx, steps: int) -> dict:
     return out
 
 
+def rgg_batch(mi, ctx, reps: int = 2) -> dict:
+    """512 random geometric graphs (unit square, seed 3, n log-uniform in
+    [1000, 200000], radius for mean degree 10, largest component): the
+    batched bisection (mi_bisect_batch) against one call per graph, both
+    timed on the host clock; the results are identical graph by graph."""
+    from ipdps_2021_graph_coarsening_amd import graphs
+    rng = np.random.default_rng(3)
+    ns = np.exp(rng.uniform(np.log(1e3), np.log(2e5), 512)).astype(int)
+    t = time.perf_counter()
+    G = [ctx.graph(*graphs.rgg(int(n), seed=3 + i)[:2]) for i, n in enumerate(ns)]
+    log(f"512 RGGs: {sum(g.n for g in G)} vertices, {sum(g.m for g in G)} entries, "
+        f"generated in {time.perf_counter() - t:.1f} s")
+    seeds, states = [1] * len(G), [1] * len(G)
+    best, ph = None, None
+    for _ in range(reps):
+        t = time.perf_counter()
+        res, p = mi.bisect_batch(ctx, G, hec_seeds=seeds, pcg_states=states)
+        dt = (time.perf_counter() - t) * 1000.0
+        if best is None or dt < best:
+            best, ph = dt, p
+    t = time.perf_counter()
+    cuts = [mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1).edge_cut for g in G]
+    seq = (time.perf_counter() - t) * 1000.0
+    same = cuts == [r.edge_cut for r in res]
+    for g in G:
+        g.close()
+    return {"workload": "512 RGGs, n log-uniform in [1e3, 2e5], seed 3", "graphs": len(G),
+            "batch_ms": round(best, 1), "one_call_per_graph_ms": round(seq, 1), "speedup": round(seq / best, 3),
+            "batch_phases_ms": {"coarsen": round(ph[0], 1), "eigen": round(ph[1], 1), "split": round(ph[2], 1)},
+            "cuts_equal_one_call_per_graph": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +172,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched measurement")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline measurement of the dominant kernel, plus a 256 MiB device copy "
                          "that calibrates rocprofv3's byte counters")
@@ -261,6 +295,11 @@ def main() -> int:
         log(f"grid3d-128: FP64 {g3['fp64_ms']:.2f} ms, mixed {g3['ms']:.2f} ms, cut change "
             f"{g3['cut_change_pct']}% ({time.perf_counter() - t:.1f} s)")
 
+    batch = None
+    if not args.roofline_only and not args.no_batch:
+        batch = rgg_batch(mi, ctx)
+        log(f"512 RGGs: batch {batch['batch_ms']} ms, one call per graph {batch['one_call_per_graph_ms']} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         t = time.perf_counter()
@@ -293,6 +332,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "mixed": mixed,
         "mixed_grid3d": g3,
+        "batched_rgg512": batch,
     }
     print(json.dumps(out), flush=True)
     g.close()
