@@ -173,6 +173,10 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched measurement")
+    ap.add_argument("--grid2d", type=int, default=0,
+                    help="bisect a k x k grid instead of the R-MAT graph (implies --no-extras)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the plain FP64 R-MAT bisection and its roofline (no mixed, 3D grid or batch)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline measurement of the dominant kernel, plus a 256 MiB device copy "
                          "that calibrates rocprofv3's byte counters")
@@ -190,9 +194,16 @@ def main() -> int:
     from ipdps_2021_graph_coarsening_amd import graphs
 
     t = time.perf_counter()
-    rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=SEED)
+    if args.grid2d:
+        # (a graph whose hierarchy is mostly small levels, for profiling those)
+        rp, col, w = graphs.grid2d(args.grid2d)
+        name = f"grid2d-{args.grid2d}"
+        args.no_extras = True
+    else:
+        rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=SEED)
+        name = f"R-MAT-{args.scale}"
     n, m = len(rp) - 1, int(rp[-1])
-    log(f"R-MAT-{args.scale}: n={n} m={m} generated in {time.perf_counter() - t:.1f} s")
+    log(f"{name}: n={n} m={m} generated in {time.perf_counter() - t:.1f} s")
 
     ctx = mi.Context(0)
     ctx.graph(np.array([0, 1, 2], dtype=np.uint32), np.array([1, 0], dtype=np.uint32)).close()  # first-use costs
@@ -239,13 +250,15 @@ def main() -> int:
             f"iterations per level {stats['level_iters']}")
         # the same bisection in mixed precision (FP32 on the levels above the
         # finest with at least 2^20 entries, FP64 finest level)
-        ctx.set_precision(True)
-        ms_mixed, st_mixed = timed()
-        ctx.set_precision(False)
+        mixed = None
+        if not args.no_extras:
+            ctx.set_precision(True)
+            ms_mixed, st_mixed = timed()
+            ctx.set_precision(False)
+            mixed = mixed_summary(ms, stats, ms_mixed, st_mixed)
+            log(f"mixed precision: {ms_mixed:.2f} ms/bisection, cut {st_mixed['edge_cut']}, "
+                f"iterations per level {st_mixed['level_iters']}")
         hip.hipFree(d_part)
-        mixed = mixed_summary(ms, stats, ms_mixed, st_mixed)
-        log(f"mixed precision: {ms_mixed:.2f} ms/bisection, cut {st_mixed['edge_cut']}, "
-            f"iterations per level {st_mixed['level_iters']}")
 
     # roofline of the dominant kernel: the streaming kernel of the power step
     # on the level with the most eigensolve time
@@ -264,12 +277,12 @@ def main() -> int:
     log(f"level {lvl} (n={gl.n}, m={gl.m}): step {b['step_ms'] * 1000:.1f} us, "
         f"k_power_stream {b['stream_ms'] * 1000:.1f} us/launch, {b['stream_bytes'] / 1e6:.1f} MB algorithmic, "
         f"{achieved:.0f} GB/s")
-    traffic = measured_traffic() if (lvl == 1 and args.scale == 20) else None
+    traffic = measured_traffic() if (lvl == 1 and args.scale == 20 and not args.grid2d) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": ((f"k_power_stream_lds ({b['column_blocks']} LDS column blocks)" if b["column_blocks"]
                             else "k_power_stream (gathers from global memory)")
-                           + f", FP64 power-iteration step, R-MAT-{args.scale} level {lvl}"),
+                           + f", FP64 power-iteration step, {name} level {lvl}"),
                 "level_n": gl.n, "level_m": gl.m,
                 "alg_bytes_per_launch": b["stream_bytes"], "impl_bytes_per_launch": b["impl_bytes"],
                 "launch_us": round(b["stream_ms"] * 1000, 2),
@@ -289,14 +302,14 @@ def main() -> int:
         return 0
 
     g3 = None
-    if not args.roofline_only:
+    if not args.roofline_only and not args.no_extras:
         t = time.perf_counter()
         g3 = grid3d_mixed(mi, ctx, max(1, min(args.steps, 5)))
         log(f"grid3d-128: FP64 {g3['fp64_ms']:.2f} ms, mixed {g3['ms']:.2f} ms, cut change "
             f"{g3['cut_change_pct']}% ({time.perf_counter() - t:.1f} s)")
 
     batch = None
-    if not args.roofline_only and not args.no_batch:
+    if not args.roofline_only and not args.no_batch and not args.no_extras:
         batch = rgg_batch(mi, ctx)
         log(f"512 RGGs: batch {batch['batch_ms']} ms, one call per graph {batch['one_call_per_graph_ms']} ms")
 
@@ -319,7 +332,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"rmat{args.scale}-ef16-lcc-weighted plain-fp64 bisection",
+        "config": {"workload": (f"grid2d-{args.grid2d} unit-weight plain-fp64 bisection" if args.grid2d else
+                                f"rmat{args.scale}-ef16-lcc-weighted plain-fp64 bisection"),
                    "n": n, "m": m, "seed": SEED, "levels": stats["nlevels"],
                    "upload_ms": round(upload_ms, 3),
                    "level_n": stats["level_n"], "level_iters": stats["level_iters"],
