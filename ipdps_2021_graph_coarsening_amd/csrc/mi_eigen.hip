@@ -45,6 +45,13 @@
 // ranges, tail[first] + head[first+1] + ... in range order; then the shift,
 // y, and the four sums, reduced per workgroup.  Every sum has a fixed order,
 // so results are bit-identical run to run.
+//
+// k_power_small_lds / k_power_small: small levels (coarse levels take
+// hundreds of iterations of a few microseconds of work) run the whole loop
+// in one 1024-thread workgroup, with the same range streaming; the batched
+// form (k_power_small_batch) runs one such workgroup per level of many
+// graphs.  Value type V = float is the mixed-precision form of the two
+// kernels above.
 #include "mi_kernels.hpp"
 
 #include <algorithm>
