@@ -170,7 +170,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--scale", type=int, default=20,
+                    help="R-MAT scale; from 22 on the graph is drawn on the GPU (graphs.rmat_device, seed 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched measurement")
     ap.add_argument("--grid2d", type=int, default=0,
@@ -199,6 +200,10 @@ def main() -> int:
         rp, col, w = graphs.grid2d(args.grid2d)
         name = f"grid2d-{args.grid2d}"
         args.no_extras = True
+    elif args.scale >= 22:
+        # (drawn on the GPU: the host generator takes minutes at these sizes)
+        rp, col, w = graphs.rmat_device(args.scale, 16, 0.57, 0.19, 0.19, seed=4)
+        name = f"R-MAT-{args.scale}"
     else:
         rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=SEED)
         name = f"R-MAT-{args.scale}"
@@ -314,7 +319,7 @@ def main() -> int:
         log(f"512 RGGs: batch {batch['batch_ms']} ms, one call per graph {batch['one_call_per_graph_ms']} ms")
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.scale < 22 and not args.grid2d:  # (the oracle would take hours beyond)
         t = time.perf_counter()
         cpu = cpu_baseline(rp, col, w, stats)
         log(f"cpu baseline {cpu['value']} ms (sampled in {time.perf_counter() - t:.1f} s)")

@@ -32,7 +32,7 @@
 // lanes; a carry joins rows that cross passes.  A row that begins and ends
 // inside the range stores its sum in part[row]; a row entering the range
 // from the previous one leaves its partial in head[w], one leaving it in
-// tail[w].  The _lds form serves levels whose x fits in LDS in at most 8
+// tail[w].  The _lds form serves levels whose x fits in LDS in at most 64
 // column blocks: the entries are laid out block by block (row r of block b
 // is "virtual row" b n + r), each 1024-thread workgroup stages its block's
 // slice of x, already shifted and scaled, in LDS and gathers from there, and
@@ -74,7 +74,9 @@ constexpr int NW = BLOCK / WAVE;        // wavefronts per workgroup, global-gath
 constexpr int LBLOCK = 1024;            // workgroup of the LDS form
 constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
-constexpr int MAXCB = 8;                // column blocks at most
+constexpr int MAXCB = MAX_COLUMN_BLOCKS; // column blocks at most
+constexpr int KB = 8;                   // blocks whose bounds travel in the kernel arguments
+constexpr int RCB = 8;                  // blocks a finish thread handles per group (loads in flight)
 constexpr uint32_t MAXF = 256;          // workgroups of the finish kernel at most (global form)
 constexpr unsigned long long MIXED_ITER_CAP = 20000;  // iterations of one mixed-precision level at most
 constexpr uint32_t MAXF_LDS = LBLOCK;   // LDS form: its 1024-thread workgroups reduce up to 1024 sums
@@ -95,18 +97,58 @@ __device__ __forceinline__ uint32_t logical_block()
 
 // Block layout of a level: block q holds entries [boff[q], boff[q+1]) and its
 // ranges start at range index bwave[q].
+// Up to KB blocks the bounds are kernel arguments, read with static indices
+// (they stay in scalar registers); levels with more blocks (up to MAXCB)
+// read all of them from device memory (boff_all / bwave_all, ncb + 1 each).
 struct Blocks {
     uint32_t ncb, pw, m;
-    uint32_t boff[MAXCB + 1];
-    uint32_t bwave[MAXCB + 1];
+    uint32_t boff[KB + 1];
+    uint32_t bwave[KB + 1];
+    const uint32_t* boff_all;
+    const uint32_t* bwave_all;
 };
 
-// range holding entry e (static indices into the block arrays)
+// The block bounds of a plan as kernel arguments.
+Blocks blocks_of(const LevelPlan& p)
+{
+    Blocks B;
+    B.ncb = p.ncb;
+    B.pw = p.pw;
+    B.m = p.boff[p.ncb > 1 ? p.ncb : 1];
+    for (int q = 0; q <= KB; q++) {
+        B.boff[q] = p.boff[q];
+        B.bwave[q] = p.bwave[q];
+    }
+    B.boff_all = p.blk.get();
+    B.bwave_all = p.blk.get() ? p.blk.get() + p.ncb + 1 : nullptr;
+    return B;
+}
+
+// last q in [0, k) with a[q] <= x (a non-decreasing, a[0] <= x); uniform
+__device__ __forceinline__ uint32_t last_le(const uint32_t* a, uint32_t k, uint32_t x)
+{
+    uint32_t lo = 0, hi = k > 1 ? k - 1 : 0;
+#pragma nounroll
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// range holding entry e
 __device__ __forceinline__ uint32_t range_of(const Blocks& B, uint32_t e)
 {
+    if (B.ncb > KB) {
+        const uint32_t q = last_le(B.boff_all, B.ncb, e);
+        return B.bwave_all[q] + (e - B.boff_all[q]) / B.pw;
+    }
     uint32_t off = B.boff[0], wb = B.bwave[0];
 #pragma unroll
-    for (int q = 1; q < MAXCB; q++) {
+    for (int q = 1; q < KB; q++) {
         if ((uint32_t)q < B.ncb && e >= B.boff[q]) {
             off = B.boff[q];
             wb = B.bwave[q];
@@ -138,18 +180,21 @@ __global__ void __launch_bounds__(BLOCK) k_vcount(uint32_t n, const uint32_t* __
 {
     const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
     if (r >= n) return;
-    uint32_t cnt[MAXCB];
+    // blocks in groups of RCB, the row read once per group
+    for (uint32_t q0 = 0; q0 < ncb; q0 += RCB) {
+        uint32_t cnt[RCB];
 #pragma unroll
-    for (int q = 0; q < MAXCB; q++) cnt[q] = 0;
-    for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) {
-        const uint32_t b = col[e] / bw;
+        for (int q = 0; q < RCB; q++) cnt[q] = 0;
+        for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) {
+            const uint32_t b = col[e] / bw - q0;
 #pragma unroll
-        for (int q = 0; q < MAXCB; q++) cnt[q] += (b == (uint32_t)q) ? 1u : 0u;
-    }
+            for (int q = 0; q < RCB; q++) cnt[q] += (b == (uint32_t)q) ? 1u : 0u;
+        }
 #pragma unroll
-    for (int q = 0; q < MAXCB; q++) {
-        const uint32_t t = wave_sum_u32(cnt[q]);
-        if (lane == 0 && (uint32_t)q < ncb) vcnt[(size_t)q * n + r] = t;
+        for (int q = 0; q < RCB; q++) {
+            const uint32_t t = wave_sum_u32(cnt[q]);
+            if (lane == 0 && q0 + q < ncb) vcnt[(size_t)(q0 + q) * n + r] = t;
+        }
     }
 }
 
@@ -164,26 +209,30 @@ __global__ void __launch_bounds__(BLOCK) k_vscatter(uint32_t n, const uint32_t* 
 {
     const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
     if (r >= n) return;
-    uint32_t pos[MAXCB];
-#pragma unroll
-    for (int q = 0; q < MAXCB; q++) pos[q] = (uint32_t)q < ncb ? vrowptr[(size_t)q * n + r] : 0u;
     const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t e0 = rowptr[r], e1 = rowptr[r + 1];
-    for (uint32_t base = e0; base < e1; base += WAVE) {
-        const uint32_t e = base + lane;
-        const bool in = e < e1;
-        const uint32_t c = in ? col[e] : 0u;
-        const uint32_t b = in ? c / bw : (uint32_t)MAXCB;
-        uint32_t dst = 0;
+    // blocks in groups of RCB, the row read once per group
+    for (uint32_t q0 = 0; q0 < ncb; q0 += RCB) {
+        uint32_t pos[RCB];
 #pragma unroll
-        for (int q = 0; q < MAXCB; q++) {
-            const unsigned long long mask = __ballot(b == (uint32_t)q);
-            if (b == (uint32_t)q) dst = pos[q] + (uint32_t)__popcll(mask & below);
-            pos[q] += (uint32_t)__popcll(mask);
-        }
-        if (in) {
-            vcol[dst] = (uint16_t)(c - b * bw);
-            if (w) vw[dst] = w[e];
+        for (int q = 0; q < RCB; q++) pos[q] = q0 + q < ncb ? vrowptr[(size_t)(q0 + q) * n + r] : 0u;
+        for (uint32_t base = e0; base < e1; base += WAVE) {
+            const uint32_t e = base + lane;
+            const bool in = e < e1;
+            const uint32_t c = in ? col[e] : 0u;
+            const uint32_t b = in ? c / bw : 0xffffffffu;
+            const uint32_t bq = b - q0;  // (wraps above RCB for blocks outside the group)
+            uint32_t dst = 0;
+#pragma unroll
+            for (int q = 0; q < RCB; q++) {
+                const unsigned long long mask = __ballot(bq == (uint32_t)q);
+                if (bq == (uint32_t)q) dst = pos[q] + (uint32_t)__popcll(mask & below);
+                pos[q] += (uint32_t)__popcll(mask);
+            }
+            if (in && bq < (uint32_t)RCB) {
+                vcol[dst] = (uint16_t)(c - b * bw);
+                if (w) vw[dst] = w[e];
+            }
         }
     }
 }
@@ -261,6 +310,12 @@ __global__ void __launch_bounds__(BLOCK) k_w16(uint64_t m, const uint32_t* __res
     if (threadIdx.x == 0) atomicMax(maxw, mx);
 }
 
+// out[q] = vrowptr[q n], q = 0 .. ncb (one workgroup of 128 threads)
+__global__ void k_block_starts(const uint32_t* __restrict__ vrowptr, uint32_t n, uint32_t ncb, uint32_t* out)
+{
+    for (uint32_t q = threadIdx.x; q <= ncb; q += blockDim.x) out[q] = vrowptr[(size_t)q * n];
+}
+
 // range metadata: first row of range w, with HEADF when that row began in
 // an earlier range
 __global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ wmeta)
@@ -268,11 +323,17 @@ __global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ 
     const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
     if (w >= nwaves) return;
     uint32_t e0 = 0, bstart = 0;
+    if (B.ncb > KB) {
+        const uint32_t q = last_le(B.bwave_all, B.ncb, w);
+        e0 = B.boff_all[q] + (w - B.bwave_all[q]) * B.pw;
+        bstart = B.boff_all[q];
+    } else {
 #pragma unroll
-    for (int q = 0; q < MAXCB; q++) {
-        if ((uint32_t)q < (B.ncb ? B.ncb : 1u) && w >= B.bwave[q]) {
-            e0 = B.boff[q] + (w - B.bwave[q]) * B.pw;
-            bstart = B.boff[q];
+        for (int q = 0; q < KB; q++) {
+            if ((uint32_t)q < (B.ncb ? B.ncb : 1u) && w >= B.bwave[q]) {
+                e0 = B.boff[q] + (w - B.bwave[q]) * B.pw;
+                bstart = B.boff[q];
+            }
         }
     }
     const uint32_t r = rid[e0];
@@ -748,7 +809,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
 // [g W / G, (g+1) W / G) in order and stages the slice of x of each column
 // block its share touches.  Which workgroup streams a range does not change
 // what is summed, so the result does not depend on the grid.
-template <int WF, bool R16, class V>
+template <int WF, bool R16, class V, bool MANY>
 __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 {
     constexpr uint32_t XN = XB;
@@ -766,9 +827,13 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     V raw[XPT];
     {
         uint32_t b = 0;
+        if (MANY) {
+            b = last_le(A.B.bwave_all, A.B.ncb, cur);
+        } else {
 #pragma unroll
-        for (int q = 1; q < MAXCB; q++)
-            if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) b = q;
+            for (int q = 1; q < KB; q++)
+                if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) b = q;
+        }
         const uint32_t x0 = b * A.bw;
         const uint32_t xn = cur < R1 ? ((A.n - x0) < A.bw ? (A.n - x0) : A.bw) : 0u;
 #pragma unroll
@@ -783,16 +848,25 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     // from the values loaded above (their registers end there), later ones
     // load again
     auto segment = [&](bool from_raw) {
-        // the column block holding range cur (static indices into the argument arrays)
+        // the column block holding range cur (static indices into the
+        // argument arrays up to KB blocks)
         uint32_t b = 0, off0 = A.B.boff[0], off1 = A.B.boff[1], wbase = A.B.bwave[0], wend = A.B.bwave[1];
+        if (MANY) {
+            b = last_le(A.B.bwave_all, A.B.ncb, cur);
+            off0 = A.B.boff_all[b];
+            off1 = A.B.boff_all[b + 1];
+            wbase = A.B.bwave_all[b];
+            wend = A.B.bwave_all[b + 1];
+        } else {
 #pragma unroll
-        for (int q = 1; q < MAXCB; q++) {
-            if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) {
-                b = q;
-                off0 = A.B.boff[q];
-                off1 = A.B.boff[q + 1];
-                wbase = A.B.bwave[q];
-                wend = A.B.bwave[q + 1];
+            for (int q = 1; q < KB; q++) {
+                if ((uint32_t)q < A.B.ncb && cur >= A.B.bwave[q]) {
+                    b = q;
+                    off0 = A.B.boff[q];
+                    off1 = A.B.boff[q + 1];
+                    wbase = A.B.bwave[q];
+                    wend = A.B.bwave[q + 1];
+                }
             }
         }
         const uint32_t seg_end = R1 < wend ? R1 : wend;
@@ -830,54 +904,71 @@ __device__ __forceinline__ double shifted(const StepArgs& A, double spmv, double
     return 0.5 * xr + 0.5 * spmv * (1.0 / (double)wd);
 }
 
-// A x at row r: the row's partials over its NB <= CB column blocks and the
-// ranges it crosses, added in range order.
-template <int CB, class V>
-__device__ __forceinline__ V row_total(const StepArgs& A, uint32_t r, uint32_t nb, const V* part, const V* head,
-                                       const V* tail)
+// A x at row r: the row's partials over its nb column blocks and the ranges
+// it crosses, added in block order and, inside a block, in range order:
+// v = vb(0), then v + vb(q) for every later block with entries.  Blocks are
+// taken G at a time, their bounds and first partials loaded together; the
+// bounds of block q come from the arguments (static indices, nb <= KB) or,
+// with ALL, from device memory.
+template <int G, bool ALL, class V>
+__device__ __forceinline__ void row_blocks(const StepArgs& A, uint32_t r, uint32_t q0, uint32_t nb, const V* part,
+                                           const V* head, const V* tail, V& v)
 {
-    // all blocks' bounds, then all first partials, as independent loads
-    uint32_t s0[CB], s1[CB], w0[CB], w1[CB];
-    V first[CB];
+    uint32_t s0[G], s1[G], w0[G], w1[G];
+    V first[G];
 #pragma unroll
-    for (int q = 0; q < CB; q++) {
-        s0[q] = s1[q] = 0;
-        if ((uint32_t)q < nb) {
-            const uint32_t vr = (uint32_t)q * A.n + r;
-            s0[q] = A.rowptr[vr];
-            s1[q] = A.rowptr[vr + 1];
+    for (int k = 0; k < G; k++) {
+        s0[k] = s1[k] = 0;
+        if (q0 + k < nb) {
+            const uint32_t vr = (q0 + k) * A.n + r;
+            s0[k] = A.rowptr[vr];
+            s1[k] = A.rowptr[vr + 1];
         }
     }
 #pragma unroll
-    for (int q = 0; q < CB; q++) {
-        first[q] = 0;
-        w0[q] = w1[q] = 0;
-        if (s0[q] != s1[q]) {
-            w0[q] = A.B.bwave[q] + (s0[q] - A.B.boff[q]) / A.B.pw;
-            w1[q] = A.B.bwave[q] + (s1[q] - 1 - A.B.boff[q]) / A.B.pw;
-            first[q] = w0[q] == w1[q] ? part[(uint32_t)q * A.n + r] : tail[w0[q]];
+    for (int k = 0; k < G; k++) {
+        first[k] = 0;
+        w0[k] = w1[k] = 0;
+        if (s0[k] != s1[k]) {
+            const uint32_t q = q0 + k;
+            const uint32_t bw0 = ALL ? A.B.bwave_all[q] : A.B.bwave[k];
+            const uint32_t bo0 = ALL ? A.B.boff_all[q] : A.B.boff[k];
+            w0[k] = bw0 + (s0[k] - bo0) / A.B.pw;
+            w1[k] = bw0 + (s1[k] - 1 - bo0) / A.B.pw;
+            first[k] = w0[k] == w1[k] ? part[q * A.n + r] : tail[w0[k]];
         }
     }
-    V v = 0;
 #pragma unroll
-    for (int q = 0; q < CB; q++) {
-        if (s0[q] != s1[q]) {
-            V vb = first[q];
-            if (w1[q] > w0[q]) {
+    for (int k = 0; k < G; k++) {
+        if (s0[k] != s1[k]) {
+            V vb = first[k];
+            if (w1[k] > w0[k]) {
                 // a row crossing ranges: in range order, eight loads at a time
-                uint32_t k = w0[q] + 1;
-                for (; k + 8 <= w1[q] + 1; k += 8) {
+                uint32_t h0 = w0[k] + 1;
+                for (; h0 + 8 <= w1[k] + 1; h0 += 8) {
                     V h[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) h[j] = head[k + j];
+                    for (int j = 0; j < 8; j++) h[j] = head[h0 + j];
 #pragma unroll
                     for (int j = 0; j < 8; j++) vb += h[j];
                 }
-                for (; k <= w1[q]; k++) vb += head[k];
+                for (; h0 <= w1[k]; h0++) vb += head[h0];
             }
-            v = q == 0 ? vb : v + vb;
+            v = q0 + k == 0 ? vb : v + vb;
         }
     }
+}
+
+// MANY: nb > KB, every bound from device memory; else nb <= G.
+template <int G, bool MANY, class V>
+__device__ __forceinline__ V row_total(const StepArgs& A, uint32_t r, uint32_t nb, const V* part, const V* head,
+                                       const V* tail)
+{
+    V v = 0;
+    if (!MANY)
+        row_blocks<G, false>(A, r, 0u, nb, part, head, tail, v);
+    else
+        for (uint32_t q0 = 0; q0 < nb; q0 += RCB) row_blocks<RCB, true>(A, r, q0, nb, part, head, tail, v);
     return v;
 }
 
@@ -887,7 +978,7 @@ __device__ __forceinline__ V row_total(const StepArgs& A, uint32_t r, uint32_t n
 // row are added in V; the shift and y are formed in FP64 and y is stored in
 // V; the four sums are FP64 over the stored values (a fifth, sum x^2, when
 // V is float).
-template <bool NORMLAP, class V>
+template <bool NORMLAP, class V, bool MANY>
 __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
 {
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
@@ -908,7 +999,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
         const V yr = yp[r];
         const uint32_t wd = A.wdeg ? A.wdeg[r] : 0u;
-        const V v = row_total<MAXCB>(A, r, nb, part, head, tail);
+        const V v = row_total<KB, MANY>(A, r, nb, part, head, tail);
         const V xv = xform(yr, a, s);  // as the streaming kernel formed x
         const double xr = (double)xv;
         const V yv = (V)shifted<NORMLAP>(A, (double)v, xr, wd, gb);
@@ -1007,7 +1098,7 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
         __syncthreads();
         double S[4] = {0.0, 0.0, 0.0, 0.0};
         for (uint32_t r = tid; r < A.n; r += LBLOCK) {
-            const double v = row_total<1>(A, r, 1u, part, head, tail);
+            const double v = row_total<1, false>(A, r, 1u, part, head, tail);
             const double xr = xl[r];
             const double y = shifted<NORMLAP>(A, v, xr, A.wdeg ? A.wdeg[r] : 0u, gb);
             yn[r] = y;
@@ -1564,13 +1655,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.wmeta = p.wmeta.get();
     A.wdeg = wdeg;
     A.gbp = gb;
-    A.B.ncb = p.ncb;
-    A.B.pw = p.pw;
-    A.B.m = g.m;
-    for (int q = 0; q <= MAXCB; q++) {
-        A.B.boff[q] = p.boff[q];
-        A.B.bwave[q] = p.bwave[q];
-    }
+    A.B = blocks_of(p);
     A.bw = p.bw;
     A.nwaves = p.nwaves;
     A.part = p.part.get();
@@ -1611,14 +1696,32 @@ void launch_stream_t(Ctx& c, const StepArgs& A)
         else                                                                   \
             K<WF, false, V><<<A.nblk_a, NT, 0, c.stream>>>(A);                 \
     } while (0)
+#define MI_STREAM_LDS(WF, MANY)                                                                 \
+    do {                                                                                        \
+        if (r16)                                                                                \
+            k_power_stream_lds<WF, true, V, MANY><<<A.nblk_a, LBLOCK, 0, c.stream>>>(A);        \
+        else                                                                                    \
+            k_power_stream_lds<WF, false, V, MANY><<<A.nblk_a, LBLOCK, 0, c.stream>>>(A);       \
+    } while (0)
     if (A.B.ncb) {
         // (nblk_a = persistent workgroups of the LDS form)
-        if (A.wfmt == W_UNIT)
-            MI_STREAM(k_power_stream_lds, LBLOCK, W_UNIT);
-        else if (A.wfmt == W_U16)
-            MI_STREAM(k_power_stream_lds, LBLOCK, W_U16);
-        else
-            MI_STREAM(k_power_stream_lds, LBLOCK, W_U32);
+        const bool many = A.B.ncb > (uint32_t)KB;
+        if (A.wfmt == W_UNIT) {
+            if (many)
+                MI_STREAM_LDS(W_UNIT, true);
+            else
+                MI_STREAM_LDS(W_UNIT, false);
+        } else if (A.wfmt == W_U16) {
+            if (many)
+                MI_STREAM_LDS(W_U16, true);
+            else
+                MI_STREAM_LDS(W_U16, false);
+        } else {
+            if (many)
+                MI_STREAM_LDS(W_U32, true);
+            else
+                MI_STREAM_LDS(W_U32, false);
+        }
     } else {
         if (A.wfmt == W_UNIT)
             MI_STREAM(k_power_stream, BLOCK, W_UNIT);
@@ -1628,6 +1731,7 @@ void launch_stream_t(Ctx& c, const StepArgs& A)
             MI_STREAM(k_power_stream, BLOCK, W_U32);
     }
 #undef MI_STREAM
+#undef MI_STREAM_LDS
     MI_HIP(hipGetLastError());
 }
 
@@ -1639,20 +1743,29 @@ void launch_stream(Ctx& c, const StepArgs& A)
         launch_stream_t<double>(c, A);
 }
 
-void launch_finish(Ctx& c, int normLap, const StepArgs& A)
+template <bool MANY>
+void launch_finish_t(Ctx& c, int normLap, const StepArgs& A)
 {
     if (A.f32) {
         if (normLap)
-            k_power_finish<true, float><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+            k_power_finish<true, float, MANY><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
         else
-            k_power_finish<false, float><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+            k_power_finish<false, float, MANY><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
     } else {
         if (normLap)
-            k_power_finish<true, double><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+            k_power_finish<true, double, MANY><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
         else
-            k_power_finish<false, double><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
+            k_power_finish<false, double, MANY><<<A.nblk_f, BLOCK, 0, c.stream>>>(A);
     }
     MI_HIP(hipGetLastError());
+}
+
+void launch_finish(Ctx& c, int normLap, const StepArgs& A)
+{
+    if (A.B.ncb > (uint32_t)KB)
+        launch_finish_t<true>(c, normLap, A);
+    else
+        launch_finish_t<false>(c, normLap, A);
 }
 
 // One step = the streaming kernel then the finish kernel.
@@ -1676,7 +1789,7 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
     uint32_t ncb = 0;
     if (m && need == 1)
         ncb = 1;
-    else if (m && need <= (uint32_t)MAXCB && (uint64_t)m >= 64ull * n)
+    else if (m && need <= (uint32_t)MAXCB && (uint64_t)m >= 64ull * n && (uint64_t)need * n < 0xffffffffull)
         ncb = need;
     p.ncb = ncb;
     p.bw = ncb ? (uint32_t)ceil_div(n, ncb) : 0u;
@@ -1726,10 +1839,14 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         }
         rows = p.vrowptr.get();
         nrows = (uint32_t)nv;
-        std::vector<uint32_t> hv((size_t)nv + 1);
-        MI_HIP(hipMemcpyAsync(hv.data(), p.vrowptr.get(), (nv + 1) * 4, hipMemcpyDeviceToHost, c.stream));
+        // block starts: the virtual row offsets at q n, q = 0 .. ncb
+        DBuf<uint32_t> bo(&c.pool, ncb + 1);
+        k_block_starts<<<1, 128, 0, c.stream>>>(p.vrowptr.get(), n, ncb, bo.get());
+        MI_HIP(hipGetLastError());
+        std::vector<uint32_t> hv(ncb + 1);
+        MI_HIP(hipMemcpyAsync(hv.data(), bo.get(), (ncb + 1) * 4, hipMemcpyDeviceToHost, c.stream));
         MI_HIP(hipStreamSynchronize(c.stream));
-        for (uint32_t q = 0; q <= ncb; q++) p.boff[q] = hv[(size_t)q * n];
+        for (uint32_t q = 0; q <= ncb; q++) p.boff[q] = hv[q];
         MI_CHECK(p.boff[ncb] == m, ST_INTERNAL, "column-block layout lost entries");
     } else {
         // the streaming kernel reads its arrays with 16-byte loads
@@ -1785,14 +1902,18 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         p.boff[q] = p.boff[q - 1];
         p.bwave[q] = p.bwave[q - 1];
     }
-    Blocks B;
-    B.ncb = ncb;
-    B.pw = p.pw;
-    B.m = m;
-    for (int q = 0; q <= MAXCB; q++) {
-        B.boff[q] = p.boff[q];
-        B.bwave[q] = p.bwave[q];
+    if (ncb > (uint32_t)KB) {
+        // all block bounds in device memory (boff then bwave)
+        std::vector<uint32_t> hb(2 * (ncb + 1));
+        for (uint32_t q = 0; q <= ncb; q++) {
+            hb[q] = p.boff[q];
+            hb[ncb + 1 + q] = p.bwave[q];
+        }
+        p.blk = DBuf<uint32_t>(&c.pool, hb.size());
+        MI_HIP(hipMemcpyAsync(p.blk.get(), hb.data(), hb.size() * 4, hipMemcpyHostToDevice, c.stream));
+        MI_HIP(hipStreamSynchronize(c.stream));
     }
+    const Blocks B = blocks_of(p);
     // row ids: per range the first row, per entry its offset from it (16 bits
     // when every range fits, else the 32-bit rows)
     p.wmeta = DBuf<uint32_t>(&c.pool, p.nwaves ? p.nwaves : 1);

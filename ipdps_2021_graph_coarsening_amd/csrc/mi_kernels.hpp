@@ -122,6 +122,7 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
 // block (vrowptr: virtual row q n + r = row r's entries in block q; col_al /
 // w_al hold that order).  Wavefront ranges of pw entries never cross a block:
 // block q holds entries [boff[q], boff[q+1]) and ranges from bwave[q].
+constexpr int MAX_COLUMN_BLOCKS = 64;  // LDS column blocks of a level at most
 struct LevelPlan {
     DBuf<uint32_t> rid, wmeta, vrowptr;
     DBuf<uint16_t> rid16, col16;  // 16-bit row offsets; 16-bit columns within a block
@@ -129,7 +130,8 @@ struct LevelPlan {
     DBuf<uint32_t> col_al, w_al;  // block layout, or 16-byte aligned copies
     DBuf<double> part, head, tail, fsum;
     uint32_t ncb = 0, bw = 0;
-    uint32_t boff[9] = {}, bwave[9] = {};
+    uint32_t boff[MAX_COLUMN_BLOCKS + 1] = {}, bwave[MAX_COLUMN_BLOCKS + 1] = {};
+    DBuf<uint32_t> blk;  // boff then bwave on the device, when more than 8 blocks
     uint32_t pw = 0, nwaves = 0;
     uint32_t nblk_a = 0;  // workgroups of the streaming kernel
     uint32_t nblk_f = 0;  // workgroups of the finish kernel

@@ -216,6 +216,47 @@ def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c:
     return from_edges(1 << scale, u, v, None, lcc=lcc)
 
 
+def rmat_device(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                seed: int = 4, lcc: bool = True):
+    """R-MAT as `rmat`, with the edges drawn, symmetrised and merged on the
+    GPU through torch (scale 24: seconds instead of minutes); the largest
+    component is found on the host.  Another random stream than `rmat`, so
+    another instance of the same distribution."""
+    import torch
+
+    n, m = 1 << scale, edge_factor << scale
+    ab, abc = a + b, a + b + c
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    u = torch.zeros(m, dtype=torch.int64, device=dev)
+    v = torch.zeros(m, dtype=torch.int64, device=dev)
+    for bit in range(scale):
+        r = torch.rand(m, generator=gen, device=dev)
+        u |= (r >= ab).long() << bit
+        v |= (((r >= a) & (r < ab)) | (r >= abc)).long() << bit
+        del r
+    keep = u != v
+    u, v = u[keep], v[keep]
+    key = torch.cat([u * n + v, v * n + u])
+    del u, v, keep
+    key, cnt = torch.unique(key, sorted=True, return_counts=True)
+    if int(cnt.max()) > np.iinfo(np.uint32).max:
+        raise ValueError("merged weight exceeds uint32")
+    rows = (key // n).to(torch.int64).cpu().numpy()
+    cols = (key % n).to(torch.int64).cpu().numpy()
+    w = cnt.to(torch.int64).cpu().numpy()
+    del key, cnt
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if lcc:
+        rows, cols, w, n = _largest_component(n, rows, cols, w)
+    cntr = np.bincount(rows, minlength=n)
+    rowptr = np.zeros(n + 1, dtype=np.uint32)
+    np.cumsum(cntr, out=rowptr[1:])
+    return rowptr, cols.astype(np.uint32), w.astype(np.uint32)
+
+
 def rgg(n: int, radius: float | None = None, seed: int = 3, lcc: bool = True):
     """Random geometric graph in the unit square, unit weights; default
     radius gives mean degree 10."""

@@ -217,7 +217,8 @@ def _dense_graph(n, avg_deg, seed, weighted):
 
 @pytest.mark.parametrize("n,weighted,wscale,shuffle", [(40000, True, 1, False), (70000, False, 1, False),
                                                        (130000, True, 1, False), (40000, True, 9000, False),
-                                                       (40000, True, 1, True)])
+                                                       (40000, True, 1, True), (300000, True, 1, False),
+                                                       (150000, False, 1, True)])
 def test_spmv_shift_column_blocks(ctx, n, weighted, wscale, shuffle):
     rp, col, w = _dense_graph(n, 72, n, weighted)
     if w is not None:
