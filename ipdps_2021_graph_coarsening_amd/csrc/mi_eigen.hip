@@ -1134,8 +1134,9 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
 }
 
 // The same loop with every vector in LDS, for levels where x, y, the row
-// partials and the range partials fit there together (3 n + 2 ranges <=
-// SMALL_LDS doubles, rows <= 4 per thread): only the matrix is read from
+// partials and the range partials fit there together (2 n + 2 ranges <=
+// SMALL_LDS doubles, the row partials sharing y's space; rows <= 8 per
+// thread, instances for 4 and 8): only the matrix is read from
 // global memory (and stays in the caches), and each thread keeps its rows'
 // range indices and weighted degrees in registers for the whole loop.
 // stream_range writes the partials through generic pointers into LDS.  Each
@@ -1154,7 +1155,7 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
 // The batched form runs this same function, one workgroup per level of a
 // graph, so its results equal the one-graph calls bit for bit.
 constexpr uint32_t SMALL_LDS = 18432;  // doubles (144 KiB)
-constexpr uint32_t SMALL_RPT = 4;      // rows per thread at most
+constexpr uint32_t SMALL_RPT = 8;      // rows per thread at most (instances for 4 and 8)
 
 // Sum of v over the workgroup, in every thread (a fixed order); red[0] is
 // reused, behind barriers on both sides.
@@ -1171,7 +1172,7 @@ __device__ __forceinline__ double small_block_sum(double v, double (*red)[LNW])
     return t;
 }
 
-template <int WF, bool R16, bool NORMLAP>
+template <int WF, bool R16, bool NORMLAP, uint32_t RPT>
 __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, double* buf1, uint32_t chunk,
                                            const double* start, const uint32_t* vcmap, double* out, double* sm,
                                            double (*red)[LNW])
@@ -1182,8 +1183,8 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
     const uint32_t n = A.n, nw = A.nwaves;
     double* xl = sm;
     double* yl = sm + n;
-    double* pl = sm + 2 * n;
-    double* hl = sm + 3 * n;
+    double* pl = yl;  // (y of the previous iteration is dead once x is staged)
+    double* hl = sm + 2 * n;
     double* tl = hl + nw;
     StepArgs B = A;  // the partials go to LDS
     B.part = pl;
@@ -1191,10 +1192,10 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
     B.tail = tl;
     const uint32_t gb = NORMLAP ? 0u : *A.gbp;
     // this thread's rows: range of the first and last entry, weighted degree
-    uint32_t w0[SMALL_RPT], w1[SMALL_RPT], wd[SMALL_RPT];
-    bool has[SMALL_RPT];
+    uint32_t w0[RPT], w1[RPT], wd[RPT];
+    bool has[RPT];
 #pragma unroll
-    for (uint32_t k = 0; k < SMALL_RPT; k++) {
+    for (uint32_t k = 0; k < RPT; k++) {
         const uint32_t r = tid + k * LBLOCK;
         has[k] = false;
         w0[k] = w1[k] = wd[k] = 0;
@@ -1294,7 +1295,7 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
         __syncthreads();
         double S[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (uint32_t k = 0; k < SMALL_RPT; k++) {
+        for (uint32_t k = 0; k < RPT; k++) {
             const uint32_t r = tid + k * LBLOCK;
             if (r < n) {
                 // row_total's order: the first partial, then the heads
@@ -1348,15 +1349,15 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
 // Whether a small level's vectors fit in LDS together (k_power_small_lds).
 bool small_lds_fits(uint32_t n, uint32_t nwaves)
 {
-    return n <= SMALL_RPT * LBLOCK && 3ull * n + 2ull * nwaves <= SMALL_LDS;
+    return n <= SMALL_RPT * LBLOCK && 2ull * n + 2ull * nwaves <= SMALL_LDS;
 }
 
-template <int WF, bool R16, bool NORMLAP>
+template <int WF, bool R16, bool NORMLAP, uint32_t RPT>
 __global__ void __launch_bounds__(LBLOCK) k_power_small_lds(StepArgs A, double* buf0, double* buf1, uint32_t chunk)
 {
     __shared__ double sm[SMALL_LDS];
     __shared__ double red[4][LNW];
-    small_loop<WF, R16, NORMLAP>(A, buf0, buf1, chunk, buf0, nullptr, nullptr, sm, red);
+    small_loop<WF, R16, NORMLAP, RPT>(A, buf0, buf1, chunk, buf0, nullptr, nullptr, sm, red);
 }
 
 // One level of the batched form: its step arguments (with its own loop
@@ -1373,7 +1374,7 @@ struct SmallJob {
 
 // Independent small levels (of different graphs) in one launch, one
 // workgroup each; the same loop as k_power_small_lds.
-template <bool NORMLAP>
+template <bool NORMLAP, uint32_t RPT>
 __global__ void __launch_bounds__(LBLOCK) k_power_small_batch(const SmallJob* __restrict__ jobs, uint32_t chunk)
 {
     __shared__ double sm[SMALL_LDS];
@@ -1381,7 +1382,7 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small_batch(const SmallJob* __
     const SmallJob& J = jobs[blockIdx.x];
     const int wf = J.A.wfmt;
     const bool r16 = J.A.r16 != 0;
-#define MI_SB(WFV, R) small_loop<WFV, R, NORMLAP>(J.A, J.b0, J.b1, chunk, J.start, J.vcmap, J.out, sm, red)
+#define MI_SB(WFV, R) small_loop<WFV, R, NORMLAP, RPT>(J.A, J.b0, J.b1, chunk, J.start, J.vcmap, J.out, sm, red)
     if (wf == W_UNIT) {
         if (r16)
             MI_SB(W_UNIT, true);
@@ -2146,11 +2147,16 @@ static void run_small(Ctx& c, int normLap, const StepArgs& A, double* b0, double
     const bool lds = small_lds_fits(A.n, A.nwaves);
 #define MI_SMALL(WF, NL)                                                                               \
     do {                                                                                               \
-        if (lds) {                                                                                     \
+        if (lds && A.n <= 4 * LBLOCK) {                                                                \
             if (r16)                                                                                   \
-                k_power_small_lds<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);    \
+                k_power_small_lds<WF, true, NL, 4><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK); \
             else                                                                                       \
-                k_power_small_lds<WF, false, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);   \
+                k_power_small_lds<WF, false, NL, 4><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);\
+        } else if (lds) {                                                                              \
+            if (r16)                                                                                   \
+                k_power_small_lds<WF, true, NL, 8><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK); \
+            else                                                                                       \
+                k_power_small_lds<WF, false, NL, 8><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);\
         } else if (r16) {                                                                              \
             k_power_small<WF, true, NL><<<1, LBLOCK, 0, c.stream>>>(A, b0, b1, SMALL_CHUNK);            \
         } else {                                                                                       \
@@ -2296,11 +2302,20 @@ void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, do
     MI_HIP(hipMemsetAsync(cnt.get(), 0, 32 * J, c.stream));
     MI_HIP(hipMemsetAsync(scal.get(), 0, 128 * J, c.stream));
     MI_HIP(hipMemsetAsync(nit.get(), 0, 8 * J, c.stream));
+    // jobs of at most 4 rows per thread first (their kernel instance keeps
+    // its rows' metadata in fewer registers), then the others
+    std::vector<size_t> order;
+    for (size_t j = 0; j < J; j++)
+        if (lv[j].g->n <= 4u * LBLOCK) order.push_back(j);
+    const size_t J4 = order.size();
+    for (size_t j = 0; j < J; j++)
+        if (lv[j].g->n > 4u * LBLOCK) order.push_back(j);
     std::vector<SmallJob> host(J);
     unsigned long long cap = 0;
-    for (size_t j = 0; j < J; j++) {
+    for (size_t k = 0; k < J; k++) {
+        const size_t j = order[k];
         const DGraph& g = *lv[j].g;
-        Own& o = own[j];
+        Own& o = own[k];
         plan_level(c, g, o.plan, false);
         MI_CHECK(o.plan.ncb == 1 && small_lds_fits(g.n, o.plan.nwaves), ST_INTERNAL,
                  "batched level does not fit the small-level form");
@@ -2309,11 +2324,11 @@ void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, do
         weighted_degree(c, g, o.plan, o.wdeg.get(), o.gb.get());
         o.b0 = DBuf<double>(&c.pool, g.n);
         o.b1 = DBuf<double>(&c.pool, g.n);
-        SmallJob& h = host[j];
+        SmallJob& h = host[k];
         h.A = make_args(c, g, o.plan, o.wdeg.get(), o.gb.get(), o.b0.get(), o.b1.get(), tol, 0);
-        h.A.cnt = cnt.get() + 8 * j;
-        h.A.scal = scal.get() + 16 * j;
-        h.A.niter = nit.get() + j;
+        h.A.cnt = cnt.get() + 8 * k;
+        h.A.scal = scal.get() + 16 * k;
+        h.A.niter = nit.get() + k;
         h.A.fprev = o.plan.fsum.get();
         h.A.fcur = o.plan.fsum.get();
         h.b0 = o.b0.get();
@@ -2329,11 +2344,20 @@ void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, do
     std::vector<uint32_t> hc(8 * J);
     for (unsigned long long launched = 0;; launched += c.small_chunk) {
         MI_CHECK(launched <= cap + c.small_chunk, ST_INTERNAL, "power iteration did not stop at its iteration cap");
-        if (normLap)
-            k_power_small_batch<true><<<(unsigned)J, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
-        else
-            k_power_small_batch<false><<<(unsigned)J, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
-        MI_HIP(hipGetLastError());
+        if (J4) {
+            if (normLap)
+                k_power_small_batch<true, 4><<<(unsigned)J4, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
+            else
+                k_power_small_batch<false, 4><<<(unsigned)J4, LBLOCK, 0, c.stream>>>(dj, c.small_chunk);
+            MI_HIP(hipGetLastError());
+        }
+        if (J > J4) {
+            if (normLap)
+                k_power_small_batch<true, 8><<<(unsigned)(J - J4), LBLOCK, 0, c.stream>>>(dj + J4, c.small_chunk);
+            else
+                k_power_small_batch<false, 8><<<(unsigned)(J - J4), LBLOCK, 0, c.stream>>>(dj + J4, c.small_chunk);
+            MI_HIP(hipGetLastError());
+        }
         MI_HIP(hipMemcpyAsync(hc.data(), cnt.get(), 32 * J, hipMemcpyDeviceToHost, c.stream));
         MI_HIP(hipStreamSynchronize(c.stream));
         bool all = true;
@@ -2343,9 +2367,9 @@ void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, do
     std::vector<unsigned long long> hn(J);
     MI_HIP(hipMemcpyAsync(hn.data(), nit.get(), 8 * J, hipMemcpyDeviceToHost, c.stream));
     MI_HIP(hipStreamSynchronize(c.stream));
-    for (size_t j = 0; j < J; j++) {
-        lv[j].iters = hn[j];
-        lv[j].max_reached = hc[8 * j + CT_MAXR] ? 1 : 0;
+    for (size_t k = 0; k < J; k++) {
+        lv[order[k]].iters = hn[k];
+        lv[order[k]].max_reached = hc[8 * k + CT_MAXR] ? 1 : 0;
     }
 }
 

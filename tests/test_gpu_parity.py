@@ -494,10 +494,11 @@ def test_context_destroyed_before_its_objects():
 
 
 @pytest.mark.parametrize("normLap", [0, 1])
-@pytest.mark.parametrize("case", ["er3000w", "grid64", "rmat11w_hubs", "grid100"])
+@pytest.mark.parametrize("case", ["er3000w", "grid64", "rmat11w_hubs", "grid80", "grid100"])
 def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
-    """The one-workgroup loop of small levels (vectors in LDS up to about
-    6000 vertices: grid100 takes the form with vectors in global memory)
+    """The one-workgroup loop of small levels (vectors in LDS up to 8192
+    vertices, 4 or 8 rows per thread: grid80 takes the 8-row form, grid100
+    the form with vectors in global memory)
     against the two-kernel step (same stop rule, sums added in another
     order, the start vector's too: the FP64 tolerances used against the
     oracle) and against the oracle; launches resumed every 7 iterations, and
@@ -507,6 +508,8 @@ def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
         rp, col, w = random_graph(rng, 3000, 0.002, weighted=True, lcc=True)
     elif case == "grid64":
         rp, col, w = graphs.grid2d(64)
+    elif case == "grid80":
+        rp, col, w = graphs.grid2d(80)
     elif case == "grid100":
         rp, col, w = graphs.grid2d(100)
     else:
