@@ -750,7 +750,8 @@ int mi_bisect_batch(mi_ctx* ctx, int count, const mi_graph* const* graphs, const
         for (size_t k = 0; k < K; k++) {
             mi_stats* st = stats ? &stats[k] : nullptr;
             if (st) std::memset(st, 0, sizeof(*st));
-            build_hierarchy(c, graphs[k]->g, hec_seeds[k], H[k], st);
+            // (no per-phase timing: it would synchronise once per level)
+            build_hierarchy(c, graphs[k]->g, hec_seeds[k], H[k], nullptr);
             fill_level_stats(H[k], st);
         }
         MI_HIP(hipStreamSynchronize(c.stream));

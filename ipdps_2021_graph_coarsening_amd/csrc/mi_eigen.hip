@@ -1933,8 +1933,10 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         MI_HIP(hipMemsetAsync(maxd.get(), 0, 4, c.stream));
         k_rid16<<<red_grid(m), BLOCK, 0, c.stream>>>(B, rid32.get(), p.wmeta.get(), p.rid16.get(), maxd.get());
         MI_HIP(hipGetLastError());
+        // (offsets are below the number of rows, so with fewer than 65536
+        // rows they fit without asking the device)
         uint32_t mx = 0;
-        read_words(c, maxd.get(), &mx, 4);
+        if (nrows > 0xffffu) read_words(c, maxd.get(), &mx, 4);
         if (mx > 0xffffu) {
             p.rid16.reset();
             p.rid = std::move(rid32);
