@@ -86,9 +86,10 @@ void mi_ctx_destroy(mi_ctx* ctx);
 const char* mi_last_error(const mi_ctx* ctx);
 /* Returns cached pool blocks to the runtime. */
 int mi_ctx_trim(mi_ctx* ctx);
-/* enable != 0 (the default): power iterations are replayed from a HIP graph
-   captured once per level; 0: every kernel is launched from the host.
-   Results are identical either way.  No counterpart in the reference. */
+/* enable != 0: power iterations are replayed from a HIP graph captured once
+   per level; 0 (the default, measured 1.5 % faster here): every kernel is
+   launched from the host, a chunk ahead of the device.  Results are
+   identical either way.  No counterpart in the reference. */
 int mi_ctx_set_graphs(mi_ctx* ctx, int enable);
 /* mixed != 0: mixed-precision eigensolve.  On every level above the finest
    with at least min_entries stored entries (0 selects 2^20: smaller levels

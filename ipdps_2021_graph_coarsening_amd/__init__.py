@@ -195,7 +195,7 @@ class Context:
         self.check(self._L.mi_ctx_set_precision(self.h, 1 if mixed else 0, min_entries), "mi_ctx_set_precision")
 
     def set_graphs(self, enable: bool) -> None:
-        """Replay power iterations from a captured HIP graph (default) or launch eagerly."""
+        """Replay power iterations from a captured HIP graph, or launch every kernel (the default)."""
         self.check(self._L.mi_ctx_set_graphs(self.h, 1 if enable else 0), "mi_ctx_set_graphs")
 
     # ------------------------------------------------------------ graphs

@@ -157,7 +157,7 @@ struct Ctx {
     uint32_t* host_mapped = nullptr;
     uint32_t* host_mapped_dev = nullptr;  // its device address
     uint64_t* xs_jump = nullptr;          // T^(2^k) of the xorshift transition, 48 x 64 words
-    bool use_graphs = true;               // replay power iterations from a captured HIP graph
+    bool use_graphs = false;              // replay power iterations from a captured HIP graph
     bool mixed = false;                   // FP32 power iterations on the levels above the finest
     uint32_t mixed_min_m = 1u << 20;      // ... that hold at least this many entries
     uint32_t small_max_m = 1u << 17;      // levels of <= 16384 vertices and this many entries: one workgroup

@@ -144,10 +144,11 @@ def test_mixed_graph_replay_equals_eager(mixed):
     u0 = np.random.default_rng(9).random(len(rp) - 1)
     g = mixed.graph(rp, col, w)
     try:
+        mixed.set_graphs(True)
         ug, itg, _ = g.power_iter(u0, tol=1e-10)
         mixed.set_graphs(False)
         ue, ite, _ = g.power_iter(u0, tol=1e-10)
     finally:
-        mixed.set_graphs(True)
+        mixed.set_graphs(False)
     assert itg == ite and np.array_equal(ug, ue)
     g.close()

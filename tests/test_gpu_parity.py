@@ -281,7 +281,7 @@ def test_power_iter_graph_replay_equals_eager(ctx, normLap):
             assert np.array_equal(ug, ue) and np.array_equal(ug2, ue)
             assert np.array_equal(y, g.spmv_shift(u0))
     finally:
-        ctx.set_graphs(True)
+        ctx.set_graphs(False)
 
 
 @pytest.mark.parametrize("normLap", [0, 1])
