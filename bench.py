@@ -102,6 +102,20 @@ def cpu_baseline(rp, col, w, gpu_stats, budget_s: float = 20.0) -> dict:
                        f"(per-iteration time x the GPU run's iteration counts) + split {t_split:.2f} s")}
 
 
+def merge_rate(st: dict) -> dict:
+    """Algorithmic bytes of the coarse builds (P^T A P with duplicate merge)
+    over their time: per level l -> l+1, read the fine CSR 4(n+1) + 4m + 4m
+    (weights; the unit-weight finest level stores none) and vcmap 4n, write
+    the coarse CSR 4(nc+1) + 8 m_c."""
+    ns, ms = st["level_n"], st["level_m"]
+    b = 0.0
+    for l in range(len(ns) - 1):
+        b += 4 * (ns[l] + 1) + 4 * ms[l] + (4 * ms[l] if l > 0 else 0) + 4 * ns[l]
+        b += 4 * (ns[l + 1] + 1) + 8 * ms[l + 1]
+    t = st["build_ms"]
+    return {"alg_bytes": b, "ms": round(t, 3), "gbps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+
+
 def mixed_summary(ms64: float, st64: dict, ms32: float, st32: dict) -> dict:
     """Mixed-precision bisection beside the plain FP64 one on the same graph."""
     return {"ms": round(ms32, 3), "fp64_ms": round(ms64, 3), "speedup": round(ms64 / ms32, 4),
@@ -346,7 +360,10 @@ def main() -> int:
                    "phases_ms": {k: round(stats[k], 3) for k in
                                  ("coarsen_ms", "eigen_ms", "partition_ms", "permute_ms", "heavy_ms",
                                   "map_ms", "build_ms")},
-                   "level_eigen_ms": [round(x, 3) for x in stats["level_eigen_ms"]]},
+                   "level_eigen_ms": [round(x, 3) for x in stats["level_eigen_ms"]],
+                   "level_us_per_iteration": [round(1000.0 * t / max(i, 1), 2) for t, i in
+                                              zip(stats["level_eigen_ms"], stats["level_iters"])],
+                   "merge": merge_rate(stats)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
