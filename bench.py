@@ -21,7 +21,9 @@ that summary is absent.  `cpu_baseline` times the CPU oracle (a scalar
 restatement of the reference's Serial path) on a bounded sample and scales
 it to one whole bisection.  `mixed` is the same bisection in mixed precision
 (FP32 vectors on the levels above the finest with at least 2^20 entries,
-FP64 sums, deflation, stopping test and finest level) beside the FP64 one;
+FP64 sums, deflation, stopping test and finest level) beside the FP64 one,
+with `per_level` comparing each FP32 level's power iteration with the FP64
+one from the same start (iterations, angle, Rayleigh quotient, residual);
 `mixed_grid3d` the same comparison on the 128^3 7-point grid with weights
 U{1..16}.  `batched_rgg512` times the batched bisection of 512 random
 geometric graphs (mi_bisect_batch) against one call per graph.
@@ -124,6 +126,47 @@ def mixed_summary(ms64: float, st64: dict, ms32: float, st32: dict) -> dict:
             "fp32_levels": st32["mixed_levels"], "level_iters": st32["level_iters"],
             "fp64_level_iters": st64["level_iters"],
             "level_eigen_ms": [round(x, 3) for x in st32["level_eigen_ms"]]}
+
+
+def mixed_levels_report(mi, ctx, g, seed: int) -> list:
+    """Per level that runs in FP32 (at least 2^20 entries, above the
+    finest): the level's power iteration in mixed precision and in FP64 from
+    the same start vector (FP64 on the device, which the tests hold to the
+    oracle), with iterations, the angle between the two vectors, and each
+    one's Rayleigh quotient and relative Fiedler residual ||L u - lam u|| /
+    ||lam u|| (L = D - A, computed on the host)."""
+    import scipy.sparse as sp
+    H = g.hierarchy(seed)
+    out = []
+    for l in range(1, H.nlevels):
+        lv = H.level(l)
+        if lv["m"] < (1 << 20):
+            continue
+        n = lv["n"]
+        A = sp.csr_matrix((lv["w"].astype(np.float64), lv["col"], lv["rowptr"]), shape=(n, n))
+        Lap = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
+        lg = ctx.graph(lv["rowptr"], lv["col"], lv["w"])
+        u0 = np.random.default_rng(l).random(n)
+        ctx.set_precision(False)
+        u64, it64, _ = lg.power_iter(u0)
+        ctx.set_precision(True, 1)
+        u32, it32, _ = lg.power_iter(u0)
+        ctx.set_precision(False)
+        lg.close()
+
+        def quality(u):
+            u = u - u.mean()
+            lam = float(u @ (Lap @ u) / (u @ u))
+            return lam, float(np.linalg.norm(Lap @ u - lam * u) / (abs(lam) * np.linalg.norm(u)))
+
+        l64, r64 = quality(u64)
+        l32, r32 = quality(u32)
+        out.append({"level": l, "n": n, "m": lv["m"], "iters_fp64": it64, "iters_mixed": it32,
+                    "cos": round(float(abs(u64 @ u32) / (np.linalg.norm(u64) * np.linalg.norm(u32))), 12),
+                    "rayleigh_fp64": l64, "rayleigh_mixed": l32, "residual_fp64": round(r64, 6),
+                    "residual_mixed": round(r32, 6)})
+    H.close()
+    return out
 
 
 def grid3d_mixed(mi, ctx, steps: int) -> dict:
@@ -275,6 +318,8 @@ def main() -> int:
             ms_mixed, st_mixed = timed()
             ctx.set_precision(False)
             mixed = mixed_summary(ms, stats, ms_mixed, st_mixed)
+            if args.scale < 22:
+                mixed["per_level"] = mixed_levels_report(mi, ctx, g, SEED)
             log(f"mixed precision: {ms_mixed:.2f} ms/bisection, cut {st_mixed['edge_cut']}, "
                 f"iterations per level {st_mixed['level_iters']}")
         hip.hipFree(d_part)
