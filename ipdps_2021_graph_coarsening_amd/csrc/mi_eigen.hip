@@ -996,10 +996,8 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     double S[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) S[k] = 0.0;
-    for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < A.n; r += gridDim.x * BLOCK) {
-        const V yr = yp[r];
-        const uint32_t wd = A.wdeg ? A.wdeg[r] : 0u;
-        const V v = row_total<KB, MANY>(A, r, nb, part, head, tail);
+    // one row's y and its terms of the sums, rows in this thread's order
+    auto emit = [&](uint32_t r, V yr, uint32_t wd, V v) {
         const V xv = xform(yr, a, s);  // as the streaming kernel formed x
         const double xr = (double)xv;
         const V yv = (V)shifted<NORMLAP>(A, (double)v, xr, wd, gb);
@@ -1010,7 +1008,56 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
         S[2] += xr * y;
         S[3] += xr;
         if (NS == 5) S[NS - 1] += xr * xr;
+    };
+    const uint32_t stride = gridDim.x * BLOCK;
+    uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (!MANY && nb == 1) {
+        // one block: four rows at a time, their loads in flight together (a
+        // thread's rows stay in the same order, so the sums do not change)
+        constexpr int RB = 4;
+        for (; r + (RB - 1) * stride < A.n; r += RB * stride) {
+            uint32_t s0[RB], s1[RB], wd[RB];
+            V yr[RB], first[RB];
+#pragma unroll
+            for (int k = 0; k < RB; k++) {
+                const uint32_t q = r + k * stride;
+                s0[k] = A.rowptr[q];
+                s1[k] = A.rowptr[q + 1];
+                yr[k] = yp[q];
+                wd[k] = A.wdeg ? A.wdeg[q] : 0u;
+            }
+            uint32_t w0[RB], w1[RB];
+#pragma unroll
+            for (int k = 0; k < RB; k++) {
+                w0[k] = A.B.bwave[0] + (s0[k] - A.B.boff[0]) / A.B.pw;
+                w1[k] = A.B.bwave[0] + (s1[k] - 1 - A.B.boff[0]) / A.B.pw;
+                first[k] = 0;
+                if (s0[k] != s1[k]) first[k] = w0[k] == w1[k] ? part[r + k * stride] : tail[w0[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < RB; k++) {
+                V v = 0;
+                if (s0[k] != s1[k]) {
+                    V vb = first[k];
+                    if (w1[k] > w0[k]) {
+                        uint32_t h0 = w0[k] + 1;
+                        for (; h0 + 8 <= w1[k] + 1; h0 += 8) {
+                            V h[8];
+#pragma unroll
+                            for (int j = 0; j < 8; j++) h[j] = head[h0 + j];
+#pragma unroll
+                            for (int j = 0; j < 8; j++) vb += h[j];
+                        }
+                        for (; h0 <= w1[k]; h0++) vb += head[h0];
+                    }
+                    v = vb;
+                }
+                emit(r + k * stride, yr[k], wd[k], v);
+            }
+        }
     }
+    for (; r < A.n; r += stride)
+        emit(r, yp[r], A.wdeg ? A.wdeg[r] : 0u, row_total<KB, MANY>(A, r, nb, part, head, tail));
     if (A.mode != 0) return;
     double tot[NS];
     block_sum_k<NS>(S, tot, red);
