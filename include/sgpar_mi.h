@@ -198,10 +198,10 @@ int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config,
  * Bisections of count independent graphs: the same results, graph by graph
  * and bit for bit, as mi_partition_graph(ctx, graphs[k], config,
  * perc_imbalance, hec_seeds[k], &pcg_states[k], pcg_inc, parts[k], ...) for
- * k = 0 .. count-1.  The reference has no batched form; it is the
- * sub-bisection work list of the survey's design (mi_bisect_batch).  The
- * hierarchies are built graph by graph; then the power iterations of the
- * small levels (those that run in one workgroup with every vector in LDS),
+ * k = 0 .. count-1.  The reference has no batched form (it bisects one
+ * graph per call).  The hierarchies are built graph by graph; then the
+ * power iterations of the small levels (those that run in one workgroup
+ * with every vector in LDS),
  * from each graph's coarsest level down, run for all graphs together, one
  * workgroup per graph and level, level by level; the larger levels and the
  * splits follow graph by graph.  parts[k] are host arrays of graphs[k]'s n
