@@ -111,6 +111,16 @@ int mi_ctx_set_precision(mi_ctx* ctx, int mixed, uint32_t min_entries);
    result bookkeeping; the sums are added in another order.  No counterpart
    in the reference. */
 int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk);
+/* enable != 0 (the default): a level above the finest with at least
+   min_vertices vertices (0: 65536) whose power step gathers x from global
+   memory (too many vertices for the LDS column blocks) is solved with its
+   vertices in the order of their smallest finest-level vertex, which keeps
+   the gathers of x local on meshes and geometric graphs (HEC numbers coarse
+   vertices without locality); the level's vector goes back to the HEC
+   numbering afterwards, so every output keeps it.  Sums run in another
+   order, so results agree with enable = 0 to rounding.  No counterpart in
+   the reference. */
+int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices);
 
 /* ---------------------------------------------------------------- graphs */
 /*

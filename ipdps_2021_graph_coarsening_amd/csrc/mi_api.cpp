@@ -337,6 +337,24 @@ DBuf<double> start_vector(Ctx& c, uint32_t gn, uint64_t* pcg_state, uint64_t pcg
 DBuf<double> fiedler_levels(Ctx& c, const std::vector<DLevel>& lv, int from, DBuf<double> cur, int normLap,
                             double tol, mi_stats* st)
 {
+    // locality order for the levels whose power step gathers x from global
+    // memory: their coarse vertices, numbered by HEC without locality, are
+    // solved in the order of their smallest finest-level vertex
+    std::vector<char> local(lv.size(), 0);
+    bool any = false;
+    for (int l = 1; l <= from; l++) {
+        local[l] = c.locality && lv[l].g.n >= c.locality_min_n && gather_form_level(lv[l].g);
+        any = any || local[l];
+    }
+    std::vector<DBuf<uint32_t>> key(lv.size());
+    if (any) {
+        key[0] = DBuf<uint32_t>(&c.pool, lv[0].g.n);
+        iota_u32(c, key[0].get(), lv[0].g.n, 0);
+        for (int l = 0; l < from; l++) {
+            key[l + 1] = DBuf<uint32_t>(&c.pool, lv[l + 1].g.n);
+            level_keys(c, key[l].get(), lv[l].vcmap.get(), lv[l].g.n, key[l + 1].get(), lv[l + 1].g.n);
+        }
+    }
     for (int l = from; l >= 0; l--) {
         PowerStats ps;
         // the reference logs the eigenvalue after the final (finest) solve
@@ -347,7 +365,17 @@ DBuf<double> fiedler_levels(Ctx& c, const std::vector<DLevel>& lv, int from, DBu
         // finest, whose FP64 solve is the final refinement
         const bool mixed = c.mixed && l > 0 && lv[l].g.m >= c.mixed_min_m;
         if (st && mixed) st->mixed++;
-        power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps, log_here ? elog : nullptr, mixed);
+        if (local[l]) {
+            DGraph pg;
+            DBuf<uint32_t> perm, inv;
+            permute_level(c, lv[l].g, key[l].get(), pg, perm, inv);
+            DBuf<double> pc(&c.pool, lv[l].g.n);
+            gather_f64(c, lv[l].g.n, perm.get(), cur.get(), pc.get());
+            power_iter(c, pg, pc.get(), normLap, tol, &ps, nullptr, mixed);
+            gather_f64(c, lv[l].g.n, inv.get(), pc.get(), cur.get());
+        } else {
+            power_iter(c, lv[l].g, cur.get(), normLap, tol, &ps, log_here ? elog : nullptr, mixed);
+        }
         if (log_here) {
             st->has_eigen_log = 1;
             st->eigenvalue = elog[0];
@@ -419,6 +447,14 @@ int mi_ctx_set_graphs(mi_ctx* ctx, int enable)
 {
     if (!ctx) return ST_ARG;
     ctx->c.use_graphs = enable != 0;
+    return ST_OK;
+}
+
+int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.locality = enable != 0;
+    ctx->c.locality_min_n = min_vertices ? min_vertices : 65536u;
     return ST_OK;
 }
 

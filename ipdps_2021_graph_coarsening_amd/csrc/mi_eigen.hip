@@ -1659,6 +1659,54 @@ __global__ void k_prolong(uint32_t n, const uint32_t* __restrict__ vcmap, const 
     if (i < n) out[i] = in[vcmap[i]];
 }
 
+// ---------------------------------------------------------------- locality order
+// key_c[vcmap[i]] = min over the members i of key_f[i] (key_c filled with ~0)
+__global__ void k_key_min(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ key_f,
+                          uint32_t* key_c)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) atomicMin(&key_c[vcmap[i]], key_f[i]);
+}
+
+// inv[perm[j]] = j; deg[j] = the degree of row perm[j]
+__global__ void k_perm_rows(uint32_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ rowptr,
+                            uint32_t* __restrict__ inv, uint32_t* __restrict__ deg)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) {
+        const uint32_t r = perm[j];
+        inv[r] = (uint32_t)j;
+        deg[j] = rowptr[r + 1] - rowptr[r];
+    }
+}
+
+// row j of the permuted level = row perm[j], columns renamed by inv (one
+// wavefront per row, entries in their order)
+__global__ void __launch_bounds__(BLOCK) k_perm_entries(uint32_t n, const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ inv,
+                                                        const uint32_t* __restrict__ rowptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ w,
+                                                        const uint32_t* __restrict__ prow, uint32_t* __restrict__ pcol,
+                                                        uint32_t* __restrict__ pw)
+{
+    const uint32_t j = blockIdx.x * NW + threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    if (j >= n) return;
+    const uint32_t r = perm[j], e0 = rowptr[r], e1 = rowptr[r + 1], d = prow[j];
+    for (uint32_t e = e0 + lane; e < e1; e += WAVE) {
+        pcol[d + (e - e0)] = inv[col[e]];
+        if (w) pw[d + (e - e0)] = w[e];
+    }
+}
+
+// out[j] = in[idx[j]]
+__global__ void k_gather_f64(uint32_t n, const uint32_t* __restrict__ idx, const double* __restrict__ in,
+                             double* __restrict__ out)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) out[j] = in[idx[j]];
+}
+
 // dot_j of iteration j lives in scal[SC_DOT + (j & 1)]; dot_0 = 0
 __global__ void k_loop_init(double* scal, uint32_t* cnt, unsigned long long* niter)
 {
@@ -2088,6 +2136,53 @@ void normalize(Ctx& c, double* d_u, uint32_t n)
     k_sumsq<<<red_grid(n), BLOCK, 0, c.stream>>>(n, d_u, c.partials, c.cnt, c.scal);
     MI_HIP(hipGetLastError());
     k_scale_inv_sqrt<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, c.scal);
+    MI_HIP(hipGetLastError());
+}
+
+bool gather_form_level(const DGraph& g)
+{
+    const uint32_t need = (uint32_t)ceil_div(g.n, XB);
+    return g.m && need > 1 && !(need <= (uint32_t)MAXCB && (uint64_t)g.m >= 64ull * g.n &&
+                                (uint64_t)need * g.n < 0xffffffffull);
+}
+
+void level_keys(Ctx& c, const uint32_t* key_f, const uint32_t* vcmap, uint32_t n_fine, uint32_t* key_c,
+                uint32_t n_coarse)
+{
+    fill_u32(c, key_c, n_coarse, 0xffffffffu);
+    k_key_min<<<grid1(n_fine), BLOCK, 0, c.stream>>>(n_fine, vcmap, key_f, key_c);
+    MI_HIP(hipGetLastError());
+}
+
+void permute_level(Ctx& c, const DGraph& g, const uint32_t* key, DGraph& out, DBuf<uint32_t>& perm,
+                   DBuf<uint32_t>& inv)
+{
+    const uint32_t n = g.n, m = g.m;
+    DBuf<uint32_t> k0(&c.pool, n), k1(&c.pool, n), v1(&c.pool, n), deg(&c.pool, n);
+    perm = DBuf<uint32_t>(&c.pool, n);
+    inv = DBuf<uint32_t>(&c.pool, n);
+    MI_HIP(hipMemcpyAsync(k0.get(), key, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    iota_u32(c, perm.get(), n, 0);
+    // the keys are distinct vertex ids of the finest level: a sort by them
+    radix_sort_pairs(c, k0.get(), perm.get(), n, 0, 32, k1.get(), v1.get());
+    k_perm_rows<<<grid1(n), BLOCK, 0, c.stream>>>(n, perm.get(), g.R(), inv.get(), deg.get());
+    MI_HIP(hipGetLastError());
+    out.n = n;
+    out.m = m;
+    out.unit = g.unit;
+    out.rowptr = DBuf<uint32_t>(&c.pool, (size_t)n + 1);
+    out.col = DBuf<uint32_t>(&c.pool, m);
+    if (!g.unit) out.w = DBuf<uint32_t>(&c.pool, m);
+    scan_exclusive_u32(c, deg.get(), out.rowptr.get(), n, out.rowptr.get() + n);
+    k_perm_entries<<<(unsigned)ceil_div(n, NW), BLOCK, 0, c.stream>>>(n, perm.get(), inv.get(), g.R(), g.Cc(), g.W(),
+                                                                      out.rowptr.get(), out.col.get(),
+                                                                      g.unit ? nullptr : out.w.get());
+    MI_HIP(hipGetLastError());
+}
+
+void gather_f64(Ctx& c, uint32_t n, const uint32_t* idx, const double* in, double* out)
+{
+    k_gather_f64<<<grid1(n), BLOCK, 0, c.stream>>>(n, idx, in, out);
     MI_HIP(hipGetLastError());
 }
 

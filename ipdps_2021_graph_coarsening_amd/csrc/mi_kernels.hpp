@@ -186,6 +186,20 @@ bool small_lds_level(const Ctx& c, const DGraph& g, bool mixed);
 void power_iter_small_batch(Ctx& c, std::vector<BatchLevel>& lv, int normLap, double tol);
 // The final eigenvalue log (eigenvalue, min, max) of level g for vector d_u.
 void eigen_log_level(Ctx& c, const DGraph& g, const double* d_u, double* out3);
+// Locality order for the solve of a level.  gather_form_level: the level's
+// power step would gather x from global memory (too many vertices for LDS
+// column blocks).  level_keys: key_c[c] = min key_f[i] over the members i of
+// coarse vertex c (with key_0 = the finest ids: each coarse vertex's
+// smallest finest vertex).  permute_level: out = the level with its rows
+// and columns renumbered by ascending key (perm[j] = old id of new row j,
+// inv = its inverse), entries in their order.
+bool gather_form_level(const DGraph& g);
+void level_keys(Ctx& c, const uint32_t* key_f, const uint32_t* vcmap, uint32_t n_fine, uint32_t* key_c,
+                uint32_t n_coarse);
+void permute_level(Ctx& c, const DGraph& g, const uint32_t* key, DGraph& out, DBuf<uint32_t>& perm,
+                   DBuf<uint32_t>& inv);
+// out[j] = in[idx[j]]
+void gather_f64(Ctx& c, uint32_t n, const uint32_t* idx, const double* in, double* out);
 // out[i] = in[vcmap[i]]
 void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, double* out);
 // u /= ||u||

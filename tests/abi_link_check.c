@@ -21,6 +21,7 @@ int main(void)
     int (*fp_mi_ctx_set_graphs)(mi_ctx*, int);
     int (*fp_mi_ctx_set_precision)(mi_ctx*, int, uint32_t);
     int (*fp_mi_ctx_set_small_levels)(mi_ctx*, uint32_t, uint32_t);
+    int (*fp_mi_ctx_set_locality)(mi_ctx*, int, uint32_t);
     int (*fp_mi_bisect_batch)(mi_ctx*, int, const mi_graph* const*, const mi_config*, int, const uint64_t*,
                               uint64_t*, uint64_t, uint32_t* const*, int, uint32_t*, mi_stats*, double*);
     int (*fp_mi_graph_create)(mi_ctx*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, mi_graph**);
@@ -62,6 +63,7 @@ int main(void)
     BIND(mi_ctx_set_graphs);
     BIND(mi_ctx_set_precision);
     BIND(mi_ctx_set_small_levels);
+    BIND(mi_ctx_set_locality);
     BIND(mi_bisect_batch);
     BIND(mi_graph_create);
     BIND(mi_graph_create_device);

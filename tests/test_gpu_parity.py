@@ -540,3 +540,29 @@ def test_small_level_workgroup_vs_steps_and_oracle(ctx, case, normLap):
         assert abs(its - oit) <= max(2, ITER_SLACK * oit)
         assert abs(us @ ou) / (np.linalg.norm(us) * np.linalg.norm(ou)) >= EV_COS
     g.close()
+
+
+@pytest.mark.parametrize("name", ["grid256", "rgg60k"])
+def test_locality_order_vs_hec_order(ctx, name):
+    """Gather-form levels solved in the locality order against the HEC
+    order (sums in another order: the FP64 tolerances used against the
+    oracle), and the result in the HEC numbering either way."""
+    if name == "grid256":
+        rp, col, w = graphs.grid2d(256)
+    else:
+        rp, col, w = graphs.rgg(60000, seed=5)
+    g = ctx.graph(rp, col, w)
+    H = g.hierarchy(1)
+    try:
+        ctx.set_locality(True, 1)
+        ev1, st1, _ = H.fiedler(3)
+        ctx.set_locality(False)
+        ev0, st0, _ = H.fiedler(3)
+    finally:
+        ctx.set_locality(True)
+    assert any(n > 16384 for n in st0["level_n"][1:])  # a level takes the gather form
+    for a, b in zip(st1["level_iters"], st0["level_iters"]):
+        assert abs(int(a) - int(b)) <= max(2, ITER_SLACK * int(b))
+    assert abs(ev1 @ ev0) / (np.linalg.norm(ev1) * np.linalg.norm(ev0)) >= EV_COS
+    H.close()
+    g.close()
