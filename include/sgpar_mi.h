@@ -117,7 +117,9 @@ int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk);
    vertices in the order of their smallest finest-level vertex, which keeps
    the gathers of x local on meshes and geometric graphs (HEC numbers coarse
    vertices without locality); the level's vector goes back to the HEC
-   numbering afterwards, so every output keeps it.  Sums run in another
+   numbering afterwards, so every output keeps it.  It applies only when
+   the finest level's own order has locality (sampled mean |row - column|
+   below 2 % of n; a random order gives about 33 %).  Sums run in another
    order, so results agree with enable = 0 to rounding.  No counterpart in
    the reference. */
 int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices);

@@ -346,6 +346,12 @@ DBuf<double> fiedler_levels(Ctx& c, const std::vector<DLevel>& lv, int from, DBu
         local[l] = c.locality && lv[l].g.n >= c.locality_min_n && gather_form_level(lv[l].g);
         any = any || local[l];
     }
+    // the order of the coarse vertices follows the finest level's: only
+    // worth it when that one has locality
+    if (any && order_spread(c, lv[0].g) > 0.02) {
+        any = false;
+        std::fill(local.begin(), local.end(), 0);
+    }
     std::vector<DBuf<uint32_t>> key(lv.size());
     if (any) {
         key[0] = DBuf<uint32_t>(&c.pool, lv[0].g.n);

@@ -1668,6 +1668,27 @@ __global__ void k_key_min(uint32_t n, const uint32_t* __restrict__ vcmap, const 
     if (i < n) atomicMin(&key_c[vcmap[i]], key_f[i]);
 }
 
+// Sampled locality of a vertex order: for rows r = k * step (k < nsample),
+// the sum of |r - c| over their first 64 entries and the entry count
+// (integer atomics: the result does not depend on their order)
+__global__ void k_order_spread(uint32_t nsample, uint32_t step, const uint32_t* __restrict__ rowptr,
+                               const uint32_t* __restrict__ col, unsigned long long* acc)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    unsigned long long d = 0, cnt = 0;
+    if (k < nsample) {
+        const uint32_t r = k * step, e0 = rowptr[r], e1 = rowptr[r + 1];
+        const uint32_t e2 = e1 - e0 > 64u ? e0 + 64u : e1;
+        for (uint32_t e = e0; e < e2; e++) {
+            const uint32_t c = col[e];
+            d += c > r ? c - r : r - c;
+        }
+        cnt = e2 - e0;
+    }
+    if (d) atomicAdd(&acc[0], d);
+    if (cnt) atomicAdd(&acc[1], cnt);
+}
+
 // inv[perm[j]] = j; deg[j] = the degree of row perm[j]
 __global__ void k_perm_rows(uint32_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ rowptr,
                             uint32_t* __restrict__ inv, uint32_t* __restrict__ deg)
@@ -2137,6 +2158,19 @@ void normalize(Ctx& c, double* d_u, uint32_t n)
     MI_HIP(hipGetLastError());
     k_scale_inv_sqrt<<<grid1(n), BLOCK, 0, c.stream>>>(n, d_u, c.scal);
     MI_HIP(hipGetLastError());
+}
+
+double order_spread(Ctx& c, const DGraph& g)
+{
+    if (g.n < 2 || g.m == 0) return 0.0;
+    const uint32_t nsample = std::min<uint32_t>(g.n, 65536u), step = g.n / nsample;
+    DBuf<unsigned long long> acc(&c.pool, 2);
+    MI_HIP(hipMemsetAsync(acc.get(), 0, 16, c.stream));
+    k_order_spread<<<grid1(nsample), BLOCK, 0, c.stream>>>(nsample, step, g.R(), g.Cc(), acc.get());
+    MI_HIP(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    read_words(c, acc.get(), h, 16);
+    return h[1] ? (double)h[0] / ((double)h[1] * (double)g.n) : 0.0;
 }
 
 bool gather_form_level(const DGraph& g)

@@ -194,6 +194,9 @@ void eigen_log_level(Ctx& c, const DGraph& g, const double* d_u, double* out3);
 // and columns renumbered by ascending key (perm[j] = old id of new row j,
 // inv = its inverse), entries in their order.
 bool gather_form_level(const DGraph& g);
+// Mean |row - column| over a sample of entries, over n: about 1/3 for a
+// random vertex order, far below for a mesh in its natural order.
+double order_spread(Ctx& c, const DGraph& g);
 void level_keys(Ctx& c, const uint32_t* key_f, const uint32_t* vcmap, uint32_t n_fine, uint32_t* key_c,
                 uint32_t n_coarse);
 void permute_level(Ctx& c, const DGraph& g, const uint32_t* key, DGraph& out, DBuf<uint32_t>& perm,
