@@ -748,6 +748,125 @@ uint32_t bisect(Ctx& c, const DGraph& g, const mi_config* config, int perc_imbal
     return cut;
 }
 
+// Bisections of the graphs gs, with the power iterations of their small
+// levels run together: per graph the same result as bisect().  chain: one
+// PCG state (pcg_states[0]) carried from graph to graph in order, as
+// consecutive bisect() calls would; else pcg_states[k] per graph.  d_parts
+// are device arrays.
+void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* config, int perc_imbalance,
+                 const uint64_t* hec_seeds, uint64_t* pcg_states, bool chain, uint64_t pcg_inc,
+                 const std::vector<uint32_t*>& d_parts, uint32_t* edge_cuts, mi_stats* stats, double* phase_ms)
+{
+    if (gs.empty()) return;
+    const int normLap = config->refine_alg;
+    const double tol = config->tol;
+    const size_t K = gs.size();
+    auto t0 = std::chrono::steady_clock::now();
+    // hierarchies, graph by graph
+    std::vector<std::vector<DLevel>> H(K);
+    for (size_t k = 0; k < K; k++) {
+        mi_stats* st = stats ? &stats[k] : nullptr;
+        if (st) std::memset(st, 0, sizeof(*st));
+        // (no per-phase timing: it would synchronise once per level)
+        build_hierarchy(c, *gs[k], hec_seeds[k], H[k], nullptr);
+        fill_level_stats(H[k], st);
+    }
+    MI_HIP(hipStreamSynchronize(c.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    // start vectors (consuming each graph's pcg state, as one call would)
+    // and the chains of small levels from the coarsest: level l of graph
+    // k is batched when every coarser level is
+    std::vector<DBuf<double>> cur(K);
+    std::vector<int> lo(K);
+    int depth = 0;
+    for (size_t k = 0; k < K; k++) {
+        const int L = (int)H[k].size();
+        MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
+        cur[k] = start_vector(c, H[k][L - 1].g.n, chain ? &pcg_states[0] : &pcg_states[k], pcg_inc);
+        int l = L - 1;
+        while (l >= 0) {
+            const DGraph& g = H[k][l].g;
+            const bool mixed = c.mixed && l > 0 && g.m >= c.mixed_min_m;
+            if (!small_lds_level(c, g, mixed)) break;
+            l--;
+        }
+        lo[k] = l + 1;
+        depth = std::max(depth, L - lo[k]);
+        if (stats) {
+            stats[k].has_eigen_log = 0;
+            stats[k].mixed = 0;
+        }
+    }
+    std::vector<std::vector<DBuf<double>>> out(K);
+    for (size_t k = 0; k < K; k++) out[k].resize(H[k].size());
+    for (int d = 0; d < depth; d++) {
+        std::vector<BatchLevel> jobs;
+        std::vector<std::pair<size_t, int>> who;
+        for (size_t k = 0; k < K; k++) {
+            const int L = (int)H[k].size(), l = L - 1 - d;
+            if (l < lo[k]) continue;
+            BatchLevel b;
+            b.g = &H[k][l].g;
+            b.start = d == 0 ? cur[k].get() : out[k][l + 1].get();
+            b.vcmap = d == 0 ? nullptr : H[k][l].vcmap.get();
+            out[k][l] = DBuf<double>(&c.pool, H[k][l].g.n);
+            b.out = out[k][l].get();
+            jobs.push_back(b);
+            who.emplace_back(k, l);
+        }
+        power_iter_small_batch(c, jobs, normLap, tol);
+        for (size_t j = 0; j < jobs.size(); j++) {
+            const size_t k = who[j].first;
+            const int l = who[j].second;
+            if (stats && l < MI_MAX_LEVELS) {
+                stats[k].level_iters[l] = jobs[j].iters;
+                stats[k].level_max_reached[l] = jobs[j].max_reached;
+            }
+        }
+    }
+    // the remaining (larger) levels, graph by graph
+    std::vector<DBuf<double>> ev(K);
+    for (size_t k = 0; k < K; k++) {
+        mi_stats* st = stats ? &stats[k] : nullptr;
+        const int L = (int)H[k].size();
+        if (lo[k] == L) {
+            ev[k] = fiedler_levels(c, H[k], L - 1, std::move(cur[k]), normLap, tol, st);
+        } else if (lo[k] == 0) {
+            ev[k] = std::move(out[k][0]);
+            if (st && !normLap) {
+                double elog[3];
+                eigen_log_level(c, H[k][0].g, ev[k].get(), elog);
+                st->has_eigen_log = 1;
+                st->eigenvalue = elog[0];
+                st->eigen_min = elog[1];
+                st->eigen_max = elog[2];
+            }
+        } else {
+            const int l = lo[k] - 1;
+            DBuf<double> fine(&c.pool, H[k][l].g.n);
+            prolong(c, H[k][l].vcmap.get(), H[k][l].g.n, out[k][l + 1].get(), fine.get());
+            ev[k] = fiedler_levels(c, H[k], l, std::move(fine), normLap, tol, st);
+        }
+        out[k].clear();
+        cur[k] = DBuf<double>();
+    }
+    MI_HIP(hipStreamSynchronize(c.stream));
+    auto t2 = std::chrono::steady_clock::now();
+    H.clear();
+    for (size_t k = 0; k < K; k++) {
+        edge_cuts[k] = compute_partition(c, *gs[k], ev[k].get(), config->num_partitions, perc_imbalance, d_parts[k]);
+        if (stats) stats[k].edge_cut = edge_cuts[k];
+        ev[k] = DBuf<double>();
+    }
+    MI_HIP(hipStreamSynchronize(c.stream));
+    auto t3 = std::chrono::steady_clock::now();
+    if (phase_ms) {
+        phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        phase_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        phase_ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    }
+}
+
 }  // namespace
 
 int mi_partition_graph(mi_ctx* ctx, const mi_graph* g, const mi_config* config, int perc_imbalance,
@@ -783,120 +902,23 @@ int mi_bisect_batch(mi_ctx* ctx, int count, const mi_graph* const* graphs, const
         Ctx& c = ctx->c;
         if (count == 0) return;
         check_config(config);
-        const int normLap = config->refine_alg;
-        const double tol = config->tol;
-        const size_t K = (size_t)count;
-        auto t0 = std::chrono::steady_clock::now();
-        // hierarchies, graph by graph
-        std::vector<std::vector<DLevel>> H(K);
-        for (size_t k = 0; k < K; k++) {
-            mi_stats* st = stats ? &stats[k] : nullptr;
-            if (st) std::memset(st, 0, sizeof(*st));
-            // (no per-phase timing: it would synchronise once per level)
-            build_hierarchy(c, graphs[k]->g, hec_seeds[k], H[k], nullptr);
-            fill_level_stats(H[k], st);
-        }
-        MI_HIP(hipStreamSynchronize(c.stream));
-        auto t1 = std::chrono::steady_clock::now();
-        // start vectors (consuming each graph's pcg state, as one call would)
-        // and the chains of small levels from the coarsest: level l of graph
-        // k is batched when every coarser level is
-        std::vector<DBuf<double>> cur(K);
-        std::vector<int> lo(K);
-        int depth = 0;
-        for (size_t k = 0; k < K; k++) {
-            const int L = (int)H[k].size();
-            MI_CHECK(L >= 1, ST_STATE, "empty hierarchy");
-            cur[k] = start_vector(c, H[k][L - 1].g.n, &pcg_states[k], pcg_inc);
-            int l = L - 1;
-            while (l >= 0) {
-                const DGraph& g = H[k][l].g;
-                const bool mixed = c.mixed && l > 0 && g.m >= c.mixed_min_m;
-                if (!small_lds_level(c, g, mixed)) break;
-                l--;
-            }
-            lo[k] = l + 1;
-            depth = std::max(depth, L - lo[k]);
-            if (stats) {
-                stats[k].has_eigen_log = 0;
-                stats[k].mixed = 0;
-            }
-        }
-        std::vector<std::vector<DBuf<double>>> out(K);
-        for (size_t k = 0; k < K; k++) out[k].resize(H[k].size());
-        for (int d = 0; d < depth; d++) {
-            std::vector<BatchLevel> jobs;
-            std::vector<std::pair<size_t, int>> who;
-            for (size_t k = 0; k < K; k++) {
-                const int L = (int)H[k].size(), l = L - 1 - d;
-                if (l < lo[k]) continue;
-                BatchLevel b;
-                b.g = &H[k][l].g;
-                b.start = d == 0 ? cur[k].get() : out[k][l + 1].get();
-                b.vcmap = d == 0 ? nullptr : H[k][l].vcmap.get();
-                out[k][l] = DBuf<double>(&c.pool, H[k][l].g.n);
-                b.out = out[k][l].get();
-                jobs.push_back(b);
-                who.emplace_back(k, l);
-            }
-            power_iter_small_batch(c, jobs, normLap, tol);
-            for (size_t j = 0; j < jobs.size(); j++) {
-                const size_t k = who[j].first;
-                const int l = who[j].second;
-                if (stats && l < MI_MAX_LEVELS) {
-                    stats[k].level_iters[l] = jobs[j].iters;
-                    stats[k].level_max_reached[l] = jobs[j].max_reached;
-                }
-            }
-        }
-        // the remaining (larger) levels, graph by graph
-        std::vector<DBuf<double>> ev(K);
-        for (size_t k = 0; k < K; k++) {
-            mi_stats* st = stats ? &stats[k] : nullptr;
-            const int L = (int)H[k].size();
-            if (lo[k] == L) {
-                ev[k] = fiedler_levels(c, H[k], L - 1, std::move(cur[k]), normLap, tol, st);
-            } else if (lo[k] == 0) {
-                ev[k] = std::move(out[k][0]);
-                if (st && !normLap) {
-                    double elog[3];
-                    eigen_log_level(c, H[k][0].g, ev[k].get(), elog);
-                    st->has_eigen_log = 1;
-                    st->eigenvalue = elog[0];
-                    st->eigen_min = elog[1];
-                    st->eigen_max = elog[2];
-                }
+        std::vector<const DGraph*> gs;
+        std::vector<DBuf<uint32_t>> own((size_t)count);
+        std::vector<uint32_t*> dp;
+        for (int k = 0; k < count; k++) {
+            gs.push_back(&graphs[k]->g);
+            if (parts_on_device) {
+                dp.push_back(parts[k]);
             } else {
-                const int l = lo[k] - 1;
-                DBuf<double> fine(&c.pool, H[k][l].g.n);
-                prolong(c, H[k][l].vcmap.get(), H[k][l].g.n, out[k][l + 1].get(), fine.get());
-                ev[k] = fiedler_levels(c, H[k], l, std::move(fine), normLap, tol, st);
+                own[k] = DBuf<uint32_t>(&c.pool, graphs[k]->g.n);
+                dp.push_back(own[k].get());
             }
-            out[k].clear();
-            cur[k] = DBuf<double>();
         }
-        MI_HIP(hipStreamSynchronize(c.stream));
-        auto t2 = std::chrono::steady_clock::now();
-        H.clear();
-        for (size_t k = 0; k < K; k++) {
-            const DGraph& g = graphs[k]->g;
-            DBuf<uint32_t> dp;
-            uint32_t* d_part = parts[k];
-            if (!parts_on_device) {
-                dp = DBuf<uint32_t>(&c.pool, g.n);
-                d_part = dp.get();
-            }
-            edge_cuts[k] = compute_partition(c, g, ev[k].get(), config->num_partitions, perc_imbalance, d_part);
-            if (!parts_on_device) d2h(c, parts[k], d_part, (size_t)g.n * 4);
-            if (stats) stats[k].edge_cut = edge_cuts[k];
-            ev[k] = DBuf<double>();
-        }
-        MI_HIP(hipStreamSynchronize(c.stream));
-        auto t3 = std::chrono::steady_clock::now();
-        if (phase_ms) {
-            phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-            phase_ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
-            phase_ms[2] = std::chrono::duration<double, std::milli>(t3 - t2).count();
+        bisect_many(c, gs, config, perc_imbalance, hec_seeds, pcg_states, false, pcg_inc, dp, edge_cuts, stats,
+                    phase_ms);
+        if (!parts_on_device) {
+            for (int k = 0; k < count; k++) d2h(c, parts[k], dp[k], (size_t)graphs[k]->g.n * 4);
+            MI_HIP(hipStreamSynchronize(c.stream));
         }
     });
 }
@@ -940,14 +962,25 @@ int mi_partition_4way(mi_ctx* ctx, const mi_graph* g, const mi_config* config, i
         DGraph side[2];
         DBuf<uint32_t> mapping;
         use_partition(c, g->g, top.get(), side[0], side[1], mapping);
+        // the two sub-bisections together (their small levels in shared
+        // launches), the PCG state carried from side 0 to side 1 as two
+        // consecutive calls would carry it
         DBuf<uint32_t> sub[2];
+        std::vector<const DGraph*> gs;
+        std::vector<uint32_t*> dp;
+        std::vector<int> which;
         for (int s = 0; s < 2; s++) {
             cuts[1 + s] = 0;
             if (side[s].n < 2) continue;  // nothing to split
             sub[s] = DBuf<uint32_t>(&c.pool, side[s].n);
-            cuts[1 + s] =
-                bisect(c, side[s], config, perc_imbalance, hec_seed, pcg_state, pcg_inc, sub[s].get(), nullptr);
+            gs.push_back(&side[s]);
+            dp.push_back(sub[s].get());
+            which.push_back(s);
         }
+        const uint64_t seeds[2] = {hec_seed, hec_seed};
+        uint32_t sc[2] = {0, 0};
+        bisect_many(c, gs, config, perc_imbalance, seeds, pcg_state, true, pcg_inc, dp, sc, nullptr, nullptr);
+        for (size_t k = 0; k < which.size(); k++) cuts[1 + which[k]] = sc[k];
         *edge_cut_4way = cuts[0] + cuts[1] + cuts[2];
         DBuf<uint32_t> out;
         uint32_t* dst = part;
