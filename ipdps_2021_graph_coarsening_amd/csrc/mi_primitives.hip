@@ -316,6 +316,57 @@ static void radix_pass(Ctx& c, const K* ksrc, const uint32_t* vsrc, K* kdst, uin
     MI_HIP(hipGetLastError());
 }
 
+// Small arrays (n <= SMALL_SORT) in one 1024-thread workgroup: a bitonic
+// network in LDS on (digit, original index), which is the stable order of
+// the digits (every pair compares distinct).  Padding entries carry the
+// largest digit and index and sort last.  One launch instead of two or
+// three per radix pass.
+constexpr uint32_t SMALL_SORT = 4096;
+
+template <class K>
+__global__ void __launch_bounds__(1024) k_sort_small(K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n,
+                                                     uint32_t np2, int begin_bit, int nbits)
+{
+    __shared__ K sk[SMALL_SORT];
+    __shared__ uint32_t sv[SMALL_SORT];
+    __shared__ uint32_t si[SMALL_SORT];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < np2; i += blockDim.x) {
+        const bool in = i < n;
+        sk[i] = in ? keys[i] : ~(K)0;
+        sv[i] = in ? vals[i] : 0u;
+        si[i] = in ? i : 0xffffffffu;
+    }
+    __syncthreads();
+    const K mask = nbits >= (int)(8 * sizeof(K)) ? ~(K)0 : (((K)1 << nbits) - (K)1);
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < (np2 >> 1); t += blockDim.x) {
+                const uint32_t i = (t / stride) * 2 * stride + (t % stride), j = i + stride;
+                const K di = (sk[i] >> begin_bit) & mask, dj = (sk[j] >> begin_bit) & mask;
+                const bool gt = di > dj || (di == dj && si[i] > si[j]);
+                const bool up = (i & size) == 0;
+                if (gt == up) {
+                    const K tk = sk[i];
+                    sk[i] = sk[j];
+                    sk[j] = tk;
+                    const uint32_t tv = sv[i];
+                    sv[i] = sv[j];
+                    sv[j] = tv;
+                    const uint32_t ti = si[i];
+                    si[i] = si[j];
+                    si[j] = ti;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+        keys[i] = sk[i];
+        vals[i] = sv[i];
+    }
+}
+
 // Sorts on bits [begin_bit, end_bit) in the fewest passes of at most 9 bits
 // (8-bit passes when that takes no more of them), the widths split evenly.
 // The result ends in whichever buffer pair the last pass wrote: on return
@@ -327,6 +378,14 @@ static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int b
     MI_CHECK(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit, ST_ARG, "radix bit range");
     if (n <= 1 || begin_bit == end_bit) return;
     const int total = end_bit - begin_bit;
+    if (n <= SMALL_SORT) {
+        // (the result stays in keys / vals)
+        uint32_t np2 = 2;
+        while (np2 < n) np2 <<= 1;
+        k_sort_small<K><<<1, 1024, 0, c.stream>>>(keys, vals, (uint32_t)n, np2, begin_bit, total);
+        MI_HIP(hipGetLastError());
+        return;
+    }
     const int p8 = (total + 7) / 8, p9 = (total + 8) / 9;
     const bool wide = p9 < p8;
     const int passes = wide ? p9 : p8;

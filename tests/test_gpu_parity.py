@@ -38,7 +38,8 @@ def test_scan(ctx, n):
 
 
 @pytest.mark.parametrize("n,b,e", [(1, 0, 64), (5000, 0, 64), (100003, 0, 38), (70000, 8, 40), (4096 * 3, 0, 8),
-                                   (100003, 0, 34), (50000, 3, 29), (8193, 0, 9), (20000, 5, 22)])
+                                   (100003, 0, 34), (50000, 3, 29), (8193, 0, 9), (20000, 5, 22),
+                                   (2, 0, 64), (3, 1, 2), (1000, 0, 64), (4096, 3, 29), (4095, 0, 9), (2049, 40, 64)])
 def test_radix_sort(ctx, n, b, e):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2)
@@ -52,7 +53,7 @@ def test_radix_sort(ctx, n, b, e):
     assert np.array_equal(v, order.astype(np.uint32)) and np.array_equal(k, keys[order])
 
 
-@pytest.mark.parametrize("n,hi_values", [(5000, 40), (20000, 3000), (600, 1)])
+@pytest.mark.parametrize("n,hi_values", [(5000, 40), (20000, 3000), (600, 1), (3000, 25)])
 def test_sort_random_keys_equal_high_halves(ctx, n, hi_values):
     """The permutation sort runs on the high halves and then orders the runs
     of equal high halves by the whole key; many and long such runs, ties on
