@@ -262,6 +262,60 @@ __global__ void k_map_propagate(uint32_t n, const uint32_t* __restrict__ hn, uin
     fl[u] = f | FL_PROP;
 }
 
+// The resolve / propagate rounds of a level of at most MAP_ONE_WG vertices in
+// one 1024-thread workgroup: the same two phases per round, a workgroup
+// barrier between them in place of the kernel boundary, until a resolve
+// leaves nothing unknown (or `cap` rounds).  Rounds after that point change
+// nothing, so the flags end as with the launch-per-phase loop.  `unknown`
+// receives the count left by the last resolve (0 on success).
+constexpr uint32_t MAP_ONE_WG = 8192;
+
+__global__ void __launch_bounds__(1024) k_map_rounds(uint32_t n, const uint32_t* __restrict__ hn, uint8_t* fl,
+                                                     const uint32_t* __restrict__ indeg, uint32_t* nonact,
+                                                     uint32_t* hasact, uint32_t cap, uint32_t* __restrict__ unknown)
+{
+    __shared__ uint32_t s_left;
+    const uint32_t tid = threadIdx.x;
+    uint32_t left_all = 0;
+    for (uint32_t round = 0; round < cap; round++) {
+        if (tid == 0) s_left = 0;
+        __syncthreads();
+        uint32_t left = 0;
+        for (uint32_t u = tid; u < n; u += blockDim.x) {
+            const uint8_t f = fl[u];
+            if ((f & FL_R) && !(f & (FL_ACT | FL_NACT))) {
+                // hasact / nonact were written by other waves of this launch
+                // (nonact by atomics, which execute in L2): read them from L2.
+                if (__hip_atomic_load(&hasact[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    fl[u] = f | FL_NACT;
+                else if (__hip_atomic_load(&nonact[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == indeg[u])
+                    fl[u] = f | FL_ACT;
+                else
+                    left++;
+            }
+        }
+        left = wave_sum_u32(left);
+        if ((tid & (WAVE - 1)) == 0 && left) atomicAdd(&s_left, left);
+        __syncthreads();
+        for (uint32_t u = tid; u < n; u += blockDim.x) {
+            const uint8_t f = fl[u];
+            if (!(f & FL_R) || !(f & (FL_ACT | FL_NACT)) || (f & FL_PROP)) continue;
+            const uint32_t v = hn[u];
+            if (fl[v] & FL_R) {
+                if (f & FL_ACT)
+                    __hip_atomic_store(&hasact[v], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    atomicAdd(&nonact[v], 1u);
+            }
+            fl[u] = f | FL_PROP;
+        }
+        left_all = s_left;
+        __syncthreads();
+        if (left_all == 0) break;
+    }
+    if (tid == 0) unknown[0] = left_all;
+}
+
 __global__ void k_map_min2(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
                            const uint8_t* __restrict__ fl, uint32_t* __restrict__ minpos2)
 {
@@ -440,7 +494,12 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     // u -> hn(u) has strictly increasing positions, so the lowest unresolved
     // vertex always resolves and at most n rounds are needed.
     uint32_t rounds = 0;
-    for (;;) {
+    if (n <= MAP_ONE_WG) {
+        k_map_rounds<<<1, 1024, 0, c.stream>>>(n, hn.get(), fl.get(), indeg.get(), nonact.get(), hasact.get(),
+                                               n + 8, totals.get() + 2);
+        MI_HIP(hipGetLastError());
+    }
+    for (; n > MAP_ONE_WG;) {
         const int batch = 4;
         for (int b = 0; b < batch; b++) {
             MI_HIP(hipMemsetAsync(totals.get() + 2, 0, 4, c.stream));
@@ -474,6 +533,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     MI_HIP(hipGetLastError());
     uint32_t tot[4] = {0, 0, 0, 0};
     read_words(c, totals.get(), tot, 16);
+    MI_CHECK(tot[2] == 0, ST_INTERNAL, "map construction did not converge");
     MI_CHECK(tot[0] + tot[1] >= 1 && tot[3] < tot[0] + tot[1], ST_INTERNAL,
              "map construction left a vertex without a cluster id");
     if (vperm_out)

@@ -90,6 +90,11 @@ def _hec_cases():
     rp4, col4, w4 = graphs.rmat(13, 16, seed=1)
     yield "rmat13-l1", rp4, col4, w4, 1, 1
     yield "rmat13-l2", rp4, col4, w4, 2, 1
+    # the map's rounds run in one workgroup up to 8192 vertices, in launches above
+    rp5, col5, w5 = random_graph(rng, 8192, 0.0008, weighted=True)
+    yield "random-8192-l2", rp5, col5, w5, 2, 3
+    rp6, col6, w6 = random_graph(rng, 8193, 0.0008, weighted=True)
+    yield "random-8193-l2", rp6, col6, w6, 2, 3
 
 
 @pytest.mark.parametrize("case", list(_hec_cases()), ids=lambda c: c[0])
