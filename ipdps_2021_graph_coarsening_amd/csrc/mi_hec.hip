@@ -469,50 +469,52 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     // ---- map construction, pass 1 (closed form)
     DBuf<uint8_t> fl(&c.pool, n);
     DBuf<uint32_t> minpos(&c.pool, n), id(&c.pool, n);
-    DBuf<uint32_t> totals(&c.pool, 4);
+    // one zero fill for the four totals and pass 2's three counters
+    DBuf<uint32_t> zeroed(&c.pool, 4 + 3 * (size_t)n);
+    MI_HIP(hipMemsetAsync(zeroed.get(), 0, (4 + 3 * (size_t)n) * 4, c.stream));
+    uint32_t* const totals = zeroed.get();
+    uint32_t* const indeg = totals + 4;
+    uint32_t* const nonact = indeg + n;
+    uint32_t* const hasact = nonact + n;
     fill_u32(c, minpos.get(), n, INF);
     k_map_mark<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(), vcmap);
     MI_HIP(hipGetLastError());
     k_map_creators1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
                                                       id.get());
     MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, id.get(), id.get(), n, totals.get() + 0);
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 0);
     k_map_assign1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), pos.get(), hn.get(), fl.get(),
                                                     minpos.get(), id.get(), vcmap);
     MI_HIP(hipGetLastError());
 
     // ---- pass 2
-    DBuf<uint32_t> indeg(&c.pool, n), nonact(&c.pool, n), hasact(&c.pool, n);
-    MI_HIP(hipMemsetAsync(indeg.get(), 0, (size_t)n * 4, c.stream));
-    MI_HIP(hipMemsetAsync(nonact.get(), 0, (size_t)n * 4, c.stream));
-    MI_HIP(hipMemsetAsync(hasact.get(), 0, (size_t)n * 4, c.stream));
     k_map_r<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, fl.get());
     MI_HIP(hipGetLastError());
-    k_map_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), indeg.get());
+    k_map_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), indeg);
     MI_HIP(hipGetLastError());
     // Rounds until every unassigned vertex is resolved; the forest formed by
     // u -> hn(u) has strictly increasing positions, so the lowest unresolved
     // vertex always resolves and at most n rounds are needed.
     uint32_t rounds = 0;
     if (n <= MAP_ONE_WG) {
-        k_map_rounds<<<1, 1024, 0, c.stream>>>(n, hn.get(), fl.get(), indeg.get(), nonact.get(), hasact.get(),
-                                               n + 8, totals.get() + 2);
+        k_map_rounds<<<1, 1024, 0, c.stream>>>(n, hn.get(), fl.get(), indeg, nonact, hasact,
+                                               n + 8, totals + 2);
         MI_HIP(hipGetLastError());
     }
     for (; n > MAP_ONE_WG;) {
         const int batch = 4;
         for (int b = 0; b < batch; b++) {
-            MI_HIP(hipMemsetAsync(totals.get() + 2, 0, 4, c.stream));
-            k_map_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, fl.get(), indeg.get(), nonact.get(),
-                                                            hasact.get(), totals.get() + 2);
+            MI_HIP(hipMemsetAsync(totals + 2, 0, 4, c.stream));
+            k_map_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, fl.get(), indeg, nonact,
+                                                            hasact, totals + 2);
             MI_HIP(hipGetLastError());
-            k_map_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), nonact.get(),
-                                                              hasact.get());
+            k_map_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), nonact,
+                                                              hasact);
             MI_HIP(hipGetLastError());
         }
         rounds += batch;
         uint32_t left = 0;
-        read_words(c, totals.get() + 2, &left, 4);
+        read_words(c, totals + 2, &left, 4);
         if (left == 0) break;
         MI_CHECK(rounds <= n + 8, ST_INTERNAL, "map construction did not converge");
     }
@@ -522,17 +524,16 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     k_map_creators2<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
                                                       id.get());
     MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, id.get(), id.get(), n, totals.get() + 1);
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 1);
     k_map_assign2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(),
-                                                    id.get(), totals.get() + 0, vcmap);
+                                                    id.get(), totals + 0, vcmap);
     MI_HIP(hipGetLastError());
     if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
 
-    MI_HIP(hipMemsetAsync(totals.get() + 3, 0, 4, c.stream));
-    k_max_u32<<<red_grid(n), BLOCK, 0, c.stream>>>(vcmap, n, totals.get() + 3);
+    k_max_u32<<<red_grid(n), BLOCK, 0, c.stream>>>(vcmap, n, totals + 3);
     MI_HIP(hipGetLastError());
     uint32_t tot[4] = {0, 0, 0, 0};
-    read_words(c, totals.get(), tot, 16);
+    read_words(c, totals, tot, 16);
     MI_CHECK(tot[2] == 0, ST_INTERNAL, "map construction did not converge");
     MI_CHECK(tot[0] + tot[1] >= 1 && tot[3] < tot[0] + tot[1], ST_INTERNAL,
              "map construction left a vertex without a cluster id");
