@@ -39,7 +39,8 @@ def test_scan(ctx, n):
 
 @pytest.mark.parametrize("n,b,e", [(1, 0, 64), (5000, 0, 64), (100003, 0, 38), (70000, 8, 40), (4096 * 3, 0, 8),
                                    (100003, 0, 34), (50000, 3, 29), (8193, 0, 9), (20000, 5, 22),
-                                   (2, 0, 64), (3, 1, 2), (1000, 0, 64), (4096, 3, 29), (4095, 0, 9), (2049, 40, 64)])
+                                   (2, 0, 64), (3, 1, 2), (1000, 0, 64), (4096, 3, 29), (4095, 0, 9), (2049, 40, 64),
+                                   (4097, 0, 64)])
 def test_radix_sort(ctx, n, b, e):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2)
@@ -108,10 +109,11 @@ def test_hec_level_bitexact(ctx, case):
     assert nc == onc and np.array_equal(vc, ovc)
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_build_coarse_bitexact(ctx, seed):
+@pytest.mark.parametrize("seed,n,p", [(0, 5000, 0.002), (1, 5000, 0.002), (2, 1500, 0.0035)])
+def test_build_coarse_bitexact(ctx, seed, n, p):
+    # (1500, 0.0035): about 7,800 stored entries, 18-bit keys (32-bit sort)
     rng = np.random.default_rng(seed)
-    rp, col, w = random_graph(rng, 5000, 0.002, weighted=True)
+    rp, col, w = random_graph(rng, n, p, weighted=True)
     vc, nc, _, _ = oracle.coarsen_hec(2, rp, col, w, seed)
     g = ctx.graph(rp, col, w)
     gc, cvw = g.build_coarse(vc, nc)
