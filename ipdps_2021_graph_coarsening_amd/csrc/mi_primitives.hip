@@ -367,6 +367,51 @@ __global__ void __launch_bounds__(1024) k_sort_small(K* __restrict__ keys, uint3
     }
 }
 
+// The same order for digits of at most 52 bits, with one array in the
+// network: each entry is the 64-bit word (digit << 12) | original index
+// (n <= 4096), so a compare-exchange moves one word instead of three; keys
+// and values are staged in LDS and gathered by the sorted indices at the
+// end.  Padding words are all ones above their position and sort last.
+template <class K>
+__global__ void __launch_bounds__(1024) k_sort_small_packed(K* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                            uint32_t n, uint32_t np2, int begin_bit, int nbits)
+{
+    __shared__ uint64_t sc[SMALL_SORT];
+    __shared__ K sk[SMALL_SORT];
+    __shared__ uint32_t sv[SMALL_SORT];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t mask = ((uint64_t)1 << nbits) - 1;  // nbits <= 52
+    for (uint32_t i = tid; i < np2; i += blockDim.x) {
+        if (i < n) {
+            const K k = keys[i];
+            sk[i] = k;
+            sv[i] = vals[i];
+            sc[i] = ((((uint64_t)k >> begin_bit) & mask) << 12) | i;
+        } else {
+            sc[i] = (~(uint64_t)0 << 12) | i;
+        }
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < (np2 >> 1); t += blockDim.x) {
+                const uint32_t i = (t / stride) * 2 * stride + (t % stride), j = i + stride;
+                const uint64_t a = sc[i], b = sc[j];
+                if ((a > b) == ((i & size) == 0)) {
+                    sc[i] = b;
+                    sc[j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = tid; i < n; i += blockDim.x) {
+        const uint32_t src = (uint32_t)(sc[i] & 0xfffu);
+        keys[i] = sk[src];
+        vals[i] = sv[src];
+    }
+}
+
 // Sorts on bits [begin_bit, end_bit) in the fewest passes of at most 9 bits
 // (8-bit passes when that takes no more of them), the widths split evenly.
 // The result ends in whichever buffer pair the last pass wrote: on return
@@ -382,7 +427,10 @@ static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int b
         // (the result stays in keys / vals)
         uint32_t np2 = 2;
         while (np2 < n) np2 <<= 1;
-        k_sort_small<K><<<1, 1024, 0, c.stream>>>(keys, vals, (uint32_t)n, np2, begin_bit, total);
+        if (total <= 52)
+            k_sort_small_packed<K><<<1, 1024, 0, c.stream>>>(keys, vals, (uint32_t)n, np2, begin_bit, total);
+        else
+            k_sort_small<K><<<1, 1024, 0, c.stream>>>(keys, vals, (uint32_t)n, np2, begin_bit, total);
         MI_HIP(hipGetLastError());
         return;
     }

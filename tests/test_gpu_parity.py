@@ -40,7 +40,7 @@ def test_scan(ctx, n):
 @pytest.mark.parametrize("n,b,e", [(1, 0, 64), (5000, 0, 64), (100003, 0, 38), (70000, 8, 40), (4096 * 3, 0, 8),
                                    (100003, 0, 34), (50000, 3, 29), (8193, 0, 9), (20000, 5, 22),
                                    (2, 0, 64), (3, 1, 2), (1000, 0, 64), (4096, 3, 29), (4095, 0, 9), (2049, 40, 64),
-                                   (4097, 0, 64)])
+                                   (4097, 0, 64), (3000, 12, 64), (2500, 0, 53)])
 def test_radix_sort(ctx, n, b, e):
     rng = np.random.default_rng(n)
     keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64) * np.uint64(2)
