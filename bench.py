@@ -1,41 +1,49 @@
-"""Benchmark: plain FP64 multilevel spectral bisection of an R-MAT graph on
-one MI355X.
+"""Benchmark: multilevel spectral bisection of an R-MAT-24 graph on one
+MI355X, plain FP64 and mixed precision.
 
-Workload: R-MAT scale 20 (1,048,576 vertices), edge factor 16,
-(a, b, c, d) = (0.57, 0.19, 0.19, 0.05), seed 1; symmetrised, self-loops
-dropped, duplicates merged with summed u32 weights, largest connected
-component kept.  One step = one whole bisection (HEC coarsening to <= 50
-vertices, P^T A P builds, multilevel FP64 power iteration, sort-and-split)
-with the graph already resident in HBM and the partition left in HBM.
+Workload (the 4-way configuration's graph): R-MAT scale 24 (16,777,216
+vertices), edge factor 16, (a, b, c, d) = (0.57, 0.19, 0.19, 0.05), seed 4,
+drawn on the GPU; symmetrised, self-loops dropped, duplicates merged with
+summed u32 weights, largest connected component kept (8,866,537 vertices,
+520,765,078 stored entries).  One step = one whole bisection (HEC coarsening
+to <= 50 vertices, P^T A P builds, multilevel FP64 power iteration,
+sort-and-split) with the graph already resident in HBM and the partition
+left in HBM; HEC and PCG seeds 4.
 
-Prints one JSON line.  `value` is milliseconds per bisection (lower is
-better).  `roofline` is the kernel that dominates the bisection: the
-streaming kernel of the FP64 power-iteration step (k_power_stream) on the
-level with the most eigensolve time (level 1 of R-MAT-20, which takes about
-400 iterations), timed with HIP events on the library stream over 50
-launches; its algorithmic bytes are col + w + x + wdeg + y of that level.
-`traffic` is the HBM traffic per launch of that kernel measured with
-rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a copy of
-known size (profiles/k_power_stream_rmat20_L1_traffic.txt), or null when
-that summary is absent.  `cpu_baseline` times the CPU oracle (a scalar
-restatement of the reference's Serial path) on a bounded sample and scales
-it to one whole bisection.  `mixed` is the same bisection in mixed precision
-(FP32 vectors on the levels above the finest with at least 2^20 entries,
-FP64 sums, deflation, stopping test and finest level) beside the FP64 one,
-with `per_level` comparing each FP32 level's power iteration with the FP64
-one from the same start (iterations, angle, Rayleigh quotient, residual);
-`mixed_grid3d` the same comparison on the 128^3 7-point grid with weights
-U{1..16}.  `batched_rgg512` times the batched bisection of 512 random
-geometric graphs (mi_bisect_batch) against one call per graph.
+Prints one JSON line.  `value` is milliseconds per FP64 bisection (lower is
+better).  Beside it:
+- `config.end_to_end_ms`: the same bisection from host CSR arrays to a host
+  partition (upload, bisection, download), what a caller of the reference's
+  sgp_partition_graph gets;
+- `mixed`: the bisection in mixed precision (FP32 vectors on the levels above
+  the finest with at least 2^20 entries; FP64 sums, deflation, stopping test
+  and finest level) beside the FP64 one, at the cut each ends with;
+- `four_way`: the 4-way partition by recursive bisection (the configuration
+  itself), FP64 and mixed;
+- `roofline`: the streaming kernel of the power-iteration step
+  (k_power_stream) on the level with the most eigensolve time, timed with HIP
+  events on the library stream over 50 launches; algorithmic bytes are
+  col + w + x of that level (per launch).  `traffic` is the HBM traffic per
+  launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the same counters
+  on a copy of known size (profiles/k_power_stream_<workload>_L<l>_traffic.txt),
+  or null when that summary is absent;
+- `cpu_baseline`: the OpenMP restatement of the path (oracle/sgpar_omp.c) on
+  all host cores, whole bisections of the same graph, median of the runs;
+- `extras`: R-MAT-20 (seed 1) FP64 and mixed with a per-level precision
+  report, the 128^3 weighted 3D grid FP64 and mixed, and the 512-RGG batch
+  (mi_bisect_batch) against one call per graph.
 
     python bench.py [--steps K] [--warmup W]
-    python bench.py --roofline-only      # only the roofline measurement (for rocprofv3 runs)
+    python bench.py --scale 20                # R-MAT-20 as the main line
+    python bench.py --roofline-only           # only the roofline measurement (for rocprofv3 runs)
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -45,15 +53,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-SEED = 1
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_power_stream_rmat20_L1_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per launch of k_power_stream on R-MAT-20 level 1 from the
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_file(workload: str, level: int) -> str:
+    return os.path.join(ROOT, "profiles", f"k_power_stream_{workload}_L{level}_traffic.txt")
+
+
+def measured_traffic(workload: str, level: int):
+    """HBM bytes per launch of k_power_stream on that level from the
     committed rocprofv3 counter summary (line 'traffic_bytes_per_launch: X')."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(workload, level)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -62,60 +76,67 @@ def measured_traffic():
     return None
 
 
-def log(msg: str) -> None:
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def cpu_baseline(rp, col, w, gpu_stats, budget_s: float = 20.0) -> dict:
-    """Times the oracle on a bounded sample of the same bisection: the full
-    coarsening, then per level a capped number of power iterations (time per
-    iteration), then the split; scales the per-iteration times by the GPU
-    run's iteration counts (the oracle follows the same iteration)."""
-    import oracle
-
-    t0 = time.perf_counter()
-    H = oracle.Hierarchy(rp, col, w, SEED)
-    t_coarsen = time.perf_counter() - t0
-    iters_gpu = gpu_stats["level_iters"]
-    per_iter = []
-    rng = np.random.default_rng(0)
-    spent = t_coarsen
-    for l in range(H.nlevels):
-        L = H.levels[l]
-        cap = 50 if L["n"] > 100000 else min(400, max(1, int(iters_gpu[l]) if l < len(iters_gpu) else 400))
-        u0 = rng.random(L["n"])
-        t = time.perf_counter()
-        _, it, _, _ = oracle.power_iter(L["rowptr"], L["col"], L["w"], u0, 0, 1e-10, cap)
-        dt = time.perf_counter() - t
-        per_iter.append(dt / max(it, 1))
-        spent += dt
-        if spent > budget_s:
-            break
-    while len(per_iter) < H.nlevels:
-        per_iter.append(per_iter[-1])
-    ev = rng.standard_normal(len(rp) - 1)
-    t = time.perf_counter()
-    oracle.compute_partition(rp, col, ev)
-    t_split = time.perf_counter() - t
-    est = t_coarsen + sum(p * int(i) for p, i in zip(per_iter, iters_gpu)) + t_split
-    return {"value": round(est * 1000.0, 1), "unit": "ms", "cores": 1, "kind": "port",
-            "sample": (f"oracle (scalar C restatement of the reference Serial path) on the same graph: "
-                       f"full coarsening {t_coarsen:.2f} s + capped power iterations per level "
-                       f"(per-iteration time x the GPU run's iteration counts) + split {t_split:.2f} s")}
-
-
 def merge_rate(st: dict) -> dict:
     """Algorithmic bytes of the coarse builds (P^T A P with duplicate merge)
     over their time: per level l -> l+1, read the fine CSR 4(n+1) + 4m + 4m
-    (weights; the unit-weight finest level stores none) and vcmap 4n, write
+    (weights; a unit-weight finest level stores none) and vcmap 4n, write
     the coarse CSR 4(nc+1) + 8 m_c."""
     ns, ms = st["level_n"], st["level_m"]
     b = 0.0
     for l in range(len(ns) - 1):
-        b += 4 * (ns[l] + 1) + 4 * ms[l] + (4 * ms[l] if l > 0 else 0) + 4 * ns[l]
+        b += 4 * (ns[l] + 1) + 4 * ms[l] + (4 * ms[l] if (l > 0 or st.get("weighted")) else 0) + 4 * ns[l]
         b += 4 * (ns[l + 1] + 1) + 8 * ms[l + 1]
     t = st["build_ms"]
-    return {"alg_bytes": b, "ms": round(t, 3), "gbps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+    return {"alg_bytes": b, "ms": round(t, 3), "gbps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None,
+            "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4) if t > 0 else None}
+
+
+class Runner:
+    """Bisections of one graph through the C ABI, partition left in HBM."""
+
+    def __init__(self, mi, ctx, g, seed: int):
+        self.mi, self.ctx, self.g, self.seed = mi, ctx, g, seed
+        self.L = ctx._L
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.d_part = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.d_part), 4 * g.n) == 0
+
+    def close(self):
+        self.hip.hipFree(self.d_part)
+
+    def bisect(self):
+        mi, ctx = self.mi, self.ctx
+        st = mi.Stats()
+        pcg = C.c_uint64(self.seed)
+        ec = C.c_uint32()
+        cfg = mi.Config(4, 0, 0, 1, 1e-10, 2)
+        ctx.check(self.L.mi_partition_graph(ctx.h, self.g.h, C.byref(cfg), 0, self.seed, C.byref(pcg), 1,
+                                            self.d_part, 1, C.byref(ec), C.byref(st)), "mi_partition_graph")
+        return st
+
+    def four_way(self):
+        mi, ctx = self.mi, self.ctx
+        pcg = C.c_uint64(self.seed)
+        cuts = np.zeros(3, dtype=np.uint32)
+        ec = C.c_uint32()
+        cfg = mi.Config(4, 0, 0, 1, 1e-10, 4)
+        ctx.check(self.L.mi_partition_4way(ctx.h, self.g.h, C.byref(cfg), 0, self.seed, C.byref(pcg), 1,
+                                           self.d_part, 1, cuts, C.byref(ec)), "mi_partition_4way")
+        return int(ec.value), [int(x) for x in cuts]
+
+    def timed(self, fn, steps: int, warmup: int, tag: str):
+        for i in range(warmup):
+            r = fn()
+            if hasattr(r, "total_ms"):
+                log(f"{tag} warmup {i}: {r.total_ms:.2f} ms, cut {r.edge_cut}, levels {r.nlevels}")
+        self.hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = fn()
+        self.hip.hipDeviceSynchronize()
+        return (time.perf_counter() - t0) * 1000.0 / steps, r
 
 
 def mixed_summary(ms64: float, st64: dict, ms32: float, st32: dict) -> dict:
@@ -125,17 +146,29 @@ def mixed_summary(ms64: float, st64: dict, ms32: float, st32: dict) -> dict:
             "cut_change_pct": round(100.0 * (st32["edge_cut"] - st64["edge_cut"]) / max(st64["edge_cut"], 1), 4),
             "fp32_levels": st32["mixed_levels"], "level_iters": st32["level_iters"],
             "fp64_level_iters": st64["level_iters"],
-            "level_eigen_ms": [round(x, 3) for x in st32["level_eigen_ms"]]}
+            "level_eigen_ms": [round(x, 3) for x in st32["level_eigen_ms"]],
+            "fp64_level_eigen_ms": [round(x, 3) for x in st64["level_eigen_ms"]],
+            "phases_ms": {k: round(st32[k], 3) for k in ("coarsen_ms", "eigen_ms", "partition_ms")}}
 
 
-def mixed_levels_report(mi, ctx, g, seed: int) -> list:
+def fiedler_quality(rp, col, w, u) -> dict:
+    """Rayleigh quotient and relative Fiedler residual ||L u - lam u|| /
+    ||lam u|| (L = D - A, u deflated) of a vector, on the host."""
+    import scipy.sparse as sp
+    n = len(rp) - 1
+    A = sp.csr_matrix((w.astype(np.float64), col, rp), shape=(n, n))
+    Lap = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
+    u = u - u.mean()
+    lam = float(u @ (Lap @ u) / (u @ u))
+    return {"rayleigh": lam, "residual": float(np.linalg.norm(Lap @ u - lam * u) / (abs(lam) * np.linalg.norm(u)))}
+
+
+def mixed_levels_report(ctx, g, seed: int) -> list:
     """Per level that runs in FP32 (at least 2^20 entries, above the
     finest): the level's power iteration in mixed precision and in FP64 from
     the same start vector (FP64 on the device, which the tests hold to the
     oracle), with iterations, the angle between the two vectors, and each
-    one's Rayleigh quotient and relative Fiedler residual ||L u - lam u|| /
-    ||lam u|| (L = D - A, computed on the host)."""
-    import scipy.sparse as sp
+    one's Rayleigh quotient and relative Fiedler residual."""
     H = g.hierarchy(seed)
     out = []
     for l in range(1, H.nlevels):
@@ -143,8 +176,6 @@ def mixed_levels_report(mi, ctx, g, seed: int) -> list:
         if lv["m"] < (1 << 20):
             continue
         n = lv["n"]
-        A = sp.csr_matrix((lv["w"].astype(np.float64), lv["col"], lv["rowptr"]), shape=(n, n))
-        Lap = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
         lg = ctx.graph(lv["rowptr"], lv["col"], lv["w"])
         u0 = np.random.default_rng(l).random(n)
         ctx.set_precision(False)
@@ -153,53 +184,77 @@ def mixed_levels_report(mi, ctx, g, seed: int) -> list:
         u32, it32, _ = lg.power_iter(u0)
         ctx.set_precision(False)
         lg.close()
-
-        def quality(u):
-            u = u - u.mean()
-            lam = float(u @ (Lap @ u) / (u @ u))
-            return lam, float(np.linalg.norm(Lap @ u - lam * u) / (abs(lam) * np.linalg.norm(u)))
-
-        l64, r64 = quality(u64)
-        l32, r32 = quality(u32)
+        q64 = fiedler_quality(lv["rowptr"], lv["col"], lv["w"], u64)
+        q32 = fiedler_quality(lv["rowptr"], lv["col"], lv["w"], u32)
         out.append({"level": l, "n": n, "m": lv["m"], "iters_fp64": it64, "iters_mixed": it32,
                     "cos": round(float(abs(u64 @ u32) / (np.linalg.norm(u64) * np.linalg.norm(u32))), 12),
-                    "rayleigh_fp64": l64, "rayleigh_mixed": l32, "residual_fp64": round(r64, 6),
-                    "residual_mixed": round(r32, 6)})
+                    "rayleigh_fp64": q64["rayleigh"], "rayleigh_mixed": q32["rayleigh"],
+                    "residual_fp64": round(q64["residual"], 6), "residual_mixed": round(q32["residual"], 6)})
     H.close()
     return out
 
 
-def grid3d_mixed(mi, ctx, steps: int) -> dict:
-    """The 128^3 7-point grid with weights U{1..16} (seed 2): FP64 and
-    mixed-precision bisections, timed on the host clock around `steps` calls."""
+def bisect_pair(mi, ctx, g, seed: int, steps: int, warmup: int, tag: str):
+    """(FP64 ms, FP64 stats, mixed ms, mixed stats) of whole bisections."""
+    R = Runner(mi, ctx, g, seed)
+    try:
+        ms64, st64 = R.timed(R.bisect, steps, warmup, f"{tag} fp64")
+        ctx.set_precision(True)
+        ms32, st32 = R.timed(R.bisect, steps, warmup, f"{tag} mixed")
+    finally:
+        ctx.set_precision(False)
+        R.close()
+    return ms64, st64.as_dict(), ms32, st32.as_dict()
+
+
+def extra_rmat20(mi, ctx, steps: int) -> dict:
     from ipdps_2021_graph_coarsening_amd import graphs
-    rp, col, w = graphs.grid3d(128, seed=2)
+    t = time.perf_counter()
+    rp, col, w = graphs.rmat(20, 16, 0.57, 0.19, 0.19, seed=1)
+    log(f"R-MAT-20: generated in {time.perf_counter() - t:.1f} s")
     g = ctx.graph(rp, col, w)
-    res = {}
-    for mixed in (False, True):
-        ctx.set_precision(mixed)
-        r = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            r = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
-        res[mixed] = ((time.perf_counter() - t0) * 1000.0 / steps, r.stats)
-    ctx.set_precision(False)
+    ms64, st64, ms32, st32 = bisect_pair(mi, ctx, g, 1, steps, 2, "rmat20")
+    out = mixed_summary(ms64, st64, ms32, st32)
+    out["workload"] = f"rmat20-ef16-lcc-weighted (n={len(rp) - 1}, m={int(rp[-1])}, seed 1)"
+    out["level_n"] = st64["level_n"]
+    out["merge"] = merge_rate(dict(st64, weighted=True))
+    out["per_level"] = mixed_levels_report(ctx, g, 1)
     g.close()
-    out = mixed_summary(res[False][0], res[False][1], res[True][0], res[True][1])
-    out["workload"] = f"grid3d-128 (n={len(rp) - 1}, m={int(rp[-1])}, weights U1..16, seed 2)"
     return out
 
 
-def rgg_batch(mi, ctx, reps: int = 2) -> dict:
+def extra_grid3d(mi, ctx, steps: int) -> dict:
+    """The 128^3 7-point grid with weights U{1..16} (seed 2): FP64 and mixed."""
+    from ipdps_2021_graph_coarsening_amd import graphs
+    rp, col, w = graphs.grid3d(128, seed=2)
+    g = ctx.graph(rp, col, w)
+    ms64, st64, ms32, st32 = bisect_pair(mi, ctx, g, 2, steps, 1, "grid3d")
+    g.close()
+    out = mixed_summary(ms64, st64, ms32, st32)
+    out["workload"] = f"grid3d-128 (n={len(rp) - 1}, m={int(rp[-1])}, weights U1..16, seed 2)"
+    out["merge"] = merge_rate(dict(st64, weighted=True))
+    return out
+
+
+def _rgg_job(args):
+    from ipdps_2021_graph_coarsening_amd import graphs
+    n, seed = args
+    return graphs.rgg(int(n), seed=seed)
+
+
+def extra_rgg_batch(mi, ctx, reps: int = 2) -> dict:
     """512 random geometric graphs (unit square, seed 3, n log-uniform in
     [1000, 200000], radius for mean degree 10, largest component): the
     batched bisection (mi_bisect_batch) against one call per graph, both
     timed on the host clock; the results are identical graph by graph."""
-    from ipdps_2021_graph_coarsening_amd import graphs
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
     rng = np.random.default_rng(3)
     ns = np.exp(rng.uniform(np.log(1e3), np.log(2e5), 512)).astype(int)
     t = time.perf_counter()
-    G = [ctx.graph(*graphs.rgg(int(n), seed=3 + i)[:2]) for i, n in enumerate(ns)]
+    with ProcessPoolExecutor(8, mp_context=mp.get_context("spawn")) as ex:
+        host = list(ex.map(_rgg_job, [(int(n), 3 + i) for i, n in enumerate(ns)], chunksize=8))
+    G = [ctx.graph(rp, col) for rp, col, _ in host]
     log(f"512 RGGs: {sum(g.n for g in G)} vertices, {sum(g.m for g in G)} entries, "
         f"generated in {time.perf_counter() - t:.1f} s")
     seeds, states = [1] * len(G), [1] * len(G)
@@ -222,19 +277,92 @@ def rgg_batch(mi, ctx, reps: int = 2) -> dict:
             "cuts_equal_one_call_per_graph": same}
 
 
+def end_to_end(mi, ctx, rp, col, w, seed: int, reps: int = 3) -> dict:
+    """Host CSR in, host partition out: upload (mi_graph_create), the
+    bisection and the download of the partition, median of reps."""
+    ts, up = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = ctx.graph(rp, col, w)
+        t1 = time.perf_counter()
+        r = mi.partition_graph(ctx, g, hec_seed=seed, pcg_state=seed)
+        t2 = time.perf_counter()
+        g.close()
+        up.append((t1 - t0) * 1000.0)
+        ts.append((t2 - t0) * 1000.0)
+        del r
+    return {"ms": round(statistics.median(ts), 3), "upload_ms": round(statistics.median(up), 3),
+            "upload_gbps": round((4.0 * (len(rp) + len(col) + (len(w) if w is not None else 0)))
+                                 / (statistics.median(up) * 1e-3) / 1e9, 2)}
+
+
+def roofline_of(mi, ctx, g, seed: int, lvl: int, workload: str, name: str) -> dict:
+    if lvl == 0:
+        gl = g
+    else:
+        H = g.hierarchy(seed)
+        lv = H.level(lvl)
+        H.close()
+        gl = ctx.graph(lv["rowptr"], lv["col"], lv["w"])
+        del lv
+    b = gl.bench_step(50)
+    achieved = b["stream_bytes"] / (b["stream_ms"] * 1e-3) / 1e9
+    log(f"level {lvl} (n={gl.n}, m={gl.m}): step {b['step_ms'] * 1000:.1f} us, "
+        f"k_power_stream {b['stream_ms'] * 1000:.1f} us/launch, {b['stream_bytes'] / 1e6:.1f} MB algorithmic, "
+        f"{achieved:.0f} GB/s")
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(workload, lvl),
+           "kernel": ((f"k_power_stream_lds ({b['column_blocks']} LDS column blocks)" if b["column_blocks"]
+                       else "k_power_stream (gathers from global memory)")
+                      + f", FP64 power-iteration step, {name} level {lvl}"),
+           "level": lvl, "level_n": gl.n, "level_m": gl.m,
+           "alg_bytes_per_launch": b["stream_bytes"], "impl_bytes_per_launch": b["impl_bytes"],
+           "launch_us": round(b["stream_ms"] * 1000, 2),
+           "step_us": round(b["step_ms"] * 1000, 2),
+           "step_gbps": round(b["step_bytes"] / (b["step_ms"] * 1e-3) / 1e9, 1)}
+    if gl is not g:
+        gl.close()
+    return out
+
+
+def cpu_baseline(rp, col, w, seed: int, reps: int, budget_s: float) -> dict | None:
+    """The OpenMP restatement of the path (oracle/sgpar_omp.c): whole
+    bisections of the same graph on all host cores, median of the runs
+    (fewer than `reps` when one run exceeds budget_s / reps)."""
+    import oracle
+    if not oracle.omp_available():
+        log("cpu baseline: oracle/libsgpar_omp.so not built")
+        return None
+    threads = oracle.omp_threads()
+    ts = []
+    for i in range(reps):
+        t = time.perf_counter()
+        r = oracle.omp_partition_graph(rp, col, w, seed, seed)
+        ts.append((time.perf_counter() - t) * 1000.0)
+        log(f"cpu baseline run {i}: {ts[-1]:.0f} ms, cut {r['edge_cut']}, levels {list(r['levels_n'])}")
+        if sum(ts) / 1000.0 > budget_s:
+            break
+    info = oracle.host_cpu_info()
+    return {"value": round(statistics.median(ts), 1), "unit": "ms", "cores": threads, "kind": "port",
+            "runs": len(ts), "edge_cut": int(r["edge_cut"]),
+            "sample": (f"oracle/sgpar_omp.c (OpenMP restatement of the reference path: HEC coarsening, "
+                       f"P^T A P build, FP64 power iteration, sort-and-split) on the whole bisection of the "
+                       f"same graph, {threads} threads, median of {len(ts)} run(s); host: {info}")}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=20,
-                    help="R-MAT scale; from 22 on the graph is drawn on the GPU (graphs.rmat_device, seed 4)")
+    ap.add_argument("--scale", type=int, default=24,
+                    help="R-MAT scale of the main line: 24 (seed 4, drawn on the GPU, the default) or "
+                         "<= 21 (seed 1, drawn on the host)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-budget", type=float, default=60.0, help="seconds of CPU baseline at most")
+    ap.add_argument("--no-extras", action="store_true", help="no R-MAT-20 / 3D grid / batch extras")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched measurement")
-    ap.add_argument("--grid2d", type=int, default=0,
-                    help="bisect a k x k grid instead of the R-MAT graph (implies --no-extras)")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="only the plain FP64 R-MAT bisection and its roofline (no mixed, 3D grid or batch)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline measurement of the dominant kernel, plus a 256 MiB device copy "
                          "that calibrates rocprofv3's byte counters")
@@ -247,23 +375,18 @@ def main() -> int:
               "(independent graphs would shard one per GPU, with no collectives).", file=sys.stderr)
         return 2
 
-    import ctypes as C
     import ipdps_2021_graph_coarsening_amd as mi
     from ipdps_2021_graph_coarsening_amd import graphs
 
     t = time.perf_counter()
-    if args.grid2d:
-        # (a graph whose hierarchy is mostly small levels, for profiling those)
-        rp, col, w = graphs.grid2d(args.grid2d)
-        name = f"grid2d-{args.grid2d}"
-        args.no_extras = True
-    elif args.scale >= 22:
-        # (drawn on the GPU: the host generator takes minutes at these sizes)
-        rp, col, w = graphs.rmat_device(args.scale, 16, 0.57, 0.19, 0.19, seed=4)
-        name = f"R-MAT-{args.scale}"
+    if args.scale >= 22:
+        seed = 4
+        rp, col, w = graphs.rmat_device(args.scale, 16, 0.57, 0.19, 0.19, seed=seed)
     else:
-        rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=SEED)
-        name = f"R-MAT-{args.scale}"
+        seed = 1
+        rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=seed)
+    name = f"R-MAT-{args.scale}"
+    workload = f"rmat{args.scale}"
     n, m = len(rp) - 1, int(rp[-1])
     log(f"{name}: n={n} m={m} generated in {time.perf_counter() - t:.1f} s")
 
@@ -273,115 +396,92 @@ def main() -> int:
     g = ctx.graph(rp, col, w)  # host -> HBM upload of the CSR (synchronous)
     upload_ms = (time.perf_counter() - t) * 1000.0
     log(f"graph upload (host arrays -> HBM): {upload_ms:.2f} ms")
-    L = ctx._L
-    cfg = mi.Config(4, 0, 0, 1, 1e-10, 2)
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    hip.hipFree.argtypes = [ctypes.c_void_p]
-    hip.hipDeviceSynchronize.argtypes = []
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
 
-    ms = None
-    stats = None
-    if not args.roofline_only:
-        d_part = C.c_void_p()
-        assert hip.hipMalloc(ctypes.byref(d_part), 4 * n) == 0
-
-        def step():
-            st = mi.Stats()
-            pcg = C.c_uint64(SEED)
-            ec = C.c_uint32()
-            ctx.check(L.mi_partition_graph(ctx.h, g.h, C.byref(cfg), 0, SEED, C.byref(pcg), 1, d_part, 1,
-                                           C.byref(ec), C.byref(st)), "mi_partition_graph")
-            return st
-
-        def timed():
-            for i in range(args.warmup):
-                st = step()
-                log(f"warmup {i}: {st.total_ms:.2f} ms, cut {st.edge_cut}, levels {st.nlevels}")
-            hip.hipDeviceSynchronize()
-            t0 = time.perf_counter()
-            for i in range(args.steps):
-                st = step()
-            hip.hipDeviceSynchronize()
-            return (time.perf_counter() - t0) * 1000.0 / args.steps, st.as_dict()
-
-        ms, stats = timed()
-        log(f"timed {args.steps} steps: {ms:.2f} ms/bisection, cut {stats['edge_cut']}, "
-            f"iterations per level {stats['level_iters']}")
-        # the same bisection in mixed precision (FP32 on the levels above the
-        # finest with at least 2^20 entries, FP64 finest level)
-        mixed = None
-        if not args.no_extras:
-            ctx.set_precision(True)
-            ms_mixed, st_mixed = timed()
-            ctx.set_precision(False)
-            mixed = mixed_summary(ms, stats, ms_mixed, st_mixed)
-            if args.scale < 22:
-                mixed["per_level"] = mixed_levels_report(mi, ctx, g, SEED)
-            log(f"mixed precision: {ms_mixed:.2f} ms/bisection, cut {st_mixed['edge_cut']}, "
-                f"iterations per level {st_mixed['level_iters']}")
-        hip.hipFree(d_part)
-
-    # roofline of the dominant kernel: the streaming kernel of the power step
-    # on the level with the most eigensolve time
-    lvl = args.roofline_level
-    if lvl < 0:
-        lvl = 1 if stats is None else int(np.argmax(stats["level_eigen_ms"]))
-    if lvl == 0:
-        gl = g
-    else:
-        H = g.hierarchy(SEED)
-        lv = H.level(lvl)
-        H.close()
-        gl = ctx.graph(lv["rowptr"], lv["col"], lv["w"])
-    b = gl.bench_step(50)
-    achieved = b["stream_bytes"] / (b["stream_ms"] * 1e-3) / 1e9
-    log(f"level {lvl} (n={gl.n}, m={gl.m}): step {b['step_ms'] * 1000:.1f} us, "
-        f"k_power_stream {b['stream_ms'] * 1000:.1f} us/launch, {b['stream_bytes'] / 1e6:.1f} MB algorithmic, "
-        f"{achieved:.0f} GB/s")
-    traffic = measured_traffic() if (lvl == 1 and args.scale == 20 and not args.grid2d) else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": ((f"k_power_stream_lds ({b['column_blocks']} LDS column blocks)" if b["column_blocks"]
-                            else "k_power_stream (gathers from global memory)")
-                           + f", FP64 power-iteration step, {name} level {lvl}"),
-                "level_n": gl.n, "level_m": gl.m,
-                "alg_bytes_per_launch": b["stream_bytes"], "impl_bytes_per_launch": b["impl_bytes"],
-                "launch_us": round(b["stream_ms"] * 1000, 2),
-                "step_us": round(b["step_ms"] * 1000, 2),
-                "step_gbps": round(b["step_bytes"] / (b["step_ms"] * 1e-3) / 1e9, 1)}
     if args.roofline_only:
+        lvl = 1 if args.roofline_level < 0 else args.roofline_level
+        roof = roofline_of(mi, ctx, g, seed, lvl, workload, name)
         # calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE: a device copy of known size
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipFree.argtypes = [C.c_void_p]
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         nb = 256 << 20
-        a_, b_ = ctypes.c_void_p(), ctypes.c_void_p()
-        assert hip.hipMalloc(ctypes.byref(a_), nb) == 0 and hip.hipMalloc(ctypes.byref(b_), nb) == 0
+        a_, b_ = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a_), nb) == 0 and hip.hipMalloc(C.byref(b_), nb) == 0
         for _ in range(3):
             hip.hipMemcpy(b_, a_, nb, 3)
         hip.hipDeviceSynchronize()
         hip.hipFree(a_)
         hip.hipFree(b_)
-        print(json.dumps({"roofline": roofline, "calibration_copy_bytes": nb}), flush=True)
+        print(json.dumps({"roofline": roof, "calibration_copy_bytes": nb}), flush=True)
+        g.close()
+        ctx.close()
         return 0
 
-    g3 = None
-    if not args.roofline_only and not args.no_extras:
-        t = time.perf_counter()
-        g3 = grid3d_mixed(mi, ctx, max(1, min(args.steps, 5)))
-        log(f"grid3d-128: FP64 {g3['fp64_ms']:.2f} ms, mixed {g3['ms']:.2f} ms, cut change "
-            f"{g3['cut_change_pct']}% ({time.perf_counter() - t:.1f} s)")
+    R = Runner(mi, ctx, g, seed)
+    ms, st = R.timed(R.bisect, args.steps, args.warmup, f"{name} fp64")
+    stats = st.as_dict()
+    log(f"timed {args.steps} FP64 bisections: {ms:.2f} ms, cut {stats['edge_cut']}, "
+        f"levels {stats['level_n']}, iterations {stats['level_iters']}")
+    ctx.set_precision(True)
+    ms32, st32 = R.timed(R.bisect, args.steps, args.warmup, f"{name} mixed")
+    ctx.set_precision(False)
+    mixed = mixed_summary(ms, stats, ms32, st32.as_dict())
+    log(f"mixed precision: {ms32:.2f} ms, cut {st32.edge_cut}, iterations {list(st32.level_iters[:st32.nlevels])}")
+    steps4 = max(1, min(args.steps, 5))
+    ms4, (cut4, cuts4) = R.timed(R.four_way, steps4, 1, f"{name} 4-way fp64")
+    ctx.set_precision(True)
+    ms4m, (cut4m, cuts4m) = R.timed(R.four_way, steps4, 1, f"{name} 4-way mixed")
+    ctx.set_precision(False)
+    four = {"fp64_ms": round(ms4, 3), "fp64_edge_cut": cut4, "fp64_cuts": cuts4,
+            "mixed_ms": round(ms4m, 3), "mixed_edge_cut": cut4m, "mixed_cuts": cuts4m,
+            "speedup": round(ms4 / ms4m, 4),
+            "cut_change_pct": round(100.0 * (cut4m - cut4) / max(cut4, 1), 4)}
+    log(f"4-way: FP64 {ms4:.2f} ms cut {cut4}, mixed {ms4m:.2f} ms cut {cut4m}")
+    R.close()
 
-    batch = None
-    if not args.roofline_only and not args.no_batch and not args.no_extras:
-        batch = rgg_batch(mi, ctx)
-        log(f"512 RGGs: batch {batch['batch_ms']} ms, one call per graph {batch['one_call_per_graph_ms']} ms")
+    # finest-level Fiedler residual of both precisions' final vectors
+    H = g.hierarchy(seed)
+    ev64, _, _ = H.fiedler(seed)
+    ctx.set_precision(True)
+    ev32, _, _ = H.fiedler(seed)
+    ctx.set_precision(False)
+    H.close()
+    q64 = fiedler_quality(rp, col, w, ev64)
+    q32 = fiedler_quality(rp, col, w, ev32)
+    mixed["finest"] = {"cos": round(float(abs(ev64 @ ev32) / (np.linalg.norm(ev64) * np.linalg.norm(ev32))), 12),
+                       "rayleigh_fp64": q64["rayleigh"], "rayleigh_mixed": q32["rayleigh"],
+                       "residual_fp64": round(q64["residual"], 6), "residual_mixed": round(q32["residual"], 6)}
+    del ev64, ev32
+
+    e2e = end_to_end(mi, ctx, rp, col, w, seed)
+    log(f"end to end (host CSR -> host partition): {e2e['ms']:.1f} ms, upload {e2e['upload_ms']:.1f} ms")
+
+    lvl = args.roofline_level if args.roofline_level >= 0 else int(np.argmax(stats["level_eigen_ms"]))
+    roof = roofline_of(mi, ctx, g, seed, lvl, workload, name)
+
+    extras = None
+    if not args.no_extras:
+        extras = {}
+        if args.scale != 20:
+            extras["rmat20"] = extra_rmat20(mi, ctx, max(1, min(args.steps, 5)))
+            log(f"R-MAT-20: FP64 {extras['rmat20']['fp64_ms']:.2f} ms, mixed {extras['rmat20']['ms']:.2f} ms")
+        extras["grid3d"] = extra_grid3d(mi, ctx, max(1, min(args.steps, 5)))
+        log(f"grid3d-128: FP64 {extras['grid3d']['fp64_ms']:.2f} ms, mixed {extras['grid3d']['ms']:.2f} ms")
+        if not args.no_batch:
+            extras["batched_rgg512"] = extra_rgg_batch(mi, ctx)
+            b = extras["batched_rgg512"]
+            log(f"512 RGGs: batch {b['batch_ms']} ms, one call per graph {b['one_call_per_graph_ms']} ms")
+
+    g.close()
+    ctx.close()
 
     cpu = None
-    if not args.no_cpu_baseline and args.scale < 22 and not args.grid2d:  # (the oracle would take hours beyond)
+    if not args.no_cpu_baseline:
         t = time.perf_counter()
-        cpu = cpu_baseline(rp, col, w, stats)
-        log(f"cpu baseline {cpu['value']} ms (sampled in {time.perf_counter() - t:.1f} s)")
+        cpu = cpu_baseline(rp, col, w, seed, args.cpu_reps, args.cpu_budget)
+        if cpu:
+            log(f"cpu baseline {cpu['value']} ms ({time.perf_counter() - t:.1f} s)")
 
     out = {
         "metric": "bisection ms + SpMV HBM GB/s: FP32-mixed/batched vs plain FP64 at equal cut",
@@ -396,11 +496,12 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": (f"grid2d-{args.grid2d} unit-weight plain-fp64 bisection" if args.grid2d else
-                                f"rmat{args.scale}-ef16-lcc-weighted plain-fp64 bisection"),
-                   "n": n, "m": m, "seed": SEED, "levels": stats["nlevels"],
+        "config": {"workload": f"{workload}-ef16-lcc-weighted plain-fp64 bisection",
+                   "n": n, "m": m, "seed": seed, "levels": stats["nlevels"],
                    "upload_ms": round(upload_ms, 3),
-                   "level_n": stats["level_n"], "level_iters": stats["level_iters"],
+                   "end_to_end": e2e,
+                   "level_n": stats["level_n"], "level_m": stats["level_m"],
+                   "level_iters": stats["level_iters"],
                    "edge_cut": stats["edge_cut"],
                    "phases_ms": {k: round(stats[k], 3) for k in
                                  ("coarsen_ms", "eigen_ms", "partition_ms", "permute_ms", "heavy_ms",
@@ -408,16 +509,14 @@ def main() -> int:
                    "level_eigen_ms": [round(x, 3) for x in stats["level_eigen_ms"]],
                    "level_us_per_iteration": [round(1000.0 * t / max(i, 1), 2) for t, i in
                                               zip(stats["level_eigen_ms"], stats["level_iters"])],
-                   "merge": merge_rate(stats)},
-        "roofline": roofline,
+                   "merge": merge_rate(dict(stats, weighted=True))},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
-        "mixed_grid3d": g3,
-        "batched_rgg512": batch,
+        "four_way": four,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
-    g.close()
-    ctx.close()
     return 0
 
 

@@ -313,19 +313,22 @@ void fill_level_stats(const std::vector<DLevel>& lv, mi_stats* st)
     }
 }
 
+// One PCG32 (XSH-RR) draw, definitions_kokkos.h:46-55.
+uint32_t pcg32_next(uint64_t* pcg_state, uint64_t pcg_inc)
+{
+    uint64_t old = *pcg_state;
+    *pcg_state = old * 6364136223846793005ULL + (pcg_inc | 1u);
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
 // The coarsest level's start vector, normalised.
 DBuf<double> start_vector(Ctx& c, uint32_t gn, uint64_t* pcg_state, uint64_t pcg_inc)
 {
     std::vector<double> cg(gn);
     // eigensolve_kokkos.h:1500-1505: pcg32()/UINT32_MAX, then normalise
-    for (uint32_t i = 0; i < gn; i++) {
-        uint64_t old = *pcg_state;
-        *pcg_state = old * 6364136223846793005ULL + (pcg_inc | 1u);
-        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-        uint32_t rot = (uint32_t)(old >> 59u);
-        uint32_t r = (xs >> rot) | (xs << ((-rot) & 31u));
-        cg[i] = ((double)r) / UINT32_MAX;
-    }
+    for (uint32_t i = 0; i < gn; i++) cg[i] = ((double)pcg32_next(pcg_state, pcg_inc)) / UINT32_MAX;
     DBuf<double> cur(&c.pool, gn);
     h2d(c, cur.get(), cg.data(), (size_t)gn * 8);
     normalize(c, cur.get(), gn);
@@ -977,9 +980,14 @@ int mi_partition_4way(mi_ctx* ctx, const mi_graph* g, const mi_config* config, i
             dp.push_back(sub[s].get());
             which.push_back(s);
         }
+        // a side that is not split still takes its start vector's draws
+        // (one per vertex), as a bisection of it would, so the state that
+        // reaches side 1 and the caller is the one of two calls in a row
+        if (side[0].n == 1) (void)pcg32_next(pcg_state, pcg_inc);
         const uint64_t seeds[2] = {hec_seed, hec_seed};
         uint32_t sc[2] = {0, 0};
         bisect_many(c, gs, config, perc_imbalance, seeds, pcg_state, true, pcg_inc, dp, sc, nullptr, nullptr);
+        if (side[1].n == 1) (void)pcg32_next(pcg_state, pcg_inc);
         for (size_t k = 0; k < which.size(); k++) cuts[1 + which[k]] = sc[k];
         *edge_cut_4way = cuts[0] + cuts[1] + cuts[2];
         DBuf<uint32_t> out;

@@ -121,24 +121,32 @@ def from_edges(n: int, u, v, w=None, lcc: bool = True):
     rows, keep the largest connected component (order-preserving relabel)."""
     u = np.asarray(u, dtype=np.int64)
     v = np.asarray(v, dtype=np.int64)
-    w = np.ones(len(u), dtype=np.int64) if w is None else np.asarray(w, dtype=np.int64)
     keep = u != v
-    u, v, w = u[keep], v[keep], w[keep]
-    src = np.concatenate([u, v])
-    dst = np.concatenate([v, u])
-    ww = np.concatenate([w, w])
-    key = src * n + dst
-    order = np.argsort(key, kind="stable")
-    key = key[order]
-    ww = ww[order]
-    del order, src, dst
+    u, v = u[keep], v[keep]
+    key = np.concatenate([u * n + v, v * n + u])
+    del u, v
+    # integer sums do not depend on the order of equal keys, so no stable
+    # sort is needed; unit weights are the run lengths
+    if w is None:
+        key.sort()
+        ww = None
+    else:
+        w = np.asarray(w, dtype=np.int64)[keep]
+        order = np.argsort(key)
+        key = key[order]
+        ww = np.concatenate([w, w])[order]
+        del order
     heads = np.empty(len(key), dtype=bool)
     if len(key):
         heads[0] = True
         np.not_equal(key[1:], key[:-1], out=heads[1:])
-    gid = np.cumsum(heads) - 1
-    wsum = np.bincount(gid, weights=ww, minlength=int(heads.sum())).astype(np.int64)
-    ukey = key[heads]
+    start = np.flatnonzero(heads)
+    if ww is None:
+        wsum = np.diff(np.append(start, len(key))).astype(np.int64)
+    else:
+        wsum = np.add.reduceat(ww, start) if len(start) else np.zeros(0, np.int64)
+    ukey = key[start]
+    del key, heads
     rows = ukey // n
     cols = ukey % n
     if np.any(wsum > np.iinfo(np.uint32).max):
@@ -195,24 +203,24 @@ def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c:
          seed: int = 1, lcc: bool = True):
     """R-MAT (Chakrabarti et al.) with 2^scale vertices and edge_factor*2^scale
     edges of weight 1; duplicates merge with summed weights."""
+    if scale > 32:
+        raise ValueError("scale above 32 does not fit 32-bit vertex ids")
     rng = np.random.default_rng(seed)
     m = edge_factor << scale
-    u = np.zeros(m, dtype=np.int64)
-    v = np.zeros(m, dtype=np.int64)
+    u = np.zeros(m, dtype=np.uint32)
+    v = np.zeros(m, dtype=np.uint32)
     ab, abc = a + b, a + b + c
     chunk = 1 << 22
     for s in range(0, m, chunk):
         e = min(m, s + chunk)
-        uu = np.zeros(e - s, dtype=np.int64)
-        vv = np.zeros(e - s, dtype=np.int64)
+        uu = u[s:e]
+        vv = v[s:e]
         for bit in range(scale):
             r = rng.random(e - s)
-            ub = r >= ab
-            vb = ((r >= a) & (r < ab)) | (r >= abc)
-            uu |= ub.astype(np.int64) << bit
-            vv |= vb.astype(np.int64) << bit
-        u[s:e] = uu
-        v[s:e] = vv
+            uu |= np.left_shift((r >= ab).view(np.uint8), bit, dtype=np.uint32)
+            vb = (r >= a) & (r < ab)
+            vb |= r >= abc
+            vv |= np.left_shift(vb.view(np.uint8), bit, dtype=np.uint32)
     return from_edges(1 << scale, u, v, None, lcc=lcc)
 
 
@@ -243,18 +251,49 @@ def rmat_device(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0
     key, cnt = torch.unique(key, sorted=True, return_counts=True)
     if int(cnt.max()) > np.iinfo(np.uint32).max:
         raise ValueError("merged weight exceeds uint32")
-    rows = (key // n).to(torch.int64).cpu().numpy()
-    cols = (key % n).to(torch.int64).cpu().numpy()
-    w = cnt.to(torch.int64).cpu().numpy()
-    del key, cnt
+    rows = key // n
+    cols = key % n
+    del key
+    if lcc:
+        rows, cols, cnt, n = _largest_component_device(n, rows, cols, cnt)
+    rowptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    rowptr[1:] = torch.cumsum(torch.bincount(rows, minlength=n), 0)
+    del rows
+    out = (rowptr.to(torch.int32).cpu().numpy().view(np.uint32),
+           cols.to(torch.int32).cpu().numpy().view(np.uint32),
+           cnt.to(torch.int32).cpu().numpy().view(np.uint32))
+    del rowptr, cols, cnt
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
-    if lcc:
-        rows, cols, w, n = _largest_component(n, rows, cols, w)
-    cntr = np.bincount(rows, minlength=n)
-    rowptr = np.zeros(n + 1, dtype=np.uint32)
-    np.cumsum(cntr, out=rowptr[1:])
-    return rowptr, cols.astype(np.uint32), w.astype(np.uint32)
+    return out
+
+
+def _largest_component_device(n, rows, cols, w):
+    """Largest connected component of a symmetric edge list on the GPU
+    (torch tensors, rows sorted): every vertex takes the smallest label among
+    its neighbours and the label of its label, until nothing changes (the
+    label is then the smallest vertex id of the component); the component
+    with the most vertices wins, ties to the smallest id, which is the one
+    `_largest_component` picks (scipy numbers components in order of their
+    smallest vertex).  Vertices keep their order when relabelled."""
+    import torch
+
+    lab = torch.arange(n, dtype=torch.int64, device=rows.device)
+    while True:
+        new = lab.scatter_reduce(0, rows, lab[cols], reduce="amin", include_self=True)
+        new = new[new]
+        if torch.equal(new, lab):
+            break
+        lab = new
+    sizes = torch.bincount(lab, minlength=n)
+    big = int(torch.argmax(sizes))
+    inside = lab == big
+    del lab, sizes
+    if bool(inside.all()):
+        return rows, cols, w, n
+    newid = torch.cumsum(inside.to(torch.int64), 0) - 1
+    e = inside[rows]
+    return newid[rows[e]], newid[cols[e]], w[e], int(inside.sum())
 
 
 def rgg(n: int, radius: float | None = None, seed: int = 3, lcc: bool = True):

@@ -80,6 +80,9 @@ class _Core:
         L.orc_eigensolve.restype = C.c_int
         L.orc_eigensolve.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.c_int,
                                      C.c_double, C.c_uint64, f64p, u64p]
+        L.orc_eigensolve_trace.restype = C.c_int
+        L.orc_eigensolve_trace.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.c_int,
+                                           C.c_double, C.c_uint64, f64p, u64p, C.c_void_p, C.c_void_p]
         L.orc_eigenvalue_log.restype = None
         L.orc_eigenvalue_log.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, f64p, f64p]
         L.orc_compute_partition.restype = C.c_int
@@ -239,6 +242,23 @@ class Hierarchy:
         self._L.orc_eigensolve(self._h, C.byref(st), pcg_inc, normLap, tol, iter_cap, ev, iters)
         return ev, iters, st.value
 
+    def eigensolve_trace(self, pcg_state: int, levels, pcg_inc: int = 1, normLap: int = 0,
+                         tol: float = 1e-10):
+        """eigensolve that also returns, for each level l in `levels`, the
+        vector its power iteration starts from and the one it returns:
+        (evec, iters, new pcg state, {l: (start, result)})."""
+        st = C.c_uint64(pcg_state)
+        ev = np.empty(self.levels[0]["n"], dtype=np.float64)
+        iters = np.zeros(self.nlevels, dtype=np.uint64)
+        trace = {l: (np.empty(self.levels[l]["n"]), np.empty(self.levels[l]["n"])) for l in levels}
+        sp = (C.c_void_p * self.nlevels)(*[trace[l][0].ctypes.data if l in trace else None
+                                          for l in range(self.nlevels)])
+        rp = (C.c_void_p * self.nlevels)(*[trace[l][1].ctypes.data if l in trace else None
+                                          for l in range(self.nlevels)])
+        self._L.orc_eigensolve_trace(self._h, C.byref(st), pcg_inc, normLap, tol, 0, ev, iters,
+                                     C.cast(sp, C.c_void_p), C.cast(rp, C.c_void_p))
+        return ev, iters, st.value, trace
+
     def __del__(self):
         try:
             self._L.orc_hier_free(self._h)
@@ -386,3 +406,66 @@ def ref():
             return None
         _ref = _Ref()
     return _ref
+
+
+# ---------------------------------------------------------------- OpenMP form (CPU baseline)
+OMP_SO = os.path.join(HERE, "libsgpar_omp.so")
+_omp = None
+
+
+def omp_available() -> bool:
+    return os.path.exists(OMP_SO)
+
+
+def _omp_lib():
+    global _omp
+    if _omp is None:
+        L = C.CDLL(OMP_SO)
+        L.omp_threads.restype = C.c_int
+        L.omp_threads.argtypes = []
+        L.omp_partition_graph.restype = C.c_int
+        L.omp_partition_graph.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                          C.c_uint64, C.c_double, u32p, C.POINTER(C.c_uint32), u32p, u64p]
+        _omp = L
+    return _omp
+
+
+def omp_threads() -> int:
+    return int(_omp_lib().omp_threads())
+
+
+def omp_partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1, tol: float = 1e-10):
+    """Whole bisection by the OpenMP restatement (sgpar_omp.c), on all
+    threads OpenMP gives; dict(part, edge_cut, levels_n, iters, pcg_state)."""
+    L = _omp_lib()
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    st = C.c_uint64(pcg_state)
+    part = np.empty(n, dtype=np.uint32)
+    ec = C.c_uint32(0)
+    lv = np.zeros(210, dtype=np.uint32)
+    it = np.zeros(210, dtype=np.uint64)
+    nl = L.omp_partition_graph(n, rowptr, col, wp, hec_seed, C.byref(st), pcg_inc, tol, part, C.byref(ec), lv, it)
+    return dict(part=part, edge_cut=ec.value, levels_n=lv[:nl].copy(), iters=it[:nl].copy(), pcg_state=st.value)
+
+
+def host_cpu_info() -> str:
+    """CPU model, sockets, physical cores and logical CPUs of this host (lscpu)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:
+        return "lscpu unavailable"
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        cores = int(kv.get("Core(s) per socket", "0")) * int(kv.get("Socket(s)", "0"))
+    except ValueError:
+        cores = 0
+    return (f"{kv.get('Model name', '?')}, {kv.get('Socket(s)', '?')} socket(s), {cores} physical cores, "
+            f"{kv.get('CPU(s)', '?')} logical CPUs, {os.cpu_count()} visible to this process")

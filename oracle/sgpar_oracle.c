@@ -598,8 +598,24 @@ void orc_eigenvalue_log(vid_t n, const eid_t *rowptr, const vid_t *col, const wg
 /* (:1556-1558), final solve at the finest level.  iters[l] receives   */
 /* the count of level l (0 = finest).                                  */
 /* ------------------------------------------------------------------ */
+int orc_eigensolve_trace(const orc_hier_t *h, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                         double tol, uint64_t iter_cap, double *evec_out, uint64_t *iters,
+                         double **starts, double **results);
+
 int orc_eigensolve(const orc_hier_t *h, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
                    double tol, uint64_t iter_cap, double *evec_out, uint64_t *iters)
+{
+    return orc_eigensolve_trace(h, pcg_state, pcg_inc, normLap, tol, iter_cap, evec_out, iters,
+                                NULL, NULL);
+}
+
+/* orc_eigensolve that also copies out, for every level l with
+   starts[l] / results[l] non-NULL (arrays of nlevels pointers, or NULL),
+   the vector the level's power iteration starts from and the one it
+   returns (level l's n doubles each). */
+int orc_eigensolve_trace(const orc_hier_t *h, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                         double tol, uint64_t iter_cap, double *evec_out, uint64_t *iters,
+                         double **starts, double **results)
 {
     int L = h->nlevels;
     vid_t gn = h->lv[L - 1].n;
@@ -608,8 +624,10 @@ int orc_eigensolve(const orc_hier_t *h, uint64_t *pcg_state, uint64_t pcg_inc, i
     normalize_seq(cur, gn);
     for (int l = L - 1; l >= 0; l--) {
         const orc_level_t *a = &h->lv[l];
+        if (starts && starts[l]) memcpy(starts[l], cur, (size_t)a->n * sizeof(double));
         uint64_t it = orc_power_iter(a->n, a->rowptr, a->col, a->w, cur, normLap, tol,
                                      iter_cap, NULL, NULL);
+        if (results && results[l]) memcpy(results[l], cur, (size_t)a->n * sizeof(double));
         if (iters) iters[l] = it;
         if (l > 0) {
             const orc_level_t *f = &h->lv[l - 1];

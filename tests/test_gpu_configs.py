@@ -1,0 +1,309 @@
+"""The five benchmark workloads through the HIP path, at their full sizes,
+against the oracle.
+
+- R-MAT-20 (ef 16, seed 1, LCC, merged weights): every level of the
+  hierarchy bit-exact; the FP64 bisection's per-level iteration counts
+  within 10 %, its Fiedler vector within |cos| >= 1 - 1e-6 of the oracle's
+  and its cut within 1 % of the cut of the oracle's vector; the
+  mixed-precision bisection at the default FP32 floor level by level
+  against the oracle and its cut within 1 % of FP64.
+- 3D grid 128^3 with weights U{1..16} (seed 2): the hierarchy bit-exact;
+  every FP32 level of the mixed-precision solve from the oracle's own start
+  vector of that level against the oracle's result; the mixed bisection's
+  cut within 1 % of the FP64 one and of the oracle's.
+- 512 random geometric graphs (seed 3): the batched bisection equal to one
+  call per graph bit for bit; the hierarchy, the iteration counts and the
+  cut of eight graphs across the size range against the oracle.
+- R-MAT-24 (ef 16, seed 4, drawn on the GPU): the 4-way partition equal to
+  the same steps composed call by call, its cut equal to the edges between
+  quarters, the quarters balanced; the level-1 HEC map (permutation,
+  heavy neighbours, cluster map) bit-exact against the oracle's.
+- The 256^2 grid is covered against the recorded reference run in
+  test_gpu_parity.py.
+
+Tolerances (FP64: sums in another order than the reference's sequential
+loops, and |d(dot)| <= 1e-10 decides the stop): iteration counts within
+10 % (at least 2), |cos| >= 1 - 1e-6, cut within 1 %.  FP32 levels (the
+mixed path keeps vectors and products in FP32, sums and the stop test in
+FP64): iteration counts within 10 %, |cos| >= 1 - 1e-4, Rayleigh quotient
+within 2 % of the oracle's from the same start vector; the whole mixed
+bisection's cut within 1 % of the FP64 one."""
+from concurrent.futures import ProcessPoolExecutor
+import multiprocessing as mp
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+import ipdps_2021_graph_coarsening_amd as mi
+from ipdps_2021_graph_coarsening_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+EV_COS = 1 - 1e-6
+ITER_SLACK = 0.10
+CUT_REL = 0.01
+MIX_COS = 1 - 1e-4
+MIX_RQ = 0.02
+MIXED_FLOOR = 1 << 20  # default: FP32 on levels above the finest with >= 2^20 entries
+
+
+def _cos(a, b):
+    return abs(a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
+
+
+def _rayleigh(rp, col, w, u):
+    n = len(rp) - 1
+    A = sp.csr_matrix((w.astype(np.float64), col, rp), shape=(n, n))
+    L = sp.diags(np.asarray(A.sum(axis=1)).ravel()) - A
+    u = u - u.mean()
+    return u @ (L @ u) / (u @ u)
+
+
+def _cut(rp, col, part):
+    src = np.repeat(np.arange(len(rp) - 1, dtype=np.uint32), np.diff(rp))
+    return int(np.count_nonzero(part[src] != part[col])) // 2
+
+
+def _hierarchy_equal(H, O):
+    assert H.nlevels == O.nlevels
+    for l in range(H.nlevels):
+        a, b = H.level(l), O.levels[l]
+        assert a["n"] == b["n"] and a["m"] == b["m"], l
+        assert np.array_equal(a["rowptr"], b["rowptr"]) and np.array_equal(a["col"], b["col"]), l
+        if l > 0:
+            assert np.array_equal(a["w"], b["w"]) and np.array_equal(a["vtx_w"], b["vtx_w"]), l
+        if l + 1 < H.nlevels:
+            assert np.array_equal(a["vcmap"], b["vcmap"]), l
+
+
+def _iters_close(a, b):
+    for x, y in zip(a, b):
+        assert abs(int(x) - int(y)) <= max(2, ITER_SLACK * int(y)), (list(a), list(b))
+
+
+def _fp32_levels(O):
+    return [l for l in range(1, O.nlevels) if O.levels[l]["m"] >= MIXED_FLOOR]
+
+
+def _mixed_levels_vs_oracle(ctx, O, fine_w, trace, oiters):
+    """Each FP32 level solved by the mixed path from the oracle's start
+    vector of that level, against the oracle's result and iteration count
+    of that level."""
+    ctx.set_precision(True, 1)
+    try:
+        for l, (start, res) in trace.items():
+            lv = O.levels[l]
+            w = lv["w"] if l > 0 else fine_w
+            g = ctx.graph(lv["rowptr"], lv["col"], w)
+            u, it, mr = g.power_iter(start, tol=1e-10)
+            g.close()
+            assert not mr, l
+            assert abs(it - int(oiters[l])) <= max(3, ITER_SLACK * int(oiters[l])), (l, it, oiters[l])
+            assert _cos(u, res) >= MIX_COS, l
+            lo = _rayleigh(lv["rowptr"], lv["col"], w, res)
+            lm = _rayleigh(lv["rowptr"], lv["col"], w, u)
+            assert abs(lm - lo) <= MIX_RQ * lo, (l, lm, lo)
+    finally:
+        ctx.set_precision(False)
+
+
+# ------------------------------------------------------------ R-MAT-20
+@pytest.fixture(scope="module")
+def rmat20():
+    rp, col, w = graphs.rmat(20, 16, 0.57, 0.19, 0.19, seed=1)
+    O = oracle.Hierarchy(rp, col, w, 1)
+    fp32 = _fp32_levels(O)
+    ev, it, st, trace = O.eigensolve_trace(1, fp32)
+    return dict(rp=rp, col=col, w=w, O=O, ev=ev, iters=it, pcg=st, trace=trace, fp32=fp32)
+
+
+def test_rmat20_hierarchy_bitexact(ctx, rmat20):
+    R = rmat20
+    g = ctx.graph(R["rp"], R["col"], R["w"])
+    H = g.hierarchy(1)
+    assert H.level(0)["n"] == len(R["rp"]) - 1 >= 600000
+    _hierarchy_equal(H, R["O"])
+    H.close()
+    g.close()
+
+
+def test_rmat20_bisection_vs_oracle(ctx, rmat20):
+    R = rmat20
+    rp, col, w = R["rp"], R["col"], R["w"]
+    g = ctx.graph(rp, col, w)
+    r = mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1)
+    H = g.hierarchy(1)
+    ev, st, pcg = H.fiedler(1)
+    H.close()
+    _iters_close(r.stats["level_iters"], R["iters"])
+    assert r.stats["level_iters"] == st["level_iters"]
+    assert pcg == R["pcg"] == r.pcg_state
+    assert _cos(ev, R["ev"]) >= EV_COS
+    _, ocut = oracle.compute_partition(rp, col, R["ev"])
+    assert abs(r.edge_cut - ocut) <= CUT_REL * ocut
+    assert r.edge_cut == _cut(rp, col, r.part)
+    assert np.count_nonzero(r.part) == (len(rp) - 1) - (len(rp) - 1) // 2
+    g.close()
+
+
+def test_rmat20_mixed_default_floor(ctx, rmat20):
+    """Mixed precision at the default floor (2^20 entries): the FP32 levels
+    against the oracle from the oracle's start vectors, and the whole
+    bisection's cut within 1 % of the FP64 one."""
+    R = rmat20
+    assert R["fp32"], "R-MAT-20 has levels above the finest with >= 2^20 entries"
+    _mixed_levels_vs_oracle(ctx, R["O"], R["w"], R["trace"], R["iters"])
+    g = ctx.graph(R["rp"], R["col"], R["w"])
+    r64 = mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1)
+    ctx.set_precision(True)
+    try:
+        r32 = mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1)
+    finally:
+        ctx.set_precision(False)
+    assert r32.stats["mixed_levels"] == len(R["fp32"])
+    assert r32.stats["level_n"] == r64.stats["level_n"]
+    assert abs(r32.edge_cut - r64.edge_cut) <= CUT_REL * r64.edge_cut
+    assert r32.edge_cut == _cut(R["rp"], R["col"], r32.part)
+    g.close()
+
+
+# ------------------------------------------------------------ 3D grid 128^3
+@pytest.fixture(scope="module")
+def grid3d128():
+    rp, col, w = graphs.grid3d(128, seed=2)
+    O = oracle.Hierarchy(rp, col, w, 2)
+    fp32 = _fp32_levels(O)
+    ev, it, st, trace = O.eigensolve_trace(2, fp32)
+    return dict(rp=rp, col=col, w=w, O=O, ev=ev, iters=it, pcg=st, trace=trace, fp32=fp32)
+
+
+def test_grid3d128_hierarchy_bitexact(ctx, grid3d128):
+    G = grid3d128
+    g = ctx.graph(G["rp"], G["col"], G["w"])
+    H = g.hierarchy(2)
+    assert H.level(0)["n"] == 128 ** 3
+    _hierarchy_equal(H, G["O"])
+    H.close()
+    g.close()
+
+
+def test_grid3d128_mixed_levels_vs_oracle(ctx, grid3d128):
+    G = grid3d128
+    assert len(G["fp32"]) >= 2
+    _mixed_levels_vs_oracle(ctx, G["O"], G["w"], G["trace"], G["iters"])
+
+
+def test_grid3d128_mixed_bisection(ctx, grid3d128):
+    G = grid3d128
+    rp, col, w = G["rp"], G["col"], G["w"]
+    g = ctx.graph(rp, col, w)
+    r64 = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
+    H = g.hierarchy(2)
+    ev64, _, _ = H.fiedler(2)
+    H.close()
+    ctx.set_precision(True)
+    try:
+        r32 = mi.partition_graph(ctx, g, hec_seed=2, pcg_state=2)
+    finally:
+        ctx.set_precision(False)
+    # FP64 against the oracle
+    _iters_close(r64.stats["level_iters"], G["iters"])
+    assert _cos(ev64, G["ev"]) >= EV_COS
+    _, ocut = oracle.compute_partition(rp, col, G["ev"])
+    assert abs(r64.edge_cut - ocut) <= CUT_REL * ocut
+    # mixed against FP64
+    assert r32.stats["mixed_levels"] == len(G["fp32"])
+    assert abs(r32.edge_cut - r64.edge_cut) <= CUT_REL * r64.edge_cut
+    assert abs(r32.edge_cut - ocut) <= CUT_REL * ocut
+    assert r32.edge_cut == _cut(rp, col, r32.part)
+    assert np.count_nonzero(r32.part == 0) == np.count_nonzero(r64.part == 0)
+    g.close()
+
+
+# ------------------------------------------------------------ 512 RGGs
+def _rgg_sizes():
+    rng = np.random.default_rng(3)
+    return np.exp(rng.uniform(np.log(1e3), np.log(2e5), 512)).astype(int)
+
+
+def _rgg_job(args):
+    n, seed = args
+    return graphs.rgg(int(n), seed=seed)
+
+
+@pytest.fixture(scope="module")
+def rgg512():
+    ns = _rgg_sizes()
+    jobs = [(int(n), 3 + i) for i, n in enumerate(ns)]
+    # (generation only; no GPU work in the workers)
+    with ProcessPoolExecutor(8, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_rgg_job, jobs, chunksize=8))
+
+
+def test_rgg512_batch_equals_single_calls_and_oracle(ctx, rgg512):
+    G = [ctx.graph(rp, col, w) for rp, col, w in rgg512]
+    assert len(G) == 512 and sum(g.n for g in G) > 15_000_000
+    res, ph = mi.bisect_batch(ctx, G, hec_seeds=[1] * 512, pcg_states=[1] * 512)
+    single = [mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1) for g in G]
+    for k, (a, b) in enumerate(zip(res, single)):
+        assert np.array_equal(a.part, b.part) and a.edge_cut == b.edge_cut, k
+        assert a.stats["level_iters"] == b.stats["level_iters"] and a.pcg_state == b.pcg_state, k
+    # eight graphs across the size range against the oracle
+    order = np.argsort([g.n for g in G])
+    for k in order[np.linspace(0, 511, 8).astype(int)]:
+        rp, col, w = rgg512[k]
+        o = oracle.partition_graph(rp, col, None, 1, 1)
+        st = res[k].stats
+        assert st["level_n"] == list(o["levels_n"]), k
+        _iters_close(st["level_iters"], o["iters"])
+        assert abs(res[k].edge_cut - o["edge_cut"]) <= max(2, CUT_REL * o["edge_cut"]), k
+        assert res[k].edge_cut == _cut(rp, col, res[k].part)
+    for g in G:
+        g.close()
+
+
+# ------------------------------------------------------------ R-MAT-24 4-way
+@pytest.fixture(scope="module")
+def rmat24():
+    return graphs.rmat_device(24, 16, 0.57, 0.19, 0.19, seed=4)
+
+
+def test_rmat24_4way_properties(ctx, rmat24):
+    rp, col, w = rmat24
+    n = len(rp) - 1
+    assert n > 8_000_000 and len(col) > 500_000_000
+    g = ctx.graph(rp, col, w)
+    r4 = mi.partition_4way(ctx, g, hec_seed=4, pcg_state=4)
+    # the same steps call by call: bisect, split on the device (tested
+    # bit-exact against the host sgp_use_partition), bisect each side with
+    # the continuing PCG state
+    top = mi.partition_graph(ctx, g, hec_seed=4, pcg_state=4)
+    g0, g1 = g.use_partition(top.part)
+    s0 = mi.partition_graph(ctx, g0, hec_seed=4, pcg_state=top.pcg_state)
+    s1 = mi.partition_graph(ctx, g1, hec_seed=4, pcg_state=s0.pcg_state)
+    assert r4.extra["cuts"] == [top.edge_cut, s0.edge_cut, s1.edge_cut]
+    assert r4.pcg_state == s1.pcg_state
+    side = top.part.astype(bool)
+    exp = np.empty(n, np.uint32)
+    exp[~side] = s0.part
+    exp[side] = 2 + s1.part
+    assert np.array_equal(r4.part, exp)
+    assert r4.edge_cut == sum(r4.extra["cuts"]) == _cut(rp, col, r4.part)
+    assert top.edge_cut == _cut(rp, col, top.part)
+    sizes = np.bincount(r4.part, minlength=4)
+    assert sizes.min() >= n // 4 - 2 and sizes.max() <= n // 4 + 2
+    for h in (g0, g1, g):
+        h.close()
+
+
+def test_rmat24_level1_hec_bitexact(ctx, rmat24):
+    rp, col, w = rmat24
+    g = ctx.graph(rp, col, w)
+    vc, nc, vperm, hn = g.hec_level(1, 4)
+    g.close()
+    ovc, onc, ovperm, ohn = oracle.coarsen_hec(1, rp, col, w, 4)
+    assert np.array_equal(vperm, ovperm)
+    assert np.array_equal(hn, ohn)
+    assert nc == onc and np.array_equal(vc, ovc)

@@ -2,6 +2,13 @@
 shift, sums, deflation, norm, dot and stopping test) against the FP64
 oracle's power iteration, level by level and for a whole bisection.
 
+These tests force FP32 onto every level above the finest
+(min_entries = 1), the small latency-bound levels included, which the
+default floor of 2^20 entries keeps in FP64: a stress case for the mixed
+path, with the looser whole-chain bounds below.  At the default floor the
+whole bisection's cut is held to 1 % of FP64's on the R-MAT-20 and 3D-grid
+workloads (test_gpu_configs.py).
+
 Tolerances.  The FP32 iteration stops by the same rule as the reference's,
 |change of u . v over two iterations| <= 1e-10 (eigensolve_kokkos.h:177-189),
 with the dot normalised by the norm of the rounded x; it then tracks the
@@ -11,7 +18,8 @@ converged eigenpairs: the Fiedler residual there is 1e-3 .. 1, which is what
 the reference's rule yields in FP64 too).  Over a whole multilevel chain the
 finest level starts from the mixed coarse result, so its end point is
 compared by Rayleigh quotient (within 5 %), angle (|cos| >= 1 - 1e-3) and
-edge cut (within 10 %)."""
+edge cut (within 10 %): stress-case bounds, widened after R-MAT-13 with
+every level in FP32 ended at |cos| = 0.99984 and a quotient 3.3 % off."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -26,7 +34,8 @@ pytestmark = pytest.mark.gpu
 MIX_COS = 1 - 1e-4
 MIX_ITER = 0.10
 MIX_RQ = 0.02
-MIX_RQ_CHAIN = 0.05
+MIX_RQ_CHAIN = 0.05     # stress case (every level FP32) only
+MIX_CUT_STRESS = 0.10   # stress case (every level FP32) only
 
 
 def _cos(a, b):
@@ -115,7 +124,7 @@ def test_mixed_bisection_vs_fp64(ctx, case):
     # sort-and-split halves exactly whatever the vector; the cut of two
     # vectors this close differs only by vertices near the median
     assert np.count_nonzero(r32.part == 0) == np.count_nonzero(r64.part == 0)
-    assert abs(r32.edge_cut - r64.edge_cut) <= 0.1 * r64.edge_cut
+    assert abs(r32.edge_cut - r64.edge_cut) <= MIX_CUT_STRESS * r64.edge_cut
     src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
     assert r32.edge_cut == int(np.count_nonzero(r32.part[src] != r32.part[col])) // 2
     g.close()

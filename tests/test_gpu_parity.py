@@ -458,6 +458,12 @@ def test_partition_4way_tiny_sides(ctx):
     r4 = mi.partition_4way(ctx, ctx.graph(rp, col), hec_seed=1, pcg_state=1)
     assert r4.edge_cut == _cut4(rp, col, r4.part)
     assert sorted(set(r4.part.tolist())) <= [0, 1, 2, 3]
+    # sides of 1 and 2 vertices, each its own coarsest level: the start
+    # vectors of the two side calls take 1 + 2 draws, the unsplit side's too
+    top = mi.partition_graph(ctx, ctx.graph(rp, col), hec_seed=1, pcg_state=1)
+    assert sorted(np.bincount(top.part, minlength=2).tolist()) == [1, 2]
+    _, after = oracle.pcg32_stream(top.pcg_state, 3)
+    assert r4.pcg_state == after
 
 
 def test_partition_4way_star_edgeless_side(ctx):
