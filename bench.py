@@ -358,6 +358,8 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=24,
                     help="R-MAT scale of the main line: 24 (seed 4, drawn on the GPU, the default) or "
                          "<= 21 (seed 1, drawn on the host)")
+    ap.add_argument("--graph-cache", default=None,
+                    help="directory that keeps the GPU-drawn R-MAT graph between runs (drawn when absent)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=60.0, help="seconds of CPU baseline at most")
@@ -381,7 +383,7 @@ def main() -> int:
     t = time.perf_counter()
     if args.scale >= 22:
         seed = 4
-        rp, col, w = graphs.rmat_device(args.scale, 16, 0.57, 0.19, 0.19, seed=seed)
+        rp, col, w = graphs.rmat_device(args.scale, 16, 0.57, 0.19, 0.19, seed=seed, cache_dir=args.graph_cache)
     else:
         seed = 1
         rp, col, w = graphs.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=seed)

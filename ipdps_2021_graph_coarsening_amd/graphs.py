@@ -16,6 +16,8 @@ original vertex order kept, as mtx2csr does (mtx2csr.cpp:133-263).
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 HEADER_WORDS = 6
@@ -225,11 +227,41 @@ def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c:
 
 
 def rmat_device(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
-                seed: int = 4, lcc: bool = True):
-    """R-MAT as `rmat`, with the edges drawn, symmetrised and merged on the
-    GPU through torch (scale 24: seconds instead of minutes); the largest
-    component is found on the host.  Another random stream than `rmat`, so
-    another instance of the same distribution."""
+                seed: int = 4, lcc: bool = True, cache_dir: str | None = None):
+    """R-MAT as `rmat`, with the edges drawn, symmetrised and merged and the
+    largest component found on the GPU through torch (scale 24: seconds
+    instead of minutes).  Another random stream than `rmat`, so another
+    instance of the same distribution.
+
+    The torch work runs in a child process.  torch ships its own HIP and HSA
+    runtimes while libsgpar_mi.so links the system ones; with both started
+    in one process the second finds no device.  The child writes the three
+    arrays as .npy files, which are read back here: into `cache_dir` when
+    given (a later call with the same arguments reads them without drawing
+    again), else into a temporary directory removed afterwards."""
+    import shutil
+    import subprocess
+    import sys
+    import tempfile
+
+    tag = f"rmat{scale}_ef{edge_factor}_{a}_{b}_{c}_seed{seed}_lcc{int(bool(lcc))}"
+    d = cache_dir if cache_dir else tempfile.mkdtemp(prefix="rmat_device_")
+    files = [os.path.join(d, f"{tag}_{x}.npy") for x in ("rowptr", "col", "w")]
+    try:
+        if not all(os.path.exists(f) for f in files):
+            os.makedirs(d, exist_ok=True)
+            # the module file runs as a script: the package (and with it
+            # libsgpar_mi.so's runtime) is not loaded in the child
+            subprocess.run([sys.executable, os.path.abspath(__file__), "rmat-device", str(scale), str(edge_factor),
+                            repr(a), repr(b), repr(c), str(seed), str(int(bool(lcc))), d, tag], check=True)
+        return tuple(np.load(f) for f in files)
+    finally:
+        if not cache_dir:
+            shutil.rmtree(d, ignore_errors=True)
+
+
+def _rmat_device_inproc(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, lcc: bool):
+    """The torch body of `rmat_device`; runs in the child process."""
     import torch
 
     n, m = 1 << scale, edge_factor << scale
@@ -311,3 +343,17 @@ def rgg(n: int, radius: float | None = None, seed: int = 3, lcc: bool = True):
 def path(n: int):
     i = np.arange(n - 1, dtype=np.int64)
     return from_edges(n, i, i + 1, None, lcc=False)
+
+
+if __name__ == "__main__":
+    import sys
+
+    if len(sys.argv) == 11 and sys.argv[1] == "rmat-device":
+        _, _, sc, ef, fa, fb, fc, sd, lc, outdir, tag = sys.argv
+        arrays = _rmat_device_inproc(int(sc), int(ef), float(fa), float(fb), float(fc), int(sd), lc == "1")
+        for name, arr in zip(("rowptr", "col", "w"), arrays):
+            tmp = os.path.join(outdir, f"{tag}_{name}.tmp.npy")
+            np.save(tmp, arr)
+            os.replace(tmp, os.path.join(outdir, f"{tag}_{name}.npy"))
+    else:
+        sys.exit("usage: graphs.py rmat-device scale edge_factor a b c seed lcc outdir tag")

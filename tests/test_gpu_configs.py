@@ -30,6 +30,7 @@ within 2 % of the oracle's from the same start vector; the whole mixed
 bisection's cut within 1 % of the FP64 one."""
 from concurrent.futures import ProcessPoolExecutor
 import multiprocessing as mp
+import os
 
 import numpy as np
 import pytest
@@ -267,7 +268,8 @@ def test_rgg512_batch_equals_single_calls_and_oracle(ctx, rgg512):
 # ------------------------------------------------------------ R-MAT-24 4-way
 @pytest.fixture(scope="module")
 def rmat24():
-    return graphs.rmat_device(24, 16, 0.57, 0.19, 0.19, seed=4)
+    # MI_GRAPH_CACHE: a directory where the drawn graph is kept between runs
+    return graphs.rmat_device(24, 16, 0.57, 0.19, 0.19, seed=4, cache_dir=os.environ.get("MI_GRAPH_CACHE"))
 
 
 def test_rmat24_4way_properties(ctx, rmat24):

@@ -65,6 +65,33 @@ def test_xorshift_linear_and_output_bijective():
     assert len(np.unique(d)) == len(d)
 
 
+def test_xorshift64star_published_definition():
+    """The oracle's generator against the published xorshift64* (Vigna,
+    "An experimental exploration of Marsaglia's xorshift generators,
+    scrambled", ACM TOMS 2016): the state transition is the product of the
+    three GF(2) matrices of the shift triple (12, 25, 27) -- right 12, left
+    25, right 27, each x -> (I + S) x -- and the output multiplies the new
+    state by M_32 = 2685821657736338717 modulo 2^64.  Kokkos' urand64
+    (Kokkos_Random.hpp:712-718) returns that product minus 1.  Here the
+    transition is assembled from the matrices, not from the oracle's code;
+    a wrong shift amount, direction or order, or a mistyped multiplier,
+    fails."""
+    def shift_matrix(k, left):
+        return [((1 << j) ^ (((1 << j) << k) & M64 if left else (1 << j) >> k)) for j in range(64)]
+
+    T = _mul(shift_matrix(27, False), _mul(shift_matrix(25, True), shift_matrix(12, False)))
+    assert _mat_of(oracle.xs_step) == T
+    M32 = 2685821657736338717
+    assert M32 % 2 == 1 and M32.bit_length() == 62
+    rng = np.random.default_rng(7)
+    for seed in [1, 2, 12345, 1318319] + [int(x) for x in rng.integers(1, 2**63, 20, dtype=np.int64)]:
+        d, _ = oracle.urand64_stream(seed, 50)
+        s = seed
+        for k in range(50):
+            s = _apply(T, s)
+            assert int(d[k]) == (s * M32 - 1) & M64, (seed, k)
+
+
 def test_pool_seed_zero_is_remapped():
     assert oracle.pool_state(0) == oracle.pool_state(1318319)
     assert oracle.pool_state(1) != oracle.pool_state(2)
