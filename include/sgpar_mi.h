@@ -196,6 +196,22 @@ int mi_compute_partition(mi_ctx* ctx, const mi_graph* g, const double* evec,
                          uint32_t* edge_cut);
 
 /*
+ * mi_compute_partition without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, serving loops, harnesses that forbid hipMalloc).
+ * d_evec (n doubles) and d_part (n words) are device arrays on the
+ * context's device; the result is mi_compute_partition's, bit for bit.
+ * Every temporary lives in one fixed device array of the library (about
+ * 49 MiB per device, held until the process ends), so g may have at most
+ * MI_NOALLOC_MAX_VERTICES vertices (MI_STATUS_ARG beyond).  Calls that
+ * overlap on one device (other threads or contexts) race on that array.
+ */
+#define MI_NOALLOC_MAX_VERTICES 2097152u
+int mi_compute_partition_noalloc(mi_ctx* ctx, const mi_graph* g, const double* d_evec,
+                                 int num_partitions, int perc_imbalance, uint32_t* d_part,
+                                 uint32_t* edge_cut);
+
+/*
  * The whole bisection: replaces sgp_partition_graph, sgpar.h:2363-2431
  * (Kokkos branch): coarsen, eigensolve, split.  part is a host array of n, or
  * a device array when part_on_device != 0.  hec_seed as in

@@ -106,6 +106,7 @@ _SIGS = [
     ("mi_power_iter", C.c_int, [vp, vp, f64p, C.c_int, C.c_double, C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_int)]),
     ("mi_compute_partition", C.c_int, [vp, vp, f64p, C.c_int, C.c_int, u32p, C.POINTER(C.c_uint32)]),
+    ("mi_compute_partition_noalloc", C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, C.POINTER(C.c_uint32)]),
     ("mi_partition_graph", C.c_int, [vp, vp, C.POINTER(Config), C.c_int, C.c_uint64,
                                      C.POINTER(C.c_uint64), C.c_uint64, vp, C.c_int,
                                      C.POINTER(C.c_uint32), C.POINTER(Stats)]),

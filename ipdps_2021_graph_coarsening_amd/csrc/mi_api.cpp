@@ -125,6 +125,7 @@ void ctx_init(Ctx& c, int device)
                          hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(c.host_mapped, 0, 128 * sizeof(uint32_t));
     MI_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.host_mapped_dev), c.host_mapped, 0));
+    noalloc_setup(device);  // the allocation-free split's scratch address on this device
 }
 
 namespace {
@@ -710,6 +711,15 @@ int mi_compute_partition(mi_ctx* ctx, const mi_graph* g, const double* evec, int
         *edge_cut = compute_partition(c, g->g, ev.get(), num_partitions, perc_imbalance, dp.get());
         d2h(c, part, dp.get(), (size_t)n * 4);
         MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mi_compute_partition_noalloc(mi_ctx* ctx, const mi_graph* g, const double* d_evec, int num_partitions,
+                                 int perc_imbalance, uint32_t* d_part, uint32_t* edge_cut)
+{
+    if (!ctx || !g || !d_evec || !d_part || !edge_cut) return ST_ARG;
+    return guard(ctx, [&] {
+        *edge_cut = compute_partition_noalloc(ctx->c, g->g, d_evec, num_partitions, perc_imbalance, d_part);
     });
 }
 

@@ -97,6 +97,14 @@ void radix_sort_pairs_ping(Ctx& c, uint32_t*& keys, uint32_t*& vals, uint64_t n,
 // ordered by the whole key (the same result as eight passes; slow only when
 // many keys share their high half, which random keys do not).
 void sort_random_keys(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, uint64_t* keys_tmp, uint32_t* vals_tmp);
+// Allocation-free forms: every temporary (radix histograms, scan sums) comes
+// from the caller's workspace of ws_words words, whose size the *_ws_words
+// functions give.  The sort leaves the pairs in whichever buffer pair the
+// last pass wrote, as radix_sort_pairs_ping.
+uint64_t scan_ws_words(uint64_t n);
+uint64_t radix_sort_ws_words(uint64_t n, int nbits);
+void radix_sort_pairs_ws(Ctx& c, uint64_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                         uint64_t*& keys_tmp, uint32_t*& vals_tmp, uint32_t* ws, uint64_t ws_words);
 
 // ---------------------------------------------------------------- RNG / HEC
 // Host: state of the single generator of Kokkos' Random_XorShift64_Pool(seed)
@@ -219,6 +227,14 @@ void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg
 // array part; returns the edge cut.
 uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
                            int perc_imbalance, uint32_t* part);
+// The same split with every temporary in a file-scope device array (no
+// allocation): at most NOALLOC_MAX_N vertices; calls overlapping on one
+// device race.  noalloc_setup(device), with that device current, takes the
+// array's address on that device once (context creation calls it).
+constexpr uint32_t NOALLOC_MAX_N = 1u << 21;
+void noalloc_setup(int device);
+uint32_t compute_partition_noalloc(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
+                                   int perc_imbalance, uint32_t* part);
 // Induced subgraphs of the sides 0 and nonzero of device array part
 // (sgp_use_partition); mapping[i] = i's index in its side's graph.
 void use_partition(Ctx& c, const DGraph& g, const uint32_t* part, DGraph& g0, DGraph& g1, DBuf<uint32_t>& mapping);

@@ -12,6 +12,10 @@
 // one workgroup.
 #include "mi_kernels.hpp"
 
+#include <cmath>
+#include <mutex>
+#include <string>
+
 namespace mi {
 
 namespace {
@@ -118,6 +122,117 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     MI_HIP(hipGetLastError());
     unsigned long long best = 0;
     read_words(c, cut.get() + 1, &best, 8);
+    return (uint32_t)(best / 2);
+}
+
+// ---------------------------------------------------------------- allocation-free split
+// The split of compute_partition with every temporary in one file-scope
+// device array, `scratch`, for callers that may not allocate device memory
+// (code under stream capture, serving loops, harnesses that forbid
+// hipMalloc).  The array's device address is taken once per device, at
+// context creation with that device current (noalloc_setup), and only that
+// address is used on the host.  Every call on a device shares that device's
+// array: calls that overlap on one device race, and each call clears what
+// must start at zero (the cut counters) itself.
+namespace {
+
+// layout of scratch (bytes): keys u64[N], keys_tmp u64[N], order u32[N],
+// order_tmp u32[N], two u64 cut counters, then the sort's workspace words
+// (radix histograms of at most 4096 keys per 256 digits, and their scan)
+constexpr uint64_t NA_N = NOALLOC_MAX_N;
+constexpr uint64_t NA_OFF_KEYS = 0;
+constexpr uint64_t NA_OFF_KTMP = NA_OFF_KEYS + 8 * NA_N;
+constexpr uint64_t NA_OFF_ORDER = NA_OFF_KTMP + 8 * NA_N;
+constexpr uint64_t NA_OFF_VTMP = NA_OFF_ORDER + 4 * NA_N;
+constexpr uint64_t NA_OFF_CUT = NA_OFF_VTMP + 4 * NA_N;
+constexpr uint64_t NA_OFF_WS = NA_OFF_CUT + 256;
+constexpr uint64_t NA_WS_WORDS = (NA_N / 4096 + 1) * 512 + 4096;
+constexpr uint64_t NA_BYTES = NA_OFF_WS + 4 * NA_WS_WORDS;
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mu;
+void* na_ptr[NA_MAX_DEVICES] = {};
+hipError_t na_err[NA_MAX_DEVICES] = {};
+
+// Cut edges counted row by row, one wavefront per row (no row index, so
+// nothing to allocate); the same integer as k_cut.
+__global__ void __launch_bounds__(BLOCK) k_cut_rows(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col,
+                                                    const uint32_t* __restrict__ part,
+                                                    unsigned long long* __restrict__ cut)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint64_t nwaves = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    uint32_t x = 0;
+    for (uint64_t r = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; r < n; r += nwaves) {
+        const uint32_t pr = part[r], e = rowptr[r + 1];
+        for (uint32_t k = rowptr[r] + lane; k < e; k += WAVE) x += part[col[k]] != pr ? 1u : 0u;
+    }
+    x = block_sum_u32(x, lds);
+    if (threadIdx.x == 0 && x) atomicAdd(cut, (unsigned long long)x);
+}
+
+}  // namespace
+
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(na_mu);
+    if (na_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    na_err[device] = e;
+    if (e == hipSuccess) na_ptr[device] = p;
+}
+
+uint32_t compute_partition_noalloc(Ctx& c, const DGraph& g, const double* ev, int num_partitions,
+                                   int perc_imbalance, uint32_t* part)
+{
+    const uint32_t n = g.n;
+    MI_CHECK(num_partitions == 2 || num_partitions == 4, ST_ARG, "num_partitions must be 2 or 4");
+    MI_CHECK(perc_imbalance >= 0 && perc_imbalance <= 100, ST_ARG, "imbalance must be in [0, 100]");
+    MI_CHECK(n <= NOALLOC_MAX_N, ST_ARG,
+             "mi_compute_partition_noalloc: more vertices than its scratch holds (at most 2097152)");
+    void* base_v = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(na_mu);
+        MI_CHECK(c.device >= 0 && c.device < NA_MAX_DEVICES, ST_ARG, "device index out of range");
+        base_v = na_ptr[c.device];
+        if (!base_v)
+            throw Error(ST_HIP, std::string("scratch address of this device unavailable: hipGetSymbolAddress: ") +
+                                    hipGetErrorString(na_err[c.device]));
+    }
+    MI_CHECK(radix_sort_ws_words(n, 64) <= NA_WS_WORDS, ST_INTERNAL, "sort workspace exceeds the scratch layout");
+    if (num_partitions == 4) num_partitions = 2;
+    const long max_part_size = (long)std::ceil(n / (double)num_partitions);
+    const uint32_t imbr = (uint32_t)std::floor(max_part_size * (1.0 + perc_imbalance / 100.0));
+    const uint32_t imbl = n - imbr;
+
+    unsigned char* base = static_cast<unsigned char*>(base_v);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(base + NA_OFF_KEYS);
+    uint64_t* ktmp = reinterpret_cast<uint64_t*>(base + NA_OFF_KTMP);
+    uint32_t* order = reinterpret_cast<uint32_t*>(base + NA_OFF_ORDER);
+    uint32_t* vtmp = reinterpret_cast<uint32_t*>(base + NA_OFF_VTMP);
+    unsigned long long* cut = reinterpret_cast<unsigned long long*>(base + NA_OFF_CUT);
+    uint32_t* ws = reinterpret_cast<uint32_t*>(base + NA_OFF_WS);
+
+    k_ev_keys<<<grid1(n), BLOCK, 0, c.stream>>>(n, ev, keys, order);
+    MI_HIP(hipGetLastError());
+    radix_sort_pairs_ws(c, keys, order, n, 0, 64, ktmp, vtmp, ws, NA_WS_WORDS);  // sorted pairs: keys / order
+    k_assign<<<grid1(n), BLOCK, 0, c.stream>>>(n, order, imbl, part);
+    MI_HIP(hipGetLastError());
+    fill_u32(c, reinterpret_cast<uint32_t*>(cut), 4, 0u);  // a kernel, not a memset node
+    if (g.m) {
+        k_cut_rows<<<red_grid((uint64_t)n * WAVE), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), part, cut);
+        MI_HIP(hipGetLastError());
+    }
+    k_walk<<<1, BLOCK, 0, c.stream>>>(imbl, imbr, order, g.R(), g.Cc(), part, cut, cut + 1);
+    MI_HIP(hipGetLastError());
+    unsigned long long best = 0;
+    read_words(c, cut + 1, &best, 8);
     return (uint32_t)(best / 2);
 }
 

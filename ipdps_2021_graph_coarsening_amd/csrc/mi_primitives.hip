@@ -255,10 +255,16 @@ __global__ void k_iota(uint32_t* out, uint64_t n, uint32_t start)
 
 }  // namespace
 
-void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev)
+namespace {
+
+// Exclusive scan; the per-tile sums come from the workspace ws (ws_words
+// words, consumed front to back by the recursion) or, when ws is null, from
+// the pool.
+void scan_impl(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev, uint32_t* ws,
+               uint64_t ws_words)
 {
     if (n == 0) {
-        if (total_dev) MI_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), c.stream));
+        if (total_dev) fill_u32(c, total_dev, 1, 0u);
         return;
     }
     const uint64_t tiles = ceil_div(n, SCAN_TILE);
@@ -268,12 +274,36 @@ void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, u
         MI_HIP(hipGetLastError());
         return;
     }
-    DBuf<uint32_t> sums(&c.pool, tiles);
-    k_scan_reduce<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, n, sums.get());
+    DBuf<uint32_t> own;
+    uint32_t* sums;
+    if (ws) {
+        MI_CHECK(ws_words >= tiles, ST_INTERNAL, "scan workspace too small");
+        sums = ws;
+        ws += tiles;
+        ws_words -= tiles;
+    } else {
+        own = DBuf<uint32_t>(&c.pool, tiles);
+        sums = own.get();
+    }
+    k_scan_reduce<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, n, sums);
     MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, sums.get(), sums.get(), tiles, nullptr);
-    k_scan_down<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, out, n, sums.get(), total_dev);
+    scan_impl(c, sums, sums, tiles, nullptr, ws, ws_words);
+    k_scan_down<<<(unsigned)tiles, BLOCK, 0, c.stream>>>(in, out, n, sums, total_dev);
     MI_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total_dev)
+{
+    scan_impl(c, in, out, n, total_dev, nullptr, 0);
+}
+
+uint64_t scan_ws_words(uint64_t n)
+{
+    uint64_t words = 0;
+    for (uint64_t t = ceil_div(n, SCAN_TILE); t > 1; t = ceil_div(t, SCAN_TILE)) words += t;
+    return words;
 }
 
 void build_row_index(Ctx& c, const DGraph& g, DBuf<uint32_t>& roc)
@@ -298,21 +328,31 @@ void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start)
     MI_HIP(hipGetLastError());
 }
 
+// The histograms (tiles x RADIX words) and their scan's sums come from the
+// workspace ws when it is given (ws_words words), else from the pool.
 template <class K, int DB>
 static void radix_pass(Ctx& c, const K* ksrc, const uint32_t* vsrc, K* kdst, uint32_t* vdst, uint64_t n, int shift,
-                       int bits, DBuf<uint32_t>& hist)
+                       int bits, DBuf<uint32_t>& hist, uint32_t* ws, uint64_t ws_words)
 {
     using S = RadixShape<DB>;
     const uint64_t tiles = ceil_div(n, S::TILE);
     MI_CHECK(tiles * S::RADIX < (1ull << 32), ST_ARG, "radix sort too large");
-    if (hist.size() < tiles * S::RADIX) hist = DBuf<uint32_t>(&c.pool, tiles * S::RADIX);
+    uint32_t* h;
+    if (ws) {
+        MI_CHECK(ws_words >= tiles * S::RADIX, ST_INTERNAL, "radix workspace too small");
+        h = ws;
+        ws += tiles * S::RADIX;
+        ws_words -= tiles * S::RADIX;
+    } else {
+        if (hist.size() < tiles * S::RADIX) hist = DBuf<uint32_t>(&c.pool, tiles * S::RADIX);
+        h = hist.get();
+    }
     const uint32_t mask = (1u << bits) - 1u;
-    k_radix_hist<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles,
-                                                                     hist.get());
+    k_radix_hist<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles, h);
     MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, hist.get(), hist.get(), tiles * S::RADIX, nullptr);
+    scan_impl(c, h, h, tiles * S::RADIX, nullptr, ws, ws_words);
     k_radix_scatter<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
-                                                                        (uint32_t)tiles, hist.get());
+                                                                        (uint32_t)tiles, h);
     MI_HIP(hipGetLastError());
 }
 
@@ -418,7 +458,7 @@ __global__ void __launch_bounds__(1024) k_sort_small_packed(K* __restrict__ keys
 // keys / vals point to it and keys_tmp / vals_tmp to the other pair.
 template <class K>
 static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit, K*& keys_tmp,
-                            uint32_t*& vals_tmp)
+                            uint32_t*& vals_tmp, uint32_t* ws = nullptr, uint64_t ws_words = 0)
 {
     MI_CHECK(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit, ST_ARG, "radix bit range");
     if (n <= 1 || begin_bit == end_bit) return;
@@ -442,9 +482,9 @@ static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int b
     for (int q = 0; q < passes; q++) {
         const int bits = (total - (shift - begin_bit) + (passes - q) - 1) / (passes - q);  // even split
         if (wide)
-            radix_pass<K, 9>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist);
+            radix_pass<K, 9>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist, ws, ws_words);
         else
-            radix_pass<K, 8>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist);
+            radix_pass<K, 8>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist, ws, ws_words);
         std::swap(keys, keys_tmp);
         std::swap(vals, vals_tmp);
         shift += bits;
@@ -486,6 +526,23 @@ void radix_sort_pairs_ping(Ctx& c, uint32_t*& keys, uint32_t*& vals, uint64_t n,
                            uint32_t*& keys_tmp, uint32_t*& vals_tmp)
 {
     radix_sort_ping<uint32_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp);
+}
+
+void radix_sort_pairs_ws(Ctx& c, uint64_t*& keys, uint32_t*& vals, uint64_t n, int begin_bit, int end_bit,
+                         uint64_t*& keys_tmp, uint32_t*& vals_tmp, uint32_t* ws, uint64_t ws_words)
+{
+    MI_CHECK(ws != nullptr, ST_INTERNAL, "radix_sort_pairs_ws needs a workspace");
+    radix_sort_ping<uint64_t>(c, keys, vals, n, begin_bit, end_bit, keys_tmp, vals_tmp, ws, ws_words);
+}
+
+uint64_t radix_sort_ws_words(uint64_t n, int nbits)
+{
+    if (n <= SMALL_SORT || nbits <= 0) return 0;
+    const int p8 = (nbits + 7) / 8, p9 = (nbits + 8) / 9;
+    const bool wide = p9 < p8;  // as radix_sort_ping chooses
+    const uint64_t h = wide ? ceil_div(n, RadixShape<9>::TILE) * RadixShape<9>::RADIX
+                            : ceil_div(n, RadixShape<8>::TILE) * RadixShape<8>::RADIX;
+    return h + scan_ws_words(h);
 }
 
 namespace {
