@@ -368,6 +368,8 @@ def main() -> int:
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline measurement of the dominant kernel, plus a 256 MiB device copy "
                          "that calibrates rocprofv3's byte counters")
+    ap.add_argument("--bisections-only", action="store_true",
+                    help="only the timed FP64 bisections (for kernel traces of the whole bisection)")
     ap.add_argument("--roofline-level", type=int, default=-1,
                     help="level for the roofline (default: the level with the most eigensolve time; "
                          "1 with --roofline-only)")
@@ -425,6 +427,12 @@ def main() -> int:
     stats = st.as_dict()
     log(f"timed {args.steps} FP64 bisections: {ms:.2f} ms, cut {stats['edge_cut']}, "
         f"levels {stats['level_n']}, iterations {stats['level_iters']}")
+    if args.bisections_only:
+        print(json.dumps({"workload": workload, "ms_per_bisection": round(ms, 3), "stats": stats}), flush=True)
+        R.close()
+        g.close()
+        ctx.close()
+        return 0
     ctx.set_precision(True)
     ms32, st32 = R.timed(R.bisect, args.steps, args.warmup, f"{name} mixed")
     ctx.set_precision(False)

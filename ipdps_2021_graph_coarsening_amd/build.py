@@ -35,13 +35,14 @@ def sources() -> list[str]:
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
+    """Runs a compiler; raises with its diagnostics on failure."""
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [l for l in (r.stderr or "").splitlines() if l.strip() and not any(k in l for k in NOISE)]
     if r.returncode != 0:
         sys.stderr.write("\n".join(msgs) + "\n")
-        raise RuntimeError(f"hipcc failed with status {r.returncode}: {' '.join(cmd[-3:])}")
+        raise RuntimeError(f"{os.path.basename(cmd[0])} failed with status {r.returncode}: {' '.join(cmd[-3:])}")
     if verbose and msgs:
         sys.stderr.write("\n".join(msgs) + "\n")
 
@@ -65,5 +66,40 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     return LIB
 
 
+DROPIN = os.path.join(PKG, "dropin")
+DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
+DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
+REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
+
+
+def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
+    """The reference-API shim (libsgpar_dropin.so over libsgpar_mi.so) and
+    its command-line driver, compiled against the reference's own
+    cmg/sgpar.h and cmg/ExperimentLoggerUtil.cpp where they lie.  Skipped
+    (returns False) when that directory is absent; the built files are then
+    the ones a previous build left."""
+    if not os.path.isfile(os.path.join(ref_cmg, "sgpar.h")):
+        return False
+    cxx = os.environ.get("CXX", "g++")
+    common = ["-O2", "-std=c++17", "-DEXPERIMENT", "-I", ref_cmg, "-I", os.path.join(os.path.dirname(PKG), "include"),
+              "-w"]
+    tmp = DROPIN_LIB + ".tmp"
+    # -Bsymbolic: calls inside the shim bind to its own definitions, not to
+    # copies a caller's translation unit of sgpar.h may define
+    # the version script exports the API only; the serial helpers the
+    # reference's header defines in every translation unit stay local and
+    # are dropped
+    _run([cxx, *common, "-fPIC", "-shared", "-ffunction-sections", os.path.join(DROPIN, "sgpar_dropin.cpp"),
+          "-o", tmp, "-L", PKG, "-l:libsgpar_mi.so", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-Wl,--gc-sections",
+          f"-Wl,--version-script={os.path.join(DROPIN, 'sgpar_dropin.map')}"], verbose)
+    os.replace(tmp, DROPIN_LIB)
+    tmp = DROPIN_EXE + ".tmp"
+    _run([cxx, *common, os.path.join(DROPIN, "sgpar_dropin_main.cpp"), "-o", tmp, "-L", PKG,
+          "-l:libsgpar_dropin.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    os.replace(tmp, DROPIN_EXE)
+    return True
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print("drop-in built" if build_dropin(verbose=True) else "drop-in skipped (no reference headers)")

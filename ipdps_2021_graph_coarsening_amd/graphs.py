@@ -351,6 +351,7 @@ if __name__ == "__main__":
     if len(sys.argv) == 11 and sys.argv[1] == "rmat-device":
         _, _, sc, ef, fa, fb, fc, sd, lc, outdir, tag = sys.argv
         arrays = _rmat_device_inproc(int(sc), int(ef), float(fa), float(fb), float(fc), int(sd), lc == "1")
+        os.makedirs(outdir, exist_ok=True)
         for name, arr in zip(("rowptr", "col", "w"), arrays):
             tmp = os.path.join(outdir, f"{tag}_{name}.tmp.npy")
             np.save(tmp, arr)

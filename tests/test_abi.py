@@ -99,3 +99,33 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         mi.lib()
     with pytest.raises(RuntimeError):
         mi.Context(0)
+
+
+DROPIN_API = {
+    "sgpar::sgp_partition_graph(unsigned int*, unsigned int*, sgpar::configuration*, int, sgpar::sgp_graph_t, "
+    "ExperimentLoggerUtil&, sgpar::sgp_pcg32_random_t*)",
+    "sgpar::sgp_load_graph(sgpar::sgp_graph_t*, char*)",
+    "sgpar::sgp_free_graph(sgpar::sgp_graph_t*)",
+    "sgpar::sgp_load_partition(unsigned int*, unsigned int, char*)",
+    "sgpar::sgp_use_partition(unsigned int*, sgpar::sgp_graph_t, sgpar::sgp_graph_t*, sgpar::sgp_graph_t*)",
+    "sgpar::sgp_load_config(char const*, sgpar::configuration*)",
+    "sgpar::compute_partition_edit_distance(unsigned int const*, unsigned int const*, unsigned int, unsigned int*)",
+    "sgpar_mi_fix_clock",
+}
+
+
+def test_dropin_exports_the_reference_api():
+    """libsgpar_dropin.so exports the functions of the reference's API
+    (cmg/sgpar.h:2230-2244 with EXPERIMENT; demangled names carry the
+    reference's own types) and the clock pin, nothing else of the reference
+    header's serial code; its driver links it."""
+    from ipdps_2021_graph_coarsening_amd import build as mibuild
+    if not os.path.exists(mibuild.DROPIN_LIB):
+        pytest.skip("the drop-in was not built (it needs the reference's cmg headers at build time)")
+    out = subprocess.run(["nm", "-DC", "--defined-only", mibuild.DROPIN_LIB], capture_output=True, text=True,
+                         check=True).stdout
+    names = {line.split(" ", 2)[2] for line in out.splitlines() if line.count(" ") >= 2}
+    ours = {x for x in names if x.startswith("sgpar") or x.startswith("sgp")}
+    assert ours == DROPIN_API
+    ldd = subprocess.run(["ldd", mibuild.DROPIN_EXE], capture_output=True, text=True, check=True).stdout
+    assert "libsgpar_dropin.so" in ldd and "libsgpar_mi.so" in ldd and "not found" not in ldd

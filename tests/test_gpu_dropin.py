@@ -1,0 +1,119 @@
+"""The reference-API drop-in (libsgpar_dropin.so: sgpar::sgp_partition_graph
+and the file loaders with the reference's own signatures, compiled against
+cmg/sgpar.h) driven end to end by its command-line driver, which follows the
+reference's cmg/sgpar.cpp: binary CSR and config files in, the metrics JSON
+of ExperimentLoggerUtil::log (ExperimentLoggerUtil.cpp:145-192) out.
+
+- the 256^2 grid with time() pinned to 12345 reproduces the recorded
+  reference run: level sizes, power iterations and cut 323;
+- two trials of one stream equal two chained mi_partition_graph calls
+  (partition, cut, per-level iterations and sizes in the metrics);
+- the 4-way config equals mi_partition_4way (labels and summed cut);
+- a partition file given for comparison gives partition-diff 0 for itself.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import ipdps_2021_graph_coarsening_amd as mi
+from ipdps_2021_graph_coarsening_amd import build as mibuild
+from ipdps_2021_graph_coarsening_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "grid256_serial_seed12345.json")
+MEASUREMENTS = ["coarsen-map", "coarsen-build", "coarsen-count", "coarsen-prefix-sum", "coarsen-bucket",
+                "coarsen-dedupe", "coarsen-radix-sort", "coarsen-radix-dedupe", "coarsen-write-graph",
+                "coarsen-permute", "coarsen-map-construct", "fm-refine", "fm-recoarsen", "heavy"]
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not (os.path.exists(mibuild.DROPIN_EXE) and os.path.exists(mibuild.DROPIN_LIB)):
+        pytest.skip("the drop-in was not built (it needs the reference's cmg headers at build time)")
+    return mibuild.DROPIN_EXE
+
+
+def _run(driver, tmp_path, rp, col, config, seed, best="-"):
+    csr, cfg = tmp_path / "g.csr", tmp_path / "config.txt"
+    metrics, part = tmp_path / "metrics.json", tmp_path / "part.txt"
+    for p in (metrics, part):
+        if p.exists():
+            p.unlink()
+    graphs.write_csr(str(csr), rp, col)
+    cfg.write_text(" ".join(str(x) for x in config) + "\n")
+    env = dict(os.environ, SGPAR_FIXED_TIME=str(seed))
+    r = subprocess.run([driver, str(csr), str(metrics), str(cfg), str(best), str(part)], env=env,
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    return json.loads(metrics.read_text()), np.loadtxt(part, dtype=np.uint32, ndmin=1), r.stdout
+
+
+def _levels_match(trial, stats):
+    lv = trial["coarse-levels"]
+    n = stats["nlevels"]
+    assert trial["number-coarse-levels"] == n == len(lv)
+    # coarsest first, as the eigensolve records them
+    assert [l["number-vertices"] for l in lv] == stats["level_n"][:n][::-1]
+    assert [l["refine-iterations"] for l in lv] == [int(x) for x in stats["level_iters"][:n][::-1]]
+    assert lv[-1]["unrefined-edge-cut"] == trial["edge-cut"]
+    for k in MEASUREMENTS:
+        assert f"{k}-duration-seconds" in trial
+    assert trial["total-duration-seconds"] > 0 and trial["coarsen-duration-seconds"] > 0
+
+
+def test_dropin_grid256_reproduces_recorded_reference_run(driver, tmp_path, grid256):
+    g = json.load(open(GOLD))
+    rp, col = grid256
+    trials, part, out = _run(driver, tmp_path, rp, col, [0, 0, 0, 2, 1, 1e-10], 12345)
+    (t,) = trials
+    assert [l["number-vertices"] for l in t["coarse-levels"]] == g["levels_n_fine_to_coarse"][::-1]
+    gold_it = g["power_iterations_fine_to_coarse"][::-1]
+    for a, b in zip([l["refine-iterations"] for l in t["coarse-levels"]], gold_it):
+        assert abs(a - b) <= max(2, 0.1 * b)
+    assert abs(t["edge-cut"] - g["edge_cut"]) <= 0.01 * g["edge_cut"]
+    src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    assert t["edge-cut"] == int(np.count_nonzero(part[src] != part[col])) // 2
+    assert "min edgecut found is" in out
+
+
+def test_dropin_two_trials_equal_chained_calls(driver, tmp_path, ctx):
+    rp, col, _ = graphs.rgg(30000, seed=5)
+    trials, part, _ = _run(driver, tmp_path, rp, col, [0, 0, 0, 2, 2, 1e-10], 777)
+    g = ctx.graph(rp, col)
+    r1 = mi.partition_graph(ctx, g, hec_seed=777, pcg_state=777)
+    r2 = mi.partition_graph(ctx, g, hec_seed=777, pcg_state=r1.pcg_state)
+    g.close()
+    assert len(trials) == 2
+    for t, r in zip(trials, (r1, r2)):
+        assert t["edge-cut"] == r.edge_cut
+        _levels_match(t, r.stats)
+    assert np.array_equal(part, r2.part)
+
+
+def test_dropin_four_way_equals_partition_4way(driver, tmp_path, ctx):
+    rp, col, w = graphs.rmat(13, 16, seed=3)
+    trials, part, _ = _run(driver, tmp_path, rp, col, [0, 0, 0, 4, 1, 1e-10], 99)
+    g = ctx.graph(rp, col)  # the reference's loader reads topology only: unit weights
+    r4 = mi.partition_4way(ctx, g, hec_seed=99, pcg_state=99)
+    top = mi.partition_graph(ctx, g, hec_seed=99, pcg_state=99)
+    g.close()
+    (t,) = trials
+    assert t["edge-cut"] == top.edge_cut
+    assert t["edge-cut-four-way"] == r4.edge_cut
+    assert np.array_equal(part, r4.part)
+
+
+def test_dropin_partition_diff_against_itself(driver, tmp_path):
+    rp, col, _ = graphs.grid2d(48)
+    _, part, _ = _run(driver, tmp_path, rp, col, [0, 0, 0, 2, 1, 1e-10], 5)
+    best = tmp_path / "best.txt"
+    best.write_text("".join(f"{int(x)}\n" for x in part))
+    trials, part2, _ = _run(driver, tmp_path, rp, col, [0, 0, 0, 2, 1, 1e-10], 5, best=best)
+    assert trials[0]["partition-diff"] == 0
+    assert np.array_equal(part, part2)
