@@ -370,6 +370,9 @@ def main() -> int:
                          "that calibrates rocprofv3's byte counters")
     ap.add_argument("--bisections-only", action="store_true",
                     help="only the timed FP64 bisections (for kernel traces of the whole bisection)")
+    ap.add_argument("--bucketed-build", type=int, default=None, metavar="MIN_ENTRIES",
+                    help="coarse builds of levels with at least MIN_ENTRIES entries take the bucketed form "
+                         "(default: the radix form everywhere)")
     ap.add_argument("--roofline-level", type=int, default=-1,
                     help="level for the roofline (default: the level with the most eigensolve time; "
                          "1 with --roofline-only)")
@@ -395,6 +398,8 @@ def main() -> int:
     log(f"{name}: n={n} m={m} generated in {time.perf_counter() - t:.1f} s")
 
     ctx = mi.Context(0)
+    if args.bucketed_build is not None:
+        ctx.set_bucketed_build(args.bucketed_build)
     ctx.graph(np.array([0, 1, 2], dtype=np.uint32), np.array([1, 0], dtype=np.uint32)).close()  # first-use costs
     t = time.perf_counter()
     g = ctx.graph(rp, col, w)  # host -> HBM upload of the CSR (synchronous)

@@ -124,6 +124,14 @@ int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk);
    the reference. */
 int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices);
 
+/* Coarse builds (P^T A P with duplicate merge) of levels with at least
+   min_entries stored entries take the bucketed form: rows grouped by coarse
+   vertex, merged by sorts and dense accumulators in LDS; smaller levels take
+   one radix sort over all entries.  Default 0xFFFFFFFF (the radix form on
+   every level); 0 puts every level on the bucketed form.  Results are
+   identical either way.  No counterpart in the reference. */
+int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries);
+
 /* ---------------------------------------------------------------- graphs */
 /*
  * Uploads a CSR graph (host arrays) to HBM.  Replaces the host->device copy

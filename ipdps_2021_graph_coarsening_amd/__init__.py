@@ -89,6 +89,7 @@ _SIGS = [
     ("mi_ctx_set_precision", C.c_int, [vp, C.c_int, C.c_uint32]),
     ("mi_ctx_set_small_levels", C.c_int, [vp, C.c_uint32, C.c_uint32]),
     ("mi_ctx_set_locality", C.c_int, [vp, C.c_int, C.c_uint32]),
+    ("mi_ctx_set_bucketed_build", C.c_int, [vp, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
@@ -189,6 +190,12 @@ class Context:
     def set_locality(self, enable: bool = True, min_vertices: int = 0) -> None:
         """Solve large gather-form levels in a locality order (default on)."""
         self.check(self._L.mi_ctx_set_locality(self.h, 1 if enable else 0, min_vertices), "mi_ctx_set_locality")
+
+    def set_bucketed_build(self, min_entries: int = 0xFFFFFFFF) -> None:
+        """Coarse builds of levels with >= min_entries entries take the
+        bucketed form (0: every level; the default: none); results are
+        identical."""
+        self.check(self._L.mi_ctx_set_bucketed_build(self.h, min_entries), "mi_ctx_set_bucketed_build")
 
     def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:
         """Levels of <= 16384 vertices and <= max_entries entries iterate in one

@@ -468,6 +468,13 @@ int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices)
     return ST_OK;
 }
 
+int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
+{
+    if (!ctx) return ST_ARG;
+    ctx->c.bucketed_min_m = min_entries;
+    return ST_OK;
+}
+
 int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk)
 {
     if (!ctx) return ST_ARG;

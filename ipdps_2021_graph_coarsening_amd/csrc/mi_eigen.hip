@@ -74,6 +74,7 @@ constexpr int NW = BLOCK / WAVE;        // wavefronts per workgroup, global-gath
 constexpr int LBLOCK = 1024;            // workgroup of the LDS form
 constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
+constexpr uint32_t XBM = 19456;         // ... with more than KB column blocks (152 KiB: fewer blocks)
 constexpr int MAXCB = MAX_COLUMN_BLOCKS; // column blocks at most
 constexpr int KB = 8;                   // blocks whose bounds travel in the kernel arguments
 constexpr int RCB = 8;                  // blocks a finish thread handles per group (loads in flight)
@@ -391,6 +392,7 @@ struct StepArgs {
     double* scal;
     unsigned long long* niter;
     uint32_t* hdone;         // done flag in mapped host memory (mode 0 loop), or null
+    double* ax;              // FP64 loop: A x of every iteration's input row by row, or null
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -812,7 +814,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
 template <int WF, bool R16, class V, bool MANY>
 __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
 {
-    constexpr uint32_t XN = XB;
+    constexpr uint32_t XN = MANY ? XBM : XB;
     __shared__ V xl[XN];
     constexpr int NS = sizeof(V) == 4 ? 5 : 4;
     __shared__ double red[NS][LNW];
@@ -998,6 +1000,7 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     for (int k = 0; k < NS; k++) S[k] = 0.0;
     // one row's y and its terms of the sums, rows in this thread's order
     auto emit = [&](uint32_t r, V yr, uint32_t wd, V v) {
+        if (A.ax) A.ax[r] = (double)v;  // the last one written is A u of the returned u
         const V xv = xform(yr, a, s);  // as the streaming kernel formed x
         const double xr = (double)xv;
         const V yv = (V)shifted<NORMLAP>(A, (double)v, xr, wd, gb);
@@ -1793,6 +1796,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.scal = c.scal;
     A.niter = c.niter;
     A.hdone = nullptr;
+    A.ax = nullptr;
     return A;
 }
 
@@ -1894,20 +1898,30 @@ void launch_step(Ctx& c, int normLap, const StepArgs& A)
 
 }  // namespace
 
+// Column blocks of a level's LDS form, 0 for the global-gather form: x fits
+// in LDS in at most MAXCB blocks; beyond one block only for dense levels,
+// whose gathers dominate (the block layout costs a pass over the level).
+// Up to KB blocks hold XB vertices each; a level needing more takes the
+// larger XBM-vertex blocks of the many-block kernel (fewer blocks, so fewer
+// per-block row sums), at least KB + 1 of them.
+static uint32_t column_blocks(uint32_t n, uint32_t m)
+{
+    const uint32_t need = (uint32_t)ceil_div(n, XB);
+    if (!m) return 0;
+    if (need == 1) return 1;
+    if ((uint64_t)m < 64ull * n) return 0;
+    uint32_t ncb = need;
+    if (need > (uint32_t)KB) ncb = std::max<uint32_t>(KB + 1, (uint32_t)ceil_div(n, XBM));
+    if (ncb > (uint32_t)MAXCB || (uint64_t)ncb * n >= 0xffffffffull) return 0;
+    return ncb;
+}
+
 void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
 {
     p.f32 = f32;
     const uint32_t n = g.n, m = g.m;
     MI_CHECK(n > 0, ST_ARG, "empty level");
-    // LDS form when x fits in LDS in at most MAXCB column blocks; beyond one
-    // block only for dense levels, whose gathers dominate (the block layout
-    // costs a pass over the level).
-    const uint32_t need = (uint32_t)ceil_div(n, XB);
-    uint32_t ncb = 0;
-    if (m && need == 1)
-        ncb = 1;
-    else if (m && need <= (uint32_t)MAXCB && (uint64_t)m >= 64ull * n && (uint64_t)need * n < 0xffffffffull)
-        ncb = need;
+    const uint32_t ncb = column_blocks(n, m);
     p.ncb = ncb;
     p.bw = ncb ? (uint32_t)ceil_div(n, ncb) : 0u;
     for (int q = 0; q <= MAXCB; q++) p.boff[q] = p.bwave[q] = 0;
@@ -2070,20 +2084,15 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
 void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg, uint32_t* gb_dev)
 {
     MI_HIP(hipMemsetAsync(gb_dev, 0, 4, c.stream));
-    if (!g.unit && p.f32) {
-        // integer row sums: a float product of ones would not be exact
+    if (!g.unit) {
+        // integer row sums of the weights (the reference's u32 sums; the
+        // same integers an FP64 product with a vector of ones gives, without
+        // its gathers)
         k_wdeg_int<<<(unsigned)ceil_div(g.n, NW), BLOCK, 0, c.stream>>>(g.n, g.R(), g.W(), wdeg);
         MI_HIP(hipGetLastError());
         k_gb_from<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, wdeg, gb_dev);
-    } else if (g.unit) {
-        k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
     } else {
-        DBuf<double> ones(&c.pool, g.n), y(&c.pool, g.n);
-        k_fill_f64<<<grid1(g.n), BLOCK, 0, c.stream>>>(g.n, ones.get(), 1.0);
-        MI_HIP(hipGetLastError());
-        StepArgs A = make_args(c, g, p, nullptr, nullptr, ones.get(), y.get(), 0.0, 2);
-        launch_step(c, 0, A);
-        k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), y.get(), wdeg, gb_dev);
+        k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
     }
     MI_HIP(hipGetLastError());
     k_gb_floor<<<1, WAVE, 0, c.stream>>>(gb_dev);
@@ -2175,9 +2184,7 @@ double order_spread(Ctx& c, const DGraph& g)
 
 bool gather_form_level(const DGraph& g)
 {
-    const uint32_t need = (uint32_t)ceil_div(g.n, XB);
-    return g.m && need > 1 && !(need <= (uint32_t)MAXCB && (uint64_t)g.m >= 64ull * g.n &&
-                                (uint64_t)need * g.n < 0xffffffffull);
+    return g.m && column_blocks(g.n, g.m) == 0 && ceil_div(g.n, XB) > 1;
 }
 
 void level_keys(Ctx& c, const uint32_t* key_f, const uint32_t* vcmap, uint32_t n_fine, uint32_t* key_c,
@@ -2228,14 +2235,20 @@ void prolong(Ctx& c, const uint32_t* vcmap, uint32_t n_fine, const double* in, d
 
 
 // eigenvalue, min, max of the diagnostic above into out[0..3) (host)
+// The final eigenvalue log of a level from its vector u; au = A u when the
+// loop kept it (the finish kernel of the last iteration wrote it), else one
+// plain product into tmp first.
 static void eigenvalue_log(Ctx& c, const DGraph& g, const LevelPlan& p, const double* d_u, double* tmp,
-                           double* out)
+                           const double* au, double* out)
 {
-    StepArgs A = make_args(c, g, p, nullptr, nullptr, d_u, tmp, 0.0, 2);
-    launch_step(c, 0, A);
+    if (!au) {
+        StepArgs A = make_args(c, g, p, nullptr, nullptr, d_u, tmp, 0.0, 2);
+        launch_step(c, 0, A);
+        au = tmp;
+    }
     const unsigned nb = red_grid(g.n);
     DBuf<double> part(&c.pool, (size_t)nb * 3), res(&c.pool, 3);
-    k_eig_log_part<<<nb, BLOCK, 0, c.stream>>>(g.n, g.R(), d_u, tmp, part.get());
+    k_eig_log_part<<<nb, BLOCK, 0, c.stream>>>(g.n, g.R(), d_u, au, part.get());
     MI_HIP(hipGetLastError());
     k_eig_log_final<<<1, BLOCK, 0, c.stream>>>(nb, part.get(), res.get());
     MI_HIP(hipGetLastError());
@@ -2415,6 +2428,10 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         buf[0] = f0.get();
         buf[1] = f1.get();
     }
+    // the loop keeps A x of its inputs for the final eigenvalue log (the
+    // product of the returned u comes from the last iteration)
+    DBuf<double> ax;
+    if (eig_log && !normLap && !mixed && !small) ax = DBuf<double>(&c.pool, n);
     StepArgs A[2];
     c.host_mapped[64] = 0u;  // no kernel of this level has run yet
     for (int k = 0; k < 2; k++) {
@@ -2426,6 +2443,7 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         A[k].hdone = c.host_mapped_dev + 64;
         A[k].fprev = plan.fsum.get() + NSUM * MAXF_LDS * (k ^ 1);  // sums of the previous launch
         A[k].fcur = plan.fsum.get() + NSUM * MAXF_LDS * k;
+        A[k].ax = ax.get();
     }
     // Small levels run in one workgroup (no mixed precision there: they are
     // latency-bound); the others in launches of the two-kernel step.
@@ -2447,7 +2465,7 @@ void power_iter(Ctx& c, const DGraph& g, double* d_u, int normLap, double tol, P
         k_materialize<double><<<grid1(n), BLOCK, 0, c.stream>>>(
             n, static_cast<const double*>(buf[(done_iters - 1) & 1]), d_u, c.scal);
     MI_HIP(hipGetLastError());
-    if (eig_log && !normLap && !mixed) eigenvalue_log(c, g, plan, d_u, other.get(), eig_log);
+    if (eig_log && !normLap && !mixed) eigenvalue_log(c, g, plan, d_u, other.get(), ax.get(), eig_log);
     if (st) {
         MI_HIP(hipStreamSynchronize(c.stream));
         st->iters = done_iters;
@@ -2556,7 +2574,7 @@ void eigen_log_level(Ctx& c, const DGraph& g, const double* d_u, double* out3)
     LevelPlan plan;
     plan_level(c, g, plan);
     DBuf<double> tmp(&c.pool, g.n);
-    eigenvalue_log(c, g, plan, d_u, tmp.get(), out3);
+    eigenvalue_log(c, g, plan, d_u, tmp.get(), nullptr, out3);
 }
 
 }  // namespace mi

@@ -40,19 +40,36 @@ __global__ void k_assign(uint32_t n, const uint32_t* __restrict__ order, uint32_
     if (k < n) part[order[k]] = k < imbl ? 0u : 1u;
 }
 
-// Counts directed edges whose ends lie in different parts.  A fixed grid
-// walks the entries with a stride and each workgroup adds its count with one
-// atomic (integer sums: the order does not matter), so the counter sees at
-// most MAX_RED_BLOCKS atomics whatever m is.
-__global__ void __launch_bounds__(BLOCK) k_cut(uint64_t m, const uint32_t* __restrict__ rowptr,
-                                               const uint32_t* __restrict__ roc, const uint32_t* __restrict__ col,
-                                               const uint32_t* __restrict__ part,
-                                               unsigned long long* __restrict__ cut)
+// The two sides as a bitmap (bit i of word i / 32: vertex i is on side 1),
+// one ballot per wavefront: 1.1 MB for 8.9M vertices, which stays in the
+// L2 caches while the cut kernel gathers from it.
+__global__ void k_side_bits(uint32_t n, const uint32_t* __restrict__ part, uint32_t* __restrict__ bits)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const unsigned long long b = __ballot(i < n && part[i] != 0u);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && i < n) {
+        bits[i >> 5] = (uint32_t)b;
+        if ((i >> 5) + 1 < ceil_div(n, 32)) bits[(i >> 5) + 1] = (uint32_t)(b >> 32);
+    }
+}
+
+// Counts directed edges whose ends lie in different parts, reading the
+// sides from their bitmap.  A fixed grid walks the entries with a stride and
+// each workgroup adds its count with one atomic (integer sums: the order
+// does not matter), so the counter sees at most MAX_RED_BLOCKS atomics
+// whatever m is.
+__global__ void __launch_bounds__(BLOCK) k_cut_bits(uint64_t m, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ roc,
+                                                    const uint32_t* __restrict__ col,
+                                                    const uint32_t* __restrict__ bits,
+                                                    unsigned long long* __restrict__ cut)
 {
     __shared__ uint32_t lds[BLOCK / WAVE];
     uint32_t x = 0;
-    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < m; e += (uint64_t)gridDim.x * BLOCK)
-        x += part[col[e]] != part[entry_row((uint32_t)e, roc, rowptr)] ? 1u : 0u;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < m; e += (uint64_t)gridDim.x * BLOCK) {
+        const uint32_t r = entry_row((uint32_t)e, roc, rowptr), c = col[e];
+        x += ((bits[r >> 5] >> (r & 31)) ^ (bits[c >> 5] >> (c & 31))) & 1u;
+    }
     x = block_sum_u32(x, lds);
     if (threadIdx.x == 0 && x) atomicAdd(cut, (unsigned long long)x);
 }
@@ -113,9 +130,11 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     DBuf<unsigned long long> cut(&c.pool, 2);
     MI_HIP(hipMemsetAsync(cut.get(), 0, 16, c.stream));
     if (g.m) {
-        DBuf<uint32_t> roc;
+        DBuf<uint32_t> roc, bits(&c.pool, ceil_div(n, 32));
         build_row_index(c, g, roc);
-        k_cut<<<red_grid(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), part, cut.get());
+        k_side_bits<<<grid1(n), BLOCK, 0, c.stream>>>(n, part, bits.get());
+        MI_HIP(hipGetLastError());
+        k_cut_bits<<<red_grid(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), bits.get(), cut.get());
         MI_HIP(hipGetLastError());
     }
     k_walk<<<1, BLOCK, 0, c.stream>>>(imbl, imbr, order.get(), g.R(), g.Cc(), part, cut.get(), cut.get() + 1);
@@ -157,7 +176,7 @@ void* na_ptr[NA_MAX_DEVICES] = {};
 hipError_t na_err[NA_MAX_DEVICES] = {};
 
 // Cut edges counted row by row, one wavefront per row (no row index, so
-// nothing to allocate); the same integer as k_cut.
+// nothing to allocate); the same integer as k_cut_bits.
 __global__ void __launch_bounds__(BLOCK) k_cut_rows(uint32_t n, const uint32_t* __restrict__ rowptr,
                                                     const uint32_t* __restrict__ col,
                                                     const uint32_t* __restrict__ part,

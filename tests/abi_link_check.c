@@ -42,6 +42,7 @@ int main(void)
     int (*fp_mi_fiedler)(mi_ctx*, const mi_hier*, uint64_t*, uint64_t, int, double, double*, mi_stats*);
     int (*fp_mi_power_iter)(mi_ctx*, const mi_graph*, double*, int, double, uint64_t*, int*);
     int (*fp_mi_compute_partition)(mi_ctx*, const mi_graph*, const double*, int, int, uint32_t*, uint32_t*);
+    int (*fp_mi_ctx_set_bucketed_build)(mi_ctx*, uint32_t);
     int (*fp_mi_compute_partition_noalloc)(mi_ctx*, const mi_graph*, const double*, int, int, uint32_t*,
                                            uint32_t*);
     int (*fp_mi_partition_graph)(mi_ctx*, const mi_graph*, const mi_config*, int, uint64_t, uint64_t*, uint64_t,
@@ -82,6 +83,7 @@ int main(void)
     BIND(mi_power_iter);
     BIND(mi_compute_partition);
     BIND(mi_compute_partition_noalloc);
+    BIND(mi_ctx_set_bucketed_build);
     BIND(mi_partition_graph);
     BIND(mi_use_partition);
     BIND(mi_partition_4way);
