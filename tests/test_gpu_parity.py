@@ -271,6 +271,27 @@ def test_power_iter_column_blocks_vs_oracle(ctx):
     assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
 
 
+def test_power_iter_many_column_blocks_vs_oracle(ctx):
+    """The whole power loop (mode 0) on a level of more than 8 column blocks,
+    whose block bounds the kernels read from device memory: 180,000
+    vertices in 10 blocks of 18,000, hub rows crossing ranges and blocks.
+    (normLap = 1 is left to the other graphs: without deflation inside the
+    loop, as the reference runs it, this graph's iteration drifts to the
+    trivial eigenvector at a rate set by rounding noise, so its count
+    depends on the summation order.)"""
+    normLap = 0
+    rp, col, w = _dense_graph(180000, 66, 17, True)
+    g = ctx.graph(rp, col, w)
+    assert g.bench_step(1)["column_blocks"] == 10
+    u0 = np.random.default_rng(18).random(len(rp) - 1)
+    u, it, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+    u2, it2, _ = g.power_iter(u0, normLap=normLap, tol=1e-10)
+    assert it == it2 and np.array_equal(u, u2)  # fixed-order reductions
+    ou, oit, _, _ = oracle.power_iter(rp, col, w, u0, normLap=normLap, tol=1e-10)
+    assert abs(it - oit) <= max(2, ITER_SLACK * oit)
+    assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
+
+
 @pytest.mark.parametrize("normLap", [0, 1])
 def test_power_iter_graph_replay_equals_eager(ctx, normLap):
     """Iterations replayed from the captured HIP graph give exactly what
