@@ -75,6 +75,7 @@ constexpr int LBLOCK = 1024;            // workgroup of the LDS form
 constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
 constexpr uint32_t XBM = 19456;         // ... with more than KB column blocks (152 KiB: fewer blocks)
+constexpr double SPAN_ROW_COST = 4.0;   // cost of a row that begins, in entries (spans of the LDS form)
 constexpr int MAXCB = MAX_COLUMN_BLOCKS; // column blocks at most
 constexpr int KB = 8;                   // blocks whose bounds travel in the kernel arguments
 constexpr int RCB = 8;                  // blocks a finish thread handles per group (loads in flight)
@@ -317,6 +318,33 @@ __global__ void k_block_starts(const uint32_t* __restrict__ vrowptr, uint32_t n,
     for (uint32_t q = threadIdx.x; q <= ncb; q += blockDim.x) out[q] = vrowptr[(size_t)q * n];
 }
 
+// Rows that begin inside range w (one wavefront per range): the cost
+// model's second term (a row that ends costs a store and the lane-segment
+// bookkeeping, so ranges of short rows take longer than ranges of long
+// ones with as many entries).
+__global__ void k_range_rows(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t w = (uint32_t)(t / WAVE), lane = threadIdx.x & (WAVE - 1);
+    if (w >= nwaves) return;
+    uint32_t q = 0;
+    if (B.ncb > KB) {
+        q = last_le(B.bwave_all, B.ncb, w);
+    } else {
+#pragma unroll
+        for (int k = 1; k < KB; k++)
+            if ((uint32_t)k < B.ncb && w >= B.bwave[k]) q = k;
+    }
+    const uint32_t bo = B.ncb > KB ? B.boff_all[q] : B.boff[q];
+    const uint32_t bo1 = B.ncb > KB ? B.boff_all[q + 1] : (B.ncb > 1 ? B.boff[q + 1] : B.m);
+    const uint32_t bw = B.ncb > KB ? B.bwave_all[q] : B.bwave[q];
+    const uint32_t e0 = bo + (w - bw) * B.pw, e1 = (bo1 - e0 > B.pw) ? e0 + B.pw : bo1;
+    uint32_t c = 0;
+    for (uint32_t e = e0 + 1 + lane; e < e1; e += WAVE) c += rid[e] != rid[e - 1] ? 1u : 0u;
+    c = wave_sum_u32(c);
+    if (lane == 0) out[w] = c + 1u;
+}
+
 // range metadata: first row of range w, with HEADF when that row began in
 // an earlier range
 __global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ wmeta)
@@ -393,6 +421,7 @@ struct StepArgs {
     unsigned long long* niter;
     uint32_t* hdone;         // done flag in mapped host memory (mode 0 loop), or null
     double* ax;              // FP64 loop: A x of every iteration's input row by row, or null
+    const uint32_t* span;    // LDS form: workgroup g streams ranges [span[g], span[g+1]), or null (even split)
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -670,12 +699,104 @@ __device__ __forceinline__ void put(const StepArgs& A, uint32_t r, V v, uint32_t
     if (r != RINF) *(r == hrow ? static_cast<V*>(A.head) + wv : static_cast<V*>(A.part) + r) = v;
 }
 
+// The 16-byte loads of a lane's 8 entries at E0 + lane SEG of [E0, E1)
+// (false: they are not all below E1, and are loaded one by one where used).
+template <int WF, bool LDSX, bool R16>
+__device__ __forceinline__ bool fetch_at(const StepArgs& A, uint32_t E0, uint32_t E1, Raw& R)
+{
+    using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
+    using RidT = typename std::conditional<R16, uint16_t, uint32_t>::type;
+    using WT = typename std::conditional<WF == W_U16, uint16_t, uint32_t>::type;
+    return fetch<WF, LDSX, R16>(static_cast<const ColT*>(A.col), static_cast<const RidT*>(A.rid),
+                                static_cast<const WT*>(A.w), E0 + (threadIdx.x & (WAVE - 1)) * SEG, E1, R);
+}
+
+// The ranges a wavefront streams: wv, wv + step, ... below end, or, with a
+// shared counter (next, in LDS), whichever range the counter hands out
+// next; range wv holds the entries [off0 + (wv - wbase) pw, min(that + pw,
+// off1)).
+struct RangeSeq {
+    uint32_t off0, off1, wbase, pw, step, end;
+    uint32_t* next;
+    __device__ __forceinline__ void bounds(uint32_t wv, uint32_t& E0, uint32_t& E1) const
+    {
+        E0 = off0 + (wv - wbase) * pw;
+        E1 = (off1 - E0 > pw) ? E0 + pw : off1;
+    }
+};
+
+// The loads of the next D passes of a wavefront's sequence of passes
+// (range by range), issued D passes ahead of their use, so a wavefront
+// keeps D passes of 16-byte loads in flight, across range boundaries too;
+// (wv, e, E1) is the next pass to load (none when wv >= end).  D = 2 on
+// the persistent LDS form, whose 16 wavefronts per CU need the deeper queue
+// to cover the memory latency; D = 1 where registers are scarcer.
+template <int D>
+struct PassQueue {
+    Raw q[D];
+    bool f[D];
+    uint32_t r[D];  // range of each queued pass (>= end: none)
+    uint32_t wv, e, E1;
+};
+
+// Issues the loads of the pass at the cursor into q[D-1] and moves the cursor.
+template <int WF, bool LDSX, bool R16, int D>
+__device__ __forceinline__ void pq_load(const StepArgs& A, PassQueue<D>& Q, const RangeSeq& S)
+{
+    Q.r[D - 1] = Q.wv;
+    if (Q.wv >= S.end) {
+        Q.f[D - 1] = false;
+        return;
+    }
+    Q.f[D - 1] = fetch_at<WF, LDSX, R16>(A, Q.e, Q.E1, Q.q[D - 1]);
+    Q.e += PASS;
+    if (Q.e >= Q.E1) {
+        if (S.next) {
+            uint32_t v = 0;
+            if ((threadIdx.x & (WAVE - 1)) == 0) v = atomicAdd(S.next, 1u);
+            Q.wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        } else {
+            Q.wv += S.step;
+        }
+        if (Q.wv < S.end) S.bounds(Q.wv, Q.e, Q.E1);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void pq_shift(PassQueue<D>& Q)
+{
+#pragma unroll
+    for (int k = 0; k + 1 < D; k++) {
+        Q.q[k] = Q.q[k + 1];
+        Q.f[k] = Q.f[k + 1];
+        Q.r[k] = Q.r[k + 1];
+    }
+}
+
+// The queue at the first pass of range `first`.
+template <int WF, bool LDSX, bool R16, int D>
+__device__ __forceinline__ void pq_init(const StepArgs& A, PassQueue<D>& Q, const RangeSeq& S, uint32_t first)
+{
+    Q.wv = first;
+    Q.e = Q.E1 = 0;
+    if (first < S.end) S.bounds(first, Q.e, Q.E1);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        if (k) pq_shift(Q);
+        pq_load<WF, LDSX, R16>(A, Q, S);
+    }
+}
+
 // Row sums of the entries [E0, E1) by one wavefront (range index wv; nrow =
 // row of entry E1 when it continues this block, else RINF).  x_j is read as
 // (yprev[j] - a) s, or from LDS (already shifted and scaled) in the LDS form.
-template <int WF, bool LDSX, bool R16, class V>
+// Q holds the loads of the range's first D passes (the later ones may
+// belong to the next ranges): each pass takes q[0] and issues the loads D
+// passes ahead (pq_load).
+template <int WF, bool LDSX, bool R16, class V, int D>
 __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uint32_t E0, uint32_t E1,
-                                             uint32_t nrow, double a, double s, const V* xl)
+                                             uint32_t nrow, double a, double s, const V* xl, PassQueue<D>& Q,
+                                             const RangeSeq& S)
 {
     const V* yp = static_cast<const V*>(A.yprev);
     using ColT = typename std::conditional<LDSX, uint16_t, uint32_t>::type;
@@ -691,16 +812,19 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
     const uint32_t hrow = (meta & HEADF) ? row0 : RINF;
     uint32_t cr = hrow;  // row carried into the pass (RINF: none)
     V cs = 0;            // its partial sum so far
-    // The 16-byte loads of pass p+1 are issued before pass p is processed.
-    Raw nx;
-    bool nfull = fetch<WF, LDSX, R16>(col, rid, w, E0 + lane * SEG, E1, nx);
     for (uint32_t base = E0; base < E1; base += PASS) {
         const uint32_t e0 = base + lane * SEG;
         // a full pass: every lane holds 8 entries (wavefront-uniform)
         const bool full = base + PASS <= E1;
+        const Raw cur = Q.q[0];
+        const bool cfull = Q.f[0];
+        if (D > 1) {
+            pq_shift(Q);
+            pq_load<WF, LDSX, R16>(A, Q, S);
+        }
         uint32_t rr[SEG], cc[SEG], ww[SEG];
-        if (nfull) {
-            unpack<WF, LDSX, R16>(nx, rr, cc, ww);
+        if (cfull) {
+            unpack<WF, LDSX, R16>(cur, rr, cc, ww);
 #pragma unroll
             for (int u = 0; u < SEG; u++) rr[u] += rbase;
         } else {
@@ -712,15 +836,15 @@ __device__ __forceinline__ void stream_range(const StepArgs& A, uint32_t wv, uin
                 if (WF != W_UNIT) ww[u] = in ? (uint32_t)w[e0 + u] : 0u;
             }
         }
+        if (D == 1) pq_load<WF, LDSX, R16>(A, Q, S);
         const uint32_t nbase = base + PASS;
-        if (nbase < E1) nfull = fetch<WF, LDSX, R16>(col, rid, w, nbase + lane * SEG, E1, nx);
         // row of the entry after lane 63's last one: lane 0's first entry of
-        // the next pass, or the next range's first row (in a partial pass
-        // lane 63 holds no entries)
+        // the next pass (whose loads q0 holds), or the next range's first
+        // row (in a partial pass lane 63 holds no entries)
         uint32_t nx63 = nrow;
         if (nbase < E1) {
             if (nbase + SEG <= E1)
-                nx63 = (uint32_t)__builtin_amdgcn_readlane((int)first_id<R16>(nx), 0) + rbase;
+                nx63 = (uint32_t)__builtin_amdgcn_readlane((int)first_id<R16>(Q.q[0]), 0) + rbase;
             else
                 nx63 = (uint32_t)rid[nbase] + rbase;
         }
@@ -804,7 +928,10 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
     const uint64_t E0 = (uint64_t)wv * A.B.pw;
     const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
     const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-    stream_range<WF, false, R16, V>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr);
+    const RangeSeq S{0u, A.m, 0u, A.B.pw, 1u, wv + 1, nullptr};
+    PassQueue<1> Q;
+    pq_init<WF, false, R16>(A, Q, S, wv);
+    stream_range<WF, false, R16, V>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr, Q, S);
 }
 
 // Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
@@ -821,9 +948,10 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     constexpr int XPT = XN / LBLOCK;  // staged values per thread
     const V* yp = static_cast<const V*>(A.yprev);
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
+    __shared__ uint32_t next_range;
     const uint32_t G = gridDim.x, g = logical_block(), wid = threadIdx.x / WAVE;
-    const uint32_t R1 = (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
-    uint32_t cur = (uint32_t)((uint64_t)g * A.nwaves / G);
+    const uint32_t R1 = A.span ? A.span[g + 1] : (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
+    uint32_t cur = A.span ? A.span[g] : (uint32_t)((uint64_t)g * A.nwaves / G);
     // The slice of y for the first block is loaded before the previous
     // iteration's sums are reduced, so the two loads overlap.
     V raw[XPT];
@@ -881,15 +1009,22 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
                 if (i < xn) xl[i] = xform(raw[j], ad, sd);
             }
         } else {
-            __syncthreads();  // the previous block's gathers are done
+            __syncthreads();  // the previous block's gathers (and counter reads) are done
             for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = xform(yp[x0 + i], ad, sd);
         }
+        if (threadIdx.x == 0) next_range = cur + LNW;
         __syncthreads();
-        for (uint32_t wv = cur + wid; wv < seg_end; wv += LNW) {
-            const uint32_t E0 = off0 + (wv - wbase) * A.B.pw;
-            const uint32_t E1 = (off1 - E0 > A.B.pw) ? E0 + A.B.pw : off1;
+        // wavefront wid starts at range cur + wid and then takes whichever
+        // range of the segment the shared counter hands out next, so the
+        // wavefronts finish together however the ranges' costs vary
+        const RangeSeq S{off0, off1, wbase, A.B.pw, (uint32_t)LNW, seg_end, &next_range};
+        PassQueue<2> Q;
+        pq_init<WF, true, R16>(A, Q, S, cur + wid);
+        for (uint32_t wv = Q.r[0]; wv < seg_end; wv = Q.r[0]) {
+            uint32_t E0, E1;
+            S.bounds(wv, E0, E1);
             const uint32_t nrow = (wv + 1 < wend) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-            stream_range<WF, true, R16, V>(A, wv, E0, E1, nrow, ad, sd, xl);
+            stream_range<WF, true, R16, V>(A, wv, E0, E1, nrow, ad, sd, xl, Q, S);
         }
         cur = seg_end;
     };
@@ -1139,11 +1274,16 @@ __global__ void __launch_bounds__(LBLOCK) k_power_small(StepArgs A, double* buf0
         double* yn = (iter & 1u) ? buf1 : buf0;
         for (uint32_t i = tid; i < A.n; i += LBLOCK) xl[i] = (yp[i] - a) * s;
         __syncthreads();
-        for (uint32_t wv = wid; wv < A.nwaves; wv += LNW) {
-            const uint32_t E0 = wv * A.B.pw;
-            const uint32_t E1 = (A.m - E0 > A.B.pw) ? E0 + A.B.pw : A.m;
-            const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-            stream_range<WF, true, R16, double>(A, wv, E0, E1, nrow, a, s, xl);
+        {
+            const RangeSeq S{0u, A.m, 0u, A.B.pw, (uint32_t)LNW, A.nwaves, nullptr};
+            PassQueue<1> Q;
+            pq_init<WF, true, R16>(A, Q, S, wid);
+            for (uint32_t wv = wid; wv < A.nwaves; wv += LNW) {
+                uint32_t E0, E1;
+                S.bounds(wv, E0, E1);
+                const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+                stream_range<WF, true, R16, double>(A, wv, E0, E1, nrow, a, s, xl, Q, S);
+            }
         }
         __syncthreads();
         double S[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1336,11 +1476,16 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
         // its final barrier)
         for (uint32_t i = tid; i < n; i += LBLOCK) xl[i] = (yl[i] - a) * s;
         __syncthreads();
-        for (uint32_t wv = wid; wv < nw; wv += LNW) {
-            const uint32_t E0 = wv * A.B.pw;
-            const uint32_t E1 = (A.m - E0 > A.B.pw) ? E0 + A.B.pw : A.m;
-            const uint32_t nrow = (wv + 1 < nw) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-            stream_range<WF, true, R16, double>(B, wv, E0, E1, nrow, a, s, xl);
+        {
+            const RangeSeq S{0u, A.m, 0u, A.B.pw, (uint32_t)LNW, nw, nullptr};
+            PassQueue<1> Q;
+            pq_init<WF, true, R16>(B, Q, S, wid);
+            for (uint32_t wv = wid; wv < nw; wv += LNW) {
+                uint32_t E0, E1;
+                S.bounds(wv, E0, E1);
+                const uint32_t nrow = (wv + 1 < nw) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+                stream_range<WF, true, R16, double>(B, wv, E0, E1, nrow, a, s, xl, Q, S);
+            }
         }
         __syncthreads();
         double S[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1797,6 +1942,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.niter = c.niter;
     A.hdone = nullptr;
     A.ax = nullptr;
+    A.span = p.span.get();
     return A;
 }
 
@@ -2067,6 +2213,39 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         // rows they fit without asking the device)
         uint32_t mx = 0;
         if (nrows > 0xffffu) read_words(c, maxd.get(), &mx, 4);
+        // the persistent LDS form: contiguous spans of ranges per workgroup
+        // of about equal estimated cost (entries + SPAN_ROW_COST per row
+        // that begins), since ranges of short rows take longer than ranges
+        // of long rows with as many entries
+        if (ncb >= 1 && p.nblk_a > 1) {
+            DBuf<uint32_t> rows_w(&c.pool, p.nwaves);
+            k_range_rows<<<(unsigned)ceil_div((uint64_t)p.nwaves * WAVE, BLOCK), BLOCK, 0, c.stream>>>(
+                B, p.nwaves, rid32.get(), rows_w.get());
+            MI_HIP(hipGetLastError());
+            std::vector<uint32_t> hr(p.nwaves);
+            MI_HIP(hipMemcpyAsync(hr.data(), rows_w.get(), (size_t)p.nwaves * 4, hipMemcpyDeviceToHost, c.stream));
+            MI_HIP(hipStreamSynchronize(c.stream));
+            std::vector<double> cost(p.nwaves + 1, 0.0);
+            uint32_t q = 0;
+            for (uint32_t w = 0; w < p.nwaves; w++) {
+                while (q + 1 < ncb && w >= p.bwave[q + 1]) q++;
+                const uint32_t e0 = p.boff[q] + (w - p.bwave[q]) * p.pw;
+                const uint32_t e1 = std::min<uint32_t>(e0 + p.pw, p.boff[q + 1]);  // (boff[1] = m with one block)
+                cost[w + 1] = cost[w] + (double)(e1 - e0) + SPAN_ROW_COST * hr[w];
+            }
+            const uint32_t G = p.nblk_a;
+            std::vector<uint32_t> sp(G + 1);
+            sp[0] = 0;
+            sp[G] = p.nwaves;
+            for (uint32_t k = 1; k < G; k++) {
+                const double target = cost[p.nwaves] * k / G;
+                sp[k] = (uint32_t)(std::lower_bound(cost.begin(), cost.end(), target) - cost.begin());
+                sp[k] = std::min(std::max(sp[k], sp[k - 1]), p.nwaves);
+            }
+            p.span = DBuf<uint32_t>(&c.pool, G + 1);
+            MI_HIP(hipMemcpyAsync(p.span.get(), sp.data(), (size_t)(G + 1) * 4, hipMemcpyHostToDevice, c.stream));
+            MI_HIP(hipStreamSynchronize(c.stream));
+        }
         if (mx > 0xffffu) {
             p.rid16.reset();
             p.rid = std::move(rid32);
