@@ -76,6 +76,9 @@ constexpr int LNW = LBLOCK / WAVE;      // its wavefronts
 constexpr uint32_t XB = 16384;          // doubles of x a workgroup stages in LDS (128 KiB)
 constexpr uint32_t XBM = 19456;         // ... with more than KB column blocks (152 KiB: fewer blocks)
 constexpr double SPAN_ROW_COST = 4.0;   // cost of a row that begins, in entries (spans of the LDS form)
+constexpr double SHARED_PART = 0.2;     // share of every span handed out to whichever workgroup is free
+constexpr uint32_t SHARED_CHUNKS = 4;   // chunks per span's shared part
+constexpr uint32_t SHARED_MIN_ENTRIES = 1u << 19;  // ... on levels of at least this many entries per workgroup
 constexpr int MAXCB = MAX_COLUMN_BLOCKS; // column blocks at most
 constexpr int KB = 8;                   // blocks whose bounds travel in the kernel arguments
 constexpr int RCB = 8;                  // blocks a finish thread handles per group (loads in flight)
@@ -421,7 +424,8 @@ struct StepArgs {
     unsigned long long* niter;
     uint32_t* hdone;         // done flag in mapped host memory (mode 0 loop), or null
     double* ax;              // FP64 loop: A x of every iteration's input row by row, or null
-    const uint32_t* span;    // LDS form: workgroup g streams ranges [span[g], span[g+1]), or null (even split)
+    const uint32_t* span;    // LDS form: the schedule table (see plan_level), or null (even split)
+    uint32_t* sched;         // ... its two counters: next shared chunk, workgroups done
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -729,8 +733,10 @@ struct RangeSeq {
 // (range by range), issued D passes ahead of their use, so a wavefront
 // keeps D passes of 16-byte loads in flight, across range boundaries too;
 // (wv, e, E1) is the next pass to load (none when wv >= end).  D = 2 on
-// the persistent LDS form, whose 16 wavefronts per CU need the deeper queue
-// to cover the memory latency; D = 1 where registers are scarcer.
+// the persistent LDS form of the largest levels (more than KB column
+// blocks), whose 16 wavefronts per CU need the deeper queue to cover the
+// memory latency (measured: R-MAT-24 level 1, 42 blocks, 1.5 % faster);
+// D = 1 elsewhere (R-MAT-20 level 1, 5 blocks: 5 % faster with D = 1).
 template <int D>
 struct PassQueue {
     Raw q[D];
@@ -948,9 +954,11 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     constexpr int XPT = XN / LBLOCK;  // staged values per thread
     const V* yp = static_cast<const V*>(A.yprev);
     if (A.mode == 0 && A.cnt[CT_DONE]) return;
-    __shared__ uint32_t next_range;
+    __shared__ uint32_t next_range, chunk;
     const uint32_t G = gridDim.x, g = logical_block(), wid = threadIdx.x / WAVE;
-    const uint32_t R1 = A.span ? A.span[g + 1] : (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
+    // own ranges [cur, R1): with a schedule table, the first part of the
+    // workgroup's span (the rest of every span is shared out in chunks)
+    uint32_t R1 = A.span ? A.span[G + 1 + g] : (uint32_t)((uint64_t)(g + 1) * A.nwaves / G);
     uint32_t cur = A.span ? A.span[g] : (uint32_t)((uint64_t)g * A.nwaves / G);
     // The slice of y for the first block is loaded before the previous
     // iteration's sums are reduced, so the two loads overlap.
@@ -976,7 +984,8 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     if (!iteration_start<LNW, NS>(A, ad, sd, red)) return;
     // one segment of ranges inside one column block; the first one stages
     // from the values loaded above (their registers end there), later ones
-    // load again
+    // load again unless their block is the one staged
+    uint32_t staged = 0xffffffffu;
     auto segment = [&](bool from_raw) {
         // the column block holding range cur (static indices into the
         // argument arrays up to KB blocks)
@@ -1010,15 +1019,17 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
             }
         } else {
             __syncthreads();  // the previous block's gathers (and counter reads) are done
-            for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = xform(yp[x0 + i], ad, sd);
+            if (b != staged)
+                for (uint32_t i = threadIdx.x; i < xn; i += LBLOCK) xl[i] = xform(yp[x0 + i], ad, sd);
         }
+        staged = b;
         if (threadIdx.x == 0) next_range = cur + LNW;
         __syncthreads();
         // wavefront wid starts at range cur + wid and then takes whichever
         // range of the segment the shared counter hands out next, so the
         // wavefronts finish together however the ranges' costs vary
         const RangeSeq S{off0, off1, wbase, A.B.pw, (uint32_t)LNW, seg_end, &next_range};
-        PassQueue<2> Q;
+        PassQueue<MANY ? 2 : 1> Q;
         pq_init<WF, true, R16>(A, Q, S, cur + wid);
         for (uint32_t wv = Q.r[0]; wv < seg_end; wv = Q.r[0]) {
             uint32_t E0, E1;
@@ -1030,6 +1041,26 @@ __global__ void __launch_bounds__(LBLOCK) k_power_stream_lds(StepArgs A)
     };
     if (cur < R1) segment(true);
     while (cur < R1) segment(false);
+    if (!A.span || !A.span[2 * G + 1]) return;
+    // Then chunks of the shared ranges, handed out by a global counter to
+    // whichever workgroup asks first, until none is left; the last
+    // workgroup to finish resets the counters for the next launch.
+    const uint32_t nch = A.span[2 * G + 1];
+    const uint32_t* ch = A.span + 2 * G + 2;
+    for (;;) {
+        __syncthreads();  // (the previous chunk's word is read by every thread)
+        if (threadIdx.x == 0) chunk = atomicAdd(&A.sched[0], 1u);
+        __syncthreads();
+        const uint32_t k = chunk;
+        if (k >= nch) break;
+        cur = ch[2 * k];
+        R1 = ch[2 * k + 1];
+        while (cur < R1) segment(false);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&A.sched[1], 1u) == G - 1) {
+        atomicExch(&A.sched[0], 0u);
+        atomicExch(&A.sched[1], 0u);
+    }
 }
 
 // y_r from the row sum (the shifted product of the reference, or A x in mode 2)
@@ -1943,6 +1974,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.hdone = nullptr;
     A.ax = nullptr;
     A.span = p.span.get();
+    A.sched = p.sched.get();
     return A;
 }
 
@@ -2234,16 +2266,47 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
                 cost[w + 1] = cost[w] + (double)(e1 - e0) + SPAN_ROW_COST * hr[w];
             }
             const uint32_t G = p.nblk_a;
-            std::vector<uint32_t> sp(G + 1);
-            sp[0] = 0;
-            sp[G] = p.nwaves;
-            for (uint32_t k = 1; k < G; k++) {
-                const double target = cost[p.nwaves] * k / G;
-                sp[k] = (uint32_t)(std::lower_bound(cost.begin(), cost.end(), target) - cost.begin());
-                sp[k] = std::min(std::max(sp[k], sp[k - 1]), p.nwaves);
+            auto at_cost = [&](double target) {
+                return (uint32_t)(std::lower_bound(cost.begin(), cost.end(), target) - cost.begin());
+            };
+            // table: span starts (G + 1), own-part ends (G), the number of
+            // shared chunks, then (first, end) of every chunk; a chunk
+            // never crosses a column block
+            std::vector<uint32_t> t(2 * G + 2);
+            std::vector<uint32_t> chunks;
+            t[0] = 0;
+            t[G] = p.nwaves;
+            for (uint32_t k = 1; k < G; k++)
+                t[k] = std::min(std::max(at_cost(cost[p.nwaves] * k / G), t[k - 1]), p.nwaves);
+            uint32_t qb = 0;
+            // (a chunk costs a global atomic, two barriers and perhaps a
+            // restaging of x: only levels with long spans share them)
+            const double share = (uint64_t)m >= (uint64_t)SHARED_MIN_ENTRIES * G ? SHARED_PART : 0.0;
+            for (uint32_t k = 0; k < G; k++) {
+                const double c0 = cost[t[k]], c1 = cost[t[k + 1]];
+                const uint32_t own =
+                    share > 0 ? std::min(std::max(at_cost(c0 + (1.0 - share) * (c1 - c0)), t[k]), t[k + 1]) : t[k + 1];
+                t[G + 1 + k] = own;
+                uint32_t lo = own;
+                for (uint32_t j = 1; j <= SHARED_CHUNKS && lo < t[k + 1]; j++) {
+                    uint32_t hi = j == SHARED_CHUNKS ? t[k + 1]
+                                                     : std::min(std::max(at_cost(c0 + (1.0 - share + share * j / SHARED_CHUNKS) * (c1 - c0)), lo), t[k + 1]);
+                    while (lo < hi) {
+                        // split at column-block ends
+                        while (qb + 1 < ncb && lo >= p.bwave[qb + 1]) qb++;
+                        const uint32_t e = std::min(hi, qb + 1 < ncb ? p.bwave[qb + 1] : p.nwaves);
+                        chunks.push_back(lo);
+                        chunks.push_back(e);
+                        lo = e;
+                    }
+                }
             }
-            p.span = DBuf<uint32_t>(&c.pool, G + 1);
-            MI_HIP(hipMemcpyAsync(p.span.get(), sp.data(), (size_t)(G + 1) * 4, hipMemcpyHostToDevice, c.stream));
+            t[2 * G + 1] = (uint32_t)(chunks.size() / 2);
+            t.insert(t.end(), chunks.begin(), chunks.end());
+            p.span = DBuf<uint32_t>(&c.pool, t.size());
+            p.sched = DBuf<uint32_t>(&c.pool, 2);
+            MI_HIP(hipMemcpyAsync(p.span.get(), t.data(), t.size() * 4, hipMemcpyHostToDevice, c.stream));
+            MI_HIP(hipMemsetAsync(p.sched.get(), 0, 8, c.stream));
             MI_HIP(hipStreamSynchronize(c.stream));
         }
         if (mx > 0xffffu) {

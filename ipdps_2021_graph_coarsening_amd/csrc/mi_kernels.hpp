@@ -142,7 +142,8 @@ struct LevelPlan {
     DBuf<uint32_t> blk;  // boff then bwave on the device, when more than 8 blocks
     uint32_t pw = 0, nwaves = 0;
     uint32_t nblk_a = 0;  // workgroups of the streaming kernel
-    DBuf<uint32_t> span;  // LDS form: each workgroup's ranges, split by estimated cost (nblk_a + 1)
+    DBuf<uint32_t> span;  // LDS form: the workgroups' schedule (own ranges, shared chunks)
+    DBuf<uint32_t> sched; // ... its counters (zero between launches)
     uint32_t nblk_f = 0;  // workgroups of the finish kernel
     bool f32 = false;     // FP32 vectors and row partials (mixed precision)
 };
