@@ -325,7 +325,8 @@ __global__ void k_block_starts(const uint32_t* __restrict__ vrowptr, uint32_t n,
 // model's second term (a row that ends costs a store and the lane-segment
 // bookkeeping, so ranges of short rows take longer than ranges of long
 // ones with as many entries).
-__global__ void k_range_rows(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ out)
+template <class RidT>
+__global__ void k_range_rows(Blocks B, uint32_t nwaves, const RidT* __restrict__ rid, uint32_t* __restrict__ out)
 {
     const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t w = (uint32_t)(t / WAVE), lane = threadIdx.x & (WAVE - 1);
@@ -348,9 +349,11 @@ __global__ void k_range_rows(Blocks B, uint32_t nwaves, const uint32_t* __restri
     if (lane == 0) out[w] = c + 1u;
 }
 
-// range metadata: first row of range w, with HEADF when that row began in
-// an earlier range
-__global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ rid, uint32_t* __restrict__ wmeta)
+// first row of range w straight from the row index (no per-entry rows):
+// the row holding the range's first entry, with HEADF when it began earlier
+// in the same block
+__global__ void k_wmeta_rows(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ roc,
+                             const uint32_t* __restrict__ rows, uint32_t* __restrict__ wmeta)
 {
     const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
     if (w >= nwaves) return;
@@ -368,25 +371,48 @@ __global__ void k_wmeta(Blocks B, uint32_t nwaves, const uint32_t* __restrict__ 
             }
         }
     }
-    const uint32_t r = rid[e0];
-    wmeta[w] = r | ((e0 > bstart && rid[e0 - 1] == r) ? HEADF : 0u);
+    const uint32_t r = entry_row(e0, roc, rows);
+    wmeta[w] = r | ((e0 > bstart && rows[r] < e0) ? HEADF : 0u);
 }
 
-// 16-bit row ids relative to the first row of each range; *maxd receives the
-// largest offset (the plan keeps the 32-bit ids when it does not fit)
-__global__ void __launch_bounds__(BLOCK) k_rid16(Blocks B, const uint32_t* __restrict__ rid,
-                                                 const uint32_t* __restrict__ wmeta, uint16_t* __restrict__ rid16,
-                                                 uint32_t* __restrict__ maxd)
+// 16-bit row ids relative to the first row of each range, from the row
+// index; *maxd receives the largest offset (the plan keeps 32-bit ids when
+// it does not fit)
+__global__ void __launch_bounds__(BLOCK) k_rid16_rows(Blocks B, const uint32_t* __restrict__ roc,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint32_t* __restrict__ wmeta,
+                                                      uint16_t* __restrict__ rid16, uint32_t* __restrict__ maxd)
 {
     __shared__ uint32_t lds[BLOCK / WAVE];
     uint32_t mx = 0;
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < B.m; e += (uint64_t)gridDim.x * BLOCK) {
-        const uint32_t d = rid[e] - (wmeta[range_of(B, (uint32_t)e)] & ~HEADF);
+        const uint32_t d = entry_row((uint32_t)e, roc, rows) - (wmeta[range_of(B, (uint32_t)e)] & ~HEADF);
         rid16[e] = (uint16_t)d;
         mx = d > mx ? d : mx;
     }
     mx = block_max_u32(mx, lds);
     if (threadIdx.x == 0) atomicMax(maxd, mx);
+}
+
+// Integer weighted degrees entry-parallel from a plan's row ids (rows in
+// entry order: the one-block and global forms): a wavefront's 64
+// consecutive entries summed row by row (segmented), one atomic per row
+// piece -- integer sums, so the order of the atomics does not matter.
+template <bool R16>
+__global__ void __launch_bounds__(BLOCK) k_wdeg_entries(Blocks B, const void* __restrict__ rid,
+                                                        const uint32_t* __restrict__ wmeta,
+                                                        const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t key = INF, v = 0;
+    if (e < B.m) {
+        key = R16 ? (wmeta[range_of(B, (uint32_t)e)] & ~HEADF) + static_cast<const uint16_t*>(rid)[e]
+                  : static_cast<const uint32_t*>(rid)[e];
+        v = w[e];
+    }
+    bool head;
+    const uint32_t t = wave_seg_reduce(key, v, [](uint32_t x, uint32_t y) { return x + y; }, &head);
+    if (head && key != INF) atomicAdd(&wdeg[key], t);
 }
 
 // ------------------------------------------------------------------ step
@@ -929,15 +955,19 @@ __global__ void __launch_bounds__(BLOCK) k_power_stream(StepArgs A)
     __shared__ double red[NS][NW];
     double a, s;
     if (!iteration_start<NW, NS>(A, a, s, red)) return;
-    const uint32_t wv = logical_block() * NW + threadIdx.x / WAVE;
-    if (wv >= A.nwaves) return;
-    const uint64_t E0 = (uint64_t)wv * A.B.pw;
-    const uint64_t E1 = (E0 + A.B.pw < A.m) ? E0 + A.B.pw : A.m;
-    const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
-    const RangeSeq S{0u, A.m, 0u, A.B.pw, 1u, wv + 1, nullptr};
+    // persistent: wavefront k of the grid streams ranges k, k + (waves of
+    // the grid), ..., its loads queued across them
+    const uint32_t nw_grid = gridDim.x * NW;
+    const uint32_t first = logical_block() * NW + threadIdx.x / WAVE;
+    const RangeSeq S{0u, A.m, 0u, A.B.pw, nw_grid, A.nwaves, nullptr};
     PassQueue<1> Q;
-    pq_init<WF, false, R16>(A, Q, S, wv);
-    stream_range<WF, false, R16, V>(A, wv, (uint32_t)E0, (uint32_t)E1, nrow, a, s, nullptr, Q, S);
+    pq_init<WF, false, R16>(A, Q, S, first);
+    for (uint32_t wv = first; wv < A.nwaves; wv += nw_grid) {
+        uint32_t E0, E1;
+        S.bounds(wv, E0, E1);
+        const uint32_t nrow = (wv + 1 < A.nwaves) ? (A.wmeta[wv + 1] & ~HEADF) : RINF;
+        stream_range<WF, false, R16, V>(A, wv, E0, E1, nrow, a, s, nullptr, Q, S);
+    }
 }
 
 // Persistent LDS form: one workgroup per CU; workgroup g takes the ranges
@@ -1644,16 +1674,44 @@ __global__ void k_to_f32(uint32_t n, const double* __restrict__ x, float* __rest
 
 // weighted degrees as u32 sums (wrapping like the reference's sgp_wgt_t),
 // one wavefront per row: exact whatever the vector precision
+// one wavefront per row; a row of more than WDEG_LONG entries is listed
+// (order free: integer sums) for k_wdeg_long instead, so no wavefront walks
+// a hub row alone
+constexpr uint32_t WDEG_LONG = 2048;
 __global__ void __launch_bounds__(BLOCK) k_wdeg_int(uint32_t n, const uint32_t* __restrict__ rowptr,
-                                                    const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg)
+                                                    const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg,
+                                                    uint32_t* __restrict__ longs, uint32_t* __restrict__ nlong)
 {
     const uint32_t r = blockIdx.x * NW + threadIdx.x / WAVE;
     if (r >= n) return;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t e0 = rowptr[r], e1 = rowptr[r + 1];
+    if (e1 - e0 > WDEG_LONG) {
+        if (lane == 0) longs[atomicAdd(nlong, 1u)] = r;
+        return;
+    }
     uint32_t t = 0;
-    for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) t += w[e];
+    for (uint32_t e = e0 + lane; e < e1; e += WAVE) t += w[e];
     t = wave_sum_u32(t);
     if (lane == 0) wdeg[r] = t;
+}
+
+// the listed long rows, one workgroup each in turn (fixed grid)
+__global__ void __launch_bounds__(BLOCK) k_wdeg_long(const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg,
+                                                     const uint32_t* __restrict__ longs,
+                                                     const uint32_t* __restrict__ nlong)
+{
+    __shared__ uint32_t lds[BLOCK / WAVE];
+    const uint32_t cnt = *nlong;
+    for (uint32_t k = blockIdx.x; k < cnt; k += gridDim.x) {
+        const uint32_t r = longs[k];
+        uint32_t t = 0;
+        for (uint32_t e = rowptr[r] + threadIdx.x; e < rowptr[r + 1]; e += BLOCK) t += w[e];
+        t = block_sum_u32(t, lds);
+        if (threadIdx.x == 0) wdeg[r] = t;
+        __syncthreads();  // (lds reused by the next row)
+    }
 }
 
 // gb = max(2 wdeg) over given degrees (fixed grid, one atomic per workgroup)
@@ -2194,7 +2252,10 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
     // ranges and workgroups
     if (ncb == 0) {
         p.nwaves = (uint32_t)ceil_div(m, p.pw);
-        p.nblk_a = (uint32_t)ceil_div(p.nwaves, NW);
+        // persistent grid: as many workgroups as the CUs hold at once (5 of
+        // the streaming kernel's per CU), at most one wavefront per range
+        const uint32_t ncu = c.num_cu > 0 ? (uint32_t)c.num_cu : 256u;
+        p.nblk_a = (uint32_t)std::min<uint64_t>(5ull * ncu, ceil_div(p.nwaves, NW));
         p.bwave[1] = p.nwaves;
     } else {
         for (uint32_t q = 0; q < ncb; q++) {
@@ -2233,26 +2294,33 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         rg.m = m;
         rg.b_rowptr = rows;
         build_row_index(c, rg, roc);
-        k_entry_rows<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc.get(), rows, rid32.get());
-        MI_HIP(hipGetLastError());
-        k_wmeta<<<grid1(p.nwaves), BLOCK, 0, c.stream>>>(B, p.nwaves, rid32.get(), p.wmeta.get());
+        k_wmeta_rows<<<grid1(p.nwaves), BLOCK, 0, c.stream>>>(B, p.nwaves, roc.get(), rows, p.wmeta.get());
         MI_HIP(hipGetLastError());
         p.rid16 = DBuf<uint16_t>(&c.pool, m);
         MI_HIP(hipMemsetAsync(maxd.get(), 0, 4, c.stream));
-        k_rid16<<<red_grid(m), BLOCK, 0, c.stream>>>(B, rid32.get(), p.wmeta.get(), p.rid16.get(), maxd.get());
+        k_rid16_rows<<<red_grid(m), BLOCK, 0, c.stream>>>(B, roc.get(), rows, p.wmeta.get(), p.rid16.get(),
+                                                           maxd.get());
         MI_HIP(hipGetLastError());
         // (offsets are below the number of rows, so with fewer than 65536
         // rows they fit without asking the device)
         uint32_t mx = 0;
         if (nrows > 0xffffu) read_words(c, maxd.get(), &mx, 4);
+        if (mx > 0xffffu) {
+            // 32-bit row ids instead
+            k_entry_rows<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc.get(), rows, rid32.get());
+            MI_HIP(hipGetLastError());
+        }
         // the persistent LDS form: contiguous spans of ranges per workgroup
         // of about equal estimated cost (entries + SPAN_ROW_COST per row
         // that begins), since ranges of short rows take longer than ranges
         // of long rows with as many entries
         if (ncb >= 1 && p.nblk_a > 1) {
             DBuf<uint32_t> rows_w(&c.pool, p.nwaves);
-            k_range_rows<<<(unsigned)ceil_div((uint64_t)p.nwaves * WAVE, BLOCK), BLOCK, 0, c.stream>>>(
-                B, p.nwaves, rid32.get(), rows_w.get());
+            const unsigned rg_grid = (unsigned)ceil_div((uint64_t)p.nwaves * WAVE, BLOCK);
+            if (mx > 0xffffu)
+                k_range_rows<<<rg_grid, BLOCK, 0, c.stream>>>(B, p.nwaves, rid32.get(), rows_w.get());
+            else  // (offsets from the range's first row: a change marks a new row as well)
+                k_range_rows<<<rg_grid, BLOCK, 0, c.stream>>>(B, p.nwaves, p.rid16.get(), rows_w.get());
             MI_HIP(hipGetLastError());
             std::vector<uint32_t> hr(p.nwaves);
             MI_HIP(hipMemcpyAsync(hr.data(), rows_w.get(), (size_t)p.nwaves * 4, hipMemcpyDeviceToHost, c.stream));
@@ -2330,8 +2398,27 @@ void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg
         // integer row sums of the weights (the reference's u32 sums; the
         // same integers an FP64 product with a vector of ones gives, without
         // its gathers)
-        k_wdeg_int<<<(unsigned)ceil_div(g.n, NW), BLOCK, 0, c.stream>>>(g.n, g.R(), g.W(), wdeg);
-        MI_HIP(hipGetLastError());
+        if (p.ncb <= 1 && g.m && (p.rid16.get() || p.rid.get())) {
+            // the plan's row ids follow the graph's rows: entry-parallel
+            const Blocks B = blocks_of(p);
+            fill_u32(c, wdeg, g.n, 0u);
+            if (p.rid16.get())
+                k_wdeg_entries<true><<<grid1(g.m), BLOCK, 0, c.stream>>>(B, p.rid16.get(), p.wmeta.get(), g.W(),
+                                                                         wdeg);
+            else
+                k_wdeg_entries<false><<<grid1(g.m), BLOCK, 0, c.stream>>>(B, p.rid.get(), p.wmeta.get(), g.W(),
+                                                                          wdeg);
+            MI_HIP(hipGetLastError());
+        } else {
+            DBuf<uint32_t> longs(&c.pool, g.n), nlong(&c.pool, 1);
+            fill_u32(c, nlong.get(), 1, 0u);
+            k_wdeg_int<<<(unsigned)ceil_div(g.n, NW), BLOCK, 0, c.stream>>>(g.n, g.R(), g.W(), wdeg, longs.get(),
+                                                                            nlong.get());
+            MI_HIP(hipGetLastError());
+            const uint32_t ncu = c.num_cu > 0 ? (uint32_t)c.num_cu : 256u;
+            k_wdeg_long<<<4 * ncu, BLOCK, 0, c.stream>>>(g.R(), g.W(), wdeg, longs.get(), nlong.get());
+            MI_HIP(hipGetLastError());
+        }
         k_gb_from<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, wdeg, gb_dev);
     } else {
         k_wdeg_finish<<<red_grid(g.n), BLOCK, 0, c.stream>>>(g.n, g.R(), nullptr, wdeg, gb_dev);
@@ -2527,7 +2614,7 @@ static void run_steps(Ctx& c, int normLap, const StepArgs (&A)[2])
         (void)hipGetLastError();
     }
     unsigned long long issued = 0;
-    auto issue = [&](uint32_t count) {  // count: a multiple of GRAPH_ITERS, issued even
+    auto issue = [&](uint32_t count) {  // count: even; a multiple of GRAPH_ITERS with the graph
         if (graph.x) {
             for (uint32_t k = 0; k < count; k += GRAPH_ITERS) MI_HIP(hipGraphLaunch(graph.x, c.stream));
             issued += count;
@@ -2551,12 +2638,16 @@ static void run_steps(Ctx& c, int normLap, const StepArgs (&A)[2])
     MI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     MI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     const unsigned long long cap = A[0].iter_max + 4ull * GRAPH_ITERS;  // the device stops at iter_max
+    // eager launches start with chunks of 2 (a level that stops after a few
+    // iterations queues few no-op launches) and double up to GRAPH_ITERS
+    uint32_t chunk = graph.x ? GRAPH_ITERS : 2u;
     int k = 0;
-    issue(GRAPH_ITERS);
+    issue(chunk);
     MI_HIP(hipEventRecord(ev[0], c.stream));
     for (;;) {
         if (!*hdone && issued < cap) {
-            issue(GRAPH_ITERS);
+            if (issued >= 4) chunk = std::min<uint32_t>(2 * chunk, GRAPH_ITERS);
+            issue(chunk);
             MI_HIP(hipEventRecord(ev[k ^ 1], c.stream));
         }
         // wait for the older chunk (or the flag)
