@@ -377,42 +377,70 @@ __global__ void k_wmeta_rows(Blocks B, uint32_t nwaves, const uint32_t* __restri
 
 // 16-bit row ids relative to the first row of each range, from the row
 // index; *maxd receives the largest offset (the plan keeps 32-bit ids when
-// it does not fit)
+// it does not fit).  A wavefront takes RW consecutive passes of 64 entries
+// (lane l: entry base + 64 k + l, coalesced); a lane finds its row once in
+// the row index and then walks it forward.  With wdeg (rows in entry order,
+// weights w), the integer weighted degrees as well: each pass's entries are
+// summed row by row (segmented over the lanes), a row still open at the
+// pass's end is carried into the next pass, and every finished row piece is
+// added with one atomic -- so a long row costs one atomic per wavefront
+// (integer sums: the order of the atomics does not matter).
+constexpr uint32_t RW = 16;
 __global__ void __launch_bounds__(BLOCK) k_rid16_rows(Blocks B, const uint32_t* __restrict__ roc,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ wmeta,
-                                                      uint16_t* __restrict__ rid16, uint32_t* __restrict__ maxd)
+                                                      uint16_t* __restrict__ rid16, uint32_t* __restrict__ maxd,
+                                                      const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg)
 {
     __shared__ uint32_t lds[BLOCK / WAVE];
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wave = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t base = wave * (uint64_t)(RW * WAVE);
     uint32_t mx = 0;
-    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < B.m; e += (uint64_t)gridDim.x * BLOCK) {
-        const uint32_t d = entry_row((uint32_t)e, roc, rows) - (wmeta[range_of(B, (uint32_t)e)] & ~HEADF);
-        rid16[e] = (uint16_t)d;
-        mx = d > mx ? d : mx;
+    uint32_t r = 0, rend = 0;
+    uint32_t carry_r = INF, carry_v = 0;  // (wave-uniform) row open at the previous pass's end
+    if (base < B.m) {
+        const uint32_t e_first = (uint32_t)base + lane;
+        if (e_first < B.m) {
+            r = entry_row(e_first, roc, rows);
+            rend = rows[r + 1];
+        }
+        for (uint32_t k = 0; k < RW; k++) {
+            const uint64_t e64 = base + (uint64_t)k * WAVE + lane;
+            if (base + (uint64_t)k * WAVE >= B.m) break;  // (wave-uniform)
+            const bool valid = e64 < B.m;
+            const uint32_t e = (uint32_t)e64;
+            if (valid) {
+                while (e >= rend) {
+                    r++;
+                    rend = rows[r + 1];
+                }
+                const uint32_t d = r - (wmeta[range_of(B, e)] & ~HEADF);
+                rid16[e] = (uint16_t)d;
+                mx = d > mx ? d : mx;
+            }
+            if (wdeg) {
+                const uint32_t key = valid ? r : INF;
+                bool head;
+                uint32_t t = wave_seg_reduce(key, valid ? w[e] : 0u, [](uint32_t x, uint32_t y) { return x + y; },
+                                             &head);
+                if (lane == 0 && key == carry_r) t += carry_v;
+                // the last segment stays open when its row goes on past this pass
+                const uint32_t key63 = (uint32_t)__builtin_amdgcn_readlane((int)key, WAVE - 1);
+                const bool more = k + 1 < RW && base + (uint64_t)(k + 1) * WAVE < B.m;
+                const uint32_t cont63 = (lane == WAVE - 1 && valid && more && rend > e + 1) ? 1u : 0u;
+                const bool open = __builtin_amdgcn_readlane((int)cont63, WAVE - 1) != 0;
+                const unsigned long long last = __ballot(key == key63);
+                const uint32_t h = (uint32_t)__ffsll((long long)last) - 1u;  // head lane of the last segment
+                const uint32_t tv = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)h);
+                if (head && key != INF && !(open && lane == h)) atomicAdd(&wdeg[key], t);
+                carry_r = open ? key63 : INF;
+                carry_v = open ? tv : 0u;
+            }
+        }
     }
     mx = block_max_u32(mx, lds);
     if (threadIdx.x == 0) atomicMax(maxd, mx);
-}
-
-// Integer weighted degrees entry-parallel from a plan's row ids (rows in
-// entry order: the one-block and global forms): a wavefront's 64
-// consecutive entries summed row by row (segmented), one atomic per row
-// piece -- integer sums, so the order of the atomics does not matter.
-template <bool R16>
-__global__ void __launch_bounds__(BLOCK) k_wdeg_entries(Blocks B, const void* __restrict__ rid,
-                                                        const uint32_t* __restrict__ wmeta,
-                                                        const uint32_t* __restrict__ w, uint32_t* __restrict__ wdeg)
-{
-    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t key = INF, v = 0;
-    if (e < B.m) {
-        key = R16 ? (wmeta[range_of(B, (uint32_t)e)] & ~HEADF) + static_cast<const uint16_t*>(rid)[e]
-                  : static_cast<const uint32_t*>(rid)[e];
-        v = w[e];
-    }
-    bool head;
-    const uint32_t t = wave_seg_reduce(key, v, [](uint32_t x, uint32_t y) { return x + y; }, &head);
-    if (head && key != INF) atomicAdd(&wdeg[key], t);
 }
 
 // ------------------------------------------------------------------ step
@@ -2298,8 +2326,14 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         MI_HIP(hipGetLastError());
         p.rid16 = DBuf<uint16_t>(&c.pool, m);
         MI_HIP(hipMemsetAsync(maxd.get(), 0, 4, c.stream));
-        k_rid16_rows<<<red_grid(m), BLOCK, 0, c.stream>>>(B, roc.get(), rows, p.wmeta.get(), p.rid16.get(),
-                                                           maxd.get());
+        // the weighted degrees in the same pass where the rows are the
+        // graph's own (one block, or the global form)
+        if (ncb <= 1 && !g.unit) {
+            p.wdeg = DBuf<uint32_t>(&c.pool, n);
+            fill_u32(c, p.wdeg.get(), n, 0u);
+        }
+        k_rid16_rows<<<(unsigned)ceil_div(ceil_div(m, RW * WAVE), NW), BLOCK, 0, c.stream>>>(
+            B, roc.get(), rows, p.wmeta.get(), p.rid16.get(), maxd.get(), g.W(), p.wdeg.get());
         MI_HIP(hipGetLastError());
         // (offsets are below the number of rows, so with fewer than 65536
         // rows they fit without asking the device)
@@ -2398,17 +2432,9 @@ void weighted_degree(Ctx& c, const DGraph& g, const LevelPlan& p, uint32_t* wdeg
         // integer row sums of the weights (the reference's u32 sums; the
         // same integers an FP64 product with a vector of ones gives, without
         // its gathers)
-        if (p.ncb <= 1 && g.m && (p.rid16.get() || p.rid.get())) {
-            // the plan's row ids follow the graph's rows: entry-parallel
-            const Blocks B = blocks_of(p);
-            fill_u32(c, wdeg, g.n, 0u);
-            if (p.rid16.get())
-                k_wdeg_entries<true><<<grid1(g.m), BLOCK, 0, c.stream>>>(B, p.rid16.get(), p.wmeta.get(), g.W(),
-                                                                         wdeg);
-            else
-                k_wdeg_entries<false><<<grid1(g.m), BLOCK, 0, c.stream>>>(B, p.rid.get(), p.wmeta.get(), g.W(),
-                                                                          wdeg);
-            MI_HIP(hipGetLastError());
+        if (p.wdeg.get()) {
+            // formed by the plan with its row ids
+            MI_HIP(hipMemcpyAsync(wdeg, p.wdeg.get(), (size_t)g.n * 4, hipMemcpyDeviceToDevice, c.stream));
         } else {
             DBuf<uint32_t> longs(&c.pool, g.n), nlong(&c.pool, 1);
             fill_u32(c, nlong.get(), 1, 0u);

@@ -144,6 +144,7 @@ struct LevelPlan {
     uint32_t nblk_a = 0;  // workgroups of the streaming kernel
     DBuf<uint32_t> span;  // LDS form: the workgroups' schedule (own ranges, shared chunks)
     DBuf<uint32_t> sched; // ... its counters (zero between launches)
+    DBuf<uint32_t> wdeg;  // integer weighted degrees, when the plan formed them with its row ids
     uint32_t nblk_f = 0;  // workgroups of the finish kernel
     bool f32 = false;     // FP32 vectors and row partials (mixed precision)
 };
