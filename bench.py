@@ -370,6 +370,8 @@ def main() -> int:
                          "that calibrates rocprofv3's byte counters")
     ap.add_argument("--bisections-only", action="store_true",
                     help="only the timed FP64 bisections (for kernel traces of the whole bisection)")
+    ap.add_argument("--extra-only", choices=["rmat20", "grid3d", "batch"], default=None,
+                    help="only one of the extra workloads (for kernel traces of it), printed as JSON")
     ap.add_argument("--bucketed-build", type=int, default=None, metavar="MIN_ENTRIES",
                     help="coarse builds of levels with at least MIN_ENTRIES entries take the bucketed form "
                          "(default: the radix form everywhere)")
@@ -384,6 +386,18 @@ def main() -> int:
 
     import ipdps_2021_graph_coarsening_amd as mi
     from ipdps_2021_graph_coarsening_amd import graphs
+
+    if args.extra_only:
+        ctx = mi.Context(0)
+        if args.bucketed_build is not None:
+            ctx.set_bucketed_build(args.bucketed_build)
+        ctx.graph(np.array([0, 1, 2], dtype=np.uint32), np.array([1, 0], dtype=np.uint32)).close()
+        steps = max(1, min(args.steps, 5))
+        fn = {"rmat20": lambda: extra_rmat20(mi, ctx, steps), "grid3d": lambda: extra_grid3d(mi, ctx, steps),
+              "batch": lambda: extra_rgg_batch(mi, ctx)}[args.extra_only]
+        print(json.dumps({"extra": args.extra_only, "result": fn()}), flush=True)
+        ctx.close()
+        return 0
 
     t = time.perf_counter()
     if args.scale >= 22:

@@ -8,8 +8,10 @@ missing or no GPU is present the calls raise -- there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -154,6 +156,24 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(vp)
 
 
+# Handles still open when the interpreter exits are closed by an exit hook,
+# while the HIP runtime is certainly up and in a fixed order (hierarchies
+# and graphs before the contexts they were made on), instead of by the
+# garbage collector's finalisers in whatever order it reaches them.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def _close_live() -> None:
+    for o in sorted(list(_live), key=lambda o: isinstance(o, Context)):
+        try:
+            o.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_live)
+
+
 class Context:
     """A library context on one HIP device (stream, pool, workspace)."""
 
@@ -164,6 +184,7 @@ class Context:
         if rc:
             raise MiError(rc, f"mi_ctx_create(device={device}) failed (no usable HIP device?)")
         self.h = h
+        _live.add(self)
 
     def check(self, rc: int, what: str) -> None:
         if rc:
@@ -263,6 +284,7 @@ class Graph:
             ctx.check(ctx._L.mi_graph_create(ctx.h, n, rowptr, col, _ptr(wa), C.byref(h)),
                       "mi_graph_create")
             self.h = h
+        _live.add(self)
         n, m = C.c_uint32(), C.c_uint32()
         ctx._L.mi_graph_size(self.h, C.byref(n), C.byref(m))
         self.n, self.m = n.value, m.value
@@ -374,6 +396,7 @@ class Hierarchy:
         c.check(c._L.mi_hierarchy_build(c.h, g.h, seed, C.byref(h), C.byref(st) if stats else None),
                 "mi_hierarchy_build")
         self.h = h
+        _live.add(self)
         self.stats = st.as_dict()
         return self
 
@@ -401,6 +424,7 @@ class Hierarchy:
         c.check(c._L.mi_hier_create_from_host(c.h, L, ns, rp, cp, wp, vc, C.byref(h)),
                 "mi_hier_create_from_host")
         self.h = h
+        _live.add(self)
         self.stats = None
         return self
 

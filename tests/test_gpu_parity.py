@@ -605,3 +605,32 @@ def test_locality_order_vs_hec_order(ctx, name):
     assert abs(ev1 @ ev0) / (np.linalg.norm(ev1) * np.linalg.norm(ev0)) >= EV_COS
     H.close()
     g.close()
+
+
+def test_handles_left_open_at_exit():
+    """A process that leaves a context, graphs and a hierarchy open (and a
+    context destroyed before its graphs) exits cleanly: the binding's exit
+    hook closes them while the HIP runtime is up, in a fixed order."""
+    import subprocess
+    import sys
+    import textwrap
+    code = textwrap.dedent("""
+        import numpy as np
+        import ipdps_2021_graph_coarsening_amd as mi
+        from ipdps_2021_graph_coarsening_amd import graphs
+        rp, col, w = graphs.rmat(10, 8, seed=3)
+        ctx = mi.Context(0)
+        g = ctx.graph(rp, col, w)
+        H = g.hierarchy(1)
+        g2 = ctx.graph(rp, col, w)
+        ctx2 = mi.Context(0)
+        g3 = ctx2.graph(rp, col, w)
+        ctx2.close()
+        cycle = [g, H, g2]
+        cycle.append(cycle)
+        print("open", flush=True)
+    """)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+    assert "open" in r.stdout
