@@ -17,12 +17,20 @@
  *   - pcg32, compute_partition, power iteration, coarse build (as a per-row
  *     multiset), dedupe sums: against the reference's own non-Kokkos
  *     sgpar.h compiled from /root/reference into oracle/_ref.
- *   - XorShift64 pool stream, BinSort permutation, heavy neighbour, map
- *     construction, level loop: against the recorded reference Serial run
- *     on the 256x256 grid with time() pinned to 12345 (level sizes,
- *     per-level iteration counts and cut), plus mathematical checks
- *     (full period of the xorshift transition, sortedness of the argsort,
- *     brute-force heavy neighbour, validity of the cluster map).
+ *   - XorShift64* transition and output scramble (urand64): against the
+ *     published definition (shifts 12/25/27, multiplier
+ *     2685821657736338717) as GF(2) matrices and a 64-bit product, and the
+ *     full period of the transition.
+ *   - BinSort permutation, heavy neighbour, map construction, level loop:
+ *     mathematical checks (sortedness of the argsort, brute-force heavy
+ *     neighbour, the closed form of the map against its literal serial
+ *     model) and the recorded reference Serial run on the 256x256 grid with
+ *     time() pinned to 12345 (level sizes, per-level iteration counts, cut).
+ *   - PARITY UNPINNED: orc_pool_state (17 discarded draws, 16-bit state
+ *     packing) and the level-1 random-neighbour / isolated-vertex draws of
+ *     orc_heavy_neighbor are checked only by that recorded run, which no
+ *     committed recipe regenerates (the reference's Kokkos build is not
+ *     rebuildable here).
  */
 #include <float.h>
 #include <math.h>
