@@ -142,6 +142,18 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries);
  */
 int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                     const uint32_t* w, mi_graph** out);
+/* Builds a graph from an edge list (host arrays u, v, and w or NULL, of
+   nedges edges, every vertex id below n) the way the reference's converter
+   mtx2csr.cpp:116-289 turns a MatrixMarket file into its binary CSR, on the
+   device: every edge stored in both directions, self-loops dropped,
+   duplicates merged (their weights summed; w == NULL gives unit weights,
+   the reference's pattern graphs), rows sorted by column, and with lcc != 0
+   only the largest connected component, its vertices renumbered in their
+   order (a tie between components of equal size goes to the one with the
+   smallest vertex, which the reference's BFS in vertex order finds first).
+   mi_graph_size gives the resulting n and m. */
+int mi_graph_from_edges(mi_ctx* ctx, uint32_t n, uint64_t nedges, const uint32_t* u, const uint32_t* v,
+                        const uint32_t* w, int lcc, mi_graph** out);
 /* Wraps device-resident CSR arrays (not copied; must outlive the graph). */
 int mi_graph_create_device(mi_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* d_rowptr,
                            const uint32_t* d_col, const uint32_t* d_w, mi_graph** out);

@@ -93,6 +93,7 @@ _SIGS = [
     ("mi_ctx_set_locality", C.c_int, [vp, C.c_int, C.c_uint32]),
     ("mi_ctx_set_bucketed_build", C.c_int, [vp, C.c_uint32]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
+    ("mi_graph_from_edges", C.c_int, [vp, C.c_uint32, C.c_uint64, u32p, u32p, vp, C.c_int, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     ("mi_graph_destroy", None, [vp]),
     ("mi_graph_size", C.c_int, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -235,6 +236,21 @@ class Context:
     # ------------------------------------------------------------ graphs
     def graph(self, rowptr, col, w=None) -> "Graph":
         return Graph(self, rowptr, col, w)
+
+    def graph_from_edges(self, n: int, u, v, w=None, lcc: bool = True) -> "Graph":
+        """The reference converter's CSR graph of an edge list, built on the
+        device (mi_graph_from_edges): symmetrised, self-loops dropped,
+        duplicates merged (weights summed; no w: unit weights), rows sorted,
+        with lcc only the largest connected component in vertex order."""
+        u = _u32(u)
+        v = _u32(v)
+        if len(u) != len(v):
+            raise ValueError("u and v differ in length")
+        wa = None if w is None else _u32(w)
+        h = vp()
+        self.check(self._L.mi_graph_from_edges(self.h, n, len(u), u, v, _ptr(wa), 1 if lcc else 0, C.byref(h)),
+                   "mi_graph_from_edges")
+        return Graph(self, None, None, None, _handle=h)
 
     # ------------------------------------------------------------ building blocks
     def xs_draws(self, seed: int, first: int, count: int) -> np.ndarray:

@@ -518,6 +518,33 @@ int mi_graph_create(mi_ctx* ctx, uint32_t n, const uint32_t* rowptr, const uint3
     });
 }
 
+int mi_graph_from_edges(mi_ctx* ctx, uint32_t n, uint64_t nedges, const uint32_t* u, const uint32_t* v,
+                        const uint32_t* w, int lcc, mi_graph** out)
+{
+    if (!ctx || !out || (nedges && (!u || !v))) return ST_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        MI_CHECK(n > 0, ST_ARG, "graph must have at least one vertex");
+        for (uint64_t e = 0; e < nedges; e++)
+            MI_CHECK(u[e] < n && v[e] < n, ST_ARG, "edge endpoint out of range");
+        Ctx& c = ctx->c;
+        DBuf<uint32_t> du(&c.pool, nedges), dv(&c.pool, nedges), dw;
+        if (nedges) {
+            h2d(c, du.get(), u, (size_t)nedges * 4);
+            h2d(c, dv.get(), v, (size_t)nedges * 4);
+            if (w) {
+                dw = DBuf<uint32_t>(&c.pool, nedges);
+                h2d(c, dw.get(), w, (size_t)nedges * 4);
+            }
+        }
+        auto G = std::make_unique<mi_graph>();
+        G->ctx = ctx;
+        graph_from_edges(c, n, nedges, du.get(), dv.get(), w ? dw.get() : nullptr, lcc != 0, G->g);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        *out = G.release();
+    });
+}
+
 int mi_graph_create_device(mi_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* d_rowptr, const uint32_t* d_col,
                            const uint32_t* d_w, mi_graph** out)
 {

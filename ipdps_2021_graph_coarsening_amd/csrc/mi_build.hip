@@ -822,19 +822,16 @@ __global__ void k_copy_rows(uint32_t nc, const uint32_t* __restrict__ rowptr, co
 
 }  // namespace
 
+// The CSR graph of nc rows whose entries are the m (key, value) pairs,
+// key = row << B | column (keys >= nc << B: dropped): the pairs sorted by
+// key, equal keys merged with their values summed (u32, exact), rows in
+// column order.  keys / vals are consumed.
 template <class K>
-static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
-                              int B, DGraph& out)
+static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m, uint32_t nc, int B, DGraph& out)
 {
-    const uint64_t m = g.m;
     const K sentinel = (K)nc << B;
-    DBuf<K> keys(&c.pool, m), ktmp(&c.pool, m);
-    DBuf<uint32_t> vals(&c.pool, m), vtmp(&c.pool, m);
-    if (m) {
-        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, keys.get(),
-                                                          vals.get());
-        MI_HIP(hipGetLastError());
-    }
+    DBuf<K> ktmp(&c.pool, m);
+    DBuf<uint32_t> vtmp(&c.pool, m);
     {
         K* kp = keys.get();
         uint32_t* vp = vals.get();
@@ -875,6 +872,21 @@ static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, cons
         MI_HIP(hipGetLastError());
     }
     scan_exclusive_u32(c, rowcnt.get(), out.rowptr.get(), (uint64_t)nc + 1, nullptr);
+}
+
+template <class K>
+static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
+                              int B, DGraph& out)
+{
+    const uint64_t m = g.m;
+    DBuf<K> keys(&c.pool, m);
+    DBuf<uint32_t> vals(&c.pool, m);
+    if (m) {
+        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, keys.get(),
+                                                          vals.get());
+        MI_HIP(hipGetLastError());
+    }
+    keys_to_csr<K>(c, keys, vals, m, nc, B, out);
 }
 
 namespace {
@@ -1101,6 +1113,197 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
         build_coarse_impl<uint32_t>(c, g, roc, vcmap, nc, B, out);
     else
         build_coarse_impl<uint64_t>(c, g, roc, vcmap, nc, B, out);
+}
+
+// ------------------------------------------------------------------ input path
+// An edge list to the CSR graph the reference's converter writes
+// (mtx2csr.cpp:116-289): every edge in both directions, self-loops dropped,
+// duplicates merged (their weights summed), rows in column order, and --
+// when asked -- only the largest connected component, relabelled in vertex
+// order.  Components are found by concurrent union-find (a root is hooked
+// under the smaller of the two roots with a compare-and-swap, so a root is
+// its component's smallest vertex and parents only decrease: every walk
+// ends); the largest component is the one of most vertices, a tie going to
+// the smallest first vertex, which is the component the reference's BFS in
+// vertex order numbers first.
+
+namespace {
+
+template <class K>
+__global__ void k_emit_pairs(uint64_t ne, const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                             const uint32_t* __restrict__ w, uint32_t n, int B, K* __restrict__ keys,
+                             uint32_t* __restrict__ vals)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= ne) return;
+    const uint32_t a = u[e], b = v[e];
+    const bool keep = a != b;  // (both below n: checked by the caller)
+    const K sentinel = (K)n << B;
+    keys[2 * e] = keep ? (((K)a << B) | (K)b) : sentinel;
+    keys[2 * e + 1] = keep ? (((K)b << B) | (K)a) : sentinel;
+    const uint32_t x = w ? w[e] : 1u;
+    vals[2 * e] = x;
+    vals[2 * e + 1] = x;
+}
+
+__device__ __forceinline__ uint32_t cc_find(uint32_t* parent, uint32_t x)
+{
+    uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+// one lane per stored entry (u, v), u < v only: the union of u's and v's sets
+__global__ void k_cc_union(uint64_t m, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ roc,
+                           const uint32_t* __restrict__ col, uint32_t* __restrict__ parent)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t a = entry_row((uint32_t)e, roc, rowptr), b = col[e];
+    if (a >= b) return;
+    uint32_t ra = cc_find(parent, a), rb = cc_find(parent, b);
+    while (ra != rb) {
+        const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
+        const uint32_t old = atomicCAS(&parent[hi], hi, lo);
+        if (old == hi) break;  // hooked
+        // hi got another parent meanwhile: walk on from there
+        ra = cc_find(parent, old);
+        rb = cc_find(parent, lo);
+    }
+}
+
+// roots, component sizes, and the largest (size, smallest root) as one key
+__global__ void k_cc_roots(uint32_t n, uint32_t* __restrict__ parent, uint32_t* __restrict__ size)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = cc_find(parent, i);
+    parent[i] = r;
+    atomicAdd(&size[r], 1u);
+}
+
+__global__ void k_cc_largest(uint32_t n, const uint32_t* __restrict__ size, unsigned long long* __restrict__ best)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || !size[i]) return;
+    atomicMax(best, ((unsigned long long)size[i] << 32) | (unsigned long long)(0xffffffffu - i));
+}
+
+__global__ void k_cc_keep(uint32_t n, const uint32_t* __restrict__ root, const unsigned long long* __restrict__ best,
+                          const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ keep, uint32_t* __restrict__ deg)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t L = 0xffffffffu - (uint32_t)(*best & 0xffffffffull);
+    const bool k = root[i] == L;
+    keep[i] = k ? 1u : 0u;
+    deg[i] = k ? rowptr[i + 1] - rowptr[i] : 0u;
+}
+
+// the kept rows' offsets in their new order
+__global__ void k_cc_rowptr(uint32_t n, const uint32_t* __restrict__ keep, const uint32_t* __restrict__ newid,
+                            const uint32_t* __restrict__ dscan, uint32_t* __restrict__ rowptr2)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && keep[i]) rowptr2[newid[i]] = dscan[i];
+}
+
+// entries of kept rows to their new places, columns relabelled
+__global__ void k_cc_copy(uint64_t m, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ roc,
+                          const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                          const uint32_t* __restrict__ keep, const uint32_t* __restrict__ newid,
+                          const uint32_t* __restrict__ rowptr2, uint32_t* __restrict__ col2, uint32_t* __restrict__ w2)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t r = entry_row((uint32_t)e, roc, rowptr);
+    if (!keep[r]) return;
+    const uint32_t d = rowptr2[newid[r]] + ((uint32_t)e - rowptr[r]);
+    col2[d] = newid[col[e]];
+    if (w2) w2[d] = w[e];
+}
+
+}  // namespace
+
+void graph_from_edges(Ctx& c, uint32_t n, uint64_t ne, const uint32_t* du, const uint32_t* dv, const uint32_t* dw,
+                      bool lcc, DGraph& out)
+{
+    MI_CHECK(n > 0, ST_ARG, "graph must have at least one vertex");
+    MI_CHECK(2 * ne < 0xffff0000ull, ST_ARG, "too many edges for 32-bit offsets");
+    const int B = bits_for(n);
+    const uint64_t m2 = 2 * ne;
+    DGraph g;
+    if (2 * B <= 32) {
+        DBuf<uint32_t> keys(&c.pool, m2), vals(&c.pool, m2);
+        if (ne) k_emit_pairs<uint32_t><<<grid1(ne), BLOCK, 0, c.stream>>>(ne, du, dv, dw, n, B, keys.get(), vals.get());
+        MI_HIP(hipGetLastError());
+        keys_to_csr<uint32_t>(c, keys, vals, m2, n, B, g);
+    } else {
+        DBuf<uint64_t> keys(&c.pool, m2);
+        DBuf<uint32_t> vals(&c.pool, m2);
+        if (ne) k_emit_pairs<uint64_t><<<grid1(ne), BLOCK, 0, c.stream>>>(ne, du, dv, dw, n, B, keys.get(), vals.get());
+        MI_HIP(hipGetLastError());
+        keys_to_csr<uint64_t>(c, keys, vals, m2, n, B, g);
+    }
+    if (!dw) {  // unit weights, as the reference's pattern graphs
+        g.unit = true;
+        g.w.reset();
+    }
+    if (!lcc) {
+        out = std::move(g);
+        return;
+    }
+    // components
+    DBuf<uint32_t> parent(&c.pool, n), size(&c.pool, n), roc;
+    iota_u32(c, parent.get(), n, 0);
+    fill_u32(c, size.get(), n, 0u);
+    if (g.m) {
+        build_row_index(c, g, roc);
+        k_cc_union<<<grid1(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), parent.get());
+        MI_HIP(hipGetLastError());
+    }
+    k_cc_roots<<<grid1(n), BLOCK, 0, c.stream>>>(n, parent.get(), size.get());
+    MI_HIP(hipGetLastError());
+    DBuf<unsigned long long> best(&c.pool, 1);
+    fill_u32(c, reinterpret_cast<uint32_t*>(best.get()), 2, 0u);
+    k_cc_largest<<<grid1(n), BLOCK, 0, c.stream>>>(n, size.get(), best.get());
+    MI_HIP(hipGetLastError());
+    unsigned long long hb = 0;
+    read_words(c, reinterpret_cast<const uint32_t*>(best.get()), reinterpret_cast<uint32_t*>(&hb), 8);
+    const uint32_t n2 = (uint32_t)(hb >> 32);
+    if (n2 == n) {
+        out = std::move(g);
+        return;
+    }
+    DBuf<uint32_t> keep(&c.pool, n), deg(&c.pool, n), newid(&c.pool, (size_t)n + 1);
+    k_cc_keep<<<grid1(n), BLOCK, 0, c.stream>>>(n, parent.get(), best.get(), g.R(), keep.get(), deg.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, keep.get(), newid.get(), n, newid.get() + n);
+    DGraph h;
+    h.n = n2;
+    h.unit = g.unit;
+    h.rowptr = DBuf<uint32_t>(&c.pool, (size_t)n2 + 1);
+    // row lengths of the kept rows in their new order: scan of deg (zero
+    // elsewhere), read at the kept rows' old indices
+    DBuf<uint32_t> dscan(&c.pool, (size_t)n + 1);
+    scan_exclusive_u32(c, deg.get(), dscan.get(), n, dscan.get() + n);
+    uint32_t m2k = 0;
+    read_words(c, dscan.get() + n, &m2k, 4);
+    h.m = m2k;
+    h.col = DBuf<uint32_t>(&c.pool, m2k);
+    if (!h.unit) h.w = DBuf<uint32_t>(&c.pool, m2k);
+    k_cc_rowptr<<<grid1(n), BLOCK, 0, c.stream>>>(n, keep.get(), newid.get(), dscan.get(), h.rowptr.get());
+    MI_HIP(hipGetLastError());
+    fill_u32(c, h.rowptr.get() + n2, 1, m2k);
+    if (g.m) {
+        k_cc_copy<<<grid1(g.m), BLOCK, 0, c.stream>>>(g.m, g.R(), roc.get(), g.Cc(), g.W(), keep.get(), newid.get(),
+                                                     h.rowptr.get(), h.col.get(), h.unit ? nullptr : h.w.get());
+        MI_HIP(hipGetLastError());
+    }
+    out = std::move(h);
 }
 
 }  // namespace mi

@@ -120,6 +120,12 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t);
 
 // Coarse graph P^T A P without diagonal, rows sorted, duplicates merged.
+// The reference converter's graph from an edge list on the device (u, v, w
+// of ne edges, every vertex below n; w null: unit weights): symmetrised,
+// self-loops dropped, duplicates merged with summed weights, rows sorted,
+// and with lcc only the largest connected component in vertex order.
+void graph_from_edges(Ctx& c, uint32_t n, uint64_t ne, const uint32_t* du, const uint32_t* dv, const uint32_t* dw,
+                      bool lcc, DGraph& out);
 void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
                   const uint32_t* vtx_w, DGraph& out, DBuf<uint32_t>& c_vtx_w);
 

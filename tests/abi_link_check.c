@@ -25,6 +25,8 @@ int main(void)
     int (*fp_mi_bisect_batch)(mi_ctx*, int, const mi_graph* const*, const mi_config*, int, const uint64_t*,
                               uint64_t*, uint64_t, uint32_t* const*, int, uint32_t*, mi_stats*, double*);
     int (*fp_mi_graph_create)(mi_ctx*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, mi_graph**);
+    int (*fp_mi_graph_from_edges)(mi_ctx*, uint32_t, uint64_t, const uint32_t*, const uint32_t*, const uint32_t*, int,
+                                  mi_graph**);
     int (*fp_mi_graph_create_device)(mi_ctx*, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
                                      const uint32_t*, mi_graph**);
     void (*fp_mi_graph_destroy)(mi_graph*);
@@ -69,6 +71,7 @@ int main(void)
     BIND(mi_ctx_set_locality);
     BIND(mi_bisect_batch);
     BIND(mi_graph_create);
+    BIND(mi_graph_from_edges);
     BIND(mi_graph_create_device);
     BIND(mi_graph_destroy);
     BIND(mi_graph_size);
