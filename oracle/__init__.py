@@ -92,6 +92,20 @@ class _Core:
         L.orc_partition_graph.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.c_uint64, C.c_int, C.c_double,
                                           u32p, C.POINTER(C.c_uint32), u32p, u64p, f64p]
+        u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_fm_refine.restype = C.c_uint32
+        L.orc_fm_refine.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_void_p, u8p, i64p]
+        L.orc_init_gggp.restype = None
+        L.orc_init_gggp.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_void_p, u8p]
+        L.orc_init_spectral.restype = None
+        L.orc_init_spectral.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                        C.c_double, u8p, f64p]
+        L.orc_fm_eigensolve.restype = C.c_uint32
+        L.orc_fm_eigensolve.argtypes = [C.c_void_p, C.c_uint64, C.c_double, u8p, u32p]
+        L.orc_partition_graph_fm.restype = C.c_int
+        L.orc_partition_graph_fm.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.c_double,
+                                             u32p, C.POINTER(C.c_uint32), u8p, u32p]
 
 
 _core = None
@@ -259,6 +273,14 @@ class Hierarchy:
                                      C.cast(sp, C.c_void_p), C.cast(rp, C.c_void_p))
         return ev, iters, st.value, trace
 
+    def fm_eigensolve(self, seed: int, tol: float = 1e-10):
+        """FM branch of the eigensolve: (finest 0/1 partition, last cut,
+        fm_refine calls per level (0 = finest))."""
+        part = np.zeros(self.levels[0]["n"] + 1, dtype=np.uint8)
+        passes = np.zeros(self.nlevels, dtype=np.uint32)
+        cut = self._L.orc_fm_eigensolve(self._h, seed, tol, part, passes)
+        return part[:-1].copy(), int(cut), passes
+
     def __del__(self):
         try:
             self._L.orc_hier_free(self._h)
@@ -315,6 +337,65 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
                                part, C.byref(ec), lv, it, ev)
     return dict(part=part, edge_cut=ec.value, levels_n=lv[:nl].copy(), iters=it[:nl].copy(),
                 evec=ev, pcg_state=st.value)
+
+
+def _csr(rowptr, col):
+    return (np.ascontiguousarray(rowptr, dtype=np.uint32), np.ascontiguousarray(col, dtype=np.uint32))
+
+
+def fm_refine(rowptr, col, w, vtx_w, part):
+    """One fm_refine call on a copy of part (0/1): (new part (uint8), min_cut,
+    info dict(swaps, argmin, min_imb, start_cut, maxE))."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vw, vwp = _wptr(vtx_w)
+    p = np.zeros(n + 1, dtype=np.uint8)
+    p[:n] = np.asarray(part) != 0
+    info = np.zeros(5, dtype=np.int64)
+    cut = L.orc_fm_refine(n, rowptr, col, wp, vwp, p, info)
+    return p[:n].copy(), int(cut), dict(swaps=int(info[0]), argmin=int(info[1]), min_imb=int(info[2]),
+                                         start_cut=int(info[3]), maxE=int(info[4]))
+
+
+def init_gggp(rowptr, col, w, vtx_w=None):
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vw, vwp = _wptr(vtx_w)
+    p = np.zeros(n + 1, dtype=np.uint8)
+    L.orc_init_gggp(n, rowptr, col, wp, vwp, p)
+    return p[:n].copy()
+
+
+def init_spectral(rowptr, col, w, vtx_w, seed: int, tol: float = 1e-10):
+    """(0/1 partition, iterated vector)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vw, vwp = _wptr(vtx_w)
+    p = np.zeros(n + 1, dtype=np.uint8)
+    x = np.zeros(n + 1, dtype=np.float64)
+    L.orc_init_spectral(n, rowptr, col, wp, vwp, seed, tol, p, x)
+    return p[:n].copy(), x[:n].copy()
+
+
+def partition_graph_fm(rowptr, col, w, hec_seed: int, tol: float = 1e-10):
+    """Whole bisection of the FM binary: dict(part, edge_cut, fm_part, passes, nlevels)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    part = np.empty(n + 1, dtype=np.uint32)
+    fmp = np.zeros(n + 1, dtype=np.uint8)
+    ps = np.zeros(210, dtype=np.uint32)
+    ec = C.c_uint32(0)
+    nl = L.orc_partition_graph_fm(n, rowptr, col, wp, hec_seed, tol, part, C.byref(ec), fmp, ps)
+    return dict(part=part[:n].copy(), edge_cut=ec.value, fm_part=fmp[:n].copy(), passes=ps[:nl].copy(),
+                nlevels=nl)
 
 
 # ---------------------------------------------------------------- reference (_ref)

@@ -729,3 +729,383 @@ int orc_partition_graph(vid_t n, const eid_t *rowptr, const vid_t *col, const wg
     orc_hier_free(h);
     return L;
 }
+
+/* ------------------------------------------------------------------ */
+/* FM refinement: the refinement of the reference's *_fm binaries       */
+/* (compiled with -DFM, CMakeLists.txt:50).  Partitions are 0/1 bytes   */
+/* (the reference keeps them as 0.0 / 1.0 doubles; every test there is  */
+/* `== 0.0`).  Edge weights are integral, as the reference assumes      */
+/* (eigensolve_kokkos.h:970-971).                                       */
+/* ------------------------------------------------------------------ */
+#define FM_W(e) (w ? w[e] : 1u)
+#define FM_VW(i) (vtx_w ? vtx_w[i] : 1u)
+
+/*
+ * One call of fm_refine, eigensolve_kokkos.h:973-1240, with the gains and
+ * bucket lists of fm_create_ds, :208-369, as the Serial backend builds them
+ * (every bucket list in increasing vertex order, its head the smallest).
+ * The selection loop is restated as written: two seek offsets that walk
+ * down over empty buckets, the balance rule choosing the side, heads
+ * taken from the bucket at the offset, neighbour gains moved between
+ * doubly linked lists (insertion at the head), the best prefix kept by
+ * the reference's balance/cut rule (:1126-1152), the search stopped 500
+ * swaps after the last improvement (:1212-1215), later swaps undone.
+ * part is updated in place; returns min_cut (twice the cut weight, the
+ * reference's convention).  info (may be NULL) receives {swaps made,
+ * argmin (-1 = none), |balance| of the kept prefix, starting cut,
+ * maxE}.
+ */
+eid_t orc_fm_refine(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                    const wgt_t *vtx_w, uint8_t *part, int64_t *info)
+{
+    /* maxE: largest weighted degree, u32 sums (:1003-1014) */
+    eid_t maxE = 0;
+    for (vid_t i = 0; i < n; i++) {
+        eid_t wd = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) wd += FM_W(e);
+        if (maxE < wd) maxE = wd;
+    }
+    const eid_t nb = 2 * maxE + 1;  /* totalBuckets (:213) */
+    int64_t *gains = (int64_t *)malloc((size_t)n * sizeof(int64_t) + 8);
+    vid_t *bA = (vid_t *)malloc((size_t)nb * sizeof(vid_t));
+    vid_t *bB = (vid_t *)malloc((size_t)nb * sizeof(vid_t));
+    vid_t *tA = (vid_t *)malloc((size_t)nb * sizeof(vid_t));
+    vid_t *tB = (vid_t *)malloc((size_t)nb * sizeof(vid_t));
+    vid_t *nx = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *pv = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    uint8_t *fr = (uint8_t *)malloc((size_t)n + 1);
+    vid_t *order = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    /* gains and cut (:230-262): gain summed as a double, cut as u32 */
+    eid_t cutsize = 0;
+    for (vid_t i = 0; i < n; i++) {
+        double g = 0;
+        eid_t lc = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            if (part[i] != part[col[e]])
+                g += FM_W(e);
+            else
+                g -= FM_W(e);
+        }
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+            if (part[i] != part[col[e]]) lc += FM_W(e);
+        gains[i] = (int64_t)g;
+        cutsize += lc;
+    }
+    /* bucket lists (:264-348): vertices of a bucket in increasing order */
+    for (eid_t b = 0; b < nb; b++) bA[b] = bB[b] = tA[b] = tB[b] = ORC_INF;
+    for (vid_t i = 0; i < n; i++) {
+        const eid_t b = (eid_t)(gains[i] + maxE);
+        vid_t *hd = part[i] == 0 ? bA : bB, *tl = part[i] == 0 ? tA : tB;
+        nx[i] = pv[i] = ORC_INF;
+        fr[i] = 1;
+        if (hd[b] == ORC_INF) {
+            hd[b] = i;
+        } else {
+            nx[tl[b]] = i;
+            pv[i] = tl[b];
+        }
+        tl[b] = i;
+    }
+    free(tA);
+    free(tB);
+    /* the search (:1017-1220) */
+    int64_t balance = 0;
+    for (vid_t i = 0; i < n; i++) {
+        if (part[i] == 0)
+            balance += FM_VW(i);
+        else
+            balance -= FM_VW(i);
+    }
+    const int64_t start_balance = llabs(balance);
+    const int64_t start_cut = cutsize;
+    int64_t offA = (int64_t)(eid_t)(2 * maxE), offB = offA;
+    vid_t total = 0;
+    eid_t min_cut = (eid_t)start_cut;
+    vid_t argmin = ORC_INF;
+    int64_t min_imb = start_balance;
+    int64_t target = start_balance / 10;
+    if (target < 20) target = 20;
+    int counter = 0;
+    while (offA >= 0 || offB >= 0) {
+        const vid_t sa = offA >= 0 ? bA[offA] : ORC_INF;
+        const vid_t sb = offB >= 0 ? bB[offB] : ORC_INF;
+        int ca = 0, cb = 0;
+        if (sa != ORC_INF && sb == ORC_INF) {
+            if (balance > 0 || offB < 0) ca = 1;
+            offB--;
+        } else if (sa == ORC_INF && sb != ORC_INF) {
+            if (balance < 0 || offA < 0) cb = 1;
+            offA--;
+        } else if (sa != ORC_INF && sb != ORC_INF) {
+            if (balance > 0)
+                ca = 1;
+            else if (balance < 0)
+                cb = 1;
+            else if (offA > offB)
+                ca = 1;
+            else
+                cb = 1;
+        } else {
+            offA--;
+            offB--;
+        }
+        vid_t s = ORC_INF;
+        if (ca) {
+            s = sa;
+            const vid_t nn = nx[s];
+            bA[offA] = nn;
+            if (nn != ORC_INF) pv[nn] = ORC_INF;
+        } else if (cb) {
+            s = sb;
+            const vid_t nn = nx[s];
+            bB[offB] = nn;
+            if (nn != ORC_INF) pv[nn] = ORC_INF;
+        }
+        if (s == ORC_INF) continue;
+        cutsize = (eid_t)((int64_t)cutsize - 2 * gains[s]);
+        if (part[s] == 0) {
+            part[s] = 1;
+            balance -= 2 * (int64_t)FM_VW(s);
+        } else {
+            part[s] = 0;
+            balance += 2 * (int64_t)FM_VW(s);
+        }
+        fr[s] = 0;
+        order[total] = s;
+        counter++;
+        const int64_t ab = llabs(balance);
+        int improve = 0;
+        if (ab <= target && cutsize <= min_cut)
+            improve = 1;
+        else if (ab > target) {
+            if (ab < min_imb && (double)cutsize < 1.05 * (double)min_cut)
+                improve = 1;
+            else if (ab <= min_imb && cutsize <= min_cut)
+                improve = 1;
+        }
+        if (improve) {
+            min_imb = ab;
+            min_cut = cutsize;
+            argmin = total;
+            counter = 0;
+        }
+        total++;
+        for (eid_t e = rowptr[s]; e < rowptr[s + 1]; e++) {
+            const vid_t v = col[e];
+            if (!fr[v]) continue;
+            int64_t gc = 2 * (int64_t)FM_W(e);
+            if (part[s] == part[v]) gc = -gc;
+            const vid_t on = nx[v], op = pv[v];
+            if (on != ORC_INF) pv[on] = op;
+            if (op != ORC_INF) nx[op] = on;
+            const int64_t og = gains[v], ng = og + gc;
+            gains[v] = ng;
+            if (op == ORC_INF) {
+                if (part[v] == 0)
+                    bA[og + maxE] = on;
+                else
+                    bB[og + maxE] = on;
+            }
+            vid_t nh;
+            if (part[v] == 0) {
+                nh = bA[ng + maxE];
+                bA[ng + maxE] = v;
+                if (ng + (int64_t)maxE > offA) offA = ng + maxE;
+            } else {
+                nh = bB[ng + maxE];
+                bB[ng + maxE] = v;
+                if (ng + (int64_t)maxE > offB) offB = ng + maxE;
+            }
+            if (nh != ORC_INF) pv[nh] = v;
+            nx[v] = nh;
+            pv[v] = ORC_INF;
+        }
+        if (counter > 500) {
+            offA = -1;
+            offB = -1;
+        }
+    }
+    /* undo the swaps after the kept prefix (:1222-1233) */
+    const vid_t undo_from = argmin != ORC_INF ? argmin + 1 : 0;
+    for (vid_t i = undo_from; i < total; i++) part[order[i]] ^= 1u;
+    if (info) {
+        info[0] = total;
+        info[1] = argmin != ORC_INF ? (int64_t)argmin : -1;
+        info[2] = min_imb;
+        info[3] = start_cut;
+        info[4] = maxE;
+    }
+    free(gains);
+    free(bA);
+    free(bB);
+    free(nx);
+    free(pv);
+    free(fr);
+    free(order);
+    return min_cut;
+}
+
+/*
+ * init_gggp, eigensolve_kokkos.h:1242-1320: from every start vertex i grow
+ * side 1 greedily (the side-0 vertex whose move adds least to the cut,
+ * lowest index on ties) until it holds half the vertex weight; keep the
+ * first start with the smallest cut.  Sums in doubles as written (exact:
+ * integral weights).  part receives 0/1.
+ */
+void orc_init_gggp(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                   const wgt_t *vtx_w, uint8_t *part)
+{
+    vid_t total = 0;
+    for (vid_t i = 0; i < n; i++) total += FM_VW(i);
+    uint8_t *cg = (uint8_t *)malloc((size_t)n + 1);
+    memset(part, 0, n);
+    double cutmin = (double)ORC_INF;
+    for (vid_t i = 0; i < n; i++) {
+        memset(cg, 0, n);
+        cg[i] = 1;
+        vid_t count = FM_VW(i);
+        while (count < total / 2) {
+            vid_t am = i;
+            double mn = (double)ORC_INF;
+            for (vid_t u = 0; u < n; u++) {
+                if (cg[u] != 0) continue;
+                double cl = 0;
+                for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++) {
+                    if (cg[col[e]] == 0)
+                        cl += FM_W(e);
+                    else
+                        cl -= FM_W(e);
+                }
+                if (cl < mn) {
+                    mn = cl;
+                    am = u;
+                }
+            }
+            cg[am] = 1;
+            count += FM_VW(am);
+        }
+        double ec = 0;
+        for (vid_t u = 0; u < n; u++)
+            for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++)
+                if (cg[col[e]] != cg[u]) ec += FM_W(e);
+        if (ec < cutmin) {
+            cutmin = ec;
+            memcpy(part, cg, n);
+        }
+    }
+    free(cg);
+}
+
+/*
+ * init_spectral, eigensolve_kokkos.h:1322-1368: x_i = 2 u_i - 1 with u_i =
+ * urand64 / 2^64 drawn in order from the single generator of the
+ * XorShift64 pool of `seed` (the reference seeds it with std::time), then
+ * normalised, one power iteration (normLap 0) at tolerance tol, the
+ * vertices ordered by (x_i, i) ascending (Kokkos BinSort with sorting
+ * inside bins, heuristics.h:603-611), and side 1 given to the prefix of
+ * that order whose weight before each vertex is below half the total.
+ * vec (may be NULL) receives the iterated vector.
+ */
+void orc_init_spectral(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                       const wgt_t *vtx_w, uint64_t seed, double tol, uint8_t *part, double *vec)
+{
+    vid_t total = 0;
+    for (vid_t i = 0; i < n; i++) total += FM_VW(i);
+    uint64_t st = orc_pool_state(seed);
+    const double mx = (double)UINT64_MAX;
+    double *x = (double *)malloc((size_t)n * sizeof(double) + 8);
+    for (vid_t i = 0; i < n; i++) {
+        x[i] = ((double)orc_xs_urand64(&st)) / mx;
+        x[i] = 2.0 * x[i] - 1.0;
+    }
+    normalize_seq(x, n);
+    orc_power_iter(n, rowptr, col, w, x, 0, tol, 0, NULL, NULL);
+    orc_vu_t *vu = (orc_vu_t *)malloc((size_t)n * sizeof(orc_vu_t) + 16);
+    for (vid_t i = 0; i < n; i++) {
+        vu[i].ev = x[i];
+        vu[i].u = i;
+    }
+    qsort(vu, n, sizeof(orc_vu_t), cmp_vu);
+    vid_t sum = 0;
+    for (vid_t k = 0; k < n; k++) {
+        const vid_t u = vu[k].u;
+        if (sum < total / 2) {
+            sum += FM_VW(u);
+            part[u] = 1;
+        } else {
+            part[u] = 0;
+        }
+    }
+    if (vec) memcpy(vec, x, (size_t)n * sizeof(double));
+    free(vu);
+    free(x);
+}
+
+/*
+ * The FM branch of sgp_eigensolve, eigensolve_kokkos.h:1506-1515,
+ * 1529-1587 (without RECOARSEN): coarsest guess by init_spectral when it
+ * has more than 200 vertices, else init_gggp; at every level, coarsest
+ * first, fm_refine repeated until it returns the cut it returned before
+ * (the last cut carried over from the coarser level, SGP_INFTY at the
+ * start); prolongation fine[i] = coarse[vcmap[i]] between levels.
+ * part0 (n0 bytes) receives the finest partition; passes[l] (may be NULL)
+ * the number of fm_refine calls at level l.  Returns the last cut.
+ */
+eid_t orc_fm_eigensolve(const orc_hier_t *h, uint64_t seed, double tol, uint8_t *part0,
+                        uint32_t *passes)
+{
+    const int L = h->nlevels;
+    const orc_level_t *c = &h->lv[L - 1];
+    uint8_t *cur = (uint8_t *)malloc((size_t)c->n + 1);
+    if (c->n > 200)
+        orc_init_spectral(c->n, c->rowptr, c->col, c->w, c->vtx_w, seed, tol, cur, NULL);
+    else
+        orc_init_gggp(c->n, c->rowptr, c->col, c->w, c->vtx_w, cur);
+    eid_t cutsize = ORC_INF;
+    for (int l = L - 1; l >= 0; l--) {
+        const orc_level_t *a = &h->lv[l];
+        eid_t old;
+        uint32_t np = 0;
+        do {
+            old = cutsize;
+            cutsize = orc_fm_refine(a->n, a->rowptr, a->col, a->w, a->vtx_w, cur, NULL);
+            np++;
+        } while (cutsize != old);
+        if (passes) passes[l] = np;
+        if (l > 0) {
+            const orc_level_t *f = &h->lv[l - 1];
+            uint8_t *fine = (uint8_t *)malloc((size_t)f->n + 1);
+            for (vid_t i = 0; i < f->n; i++) fine[i] = cur[f->vcmap[i]];
+            free(cur);
+            cur = fine;
+        }
+    }
+    memcpy(part0, cur, h->lv[0].n);
+    free(cur);
+    return cutsize;
+}
+
+/*
+ * Whole bisection of the hec_fm binary, sgpar.h:2363-2431 with the FM
+ * eigensolve: the 0/1 partition goes to sgp_compute_partition as the
+ * "eigenvector" (sort-and-split, local_search_alg 0).  hec_seed seeds both
+ * the coarsening and init_spectral (the reference seeds both from
+ * std::time).  Returns the number of levels.
+ */
+int orc_partition_graph_fm(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                           uint64_t hec_seed, double tol, vid_t *part, eid_t *edgecut,
+                           uint8_t *fm_part, uint32_t *passes)
+{
+    orc_hier_t *h = orc_hierarchy_build(n, rowptr, col, w, hec_seed);
+    const int L = h->nlevels;
+    uint8_t *p0 = (uint8_t *)malloc((size_t)n + 1);
+    orc_fm_eigensolve(h, hec_seed, tol, p0, passes);
+    double *ev = (double *)malloc((size_t)n * sizeof(double) + 8);
+    for (vid_t i = 0; i < n; i++) ev[i] = p0[i] ? 1.0 : 0.0;
+    orc_compute_partition(n, rowptr, col, ev, 2, 0, part, edgecut);
+    if (fm_part) memcpy(fm_part, p0, n);
+    free(ev);
+    free(p0);
+    orc_hier_free(h);
+    return L;
+}
