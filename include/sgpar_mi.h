@@ -46,7 +46,8 @@ typedef struct mi_hier mi_hier;
 typedef struct {
     int coarsening_alg;   /* ignored on this path, as in the reference's Kokkos branch */
     int refine_alg;       /* 0: Laplacian, 1: normalised Laplacian (normLap) */
-    int local_search_alg; /* 0 only (the reference returns before local search) */
+    int local_search_alg; /* 0 only (the reference returns before local search); FM refinement is
+                             the context's mi_ctx_set_refinement, as it is a build flag there */
     int num_iter;         /* trials; the library runs one trial per call */
     double tol;           /* power-iteration tolerance, default 1e-10 */
     int num_partitions;   /* 2 or 4 (4 splits in two, like sgp_compute_partition) */
@@ -57,7 +58,7 @@ typedef struct {
     int nlevels;
     uint32_t level_n[MI_MAX_LEVELS];
     uint32_t level_m[MI_MAX_LEVELS];          /* stored nonzeros */
-    uint64_t level_iters[MI_MAX_LEVELS];      /* power iterations */
+    uint64_t level_iters[MI_MAX_LEVELS];      /* power iterations (FM: fm_refine calls) */
     int level_max_reached[MI_MAX_LEVELS];     /* 1e14/n cap hit */
     double level_eigen_ms[MI_MAX_LEVELS];
     double coarsen_ms, eigen_ms, partition_ms, total_ms;
@@ -131,6 +132,20 @@ int mi_ctx_set_locality(mi_ctx* ctx, int enable, uint32_t min_vertices);
    every level); 0 puts every level on the bucketed form.  Results are
    identical either way.  No counterpart in the reference. */
 int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries);
+/*
+ * Refinement of the bisections of this context (mi_partition_graph,
+ * mi_bisect_batch, mi_partition_4way): fm = 0 (default) the multilevel
+ * Fiedler power iteration of the reference's spectral binaries; fm = 1 FM
+ * refinement as the reference's *_fm binaries (hec_fm, the documented
+ * default, build_and_usage_instructions.txt:80; compiled with -DFM,
+ * cmg/CMakeLists.txt:50): the FM branch of sgp_eigensolve
+ * (eigensolve_kokkos.h:1506-1587, see mi_fm_eigensolve) whose 0/1
+ * partition goes to the sort-and-split as the eigenvector.  With fm = 1
+ * the pcg state is not consumed, the coarsest level's init_spectral (more
+ * than 200 vertices) is seeded with hec_seed, and mi_stats.level_iters
+ * counts fm_refine calls.
+ */
+int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
 
 /* ---------------------------------------------------------------- graphs */
 /*
@@ -200,6 +215,40 @@ int mi_hier_level_copy(mi_ctx* ctx, const mi_hier* h, int level, uint32_t* rowpt
  */
 int mi_fiedler(mi_ctx* ctx, const mi_hier* h, uint64_t* pcg_state, uint64_t pcg_inc,
                int normLap, double tol, double* evec, mi_stats* stats);
+/*
+ * The FM branch of sgp_eigensolve, eigensolve_kokkos.h:1506-1587 (without
+ * RECOARSEN): the coarsest level's partition from init_gggp (at most 200
+ * vertices, :1242-1320) or init_spectral (:1322-1368, its random start from
+ * the XorShift64 pool of `seed` where the reference uses std::time); at
+ * each level, coarsest first, fm_refine repeated until it returns the cut
+ * it returned before (the last cut carried over from the coarser level);
+ * the partition prolonged between levels.  evec (host, n0 doubles, may be
+ * NULL) receives the finest partition as 0.0 / 1.0, like the reference's
+ * eigenvec; *last_cut (may be NULL) the last fm_refine result (twice the
+ * cut weight).  stats.level_iters = fm_refine calls per level.
+ */
+int mi_fm_eigensolve(mi_ctx* ctx, const mi_hier* h, uint64_t seed, double tol, double* evec,
+                     uint32_t* last_cut, mi_stats* stats);
+/*
+ * One fm_refine call (eigensolve_kokkos.h:973-1240, with the gains and
+ * bucket lists of fm_create_ds, :208-369) on graph g with vertex weights
+ * vtx_w (host, n; NULL = all 1): part (host, n, in/out; 0 = side 0,
+ * anything else side 1) receives the refined sides, *min_cut the value
+ * fm_refine returns (twice the cut weight).  info (host, 4, may be NULL):
+ * moves made, index of the last kept move (-1 = none kept), |balance| of
+ * the result, the largest weighted degree.  The largest weighted degree
+ * must be below 2^30.
+ */
+int mi_fm_refine(mi_ctx* ctx, const mi_graph* g, const uint32_t* vtx_w, uint32_t* part,
+                 uint32_t* min_cut, int64_t* info);
+/*
+ * The FM path's initial partition of a coarsest graph, chosen by size as
+ * eigensolve_kokkos.h:1510-1514 does: init_gggp (:1242-1320) for at most
+ * 200 vertices, else init_spectral (:1322-1368) with the XorShift64 pool of
+ * `seed` and power-iteration tolerance tol.  part (host, n) receives 0/1.
+ */
+int mi_fm_init(mi_ctx* ctx, const mi_graph* g, const uint32_t* vtx_w, uint64_t seed, double tol,
+               uint32_t* part);
 /* One reference power iteration (eigensolve_kokkos.h:102-206) on graph g,
    in place on host vector u (n); *iters receives the iteration count. */
 int mi_power_iter(mi_ctx* ctx, const mi_graph* g, double* u, int normLap, double tol,

@@ -92,6 +92,10 @@ _SIGS = [
     ("mi_ctx_set_small_levels", C.c_int, [vp, C.c_uint32, C.c_uint32]),
     ("mi_ctx_set_locality", C.c_int, [vp, C.c_int, C.c_uint32]),
     ("mi_ctx_set_bucketed_build", C.c_int, [vp, C.c_uint32]),
+    ("mi_ctx_set_refinement", C.c_int, [vp, C.c_int]),
+    ("mi_fm_eigensolve", C.c_int, [vp, vp, C.c_uint64, C.c_double, vp, C.POINTER(C.c_uint32), C.POINTER(Stats)]),
+    ("mi_fm_refine", C.c_int, [vp, vp, vp, u32p, C.POINTER(C.c_uint32), vp]),
+    ("mi_fm_init", C.c_int, [vp, vp, vp, C.c_uint64, C.c_double, u32p]),
     ("mi_graph_create", C.c_int, [vp, C.c_uint32, u32p, u32p, vp, C.POINTER(vp)]),
     ("mi_graph_from_edges", C.c_int, [vp, C.c_uint32, C.c_uint64, u32p, u32p, vp, C.c_int, C.POINTER(vp)]),
     ("mi_graph_create_device", C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
@@ -218,6 +222,11 @@ class Context:
         bucketed form (0: every level; the default: none); results are
         identical."""
         self.check(self._L.mi_ctx_set_bucketed_build(self.h, min_entries), "mi_ctx_set_bucketed_build")
+
+    def set_refinement(self, fm: bool) -> None:
+        """Bisections refine by FM, as the reference's *_fm binaries (True), or
+        by the multilevel power iteration (False, the default)."""
+        self.check(self._L.mi_ctx_set_refinement(self.h, 1 if fm else 0), "mi_ctx_set_refinement")
 
     def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:
         """Levels of <= 16384 vertices and <= max_entries entries iterate in one
@@ -365,6 +374,26 @@ class Graph:
                 "mi_power_iter")
         return u, it.value, bool(mr.value)
 
+    def fm_refine(self, part, vtx_w=None):
+        """One fm_refine call (eigensolve_kokkos.h:973-1240): (sides, min_cut,
+        info dict(swaps, argmin, min_imb, maxE))."""
+        c = self.ctx
+        p = np.ascontiguousarray(np.asarray(part) != 0, dtype=np.uint32)
+        vw = None if vtx_w is None else _u32(vtx_w)
+        mc = C.c_uint32()
+        info = np.zeros(4, dtype=np.int64)
+        c.check(c._L.mi_fm_refine(c.h, self.h, _ptr(vw), p, C.byref(mc), _ptr(info)), "mi_fm_refine")
+        return p, mc.value, dict(swaps=int(info[0]), argmin=int(info[1]), min_imb=int(info[2]), maxE=int(info[3]))
+
+    def fm_init(self, vtx_w=None, seed: int = 0, tol: float = 1e-10) -> np.ndarray:
+        """The FM path's coarsest-level partition: init_gggp (n <= 200) or
+        init_spectral (eigensolve_kokkos.h:1242-1368)."""
+        c = self.ctx
+        vw = None if vtx_w is None else _u32(vtx_w)
+        p = np.empty(self.n, dtype=np.uint32)
+        c.check(c._L.mi_fm_init(c.h, self.h, _ptr(vw), seed, tol, p), "mi_fm_init")
+        return p
+
     def compute_partition(self, evec, num_partitions: int = 2, imbalance: int = 0):
         c = self.ctx
         part = np.empty(self.n, dtype=np.uint32)
@@ -485,6 +514,19 @@ class Hierarchy:
         c.check(c._L.mi_fiedler(c.h, self.h, C.byref(st), pcg_inc, normLap, tol, _ptr(ev), C.byref(stats)),
                 "mi_fiedler")
         return ev, stats.as_dict(), st.value
+
+
+    def fm_eigensolve(self, seed: int = 0, tol: float = 1e-10):
+        """FM branch of the eigensolve: (0/1 evec, last cut, stats dict)."""
+        c = self.ctx
+        n0 = C.c_uint32()
+        c._L.mi_hier_level_size(self.h, 0, C.byref(n0), None)
+        ev = np.empty(n0.value, dtype=np.float64)
+        cut = C.c_uint32()
+        stats = Stats()
+        c.check(c._L.mi_fm_eigensolve(c.h, self.h, seed, tol, _ptr(ev), C.byref(cut), C.byref(stats)),
+                "mi_fm_eigensolve")
+        return ev, cut.value, stats.as_dict()
 
 
 @dataclass

@@ -475,6 +475,13 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
     return ST_OK;
 }
 
+int mi_ctx_set_refinement(mi_ctx* ctx, int fm)
+{
+    if (!ctx || (fm != 0 && fm != 1)) return ST_ARG;
+    ctx->c.fm = fm != 0;
+    return ST_OK;
+}
+
 int mi_ctx_set_small_levels(mi_ctx* ctx, uint32_t max_entries, uint32_t chunk)
 {
     if (!ctx) return ST_ARG;
@@ -715,6 +722,84 @@ int mi_fiedler(mi_ctx* ctx, const mi_hier* h, uint64_t* pcg_state, uint64_t pcg_
     });
 }
 
+int mi_fm_eigensolve(mi_ctx* ctx, const mi_hier* h, uint64_t seed, double tol, double* evec, uint32_t* last_cut,
+                     mi_stats* stats)
+{
+    if (!ctx || !h) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        auto t0 = std::chrono::steady_clock::now();
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        const uint32_t n0 = h->lv[0].g.n;
+        DBuf<uint32_t> fp(&c.pool, n0);
+        std::vector<uint32_t> passes(h->lv.size());
+        std::vector<double> lms(h->lv.size());
+        const uint32_t cut = fm_eigensolve(c, h->lv, seed, tol, fp.get(), passes.data(), stats ? lms.data() : nullptr);
+        if (evec) {
+            DBuf<double> ev(&c.pool, n0);
+            part_to_ev(c, n0, fp.get(), ev.get());
+            d2h(c, evec, ev.get(), (size_t)n0 * 8);
+        }
+        MI_HIP(hipStreamSynchronize(c.stream));
+        if (last_cut) *last_cut = cut;
+        if (stats) {
+            stats->eigen_ms = ms_since(t0);
+            fill_level_stats(h->lv, stats);
+            for (size_t l = 0; l < h->lv.size() && l < MI_MAX_LEVELS; l++) {
+                stats->level_iters[l] = passes[l];
+                stats->level_eigen_ms[l] = lms[l];
+            }
+        }
+    });
+}
+
+int mi_fm_refine(mi_ctx* ctx, const mi_graph* g, const uint32_t* vtx_w, uint32_t* part, uint32_t* min_cut,
+                 int64_t* info)
+{
+    if (!ctx || !g || !part || !min_cut) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        MI_CHECK(n > 0, ST_ARG, "empty graph");
+        DBuf<uint32_t> vw;
+        if (vtx_w) {
+            vw = DBuf<uint32_t>(&c.pool, n);
+            h2d(c, vw.get(), vtx_w, (size_t)n * 4);
+        }
+        std::vector<uint32_t> side(n);
+        for (uint32_t i = 0; i < n; i++) side[i] = part[i] != 0 ? 1u : 0u;
+        FmLevel F(c, g->g, vw.get());
+        h2d(c, F.st.get(), side.data(), (size_t)n * 4);
+        long long inf[4] = {0, 0, 0, 0};
+        *min_cut = fm_pass(c, F, inf);
+        DBuf<uint32_t> out(&c.pool, n);
+        fm_sides(c, F, out.get());
+        d2h(c, part, out.get(), (size_t)n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+        if (info)
+            for (int k = 0; k < 4; k++) info[k] = inf[k];
+    });
+}
+
+int mi_fm_init(mi_ctx* ctx, const mi_graph* g, const uint32_t* vtx_w, uint64_t seed, double tol, uint32_t* part)
+{
+    if (!ctx || !g || !part) return ST_ARG;
+    return guard(ctx, [&] {
+        Ctx& c = ctx->c;
+        const uint32_t n = g->g.n;
+        MI_CHECK(n > 0, ST_ARG, "empty graph");
+        DBuf<uint32_t> vw;
+        if (vtx_w) {
+            vw = DBuf<uint32_t>(&c.pool, n);
+            h2d(c, vw.get(), vtx_w, (size_t)n * 4);
+        }
+        DBuf<uint32_t> out(&c.pool, n);
+        fm_init(c, g->g, vw.get(), seed, tol, out.get());
+        d2h(c, part, out.get(), (size_t)n * 4);
+        MI_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
 int mi_power_iter(mi_ctx* ctx, const mi_graph* g, double* u, int normLap, double tol, uint64_t* iters,
                   int* max_reached)
 {
@@ -778,7 +863,24 @@ uint32_t bisect(Ctx& c, const DGraph& g, const mi_config* config, int perc_imbal
     build_hierarchy(c, g, hec_seed, lv, stats);
     MI_HIP(hipStreamSynchronize(c.stream));
     auto t1 = std::chrono::steady_clock::now();
-    DBuf<double> ev = fiedler(c, lv, pcg_state, pcg_inc, config->refine_alg, config->tol, stats);
+    DBuf<double> ev;
+    if (c.fm) {
+        // the *_fm binaries: FM refinement in place of the power iterations,
+        // its 0/1 partition handed to the split as the eigenvector
+        DBuf<uint32_t> fp(&c.pool, g.n);
+        std::vector<uint32_t> passes(lv.size());
+        std::vector<double> lms(lv.size());
+        fm_eigensolve(c, lv, hec_seed, config->tol, fp.get(), passes.data(), stats ? lms.data() : nullptr);
+        ev = DBuf<double>(&c.pool, g.n);
+        part_to_ev(c, g.n, fp.get(), ev.get());
+        if (stats)
+            for (size_t l = 0; l < lv.size() && l < MI_MAX_LEVELS; l++) {
+                stats->level_iters[l] = passes[l];
+                stats->level_eigen_ms[l] = lms[l];
+            }
+    } else {
+        ev = fiedler(c, lv, pcg_state, pcg_inc, config->refine_alg, config->tol, stats);
+    }
     MI_HIP(hipStreamSynchronize(c.stream));
     auto t2 = std::chrono::steady_clock::now();
     fill_level_stats(lv, stats);
@@ -805,6 +907,18 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
                  const std::vector<uint32_t*>& d_parts, uint32_t* edge_cuts, mi_stats* stats, double* phase_ms)
 {
     if (gs.empty()) return;
+    if (c.fm) {
+        // FM refinement is one search per level and graph: graph by graph
+        auto t0 = std::chrono::steady_clock::now();
+        for (size_t k = 0; k < gs.size(); k++)
+            edge_cuts[k] = bisect(c, *gs[k], config, perc_imbalance, hec_seeds[k], chain ? &pcg_states[0] : &pcg_states[k],
+                                  pcg_inc, d_parts[k], stats ? &stats[k] : nullptr);
+        if (phase_ms) {
+            phase_ms[0] = phase_ms[2] = 0.0;
+            phase_ms[1] = ms_since(t0);
+        }
+        return;
+    }
     const int normLap = config->refine_alg;
     const double tol = config->tol;
     const size_t K = gs.size();

@@ -163,6 +163,7 @@ struct Ctx {
     uint32_t small_max_m = 1u << 17;      // levels of <= 16384 vertices and this many entries: one workgroup
     uint32_t small_chunk = 8192;          // iterations per launch of that workgroup at most
     uint32_t bucketed_min_m = 0xffffffffu;  // coarse builds of levels with this many entries: bucketed form
+    bool fm = false;                      // bisections refine by FM (the *_fm binaries) instead of power iterations
     bool locality = true;                 // solve gather-form levels in a locality order
     uint32_t locality_min_n = 65536;      // ... those with at least this many vertices
     ~Ctx();

@@ -250,5 +250,40 @@ void use_partition(Ctx& c, const DGraph& g, const uint32_t* part, DGraph& g0, DG
 // out[i] = 2 side(i) + sub-part of i in its side's graph (p0 / p1 may be null: 0)
 void compose_4way(Ctx& c, uint32_t n, const uint32_t* part, const uint32_t* mapping, const uint32_t* p0,
                   const uint32_t* p1, uint32_t* out);
+// order[k] = the vertex of rank k by (ev, index) ascending, with -0.0 == 0.0
+void order_by_value(Ctx& c, uint32_t n, const double* ev, uint32_t* order);
+
+// ---------------------------------------------------------------- FM refinement
+// One level's state for fm_refine (eigensolve_kokkos.h:973-1240).  st[i]:
+// bit 0 = the side of vertex i (the caller sets it before the first pass),
+// bit 1 = free, used by the search.  maxE = the largest weighted degree;
+// the bucket lists of each side have NB = 2 maxE + 1 buckets.
+struct FmLevel {
+    const DGraph* g = nullptr;
+    const uint32_t* vtx_w = nullptr;  // null: unit vertex weights
+    uint32_t maxE = 0, NB = 0, NB1 = 0, NB2 = 0;
+    DBuf<uint32_t> st, nxt, prv, mark, order, heads, cnt1, cnt2;
+    DBuf<int32_t> gain;
+    DBuf<unsigned long long> acc;
+    FmLevel(Ctx& c, const DGraph& g, const uint32_t* vtx_w);
+};
+// One fm_refine call on L.st; returns its min_cut (twice the cut weight).
+// info (host, 4, may be null): moves made, kept prefix (-1 = none), the
+// kept |balance|, maxE.
+uint32_t fm_pass(Ctx& c, FmLevel& L, long long* info);
+// part[i] = side of i after the passes
+void fm_sides(Ctx& c, const FmLevel& L, uint32_t* part);
+// The coarsest level's partition of the FM path: init_gggp for at most 200
+// vertices, else init_spectral from the XorShift64 pool of seed
+// (eigensolve_kokkos.h:1242-1368, chosen as at :1510-1514).
+void fm_init(Ctx& c, const DGraph& g, const uint32_t* vtx_w, uint64_t seed, double tol, uint32_t* part);
+// The FM branch of sgp_eigensolve (eigensolve_kokkos.h:1506-1587): d_part0
+// (device, n0) receives the finest partition; passes / level_ms (host, one
+// per level, may be null) the fm_refine calls and the time of each level.
+// Returns the last min_cut.
+uint32_t fm_eigensolve(Ctx& c, const std::vector<DLevel>& lv, uint64_t seed, double tol, uint32_t* d_part0,
+                       uint32_t* passes, double* level_ms);
+// ev[i] = part[i] ? 1.0 : 0.0 (the partition as the reference's eigenvector)
+void part_to_ev(Ctx& c, uint32_t n, const uint32_t* part, double* ev);
 
 }  // namespace mi

@@ -144,6 +144,15 @@ uint32_t compute_partition(Ctx& c, const DGraph& g, const double* ev, int num_pa
     return (uint32_t)(best / 2);
 }
 
+void order_by_value(Ctx& c, uint32_t n, const double* ev, uint32_t* order)
+{
+    DBuf<uint64_t> keys(&c.pool, n), ktmp(&c.pool, n);
+    DBuf<uint32_t> vtmp(&c.pool, n);
+    k_ev_keys<<<grid1(n), BLOCK, 0, c.stream>>>(n, ev, keys.get(), order);
+    MI_HIP(hipGetLastError());
+    radix_sort_pairs(c, keys.get(), order, n, 0, 64, ktmp.get(), vtmp.get());
+}
+
 // ---------------------------------------------------------------- allocation-free split
 // The split of compute_partition with every temporary in one file-scope
 // device array, `scratch`, for callers that may not allocate device memory

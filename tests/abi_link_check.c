@@ -45,6 +45,10 @@ int main(void)
     int (*fp_mi_power_iter)(mi_ctx*, const mi_graph*, double*, int, double, uint64_t*, int*);
     int (*fp_mi_compute_partition)(mi_ctx*, const mi_graph*, const double*, int, int, uint32_t*, uint32_t*);
     int (*fp_mi_ctx_set_bucketed_build)(mi_ctx*, uint32_t);
+    int (*fp_mi_ctx_set_refinement)(mi_ctx*, int);
+    int (*fp_mi_fm_eigensolve)(mi_ctx*, const mi_hier*, uint64_t, double, double*, uint32_t*, mi_stats*);
+    int (*fp_mi_fm_refine)(mi_ctx*, const mi_graph*, const uint32_t*, uint32_t*, uint32_t*, int64_t*);
+    int (*fp_mi_fm_init)(mi_ctx*, const mi_graph*, const uint32_t*, uint64_t, double, uint32_t*);
     int (*fp_mi_compute_partition_noalloc)(mi_ctx*, const mi_graph*, const double*, int, int, uint32_t*,
                                            uint32_t*);
     int (*fp_mi_partition_graph)(mi_ctx*, const mi_graph*, const mi_config*, int, uint64_t, uint64_t*, uint64_t,
@@ -87,6 +91,10 @@ int main(void)
     BIND(mi_compute_partition);
     BIND(mi_compute_partition_noalloc);
     BIND(mi_ctx_set_bucketed_build);
+    BIND(mi_ctx_set_refinement);
+    BIND(mi_fm_eigensolve);
+    BIND(mi_fm_refine);
+    BIND(mi_fm_init);
     BIND(mi_partition_graph);
     BIND(mi_use_partition);
     BIND(mi_partition_4way);
