@@ -236,6 +236,65 @@ def extra_grid3d(mi, ctx, steps: int) -> dict:
     return out
 
 
+FM_GRAPHS = {}
+
+
+def _fm_graph(name: str):
+    from ipdps_2021_graph_coarsening_amd import graphs
+    if name not in FM_GRAPHS:
+        if name == "grid1024":
+            rp, col = graphs.grid2d(1024)[:2]
+            FM_GRAPHS[name] = (rp, col, None, "grid2d-1024 (n=1048576, 5-point, unit weights)", 1)
+        else:
+            rp, col, w = graphs.rmat(20, 16, 0.57, 0.19, 0.19, seed=1)
+            FM_GRAPHS[name] = (rp, col, w, f"rmat20-ef16-lcc-weighted (n={len(rp) - 1}, m={int(rp[-1])}, seed 1)", 1)
+    return FM_GRAPHS[name]
+
+
+def extra_fm(mi, ctx, steps: int) -> dict:
+    """Bisections with FM refinement (the reference's hec_fm binary,
+    mi_ctx_set_refinement): whole-bisection time, cut, fm_refine calls and
+    time per level."""
+    out = {}
+    ctx.set_refinement(True)
+    try:
+        for name, reps in (("grid1024", steps), ("rmat20", 1)):
+            rp, col, w, desc, seed = _fm_graph(name)
+            g = ctx.graph(rp, col, w)
+            R = Runner(mi, ctx, g, seed)
+            ms, st = R.timed(R.bisect, reps, 1, f"{name} fm")
+            R.close()
+            g.close()
+            d = st.as_dict()
+            L = d["nlevels"]
+            out[name] = {"workload": desc, "ms": round(ms, 3), "edge_cut": d["edge_cut"],
+                         "coarsen_ms": round(d["coarsen_ms"], 3), "fm_ms": round(d["eigen_ms"], 3),
+                         "split_ms": round(d["partition_ms"], 3), "level_n": d["level_n"][:L],
+                         "fm_calls_per_level": d["level_iters"][:L],
+                         "level_fm_ms": [round(x, 3) for x in d["level_eigen_ms"][:L]]}
+            log(f"FM bisection {name}: {ms:.2f} ms, cut {d['edge_cut']}")
+    finally:
+        ctx.set_refinement(False)
+    return out
+
+
+def cpu_baseline_fm() -> dict:
+    """The oracle's FM bisection (scalar C, one core) on the graphs of
+    extra_fm: the reference hec_fm binary's algorithm, every fm_refine
+    on host arrays."""
+    import oracle
+    out = {}
+    for name in ("grid1024", "rmat20"):
+        rp, col, w, desc, seed = _fm_graph(name)
+        t = time.perf_counter()
+        r = oracle.partition_graph_fm(rp, col, w, seed)
+        ms = (time.perf_counter() - t) * 1000.0
+        out[name] = {"value": round(ms, 1), "unit": "ms", "cores": 1, "kind": "port", "edge_cut": int(r["edge_cut"]),
+                     "sample": f"oracle/sgpar_oracle.c orc_partition_graph_fm on {desc}, one run"}
+        log(f"cpu FM baseline {name}: {ms:.0f} ms, cut {r['edge_cut']}")
+    return out
+
+
 def _rgg_job(args):
     from ipdps_2021_graph_coarsening_amd import graphs
     n, seed = args
@@ -370,7 +429,7 @@ def main() -> int:
                          "that calibrates rocprofv3's byte counters")
     ap.add_argument("--bisections-only", action="store_true",
                     help="only the timed FP64 bisections (for kernel traces of the whole bisection)")
-    ap.add_argument("--extra-only", choices=["rmat20", "grid3d", "batch"], default=None,
+    ap.add_argument("--extra-only", choices=["rmat20", "grid3d", "batch", "fm"], default=None,
                     help="only one of the extra workloads (for kernel traces of it), printed as JSON")
     ap.add_argument("--bucketed-build", type=int, default=None, metavar="MIN_ENTRIES",
                     help="coarse builds of levels with at least MIN_ENTRIES entries take the bucketed form "
@@ -394,9 +453,12 @@ def main() -> int:
         ctx.graph(np.array([0, 1, 2], dtype=np.uint32), np.array([1, 0], dtype=np.uint32)).close()
         steps = max(1, min(args.steps, 5))
         fn = {"rmat20": lambda: extra_rmat20(mi, ctx, steps), "grid3d": lambda: extra_grid3d(mi, ctx, steps),
-              "batch": lambda: extra_rgg_batch(mi, ctx)}[args.extra_only]
-        print(json.dumps({"extra": args.extra_only, "result": fn()}), flush=True)
+              "batch": lambda: extra_rgg_batch(mi, ctx), "fm": lambda: extra_fm(mi, ctx, steps)}[args.extra_only]
+        res = {"extra": args.extra_only, "result": fn()}
         ctx.close()
+        if args.extra_only == "fm" and not args.no_cpu_baseline:
+            res["cpu_baseline"] = cpu_baseline_fm()
+        print(json.dumps(res), flush=True)
         return 0
 
     t = time.perf_counter()
@@ -497,6 +559,7 @@ def main() -> int:
             log(f"R-MAT-20: FP64 {extras['rmat20']['fp64_ms']:.2f} ms, mixed {extras['rmat20']['ms']:.2f} ms")
         extras["grid3d"] = extra_grid3d(mi, ctx, max(1, min(args.steps, 5)))
         log(f"grid3d-128: FP64 {extras['grid3d']['fp64_ms']:.2f} ms, mixed {extras['grid3d']['ms']:.2f} ms")
+        extras["fm"] = extra_fm(mi, ctx, max(1, min(args.steps, 3)))
         if not args.no_batch:
             extras["batched_rgg512"] = extra_rgg_batch(mi, ctx)
             b = extras["batched_rgg512"]
@@ -511,6 +574,8 @@ def main() -> int:
         cpu = cpu_baseline(rp, col, w, seed, args.cpu_reps, args.cpu_budget)
         if cpu:
             log(f"cpu baseline {cpu['value']} ms ({time.perf_counter() - t:.1f} s)")
+        if cpu is not None and extras is not None and "fm" in extras:
+            cpu["fm"] = cpu_baseline_fm()
 
     out = {
         "metric": "bisection ms + SpMV HBM GB/s: FP32-mixed/batched vs plain FP64 at equal cut",

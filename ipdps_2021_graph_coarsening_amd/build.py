@@ -69,6 +69,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
 DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
+# the same compiled with -DFM, as the reference's *_fm binaries are
+DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
+DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -83,20 +86,22 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-DEXPERIMENT", "-I", ref_cmg, "-I", os.path.join(os.path.dirname(PKG), "include"),
               "-w"]
-    tmp = DROPIN_LIB + ".tmp"
-    # -Bsymbolic: calls inside the shim bind to its own definitions, not to
-    # copies a caller's translation unit of sgpar.h may define
-    # the version script exports the API only; the serial helpers the
-    # reference's header defines in every translation unit stay local and
-    # are dropped
-    _run([cxx, *common, "-fPIC", "-shared", "-ffunction-sections", os.path.join(DROPIN, "sgpar_dropin.cpp"),
-          "-o", tmp, "-L", PKG, "-l:libsgpar_mi.so", "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-Wl,--gc-sections",
-          f"-Wl,--version-script={os.path.join(DROPIN, 'sgpar_dropin.map')}"], verbose)
-    os.replace(tmp, DROPIN_LIB)
-    tmp = DROPIN_EXE + ".tmp"
-    _run([cxx, *common, os.path.join(DROPIN, "sgpar_dropin_main.cpp"), "-o", tmp, "-L", PKG,
-          "-l:libsgpar_dropin.so", "-Wl,-rpath,$ORIGIN/.."], verbose)
-    os.replace(tmp, DROPIN_EXE)
+    for lib, exe, extra in ((DROPIN_LIB, DROPIN_EXE, []), (DROPIN_FM_LIB, DROPIN_FM_EXE, ["-DFM"])):
+        tmp = lib + ".tmp"
+        # -Bsymbolic: calls inside the shim bind to its own definitions, not
+        # to copies a caller's translation unit of sgpar.h may define
+        # the version script exports the API only; the serial helpers the
+        # reference's header defines in every translation unit stay local
+        # and are dropped
+        _run([cxx, *common, *extra, "-fPIC", "-shared", "-ffunction-sections",
+              os.path.join(DROPIN, "sgpar_dropin.cpp"), "-o", tmp, "-L", PKG, "-l:libsgpar_mi.so",
+              "-Wl,-rpath,$ORIGIN", "-Wl,-Bsymbolic", "-Wl,--gc-sections",
+              f"-Wl,--version-script={os.path.join(DROPIN, 'sgpar_dropin.map')}"], verbose)
+        os.replace(tmp, lib)
+        tmp = exe + ".tmp"
+        _run([cxx, *common, *extra, os.path.join(DROPIN, "sgpar_dropin_main.cpp"), "-o", tmp, "-L", PKG,
+              f"-l:{os.path.basename(lib)}", "-Wl,-rpath,$ORIGIN/.."], verbose)
+        os.replace(tmp, exe)
     return True
 
 

@@ -769,7 +769,7 @@ int mi_fm_refine(mi_ctx* ctx, const mi_graph* g, const uint32_t* vtx_w, uint32_t
         std::vector<uint32_t> side(n);
         for (uint32_t i = 0; i < n; i++) side[i] = part[i] != 0 ? 1u : 0u;
         FmLevel F(c, g->g, vw.get());
-        h2d(c, F.st.get(), side.data(), (size_t)n * 4);
+        h2d(c, F.side.get(), side.data(), (size_t)n * 4);
         long long inf[4] = {0, 0, 0, 0};
         *min_cut = fm_pass(c, F, inf);
         DBuf<uint32_t> out(&c.pool, n);

@@ -105,15 +105,15 @@ __global__ void __launch_bounds__(BLOCK) k_fm_maxe(uint32_t n, const uint32_t* _
 
 // Gains, cut and balance of fm_create_ds (eigensolve_kokkos.h:230-262) and
 // fm_refine (:1030-1038); the bucket keys (side, gain + maxE) of :297-312.
-// st[i]: bit 0 the side, bit 1 "free".  acc[0] += twice the cut weight,
-// acc[1] += balance (two's complement).
+// rec[i].x = side | free << 1, rec[i].y = gain.  acc[0] += twice the cut
+// weight, acc[1] += balance (two's complement).
 __global__ void __launch_bounds__(BLOCK) k_fm_gains(uint32_t n, const uint32_t* __restrict__ rowptr,
                                                     const uint32_t* __restrict__ col,
                                                     const uint32_t* __restrict__ w,
-                                                    const uint32_t* __restrict__ vtx_w, uint32_t* st,
-                                                    int32_t* __restrict__ gain, uint32_t maxE, uint32_t NB,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                    unsigned long long* __restrict__ acc)
+                                                    const uint32_t* __restrict__ vtx_w,
+                                                    const uint32_t* __restrict__ side, uint32_t* __restrict__ rec,
+                                                    uint32_t maxE, uint32_t NB, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals, unsigned long long* __restrict__ acc)
 {
     __shared__ long long lds[2][BLOCK / WAVE];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
@@ -121,12 +121,12 @@ __global__ void __launch_bounds__(BLOCK) k_fm_gains(uint32_t n, const uint32_t* 
     const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
     long long cut = 0, bal = 0;
     for (uint64_t i = wid; i < n; i += nw) {
-        const uint32_t p = st[i] & 1u;
+        const uint32_t p = side[i] & 1u;
         const uint32_t r0 = rowptr[i], r1 = rowptr[i + 1];
         long long g = 0, c = 0;
         for (uint32_t e = r0 + lane; e < r1; e += WAVE) {
             const uint32_t x = w ? w[e] : 1u;
-            if ((st[col[e]] & 1u) != p) {
+            if ((side[col[e]] & 1u) != p) {
                 g += x;
                 c += x;
             } else {
@@ -136,10 +136,10 @@ __global__ void __launch_bounds__(BLOCK) k_fm_gains(uint32_t n, const uint32_t* 
         g = wave_sum_i64(g);
         c = wave_sum_i64(c);
         if (lane == 0) {
-            gain[i] = (int32_t)g;
+            rec[4 * i] = p | 2u;
+            rec[4 * i + 1] = (uint32_t)(int32_t)g;
             keys[i] = p * NB + (uint32_t)(g + (long long)maxE);
             vals[i] = (uint32_t)i;
-            st[i] = p | 2u;
             const long long vw = vtx_w ? vtx_w[i] : 1u;
             cut += c;
             bal += p == 0 ? vw : -vw;
@@ -166,17 +166,16 @@ __global__ void __launch_bounds__(BLOCK) k_fm_gains(uint32_t n, const uint32_t* 
 // vertices in increasing order, the smallest at the head).  The counts
 // per 64 / 4096 buckets get each run's length (two atomics per run).
 __global__ void k_fm_link(uint32_t n, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                          uint32_t* __restrict__ nxt, uint32_t* __restrict__ prv, uint32_t* __restrict__ heads,
-                          uint32_t* __restrict__ cnt1, uint32_t* __restrict__ cnt2, uint32_t NB, uint32_t NB1,
-                          uint32_t NB2)
+                          uint32_t* __restrict__ rec, uint32_t* __restrict__ heads, uint32_t* __restrict__ cnt1,
+                          uint32_t* __restrict__ cnt2, uint32_t NB, uint32_t NB1, uint32_t NB2)
 {
     const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
     if (p >= n) return;
     const uint32_t k = keys[p], v = vals[p];
     const bool first = p == 0 || keys[p - 1] != k;
     const bool last = p + 1 == n || keys[p + 1] != k;
-    prv[v] = first ? NIL : vals[p - 1];
-    nxt[v] = last ? NIL : vals[p + 1];
+    rec[4 * (size_t)v + 2] = first ? NIL : vals[p - 1];
+    rec[4 * (size_t)v + 3] = last ? NIL : vals[p + 1];
     const uint32_t P = k >= NB ? 1u : 0u, b = k - P * NB;
     uint32_t* c1 = &cnt1[P * NB1 + (b >> B1)];
     uint32_t* c2 = &cnt2[P * NB2 + (b >> B2)];
@@ -196,11 +195,7 @@ struct FmSearch {
     const uint32_t* col;
     const uint32_t* w;
     const uint32_t* vtx_w;
-    uint32_t* st;
-    int32_t* gain;
-    uint32_t* nxt;
-    uint32_t* prv;
-    uint32_t* mark;
+    uint4* rec;  // per vertex: x = side | free << 1, y = gain, z = previous, w = next in its bucket
     uint32_t* heads;
     uint32_t* cnt1;
     uint32_t* cnt2;
@@ -210,18 +205,24 @@ struct FmSearch {
     uint32_t n, maxE, NB, NB1, NB2;
 };
 
-// The highest bucket <= off of side P whose list is not empty, or -1.
-__device__ long long fm_top(const FmSearch& a, uint32_t P, long long off)
+// The highest bucket <= off of side P whose list is not empty, or -1;
+// *head receives its head (NIL for -1).
+__device__ long long fm_top(const FmSearch& a, uint32_t P, long long off, uint32_t* head)
 {
     const int lane = threadIdx.x;
     const uint32_t* H = a.heads + (size_t)P * a.NB;
     const uint32_t* C1 = a.cnt1 + (size_t)P * a.NB1;
     const uint32_t* C2 = a.cnt2 + (size_t)P * a.NB2;
+    *head = NIL;
     while (off >= 0) {
         const long long b = off - lane;
         const uint32_t h = b >= 0 ? ldc(&H[b]) : NIL;
         unsigned long long m = __ballot(h != NIL);
-        if (m) return off - (long long)__builtin_ctzll(m);
+        if (m) {
+            const int l = __builtin_ctzll(m);
+            *head = (uint32_t)__builtin_amdgcn_readlane((int)h, l);
+            return off - l;
+        }
         off -= WAVE;
         if (off < 0) return -1;
         // 64 counts of 64 buckets: the first one below that holds a vertex
@@ -246,19 +247,32 @@ __device__ long long fm_top(const FmSearch& a, uint32_t P, long long off)
     return -1;
 }
 
+__device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
 // The search of fm_refine, eigensolve_kokkos.h:1017-1220, one wavefront.
 // The reference's seek offsets walk down one empty bucket per loop turn and
 // choose a side from the heads at the offsets; the moves that result are
 // those of choosing from each side's highest non-empty bucket (side 0 when
 // side 1 has none and the reverse; else by the sign of the balance, a zero
 // balance going to the higher bucket, side 1 on equal buckets), which is
-// what this loop does.  mark[] must be zero on entry.
+// what this loop does.  Each side's highest bucket and its head are kept in
+// registers (off[X], hd[X]) and followed through every change of that
+// bucket; the buckets are read only when that bucket runs empty.
+//
+// A step takes up to 64 entries of the moved vertex's row (one per lane) and
+// moves the free neighbours between lists.  Which of them are list
+// neighbours of each other is found in registers (each lane compares its
+// previous / next vertex with the other lanes' vertices), so a run of
+// adjacent leaving vertices is walked lane to lane, and the stores of the
+// removals and insertions of one step go to distinct words.  Every step
+// begins by waiting for the previous step's stores.
 __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
 {
     const int lane = threadIdx.x;
     const unsigned long long lt = lane ? (~0ull >> (WAVE - lane)) : 0ull;
     const long long maxE = a.maxE;
     const uint32_t NB = a.NB;
+    uint32_t* R32 = reinterpret_cast<uint32_t*>(a.rec);
     long long cut = (long long)a.acc[0];
     long long balance = (long long)a.acc[1];
     const long long start_bal = balance < 0 ? -balance : balance;
@@ -266,43 +280,57 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
     if (target < 20) target = 20;
     long long min_cut = cut, min_imb = start_bal, argmin = -1;
     int counter = 0;
-    uint32_t total = 0, stamp = 0;
+    uint32_t total = 0;
     int bad = 0;  // 1: a gain left +-maxE, 2: inconsistent lists
-    long long offA = 2 * maxE, offB = offA;
+    long long off0 = 2 * maxE, off1 = off0;
+    uint32_t hd0 = NIL, hd1 = NIL;
+    bool scan0 = true, scan1 = true;
     for (;;) {
-        if (offA >= 0) offA = fm_top(a, 0, offA);
-        if (offB >= 0) offB = fm_top(a, 1, offB);
-        if (offA < 0 && offB < 0) break;
+        wg_fence();
+        if (scan0 || hd0 == NIL) {
+            if (off0 >= 0) off0 = fm_top(a, 0, off0, &hd0);
+            scan0 = false;
+        }
+        if (scan1 || hd1 == NIL) {
+            if (off1 >= 0) off1 = fm_top(a, 1, off1, &hd1);
+            scan1 = false;
+        }
+        const bool has0 = hd0 != NIL, has1 = hd1 != NIL;
+        if (!has0 && !has1) break;
         uint32_t P;
-        if (offB < 0)
+        if (!has1)
             P = 0;
-        else if (offA < 0)
+        else if (!has0)
             P = 1;
         else if (balance > 0)
             P = 0;
         else if (balance < 0)
             P = 1;
         else
-            P = offA > offB ? 0u : 1u;
-        const long long top = P ? offB : offA;
+            P = off0 > off1 ? 0u : 1u;
+        const long long top = P ? off1 : off0;
         const uint32_t key = P * NB + (uint32_t)top;
-        const uint32_t s = ldc(&a.heads[key]);
+        const uint32_t s = P ? hd1 : hd0;
         if (s >= a.n) {  // (a list head outside the graph: never on consistent lists)
             bad = 2;
             break;
         }
-        const uint32_t ns = ldc(&a.nxt[s]);
+        const uint32_t ns = ldc(&R32[4 * (size_t)s + 3]);
         const uint32_t vw = a.vtx_w ? a.vtx_w[s] : 1u;
         const uint32_t r0 = a.rowptr[s], r1 = a.rowptr[s + 1];
         const uint32_t sp = P ^ 1u;  // the side s moves to
         if (lane == 0) {
             a.heads[key] = ns;
-            if (ns != NIL) a.prv[ns] = NIL;
-            a.st[s] = sp;  // moved, no longer free
+            if (ns != NIL) R32[4 * (size_t)ns + 2] = NIL;
+            R32[4 * (size_t)s] = sp;  // moved, no longer free
             a.order[total] = s;
             atomicSub(&a.cnt1[P * a.NB1 + ((uint32_t)top >> B1)], 1u);
             atomicSub(&a.cnt2[P * a.NB2 + ((uint32_t)top >> B2)], 1u);
         }
+        if (P)
+            hd1 = ns;
+        else
+            hd0 = ns;
         cut -= 2 * (top - maxE);
         balance += P == 0 ? -2ll * vw : 2ll * vw;
         counter++;
@@ -320,7 +348,6 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
         argmin = imp ? (long long)total : argmin;
         counter = imp ? 0 : counter;
         total++;
-        wg_fence();
         for (uint32_t base = r0; base < r1; base += WAVE) {
             const uint32_t e = base + lane;
             const bool valid = e < r1;
@@ -333,7 +360,7 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
                 unsigned long long todo = pend;
                 while (todo) {
                     const int l = __builtin_ctzll(todo);
-                    const uint32_t vl = __shfl(v, l, WAVE);
+                    const uint32_t vl = rdl(v, l);
                     const unsigned long long gm = __ballot(valid && v == vl);
                     todo &= ~gm;
                     dup |= gm & (gm - 1);
@@ -343,16 +370,17 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
                 const unsigned long long step = dup ? (pend & (~pend + 1)) : pend;
                 pend &= ~step;
                 const bool mine = (step >> lane) & 1ull;
-                uint32_t sv = 0;
-                if (mine) sv = ldc(&a.st[v]);
-                const bool act = mine && (sv & 2u);
-                int32_t og = 0;
-                uint32_t pn = NIL, nn = NIL;
-                if (act) {
-                    og = ldc(&a.gain[v]);
-                    pn = ldc(&a.prv[v]);
-                    nn = ldc(&a.nxt[v]);
+                wg_fence();
+                uint32_t sv = 0, gv = 0, pn = NIL, nn = NIL;
+                if (mine) {
+                    const uint32_t* r = &R32[4 * (size_t)v];
+                    sv = ldc(r);
+                    gv = ldc(r + 1);
+                    pn = ldc(r + 2);
+                    nn = ldc(r + 3);
                 }
+                const bool act = mine && (sv & 2u);
+                const int32_t og = (int32_t)gv;
                 const uint32_t pv = sv & 1u;
                 const int32_t ng = og + (pv == sp ? -2 * (int32_t)wt : 2 * (int32_t)wt);
                 // on a symmetric graph a gain stays within +-maxE; anything
@@ -362,50 +390,43 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
                     bad = 1;
                     break;
                 }
-                const uint32_t ob = (uint32_t)(og + maxE), nb = (uint32_t)(ng + maxE);
-                const uint32_t okey = pv * NB + ob, nkey = pv * NB + nb;
-                stamp++;
-                if (act) a.mark[v] = stamp;
-                wg_fence();
-                // leave the old lists: the first of each run of leaving
-                // vertices links its predecessor to the first stayer after
-                bool broken = act && ((pn != NIL && pn >= a.n) || (nn != NIL && nn >= a.n));
-                if (act && !broken) {
-                    if (pn == NIL || ldc(&a.mark[pn]) != stamp) {
-                        // a run holds at most the 64 vertices marked now
-                        uint32_t q = nn;
-                        for (int k = 0; q != NIL; k++) {
-                            if (q >= a.n || k == WAVE) {
-                                broken = true;
-                                break;
-                            }
-                            if (ldc(&a.mark[q]) != stamp) break;
-                            q = ldc(&a.nxt[q]);
-                        }
-                        if (!broken) {
-                            if (pn == NIL)
-                                a.heads[okey] = q;
-                            else
-                                a.nxt[pn] = q;
-                            if (q != NIL) a.prv[q] = pn;
-                        }
-                    }
-                    atomicSub(&a.cnt1[pv * a.NB1 + (ob >> B1)], 1u);
-                    atomicSub(&a.cnt2[pv * a.NB2 + (ob >> B2)], 1u);
-                }
-                if (__ballot(broken)) {
+                if (__ballot(act && ((pn != NIL && pn >= a.n) || (nn != NIL && nn >= a.n)))) {
                     bad = 2;
                     break;
                 }
-                wg_fence();
-                // enter the new lists: per bucket, the lanes in reverse
-                // order in front of the old head
+                const uint32_t ob = (uint32_t)(og + maxE), nb = (uint32_t)(ng + maxE);
+                const uint32_t okey = pv * NB + ob, nkey = pv * NB + nb;
+                const unsigned long long actm = __ballot(act);
+                // the lanes holding this lane's list neighbours, if they leave too
+                int pl = -1, sl = -1;
+                for (unsigned long long t = actm; t; t &= t - 1) {
+                    const int j = __builtin_ctzll(t);
+                    const uint32_t vj = rdl(v, j);
+                    if (act && pn == vj) pl = j;
+                    if (act && nn == vj) sl = j;
+                }
+                // the first lane of each run of leaving vertices finds the
+                // first vertex after the run that stays
+                const bool first = act && pl < 0;
+                uint32_t q = nn;
+                int j = first ? sl : -1;
+                for (int k = 0; k < WAVE; k++) {
+                    if (!__ballot(j >= 0)) break;
+                    const int src = j < 0 ? lane : j;
+                    const uint32_t qn = __shfl(nn, src, WAVE);
+                    const int jn = __shfl(sl, src, WAVE);
+                    if (j >= 0) {
+                        q = qn;
+                        j = jn;
+                    }
+                }
+                // lanes moving into the same bucket, in lane order
                 unsigned long long grp = 0;
                 {
-                    unsigned long long todo = __ballot(act);
+                    unsigned long long todo = actm;
                     while (todo) {
                         const int l = __builtin_ctzll(todo);
-                        const uint32_t kl = __shfl(nkey, l, WAVE);
+                        const uint32_t kl = rdl(nkey, l);
                         const unsigned long long gm = __ballot(act && nkey == kl);
                         todo &= ~gm;
                         if (act && nkey == kl) grp = gm;
@@ -416,33 +437,78 @@ __global__ void __launch_bounds__(WAVE) k_fm_search(FmSearch a)
                 const int lo = below ? 63 - __builtin_clzll(below) : lane;
                 const int hi = above ? __builtin_ctzll(above) : lane;
                 const uint32_t vlo = __shfl(v, lo, WAVE), vhi = __shfl(v, hi, WAVE);
+                // the old head of each target bucket: where a run left from
+                // the head of that bucket, the vertex after the run; else
+                // the stored head.  A run leaving from the head of a bucket
+                // that also receives lanes leaves its head and back link to
+                // the insertion.
+                const bool hrm = first && pn == NIL;
                 uint32_t oh = NIL;
-                if (act) oh = ldc(&a.heads[nkey]);
+                bool ohk = false, ins_here = false;
+                for (unsigned long long t = __ballot(hrm); t; t &= t - 1) {
+                    const int l = __builtin_ctzll(t);
+                    const uint32_t kl = rdl(okey, l), ql = rdl(q, l);
+                    if (act && nkey == kl) {
+                        oh = ql;
+                        ohk = true;
+                    }
+                    const bool hit = __ballot(act && nkey == kl) != 0;
+                    if (lane == l) ins_here = hit;
+                }
+                if (act && !ohk) oh = ldc(&a.heads[nkey]);
+                if (first) {
+                    if (pn == NIL) {
+                        if (!ins_here) {
+                            a.heads[okey] = q;
+                            if (q != NIL) R32[4 * (size_t)q + 2] = NIL;
+                        }
+                    } else {
+                        R32[4 * (size_t)pn + 3] = q;
+                        if (q != NIL) R32[4 * (size_t)q + 2] = pn;
+                    }
+                }
                 if (act) {
-                    a.nxt[v] = below ? vlo : oh;
-                    a.prv[v] = above ? vhi : NIL;
-                    if (!below && oh < a.n) a.prv[oh] = v;
-                    // (the store waits for the load of the same word)
-                    if (!above && oh != NIL - 1u) a.heads[nkey] = v;
-                    a.gain[v] = ng;
+                    atomicSub(&a.cnt1[pv * a.NB1 + (ob >> B1)], 1u);
+                    atomicSub(&a.cnt2[pv * a.NB2 + (ob >> B2)], 1u);
+                    a.rec[v] = make_uint4(sv, (uint32_t)ng, above ? vhi : NIL, below ? vlo : oh);
+                    if (!below && oh < a.n) R32[4 * (size_t)oh + 2] = v;
+                    if (!above) a.heads[nkey] = v;
                     atomicAdd(&a.cnt1[pv * a.NB1 + (nb >> B1)], 1u);
                     atomicAdd(&a.cnt2[pv * a.NB2 + (nb >> B2)], 1u);
                 }
-                // the seek offsets move up to the highest bucket filled
-                const long long rA = wave_max_i64(act && pv == 0 ? (long long)nb : -1);
-                const long long rB = wave_max_i64(act && pv == 1 ? (long long)nb : -1);
-                if (rA > offA) offA = rA;
-                if (rB > offB) offB = rB;
-                wg_fence();
+                // each side's highest bucket and its head
+                {
+                    const long long bi = wave_max_i64(act && pv == 0 ? (long long)nb : -1);
+                    if (bi >= 0 && bi >= off0) {
+                        const unsigned long long hl = __ballot(act && pv == 0 && (long long)nb == bi && !above);
+                        off0 = bi;
+                        hd0 = rdl(v, __builtin_ctzll(hl));
+                    } else {
+                        const unsigned long long hl = __ballot(hrm && pv == 0 && (long long)ob == off0);
+                        if (hl) hd0 = rdl(q, __builtin_ctzll(hl));
+                    }
+                }
+                {
+                    const long long bi = wave_max_i64(act && pv == 1 ? (long long)nb : -1);
+                    if (bi >= 0 && bi >= off1) {
+                        const unsigned long long hl = __ballot(act && pv == 1 && (long long)nb == bi && !above);
+                        off1 = bi;
+                        hd1 = rdl(v, __builtin_ctzll(hl));
+                    } else {
+                        const unsigned long long hl = __ballot(hrm && pv == 1 && (long long)ob == off1);
+                        if (hl) hd1 = rdl(q, __builtin_ctzll(hl));
+                    }
+                }
             }
             if (bad) break;
         }
         if (bad || counter > 500) break;
     }
+    wg_fence();
     // undo the moves after the kept prefix (:1222-1233)
     for (uint32_t i = (uint32_t)(argmin + 1) + lane; i < total; i += WAVE) {
         const uint32_t s = a.order[i];
-        a.st[s] = ldc(&a.st[s]) ^ 1u;
+        R32[4 * (size_t)s] = ldc(&R32[4 * (size_t)s]) ^ 1u;
     }
     if (lane == 0) {
         a.res[0] = min_cut;
@@ -461,10 +527,10 @@ __global__ void k_fm_prolong(uint32_t n, const uint32_t* __restrict__ vcmap, con
     if (i < n) out[i] = in[vcmap[i]] & 1u;
 }
 
-__global__ void k_fm_side(uint32_t n, const uint32_t* __restrict__ st, uint32_t* __restrict__ out)
+__global__ void k_fm_side(uint32_t n, const uint32_t* __restrict__ rec, uint32_t* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) out[i] = st[i] & 1u;
+    if (i < n) out[i] = rec[4 * (size_t)i] & 1u;
 }
 
 __global__ void k_fm_to_ev(uint32_t n, const uint32_t* __restrict__ part, double* __restrict__ ev)
@@ -634,11 +700,8 @@ FmLevel::FmLevel(Ctx& c, const DGraph& g_, const uint32_t* vtx_w_) : g(&g_), vtx
     NB = 2 * maxE + 1;
     NB1 = (uint32_t)ceil_div(NB, 1u << B1);
     NB2 = (uint32_t)ceil_div(NB, 1u << B2);
-    st = DBuf<uint32_t>(&c.pool, n);
-    gain = DBuf<int32_t>(&c.pool, n);
-    nxt = DBuf<uint32_t>(&c.pool, n);
-    prv = DBuf<uint32_t>(&c.pool, n);
-    mark = DBuf<uint32_t>(&c.pool, n);
+    side = DBuf<uint32_t>(&c.pool, n);
+    rec = DBuf<uint4>(&c.pool, n);
     order = DBuf<uint32_t>(&c.pool, n);
     heads = DBuf<uint32_t>(&c.pool, 2ull * NB);
     cnt1 = DBuf<uint32_t>(&c.pool, 2ull * NB1);
@@ -651,29 +714,25 @@ uint32_t fm_pass(Ctx& c, FmLevel& L, long long* info)
     const DGraph& g = *L.g;
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "FM refinement of an empty graph");
-    fill_u32(c, L.mark.get(), n, 0u);
     fill_u32(c, L.heads.get(), 2ull * L.NB, NIL);
     fill_u32(c, L.cnt1.get(), 2ull * L.NB1, 0u);
     fill_u32(c, L.cnt2.get(), 2ull * L.NB2, 0u);
     fill_u32(c, reinterpret_cast<uint32_t*>(L.acc.get()), 16, 0u);
     DBuf<uint32_t> keys(&c.pool, n), vals(&c.pool, n), ktmp(&c.pool, n), vtmp(&c.pool, n);
-    k_fm_gains<<<wave_grid(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), L.vtx_w, L.st.get(), L.gain.get(),
-                                                     L.maxE, L.NB, keys.get(), vals.get(), L.acc.get());
+    uint32_t* rec = reinterpret_cast<uint32_t*>(L.rec.get());
+    k_fm_gains<<<wave_grid(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), L.vtx_w, L.side.get(), rec, L.maxE,
+                                                     L.NB, keys.get(), vals.get(), L.acc.get());
     MI_HIP(hipGetLastError());
     radix_sort_pairs(c, keys.get(), vals.get(), n, 0, (int)bits_for(2ull * L.NB - 1), ktmp.get(), vtmp.get());
-    k_fm_link<<<grid1(n), BLOCK, 0, c.stream>>>(n, keys.get(), vals.get(), L.nxt.get(), L.prv.get(),
-                                                L.heads.get(), L.cnt1.get(), L.cnt2.get(), L.NB, L.NB1, L.NB2);
+    k_fm_link<<<grid1(n), BLOCK, 0, c.stream>>>(n, keys.get(), vals.get(), rec, L.heads.get(), L.cnt1.get(),
+                                                L.cnt2.get(), L.NB, L.NB1, L.NB2);
     MI_HIP(hipGetLastError());
     FmSearch a;
     a.rowptr = g.R();
     a.col = g.Cc();
     a.w = g.W();
     a.vtx_w = L.vtx_w;
-    a.st = L.st.get();
-    a.gain = L.gain.get();
-    a.nxt = L.nxt.get();
-    a.prv = L.prv.get();
-    a.mark = L.mark.get();
+    a.rec = L.rec.get();
     a.heads = L.heads.get();
     a.cnt1 = L.cnt1.get();
     a.cnt2 = L.cnt2.get();
@@ -691,6 +750,8 @@ uint32_t fm_pass(Ctx& c, FmLevel& L, long long* info)
     read_words(c, L.acc.get() + 2, res, sizeof(res));
     MI_CHECK(res[4] != 1, ST_ARG, "FM refinement: a gain left +-maxE (the graph is not symmetric)");
     MI_CHECK(res[4] == 0, ST_INTERNAL, "FM refinement: inconsistent bucket lists");
+    k_fm_side<<<grid1(n), BLOCK, 0, c.stream>>>(n, rec, L.side.get());
+    MI_HIP(hipGetLastError());
     if (info) {
         info[0] = res[1];
         info[1] = res[2];
@@ -702,8 +763,7 @@ uint32_t fm_pass(Ctx& c, FmLevel& L, long long* info)
 
 void fm_sides(Ctx& c, const FmLevel& L, uint32_t* part)
 {
-    k_fm_side<<<grid1(L.g->n), BLOCK, 0, c.stream>>>(L.g->n, L.st.get(), part);
-    MI_HIP(hipGetLastError());
+    MI_HIP(hipMemcpyAsync(part, L.side.get(), (size_t)L.g->n * 4, hipMemcpyDeviceToDevice, c.stream));
 }
 
 void fm_init(Ctx& c, const DGraph& g, const uint32_t* vtx_w, uint64_t seed, double tol, uint32_t* part)
@@ -750,7 +810,7 @@ uint32_t fm_eigensolve(Ctx& c, const std::vector<DLevel>& lv, uint64_t seed, dou
     for (int l = L - 1; l >= 0; l--) {
         auto t0 = std::chrono::steady_clock::now();
         FmLevel F(c, lv[l].g, lv[l].vtx_w.get());
-        MI_HIP(hipMemcpyAsync(F.st.get(), cur.get(), (size_t)lv[l].g.n * 4, hipMemcpyDeviceToDevice, c.stream));
+        MI_HIP(hipMemcpyAsync(F.side.get(), cur.get(), (size_t)lv[l].g.n * 4, hipMemcpyDeviceToDevice, c.stream));
         uint32_t old, np = 0;
         do {  // eigensolve_kokkos.h:1533-1537, 1566-1570
             old = cut;
@@ -761,7 +821,7 @@ uint32_t fm_eigensolve(Ctx& c, const std::vector<DLevel>& lv, uint64_t seed, dou
         if (l > 0) {
             const uint32_t nf = lv[l - 1].g.n;
             DBuf<uint32_t> fine(&c.pool, nf);
-            k_fm_prolong<<<grid1(nf), BLOCK, 0, c.stream>>>(nf, lv[l - 1].vcmap.get(), F.st.get(), fine.get());
+            k_fm_prolong<<<grid1(nf), BLOCK, 0, c.stream>>>(nf, lv[l - 1].vcmap.get(), F.side.get(), fine.get());
             MI_HIP(hipGetLastError());
             cur = std::move(fine);
         } else {

@@ -254,20 +254,21 @@ void compose_4way(Ctx& c, uint32_t n, const uint32_t* part, const uint32_t* mapp
 void order_by_value(Ctx& c, uint32_t n, const double* ev, uint32_t* order);
 
 // ---------------------------------------------------------------- FM refinement
-// One level's state for fm_refine (eigensolve_kokkos.h:973-1240).  st[i]:
-// bit 0 = the side of vertex i (the caller sets it before the first pass),
-// bit 1 = free, used by the search.  maxE = the largest weighted degree;
-// the bucket lists of each side have NB = 2 maxE + 1 buckets.
+// One level's state for fm_refine (eigensolve_kokkos.h:973-1240).
+// side[i] = the side of vertex i (0 / 1; the caller sets it before the
+// first pass, each pass leaves its result there).  rec[i] = the search's
+// record of vertex i (side and free flag, gain, bucket-list links).  maxE =
+// the largest weighted degree; each side has NB = 2 maxE + 1 buckets.
 struct FmLevel {
     const DGraph* g = nullptr;
     const uint32_t* vtx_w = nullptr;  // null: unit vertex weights
     uint32_t maxE = 0, NB = 0, NB1 = 0, NB2 = 0;
-    DBuf<uint32_t> st, nxt, prv, mark, order, heads, cnt1, cnt2;
-    DBuf<int32_t> gain;
+    DBuf<uint32_t> side, order, heads, cnt1, cnt2;
+    DBuf<uint4> rec;
     DBuf<unsigned long long> acc;
     FmLevel(Ctx& c, const DGraph& g, const uint32_t* vtx_w);
 };
-// One fm_refine call on L.st; returns its min_cut (twice the cut weight).
+// One fm_refine call on L.side; returns its min_cut (twice the cut weight).
 // info (host, 4, may be null): moves made, kept prefix (-1 = none), the
 // kept |balance|, maxE.
 uint32_t fm_pass(Ctx& c, FmLevel& L, long long* info);

@@ -33,6 +33,13 @@
 //   pin (the reference pins time() itself for reproducible runs);
 // - the graph upload and every device buffer belong to one library context
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
+//
+// Compiled with -DFM (as the reference's *_fm binaries are,
+// cmg/CMakeLists.txt:50) the context refines by FM (mi_ctx_set_refinement)
+// and the metrics record what the reference's FM path records: no coarse
+// level per power iteration (that path runs none) and the FM time under
+// fm-refine (eigensolve_kokkos.h:1236; here the whole FM eigensolve,
+// initial partition and prolongations included).
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -57,6 +64,12 @@ mi_ctx* context()
     if (!g_ctx) {
         const char* dev = std::getenv("SGPAR_MI_DEVICE");
         if (mi_ctx_create(dev ? std::atoi(dev) : 0, &g_ctx) != 0) g_ctx = nullptr;
+#ifdef FM
+        if (g_ctx && mi_ctx_set_refinement(g_ctx, 1) != 0) {
+            mi_ctx_destroy(g_ctx);
+            g_ctx = nullptr;
+        }
+#endif
     }
     return g_ctx;
 }
@@ -254,9 +267,13 @@ int sgp_partition_graph(sgp_vid_t* part, sgp_eid_t* edge_cut, config_t* config, 
     // coarsest first (eigensolve_kokkos.h:198); map and build times
     // (coarsen_kokkos.h:1151,1159) with the map's permutation, heavy
     // neighbour and construction parts (heuristics.h:956,999,1008)
+    using M = ExperimentLoggerUtil::Measurement;
+#ifdef FM
+    experiment.addMeasurement(M::FMRefine, st.eigen_ms * 1e-3);
+#else
     for (int l = st.nlevels - 1; l >= 0; l--)
         experiment.addCoarseLevel((int)st.level_iters[l], st.level_max_reached[l] != 0, st.level_n[l]);
-    using M = ExperimentLoggerUtil::Measurement;
+#endif
     experiment.addMeasurement(M::Map, (st.permute_ms + st.heavy_ms + st.map_ms) * 1e-3);
     experiment.addMeasurement(M::Permute, st.permute_ms * 1e-3);
     experiment.addMeasurement(M::Heavy, st.heavy_ms * 1e-3);

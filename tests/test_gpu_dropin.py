@@ -117,3 +117,20 @@ def test_dropin_partition_diff_against_itself(driver, tmp_path):
     trials, part2, _ = _run(driver, tmp_path, rp, col, [0, 0, 0, 2, 1, 1e-10], 5, best=best)
     assert trials[0]["partition-diff"] == 0
     assert np.array_equal(part, part2)
+
+
+def test_dropin_fm_equals_oracle_fm_bisection(tmp_path):
+    """The drop-in compiled with -DFM (the reference's hec_fm build) bisects
+    with FM refinement: partition and cut equal the oracle's FM bisection
+    with the pinned time as its seed; no coarse level is recorded (the FM
+    path runs no power iteration) and the FM time is."""
+    import oracle
+    if not (os.path.exists(mibuild.DROPIN_FM_EXE) and os.path.exists(mibuild.DROPIN_FM_LIB)):
+        pytest.skip("the FM drop-in was not built (it needs the reference's cmg headers at build time)")
+    rp, col, _ = graphs.rgg(20000, seed=8)
+    trials, part, _ = _run(mibuild.DROPIN_FM_EXE, tmp_path, rp, col, [0, 0, 0, 2, 1, 1e-10], 4242)
+    o = oracle.partition_graph_fm(rp, col, None, 4242)
+    (t,) = trials
+    assert t["edge-cut"] == o["edge_cut"]
+    assert np.array_equal(part, o["part"])
+    assert t["number-coarse-levels"] == 0 and t["fm-refine-duration-seconds"] > 0
