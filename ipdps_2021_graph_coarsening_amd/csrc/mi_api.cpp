@@ -14,6 +14,9 @@
 #include <memory>
 #include <new>
 #include <sstream>
+#include <thread>
+#include <mutex>
+#include <exception>
 
 namespace mi {
 
@@ -897,6 +900,69 @@ uint32_t bisect(Ctx& c, const DGraph& g, const mi_config* config, int perc_imbal
     return cut;
 }
 
+// Worker contexts of the batched bisections (their own stream, pool and
+// reduction scratch), made once per context and kept; each call copies the
+// level policies of c into them.
+constexpr size_t BATCH_WORKERS = 8;
+std::vector<Ctx*> batch_workers(Ctx& c, size_t want)
+{
+    while (c.workers.size() < want) {
+        auto w = std::make_unique<Ctx>();
+        ctx_init(*w, c.device);
+        c.workers.push_back(std::move(w));
+    }
+    MI_HIP(hipSetDevice(c.device));
+    std::vector<Ctx*> out;
+    for (size_t i = 0; i < want; i++) {
+        Ctx& w = *c.workers[i];
+        w.use_graphs = c.use_graphs;
+        w.mixed = c.mixed;
+        w.mixed_min_m = c.mixed_min_m;
+        w.small_max_m = c.small_max_m;
+        w.small_chunk = c.small_chunk;
+        w.bucketed_min_m = c.bucketed_min_m;
+        w.locality = c.locality;
+        w.locality_min_n = c.locality_min_n;
+        w.fm = c.fm;
+        out.push_back(&w);
+    }
+    return out;
+}
+
+// f(worker, i) for i = 0 .. K-1, one host thread per worker taking the next
+// i; returns when every worker's stream has drained.  The first error of any
+// worker is rethrown after all have stopped.
+extern "C++" {
+template <class F>
+void run_on_workers(const std::vector<Ctx*>& ws, size_t K, F&& f)
+{
+    std::atomic<size_t> next{0};
+    std::mutex mu;
+    std::exception_ptr err;
+    std::vector<std::thread> th;
+    th.reserve(ws.size());
+    for (Ctx* w : ws)
+        th.emplace_back([&, w] {
+            try {
+                MI_HIP(hipSetDevice(w->device));
+                for (;;) {
+                    const size_t i = next.fetch_add(1);
+                    if (i >= K) break;
+                    f(*w, i);
+                }
+                MI_HIP(hipStreamSynchronize(w->stream));
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(mu);
+                if (!err) err = std::current_exception();
+                next.store(K);
+                (void)hipStreamSynchronize(w->stream);
+            }
+        });
+    for (auto& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
+}
+
 // Bisections of the graphs gs, with the power iterations of their small
 // levels run together: per graph the same result as bisect().  chain: one
 // PCG state (pcg_states[0]) carried from graph to graph in order, as
@@ -923,16 +989,24 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
     const double tol = config->tol;
     const size_t K = gs.size();
     auto t0 = std::chrono::steady_clock::now();
-    // hierarchies, graph by graph
+    // the graphs' own work (hierarchies; the levels above the batched ones
+    // and the splits) runs on worker contexts, one host thread and stream
+    // each, largest graphs first; results are those of one call per graph
+    std::vector<Ctx*> ws = batch_workers(c, std::min<size_t>(K, BATCH_WORKERS));
+    std::vector<size_t> by_size(K);
+    for (size_t k = 0; k < K; k++) by_size[k] = k;
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](size_t x, size_t y) { return (uint64_t)gs[x]->m + gs[x]->n > (uint64_t)gs[y]->m + gs[y]->n; });
+    MI_HIP(hipStreamSynchronize(c.stream));  // the graphs' uploads
     std::vector<std::vector<DLevel>> H(K);
-    for (size_t k = 0; k < K; k++) {
-        mi_stats* st = stats ? &stats[k] : nullptr;
-        if (st) std::memset(st, 0, sizeof(*st));
+    for (size_t k = 0; k < K; k++)
+        if (stats) std::memset(&stats[k], 0, sizeof(stats[k]));
+    run_on_workers(ws, K, [&](Ctx& w, size_t i) {
+        const size_t k = by_size[i];
         // (no per-phase timing: it would synchronise once per level)
-        build_hierarchy(c, *gs[k], hec_seeds[k], H[k], nullptr);
-        fill_level_stats(H[k], st);
-    }
-    MI_HIP(hipStreamSynchronize(c.stream));
+        build_hierarchy(w, *gs[k], hec_seeds[k], H[k], nullptr);
+        fill_level_stats(H[k], stats ? &stats[k] : nullptr);
+    });
     auto t1 = std::chrono::steady_clock::now();
     // start vectors (consuming each graph's pcg state, as one call would)
     // and the chains of small levels from the coarsest: level l of graph
@@ -985,18 +1059,27 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
             }
         }
     }
-    // the remaining (larger) levels, graph by graph
+    MI_HIP(hipStreamSynchronize(c.stream));
+    // the remaining (larger) levels, graph by graph on the workers; the
+    // vectors of the main context are only read there (each worker
+    // allocates and frees in its own pool)
     std::vector<DBuf<double>> ev(K);
-    for (size_t k = 0; k < K; k++) {
+    run_on_workers(ws, K, [&](Ctx& w, size_t i) {
+        const size_t k = by_size[i];
         mi_stats* st = stats ? &stats[k] : nullptr;
         const int L = (int)H[k].size();
         if (lo[k] == L) {
-            ev[k] = fiedler_levels(c, H[k], L - 1, std::move(cur[k]), normLap, tol, st);
+            DBuf<double> x(&w.pool, H[k][L - 1].g.n);
+            MI_HIP(hipMemcpyAsync(x.get(), cur[k].get(), (size_t)H[k][L - 1].g.n * 8, hipMemcpyDeviceToDevice,
+                                  w.stream));
+            ev[k] = fiedler_levels(w, H[k], L - 1, std::move(x), normLap, tol, st);
         } else if (lo[k] == 0) {
-            ev[k] = std::move(out[k][0]);
+            ev[k] = DBuf<double>(&w.pool, H[k][0].g.n);
+            MI_HIP(hipMemcpyAsync(ev[k].get(), out[k][0].get(), (size_t)H[k][0].g.n * 8, hipMemcpyDeviceToDevice,
+                                  w.stream));
             if (st && !normLap) {
                 double elog[3];
-                eigen_log_level(c, H[k][0].g, ev[k].get(), elog);
+                eigen_log_level(w, H[k][0].g, ev[k].get(), elog);
                 st->has_eigen_log = 1;
                 st->eigenvalue = elog[0];
                 st->eigen_min = elog[1];
@@ -1004,21 +1087,21 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
             }
         } else {
             const int l = lo[k] - 1;
-            DBuf<double> fine(&c.pool, H[k][l].g.n);
-            prolong(c, H[k][l].vcmap.get(), H[k][l].g.n, out[k][l + 1].get(), fine.get());
-            ev[k] = fiedler_levels(c, H[k], l, std::move(fine), normLap, tol, st);
+            DBuf<double> fine(&w.pool, H[k][l].g.n);
+            prolong(w, H[k][l].vcmap.get(), H[k][l].g.n, out[k][l + 1].get(), fine.get());
+            ev[k] = fiedler_levels(w, H[k], l, std::move(fine), normLap, tol, st);
         }
-        out[k].clear();
-        cur[k] = DBuf<double>();
-    }
-    MI_HIP(hipStreamSynchronize(c.stream));
+    });
+    out.clear();
+    cur.clear();
     auto t2 = std::chrono::steady_clock::now();
     H.clear();
-    for (size_t k = 0; k < K; k++) {
-        edge_cuts[k] = compute_partition(c, *gs[k], ev[k].get(), config->num_partitions, perc_imbalance, d_parts[k]);
+    run_on_workers(ws, K, [&](Ctx& w, size_t i) {
+        const size_t k = by_size[i];
+        edge_cuts[k] = compute_partition(w, *gs[k], ev[k].get(), config->num_partitions, perc_imbalance, d_parts[k]);
         if (stats) stats[k].edge_cut = edge_cuts[k];
         ev[k] = DBuf<double>();
-    }
+    });
     MI_HIP(hipStreamSynchronize(c.stream));
     auto t3 = std::chrono::steady_clock::now();
     if (phase_ms) {
