@@ -974,11 +974,18 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
 {
     if (gs.empty()) return;
     if (c.fm) {
-        // FM refinement is one search per level and graph: graph by graph
+        // FM refinement: whole bisections, one graph per worker at a time
+        // (each search is one wavefront, so graphs run side by side; the
+        // FM path draws nothing from the pcg stream, so chaining it is
+        // order-free)
         auto t0 = std::chrono::steady_clock::now();
-        for (size_t k = 0; k < gs.size(); k++)
-            edge_cuts[k] = bisect(c, *gs[k], config, perc_imbalance, hec_seeds[k], chain ? &pcg_states[0] : &pcg_states[k],
-                                  pcg_inc, d_parts[k], stats ? &stats[k] : nullptr);
+        std::vector<Ctx*> fw = batch_workers(c, std::min<size_t>(gs.size(), BATCH_WORKERS));
+        MI_HIP(hipStreamSynchronize(c.stream));
+        run_on_workers(fw, gs.size(), [&](Ctx& w, size_t k) {
+            uint64_t pcg = chain ? pcg_states[0] : pcg_states[k];
+            edge_cuts[k] = bisect(w, *gs[k], config, perc_imbalance, hec_seeds[k], &pcg, pcg_inc, d_parts[k],
+                                  stats ? &stats[k] : nullptr);
+        });
         if (phase_ms) {
             phase_ms[0] = phase_ms[2] = 0.0;
             phase_ms[1] = ms_since(t0);

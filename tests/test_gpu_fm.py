@@ -195,3 +195,31 @@ def test_bisect_batch_fm_equals_single(ctx):
             assert res[k].edge_cut == o["edge_cut"] and np.array_equal(res[k].part, o["part"])
     finally:
         ctx.set_refinement(False)
+
+
+@pytest.mark.parametrize("name", ["grid96", "rgg"])
+def test_partition_4way_fm_vs_oracle(ctx, name):
+    """The 4-way driver (sgpar.cpp:53-89) in FM mode against the oracle's FM
+    bisections composed the same way (bisect, sgp_use_partition, bisect each
+    side with the same seed): bit-exact labels and cuts.  The two sides run
+    side by side on the worker contexts."""
+    import ipdps_2021_graph_coarsening_amd as mi
+    if name == "grid96":
+        rp, col = graphs.grid2d(96)[:2]
+    else:
+        rp, col, _ = graphs.rgg(12000, seed=3)
+    ctx.set_refinement(True)
+    try:
+        r4 = mi.partition_4way(ctx, ctx.graph(rp, col), hec_seed=6, pcg_state=2)
+    finally:
+        ctx.set_refinement(False)
+    o = oracle.partition_graph_fm(rp, col, None, 6)
+    (r0, c0), (r1, c1) = graphs.use_partition(rp, col, o["part"])
+    o0 = oracle.partition_graph_fm(r0, c0, None, 6)
+    o1 = oracle.partition_graph_fm(r1, c1, None, 6)
+    assert r4.extra["cuts"] == [o["edge_cut"], o0["edge_cut"], o1["edge_cut"]]
+    side = o["part"].astype(bool)
+    exp = np.empty(len(rp) - 1, np.uint32)
+    exp[~side] = o0["part"]
+    exp[side] = 2 + o1["part"]
+    assert np.array_equal(r4.part, exp)
