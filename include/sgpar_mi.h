@@ -146,6 +146,17 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries);
  * counts fm_refine calls.
  */
 int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
+/*
+ * Coarsening map of this context's hierarchies (mi_hierarchy_build, every
+ * bisection, mi_hec_level): variant 0 (default) the HEC map of the hec_*
+ * binaries (heuristics.h:633-693); variant 3 the map of the hec3_* binaries
+ * (compiled with -DHEC_V3, cmg/CMakeLists.txt:42,52): parallel_map_construct_v3
+ * (heuristics.h:766-826) on the same heavy neighbours -- mutual heavy pairs
+ * keyed by their smaller end, heavy targets keyed by themselves, every
+ * other vertex joining its heavy target's key, keys numbered in increasing
+ * order.  Other values: MI_STATUS_ARG.
+ */
+int mi_ctx_set_coarsening(mi_ctx* ctx, int variant);
 
 /* ---------------------------------------------------------------- graphs */
 /*

@@ -93,6 +93,7 @@ _SIGS = [
     ("mi_ctx_set_locality", C.c_int, [vp, C.c_int, C.c_uint32]),
     ("mi_ctx_set_bucketed_build", C.c_int, [vp, C.c_uint32]),
     ("mi_ctx_set_refinement", C.c_int, [vp, C.c_int]),
+    ("mi_ctx_set_coarsening", C.c_int, [vp, C.c_int]),
     ("mi_fm_eigensolve", C.c_int, [vp, vp, C.c_uint64, C.c_double, vp, C.POINTER(C.c_uint32), C.POINTER(Stats)]),
     ("mi_fm_refine", C.c_int, [vp, vp, vp, u32p, C.POINTER(C.c_uint32), vp]),
     ("mi_fm_init", C.c_int, [vp, vp, vp, C.c_uint64, C.c_double, u32p]),
@@ -227,6 +228,11 @@ class Context:
         """Bisections refine by FM, as the reference's *_fm binaries (True), or
         by the multilevel power iteration (False, the default)."""
         self.check(self._L.mi_ctx_set_refinement(self.h, 1 if fm else 0), "mi_ctx_set_refinement")
+
+    def set_coarsening(self, variant: int) -> None:
+        """The HEC map of the hierarchies: 0 the hec_* binaries' (default),
+        3 the hec3_* binaries' (parallel_map_construct_v3)."""
+        self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")
 
     def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:
         """Levels of <= 16384 vertices and <= max_entries entries iterate in one

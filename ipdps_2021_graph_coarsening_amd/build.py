@@ -69,9 +69,14 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
 DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
-# the same compiled with -DFM, as the reference's *_fm binaries are
+# the same compiled with -DFM, as the reference's *_fm binaries are, and with
+# -DHEC_V3 (the hec3_* binaries)
 DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
 DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
+DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
+DROPIN_HEC3_EXE = os.path.join(DROPIN, "sgpar_dropin_hec3")
+DROPIN_HEC3_FM_LIB = os.path.join(PKG, "libsgpar_dropin_hec3_fm.so")
+DROPIN_HEC3_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_hec3_fm")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -86,7 +91,9 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
     cxx = os.environ.get("CXX", "g++")
     common = ["-O2", "-std=c++17", "-DEXPERIMENT", "-I", ref_cmg, "-I", os.path.join(os.path.dirname(PKG), "include"),
               "-w"]
-    for lib, exe, extra in ((DROPIN_LIB, DROPIN_EXE, []), (DROPIN_FM_LIB, DROPIN_FM_EXE, ["-DFM"])):
+    for lib, exe, extra in ((DROPIN_LIB, DROPIN_EXE, []), (DROPIN_FM_LIB, DROPIN_FM_EXE, ["-DFM"]),
+                            (DROPIN_HEC3_LIB, DROPIN_HEC3_EXE, ["-DHEC_V3"]),
+                            (DROPIN_HEC3_FM_LIB, DROPIN_HEC3_FM_EXE, ["-DHEC_V3", "-DFM"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

@@ -478,6 +478,13 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
     return ST_OK;
 }
 
+int mi_ctx_set_coarsening(mi_ctx* ctx, int variant)
+{
+    if (!ctx || (variant != 0 && variant != 3)) return ST_ARG;
+    ctx->c.coarsen = variant;
+    return ST_OK;
+}
+
 int mi_ctx_set_refinement(mi_ctx* ctx, int fm)
 {
     if (!ctx || (fm != 0 && fm != 1)) return ST_ARG;
@@ -924,6 +931,7 @@ std::vector<Ctx*> batch_workers(Ctx& c, size_t want)
         w.locality = c.locality;
         w.locality_min_n = c.locality_min_n;
         w.fm = c.fm;
+        w.coarsen = c.coarsen;
         out.push_back(&w);
     }
     return out;

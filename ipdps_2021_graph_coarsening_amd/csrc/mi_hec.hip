@@ -390,6 +390,71 @@ const uint64_t* jump_table(Ctx& c)
 
 
 
+
+// ---- the v3 map (parallel_map_construct_v3, heuristics.h:766-826; the
+// hec3_* binaries).  Each pass reads only values the previous pass wrote
+// or values this pass cannot change, so it has one result however its
+// threads interleave, the one of the reference's passes run in order.
+
+// mutual heavy pairs (hn(hn(u)) = u, hn(u) = u included): key min(u, hn(u))
+__global__ void k_v3_mutual(uint32_t n, const uint32_t* __restrict__ hn, uint32_t* __restrict__ m)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t h = hn[u];
+    m[u] = hn[h] == u ? (h < u ? h : u) : INF;
+}
+
+// every heavy target outside a mutual pair keys itself (only INF -> v
+// stores, all of the same value)
+__global__ void k_v3_targets(uint32_t n, const uint32_t* __restrict__ hn, uint32_t* m)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t v = hn[u];
+    if (m[v] == INF) m[v] = v;
+}
+
+// the rest take their heavy target's key (a target's key is set already
+// and no store of this pass changes it)
+__global__ void k_v3_attach(uint32_t n, const uint32_t* __restrict__ hn, uint32_t* m)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    if (m[u] == INF) m[u] = m[hn[u]];
+}
+
+// the fixed point of each key chain (path halving, as :801-807); flag the
+// keys in use.  A chain longer than the graph is an error (counted).
+__global__ void k_v3_root(uint32_t n, const uint32_t* __restrict__ m, uint32_t* __restrict__ root,
+                          uint32_t* __restrict__ used, uint32_t* __restrict__ err)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint32_t p = m[u];
+    for (uint32_t k = 0; p < n && m[p] != p; k++) {
+        const uint32_t q = m[p];
+        if (k > n || q >= n) {
+            p = INF;
+            break;
+        }
+        p = m[q];
+    }
+    if (p >= n) {
+        atomicAdd(err, 1u);
+        p = 0;
+    }
+    root[u] = p;
+    used[p] = 1u;
+}
+
+__global__ void k_v3_assign(uint32_t n, const uint32_t* __restrict__ root, const uint32_t* __restrict__ id,
+                            uint32_t* __restrict__ vcmap)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u < n) vcmap[u] = id[root[u]];
+}
+
 }  // namespace
 
 uint64_t xs_pool_state(uint64_t seed)
@@ -465,6 +530,47 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     }
     draws.reset();
     if (t) MI_HIP(hipEventRecord(ev[2], c.stream));
+
+    uint32_t nc_v3 = 0;
+    if (c.coarsen == 3) {
+        // ---- the v3 map: keys by the four passes, numbered in order
+        DBuf<uint32_t> m(&c.pool, n), root(&c.pool, n), used(&c.pool, n), id(&c.pool, n), tw(&c.pool, 2);
+        fill_u32(c, used.get(), n, 0u);
+        fill_u32(c, tw.get(), 2, 0u);
+        k_v3_mutual<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), m.get());
+        MI_HIP(hipGetLastError());
+        k_v3_targets<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), m.get());
+        MI_HIP(hipGetLastError());
+        k_v3_attach<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), m.get());
+        MI_HIP(hipGetLastError());
+        k_v3_root<<<grid1(n), BLOCK, 0, c.stream>>>(n, m.get(), root.get(), used.get(), tw.get() + 1);
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, used.get(), id.get(), n, tw.get());
+        k_v3_assign<<<grid1(n), BLOCK, 0, c.stream>>>(n, root.get(), id.get(), vcmap);
+        MI_HIP(hipGetLastError());
+        uint32_t h[2] = {0, 0};
+        read_words(c, tw.get(), h, 8);
+        MI_CHECK(h[1] == 0, ST_INTERNAL, "v3 map: a key chain without a fixed point");
+        nc_v3 = h[0];
+    }
+    if (c.coarsen == 3) {
+        if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
+        if (vperm_out)
+            MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        if (t) {
+            float a = 0, b = 0, d = 0;
+            MI_HIP(hipEventSynchronize(ev[3]));
+            MI_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            MI_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+            MI_HIP(hipEventElapsedTime(&d, ev[2], ev[3]));
+            t->permute_ms += a;
+            t->heavy_ms += b;
+            t->map_ms += d;
+            for (auto& e : ev) MI_HIP(hipEventDestroy(e));
+        }
+        return nc_v3;
+    }
 
     // ---- map construction, pass 1 (closed form)
     DBuf<uint8_t> fl(&c.pool, n);

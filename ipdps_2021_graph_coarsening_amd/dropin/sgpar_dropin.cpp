@@ -34,6 +34,8 @@
 // - the graph upload and every device buffer belong to one library context
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
 //
+// Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
+// v3 map (mi_ctx_set_coarsening).
 // Compiled with -DFM (as the reference's *_fm binaries are,
 // cmg/CMakeLists.txt:50) the context refines by FM (mi_ctx_set_refinement)
 // and the metrics record what the reference's FM path records: no coarse
@@ -66,6 +68,12 @@ mi_ctx* context()
         if (mi_ctx_create(dev ? std::atoi(dev) : 0, &g_ctx) != 0) g_ctx = nullptr;
 #ifdef FM
         if (g_ctx && mi_ctx_set_refinement(g_ctx, 1) != 0) {
+            mi_ctx_destroy(g_ctx);
+            g_ctx = nullptr;
+        }
+#endif
+#ifdef HEC_V3
+        if (g_ctx && mi_ctx_set_coarsening(g_ctx, 3) != 0) {
             mi_ctx_destroy(g_ctx);
             g_ctx = nullptr;
         }

@@ -92,6 +92,16 @@ class _Core:
         L.orc_partition_graph.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.c_uint64, C.c_int, C.c_double,
                                           u32p, C.POINTER(C.c_uint32), u32p, u64p, f64p]
+        L.orc_map_construct_v3.restype = C.c_uint32
+        L.orc_map_construct_v3.argtypes = [C.c_uint32, u32p, u32p]
+        L.orc_coarsen_hec3.restype = C.c_uint32
+        L.orc_coarsen_hec3.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p, u32p]
+        L.orc_hierarchy_build_variant.restype = C.c_void_p
+        L.orc_hierarchy_build_variant.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.c_int]
+        L.orc_partition_graph_variant.restype = C.c_int
+        L.orc_partition_graph_variant.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64,
+                                                  C.POINTER(C.c_uint64), C.c_uint64, C.c_int, C.c_double,
+                                                  u32p, C.POINTER(C.c_uint32), u32p, u64p, f64p, C.c_int]
         u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
         i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.orc_fm_refine.restype = C.c_uint32
@@ -106,6 +116,9 @@ class _Core:
         L.orc_partition_graph_fm.restype = C.c_int
         L.orc_partition_graph_fm.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.c_double,
                                              u32p, C.POINTER(C.c_uint32), u8p, u32p]
+        L.orc_partition_graph_fm_variant.restype = C.c_int
+        L.orc_partition_graph_fm_variant.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.c_double,
+                                                     u32p, C.POINTER(C.c_uint32), u8p, u32p, C.c_int]
 
 
 _core = None
@@ -222,12 +235,12 @@ def build_coarse(rowptr, col, w, vcmap, nc: int, vtx_w=None):
 class Hierarchy:
     """Multilevel hierarchy from the oracle (level 0 = finest)."""
 
-    def __init__(self, rowptr, col, w, seed: int):
+    def __init__(self, rowptr, col, w, seed: int, variant: int = 0):
         L = core().L
         self._L = L
         n = len(rowptr) - 1
         w, wp = _wptr(w)
-        self._h = L.orc_hierarchy_build(n, rowptr, col, wp, seed)
+        self._h = L.orc_hierarchy_build_variant(n, rowptr, col, wp, seed, variant)
         self.nlevels = L.orc_hier_nlevels(self._h)
         self.levels = []
         for l in range(self.nlevels):
@@ -321,9 +334,31 @@ def compute_partition(rowptr, col, evec, num_partitions: int = 2, imbalance: int
     return part, ec.value
 
 
+def map_construct_v3(hn):
+    """The HEC v3 map (heuristics.h:766-826) of heavy neighbours hn: (vcmap, nc)."""
+    L = core().L
+    hn = np.ascontiguousarray(hn, dtype=np.uint32)
+    vc = np.empty(len(hn) + 1, dtype=np.uint32)
+    nc = L.orc_map_construct_v3(len(hn), hn, vc)
+    return vc[:len(hn)].copy(), int(nc)
+
+
+def coarsen_hec3(level: int, rowptr, col, w, seed: int):
+    """(vcmap, nc, hn) of one HEC level with the v3 map."""
+    L = core().L
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    hn = np.empty(n + 1, dtype=np.uint32)
+    nc = L.orc_coarsen_hec3(level, n, np.ascontiguousarray(rowptr, dtype=np.uint32),
+                            np.ascontiguousarray(col, dtype=np.uint32), wp, seed, vc, hn)
+    return vc[:n].copy(), int(nc), hn[:n].copy()
+
+
 def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
-                    normLap: int = 0, tol: float = 1e-10):
-    """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state)."""
+                    normLap: int = 0, tol: float = 1e-10, variant: int = 0):
+    """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
+    variant: 0 HEC, 3 HEC with the v3 map (the hec3_* binaries)."""
     L = core().L
     n = len(rowptr) - 1
     w, wp = _wptr(w)
@@ -333,8 +368,8 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
     lv = np.zeros(210, dtype=np.uint32)
     it = np.zeros(210, dtype=np.uint64)
     ev = np.empty(n, dtype=np.float64)
-    nl = L.orc_partition_graph(n, rowptr, col, wp, hec_seed, C.byref(st), pcg_inc, normLap, tol,
-                               part, C.byref(ec), lv, it, ev)
+    nl = L.orc_partition_graph_variant(n, rowptr, col, wp, hec_seed, C.byref(st), pcg_inc, normLap, tol,
+                                       part, C.byref(ec), lv, it, ev, variant)
     return dict(part=part, edge_cut=ec.value, levels_n=lv[:nl].copy(), iters=it[:nl].copy(),
                 evec=ev, pcg_state=st.value)
 
@@ -383,7 +418,7 @@ def init_spectral(rowptr, col, w, vtx_w, seed: int, tol: float = 1e-10):
     return p[:n].copy(), x[:n].copy()
 
 
-def partition_graph_fm(rowptr, col, w, hec_seed: int, tol: float = 1e-10):
+def partition_graph_fm(rowptr, col, w, hec_seed: int, tol: float = 1e-10, variant: int = 0):
     """Whole bisection of the FM binary: dict(part, edge_cut, fm_part, passes, nlevels)."""
     L = core().L
     rowptr, col = _csr(rowptr, col)
@@ -393,7 +428,7 @@ def partition_graph_fm(rowptr, col, w, hec_seed: int, tol: float = 1e-10):
     fmp = np.zeros(n + 1, dtype=np.uint8)
     ps = np.zeros(210, dtype=np.uint32)
     ec = C.c_uint32(0)
-    nl = L.orc_partition_graph_fm(n, rowptr, col, wp, hec_seed, tol, part, C.byref(ec), fmp, ps)
+    nl = L.orc_partition_graph_fm_variant(n, rowptr, col, wp, hec_seed, tol, part, C.byref(ec), fmp, ps, variant)
     return dict(part=part[:n].copy(), edge_cut=ec.value, fm_part=fmp[:n].copy(), passes=ps[:nl].copy(),
                 nlevels=nl)
 

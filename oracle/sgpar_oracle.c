@@ -289,6 +289,68 @@ vid_t orc_coarsen_hec(int level, vid_t n, const eid_t *rowptr, const vid_t *col,
 }
 
 /* ------------------------------------------------------------------ */
+/* parallel_map_construct_v3 (the hec3_* binaries, -DHEC_V3,          */
+/* heuristics.h:766-826, selected at :1003-1004) on a one-thread       */
+/* backend, as written: mutual heavy pairs keyed by their smaller end, */
+/* every heavy target its own key, every other vertex its heavy        */
+/* target's key, path halving to the fixed point, keys numbered in     */
+/* increasing order; nc = the largest id + 1.                          */
+/* ------------------------------------------------------------------ */
+vid_t orc_map_construct_v3(vid_t n, const vid_t *hn, vid_t *vcmap)
+{
+    vid_t *m = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *dense = (vid_t *)calloc((size_t)n + 1, sizeof(vid_t));
+    for (vid_t u = 0; u < n; u++) {
+        m[u] = ORC_INF;
+        if (hn[hn[u]] == u) {
+            m[u] = u;
+            if (hn[u] < u) m[u] = hn[u];
+        }
+    }
+    for (vid_t u = 0; u < n; u++) {
+        const vid_t v = hn[u];
+        if (m[v] == ORC_INF) m[v] = v;
+    }
+    for (vid_t u = 0; u < n; u++)
+        if (m[u] == ORC_INF) m[u] = m[hn[u]];
+    for (vid_t u = 0; u < n; u++) {
+        vid_t p = m[u];
+        while (m[p] != p) p = m[m[p]];
+        m[u] = p;
+    }
+    for (vid_t u = 0; u < n; u++) dense[m[u]]++;
+    vid_t upd = 0;
+    for (vid_t u = 0; u < n; u++)
+        if (dense[u] > 0) dense[u] = upd++;
+    vid_t nc = 0;
+    for (vid_t u = 0; u < n; u++) {
+        vcmap[u] = dense[m[u]];
+        if (nc <= vcmap[u]) nc = vcmap[u];
+    }
+    free(m);
+    free(dense);
+    return nc + 1;
+}
+
+/* One HEC level with the v3 map (sgp_coarsen_HEC with -DHEC_V3,
+   heuristics.h:930-1031): the same pool, permutation draws and heavy
+   neighbours as orc_coarsen_hec, then orc_map_construct_v3. */
+vid_t orc_coarsen_hec3(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                       uint64_t seed, vid_t *vcmap, vid_t *hn_out)
+{
+    uint64_t st = orc_pool_state(seed);
+    vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    orc_generate_permutation(n, &st, vperm, NULL);
+    orc_heavy_neighbor(level, n, rowptr, col, w, &st, hn);
+    vid_t nc = orc_map_construct_v3(n, hn, vcmap);
+    if (hn_out) memcpy(hn_out, hn, (size_t)n * sizeof(vid_t));
+    free(vperm);
+    free(hn);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Coarse graph, coarsen_kokkos.h:1061-1127 (map/count) with the       */
 /* non-skew builder :799-900 and the RADIX dedupe :750-762, :445-510:  */
 /* rows sorted by column, equal columns merged with summed weights,    */
@@ -387,8 +449,18 @@ typedef struct {
     orc_level_t lv[ORC_MAX_GRAPHS + 2];
 } orc_hier_t;
 
+orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
+                                        const wgt_t *w, uint64_t seed, int variant);
+
 orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
                                 const wgt_t *w, uint64_t seed)
+{
+    return orc_hierarchy_build_variant(n, rowptr, col, w, seed, 0);
+}
+
+/* The level loop with HEC (variant 0) or HEC with the v3 map (3). */
+orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
+                                        const wgt_t *w, uint64_t seed, int variant)
 {
     orc_hier_t *h = (orc_hier_t *)calloc(1, sizeof(orc_hier_t));
     orc_level_t *f = &h->lv[0];
@@ -409,8 +481,10 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
         orc_level_t *c = &h->lv[h->nlevels];
         level++;
         a->vcmap = (vid_t *)malloc((size_t)a->n * sizeof(vid_t) + 4);
-        vid_t nc = orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
-                                   NULL, NULL);
+        vid_t nc = variant == 3
+                       ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
+                       : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
+                                         NULL, NULL);
         c->n = nc;
         c->vtx_w = (wgt_t *)calloc((size_t)nc + 1, sizeof(wgt_t));
         c->m = orc_build_coarse(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
@@ -712,12 +786,27 @@ int orc_compute_partition(vid_t n, const eid_t *rowptr, const vid_t *col, const 
 /* levels_n[l], iters[l] (room for 202) receive per-level sizes and    */
 /* iteration counts (0 = finest); returns the number of levels.        */
 /* ------------------------------------------------------------------ */
+int orc_partition_graph_variant(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                                uint64_t hec_seed, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                                double tol, vid_t *part, eid_t *edgecut, vid_t *levels_n,
+                                uint64_t *iters, double *evec_out, int variant);
+
 int orc_partition_graph(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
                         uint64_t hec_seed, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
                         double tol, vid_t *part, eid_t *edgecut, vid_t *levels_n,
                         uint64_t *iters, double *evec_out)
 {
-    orc_hier_t *h = orc_hierarchy_build(n, rowptr, col, w, hec_seed);
+    return orc_partition_graph_variant(n, rowptr, col, w, hec_seed, pcg_state, pcg_inc, normLap, tol,
+                                       part, edgecut, levels_n, iters, evec_out, 0);
+}
+
+/* orc_partition_graph with the coarsening variant of orc_hierarchy_build_variant. */
+int orc_partition_graph_variant(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                                uint64_t hec_seed, uint64_t *pcg_state, uint64_t pcg_inc, int normLap,
+                                double tol, vid_t *part, eid_t *edgecut, vid_t *levels_n,
+                                uint64_t *iters, double *evec_out, int variant)
+{
+    orc_hier_t *h = orc_hierarchy_build_variant(n, rowptr, col, w, hec_seed, variant);
     int L = h->nlevels;
     double *ev = (double *)malloc((size_t)n * sizeof(double) + 8);
     orc_eigensolve(h, pcg_state, pcg_inc, normLap, tol, 0, ev, iters);
@@ -1092,11 +1181,23 @@ eid_t orc_fm_eigensolve(const orc_hier_t *h, uint64_t seed, double tol, uint8_t 
  * the coarsening and init_spectral (the reference seeds both from
  * std::time).  Returns the number of levels.
  */
+int orc_partition_graph_fm_variant(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                                   uint64_t hec_seed, double tol, vid_t *part, eid_t *edgecut,
+                                   uint8_t *fm_part, uint32_t *passes, int variant);
+
 int orc_partition_graph_fm(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
                            uint64_t hec_seed, double tol, vid_t *part, eid_t *edgecut,
                            uint8_t *fm_part, uint32_t *passes)
 {
-    orc_hier_t *h = orc_hierarchy_build(n, rowptr, col, w, hec_seed);
+    return orc_partition_graph_fm_variant(n, rowptr, col, w, hec_seed, tol, part, edgecut, fm_part,
+                                          passes, 0);
+}
+
+int orc_partition_graph_fm_variant(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                                   uint64_t hec_seed, double tol, vid_t *part, eid_t *edgecut,
+                                   uint8_t *fm_part, uint32_t *passes, int variant)
+{
+    orc_hier_t *h = orc_hierarchy_build_variant(n, rowptr, col, w, hec_seed, variant);
     const int L = h->nlevels;
     uint8_t *p0 = (uint8_t *)malloc((size_t)n + 1);
     orc_fm_eigensolve(h, hec_seed, tol, p0, passes);
