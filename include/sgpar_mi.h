@@ -154,7 +154,13 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * (heuristics.h:766-826) on the same heavy neighbours -- mutual heavy pairs
  * keyed by their smaller end, heavy targets keyed by themselves, every
  * other vertex joining its heavy target's key, keys numbered in increasing
- * order.  Other values: MI_STATUS_ARG.
+ * order; variant 5 the MIS-2 coarsening of the mis_* binaries (-DMIS,
+ * cmg/CMakeLists.txt:47,55): mis_2 (heuristics.h:8-100), a maximal
+ * distance-2 independent set from the XorShift64 pool of the level's seed,
+ * one aggregate per set vertex in index order with its neighbours, the
+ * rest joined by sgp_coarsen_mis_2's index-order sweep (:218-289); with
+ * variant 5 mi_hec_level leaves vperm and hn untouched.  Other values:
+ * MI_STATUS_ARG.
  */
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant);
 

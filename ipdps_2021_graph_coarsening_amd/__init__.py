@@ -230,8 +230,9 @@ class Context:
         self.check(self._L.mi_ctx_set_refinement(self.h, 1 if fm else 0), "mi_ctx_set_refinement")
 
     def set_coarsening(self, variant: int) -> None:
-        """The HEC map of the hierarchies: 0 the hec_* binaries' (default),
-        3 the hec3_* binaries' (parallel_map_construct_v3)."""
+        """The coarsening of the hierarchies: 0 the hec_* binaries' (default),
+        3 the hec3_* binaries' (parallel_map_construct_v3), 5 the mis_*
+        binaries' (MIS-2 aggregates)."""
         self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")
 
     def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:

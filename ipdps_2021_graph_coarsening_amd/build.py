@@ -77,6 +77,8 @@ DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
 DROPIN_HEC3_EXE = os.path.join(DROPIN, "sgpar_dropin_hec3")
 DROPIN_HEC3_FM_LIB = os.path.join(PKG, "libsgpar_dropin_hec3_fm.so")
 DROPIN_HEC3_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_hec3_fm")
+DROPIN_MIS_LIB = os.path.join(PKG, "libsgpar_dropin_mis.so")
+DROPIN_MIS_EXE = os.path.join(DROPIN, "sgpar_dropin_mis")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -93,7 +95,8 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
               "-w"]
     for lib, exe, extra in ((DROPIN_LIB, DROPIN_EXE, []), (DROPIN_FM_LIB, DROPIN_FM_EXE, ["-DFM"]),
                             (DROPIN_HEC3_LIB, DROPIN_HEC3_EXE, ["-DHEC_V3"]),
-                            (DROPIN_HEC3_FM_LIB, DROPIN_HEC3_FM_EXE, ["-DHEC_V3", "-DFM"])):
+                            (DROPIN_HEC3_FM_LIB, DROPIN_HEC3_FM_EXE, ["-DHEC_V3", "-DFM"]),
+                            (DROPIN_MIS_LIB, DROPIN_MIS_EXE, ["-DMIS"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

@@ -19,6 +19,8 @@
 // evaluated here with data-parallel kernels (see hec_map below).
 #include "mi_kernels.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <vector>
 
@@ -455,6 +457,173 @@ __global__ void k_v3_assign(uint32_t n, const uint32_t* __restrict__ root, const
     if (u < n) vcmap[u] = id[root[u]];
 }
 
+
+
+__global__ void k_mis_flag(uint32_t n, const int* __restrict__ state, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) flag[i] = state[i] == 1 ? 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const uint32_t t2 = __shfl_xor(v, o, WAVE);
+        v = t2 < v ? t2 : v;
+    }
+    return v;
+}
+
+// ---- MIS-2 coarsening (mis_2, heuristics.h:8-100, and the aggregation of
+// sgp_coarsen_mis_2, :218-289; the mis_* binaries).  A round gives every
+// vertex the largest (state, random, index) of its two-hop ball by two
+// passes of neighbour maxima over double-buffered tuples, then settles the
+// states from those tuples alone; every pass has one result whatever the
+// order of its threads.
+
+__device__ __forceinline__ bool mis_gt(int as, uint64_t ar, uint32_t ai, int bs, uint64_t br, uint32_t bi)
+{
+    return as > bs || (as == bs && (ar > br || (ar == br && ai > bi)));
+}
+
+__global__ void k_mis_init(uint32_t n, const int* __restrict__ state, const uint64_t* __restrict__ rnd,
+                           int* __restrict__ ts, uint64_t* __restrict__ tr, uint32_t* __restrict__ ti)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    ts[i] = state[i];
+    tr[i] = rnd[i];
+    ti[i] = i;
+}
+
+// one pass of neighbour maxima: a wavefront per row
+__global__ void __launch_bounds__(BLOCK) k_mis_max(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                   const uint32_t* __restrict__ col, const int* __restrict__ ts,
+                                                   const uint64_t* __restrict__ tr, const uint32_t* __restrict__ ti,
+                                                   int* __restrict__ os, uint64_t* __restrict__ orr,
+                                                   uint32_t* __restrict__ oi)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t r = wid; r < n; r += nw) {
+        int ms = ts[r];
+        uint64_t mr = tr[r];
+        uint32_t mi = ti[r];
+        for (uint32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += WAVE) {
+            const uint32_t v = col[e];
+            if (mis_gt(ts[v], tr[v], ti[v], ms, mr, mi)) {
+                ms = ts[v];
+                mr = tr[v];
+                mi = ti[v];
+            }
+        }
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const int s2 = __shfl_xor(ms, o, WAVE);
+            const uint64_t r2 = __shfl_xor(mr, o, WAVE);
+            const uint32_t i2 = __shfl_xor(mi, o, WAVE);
+            if (mis_gt(s2, r2, i2, ms, mr, mi)) {
+                ms = s2;
+                mr = r2;
+                mi = i2;
+            }
+        }
+        if (lane == 0) {
+            os[r] = ms;
+            orr[r] = mr;
+            oi[r] = mi;
+        }
+    }
+}
+
+// :81-96 -- the states from the two-hop maxima; *left counts the vertices
+// still undecided
+__global__ void k_mis_settle(uint32_t n, int* __restrict__ state, const int* __restrict__ ts,
+                             const uint32_t* __restrict__ ti, uint32_t* __restrict__ left)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int s = state[i];
+    if (s == 0) {
+        if (ti[i] == i)
+            s = 1;
+        else if (ts[i] == 1 || ti[ti[i]] == ti[i])
+            s = -1;
+        state[i] = s;
+    }
+    if (s == 0) atomicAdd(left, 1u);
+}
+
+// the set's aggregates (ids in index order) and their direct neighbours;
+// reached[v] = 0 for those (reached before the sweep), else INF
+__global__ void k_mis_own(uint32_t n, const int* __restrict__ state, const uint32_t* __restrict__ id,
+                          const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                          uint32_t* __restrict__ vcmap, uint32_t* __restrict__ reached)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || state[i] != 1) return;
+    vcmap[i] = id[i];
+    reached[i] = 0;
+    for (uint32_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+        vcmap[col[e]] = id[i];
+        reached[col[e]] = 0;
+    }
+}
+
+// The sweep of :263-275 visits vertices in index order and a visited vertex
+// that is mapped maps its unmapped neighbours.  With reached[v] = 1 + the
+// step at which v is mapped (0: before the sweep), v is mapped at the
+// smallest index i of a neighbour with reached[i] <= i (mapped before its
+// own visit).  A relaxation pass lowers reached[] towards that fixed point
+// from above (every value it writes is a valid upper bound), so repeated
+// passes end at the sweep's steps; *changed counts the lowered values.
+__global__ void __launch_bounds__(BLOCK) k_mis_relax(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col, uint32_t* reached,
+                                                     uint32_t* __restrict__ changed)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t v = wid; v < n; v += nw) {
+        const uint32_t cur = reached[v];
+        if (cur == 0) continue;
+        uint32_t best = cur;
+        for (uint32_t e = rowptr[v] + lane; e < rowptr[v + 1]; e += WAVE) {
+            const uint32_t i = col[e];
+            if (reached[i] <= i && i + 1 < best) best = i + 1;
+        }
+        best = wave_min_u32(best);
+        if (lane == 0 && best < cur) {
+            reached[v] = best;
+            atomicAdd(changed, 1u);
+        }
+    }
+}
+
+// each swept vertex takes the aggregate of the vertex that reached it,
+// following reachers (their steps strictly decrease) to one mapped before
+// the sweep
+__global__ void k_mis_follow(uint32_t n, const uint32_t* __restrict__ reached, uint32_t* vcmap,
+                             uint32_t* __restrict__ err)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n) return;
+    uint32_t r = reached[v];
+    if (r == 0) return;
+    uint32_t p = v;
+    for (uint32_t k = 0; r != 0; k++) {
+        if (r == INF || k > n) {
+            atomicAdd(err, 1u);
+            return;
+        }
+        p = r - 1;
+        r = reached[p];
+    }
+    vcmap[v] = vcmap[p];
+}
+
 }  // namespace
 
 uint64_t xs_pool_state(uint64_t seed)
@@ -482,11 +651,74 @@ void xs_draws(Ctx& c, uint64_t state0, uint64_t first, uint64_t count, uint64_t*
     MI_HIP(hipGetLastError());
 }
 
+// One MIS-2 level (the mis_* binaries): the set, then the aggregates.
+static uint32_t mis2_map(Ctx& c, const DGraph& g, uint64_t seed, uint32_t* vcmap, HecTimes* t)
+{
+    const uint32_t n = g.n;
+    auto t0 = std::chrono::steady_clock::now();
+    DBuf<uint64_t> rnd(&c.pool, n), tr(&c.pool, n), tr2(&c.pool, n);
+    DBuf<int> state(&c.pool, n), ts(&c.pool, n), ts2(&c.pool, n);
+    DBuf<uint32_t> ti(&c.pool, n), ti2(&c.pool, n), cnt(&c.pool, 2);
+    xs_draws(c, xs_pool_state(seed), 0, n, rnd.get());
+    fill_u32(c, reinterpret_cast<uint32_t*>(state.get()), n, 0u);
+    const unsigned wg = (unsigned)std::min<uint64_t>(ceil_div(n, BLOCK / WAVE), 65536);
+    for (uint32_t round = 0;; round++) {
+        MI_CHECK(round <= n + 2, ST_INTERNAL, "MIS-2 rounds did not end");
+        k_mis_init<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), rnd.get(), ts.get(), tr.get(), ti.get());
+        MI_HIP(hipGetLastError());
+        for (int k = 0; k < 2; k++) {
+            k_mis_max<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), ts.get(), tr.get(), ti.get(), ts2.get(),
+                                                  tr2.get(), ti2.get());
+            MI_HIP(hipGetLastError());
+            ts.swap(ts2);
+            tr.swap(tr2);
+            ti.swap(ti2);
+        }
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_mis_settle<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), ts.get(), ti.get(), cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t left = 0;
+        read_words(c, cnt.get(), &left, 4);
+        if (left == 0) break;
+    }
+    // aggregates: the set in index order, its neighbours, then the sweep
+    DBuf<uint32_t> flag(&c.pool, n), id(&c.pool, n), reached(&c.pool, n), tw(&c.pool, 2);
+    k_mis_flag<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), flag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, flag.get(), id.get(), n, tw.get());
+    fill_u32(c, vcmap, n, INF);
+    fill_u32(c, reached.get(), n, INF);
+    k_mis_own<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), id.get(), g.R(), g.Cc(), vcmap, reached.get());
+    MI_HIP(hipGetLastError());
+    for (uint32_t pass = 0;; pass++) {
+        MI_CHECK(pass <= n + 2, ST_INTERNAL, "MIS-2 sweep did not settle");
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_mis_relax<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), reached.get(), cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t ch = 0;
+        read_words(c, cnt.get(), &ch, 4);
+        if (ch == 0) break;
+    }
+    fill_u32(c, cnt.get() + 1, 1, 0u);
+    k_mis_follow<<<grid1(n), BLOCK, 0, c.stream>>>(n, reached.get(), vcmap, cnt.get() + 1);
+    MI_HIP(hipGetLastError());
+    uint32_t h[2] = {0, 0};
+    read_words(c, tw.get(), &h[0], 4);
+    read_words(c, cnt.get() + 1, &h[1], 4);
+    MI_CHECK(h[1] == 0, ST_INTERNAL, "MIS-2: a vertex left without an aggregate");
+    if (t) t->map_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return h[0];
+}
+
 uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed, uint32_t* vcmap,
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
+    if (c.coarsen == 5) {
+        MI_CHECK(!vperm_out && !hn_out, ST_ARG, "MIS-2 coarsening has no permutation or heavy neighbours");
+        return mis2_map(c, g, seed, vcmap, t);
+    }
     hipEvent_t ev[4];
     if (t)
         for (auto& e : ev) MI_HIP(hipEventCreate(&e));

@@ -35,7 +35,7 @@
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
-// v3 map (mi_ctx_set_coarsening).
+// v3 map, with -DMIS (the mis_* binaries) with MIS-2 (mi_ctx_set_coarsening).
 // Compiled with -DFM (as the reference's *_fm binaries are,
 // cmg/CMakeLists.txt:50) the context refines by FM (mi_ctx_set_refinement)
 // and the metrics record what the reference's FM path records: no coarse
@@ -72,8 +72,13 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#ifdef HEC_V3
-        if (g_ctx && mi_ctx_set_coarsening(g_ctx, 3) != 0) {
+#if defined HEC_V3 || defined MIS
+#ifdef MIS
+        const int variant = 5;
+#else
+        const int variant = 3;
+#endif
+        if (g_ctx && mi_ctx_set_coarsening(g_ctx, variant) != 0) {
             mi_ctx_destroy(g_ctx);
             g_ctx = nullptr;
         }

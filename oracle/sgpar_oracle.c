@@ -351,6 +351,92 @@ vid_t orc_coarsen_hec3(int level, vid_t n, const eid_t *rowptr, const vid_t *col
 }
 
 /* ------------------------------------------------------------------ */
+/* MIS-2 coarsening (the mis_* binaries, -DMIS): mis_2, heuristics.h:8-100, */
+/* and the aggregation of sgp_coarsen_mis_2, :218-289, on a one-thread  */
+/* backend.  state: 1 in the distance-2 independent set, -1 excluded.   */
+/* ------------------------------------------------------------------ */
+typedef struct { int s; uint64_t r; vid_t i; } orc_tup_t;
+
+static int tup_gt(orc_tup_t a, orc_tup_t b)  /* a beats b (:38-51) */
+{
+    if (a.s > b.s) return 1;
+    if (a.s == b.s) {
+        if (a.r > b.r) return 1;
+        if (a.r == b.r && a.i > b.i) return 1;
+    }
+    return 0;
+}
+
+/* states (n ints) of mis_2 with the pool of `seed` (std::time in the
+   reference); returns the number of rounds. */
+int orc_mis2(vid_t n, const eid_t *rowptr, const vid_t *col, uint64_t seed, int *state)
+{
+    uint64_t st = orc_pool_state(seed);
+    uint64_t *rnd = (uint64_t *)malloc((size_t)n * sizeof(uint64_t) + 8);
+    orc_tup_t *t = (orc_tup_t *)malloc((size_t)n * sizeof(orc_tup_t) + 16);
+    orc_tup_t *u = (orc_tup_t *)malloc((size_t)n * sizeof(orc_tup_t) + 16);
+    for (vid_t i = 0; i < n; i++) {
+        rnd[i] = orc_xs_urand64(&st);
+        state[i] = 0;
+    }
+    vid_t unassigned = n;
+    int rounds = 0;
+    while (unassigned > 0) {
+        rounds++;
+        for (vid_t i = 0; i < n; i++) {
+            t[i].s = state[i];
+            t[i].r = rnd[i];
+            t[i].i = i;
+        }
+        for (int k = 0; k < 2; k++) {
+            for (vid_t i = 0; i < n; i++) {
+                orc_tup_t m = t[i];
+                for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+                    if (tup_gt(t[col[e]], m)) m = t[col[e]];
+                u[i] = m;
+            }
+            memcpy(t, u, (size_t)n * sizeof(orc_tup_t));
+        }
+        unassigned = 0;
+        for (vid_t i = 0; i < n; i++) {
+            if (state[i] == 0) {
+                if (t[i].i == i)
+                    state[i] = 1;
+                else if (t[i].s == 1 || t[t[i].i].i == t[i].i)
+                    state[i] = -1;
+            }
+            if (state[i] == 0) unassigned++;
+        }
+    }
+    free(rnd);
+    free(t);
+    free(u);
+    return rounds;
+}
+
+/* One MIS-2 level: the set, one aggregate per set vertex (in index order),
+   its neighbours, then every vertex still unmapped takes the aggregate of
+   the first mapped neighbour that reaches it in a sweep over vertices in
+   index order (:263-275).  Returns nc. */
+vid_t orc_coarsen_mis2(vid_t n, const eid_t *rowptr, const vid_t *col, uint64_t seed, vid_t *vcmap)
+{
+    int *state = (int *)malloc((size_t)n * sizeof(int) + 4);
+    orc_mis2(n, rowptr, col, seed, state);
+    vid_t nc = 0;
+    for (vid_t i = 0; i < n; i++) vcmap[i] = state[i] == 1 ? nc++ : ORC_INF;
+    for (vid_t i = 0; i < n; i++)
+        if (state[i] == 1)
+            for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) vcmap[col[e]] = vcmap[i];
+    for (vid_t i = 0; i < n; i++) {
+        if (vcmap[i] == ORC_INF) continue;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+            if (vcmap[col[e]] == ORC_INF) vcmap[col[e]] = vcmap[i];
+    }
+    free(state);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Coarse graph, coarsen_kokkos.h:1061-1127 (map/count) with the       */
 /* non-skew builder :799-900 and the RADIX dedupe :750-762, :445-510:  */
 /* rows sorted by column, equal columns merged with summed weights,    */
@@ -458,7 +544,7 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
     return orc_hierarchy_build_variant(n, rowptr, col, w, seed, 0);
 }
 
-/* The level loop with HEC (variant 0) or HEC with the v3 map (3). */
+/* The level loop with HEC (variant 0), HEC with the v3 map (3) or MIS-2 (5). */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
 {
@@ -481,10 +567,10 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
         orc_level_t *c = &h->lv[h->nlevels];
         level++;
         a->vcmap = (vid_t *)malloc((size_t)a->n * sizeof(vid_t) + 4);
-        vid_t nc = variant == 3
-                       ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
-                       : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
-                                         NULL, NULL);
+        vid_t nc = variant == 3   ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
+                   : variant == 5 ? orc_coarsen_mis2(a->n, a->rowptr, a->col, seed, a->vcmap)
+                                  : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
+                                                    NULL, NULL);
         c->n = nc;
         c->vtx_w = (wgt_t *)calloc((size_t)nc + 1, sizeof(wgt_t));
         c->m = orc_build_coarse(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
