@@ -158,9 +158,15 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * cmg/CMakeLists.txt:47,55): mis_2 (heuristics.h:8-100), a maximal
  * distance-2 independent set from the XorShift64 pool of the level's seed,
  * one aggregate per set vertex in index order with its neighbours, the
- * rest joined by sgp_coarsen_mis_2's index-order sweep (:218-289); with
- * variant 5 mi_hec_level leaves vperm and hn untouched.  Other values:
- * MI_STATUS_ARG.
+ * rest joined by sgp_coarsen_mis_2's index-order sweep (:218-289); variant
+ * 6 the GOSH coarsening of the gosh_* binaries (-DGOSH,
+ * cmg/CMakeLists.txt:48,56): GOSH_clusters (heuristics.h:102-217), centres
+ * by rounds of (state, entries, index) neighbour maxima that skip edges
+ * whose ends both have more than nnz/n entries, one aggregate per centre in
+ * index order, every other vertex joining its centre neighbour with the
+ * most entries, the first of equals in its row (sgp_coarsen_GOSH,
+ * :312-377).  With variants 5 and 6 mi_hec_level leaves vperm and hn
+ * untouched.  Other values: MI_STATUS_ARG.
  */
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant);
 

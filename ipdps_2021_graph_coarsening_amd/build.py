@@ -70,7 +70,7 @@ DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
 # the same compiled with -DFM, as the reference's *_fm binaries are, and with
-# -DHEC_V3 (the hec3_* binaries)
+# -DHEC_V3 (the hec3_* binaries); -DMIS and -DGOSH likewise
 DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
 DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
 DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
@@ -79,6 +79,8 @@ DROPIN_HEC3_FM_LIB = os.path.join(PKG, "libsgpar_dropin_hec3_fm.so")
 DROPIN_HEC3_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_hec3_fm")
 DROPIN_MIS_LIB = os.path.join(PKG, "libsgpar_dropin_mis.so")
 DROPIN_MIS_EXE = os.path.join(DROPIN, "sgpar_dropin_mis")
+DROPIN_GOSH_LIB = os.path.join(PKG, "libsgpar_dropin_gosh.so")
+DROPIN_GOSH_EXE = os.path.join(DROPIN, "sgpar_dropin_gosh")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -96,7 +98,8 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
     for lib, exe, extra in ((DROPIN_LIB, DROPIN_EXE, []), (DROPIN_FM_LIB, DROPIN_FM_EXE, ["-DFM"]),
                             (DROPIN_HEC3_LIB, DROPIN_HEC3_EXE, ["-DHEC_V3"]),
                             (DROPIN_HEC3_FM_LIB, DROPIN_HEC3_FM_EXE, ["-DHEC_V3", "-DFM"]),
-                            (DROPIN_MIS_LIB, DROPIN_MIS_EXE, ["-DMIS"])):
+                            (DROPIN_MIS_LIB, DROPIN_MIS_EXE, ["-DMIS"]),
+                            (DROPIN_GOSH_LIB, DROPIN_GOSH_EXE, ["-DGOSH"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

@@ -624,6 +624,110 @@ __global__ void k_mis_follow(uint32_t n, const uint32_t* __restrict__ reached, u
     vcmap[v] = vcmap[p];
 }
 
+// ---- GOSH coarsening (GOSH_clusters, heuristics.h:102-217, and the
+// aggregation of sgp_coarsen_GOSH, :312-377; the gosh_* binaries).  A round
+// gives every undecided vertex the largest (state, entries, index) of itself
+// and its neighbours through the edges that count (not both ends above
+// nnz/n entries, :157-160).  An excluded neighbour (state -1) never beats an
+// undecided vertex and any centre neighbour decides it, so best[r] keeps
+// INF for "a centre is a neighbour", else the index of the largest
+// (entries, index) among r and its undecided neighbours.
+__global__ void __launch_bounds__(BLOCK) k_gosh_max(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col, uint32_t thr,
+                                                    const int* __restrict__ state, uint32_t* __restrict__ best)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t r = wid; r < n; r += nw) {
+        if (state[r] != 0) continue;
+        const uint32_t r0 = rowptr[r], r1 = rowptr[r + 1];
+        const bool heavy = r1 - r0 > thr;
+        uint64_t key = ((uint64_t)(r1 - r0) << 32) | r;
+        bool centre = false;
+        for (uint32_t e = r0 + lane; e < r1; e += WAVE) {
+            const uint32_t v = col[e];
+            const uint32_t dv = rowptr[v + 1] - rowptr[v];
+            if (heavy && dv > thr) continue;
+            const int s = state[v];
+            if (s == 1) {
+                centre = true;
+            } else if (s == 0) {
+                const uint64_t k = ((uint64_t)dv << 32) | v;
+                key = k > key ? k : key;
+            }
+        }
+        centre = __ballot(centre) != 0;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const uint64_t k = __shfl_xor(key, o, WAVE);
+            key = k > key ? k : key;
+        }
+        if (lane == 0) best[r] = centre ? INF : (uint32_t)key;
+    }
+}
+
+// :193-206 -- a vertex that is its own best becomes a centre; one next to a
+// centre, or whose best becomes one now, is excluded; *left counts the rest
+__global__ void k_gosh_settle(uint32_t n, int* __restrict__ state, const uint32_t* __restrict__ best,
+                              uint32_t* __restrict__ left)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int s = state[i];
+    if (s == 0) {
+        const uint32_t b = best[i];
+        if (b == INF)
+            s = -1;
+        else if (b == i)
+            s = 1;
+        else if (best[b] == b)
+            s = -1;
+        state[i] = s;
+    }
+    if (s == 0) atomicAdd(left, 1u);
+}
+
+// :352-365 -- a centre keeps its own aggregate; any other vertex takes that
+// of its centre neighbour with the most entries, the first of equals in its
+// row (largest (entries, -position)); *err counts vertices with none
+__global__ void __launch_bounds__(BLOCK) k_gosh_assign(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                       const uint32_t* __restrict__ col,
+                                                       const int* __restrict__ state, const uint32_t* __restrict__ id,
+                                                       uint32_t* __restrict__ vcmap, uint32_t* __restrict__ err)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t r = wid; r < n; r += nw) {
+        if (state[r] == 1) {
+            if (lane == 0) vcmap[r] = id[r];
+            continue;
+        }
+        const uint32_t r0 = rowptr[r], r1 = rowptr[r + 1];
+        uint64_t key = 0;
+        for (uint32_t e = r0 + lane; e < r1; e += WAVE) {
+            const uint32_t v = col[e];
+            if (state[v] != 1) continue;
+            const uint32_t dv = rowptr[v + 1] - rowptr[v];
+            const uint64_t k = dv ? ((uint64_t)dv << 32) | (0xFFFFFFFFu - (e - r0)) : 0;
+            key = k > key ? k : key;
+        }
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const uint64_t k = __shfl_xor(key, o, WAVE);
+            key = k > key ? k : key;
+        }
+        if (lane == 0) {
+            if (key == 0) {
+                atomicAdd(err, 1u);
+            } else {
+                vcmap[r] = id[col[r0 + (0xFFFFFFFFu - (uint32_t)key)]];
+            }
+        }
+    }
+}
+
 }  // namespace
 
 uint64_t xs_pool_state(uint64_t seed)
@@ -710,14 +814,51 @@ static uint32_t mis2_map(Ctx& c, const DGraph& g, uint64_t seed, uint32_t* vcmap
     return h[0];
 }
 
+// One GOSH level (the gosh_* binaries): the centres, then the aggregates.
+static uint32_t gosh_map(Ctx& c, const DGraph& g, uint32_t* vcmap, HecTimes* t)
+{
+    const uint32_t n = g.n;
+    const uint32_t thr = g.m / n;  // nnz / n, :122
+    auto t0 = std::chrono::steady_clock::now();
+    DBuf<int> state(&c.pool, n);
+    DBuf<uint32_t> best(&c.pool, n), cnt(&c.pool, 2);
+    fill_u32(c, reinterpret_cast<uint32_t*>(state.get()), n, 0u);
+    const unsigned wg = (unsigned)std::min<uint64_t>(ceil_div(n, BLOCK / WAVE), 65536);
+    for (uint32_t round = 0;; round++) {
+        MI_CHECK(round <= n + 2, ST_INTERNAL, "GOSH rounds did not end");
+        k_gosh_max<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), thr, state.get(), best.get());
+        MI_HIP(hipGetLastError());
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_gosh_settle<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), best.get(), cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t left = 0;
+        read_words(c, cnt.get(), &left, 4);
+        if (left == 0) break;
+    }
+    // aggregates: the centres in index order, every other vertex with a centre
+    DBuf<uint32_t> flag(&c.pool, n), id(&c.pool, n), tw(&c.pool, 2);
+    k_mis_flag<<<grid1(n), BLOCK, 0, c.stream>>>(n, state.get(), flag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, flag.get(), id.get(), n, tw.get());
+    fill_u32(c, cnt.get() + 1, 1, 0u);
+    k_gosh_assign<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), state.get(), id.get(), vcmap, cnt.get() + 1);
+    MI_HIP(hipGetLastError());
+    uint32_t h[2] = {0, 0};
+    read_words(c, tw.get(), &h[0], 4);
+    read_words(c, cnt.get() + 1, &h[1], 4);
+    MI_CHECK(h[1] == 0, ST_ARG, "GOSH: a vertex next to no centre (rows that disagree)");
+    if (t) t->map_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return h[0];
+}
+
 uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed, uint32_t* vcmap,
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
-    if (c.coarsen == 5) {
-        MI_CHECK(!vperm_out && !hn_out, ST_ARG, "MIS-2 coarsening has no permutation or heavy neighbours");
-        return mis2_map(c, g, seed, vcmap, t);
+    if (c.coarsen == 5 || c.coarsen == 6) {
+        MI_CHECK(!vperm_out && !hn_out, ST_ARG, "MIS-2 and GOSH coarsening have no permutation or heavy neighbours");
+        return c.coarsen == 5 ? mis2_map(c, g, seed, vcmap, t) : gosh_map(c, g, vcmap, t);
     }
     hipEvent_t ev[4];
     if (t)

@@ -35,7 +35,8 @@
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
-// v3 map, with -DMIS (the mis_* binaries) with MIS-2 (mi_ctx_set_coarsening).
+// v3 map, with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
+// gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening).
 // Compiled with -DFM (as the reference's *_fm binaries are,
 // cmg/CMakeLists.txt:50) the context refines by FM (mi_ctx_set_refinement)
 // and the metrics record what the reference's FM path records: no coarse
@@ -72,9 +73,11 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#if defined HEC_V3 || defined MIS
-#ifdef MIS
+#if defined HEC_V3 || defined MIS || defined GOSH
+#if defined MIS
         const int variant = 5;
+#elif defined GOSH
+        const int variant = 6;
 #else
         const int variant = 3;
 #endif

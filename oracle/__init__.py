@@ -100,6 +100,10 @@ class _Core:
         L.orc_mis2.argtypes = [C.c_uint32, u32p, u32p, C.c_uint64, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_mis2.restype = C.c_uint32
         L.orc_coarsen_mis2.argtypes = [C.c_uint32, u32p, u32p, C.c_uint64, u32p]
+        L.orc_gosh_clusters.restype = C.c_int
+        L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
+        L.orc_coarsen_gosh.restype = C.c_uint32
+        L.orc_coarsen_gosh.argtypes = [C.c_uint32, u32p, u32p, u32p]
         L.orc_hierarchy_build_variant.restype = C.c_void_p
         L.orc_hierarchy_build_variant.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, C.c_int]
         L.orc_partition_graph_variant.restype = C.c_int
@@ -367,6 +371,27 @@ def coarsen_mis2(rowptr, col, seed: int):
     return vc[:n].copy(), int(nc)
 
 
+def gosh_clusters(rowptr, col):
+    """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
+    (heuristics.h:102-217)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    st = np.zeros(n + 1, dtype=np.int32)
+    r = L.orc_gosh_clusters(n, rowptr, col, st)
+    return st[:n].copy(), int(r)
+
+
+def coarsen_gosh(rowptr, col):
+    """(vcmap, nc) of one GOSH level (sgp_coarsen_GOSH, heuristics.h:312-377)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    vc = np.empty(n + 1, dtype=np.uint32)
+    nc = L.orc_coarsen_gosh(n, rowptr, col, vc)
+    return vc[:n].copy(), int(nc)
+
+
 def coarsen_hec3(level: int, rowptr, col, w, seed: int):
     """(vcmap, nc, hn) of one HEC level with the v3 map."""
     L = core().L
@@ -382,7 +407,8 @@ def coarsen_hec3(level: int, rowptr, col, w, seed: int):
 def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
                     normLap: int = 0, tol: float = 1e-10, variant: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
-    variant: 0 HEC, 3 HEC with the v3 map (the hec3_* binaries), 5 MIS-2 (mis_*)."""
+    variant: 0 HEC, 3 HEC with the v3 map (the hec3_* binaries), 5 MIS-2 (mis_*),
+    6 GOSH (gosh_*)."""
     L = core().L
     n = len(rowptr) - 1
     w, wp = _wptr(w)

@@ -437,6 +437,102 @@ vid_t orc_coarsen_mis2(vid_t n, const eid_t *rowptr, const vid_t *col, uint64_t 
 }
 
 /* ------------------------------------------------------------------ */
+/* GOSH coarsening (the gosh_* binaries, -DGOSH): GOSH_clusters,        */
+/* heuristics.h:102-217, and sgp_coarsen_GOSH, :312-377, on a one-thread */
+/* backend.  Tuples (state, degree, index); a round takes the largest of */
+/* each undecided vertex and its neighbours, skipping edges whose two    */
+/* ends both have more than nnz/n entries (:157-160).                    */
+/* ------------------------------------------------------------------ */
+typedef struct { int s; eid_t d; vid_t i; } orc_gtup_t;
+
+static int gtup_gt(orc_gtup_t a, orc_gtup_t b)  /* a beats b (:145-156) */
+{
+    if (a.s > b.s) return 1;
+    if (a.s == b.s) {
+        if (a.d > b.d) return 1;
+        if (a.d == b.d && a.i > b.i) return 1;
+    }
+    return 0;
+}
+
+/* states (n ints; 1 centre, -1 next to one) of GOSH_clusters; returns the
+   number of rounds. */
+int orc_gosh_clusters(vid_t n, const eid_t *rowptr, const vid_t *col, int *state)
+{
+    const eid_t thr = rowptr[n] / n;
+    orc_gtup_t *t = (orc_gtup_t *)malloc((size_t)n * sizeof(orc_gtup_t) + 16);
+    orc_gtup_t *u = (orc_gtup_t *)malloc((size_t)n * sizeof(orc_gtup_t) + 16);
+    vid_t *list = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    for (vid_t i = 0; i < n; i++) {
+        state[i] = 0;
+        list[i] = i;
+    }
+    vid_t left = n;
+    int rounds = 0;
+    while (left > 0) {
+        rounds++;
+        for (vid_t i = 0; i < n; i++) {
+            t[i].s = state[i];
+            t[i].d = rowptr[i + 1] - rowptr[i];
+            t[i].i = i;
+        }
+        for (vid_t k = 0; k < left; k++) {
+            const vid_t x = list[k];
+            const eid_t dx = rowptr[x + 1] - rowptr[x];
+            orc_gtup_t m = t[x];
+            for (eid_t e = rowptr[x]; e < rowptr[x + 1]; e++) {
+                const vid_t v = col[e];
+                if (dx > thr && rowptr[v + 1] - rowptr[v] > thr) continue;
+                if (gtup_gt(t[v], m)) m = t[v];
+            }
+            u[x] = m;
+        }
+        for (vid_t k = 0; k < left; k++) t[list[k]] = u[list[k]];
+        vid_t nl = 0;
+        for (vid_t k = 0; k < left; k++) {
+            const vid_t x = list[k];
+            if (t[x].i == x)
+                state[x] = 1;
+            else if (t[x].s == 1 || t[t[x].i].i == t[x].i)
+                state[x] = -1;
+            if (state[x] == 0) list[nl++] = x;
+        }
+        left = nl;
+    }
+    free(t);
+    free(u);
+    free(list);
+    return rounds;
+}
+
+/* One GOSH level: an aggregate per centre (ids in index order), every other
+   vertex in the aggregate of its centre neighbour with the most entries, the
+   first of equals in its row (:352-365).  A vertex left without one (none
+   is, on any input: every vertex that is not a centre was excluded through
+   an edge to one) keeps ORC_INF.  Returns nc. */
+vid_t orc_coarsen_gosh(vid_t n, const eid_t *rowptr, const vid_t *col, vid_t *vcmap)
+{
+    int *state = (int *)malloc((size_t)n * sizeof(int) + 4);
+    orc_gosh_clusters(n, rowptr, col, state);
+    vid_t nc = 0;
+    for (vid_t i = 0; i < n; i++) vcmap[i] = state[i] == 1 ? nc++ : ORC_INF;
+    for (vid_t i = 0; i < n; i++) {
+        if (state[i] == 1) continue;
+        eid_t best = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const vid_t v = col[e];
+            const eid_t d = rowptr[v + 1] - rowptr[v];
+            if (state[v] == 1 && d > best) {
+                best = d;
+                vcmap[i] = vcmap[v];
+            }
+        }
+    }
+    free(state);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Coarse graph, coarsen_kokkos.h:1061-1127 (map/count) with the       */
 /* non-skew builder :799-900 and the RADIX dedupe :750-762, :445-510:  */
 /* rows sorted by column, equal columns merged with summed weights,    */
@@ -544,7 +640,8 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
     return orc_hierarchy_build_variant(n, rowptr, col, w, seed, 0);
 }
 
-/* The level loop with HEC (variant 0), HEC with the v3 map (3) or MIS-2 (5). */
+/* The level loop with HEC (variant 0), HEC with the v3 map (3), MIS-2 (5)
+   or GOSH (6). */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
 {
@@ -569,6 +666,7 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
         a->vcmap = (vid_t *)malloc((size_t)a->n * sizeof(vid_t) + 4);
         vid_t nc = variant == 3   ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 5 ? orc_coarsen_mis2(a->n, a->rowptr, a->col, seed, a->vcmap)
+                   : variant == 6 ? orc_coarsen_gosh(a->n, a->rowptr, a->col, a->vcmap)
                                   : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
                                                     NULL, NULL);
         c->n = nc;
