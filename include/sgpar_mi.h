@@ -170,6 +170,20 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  */
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant);
 
+/*
+ * Coarse-graph build of this context's hierarchies (every level build and
+ * mi_build_coarse): builder 0 (default) the RADIX build of the reference's
+ * binaries (coarsen_kokkos.h:750-762), rows sorted by column; builder 1 the
+ * HASHMAP build of hec_spec_hashmap (-DHASHMAP, cmg/CMakeLists.txt:44;
+ * coarsen_kokkos.h:658-749 with the accumulator of :520-645) as it runs on
+ * one thread: each coarse row lists its columns in order of first
+ * occurrence among the fine entries (fine vertices ascending, each row in
+ * CSR order), weights summed.  The same graph as a set; the order decides
+ * the heavy-neighbour ties of the next level and the summation order of the
+ * products.  Other values: MI_STATUS_ARG.
+ */
+int mi_ctx_set_builder(mi_ctx* ctx, int builder);
+
 /* ---------------------------------------------------------------- graphs */
 /*
  * Uploads a CSR graph (host arrays) to HBM.  Replaces the host->device copy

@@ -94,6 +94,7 @@ _SIGS = [
     ("mi_ctx_set_bucketed_build", C.c_int, [vp, C.c_uint32]),
     ("mi_ctx_set_refinement", C.c_int, [vp, C.c_int]),
     ("mi_ctx_set_coarsening", C.c_int, [vp, C.c_int]),
+    ("mi_ctx_set_builder", C.c_int, [vp, C.c_int]),
     ("mi_fm_eigensolve", C.c_int, [vp, vp, C.c_uint64, C.c_double, vp, C.POINTER(C.c_uint32), C.POINTER(Stats)]),
     ("mi_fm_refine", C.c_int, [vp, vp, vp, u32p, C.POINTER(C.c_uint32), vp]),
     ("mi_fm_init", C.c_int, [vp, vp, vp, C.c_uint64, C.c_double, u32p]),
@@ -234,6 +235,12 @@ class Context:
         3 the hec3_* binaries' (parallel_map_construct_v3), 5 the mis_*
         binaries' (MIS-2 aggregates), 6 the gosh_* binaries' (GOSH centres)."""
         self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")
+
+    def set_builder(self, builder: int) -> None:
+        """The coarse-graph build: 0 the RADIX build (rows by column, default),
+        1 the HASHMAP build of hec_spec_hashmap (rows in order of first
+        occurrence)."""
+        self.check(self._L.mi_ctx_set_builder(self.h, builder), "mi_ctx_set_builder")
 
     def set_small_levels(self, max_entries: int = 1 << 17, chunk: int = 0) -> None:
         """Levels of <= 16384 vertices and <= max_entries entries iterate in one

@@ -485,6 +485,13 @@ int mi_ctx_set_coarsening(mi_ctx* ctx, int variant)
     return ST_OK;
 }
 
+int mi_ctx_set_builder(mi_ctx* ctx, int builder)
+{
+    if (!ctx || (builder != 0 && builder != 1)) return ST_ARG;
+    ctx->c.hashmap = builder == 1;
+    return ST_OK;
+}
+
 int mi_ctx_set_refinement(mi_ctx* ctx, int fm)
 {
     if (!ctx || (fm != 0 && fm != 1)) return ST_ARG;
@@ -932,6 +939,7 @@ std::vector<Ctx*> batch_workers(Ctx& c, size_t want)
         w.locality_min_n = c.locality_min_n;
         w.fm = c.fm;
         w.coarsen = c.coarsen;
+        w.hashmap = c.hashmap;
         out.push_back(&w);
     }
     return out;

@@ -37,21 +37,22 @@ namespace {
 
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
-// one lane per fine entry; the row comes from the row index
+// one lane per fine entry; the row comes from the row index.  The value is
+// the entry's weight, or (eidx) its index among the fine entries.
 template <class K>
 __global__ void __launch_bounds__(BLOCK) k_emit_edges(uint64_t m, const uint32_t* __restrict__ rowptr,
                                                       const uint32_t* __restrict__ roc,
                                                       const uint32_t* __restrict__ col,
                                                       const uint32_t* __restrict__ w,
                                                       const uint32_t* __restrict__ vcmap, uint32_t nc, int B,
-                                                      K* __restrict__ keys, uint32_t* __restrict__ vals)
+                                                      bool eidx, K* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e >= m) return;
     const uint32_t r = entry_row((uint32_t)e, roc, rowptr);
     const uint32_t cu = vcmap[r], cv = vcmap[col[e]];
     keys[e] = (cu == cv) ? ((K)nc << B) : (((K)cu << B) | (K)cv);
-    vals[e] = w ? w[e] : 1u;
+    vals[e] = eidx ? (uint32_t)e : (w ? w[e] : 1u);
 }
 
 __global__ void k_cluster_weights(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ vtx_w,
@@ -118,6 +119,56 @@ __global__ void __launch_bounds__(BLOCK) k_write_coarse(uint64_t m, const K* __r
     if (first && grp != INF) atomicAdd(&w_c[grp], ws);
     const uint32_t rc = wave_seg_reduce(row, one, plus, &first);
     if (first && row != INF && rc) atomicAdd(&rowcnt[row], rc);
+}
+
+// The HASHMAP build's merge: the sorted keys carry their fine entries'
+// indices (a stable sort keeps equal keys in fine-entry order, so a run's
+// head holds its first occurrence).  Weights are gathered from the fine
+// entries; every merged entry also gets the key row << EB | first
+// occurrence, whose order is the rows' order of first occurrence.
+template <class K>
+__global__ void __launch_bounds__(BLOCK) k_write_coarse_first(uint64_t m, const K* __restrict__ keys,
+                                                              const uint32_t* __restrict__ eidx,
+                                                              const uint32_t* __restrict__ fw,
+                                                              const uint32_t* __restrict__ uidx, K sentinel, int B,
+                                                              int EB, uint32_t* __restrict__ col_c,
+                                                              uint32_t* __restrict__ w_c, uint32_t* __restrict__ rowcnt,
+                                                              uint64_t* __restrict__ okey)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t grp = INF, row = INF, v = 0, one = 0;
+    bool head = false;
+    if (k < m) {
+        const K key = keys[k];
+        if (key < sentinel) {
+            head = (k == 0 || keys[k - 1] != key);
+            grp = head ? uidx[k] : uidx[k] - 1u;
+            row = (uint32_t)(key >> B);
+            const uint32_t e = eidx[k];
+            v = fw ? fw[e] : 1u;
+            one = head ? 1u : 0u;
+            if (head) {
+                col_c[grp] = (uint32_t)(key & (((K)1 << B) - (K)1));
+                okey[grp] = ((uint64_t)row << EB) | e;
+            }
+        }
+    }
+    auto plus = [](uint32_t x, uint32_t y) { return x + y; };
+    bool first;
+    const uint32_t ws = wave_seg_reduce(grp, v, plus, &first);
+    if (first && grp != INF) atomicAdd(&w_c[grp], ws);
+    const uint32_t rc = wave_seg_reduce(row, one, plus, &first);
+    if (first && row != INF && rc) atomicAdd(&rowcnt[row], rc);
+}
+
+__global__ void k_gather_entries(uint32_t mc, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ col,
+                                 const uint32_t* __restrict__ w, uint32_t* __restrict__ col2, uint32_t* __restrict__ w2)
+{
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= mc) return;
+    const uint32_t q = perm[p];
+    col2[p] = col[q];
+    w2[p] = w[q];
 }
 
 int bits_for(uint32_t nc)
@@ -825,9 +876,12 @@ __global__ void k_copy_rows(uint32_t nc, const uint32_t* __restrict__ rowptr, co
 // The CSR graph of nc rows whose entries are the m (key, value) pairs,
 // key = row << B | column (keys >= nc << B: dropped): the pairs sorted by
 // key, equal keys merged with their values summed (u32, exact), rows in
-// column order.  keys / vals are consumed.
+// column order.  keys / vals are consumed.  With first_order the values are
+// the fine entries' indices (fine weights fw, nullptr: unit) and each row
+// ends in order of first occurrence instead.
 template <class K>
-static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m, uint32_t nc, int B, DGraph& out)
+static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m, uint32_t nc, int B, DGraph& out,
+                        bool first_order = false, const uint32_t* fw = nullptr)
 {
     const K sentinel = (K)nc << B;
     DBuf<K> ktmp(&c.pool, m);
@@ -866,10 +920,33 @@ static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m,
     DBuf<uint32_t> rowcnt(&c.pool, (size_t)nc + 1);
     MI_HIP(hipMemsetAsync(rowcnt.get(), 0, ((size_t)nc + 1) * 4, c.stream));
     if (mc) MI_HIP(hipMemsetAsync(out.w.get(), 0, (size_t)mc * 4, c.stream));
-    if (m) {
+    if (m && !first_order) {
         k_write_coarse<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), vals.get(), uidx.get(), sentinel, B,
                                                             out.col.get(), out.w.get(), rowcnt.get());
         MI_HIP(hipGetLastError());
+    }
+    if (m && first_order) {
+        const int EB = bits_for((uint32_t)std::min<uint64_t>(m, 0xffffffffull));  // fine entry indices < 2^EB
+        MI_CHECK(EB + B <= 64, ST_INTERNAL, "first-occurrence keys too wide");
+        DBuf<uint64_t> okey(&c.pool, mc), okt(&c.pool, mc);
+        DBuf<uint32_t> perm(&c.pool, mc), pt(&c.pool, mc);
+        k_write_coarse_first<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), vals.get(), fw, uidx.get(),
+                                                                  sentinel, B, EB, out.col.get(), out.w.get(),
+                                                                  rowcnt.get(), okey.get());
+        MI_HIP(hipGetLastError());
+        keys.reset();
+        vals.reset();
+        uidx.reset();
+        if (mc) {
+            iota_u32(c, perm.get(), mc, 0);
+            radix_sort_pairs(c, okey.get(), perm.get(), mc, 0, EB + B, okt.get(), pt.get());
+            DBuf<uint32_t> col2(&c.pool, mc), w2(&c.pool, mc);
+            k_gather_entries<<<grid1(mc), BLOCK, 0, c.stream>>>(mc, perm.get(), out.col.get(), out.w.get(),
+                                                                col2.get(), w2.get());
+            MI_HIP(hipGetLastError());
+            std::swap(out.col, col2);
+            std::swap(out.w, w2);
+        }
     }
     scan_exclusive_u32(c, rowcnt.get(), out.rowptr.get(), (uint64_t)nc + 1, nullptr);
 }
@@ -882,11 +959,11 @@ static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, cons
     DBuf<K> keys(&c.pool, m);
     DBuf<uint32_t> vals(&c.pool, m);
     if (m) {
-        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, keys.get(),
-                                                          vals.get());
+        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, c.hashmap,
+                                                          keys.get(), vals.get());
         MI_HIP(hipGetLastError());
     }
-    keys_to_csr<K>(c, keys, vals, m, nc, B, out);
+    keys_to_csr<K>(c, keys, vals, m, nc, B, out, c.hashmap, g.W());
 }
 
 namespace {
@@ -1097,7 +1174,7 @@ void build_coarse(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* 
 
     c_vtx_w = DBuf<uint32_t>(&c.pool, nc);
     MI_HIP(hipMemsetAsync(c_vtx_w.get(), 0, (size_t)nc * 4, c.stream));
-    if (g.m >= c.bucketed_min_m) {
+    if (g.m >= c.bucketed_min_m && !c.hashmap) {
         // the bucketed form counts the cluster weights with its row counts
         if (build_coarse_bucketed(c, g, vcmap, nc, vtx_w, c_vtx_w.get(), out)) return;
     } else if (nc <= CW_LDS) {

@@ -166,6 +166,7 @@ struct Ctx {
     uint32_t bucketed_min_m = 0xffffffffu;  // coarse builds of levels with this many entries: bucketed form
     bool fm = false;                      // bisections refine by FM (the *_fm binaries) instead of power iterations
     int coarsen = 0;                      // HEC map: 0 the hec_* binaries', 3 the hec3_* binaries' (v3)
+    bool hashmap = false;                 // coarse rows in order of first occurrence (HASHMAP build), not by column
     std::vector<std::unique_ptr<Ctx>> workers;  // contexts of the batched bisections' host threads
     bool locality = true;                 // solve gather-form levels in a locality order
     uint32_t locality_min_n = 65536;      // ... those with at least this many vertices

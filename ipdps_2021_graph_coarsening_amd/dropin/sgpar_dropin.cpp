@@ -36,7 +36,9 @@
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
 // v3 map, with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
-// gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening).
+// gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening); with -DHASHMAP
+// (hec_spec_hashmap) its coarse rows keep the HASHMAP build's order
+// (mi_ctx_set_builder).
 // Compiled with -DFM (as the reference's *_fm binaries are,
 // cmg/CMakeLists.txt:50) the context refines by FM (mi_ctx_set_refinement)
 // and the metrics record what the reference's FM path records: no coarse
@@ -82,6 +84,12 @@ mi_ctx* context()
         const int variant = 3;
 #endif
         if (g_ctx && mi_ctx_set_coarsening(g_ctx, variant) != 0) {
+            mi_ctx_destroy(g_ctx);
+            g_ctx = nullptr;
+        }
+#endif
+#ifdef HASHMAP
+        if (g_ctx && mi_ctx_set_builder(g_ctx, 1) != 0) {
             mi_ctx_destroy(g_ctx);
             g_ctx = nullptr;
         }

@@ -63,6 +63,8 @@ class _Core:
         L.orc_build_coarse.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, u32p, C.c_uint32,
                                        C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p), u32p]
+        L.orc_build_coarse_order.restype = C.c_uint32
+        L.orc_build_coarse_order.argtypes = L.orc_build_coarse.argtypes + [C.c_int]
         L.orc_free.restype = None
         L.orc_free.argtypes = [C.c_void_p]
         L.orc_hierarchy_build.restype = C.c_void_p
@@ -222,16 +224,20 @@ def coarsen_hec(level: int, rowptr, col, w, seed: int):
     return vcmap, int(nc), vperm, hn
 
 
-def build_coarse(rowptr, col, w, vcmap, nc: int, vtx_w=None):
-    """Coarse CSR (rowptr_c, col_c, w_c, c_vtx_w) with sorted, merged rows."""
+def build_coarse(rowptr, col, w, vcmap, nc: int, vtx_w=None, builder: int = 0):
+    """Coarse CSR (rowptr_c, col_c, w_c, c_vtx_w) with merged rows: sorted by
+    column (builder 0, RADIX) or in order of first occurrence (builder 1,
+    the HASHMAP build of hec_spec_hashmap)."""
     L = core().L
     n = len(rowptr) - 1
     w, wp = _wptr(w)
     vw, vwp = _wptr(vtx_w)
     rp, cp, wc = C.c_void_p(), C.c_void_p(), C.c_void_p()
     cvw = np.zeros(nc + 1, dtype=np.uint32)
-    mc = L.orc_build_coarse(n, rowptr, col, wp, np.ascontiguousarray(vcmap, dtype=np.uint32),
-                            nc, vwp, C.byref(rp), C.byref(cp), C.byref(wc), cvw)
+    mc = L.orc_build_coarse_order(n, np.ascontiguousarray(rowptr, dtype=np.uint32),
+                                  np.ascontiguousarray(col, dtype=np.uint32), wp,
+                                  np.ascontiguousarray(vcmap, dtype=np.uint32),
+                                  nc, vwp, C.byref(rp), C.byref(cp), C.byref(wc), cvw, builder)
     r = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint32)), shape=(nc + 1,)).copy()
     c = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_uint32)), shape=(max(mc, 1),))[:mc].copy()
     ww = np.ctypeslib.as_array(C.cast(wc, C.POINTER(C.c_uint32)), shape=(max(mc, 1),))[:mc].copy()
@@ -243,12 +249,12 @@ def build_coarse(rowptr, col, w, vcmap, nc: int, vtx_w=None):
 class Hierarchy:
     """Multilevel hierarchy from the oracle (level 0 = finest)."""
 
-    def __init__(self, rowptr, col, w, seed: int, variant: int = 0):
+    def __init__(self, rowptr, col, w, seed: int, variant: int = 0, builder: int = 0):
         L = core().L
         self._L = L
         n = len(rowptr) - 1
         w, wp = _wptr(w)
-        self._h = L.orc_hierarchy_build_variant(n, rowptr, col, wp, seed, variant)
+        self._h = L.orc_hierarchy_build_variant(n, rowptr, col, wp, seed, variant | (builder << 8))
         self.nlevels = L.orc_hier_nlevels(self._h)
         self.levels = []
         for l in range(self.nlevels):
@@ -405,10 +411,10 @@ def coarsen_hec3(level: int, rowptr, col, w, seed: int):
 
 
 def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
-                    normLap: int = 0, tol: float = 1e-10, variant: int = 0):
+                    normLap: int = 0, tol: float = 1e-10, variant: int = 0, builder: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
     variant: 0 HEC, 3 HEC with the v3 map (the hec3_* binaries), 5 MIS-2 (mis_*),
-    6 GOSH (gosh_*)."""
+    6 GOSH (gosh_*); builder 1: the HASHMAP build (hec_spec_hashmap)."""
     L = core().L
     n = len(rowptr) - 1
     w, wp = _wptr(w)
@@ -419,7 +425,7 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
     it = np.zeros(210, dtype=np.uint64)
     ev = np.empty(n, dtype=np.float64)
     nl = L.orc_partition_graph_variant(n, rowptr, col, wp, hec_seed, C.byref(st), pcg_inc, normLap, tol,
-                                       part, C.byref(ec), lv, it, ev, variant)
+                                       part, C.byref(ec), lv, it, ev, variant | (builder << 8))
     return dict(part=part, edge_cut=ec.value, levels_n=lv[:nl].copy(), iters=it[:nl].copy(),
                 evec=ev, pcg_state=st.value)
 

@@ -538,6 +538,14 @@ vid_t orc_coarsen_gosh(vid_t n, const eid_t *rowptr, const vid_t *col, vid_t *vc
 /* rows sorted by column, equal columns merged with summed weights,    */
 /* self-loops dropped.  Returns m_c; *rowptr_c (nc+1), *col_c, *w_c    */
 /* are malloc'd.  c_vtx_w[c] = sum of f_vtx_w over the cluster.        */
+/* orc_build_coarse_order with hashmap = 1 is the HASHMAP dedupe of the */
+/* hec_spec_hashmap binary (-DHASHMAP, :658-749, functor :520-645) on a */
+/* one-thread backend instead: a bucket holds its entries in fine-entry */
+/* order (fine vertices ascending, each row in CSR order, :862-876) and */
+/* the accumulator writes each new column at the next free slot of the  */
+/* bucket (keys = entries + row_map(idx), :583-590; the row's length is */
+/* used_hash_size, :641), adding the weights of repeats: rows in order  */
+/* of first occurrence.  The Serial build is the non-skew one (:1113).  */
 /* ------------------------------------------------------------------ */
 typedef struct { vid_t c; wgt_t w; } orc_cw_t;
 
@@ -548,9 +556,20 @@ static int cmp_cw(const void *a, const void *b)
     return 0;
 }
 
+eid_t orc_build_coarse_order(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                             const vid_t *vcmap, vid_t nc, const wgt_t *f_vtx_w,
+                             eid_t **rowptr_c, vid_t **col_c, wgt_t **w_c, wgt_t *c_vtx_w, int hashmap);
+
 eid_t orc_build_coarse(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
                        const vid_t *vcmap, vid_t nc, const wgt_t *f_vtx_w,
                        eid_t **rowptr_c, vid_t **col_c, wgt_t **w_c, wgt_t *c_vtx_w)
+{
+    return orc_build_coarse_order(n, rowptr, col, w, vcmap, nc, f_vtx_w, rowptr_c, col_c, w_c, c_vtx_w, 0);
+}
+
+eid_t orc_build_coarse_order(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                             const vid_t *vcmap, vid_t nc, const wgt_t *f_vtx_w,
+                             eid_t **rowptr_c, vid_t **col_c, wgt_t **w_c, wgt_t *c_vtx_w, int hashmap)
 {
     eid_t *cnt = (eid_t *)calloc((size_t)nc + 1, sizeof(eid_t));
     for (vid_t i = 0; i < n; i++) {
@@ -576,7 +595,30 @@ eid_t orc_build_coarse(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt
     }
     eid_t *rc = (eid_t *)calloc((size_t)nc + 1, sizeof(eid_t));
     eid_t mc = 0;
-    for (vid_t c = 0; c < nc; c++) {
+    eid_t *slot = NULL;
+    if (hashmap) {
+        slot = (eid_t *)malloc(((size_t)nc + 1) * sizeof(eid_t));
+        for (vid_t c = 0; c <= nc; c++) slot[c] = (eid_t)ORC_INF;
+    }
+    for (vid_t c = 0; c < nc && hashmap; c++) {
+        orc_cw_t *row = buf + bucket[c];
+        eid_t len = bucket[c + 1] - bucket[c];
+        eid_t out = 0;
+        for (eid_t k = 0; k < len; k++) {
+            const vid_t v = row[k].c;
+            if (slot[v] == (eid_t)ORC_INF) {
+                slot[v] = out;
+                row[out++] = row[k];
+            } else {
+                row[slot[v]].w += row[k].w;
+            }
+        }
+        for (eid_t k = 0; k < out; k++) slot[row[k].c] = (eid_t)ORC_INF;
+        cnt[c] = out;
+        mc += out;
+    }
+    free(slot);
+    for (vid_t c = 0; c < nc && !hashmap; c++) {
         orc_cw_t *row = buf + bucket[c];
         eid_t len = bucket[c + 1] - bucket[c];
         qsort(row, len, sizeof(orc_cw_t), cmp_cw);
@@ -641,10 +683,13 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
 }
 
 /* The level loop with HEC (variant 0), HEC with the v3 map (3), MIS-2 (5)
-   or GOSH (6). */
+   or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+   (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
 {
+    const int hashmap = (variant >> 8) & 1;
+    variant &= 0xff;
     orc_hier_t *h = (orc_hier_t *)calloc(1, sizeof(orc_hier_t));
     orc_level_t *f = &h->lv[0];
     f->n = n;
@@ -671,8 +716,8 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
                                                     NULL, NULL);
         c->n = nc;
         c->vtx_w = (wgt_t *)calloc((size_t)nc + 1, sizeof(wgt_t));
-        c->m = orc_build_coarse(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
-                                &c->rowptr, &c->col, &c->w, c->vtx_w);
+        c->m = orc_build_coarse_order(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
+                                      &c->rowptr, &c->col, &c->w, c->vtx_w, hashmap);
         h->nlevels++;
         if ((unsigned)h->nlevels > ORC_MAX_GRAPHS) break;
     }

@@ -47,6 +47,7 @@ int main(void)
     int (*fp_mi_ctx_set_bucketed_build)(mi_ctx*, uint32_t);
     int (*fp_mi_ctx_set_refinement)(mi_ctx*, int);
     int (*fp_mi_ctx_set_coarsening)(mi_ctx*, int);
+    int (*fp_mi_ctx_set_builder)(mi_ctx*, int);
     int (*fp_mi_fm_eigensolve)(mi_ctx*, const mi_hier*, uint64_t, double, double*, uint32_t*, mi_stats*);
     int (*fp_mi_fm_refine)(mi_ctx*, const mi_graph*, const uint32_t*, uint32_t*, uint32_t*, int64_t*);
     int (*fp_mi_fm_init)(mi_ctx*, const mi_graph*, const uint32_t*, uint64_t, double, uint32_t*);
@@ -94,6 +95,7 @@ int main(void)
     BIND(mi_ctx_set_bucketed_build);
     BIND(mi_ctx_set_refinement);
     BIND(mi_ctx_set_coarsening);
+    BIND(mi_ctx_set_builder);
     BIND(mi_fm_eigensolve);
     BIND(mi_fm_refine);
     BIND(mi_fm_init);
