@@ -149,7 +149,17 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
 /*
  * Coarsening map of this context's hierarchies (mi_hierarchy_build, every
  * bisection, mi_hec_level): variant 0 (default) the HEC map of the hec_*
- * binaries (heuristics.h:633-693); variant 3 the map of the hec3_* binaries
+ * binaries (heuristics.h:633-693); variant 1 the heavy-edge matching of the
+ * match_* binaries (-DPUREMATCH, cmg/CMakeLists.txt:45,53; sgp_coarsen_match,
+ * heuristics.h:1584-1735, 1890-1895): HEC's permutation and level-1 random
+ * neighbours, the first heaviest neighbour above level 1, rounds in
+ * permutation order with the ordering condition alternating in which a
+ * vertex pairs with its heavy neighbour when both are unmatched at its turn
+ * (ids in order of pairing), unmatched vertices moving on with their last
+ * unmatched neighbour of the largest position (level 1) or weight, the rest
+ * singletons in index order -- a level with a vertex without entries, or
+ * with only self-loops left, has no map in the reference and gives
+ * MI_STATUS_ARG; variant 3 the map of the hec3_* binaries
  * (compiled with -DHEC_V3, cmg/CMakeLists.txt:42,52): parallel_map_construct_v3
  * (heuristics.h:766-826) on the same heavy neighbours -- mutual heavy pairs
  * keyed by their smaller end, heavy targets keyed by themselves, every

@@ -70,7 +70,7 @@ DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
 # the same compiled with -DFM, as the reference's *_fm binaries are, and with
-# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH and -DHASHMAP likewise
+# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH, -DHASHMAP and -DPUREMATCH likewise
 DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
 DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
 DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
@@ -83,6 +83,10 @@ DROPIN_GOSH_LIB = os.path.join(PKG, "libsgpar_dropin_gosh.so")
 DROPIN_GOSH_EXE = os.path.join(DROPIN, "sgpar_dropin_gosh")
 DROPIN_HASHMAP_LIB = os.path.join(PKG, "libsgpar_dropin_hashmap.so")
 DROPIN_HASHMAP_EXE = os.path.join(DROPIN, "sgpar_dropin_hashmap")
+DROPIN_MATCH_LIB = os.path.join(PKG, "libsgpar_dropin_match.so")
+DROPIN_MATCH_EXE = os.path.join(DROPIN, "sgpar_dropin_match")
+DROPIN_MATCH_FM_LIB = os.path.join(PKG, "libsgpar_dropin_match_fm.so")
+DROPIN_MATCH_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_match_fm")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -102,7 +106,9 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
                             (DROPIN_HEC3_FM_LIB, DROPIN_HEC3_FM_EXE, ["-DHEC_V3", "-DFM"]),
                             (DROPIN_MIS_LIB, DROPIN_MIS_EXE, ["-DMIS"]),
                             (DROPIN_GOSH_LIB, DROPIN_GOSH_EXE, ["-DGOSH"]),
-                            (DROPIN_HASHMAP_LIB, DROPIN_HASHMAP_EXE, ["-DHASHMAP"])):
+                            (DROPIN_HASHMAP_LIB, DROPIN_HASHMAP_EXE, ["-DHASHMAP"]),
+                            (DROPIN_MATCH_LIB, DROPIN_MATCH_EXE, ["-DPUREMATCH"]),
+                            (DROPIN_MATCH_FM_LIB, DROPIN_MATCH_FM_EXE, ["-DPUREMATCH", "-DFM"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

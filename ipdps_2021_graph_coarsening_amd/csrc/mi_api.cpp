@@ -480,7 +480,7 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
 
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant)
 {
-    if (!ctx || (variant != 0 && variant != 3 && variant != 5 && variant != 6)) return ST_ARG;
+    if (!ctx || (variant != 0 && variant != 1 && variant != 3 && variant != 5 && variant != 6)) return ST_ARG;
     ctx->c.coarsen = variant;
     return ST_OK;
 }
@@ -917,6 +917,7 @@ uint32_t bisect(Ctx& c, const DGraph& g, const mi_config* config, int perc_imbal
 // Worker contexts of the batched bisections (their own stream, pool and
 // reduction scratch), made once per context and kept; each call copies the
 // level policies of c into them.
+extern "C++" {
 constexpr size_t BATCH_WORKERS = 8;
 std::vector<Ctx*> batch_workers(Ctx& c, size_t want)
 {
@@ -948,7 +949,6 @@ std::vector<Ctx*> batch_workers(Ctx& c, size_t want)
 // f(worker, i) for i = 0 .. K-1, one host thread per worker taking the next
 // i; returns when every worker's stream has drained.  The first error of any
 // worker is rethrown after all have stopped.
-extern "C++" {
 template <class F>
 void run_on_workers(const std::vector<Ctx*>& ws, size_t K, F&& f)
 {

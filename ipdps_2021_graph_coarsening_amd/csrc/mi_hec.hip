@@ -100,13 +100,14 @@ __global__ void k_hn_random(uint32_t n, const uint32_t* __restrict__ rowptr,
     hn[i] = d ? col[b + (uint32_t)(r % d)] : (uint32_t)(r % n);
 }
 
-// level > 1: heaviest neighbour, ties to the last CSR index: per row, the max
-// of (weight, edge index) packed in 64 bits.  One lane per entry; a
+// level > 1: heaviest neighbour, ties to the last CSR index (first: to the
+// first, as heavy-edge matching takes it): per row, the max of (weight,
+// edge index or its complement) packed in 64 bits.  One lane per entry; a
 // wavefront reduces each run of equal rows and its first lane merges the
 // run's max with one atomicMax, so hub rows cost no serial walk.
 __global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint64_t m, const uint32_t* __restrict__ rowptr,
                                                     const uint32_t* __restrict__ roc,
-                                                    const uint32_t* __restrict__ w,
+                                                    const uint32_t* __restrict__ w, bool first,
                                                     unsigned long long* __restrict__ best)
 {
     const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint64_t m, const uint32_t* 
     unsigned long long key = 0;
     if (e < m) {
         row = entry_row((uint32_t)e, roc, rowptr);
-        key = ((unsigned long long)(w ? w[e] : 1u) << 32) | e;
+        key = ((unsigned long long)(w ? w[e] : 1u) << 32) | (first ? 0xFFFFFFFFu - (uint32_t)e : (uint32_t)e);
     }
     bool head;
     key = wave_seg_reduce(row, key, [](unsigned long long x, unsigned long long y) { return x > y ? x : y; },
@@ -123,13 +124,14 @@ __global__ void __launch_bounds__(BLOCK) k_hn_heavy(uint64_t m, const uint32_t* 
 }
 
 __global__ void k_hn_from_best(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
-                               const unsigned long long* __restrict__ best, uint32_t* __restrict__ hn,
+                               const unsigned long long* __restrict__ best, bool first, uint32_t* __restrict__ hn,
                                uint32_t* __restrict__ iso)
 {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     if (rowptr[i + 1] > rowptr[i]) {
-        hn[i] = col[(uint32_t)best[i]];
+        const uint32_t lo = (uint32_t)best[i];
+        hn[i] = col[first ? 0xFFFFFFFFu - lo : lo];
         iso[i] = 0;
     } else {
         iso[i] = 1;
@@ -624,6 +626,163 @@ __global__ void k_mis_follow(uint32_t n, const uint32_t* __restrict__ reached, u
     vcmap[v] = vcmap[p];
 }
 
+// ---- Heavy-edge matching (sgp_coarsen_match, heuristics.h:1584-1735 and
+// :1890-1895; the match_* binaries).  A round walks its list in order and
+// pairs u with hn(u) when both are unmatched at u's turn (active edges only:
+// the ordering condition holds, hn(u) != u and hn(u) unmatched when the
+// round starts).  That is the lexicographically first maximal matching of
+// the active edges in list order, found here by rounds of local minima:
+// an undecided edge that is the earliest undecided edge at both of its ends
+// is taken (every earlier edge at its ends has been dropped, so the walk
+// takes it too), and edges that touch a taken vertex are dropped.
+constexpr uint32_t PENDING = 0xFFFFFFFEu;  // vcmap of a vertex taken this round, before its id
+
+__global__ void k_match_empty(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && rowptr[i + 1] == rowptr[i]) atomicAdd(err, 1u);
+}
+
+// est[k]: 1 undecided, 2 taken, 0 not in the round or dropped
+__global__ void k_match_active(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                               const uint32_t* __restrict__ pos, const uint32_t* __restrict__ vcmap, int swap,
+                               uint8_t* __restrict__ est, uint32_t* __restrict__ und)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len) return;
+    const uint32_t u = list[k], v = hn[u];
+    const int cond = pos[u] < pos[v] ? 1 : 0;
+    const bool a = (cond ^ swap) && u != v && vcmap[v] == INF;
+    est[k] = a ? 1 : 0;
+    if (a) atomicAdd(und, 1u);
+}
+
+__global__ void k_match_min(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                            const uint8_t* __restrict__ est, uint32_t* __restrict__ first)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len || est[k] != 1) return;
+    const uint32_t u = list[k];
+    atomicMin(&first[u], k);
+    atomicMin(&first[hn[u]], k);
+}
+
+__global__ void k_match_take(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                             uint8_t* __restrict__ est, const uint32_t* __restrict__ first,
+                             uint32_t* __restrict__ vcmap)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len || est[k] != 1) return;
+    const uint32_t u = list[k], v = hn[u];
+    if (first[u] == k && first[v] == k) {
+        est[k] = 2;
+        vcmap[u] = PENDING;
+        vcmap[v] = PENDING;
+    }
+}
+
+// drops the undecided edges that touch a taken vertex, clears the minima of
+// this step's edges, and counts the edges still undecided
+__global__ void k_match_drop(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                             uint8_t* __restrict__ est, uint32_t* __restrict__ first,
+                             const uint32_t* __restrict__ vcmap, uint32_t* __restrict__ und)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len) return;
+    const uint8_t s = est[k];
+    if (s == 0) return;
+    const uint32_t u = list[k], v = hn[u];
+    first[u] = INF;
+    first[v] = INF;
+    if (s != 1) return;
+    if (vcmap[u] != INF || vcmap[v] != INF) {
+        est[k] = 0;
+    } else {
+        atomicAdd(und, 1u);
+    }
+}
+
+__global__ void k_match_flag(uint32_t len, const uint8_t* __restrict__ est, uint32_t* __restrict__ flag)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < len) flag[k] = est[k] == 2 ? 1u : 0u;
+}
+
+// ids in list order, after the nc0 of earlier rounds
+__global__ void k_match_ids(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                            const uint8_t* __restrict__ est, const uint32_t* __restrict__ rank, uint32_t nc0,
+                            uint32_t* __restrict__ vcmap)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len || est[k] != 2) return;
+    const uint32_t u = list[k];
+    vcmap[u] = nc0 + rank[k];
+    vcmap[hn[u]] = nc0 + rank[k];
+}
+
+// :1682-1719 -- an unmatched u keeps going with its last unmatched neighbour
+// of the largest position (level 1) or weight; one wavefront per list slot
+__global__ void __launch_bounds__(BLOCK) k_match_repick(uint32_t len, const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ rowptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ w,
+                                                        const uint32_t* __restrict__ pos, int level,
+                                                        const uint32_t* __restrict__ vcmap,
+                                                        uint32_t* __restrict__ hn, uint32_t* __restrict__ keep)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t k = wid; k < len; k += nw) {
+        const uint32_t u = list[k];
+        if (vcmap[u] != INF) {
+            if (lane == 0) keep[k] = 0;
+            continue;
+        }
+        const uint32_t r0 = rowptr[u], r1 = rowptr[u + 1];
+        uint64_t key = 0;
+        bool any = false;
+        for (uint32_t e = r0 + lane; e < r1; e += WAVE) {
+            const uint32_t v = col[e];
+            if (vcmap[v] != INF) continue;
+            const uint32_t kv = level == 1 ? pos[v] : (w ? w[e] : 1u);
+            const uint64_t x = ((uint64_t)kv << 32) | e;
+            key = (!any || x > key) ? x : key;
+            any = true;
+        }
+        const bool found = __ballot(any) != 0;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const uint64_t x = __shfl_xor(key, o, WAVE);
+            key = x > key ? x : key;
+        }
+        if (lane == 0) {
+            keep[k] = found ? 1u : 0u;
+            if (found) hn[u] = col[(uint32_t)key];
+        }
+    }
+}
+
+__global__ void k_match_compact(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ keep,
+                                const uint32_t* __restrict__ at, uint32_t* __restrict__ next)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < len && keep[k]) next[at[k]] = list[k];
+}
+
+__global__ void k_match_unmatched(uint32_t n, const uint32_t* __restrict__ vcmap, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) flag[i] = vcmap[i] == INF ? 1u : 0u;
+}
+
+__global__ void k_match_singletons(uint32_t n, const uint32_t* __restrict__ rank, uint32_t nc0,
+                                   uint32_t* __restrict__ vcmap)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && vcmap[i] == INF) vcmap[i] = nc0 + rank[i];
+}
+
 // ---- GOSH coarsening (GOSH_clusters, heuristics.h:102-217, and the
 // aggregation of sgp_coarsen_GOSH, :312-377; the gosh_* binaries).  A round
 // gives every undecided vertex the largest (state, entries, index) of itself
@@ -814,6 +973,75 @@ static uint32_t mis2_map(Ctx& c, const DGraph& g, uint64_t seed, uint32_t* vcmap
     return h[0];
 }
 
+// The rounds of heavy-edge matching from the permutation's positions and the
+// first heavy neighbours (hn is consumed), then the singletons.
+static uint32_t match_rounds(Ctx& c, const DGraph& g, int level, const uint32_t* vperm, const uint32_t* pos,
+                             uint32_t* hn, uint32_t* vcmap)
+{
+    const uint32_t n = g.n;
+    DBuf<uint32_t> la(&c.pool, n), lb(&c.pool, n), first(&c.pool, n), flag(&c.pool, n), rank(&c.pool, n),
+        cnt(&c.pool, 4);
+    DBuf<uint8_t> est(&c.pool, n);
+    MI_HIP(hipMemcpyAsync(la.get(), vperm, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    fill_u32(c, vcmap, n, INF);
+    fill_u32(c, first.get(), n, INF);
+    uint32_t len = n, nc = 0;
+    int swap = 1, idle = 0;
+    for (uint32_t round = 0; len > 0; round++) {
+        MI_CHECK(round <= 2ull * n + 4, ST_INTERNAL, "matching rounds did not end");
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_match_active<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, pos, vcmap, swap, est.get(),
+                                                           cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t und = 0;
+        read_words(c, cnt.get(), &und, 4);
+        for (uint32_t step = 0; und > 0; step++) {
+            MI_CHECK(step <= len, ST_INTERNAL, "matching steps did not end");
+            k_match_min<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get());
+            MI_HIP(hipGetLastError());
+            k_match_take<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get(), vcmap);
+            MI_HIP(hipGetLastError());
+            fill_u32(c, cnt.get(), 1, 0u);
+            k_match_drop<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get(), vcmap,
+                                                             cnt.get());
+            MI_HIP(hipGetLastError());
+            read_words(c, cnt.get(), &und, 4);
+        }
+        // pair ids in list order
+        k_match_flag<<<grid1(len), BLOCK, 0, c.stream>>>(len, est.get(), flag.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, flag.get(), rank.get(), len, cnt.get() + 1);
+        k_match_ids<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), rank.get(), nc, vcmap);
+        MI_HIP(hipGetLastError());
+        // the next list: unmatched vertices with an unmatched neighbour, in order
+        k_match_repick<<<(unsigned)std::min<uint64_t>(ceil_div(len, BLOCK / WAVE), 65536), BLOCK, 0, c.stream>>>(
+            len, la.get(), g.R(), g.Cc(), g.W(), pos, level, vcmap, hn, flag.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, flag.get(), rank.get(), len, cnt.get() + 2);
+        k_match_compact<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), flag.get(), rank.get(), lb.get());
+        MI_HIP(hipGetLastError());
+        uint32_t h[2] = {0, 0};
+        read_words(c, cnt.get() + 1, h, 8);
+        nc += h[0];
+        idle = h[0] ? 0 : idle + 1;
+        // three rounds in a row without a pair: the last two held the same
+        // edges under both orderings, so only self-loops are left (the
+        // reference loops for ever)
+        MI_CHECK(!(idle >= 3 && h[1] > 0), ST_ARG, "matching: only self-loops left");
+        len = h[1];
+        swap ^= 1;
+        la.swap(lb);
+    }
+    k_match_unmatched<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, flag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, flag.get(), rank.get(), n, cnt.get() + 3);
+    k_match_singletons<<<grid1(n), BLOCK, 0, c.stream>>>(n, rank.get(), nc, vcmap);
+    MI_HIP(hipGetLastError());
+    uint32_t ns = 0;
+    read_words(c, cnt.get() + 3, &ns, 4);
+    return nc + ns;
+}
+
 // One GOSH level (the gosh_* binaries): the centres, then the aggregates.
 static uint32_t gosh_map(Ctx& c, const DGraph& g, uint32_t* vcmap, HecTimes* t)
 {
@@ -882,7 +1110,17 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     vtmp.reset();
     if (t) MI_HIP(hipEventRecord(ev[1], c.stream));
 
-    // ---- heavy neighbour
+    // ---- heavy neighbour (matching: the first heaviest; no row may be empty)
+    const bool matching = c.coarsen == 1;
+    if (matching) {
+        DBuf<uint32_t> e(&c.pool, 1);
+        fill_u32(c, e.get(), 1, 0u);
+        k_match_empty<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), e.get());
+        MI_HIP(hipGetLastError());
+        uint32_t ne = 0;
+        read_words(c, e.get(), &ne, 4);
+        MI_CHECK(ne == 0, ST_ARG, "matching: a vertex without entries has no heavy neighbour");
+    }
     DBuf<uint32_t> hn(&c.pool, n);
     if (level == 1) {
         k_hn_random<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), draws.get(), hn.get());
@@ -892,10 +1130,12 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
         DBuf<unsigned long long> best(&c.pool, n);
         MI_HIP(hipMemsetAsync(best.get(), 0, (size_t)n * 8, c.stream));
         if (g.m) {
-            k_hn_heavy<<<(unsigned)ceil_div(g.m, BLOCK), BLOCK, 0, c.stream>>>(g.m, g.R(), roc, g.W(), best.get());
+            k_hn_heavy<<<(unsigned)ceil_div(g.m, BLOCK), BLOCK, 0, c.stream>>>(g.m, g.R(), roc, g.W(), matching,
+                                                                              best.get());
             MI_HIP(hipGetLastError());
         }
-        k_hn_from_best<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), best.get(), hn.get(), iso.get());
+        k_hn_from_best<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), best.get(), matching, hn.get(),
+                                                         iso.get());
         MI_HIP(hipGetLastError());
         scan_exclusive_u32(c, iso.get(), rank.get(), n, nullptr);
         k_hn_iso<<<grid1(n), BLOCK, 0, c.stream>>>(n, iso.get(), rank.get(), draws.get(), hn.get());
@@ -903,6 +1143,26 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     }
     draws.reset();
     if (t) MI_HIP(hipEventRecord(ev[2], c.stream));
+
+    if (matching) {
+        if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        if (vperm_out)
+            MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        const uint32_t nc = match_rounds(c, g, level, vperm.get(), pos.get(), hn.get(), vcmap);
+        if (t) {
+            MI_HIP(hipEventRecord(ev[3], c.stream));
+            float a = 0, b = 0, d = 0;
+            MI_HIP(hipEventSynchronize(ev[3]));
+            MI_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            MI_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+            MI_HIP(hipEventElapsedTime(&d, ev[2], ev[3]));
+            t->permute_ms += a;
+            t->heavy_ms += b;
+            t->map_ms += d;
+            for (auto& e : ev) MI_HIP(hipEventDestroy(e));
+        }
+        return nc;
+    }
 
     uint32_t nc_v3 = 0;
     if (c.coarsen == 3) {

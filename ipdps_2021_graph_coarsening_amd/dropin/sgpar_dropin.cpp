@@ -35,7 +35,8 @@
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
-// v3 map, with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
+// v3 map, with -DPUREMATCH (the match_* binaries) with heavy-edge matching,
+// with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
 // gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening); with -DHASHMAP
 // (hec_spec_hashmap) its coarse rows keep the HASHMAP build's order
 // (mi_ctx_set_builder).
@@ -75,9 +76,11 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#if defined HEC_V3 || defined MIS || defined GOSH
+#if defined HEC_V3 || defined MIS || defined GOSH || defined PUREMATCH
 #if defined MIS
         const int variant = 5;
+#elif defined PUREMATCH
+        const int variant = 1;
 #elif defined GOSH
         const int variant = 6;
 #else

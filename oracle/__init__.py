@@ -102,6 +102,9 @@ class _Core:
         L.orc_mis2.argtypes = [C.c_uint32, u32p, u32p, C.c_uint64, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_mis2.restype = C.c_uint32
         L.orc_coarsen_mis2.argtypes = [C.c_uint32, u32p, u32p, C.c_uint64, u32p]
+        L.orc_coarsen_match.restype = C.c_uint32
+        L.orc_coarsen_match.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p,
+                                        C.POINTER(C.c_int)]
         L.orc_gosh_clusters.restype = C.c_int
         L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_gosh.restype = C.c_uint32
@@ -377,6 +380,20 @@ def coarsen_mis2(rowptr, col, seed: int):
     return vc[:n].copy(), int(nc)
 
 
+def coarsen_match(level: int, rowptr, col, w, seed: int):
+    """(vcmap, nc, rounds) of one heavy-edge matching level (sgp_coarsen_match,
+    heuristics.h:1584-1735, -DPUREMATCH); nc = None when the reference has no
+    result (a row without entries, or only self-loops left)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    r = C.c_int(0)
+    nc = L.orc_coarsen_match(level, n, rowptr, col, wp, seed, vc, C.byref(r))
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
+
+
 def gosh_clusters(rowptr, col):
     """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
     (heuristics.h:102-217)."""
@@ -413,8 +430,9 @@ def coarsen_hec3(level: int, rowptr, col, w, seed: int):
 def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
                     normLap: int = 0, tol: float = 1e-10, variant: int = 0, builder: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
-    variant: 0 HEC, 3 HEC with the v3 map (the hec3_* binaries), 5 MIS-2 (mis_*),
-    6 GOSH (gosh_*); builder 1: the HASHMAP build (hec_spec_hashmap)."""
+    variant: 0 HEC, 1 heavy-edge matching (match_*), 3 HEC with the v3 map (the
+    hec3_* binaries), 5 MIS-2 (mis_*), 6 GOSH (gosh_*); builder 1: the HASHMAP
+    build (hec_spec_hashmap)."""
     L = core().L
     n = len(rowptr) - 1
     w, wp = _wptr(w)

@@ -351,6 +351,119 @@ vid_t orc_coarsen_hec3(int level, vid_t n, const eid_t *rowptr, const vid_t *col
 }
 
 /* ------------------------------------------------------------------ */
+/* Heavy-edge matching (the match_* binaries, -DPUREMATCH):             */
+/* sgp_coarsen_match, heuristics.h:1584-1735 and :1890-1895, on a       */
+/* one-thread backend.  The pool, permutation and level-1 draws are     */
+/* HEC's (:1603-1626); above level 1 the heavy neighbour is the first   */
+/* heaviest entry of the row (:1628-1640).  Rounds in list order with   */
+/* the ordering condition alternating (:1659-1677): u pairs with hn(u)  */
+/* when both are unmatched at its turn (a failed second CAS undoes the  */
+/* first), ids in order of pairing; every unmatched u that still has an */
+/* unmatched neighbour goes to the next round with the last neighbour   */
+/* of the largest position (level 1) or weight (:1682-1719); the rest   */
+/* become singletons in index order (:1890-1895).  A vertex with no     */
+/* entries at level 1, or with an empty row above it, has no heavy      */
+/* neighbour in the reference (a division by zero, a read past the      */
+/* row): returns ORC_INF.  So do three rounds in a row that pair       */
+/* nothing (the last two then hold the same edges under both orderings,  */
+/* so only self-loops are left and the reference loops for ever).       */
+/* ------------------------------------------------------------------ */
+vid_t orc_coarsen_match(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                        uint64_t seed, vid_t *vcmap, int *rounds_out)
+{
+    uint64_t st = orc_pool_state(seed);
+    vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *pos = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *list = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *next = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t nc = 0;
+    int rounds = 0;
+    orc_generate_permutation(n, &st, vperm, NULL);
+    for (vid_t i = 0; i < n; i++) pos[vperm[i]] = i;
+    for (vid_t i = 0; i < n; i++) {
+        const eid_t b = rowptr[i], deg = rowptr[i + 1] - b;
+        if (deg == 0) {
+            nc = ORC_INF;
+            goto done;
+        }
+        if (level == 1) {
+            hn[i] = col[b + (eid_t)(orc_xs_urand64(&st) % deg)];
+        } else {
+            vid_t h = col[b];
+            wgt_t best = w ? w[b] : 1u;
+            for (eid_t e = b + 1; e < b + deg; e++)
+                if (best < (w ? w[e] : 1u)) {
+                    best = w ? w[e] : 1u;
+                    h = col[e];
+                }
+            hn[i] = h;
+        }
+    }
+    for (vid_t i = 0; i < n; i++) {
+        vcmap[i] = ORC_INF;
+        list[i] = vperm[i];
+    }
+    {
+        vid_t len = n;
+        int swap = 1, idle = 0;
+        while (len > 0) {
+            const vid_t nc0 = nc;
+            rounds++;
+            for (vid_t k = 0; k < len; k++) {
+                const vid_t u = list[k], v = hn[u];
+                const int cond = pos[u] < pos[v];
+                if ((cond ^ swap) && u != v && vcmap[u] == ORC_INF && vcmap[v] == ORC_INF) {
+                    vcmap[u] = nc;
+                    vcmap[v] = nc;
+                    nc++;
+                }
+            }
+            vid_t nl = 0;
+            for (vid_t k = 0; k < len; k++) {
+                const vid_t u = list[k];
+                if (vcmap[u] != ORC_INF) continue;
+                vid_t h = ORC_INF;
+                uint64_t key = 0;
+                for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++) {
+                    const vid_t v = col[e];
+                    if (vcmap[v] != ORC_INF) continue;
+                    const uint64_t kv = level == 1 ? pos[v] : (w ? w[e] : 1u);
+                    if (key <= kv) {
+                        key = kv;
+                        h = v;
+                    }
+                }
+                if (h != ORC_INF) {
+                    next[nl++] = u;
+                    hn[u] = h;
+                }
+            }
+            idle = nc == nc0 ? idle + 1 : 0;
+            if (idle >= 3 && nl > 0) {
+                nc = ORC_INF;
+                goto done;
+            }
+            swap ^= 1;
+            len = nl;
+            vid_t *t = list;
+            list = next;
+            next = t;
+        }
+    }
+    for (vid_t i = 0; i < n; i++)
+        if (vcmap[i] == ORC_INF) vcmap[i] = nc++;
+done:
+    if (rounds_out) *rounds_out = rounds;
+    free(vperm);
+    free(pos);
+    free(hn);
+    free(list);
+    free(next);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
 /* MIS-2 coarsening (the mis_* binaries, -DMIS): mis_2, heuristics.h:8-100, */
 /* and the aggregation of sgp_coarsen_mis_2, :218-289, on a one-thread  */
 /* backend.  state: 1 in the distance-2 independent set, -1 excluded.   */
@@ -682,8 +795,8 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
     return orc_hierarchy_build_variant(n, rowptr, col, w, seed, 0);
 }
 
-/* The level loop with HEC (variant 0), HEC with the v3 map (3), MIS-2 (5)
-   or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+/* The level loop with HEC (variant 0), heavy-edge matching (1), HEC with
+   the v3 map (3), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
    (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
@@ -712,8 +825,14 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
         vid_t nc = variant == 3   ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 5 ? orc_coarsen_mis2(a->n, a->rowptr, a->col, seed, a->vcmap)
                    : variant == 6 ? orc_coarsen_gosh(a->n, a->rowptr, a->col, a->vcmap)
+                   : variant == 1 ? orc_coarsen_match(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                                   : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
                                                     NULL, NULL);
+        if (nc == ORC_INF) {  /* no map (matching on a row without entries): the levels so far */
+            free(a->vcmap);
+            a->vcmap = NULL;
+            break;
+        }
         c->n = nc;
         c->vtx_w = (wgt_t *)calloc((size_t)nc + 1, sizeof(wgt_t));
         c->m = orc_build_coarse_order(a->n, a->rowptr, a->col, a->w, a->vcmap, nc, a->vtx_w,
