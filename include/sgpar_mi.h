@@ -159,7 +159,14 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * unmatched neighbour of the largest position (level 1) or weight, the rest
  * singletons in index order -- a level with a vertex without entries, or
  * with only self-loops left, has no map in the reference and gives
- * MI_STATUS_ARG; variant 3 the map of the hec3_* binaries
+ * MI_STATUS_ARG; variant 2 the mtmetis_* binaries' (-DMTMETIS,
+ * cmg/CMakeLists.txt:46,54): that matching, then, each while more than a
+ * quarter of the vertices is unmapped, the leaf, twin and relative passes
+ * of heuristics.h:1737-1888 (leaves of a mapped vertex paired in row order;
+ * vertices with equal sums of their neighbours' MurmurHash3 digests paired
+ * along Kokkos::BinSort's order; unmapped neighbours of mapped vertices
+ * naming each other, with one matching round per naming); variant 3 the map
+ * of the hec3_* binaries
  * (compiled with -DHEC_V3, cmg/CMakeLists.txt:42,52): parallel_map_construct_v3
  * (heuristics.h:766-826) on the same heavy neighbours -- mutual heavy pairs
  * keyed by their smaller end, heavy targets keyed by themselves, every

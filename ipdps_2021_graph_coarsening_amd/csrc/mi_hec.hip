@@ -783,6 +783,252 @@ __global__ void k_match_singletons(uint32_t n, const uint32_t* __restrict__ rank
     if (i < n && vcmap[i] == INF) vcmap[i] = nc0 + rank[i];
 }
 
+__global__ void k_count_inf(uint32_t n, const uint32_t* __restrict__ vcmap, uint32_t* __restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const bool inf = i < n && vcmap[i] == INF;
+    const unsigned long long b = __ballot(inf);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && b) atomicAdd(cnt, (uint32_t)__popcll(b));
+}
+
+// ---- the mt-metis passes (the mtmetis_* binaries, -DMTMETIS;
+// heuristics.h:1737-1888).  Leaves: an unmapped vertex of one entry next to
+// a mapped vertex; the row's leaves pair in row order, (0,1), (2,3), ...
+__device__ __forceinline__ bool is_leaf(const uint32_t* rowptr, const uint32_t* vcmap, uint32_t v)
+{
+    return vcmap[v] == INF && rowptr[v + 1] - rowptr[v] == 1;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_leaf_count(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                      const uint32_t* __restrict__ col,
+                                                      const uint32_t* __restrict__ vcmap, uint32_t* __restrict__ pc)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t u = wid; u < n; u += nw) {
+        uint32_t cnt = 0;
+        if (vcmap[u] != INF)
+            for (uint32_t e0 = rowptr[u]; e0 < rowptr[u + 1]; e0 += WAVE) {
+                const uint32_t e = e0 + lane;
+                const bool lf = e < rowptr[u + 1] && is_leaf(rowptr, vcmap, col[e]);
+                cnt += (uint32_t)__popcll(__ballot(lf));
+            }
+        if (lane == 0) pc[u] = cnt / 2;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_leaf_pair(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col, uint32_t* vcmap,
+                                                     const uint32_t* __restrict__ base, uint32_t nc0)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long lt = lane ? (~0ull >> (WAVE - lane)) : 0ull;
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t u = wid; u < n; u += nw) {
+        if (vcmap[u] == INF || vcmap[u] >= nc0) continue;  // unmapped, or a leaf paired in this pass
+        const uint32_t r0 = rowptr[u], r1 = rowptr[u + 1];
+        // the row's leaves are decided before any of them is written
+        uint32_t total = 0;
+        for (uint32_t e0 = r0; e0 < r1; e0 += WAVE) {
+            const uint32_t e = e0 + lane;
+            total += (uint32_t)__popcll(__ballot(e < r1 && is_leaf(rowptr, vcmap, col[e])));
+        }
+        const uint32_t pairs = total / 2;
+        uint32_t seen = 0;
+        for (uint32_t e0 = r0; e0 < r1 && seen < 2 * pairs; e0 += WAVE) {
+            const uint32_t e = e0 + lane;
+            const bool lf = e < r1 && is_leaf(rowptr, vcmap, col[e]);
+            const unsigned long long b = __ballot(lf);
+            const uint32_t rk = seen + (uint32_t)__popcll(b & lt);
+            __builtin_amdgcn_wave_barrier();
+            if (lf && rk < 2 * pairs) vcmap[col[e]] = nc0 + base[u] + rk / 2;
+            seen += (uint32_t)__popcll(b);
+        }
+    }
+}
+
+// Twins: the unmapped vertices in index order, each with the sum (mod 2^32)
+// of its neighbours' MurmurHash3_x86_32 (seed 0, the 4 bytes of the index:
+// Kokkos' pod_hash), ordered as Kokkos::BinSort(BinOp1D(un, 0, 2^32-1))
+// orders them on one thread, then paired at even offsets of each run of
+// equal digests with the next one (MatchByHashSorted, heuristics.h:1168-1213).
+__global__ void k_twin_list(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ rank,
+                            uint32_t* __restrict__ vtx)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && vcmap[i] == INF) vtx[rank[i]] = i;
+}
+
+__device__ __forceinline__ uint32_t murmur3_u32(uint32_t key)
+{
+    uint32_t k = key * 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    uint32_t h = k;  // seed 0
+    h = (h << 13) | (h >> 19);
+    h = h * 5u + 0xe6546b64u;
+    h ^= 4u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_twin_digest(uint32_t un, const uint32_t* __restrict__ vtx,
+                                                       const uint32_t* __restrict__ rowptr,
+                                                       const uint32_t* __restrict__ col, uint32_t* __restrict__ hs)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t k = wid; k < un; k += nw) {
+        const uint32_t u = vtx[k];
+        uint32_t h = 0;
+        for (uint32_t e = rowptr[u] + lane; e < rowptr[u + 1]; e += WAVE) h += murmur3_u32(col[e]);
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) h += __shfl_xor(h, o, WAVE);
+        if (lane == 0) hs[k] = h;
+    }
+}
+
+// BinOp1D's bin, int(mul * key) with mul = 1.0 * un / (2^32 - 1)
+__global__ void k_twin_bins(uint32_t un, const uint32_t* __restrict__ hs, uint32_t* __restrict__ bin)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= un) return;
+    const double mul = 1.0 * (double)un / 4294967295.0;
+    bin[k] = (uint32_t)(int)(mul * (double)hs[k]);
+}
+
+// The in-bin bubble passes of Kokkos_Sort.hpp:376-400, one thread per bin
+// (the bins hold their items in index order after the stable sort by bin):
+// an item moves right past every item whose key is not larger (ties too).
+__global__ void k_twin_bubble(uint32_t un, const uint32_t* __restrict__ bin, const uint32_t* __restrict__ hs,
+                              uint32_t* __restrict__ ord)
+{
+    const uint32_t s0 = blockIdx.x * BLOCK + threadIdx.x;
+    if (s0 >= un || (s0 > 0 && bin[s0 - 1] == bin[s0])) return;
+    uint32_t e = s0 + 1;
+    while (e < un && bin[e] == bin[s0]) e++;
+    if (e - s0 <= 1) return;
+    uint32_t ub = e;
+    bool sorted = false;
+    while (!sorted) {
+        sorted = true;
+        uint32_t old = ord[s0];
+        for (uint32_t k = s0 + 1; k < ub; k++) {
+            const uint32_t nw = ord[k];
+            if (!(hs[old] < hs[nw])) {
+                ord[k - 1] = nw;
+                ord[k] = old;
+                sorted = false;
+            } else {
+                old = nw;
+            }
+        }
+        ub--;
+    }
+}
+
+__global__ void k_twin_heads(uint32_t un, const uint32_t* __restrict__ hs, const uint32_t* __restrict__ ord,
+                             uint32_t* __restrict__ head)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < un) head[i] = (i == 0 || hs[ord[i - 1]] != hs[ord[i]]) ? 1u : 0u;
+}
+
+// gid: exclusive count of heads; a run's index is gid at its head, gid - 1 after
+__global__ void k_twin_starts(uint32_t un, const uint32_t* __restrict__ head, const uint32_t* __restrict__ gid,
+                              uint32_t* __restrict__ gstart)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < un && head[i]) gstart[gid[i]] = i;
+}
+
+__global__ void k_twin_flags(uint32_t un, const uint32_t* __restrict__ hs, const uint32_t* __restrict__ ord,
+                             const uint32_t* __restrict__ head, const uint32_t* __restrict__ gid,
+                             const uint32_t* __restrict__ gstart, uint32_t* __restrict__ pf)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= un) return;
+    const uint32_t g = head[i] ? gid[i] : gid[i] - 1u;
+    pf[i] = (((i - gstart[g]) & 1u) == 0 && i + 1 < un && hs[ord[i]] == hs[ord[i + 1]]) ? 1u : 0u;
+}
+
+__global__ void k_twin_assign(uint32_t un, const uint32_t* __restrict__ vtx, const uint32_t* __restrict__ ord,
+                              const uint32_t* __restrict__ pf, const uint32_t* __restrict__ prank, uint32_t nc0,
+                              uint32_t* __restrict__ vcmap)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= un || !pf[i]) return;
+    vcmap[vtx[ord[i]]] = nc0 + prank[i];
+    vcmap[vtx[ord[i + 1]]] = nc0 + prank[i];
+}
+
+// Relatives: a mapped vertex's unmapped neighbours pair in row order,
+// (0,1), (2,3), ...; each names the other, and the last naming of a vertex
+// in (vertex, row) order -- the largest entry index of the pair's second
+// member -- wins (the one-thread overwrites of :1829-1842).
+__global__ void k_rel_reset(uint32_t len, const uint32_t* __restrict__ list, unsigned long long* __restrict__ evt)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < len) evt[list[k]] = 0ull;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rel_events(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                      const uint32_t* __restrict__ col,
+                                                      const uint32_t* __restrict__ vcmap,
+                                                      unsigned long long* __restrict__ evt)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long lt = lane ? (~0ull >> (WAVE - lane)) : 0ull;
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t i = wid; i < n; i += nw) {
+        if (vcmap[i] == INF) continue;
+        const uint32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+        uint32_t carry = INF;  // an unpaired free neighbour from the chunks before
+        uint32_t seen = 0;
+        for (uint32_t e0 = r0; e0 < r1; e0 += WAVE) {
+            const uint32_t e = e0 + lane;
+            const uint32_t v = e < r1 ? col[e] : INF;
+            const bool fr = v != INF && vcmap[v] == INF;
+            const unsigned long long b = __ballot(fr);
+            const uint32_t rk = seen + (uint32_t)__popcll(b & lt);
+            const unsigned long long below = b & lt;
+            const int pl = below ? 63 - __clzll((long long)below) : 0;  // lane of the free neighbour before
+            const uint32_t pv = __shfl(v, pl, WAVE);
+            if (fr && (rk & 1u)) {
+                const uint32_t partner = below ? pv : carry;
+                atomicMax(&evt[v], ((unsigned long long)e << 32) | partner);
+                atomicMax(&evt[partner], ((unsigned long long)e << 32) | v);
+            }
+            seen += (uint32_t)__popcll(b);
+            if (b) {
+                const int hl = 63 - __clzll((long long)b);  // the chunk's last free neighbour
+                const uint32_t lv = __shfl(v, hl, WAVE);
+                carry = (seen & 1u) ? lv : INF;
+            }
+        }
+    }
+}
+
+__global__ void k_rel_keep(uint32_t len, const uint32_t* __restrict__ list,
+                           const unsigned long long* __restrict__ evt, uint32_t* __restrict__ hn,
+                           uint32_t* __restrict__ keep)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len) return;
+    const uint32_t u = list[k];
+    const unsigned long long x = evt[u];
+    keep[k] = x ? 1u : 0u;
+    if (x) hn[u] = (uint32_t)x;
+}
+
 // ---- GOSH coarsening (GOSH_clusters, heuristics.h:102-217, and the
 // aggregation of sgp_coarsen_GOSH, :312-377; the gosh_* binaries).  A round
 // gives every undecided vertex the largest (state, entries, index) of itself
@@ -973,72 +1219,198 @@ static uint32_t mis2_map(Ctx& c, const DGraph& g, uint64_t seed, uint32_t* vcmap
     return h[0];
 }
 
-// The rounds of heavy-edge matching from the permutation's positions and the
-// first heavy neighbours (hn is consumed), then the singletons.
-static uint32_t match_rounds(Ctx& c, const DGraph& g, int level, const uint32_t* vperm, const uint32_t* pos,
-                             uint32_t* hn, uint32_t* vcmap)
+// Buffers of one matching round over a list of up to n slots.
+struct MatchBufs {
+    DBuf<uint32_t> first, flag, rank, cnt;
+    DBuf<uint8_t> est;
+    MatchBufs(Ctx& c, uint32_t n)
+        : first(&c.pool, n), flag(&c.pool, n), rank(&c.pool, n), cnt(&c.pool, 8), est(&c.pool, n)
+    {
+        fill_u32(c, first.get(), n, INF);
+    }
+};
+
+// One round in list order (the list's vertices unmatched when it starts):
+// the pairs, numbered from nc in list order; returns their number.
+static uint32_t match_round(Ctx& c, MatchBufs& B, uint32_t len, const uint32_t* list, const uint32_t* hn,
+                            const uint32_t* pos, int swap, uint32_t nc, uint32_t* vcmap)
+{
+    if (len == 0) return 0;
+    fill_u32(c, B.cnt.get(), 1, 0u);
+    k_match_active<<<grid1(len), BLOCK, 0, c.stream>>>(len, list, hn, pos, vcmap, swap, B.est.get(), B.cnt.get());
+    MI_HIP(hipGetLastError());
+    uint32_t und = 0;
+    read_words(c, B.cnt.get(), &und, 4);
+    for (uint32_t step = 0; und > 0; step++) {
+        MI_CHECK(step <= len, ST_INTERNAL, "matching steps did not end");
+        k_match_min<<<grid1(len), BLOCK, 0, c.stream>>>(len, list, hn, B.est.get(), B.first.get());
+        MI_HIP(hipGetLastError());
+        k_match_take<<<grid1(len), BLOCK, 0, c.stream>>>(len, list, hn, B.est.get(), B.first.get(), vcmap);
+        MI_HIP(hipGetLastError());
+        fill_u32(c, B.cnt.get(), 1, 0u);
+        k_match_drop<<<grid1(len), BLOCK, 0, c.stream>>>(len, list, hn, B.est.get(), B.first.get(), vcmap,
+                                                         B.cnt.get());
+        MI_HIP(hipGetLastError());
+        read_words(c, B.cnt.get(), &und, 4);
+    }
+    k_match_flag<<<grid1(len), BLOCK, 0, c.stream>>>(len, B.est.get(), B.flag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, B.flag.get(), B.rank.get(), len, B.cnt.get() + 1);
+    k_match_ids<<<grid1(len), BLOCK, 0, c.stream>>>(len, list, hn, B.est.get(), B.rank.get(), nc, vcmap);
+    MI_HIP(hipGetLastError());
+    uint32_t pairs = 0;
+    read_words(c, B.cnt.get() + 1, &pairs, 4);
+    return pairs;
+}
+
+static uint32_t count_unmapped(Ctx& c, MatchBufs& B, uint32_t n, const uint32_t* vcmap)
+{
+    fill_u32(c, B.cnt.get() + 4, 1, 0u);
+    k_count_inf<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.cnt.get() + 4);
+    MI_HIP(hipGetLastError());
+    uint32_t un = 0;
+    read_words(c, B.cnt.get() + 4, &un, 4);
+    return un;
+}
+
+static int nbits(uint32_t x)
+{
+    int b = 1;
+    while (b < 32 && (1ull << b) <= x) b++;
+    return b;
+}
+
+// The mt-metis passes (heuristics.h:1737-1888), each while more than a
+// quarter of the vertices is unmapped; returns the new nc.
+static uint32_t mtmetis_passes(Ctx& c, MatchBufs& B, const DGraph& g, const uint32_t* pos, uint32_t* hn,
+                               uint32_t* vcmap, uint32_t nc, int swap)
 {
     const uint32_t n = g.n;
-    DBuf<uint32_t> la(&c.pool, n), lb(&c.pool, n), first(&c.pool, n), flag(&c.pool, n), rank(&c.pool, n),
-        cnt(&c.pool, 4);
-    DBuf<uint8_t> est(&c.pool, n);
+    const unsigned wgn = (unsigned)std::min<uint64_t>(ceil_div(n, BLOCK / WAVE), 65536);
+    auto over = [n](uint32_t un) { return (double)un / (double)n > 0.25; };
+    uint32_t un = count_unmapped(c, B, n, vcmap);
+    // leaves of the mapped vertices, paired in row order; ids in (vertex, row) order
+    if (over(un)) {
+        DBuf<uint32_t> pc(&c.pool, n), base(&c.pool, n);
+        k_leaf_count<<<wgn, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), vcmap, pc.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, pc.get(), base.get(), n, B.cnt.get() + 5);
+        k_leaf_pair<<<wgn, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), vcmap, base.get(), nc);
+        MI_HIP(hipGetLastError());
+        uint32_t p = 0;
+        read_words(c, B.cnt.get() + 5, &p, 4);
+        nc += p;
+        un = count_unmapped(c, B, n, vcmap);
+    }
+    // twins: equal digests, paired along the BinSort order
+    if (over(un)) {
+        DBuf<uint32_t> vtx(&c.pool, un), hs(&c.pool, un), bin(&c.pool, un), ord(&c.pool, un), bt(&c.pool, un),
+            ot(&c.pool, un), head(&c.pool, un), gid(&c.pool, un), gstart(&c.pool, un), pf(&c.pool, un),
+            prank(&c.pool, un);
+        k_match_unmatched<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.flag.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, B.flag.get(), B.rank.get(), n, nullptr);
+        k_twin_list<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.rank.get(), vtx.get());
+        MI_HIP(hipGetLastError());
+        k_twin_digest<<<(unsigned)std::min<uint64_t>(ceil_div(un, BLOCK / WAVE), 65536), BLOCK, 0, c.stream>>>(
+            un, vtx.get(), g.R(), g.Cc(), hs.get());
+        MI_HIP(hipGetLastError());
+        k_twin_bins<<<grid1(un), BLOCK, 0, c.stream>>>(un, hs.get(), bin.get());
+        MI_HIP(hipGetLastError());
+        iota_u32(c, ord.get(), un, 0);
+        radix_sort_pairs(c, bin.get(), ord.get(), un, 0, nbits(un), bt.get(), ot.get());
+        k_twin_bubble<<<grid1(un), BLOCK, 0, c.stream>>>(un, bin.get(), hs.get(), ord.get());
+        MI_HIP(hipGetLastError());
+        k_twin_heads<<<grid1(un), BLOCK, 0, c.stream>>>(un, hs.get(), ord.get(), head.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, head.get(), gid.get(), un, nullptr);
+        k_twin_starts<<<grid1(un), BLOCK, 0, c.stream>>>(un, head.get(), gid.get(), gstart.get());
+        MI_HIP(hipGetLastError());
+        k_twin_flags<<<grid1(un), BLOCK, 0, c.stream>>>(un, hs.get(), ord.get(), head.get(), gid.get(), gstart.get(),
+                                                        pf.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, pf.get(), prank.get(), un, B.cnt.get() + 5);
+        k_twin_assign<<<grid1(un), BLOCK, 0, c.stream>>>(un, vtx.get(), ord.get(), pf.get(), prank.get(), nc, vcmap);
+        MI_HIP(hipGetLastError());
+        uint32_t p = 0;
+        read_words(c, B.cnt.get() + 5, &p, 4);
+        nc += p;
+        un = count_unmapped(c, B, n, vcmap);
+    }
+    // relatives: named by the last pairing event, one round per naming
+    if (over(un)) {
+        DBuf<uint32_t> la(&c.pool, un), lb(&c.pool, un);
+        DBuf<unsigned long long> evt(&c.pool, n);
+        MI_HIP(hipMemsetAsync(evt.get(), 0, (size_t)n * 8, c.stream));
+        k_match_unmatched<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.flag.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, B.flag.get(), B.rank.get(), n, nullptr);
+        k_twin_list<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.rank.get(), la.get());
+        MI_HIP(hipGetLastError());
+        uint32_t len = un;
+        for (uint32_t it = 0; len > 0; it++) {
+            MI_CHECK(it <= 4ull * n + 8, ST_ARG, "mt-metis relatives did not end (a row naming a vertex twice)");
+            k_rel_reset<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), evt.get());
+            MI_HIP(hipGetLastError());
+            k_rel_events<<<wgn, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), vcmap, evt.get());
+            MI_HIP(hipGetLastError());
+            k_rel_keep<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), evt.get(), hn, B.flag.get());
+            MI_HIP(hipGetLastError());
+            scan_exclusive_u32(c, B.flag.get(), B.rank.get(), len, B.cnt.get() + 6);
+            k_match_compact<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), B.flag.get(), B.rank.get(), lb.get());
+            MI_HIP(hipGetLastError());
+            read_words(c, B.cnt.get() + 6, &len, 4);
+            la.swap(lb);
+            nc += match_round(c, B, len, la.get(), hn, pos, swap, nc, vcmap);
+            swap ^= 1;
+        }
+    }
+    return nc;
+}
+
+// The rounds of heavy-edge matching from the permutation's positions and the
+// first heavy neighbours (hn is consumed), the mt-metis passes when asked,
+// then the singletons.
+static uint32_t match_rounds(Ctx& c, const DGraph& g, int level, const uint32_t* vperm, const uint32_t* pos,
+                             uint32_t* hn, uint32_t* vcmap, bool mtmetis)
+{
+    const uint32_t n = g.n;
+    DBuf<uint32_t> la(&c.pool, n), lb(&c.pool, n);
+    MatchBufs B(c, n);
     MI_HIP(hipMemcpyAsync(la.get(), vperm, (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
     fill_u32(c, vcmap, n, INF);
-    fill_u32(c, first.get(), n, INF);
     uint32_t len = n, nc = 0;
     int swap = 1, idle = 0;
     for (uint32_t round = 0; len > 0; round++) {
         MI_CHECK(round <= 2ull * n + 4, ST_INTERNAL, "matching rounds did not end");
-        fill_u32(c, cnt.get(), 1, 0u);
-        k_match_active<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, pos, vcmap, swap, est.get(),
-                                                           cnt.get());
-        MI_HIP(hipGetLastError());
-        uint32_t und = 0;
-        read_words(c, cnt.get(), &und, 4);
-        for (uint32_t step = 0; und > 0; step++) {
-            MI_CHECK(step <= len, ST_INTERNAL, "matching steps did not end");
-            k_match_min<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get());
-            MI_HIP(hipGetLastError());
-            k_match_take<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get(), vcmap);
-            MI_HIP(hipGetLastError());
-            fill_u32(c, cnt.get(), 1, 0u);
-            k_match_drop<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), first.get(), vcmap,
-                                                             cnt.get());
-            MI_HIP(hipGetLastError());
-            read_words(c, cnt.get(), &und, 4);
-        }
-        // pair ids in list order
-        k_match_flag<<<grid1(len), BLOCK, 0, c.stream>>>(len, est.get(), flag.get());
-        MI_HIP(hipGetLastError());
-        scan_exclusive_u32(c, flag.get(), rank.get(), len, cnt.get() + 1);
-        k_match_ids<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn, est.get(), rank.get(), nc, vcmap);
-        MI_HIP(hipGetLastError());
+        const uint32_t pairs = match_round(c, B, len, la.get(), hn, pos, swap, nc, vcmap);
+        nc += pairs;
         // the next list: unmatched vertices with an unmatched neighbour, in order
         k_match_repick<<<(unsigned)std::min<uint64_t>(ceil_div(len, BLOCK / WAVE), 65536), BLOCK, 0, c.stream>>>(
-            len, la.get(), g.R(), g.Cc(), g.W(), pos, level, vcmap, hn, flag.get());
+            len, la.get(), g.R(), g.Cc(), g.W(), pos, level, vcmap, hn, B.flag.get());
         MI_HIP(hipGetLastError());
-        scan_exclusive_u32(c, flag.get(), rank.get(), len, cnt.get() + 2);
-        k_match_compact<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), flag.get(), rank.get(), lb.get());
+        scan_exclusive_u32(c, B.flag.get(), B.rank.get(), len, B.cnt.get() + 2);
+        k_match_compact<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), B.flag.get(), B.rank.get(), lb.get());
         MI_HIP(hipGetLastError());
-        uint32_t h[2] = {0, 0};
-        read_words(c, cnt.get() + 1, h, 8);
-        nc += h[0];
-        idle = h[0] ? 0 : idle + 1;
+        uint32_t nl = 0;
+        read_words(c, B.cnt.get() + 2, &nl, 4);
+        idle = pairs ? 0 : idle + 1;
         // three rounds in a row without a pair: the last two held the same
         // edges under both orderings, so only self-loops are left (the
         // reference loops for ever)
-        MI_CHECK(!(idle >= 3 && h[1] > 0), ST_ARG, "matching: only self-loops left");
-        len = h[1];
+        MI_CHECK(!(idle >= 3 && nl > 0), ST_ARG, "matching: only self-loops left");
+        len = nl;
         swap ^= 1;
         la.swap(lb);
     }
-    k_match_unmatched<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, flag.get());
+    if (mtmetis) nc = mtmetis_passes(c, B, g, pos, hn, vcmap, nc, swap);
+    k_match_unmatched<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, B.flag.get());
     MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, flag.get(), rank.get(), n, cnt.get() + 3);
-    k_match_singletons<<<grid1(n), BLOCK, 0, c.stream>>>(n, rank.get(), nc, vcmap);
+    scan_exclusive_u32(c, B.flag.get(), B.rank.get(), n, B.cnt.get() + 3);
+    k_match_singletons<<<grid1(n), BLOCK, 0, c.stream>>>(n, B.rank.get(), nc, vcmap);
     MI_HIP(hipGetLastError());
     uint32_t ns = 0;
-    read_words(c, cnt.get() + 3, &ns, 4);
+    read_words(c, B.cnt.get() + 3, &ns, 4);
     return nc + ns;
 }
 
@@ -1111,7 +1483,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
     if (t) MI_HIP(hipEventRecord(ev[1], c.stream));
 
     // ---- heavy neighbour (matching: the first heaviest; no row may be empty)
-    const bool matching = c.coarsen == 1;
+    const bool matching = c.coarsen == 1 || c.coarsen == 2;
     if (matching) {
         DBuf<uint32_t> e(&c.pool, 1);
         fill_u32(c, e.get(), 1, 0u);
@@ -1148,7 +1520,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
         if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
         if (vperm_out)
             MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
-        const uint32_t nc = match_rounds(c, g, level, vperm.get(), pos.get(), hn.get(), vcmap);
+        const uint32_t nc = match_rounds(c, g, level, vperm.get(), pos.get(), hn.get(), vcmap, c.coarsen == 2);
         if (t) {
             MI_HIP(hipEventRecord(ev[3], c.stream));
             float a = 0, b = 0, d = 0;

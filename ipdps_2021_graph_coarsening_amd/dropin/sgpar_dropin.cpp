@@ -36,6 +36,7 @@
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
 // v3 map, with -DPUREMATCH (the match_* binaries) with heavy-edge matching,
+// with -DMTMETIS (the mtmetis_* binaries) with that matching and its passes,
 // with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
 // gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening); with -DHASHMAP
 // (hec_spec_hashmap) its coarse rows keep the HASHMAP build's order
@@ -76,11 +77,13 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#if defined HEC_V3 || defined MIS || defined GOSH || defined PUREMATCH
+#if defined HEC_V3 || defined MIS || defined GOSH || defined PUREMATCH || defined MTMETIS
 #if defined MIS
         const int variant = 5;
 #elif defined PUREMATCH
         const int variant = 1;
+#elif defined MTMETIS
+        const int variant = 2;
 #elif defined GOSH
         const int variant = 6;
 #else

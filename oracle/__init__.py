@@ -105,6 +105,12 @@ class _Core:
         L.orc_coarsen_match.restype = C.c_uint32
         L.orc_coarsen_match.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p,
                                         C.POINTER(C.c_int)]
+        L.orc_coarsen_mtmetis.restype = C.c_uint32
+        L.orc_coarsen_mtmetis.argtypes = L.orc_coarsen_match.argtypes
+        L.orc_murmur3_x86_32.restype = C.c_uint32
+        L.orc_murmur3_x86_32.argtypes = [C.c_char_p, C.c_int, C.c_uint32]
+        L.orc_binsort_u32.restype = None
+        L.orc_binsort_u32.argtypes = [C.c_uint32, u32p, u32p]
         L.orc_gosh_clusters.restype = C.c_int
         L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_gosh.restype = C.c_uint32
@@ -394,6 +400,34 @@ def coarsen_match(level: int, rowptr, col, w, seed: int):
     return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
 
 
+def coarsen_mtmetis(level: int, rowptr, col, w, seed: int):
+    """(vcmap, nc, rounds) of one level of the mtmetis_* binaries (-DMTMETIS):
+    the matching, then the leaf, twin and relative passes
+    (heuristics.h:1737-1888); nc = None when the reference has no result."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    r = C.c_int(0)
+    nc = L.orc_coarsen_mtmetis(level, n, rowptr, col, wp, seed, vc, C.byref(r))
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
+
+
+def murmur3_x86_32(data: bytes, seed: int = 0) -> int:
+    """MurmurHash3_x86_32 of a byte string."""
+    return int(core().L.orc_murmur3_x86_32(data, len(data), seed))
+
+
+def binsort_u32(keys):
+    """The order Kokkos::BinSort(BinOp1D(n, 0, 2^32-1)) with in-bin bubble
+    passes gives u32 keys on one thread."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    out = np.empty(len(keys) + 1, dtype=np.uint32)
+    core().L.orc_binsort_u32(len(keys), keys, out)
+    return out[:len(keys)].copy()
+
+
 def gosh_clusters(rowptr, col):
     """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
     (heuristics.h:102-217)."""
@@ -430,7 +464,8 @@ def coarsen_hec3(level: int, rowptr, col, w, seed: int):
 def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int = 1,
                     normLap: int = 0, tol: float = 1e-10, variant: int = 0, builder: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
-    variant: 0 HEC, 1 heavy-edge matching (match_*), 3 HEC with the v3 map (the
+    variant: 0 HEC, 1 heavy-edge matching (match_*), 2 matching with the
+    mt-metis passes (mtmetis_*), 3 HEC with the v3 map (the
     hec3_* binaries), 5 MIS-2 (mis_*), 6 GOSH (gosh_*); builder 1: the HASHMAP
     build (hec_spec_hashmap)."""
     L = core().L

@@ -368,8 +368,221 @@ vid_t orc_coarsen_hec3(int level, vid_t n, const eid_t *rowptr, const vid_t *col
 /* nothing (the last two then hold the same edges under both orderings,  */
 /* so only self-loops are left and the reference loops for ever).       */
 /* ------------------------------------------------------------------ */
+/* MurmurHash3_x86_32 (Appleby's published definition, which Kokkos'
+   pod_hash uses with seed 0 for the twin digests: definitions_kokkos.h:153,
+   Kokkos_Functional.hpp:62). */
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+uint32_t orc_murmur3_x86_32(const uint8_t *data, int len, uint32_t seed)
+{
+    const uint32_t c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    uint32_t h = seed;
+    const int nb = len / 4;
+    for (int i = 0; i < nb; i++) {
+        uint32_t k = (uint32_t)data[4 * i] | ((uint32_t)data[4 * i + 1] << 8) | ((uint32_t)data[4 * i + 2] << 16) |
+                     ((uint32_t)data[4 * i + 3] << 24);
+        k *= c1;
+        k = rotl32(k, 15);
+        k *= c2;
+        h ^= k;
+        h = rotl32(h, 13);
+        h = h * 5u + 0xe6546b64u;
+    }
+    uint32_t k = 0;
+    const uint8_t *t = data + 4 * nb;
+    switch (len & 3) {
+    case 3: k ^= (uint32_t)t[2] << 16; /* fall through */
+    case 2: k ^= (uint32_t)t[1] << 8;  /* fall through */
+    case 1:
+        k ^= t[0];
+        k *= c1;
+        k = rotl32(k, 15);
+        k *= c2;
+        h ^= k;
+    }
+    h ^= (uint32_t)len;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* Kokkos::BinSort with BinOp1D(nbins = n, 0, UINT32_MAX) over u32 keys on
+   one thread (Kokkos_Sort.hpp:366-400, 404-440): bins by int(mul * key)
+   filled in index order, then the in-bin bubble passes, which move an item
+   past every item not larger than itself (ties too), until a pass swaps
+   nothing.  order[k] = the index placed k-th. */
+void orc_binsort_u32(vid_t n, const uint32_t *keys, vid_t *order)
+{
+    const int max_bins = (int)n + 1;
+    const double mul = 1.0 * (int)n / (double)(uint32_t)(UINT32_MAX - 0u);
+    uint32_t *count = (uint32_t *)calloc((size_t)max_bins, sizeof(uint32_t));
+    uint32_t *offs = (uint32_t *)calloc((size_t)max_bins, sizeof(uint32_t));
+    for (vid_t i = 0; i < n; i++) count[(int)(mul * (double)keys[i])]++;
+    uint32_t run = 0;
+    for (int b = 0; b < max_bins; b++) {
+        offs[b] = run;
+        run += count[b];
+    }
+    memset(count, 0, (size_t)max_bins * sizeof(uint32_t));
+    for (vid_t i = 0; i < n; i++) {
+        const int b = (int)(mul * (double)keys[i]);
+        order[offs[b] + count[b]++] = i;
+    }
+    for (int b = 0; b < max_bins; b++) {
+        if (count[b] <= 1) continue;
+        uint32_t ub = offs[b] + count[b];
+        int sorted = 0;
+        while (!sorted) {
+            sorted = 1;
+            vid_t old = order[offs[b]];
+            for (uint32_t k = offs[b] + 1; k < ub; k++) {
+                const vid_t nw = order[k];
+                if (!(keys[old] < keys[nw])) {
+                    order[k - 1] = nw;
+                    order[k] = old;
+                    sorted = 0;
+                } else {
+                    old = nw;
+                }
+            }
+            ub--;
+        }
+    }
+    free(count);
+    free(offs);
+}
+
+static vid_t count_inf(vid_t n, const vid_t *vcmap)
+{
+    vid_t c = 0;
+    for (vid_t i = 0; i < n; i++) c += vcmap[i] == ORC_INF;
+    return c;
+}
+
+/* The mt-metis binaries' passes after the matching rounds
+   (heuristics.h:1737-1888), each only while more than a quarter of the
+   vertices is unmapped: leaves (degree 1) of every mapped vertex paired in
+   row order; twins (equal sums of the neighbours' hashes) paired along the
+   BinSort order; then relatives -- two unmapped neighbours of a mapped
+   vertex, consecutive in its row, name each other (the last such pair in
+   (vertex, row) order wins), and one matching round over the vertices so
+   named, with the ordering condition still alternating -- until no vertex
+   is named.  Returns ORC_INF if the relatives never end (a row naming one
+   vertex twice pairs it with itself). */
+static vid_t mtmetis_passes(vid_t n, const eid_t *rowptr, const vid_t *col, const vid_t *pos, vid_t *hn,
+                            vid_t *vcmap, vid_t nc, int swap)
+{
+    if ((double)count_inf(n, vcmap) / (double)n > 0.25) {
+        for (vid_t u = 0; u < n; u++) {
+            if (vcmap[u] == ORC_INF) continue;
+            vid_t last = ORC_INF;
+            for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++) {
+                const vid_t v = col[e];
+                if (vcmap[v] != ORC_INF || rowptr[v + 1] - rowptr[v] != 1) continue;
+                if (last == ORC_INF) {
+                    last = v;
+                } else {
+                    vcmap[last] = nc;
+                    vcmap[v] = nc++;
+                    last = ORC_INF;
+                }
+            }
+        }
+    }
+    vid_t un = count_inf(n, vcmap);
+    if ((double)un / (double)n > 0.25) {
+        vid_t *vtx = (vid_t *)malloc((size_t)un * sizeof(vid_t) + 4);
+        uint32_t *hs = (uint32_t *)malloc((size_t)un * sizeof(uint32_t) + 4);
+        vid_t *order = (vid_t *)malloc((size_t)un * sizeof(vid_t) + 4);
+        vid_t k = 0;
+        for (vid_t u = 0; u < n; u++) {
+            if (vcmap[u] != ORC_INF) continue;
+            uint32_t h = 0;
+            for (eid_t e = rowptr[u]; e < rowptr[u + 1]; e++) h += orc_murmur3_x86_32((const uint8_t *)&col[e], 4, 0);
+            vtx[k] = u;
+            hs[k++] = h;
+        }
+        orc_binsort_u32(un, hs, order);
+        vid_t start = 0;
+        for (vid_t i = 0; i < un; i++) {
+            if (i == 0 || hs[order[i - 1]] != hs[order[i]]) start = i;
+            if (((i - start) & 1u) == 0 && i + 1 < un && hs[order[i]] == hs[order[i + 1]]) {
+                vcmap[vtx[order[i]]] = nc;
+                vcmap[vtx[order[i + 1]]] = nc++;
+            }
+        }
+        free(vtx);
+        free(hs);
+        free(order);
+    }
+    un = count_inf(n, vcmap);
+    if ((double)un / (double)n > 0.25) {
+        vid_t *list = (vid_t *)malloc((size_t)un * sizeof(vid_t) + 4);
+        vid_t len = 0;
+        for (vid_t u = 0; u < n; u++)
+            if (vcmap[u] == ORC_INF) list[len++] = u;
+        for (int it = 0; len > 0; it++) {
+            if (it > 4 * (int)n + 8) {
+                free(list);
+                return ORC_INF;
+            }
+            for (vid_t k = 0; k < len; k++) hn[list[k]] = ORC_INF;
+            for (vid_t i = 0; i < n; i++) {
+                if (vcmap[i] == ORC_INF) continue;
+                vid_t last = ORC_INF;
+                for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+                    const vid_t v = col[e];
+                    if (vcmap[v] != ORC_INF) continue;
+                    if (last != ORC_INF) {
+                        hn[last] = v;
+                        hn[v] = last;
+                        last = ORC_INF;
+                    } else {
+                        last = v;
+                    }
+                }
+            }
+            vid_t nl = 0;
+            for (vid_t k = 0; k < len; k++)
+                if (hn[list[k]] != ORC_INF) list[nl++] = list[k];
+            len = nl;
+            for (vid_t k = 0; k < len; k++) {
+                const vid_t u = list[k], v = hn[u];
+                const int cond = pos[u] < pos[v];
+                if ((cond ^ swap) && u != v && vcmap[u] == ORC_INF && vcmap[v] == ORC_INF) {
+                    vcmap[u] = nc;
+                    vcmap[v] = nc++;
+                }
+            }
+            swap ^= 1;
+        }
+        free(list);
+    }
+    return nc;
+}
+
+static vid_t match_level(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                         uint64_t seed, vid_t *vcmap, int *rounds_out, int mtmetis);
+
 vid_t orc_coarsen_match(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
                         uint64_t seed, vid_t *vcmap, int *rounds_out)
+{
+    return match_level(level, n, rowptr, col, w, seed, vcmap, rounds_out, 0);
+}
+
+/* The mtmetis_* binaries (-DMTMETIS): the matching rounds, then the leaf,
+   twin and relative passes, then singletons. */
+vid_t orc_coarsen_mtmetis(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                          uint64_t seed, vid_t *vcmap, int *rounds_out)
+{
+    return match_level(level, n, rowptr, col, w, seed, vcmap, rounds_out, 1);
+}
+
+static vid_t match_level(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                         uint64_t seed, vid_t *vcmap, int *rounds_out, int mtmetis)
 {
     uint64_t st = orc_pool_state(seed);
     vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
@@ -449,6 +662,10 @@ vid_t orc_coarsen_match(int level, vid_t n, const eid_t *rowptr, const vid_t *co
             vid_t *t = list;
             list = next;
             next = t;
+        }
+        if (mtmetis) {
+            nc = mtmetis_passes(n, rowptr, col, pos, hn, vcmap, nc, swap);
+            if (nc == ORC_INF) goto done;
         }
     }
     for (vid_t i = 0; i < n; i++)
@@ -795,8 +1012,8 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
     return orc_hierarchy_build_variant(n, rowptr, col, w, seed, 0);
 }
 
-/* The level loop with HEC (variant 0), heavy-edge matching (1), HEC with
-   the v3 map (3), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+/* The level loop with HEC (variant 0), heavy-edge matching (1), matching
+   with the mt-metis passes (2), HEC with the v3 map (3), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
    (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
@@ -826,6 +1043,7 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
                    : variant == 5 ? orc_coarsen_mis2(a->n, a->rowptr, a->col, seed, a->vcmap)
                    : variant == 6 ? orc_coarsen_gosh(a->n, a->rowptr, a->col, a->vcmap)
                    : variant == 1 ? orc_coarsen_match(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
+                   : variant == 2 ? orc_coarsen_mtmetis(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                                   : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
                                                     NULL, NULL);
         if (nc == ORC_INF) {  /* no map (matching on a row without entries): the levels so far */
