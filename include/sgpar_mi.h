@@ -165,8 +165,15 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * of heuristics.h:1737-1888 (leaves of a mapped vertex paired in row order;
  * vertices with equal sums of their neighbours' MurmurHash3 digests paired
  * along Kokkos::BinSort's order; unmapped neighbours of mapped vertices
- * naming each other, with one matching round per naming); variant 3 the map
- * of the hec3_* binaries
+ * naming each other, with one matching round per naming); variant 4 the map
+ * of the hec2_* binaries (-DHEC_V2, cmg/CMakeLists.txt:41,51):
+ * parallel_map_construct_v2 (heuristics.h:695-764) on HEC's heavy
+ * neighbours as one thread runs it -- rounds in which the first list vertex
+ * naming each heavy-neighbour slot claims it, slots take pseudo-locks in
+ * order and pair their vertex with its heavy neighbour (a later pair may
+ * re-map a vertex, which can leave ids without members, as there), and
+ * unmapped vertices join their heavy neighbour's id; a round that maps
+ * nothing gives MI_STATUS_ARG; variant 3 the map of the hec3_* binaries
  * (compiled with -DHEC_V3, cmg/CMakeLists.txt:42,52): parallel_map_construct_v3
  * (heuristics.h:766-826) on the same heavy neighbours -- mutual heavy pairs
  * keyed by their smaller end, heavy targets keyed by themselves, every

@@ -233,7 +233,8 @@ class Context:
     def set_coarsening(self, variant: int) -> None:
         """The coarsening of the hierarchies: 0 the hec_* binaries' (default),
         1 the match_* binaries' (heavy-edge matching), 2 the mtmetis_* binaries'
-        (that matching plus the leaf, twin and relative passes),
+        (that matching plus the leaf, twin and relative passes), 4 the hec2_*
+        binaries' (parallel_map_construct_v2),
         3 the hec3_* binaries' (parallel_map_construct_v3), 5 the mis_*
         binaries' (MIS-2 aggregates), 6 the gosh_* binaries' (GOSH centres)."""
         self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")

@@ -480,7 +480,7 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
 
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant)
 {
-    if (!ctx || variant < 0 || variant > 6 || variant == 4) return ST_ARG;
+    if (!ctx || variant < 0 || variant > 6) return ST_ARG;
     ctx->c.coarsen = variant;
     return ST_OK;
 }

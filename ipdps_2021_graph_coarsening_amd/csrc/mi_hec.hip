@@ -452,6 +452,154 @@ __global__ void k_v3_root(uint32_t n, const uint32_t* __restrict__ m, uint32_t* 
     used[p] = 1u;
 }
 
+// ---- the v2 map (parallel_map_construct_v2, heuristics.h:695-764; the
+// hec2_* binaries) as one thread runs it, round by round.  Slot v (a
+// position) holds the first list vertex u whose heavy neighbour sits at v.
+// Slot v then touches the pseudo-locks min(u, v) and max(u, v) (a vertex id
+// against a position, as written), the second only if it took the first.
+// A lock L is touched by two slots at most -- slot L and the slot whose
+// vertex is L -- so "slot v took its first lock" (no earlier slot touched
+// it) resolves in a few passes over decided neighbours, and v pairs u with
+// hn(u) when it took both and they differ.
+__global__ void k_v2_claim(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                           const uint32_t* __restrict__ pos, uint32_t* __restrict__ claim)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < len) atomicMin(&claim[pos[hn[list[k]]]], k);
+}
+
+__global__ void k_v2_samples(uint32_t n, const uint32_t* __restrict__ claim, const uint32_t* __restrict__ list,
+                             uint32_t* __restrict__ smp, uint32_t* __restrict__ slot_of, uint8_t* __restrict__ st)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t k = claim[v];
+    const uint32_t u = k != INF ? list[k] : INF;
+    smp[v] = u;
+    st[v] = u != INF ? 0 : 3;  // 0 undecided, 1 took its first lock, 2 did not, 3 no sample
+    if (u != INF) slot_of[u] = v;
+}
+
+// does slot t (t < v) touch lock L before v does?  1 yes, 0 no, 2 not known yet
+__device__ __forceinline__ int v2_touches(uint32_t t, uint32_t L, const uint32_t* smp, const uint8_t* st)
+{
+    const uint32_t u = smp[t];
+    const uint32_t a = u < t ? u : t, b = u < t ? t : u;
+    if (a == L) return 1;
+    if (b != L) return 0;
+    const uint8_t s = st[t];
+    return s == 1 ? 1 : s == 2 ? 0 : 2;
+}
+
+// 0: no earlier slot touches L, 1: one does, 2: not known yet
+__device__ __forceinline__ int v2_earlier(uint32_t v, uint32_t L, const uint32_t* smp, const uint32_t* slot_of,
+                                          const uint8_t* st)
+{
+    int r = 0;
+    const uint32_t t1 = L, t2 = slot_of[L];
+    if (t1 < v && smp[t1] != INF) {
+        const int x = v2_touches(t1, L, smp, st);
+        if (x == 1) return 1;
+        if (x == 2) r = 2;
+    }
+    if (t2 != INF && t2 != t1 && t2 < v) {
+        const int x = v2_touches(t2, L, smp, st);
+        if (x == 1) return 1;
+        if (x == 2) r = 2;
+    }
+    return r;
+}
+
+__global__ void k_v2_first(uint32_t n, const uint32_t* __restrict__ smp, const uint32_t* __restrict__ slot_of,
+                           uint8_t* st, uint32_t* __restrict__ und)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n || st[v] != 0) return;
+    const uint32_t u = smp[v];
+    const int e = v2_earlier(v, u < v ? u : v, smp, slot_of, st);
+    if (e == 2) {
+        atomicAdd(und, 1u);
+        return;
+    }
+    st[v] = e ? 2 : 1;
+}
+
+__global__ void k_v2_ok(uint32_t n, const uint32_t* __restrict__ smp, const uint32_t* __restrict__ slot_of,
+                        const uint8_t* __restrict__ st, uint32_t* __restrict__ ok)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n) return;
+    bool good = false;
+    if (st[v] == 1) {
+        const uint32_t u = smp[v];
+        const uint32_t a = u < v ? u : v, b = u < v ? v : u;
+        good = a != b && v2_earlier(v, b, smp, slot_of, st) == 0;
+    }
+    ok[v] = good ? 1u : 0u;
+}
+
+// a vertex written by several pairs keeps the last: the largest id (+1)
+__global__ void k_v2_write(uint32_t n, const uint32_t* __restrict__ smp, const uint32_t* __restrict__ vperm,
+                           const uint32_t* __restrict__ ok, const uint32_t* __restrict__ rank, uint32_t nc0,
+                           uint32_t* __restrict__ wr)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n || !ok[v]) return;
+    const uint32_t c = nc0 + rank[v] + 1u;
+    atomicMax(&wr[smp[v]], c);
+    atomicMax(&wr[vperm[v]], c);
+}
+
+__global__ void k_v2_apply(uint32_t n, const uint32_t* __restrict__ wr, uint32_t* __restrict__ vcmap)
+{
+    const uint32_t x = blockIdx.x * BLOCK + threadIdx.x;
+    if (x < n && wr[x]) vcmap[x] = wr[x] - 1u;
+}
+
+__global__ void k_v2_lpos(uint32_t len, const uint32_t* __restrict__ list, uint32_t* __restrict__ lpos)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k < len) lpos[list[k]] = k;
+}
+
+// :731-743 -- an unmapped list vertex joins hn(u)'s id; a heavy neighbour
+// that is an unmapped list vertex earlier in the walk has joined first, so
+// the chase follows hn through earlier list positions
+__global__ void k_v2_join(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hn,
+                          const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ vcmap,
+                          uint32_t* __restrict__ jv)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len) return;
+    const uint32_t u = list[k];
+    uint32_t val = INF;
+    if (vcmap[u] == INF) {
+        uint32_t x = u, px = k;
+        for (uint32_t step = 0; step <= len; step++) {
+            const uint32_t t = hn[x];
+            if (vcmap[t] != INF) {
+                val = vcmap[t];
+                break;
+            }
+            const uint32_t pt = lpos[t];
+            if (pt == INF || pt >= px) break;
+            x = t;
+            px = pt;
+        }
+    }
+    jv[k] = val;
+}
+
+__global__ void k_v2_join_apply(uint32_t len, const uint32_t* __restrict__ list, const uint32_t* __restrict__ jv,
+                                uint32_t* __restrict__ vcmap, uint32_t* __restrict__ keep)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= len) return;
+    const uint32_t u = list[k];
+    if (jv[k] != INF) vcmap[u] = jv[k];
+    keep[k] = vcmap[u] == INF ? 1u : 0u;
+}
+
 __global__ void k_v3_assign(uint32_t n, const uint32_t* __restrict__ root, const uint32_t* __restrict__ id,
                             uint32_t* __restrict__ vcmap)
 {
@@ -1523,6 +1671,76 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
         const uint32_t nc = match_rounds(c, g, level, vperm.get(), pos.get(), hn.get(), vcmap, c.coarsen == 2);
         if (t) {
             MI_HIP(hipEventRecord(ev[3], c.stream));
+            float a = 0, b = 0, d = 0;
+            MI_HIP(hipEventSynchronize(ev[3]));
+            MI_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            MI_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+            MI_HIP(hipEventElapsedTime(&d, ev[2], ev[3]));
+            t->permute_ms += a;
+            t->heavy_ms += b;
+            t->map_ms += d;
+            for (auto& e : ev) MI_HIP(hipEventDestroy(e));
+        }
+        return nc;
+    }
+
+    if (c.coarsen == 4) {
+        // ---- the v2 map, round by round
+        DBuf<uint32_t> la(&c.pool, n), lb(&c.pool, n), claim(&c.pool, n), smp(&c.pool, n), slot_of(&c.pool, n),
+            ok(&c.pool, n), rank(&c.pool, n), wr(&c.pool, n), lpos(&c.pool, n), jv(&c.pool, n), cnt(&c.pool, 4);
+        DBuf<uint8_t> st(&c.pool, n);
+        MI_HIP(hipMemcpyAsync(la.get(), vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        fill_u32(c, vcmap, n, INF);
+        uint32_t len = n, nc = 0;
+        for (uint32_t round = 0; len > 0; round++) {
+            MI_CHECK(round <= n + 2, ST_INTERNAL, "v2 map rounds did not end");
+            fill_u32(c, claim.get(), n, INF);
+            fill_u32(c, slot_of.get(), n, INF);
+            fill_u32(c, wr.get(), n, 0u);
+            k_v2_claim<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn.get(), pos.get(), claim.get());
+            MI_HIP(hipGetLastError());
+            k_v2_samples<<<grid1(n), BLOCK, 0, c.stream>>>(n, claim.get(), la.get(), smp.get(), slot_of.get(),
+                                                           st.get());
+            MI_HIP(hipGetLastError());
+            for (uint32_t pass = 0;; pass++) {
+                MI_CHECK(pass <= n + 2, ST_INTERNAL, "v2 map: pseudo-locks did not resolve");
+                fill_u32(c, cnt.get(), 1, 0u);
+                k_v2_first<<<grid1(n), BLOCK, 0, c.stream>>>(n, smp.get(), slot_of.get(), st.get(), cnt.get());
+                MI_HIP(hipGetLastError());
+                uint32_t und = 0;
+                read_words(c, cnt.get(), &und, 4);
+                if (und == 0) break;
+            }
+            k_v2_ok<<<grid1(n), BLOCK, 0, c.stream>>>(n, smp.get(), slot_of.get(), st.get(), ok.get());
+            MI_HIP(hipGetLastError());
+            scan_exclusive_u32(c, ok.get(), rank.get(), n, cnt.get() + 1);
+            k_v2_write<<<grid1(n), BLOCK, 0, c.stream>>>(n, smp.get(), vperm.get(), ok.get(), rank.get(), nc,
+                                                         wr.get());
+            MI_HIP(hipGetLastError());
+            k_v2_apply<<<grid1(n), BLOCK, 0, c.stream>>>(n, wr.get(), vcmap);
+            MI_HIP(hipGetLastError());
+            fill_u32(c, lpos.get(), n, INF);
+            k_v2_lpos<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), lpos.get());
+            MI_HIP(hipGetLastError());
+            k_v2_join<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), hn.get(), lpos.get(), vcmap, jv.get());
+            MI_HIP(hipGetLastError());
+            k_v2_join_apply<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), jv.get(), vcmap, ok.get());
+            MI_HIP(hipGetLastError());
+            scan_exclusive_u32(c, ok.get(), rank.get(), len, cnt.get() + 2);
+            k_match_compact<<<grid1(len), BLOCK, 0, c.stream>>>(len, la.get(), ok.get(), rank.get(), lb.get());
+            MI_HIP(hipGetLastError());
+            uint32_t h[2] = {0, 0};
+            read_words(c, cnt.get() + 1, h, 8);
+            nc += h[0];
+            MI_CHECK(h[1] < len, ST_ARG, "v2 map: a round mapped nothing (the reference loops for ever)");
+            len = h[1];
+            la.swap(lb);
+        }
+        if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
+        if (vperm_out)
+            MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+        if (t) {
             float a = 0, b = 0, d = 0;
             MI_HIP(hipEventSynchronize(ev[3]));
             MI_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));

@@ -35,7 +35,7 @@
 //   on HIP device 0 (SGPAR_MI_DEVICE selects another), made on first use.
 //
 // Compiled with -DHEC_V3 (the hec3_* binaries) the context coarsens with the
-// v3 map, with -DPUREMATCH (the match_* binaries) with heavy-edge matching,
+// v3 map, with -DHEC_V2 (the hec2_* binaries) with the v2 map, with -DPUREMATCH (the match_* binaries) with heavy-edge matching,
 // with -DMTMETIS (the mtmetis_* binaries) with that matching and its passes,
 // with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
 // gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening); with -DHASHMAP
@@ -77,8 +77,10 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#if defined HEC_V3 || defined MIS || defined GOSH || defined PUREMATCH || defined MTMETIS
-#if defined MIS
+#if defined HEC_V3 || defined HEC_V2 || defined MIS || defined GOSH || defined PUREMATCH || defined MTMETIS
+#if defined HEC_V2
+        const int variant = 4;
+#elif defined MIS
         const int variant = 5;
 #elif defined PUREMATCH
         const int variant = 1;

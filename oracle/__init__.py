@@ -111,6 +111,10 @@ class _Core:
         L.orc_murmur3_x86_32.argtypes = [C.c_char_p, C.c_int, C.c_uint32]
         L.orc_binsort_u32.restype = None
         L.orc_binsort_u32.argtypes = [C.c_uint32, u32p, u32p]
+        L.orc_map_construct_v2.restype = C.c_uint32
+        L.orc_map_construct_v2.argtypes = [C.c_uint32, u32p, u32p, u32p, u32p, C.POINTER(C.c_int)]
+        L.orc_coarsen_hec2.restype = C.c_uint32
+        L.orc_coarsen_hec2.argtypes = L.orc_coarsen_match.argtypes
         L.orc_gosh_clusters.restype = C.c_int
         L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_gosh.restype = C.c_uint32
@@ -428,6 +432,33 @@ def binsort_u32(keys):
     return out[:len(keys)].copy()
 
 
+def map_construct_v2(vperm, hn):
+    """(vcmap, nc, rounds) of parallel_map_construct_v2 (heuristics.h:695-764)
+    on one thread; nc = None if a round maps nothing."""
+    L = core().L
+    vperm = np.ascontiguousarray(vperm, dtype=np.uint32)
+    hn = np.ascontiguousarray(hn, dtype=np.uint32)
+    n = len(vperm)
+    pos = np.empty(n + 1, dtype=np.uint32)
+    pos[vperm] = np.arange(n, dtype=np.uint32)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    r = C.c_int(0)
+    nc = L.orc_map_construct_v2(n, vperm, hn, pos, vc, C.byref(r))
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
+
+
+def coarsen_hec2(level: int, rowptr, col, w, seed: int):
+    """(vcmap, nc, rounds) of one HEC level with the v2 map (the hec2_* binaries)."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    r = C.c_int(0)
+    nc = L.orc_coarsen_hec2(level, n, rowptr, col, wp, seed, vc, C.byref(r))
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
+
+
 def gosh_clusters(rowptr, col):
     """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
     (heuristics.h:102-217)."""
@@ -465,7 +496,7 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
                     normLap: int = 0, tol: float = 1e-10, variant: int = 0, builder: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
     variant: 0 HEC, 1 heavy-edge matching (match_*), 2 matching with the
-    mt-metis passes (mtmetis_*), 3 HEC with the v3 map (the
+    mt-metis passes (mtmetis_*), 4 HEC with the v2 map (hec2_*), 3 HEC with the v3 map (the
     hec3_* binaries), 5 MIS-2 (mis_*), 6 GOSH (gosh_*); builder 1: the HASHMAP
     build (hec_spec_hashmap)."""
     L = core().L

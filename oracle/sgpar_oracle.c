@@ -332,6 +332,98 @@ vid_t orc_map_construct_v3(vid_t n, const vid_t *hn, vid_t *vcmap)
     return nc + 1;
 }
 
+/* ------------------------------------------------------------------ */
+/* parallel_map_construct_v2 (the hec2_* binaries, -DHEC_V2,           */
+/* heuristics.h:695-764, selected at :1001-1002) on a one-thread       */
+/* backend, as written.  A round: every list vertex u claims the slot  */
+/* pos(hn(u)), the first claim winning (:708-712); slots in increasing */
+/* order take pseudo-locks at min(u, slot) and max(u, slot) -- a vertex */
+/* id against a position, as written -- the second only when the first */
+/* was free, and pair u with hn(u) under a new id when both were free   */
+/* (:716-729; a later pair may re-map a vertex); list vertices still    */
+/* unmapped join hn(u)'s id when it has one, seeing the joins made      */
+/* earlier in the same walk (:731-743); the rest form the next list.    */
+/* Returns ORC_INF if a round maps nothing (the reference loops).       */
+/* ------------------------------------------------------------------ */
+vid_t orc_map_construct_v2(vid_t n, const vid_t *vperm, const vid_t *hn, const vid_t *pos, vid_t *vcmap,
+                           int *rounds_out)
+{
+    vid_t *list = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *next = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *smp = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    uint32_t *locks = (uint32_t *)malloc((size_t)n * sizeof(uint32_t) + 4);
+    vid_t len = n, nc = 0;
+    int rounds = 0;
+    for (vid_t i = 0; i < n; i++) {
+        vcmap[i] = ORC_INF;
+        list[i] = vperm[i];
+    }
+    while (len > 0) {
+        rounds++;
+        for (vid_t v = 0; v < n; v++) {
+            smp[v] = ORC_INF;
+            locks[v] = 0;
+        }
+        for (vid_t k = 0; k < len; k++) {
+            const vid_t u = list[k], v = pos[hn[u]];
+            if (smp[v] == ORC_INF) smp[v] = u;
+        }
+        for (vid_t v = 0; v < n; v++) {
+            const vid_t u = smp[v];
+            if (u == ORC_INF) continue;
+            const vid_t a = v < u ? v : u, b = v < u ? u : v;
+            if (locks[a]++ == 0 && locks[b]++ == 0) {
+                vcmap[u] = nc;
+                vcmap[vperm[v]] = nc;
+                nc++;
+            }
+        }
+        vid_t nl = 0;
+        for (vid_t k = 0; k < len; k++) {
+            const vid_t u = list[k];
+            if (vcmap[u] == ORC_INF) {
+                if (vcmap[hn[u]] != ORC_INF)
+                    vcmap[u] = vcmap[hn[u]];
+                else
+                    next[nl++] = u;
+            }
+        }
+        if (nl == len) {
+            nc = ORC_INF;
+            break;
+        }
+        len = nl;
+        vid_t *t = list;
+        list = next;
+        next = t;
+    }
+    if (rounds_out) *rounds_out = rounds;
+    free(list);
+    free(next);
+    free(smp);
+    free(locks);
+    return nc;
+}
+
+/* One HEC level with the v2 map: HEC's pool, permutation and heavy
+   neighbours (heuristics.h:930-1000), then orc_map_construct_v2. */
+vid_t orc_coarsen_hec2(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                       uint64_t seed, vid_t *vcmap, int *rounds_out)
+{
+    uint64_t st = orc_pool_state(seed);
+    vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *pos = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    orc_generate_permutation(n, &st, vperm, NULL);
+    for (vid_t i = 0; i < n; i++) pos[vperm[i]] = i;
+    orc_heavy_neighbor(level, n, rowptr, col, w, &st, hn);
+    vid_t nc = orc_map_construct_v2(n, vperm, hn, pos, vcmap, rounds_out);
+    free(vperm);
+    free(pos);
+    free(hn);
+    return nc;
+}
+
 /* One HEC level with the v3 map (sgp_coarsen_HEC with -DHEC_V3,
    heuristics.h:930-1031): the same pool, permutation draws and heavy
    neighbours as orc_coarsen_hec, then orc_map_construct_v3. */
@@ -1013,7 +1105,7 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
 }
 
 /* The level loop with HEC (variant 0), heavy-edge matching (1), matching
-   with the mt-metis passes (2), HEC with the v3 map (3), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+   with the mt-metis passes (2), HEC with the v3 map (3) or the v2 map (4), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
    (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
@@ -1044,6 +1136,7 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
                    : variant == 6 ? orc_coarsen_gosh(a->n, a->rowptr, a->col, a->vcmap)
                    : variant == 1 ? orc_coarsen_match(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 2 ? orc_coarsen_mtmetis(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
+                   : variant == 4 ? orc_coarsen_hec2(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                                   : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
                                                     NULL, NULL);
         if (nc == ORC_INF) {  /* no map (matching on a row without entries): the levels so far */
