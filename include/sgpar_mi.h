@@ -189,8 +189,18 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * whose ends both have more than nnz/n entries, one aggregate per centre in
  * index order, every other vertex joining its centre neighbour with the
  * most entries, the first of equals in its row (sgp_coarsen_GOSH,
- * :312-377).  With variants 5 and 6 mi_hec_level leaves vperm and hn
- * untouched.  Other values: MI_STATUS_ARG.
+ * :312-377); variant 7 the GOSH v2 coarsening of the gosh2_* binaries
+ * (-DGOSH_V2, cmg/CMakeLists.txt:49,57; sgp_coarsen_GOSH_v2,
+ * heuristics.h:444-601): seeds of more than max(nnz/n, 50) entries numbered
+ * in index order, three index-order sweeps that see their own writes
+ * (unmapped vertices join the mapped neighbour of the largest weight, then
+ * of the most entries; mapped vertices hand their id to unmapped
+ * neighbours), then the remaining vertices choose a heavy neighbour by
+ * (weight, XorShift64 draw of the level's seed, one per entry) and go
+ * through parallel_map_construct_prefilled (:378-442) -- a remaining
+ * vertex without entries gives MI_STATUS_ARG.  With variants 5 to 7
+ * mi_hec_level leaves vperm and hn untouched.  Other values:
+ * MI_STATUS_ARG.
  */
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant);
 

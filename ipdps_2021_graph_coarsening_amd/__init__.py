@@ -236,7 +236,8 @@ class Context:
         (that matching plus the leaf, twin and relative passes), 4 the hec2_*
         binaries' (parallel_map_construct_v2),
         3 the hec3_* binaries' (parallel_map_construct_v3), 5 the mis_*
-        binaries' (MIS-2 aggregates), 6 the gosh_* binaries' (GOSH centres)."""
+        binaries' (MIS-2 aggregates), 6 the gosh_* binaries' (GOSH centres), 7 the
+        gosh2_* binaries' (GOSH v2)."""
         self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")
 
     def set_builder(self, builder: int) -> None:

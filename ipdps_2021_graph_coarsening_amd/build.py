@@ -70,7 +70,7 @@ DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
 # the same compiled with -DFM, as the reference's *_fm binaries are, and with
-# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH, -DHEC_V2, -DHASHMAP, -DPUREMATCH and -DMTMETIS likewise
+# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH, -DHEC_V2, -DGOSH_V2, -DHASHMAP, -DPUREMATCH and -DMTMETIS likewise
 DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
 DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
 DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
@@ -93,6 +93,8 @@ DROPIN_HEC2_LIB = os.path.join(PKG, "libsgpar_dropin_hec2.so")
 DROPIN_HEC2_EXE = os.path.join(DROPIN, "sgpar_dropin_hec2")
 DROPIN_HEC2_FM_LIB = os.path.join(PKG, "libsgpar_dropin_hec2_fm.so")
 DROPIN_HEC2_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_hec2_fm")
+DROPIN_GOSH2_LIB = os.path.join(PKG, "libsgpar_dropin_gosh2.so")
+DROPIN_GOSH2_EXE = os.path.join(DROPIN, "sgpar_dropin_gosh2")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -117,7 +119,8 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
                             (DROPIN_MATCH_FM_LIB, DROPIN_MATCH_FM_EXE, ["-DPUREMATCH", "-DFM"]),
                             (DROPIN_MTMETIS_LIB, DROPIN_MTMETIS_EXE, ["-DMTMETIS"]),
                             (DROPIN_HEC2_LIB, DROPIN_HEC2_EXE, ["-DHEC_V2"]),
-                            (DROPIN_HEC2_FM_LIB, DROPIN_HEC2_FM_EXE, ["-DHEC_V2", "-DFM"])):
+                            (DROPIN_HEC2_FM_LIB, DROPIN_HEC2_FM_EXE, ["-DHEC_V2", "-DFM"]),
+                            (DROPIN_GOSH2_LIB, DROPIN_GOSH2_EXE, ["-DGOSH_V2"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

@@ -155,42 +155,46 @@ enum : uint8_t {
     FL_PROP = 32   // pass-2 state already pushed to hn(u)
 };
 
+// pre (or null): vertices that hold an id already (a prefilled map): they
+// never act and keep their id; a walker whose heavy neighbour is one joins it
 __global__ void k_map_mark(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
-                           uint8_t* __restrict__ fl, uint32_t* __restrict__ minpos,
-                           uint32_t* __restrict__ vcmap)
+                           const uint8_t* __restrict__ pre, uint8_t* __restrict__ fl,
+                           uint32_t* __restrict__ minpos, uint32_t* __restrict__ vcmap)
 {
     uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= n) return;
+    const bool p = pre && pre[u];
     uint32_t v = hn[u], pu = pos[u];
-    bool e = pu >= pos[v];
+    bool e = !p && pu >= pos[v];
     fl[u] = e ? FL_E : 0;
-    vcmap[u] = INF;
+    if (!p) vcmap[u] = INF;
     if (e && v != u) atomicMin(&minpos[v], pu);
 }
 
 __global__ void k_map_creators1(uint32_t n, const uint32_t* __restrict__ vperm,
-                                const uint32_t* __restrict__ hn, uint8_t* fl,
+                                const uint32_t* __restrict__ hn, const uint8_t* __restrict__ pre, uint8_t* fl,
                                 const uint32_t* __restrict__ minpos, uint32_t* __restrict__ cflag)
 {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     uint32_t u = vperm[i], v = hn[u];
     uint8_t fu = fl[u];
-    bool cr = (fu & FL_E) && (v == u || (!(fl[v] & FL_E) && minpos[v] == i));
+    bool cr = (fu & FL_E) && (v == u || (!(fl[v] & FL_E) && !(pre && pre[v]) && minpos[v] == i));
     cflag[i] = cr ? 1u : 0u;
     if (cr) fl[u] = fu | FL_CR;
 }
 
 __global__ void k_map_assign1(uint32_t n, const uint32_t* __restrict__ vperm,
                               const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
-                              const uint8_t* __restrict__ fl, const uint32_t* __restrict__ minpos,
-                              const uint32_t* __restrict__ id, uint32_t* __restrict__ vcmap)
+                              const uint8_t* __restrict__ pre, const uint8_t* __restrict__ fl,
+                              const uint32_t* __restrict__ minpos, const uint32_t* __restrict__ id, uint32_t nc0,
+                              uint32_t* __restrict__ vcmap)
 {
     uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= n) return;
     uint8_t fu = fl[u];
     if (fu & FL_CR) {
-        uint32_t c = id[pos[u]];
+        uint32_t c = nc0 + id[pos[u]];
         vcmap[u] = c;
         if (hn[u] != u) vcmap[hn[u]] = c;
     } else if (fu & FL_E) {
@@ -204,12 +208,15 @@ __global__ void k_map_assign1(uint32_t n, const uint32_t* __restrict__ vperm,
             uint32_t v = hn[x];
             if (fl[v] & FL_E) {
                 x = v;
+            } else if (pre && pre[v]) {
+                vcmap[u] = vcmap[v];  // a prefilled id: never written here
+                return;
             } else {
                 root = vperm[minpos[v]];
                 break;
             }
         }
-        vcmap[u] = id[pos[root]];
+        vcmap[u] = nc0 + id[pos[root]];
     }
 }
 
@@ -343,14 +350,14 @@ __global__ void k_map_creators2(uint32_t n, const uint32_t* __restrict__ vperm,
 
 __global__ void k_map_assign2(uint32_t n, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ hn,
                               const uint8_t* __restrict__ fl, const uint32_t* __restrict__ minpos2,
-                              const uint32_t* __restrict__ id2, const uint32_t* __restrict__ nc1p,
+                              const uint32_t* __restrict__ id2, const uint32_t* __restrict__ nc1p, uint32_t nc0,
                               uint32_t* vcmap)
 {
     uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= n) return;
     uint8_t f = fl[u];
     if (!(f & FL_R)) return;
-    const uint32_t nc1 = *nc1p;
+    const uint32_t nc1 = nc0 + *nc1p;
     if (f & FL_ACT) {
         uint32_t v = hn[u];
         if (!(fl[v] & FL_R))
@@ -1177,6 +1184,166 @@ __global__ void k_rel_keep(uint32_t len, const uint32_t* __restrict__ list,
     if (x) hn[u] = (uint32_t)x;
 }
 
+// ---- GOSH v2 coarsening (sgp_coarsen_GOSH_v2, heuristics.h:444-601; the
+// gosh2_* binaries).  Seeds (more entries than max(nnz/n, 50)) take ids in
+// index order; two index-order sweeps that see their own writes hand an
+// unmapped vertex the id of its mapped neighbour of the largest weight, then
+// of the most entries (the last of equals in its row); a third hands every
+// mapped vertex's id to its unmapped neighbours (MIS-2's sweep); the rest
+// choose a heavy neighbour by (weight, draw) and go through the prefilled
+// map (map_closed_form with positions equal to vertex ids).
+__global__ void k_g2_seeds(uint32_t n, const uint32_t* __restrict__ rowptr, uint32_t thr,
+                           uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) flag[i] = rowptr[i + 1] - rowptr[i] > thr ? 1u : 0u;
+}
+
+__global__ void k_g2_seed_ids(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ id,
+                              uint32_t* __restrict__ vcmap)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) vcmap[i] = flag[i] ? id[i] : INF;
+}
+
+__global__ void k_g2_mapped(uint32_t n, const uint32_t* __restrict__ vcmap, uint8_t* __restrict__ a0,
+                            uint8_t* __restrict__ reach)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    a0[i] = vcmap[i] != INF ? 1 : 0;
+    reach[i] = 0;
+}
+
+// A joining sweep's fixed point: an unmapped vertex is reached (mapped at
+// its turn) when a neighbour held an id before the sweep or is a smaller
+// reached vertex; a pass raises reach towards it, *changed counts raises.
+__global__ void __launch_bounds__(BLOCK) k_g2_reach(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                    const uint32_t* __restrict__ col, const uint8_t* __restrict__ a0,
+                                                    uint8_t* reach, uint32_t* __restrict__ changed)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t i = wid; i < n; i += nw) {
+        if (a0[i] || reach[i]) continue;
+        bool hit = false;
+        for (uint32_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += WAVE) {
+            const uint32_t v = col[e];
+            hit = hit || a0[v] || (v < i && reach[v]);
+        }
+        if (__ballot(hit) != 0 && lane == 0) {
+            reach[i] = 1;
+            atomicAdd(changed, 1u);
+        }
+    }
+}
+
+// the candidate a reached vertex takes: the largest (key, entry) among its
+// mapped-at-its-turn neighbours; key = weight (by_deg = 0) or entries
+__global__ void __launch_bounds__(BLOCK) k_g2_pick(uint32_t n, const uint32_t* __restrict__ rowptr,
+                                                   const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                                                   int by_deg, const uint8_t* __restrict__ a0,
+                                                   const uint8_t* __restrict__ reach, uint32_t* __restrict__ p)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t wid = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t i = wid; i < n; i += nw) {
+        if (!reach[i]) continue;
+        uint64_t best = 0;
+        for (uint32_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += WAVE) {
+            const uint32_t v = col[e];
+            if (!(a0[v] || (v < i && reach[v]))) continue;
+            const uint32_t key = by_deg ? rowptr[v + 1] - rowptr[v] : (w ? w[e] : 1u);
+            const uint64_t x = ((uint64_t)key << 32) | e;
+            best = x > best ? x : best;
+        }
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) {
+            const uint64_t x = __shfl_xor(best, o, WAVE);
+            best = x > best ? x : best;
+        }
+        if (lane == 0) p[i] = col[(uint32_t)best];
+    }
+}
+
+// each reached vertex follows its candidates (smaller reached vertices) to
+// one mapped before the sweep and takes its id
+__global__ void k_g2_chase(uint32_t n, const uint8_t* __restrict__ a0, const uint8_t* __restrict__ reach,
+                           const uint32_t* __restrict__ p, uint32_t* vcmap)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || !reach[i]) return;
+    uint32_t x = i;
+    for (uint32_t step = 0; step <= n && !a0[x]; step++) x = p[x];
+    vcmap[i] = vcmap[x];
+}
+
+__global__ void k_g2_reached_init(uint32_t n, const uint32_t* __restrict__ vcmap, uint32_t* __restrict__ reached)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) reached[i] = vcmap[i] != INF ? 0u : INF;
+}
+
+// the hand-out sweep's result for the vertices it reaches (others keep INF)
+__global__ void k_g2_follow(uint32_t n, const uint32_t* __restrict__ reached, uint32_t* vcmap)
+{
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= n) return;
+    uint32_t r = reached[v];
+    if (r == 0 || r == INF) return;
+    uint32_t p = v;
+    for (uint32_t k = 0; r != 0 && r != INF && k <= n; k++) {
+        p = r - 1;
+        r = reached[p];
+    }
+    if (r == 0) vcmap[v] = vcmap[p];
+}
+
+__global__ void k_g2_unmapped(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ rowptr,
+                              uint32_t* __restrict__ flag, uint32_t* __restrict__ dg, uint8_t* __restrict__ pre,
+                              uint32_t* __restrict__ hn)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const bool un = vcmap[i] == INF;
+    flag[i] = un ? 1u : 0u;
+    dg[i] = un ? rowptr[i + 1] - rowptr[i] : 0u;
+    pre[i] = un ? 0 : 1;
+    hn[i] = i;
+}
+
+// :540-567 -- a remaining vertex's heavy neighbour: the largest (weight,
+// draw), the last of equals, one draw per entry at its row's offset in the
+// stream; one thread per vertex (remaining rows hold at most the seed
+// threshold of entries)
+__global__ void k_g2_hn(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                        const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                        const uint32_t* __restrict__ w, const uint64_t* __restrict__ draws, uint32_t* __restrict__ hn,
+                        uint32_t* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const uint32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+    if (r0 == r1) {
+        atomicAdd(err, 1u);
+        return;
+    }
+    uint32_t bw = 0, h = INF;
+    uint64_t br = 0;
+    for (uint32_t e = r0; e < r1; e++) {
+        const uint32_t x = w ? w[e] : 1u;
+        const uint64_t r = draws[(uint64_t)off[i] + (e - r0)];
+        if (bw < x || (bw == x && br <= r)) {
+            bw = x;
+            br = r;
+            h = col[e];
+        }
+    }
+    hn[i] = h;
+}
+
 // ---- GOSH coarsening (GOSH_clusters, heuristics.h:102-217, and the
 // aggregation of sgp_coarsen_GOSH, :312-377; the gosh_* binaries).  A round
 // gives every undecided vertex the largest (state, entries, index) of itself
@@ -1562,6 +1729,88 @@ static uint32_t match_rounds(Ctx& c, const DGraph& g, int level, const uint32_t*
     return nc + ns;
 }
 
+static uint32_t map_closed_form(Ctx& c, uint32_t n, const uint32_t* vperm, const uint32_t* pos, const uint32_t* hn,
+                                uint32_t* vcmap, const uint8_t* pre, uint32_t nc0);
+
+// One joining sweep of GOSH v2 (by_deg: the second one).
+static void g2_join_sweep(Ctx& c, const DGraph& g, int by_deg, uint32_t* vcmap, uint8_t* a0, uint8_t* reach,
+                          uint32_t* p, uint32_t* cnt)
+{
+    const uint32_t n = g.n;
+    const unsigned wg = (unsigned)std::min<uint64_t>(ceil_div(n, BLOCK / WAVE), 65536);
+    k_g2_mapped<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, a0, reach);
+    MI_HIP(hipGetLastError());
+    for (uint32_t pass = 0;; pass++) {
+        MI_CHECK(pass <= n + 2, ST_INTERNAL, "GOSH v2 sweep did not settle");
+        fill_u32(c, cnt, 1, 0u);
+        k_g2_reach<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), a0, reach, cnt);
+        MI_HIP(hipGetLastError());
+        uint32_t ch = 0;
+        read_words(c, cnt, &ch, 4);
+        if (ch == 0) break;
+    }
+    k_g2_pick<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), g.W(), by_deg, a0, reach, p);
+    MI_HIP(hipGetLastError());
+    k_g2_chase<<<grid1(n), BLOCK, 0, c.stream>>>(n, a0, reach, p, vcmap);
+    MI_HIP(hipGetLastError());
+}
+
+// One GOSH v2 level (the gosh2_* binaries).
+static uint32_t gosh2_map(Ctx& c, const DGraph& g, uint64_t seed, uint32_t* vcmap, HecTimes* t)
+{
+    const uint32_t n = g.n;
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t thr = std::max<uint32_t>(g.m / n, 50u);  // :454-457
+    DBuf<uint32_t> flag(&c.pool, n), id(&c.pool, n), p(&c.pool, n), cnt(&c.pool, 4);
+    DBuf<uint8_t> a0(&c.pool, n), reach(&c.pool, n);
+    k_g2_seeds<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), thr, flag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, flag.get(), id.get(), n, cnt.get() + 1);
+    k_g2_seed_ids<<<grid1(n), BLOCK, 0, c.stream>>>(n, flag.get(), id.get(), vcmap);
+    MI_HIP(hipGetLastError());
+    uint32_t nseed = 0;
+    read_words(c, cnt.get() + 1, &nseed, 4);
+    g2_join_sweep(c, g, 0, vcmap, a0.get(), reach.get(), p.get(), cnt.get());
+    g2_join_sweep(c, g, 1, vcmap, a0.get(), reach.get(), p.get(), cnt.get());
+    // the hand-out sweep
+    const unsigned wg = (unsigned)std::min<uint64_t>(ceil_div(n, BLOCK / WAVE), 65536);
+    k_g2_reached_init<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, p.get());
+    MI_HIP(hipGetLastError());
+    for (uint32_t pass = 0;; pass++) {
+        MI_CHECK(pass <= n + 2, ST_INTERNAL, "GOSH v2 hand-out sweep did not settle");
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_mis_relax<<<wg, BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), p.get(), cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t ch = 0;
+        read_words(c, cnt.get(), &ch, 4);
+        if (ch == 0) break;
+    }
+    k_g2_follow<<<grid1(n), BLOCK, 0, c.stream>>>(n, p.get(), vcmap);
+    MI_HIP(hipGetLastError());
+    // the remaining vertices: draws at their entries' offsets, heavy neighbours
+    DBuf<uint32_t> dg(&c.pool, n), off(&c.pool, n), hn(&c.pool, n), iota(&c.pool, n);
+    DBuf<uint8_t> pre(&c.pool, n);
+    k_g2_unmapped<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, g.R(), flag.get(), dg.get(), pre.get(), hn.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, dg.get(), off.get(), n, cnt.get() + 2);
+    uint32_t ndraw = 0;
+    read_words(c, cnt.get() + 2, &ndraw, 4);
+    DBuf<uint64_t> draws(&c.pool, ndraw ? ndraw : 1);
+    xs_draws(c, xs_pool_state(seed), 0, ndraw, draws.get());
+    fill_u32(c, cnt.get() + 3, 1, 0u);
+    k_g2_hn<<<grid1(n), BLOCK, 0, c.stream>>>(n, flag.get(), off.get(), g.R(), g.Cc(), g.W(), draws.get(), hn.get(),
+                                              cnt.get() + 3);
+    MI_HIP(hipGetLastError());
+    uint32_t ne = 0;
+    read_words(c, cnt.get() + 3, &ne, 4);
+    MI_CHECK(ne == 0, ST_ARG, "GOSH v2: a remaining vertex without entries has no heavy neighbour");
+    draws.reset();
+    iota_u32(c, iota.get(), n, 0);
+    const uint32_t nc = map_closed_form(c, n, iota.get(), iota.get(), hn.get(), vcmap, pre.get(), nseed);
+    if (t) t->map_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return nc;
+}
+
 // One GOSH level (the gosh_* binaries): the centres, then the aggregates.
 static uint32_t gosh_map(Ctx& c, const DGraph& g, uint32_t* vcmap, HecTimes* t)
 {
@@ -1599,14 +1848,96 @@ static uint32_t gosh_map(Ctx& c, const DGraph& g, uint32_t* vcmap, HecTimes* t)
     return h[0];
 }
 
+// The closed form of the one-thread map construction (heuristics.h:633-693;
+// the prefilled form :378-442 with pre and nc0): pass 1 -- creators by a
+// position scan, joins by walks inside E -- and pass 2 -- who acts, resolved
+// in rounds over the forest of unassigned vertices -- then the ids.
+// Returns nc.
+static uint32_t map_closed_form(Ctx& c, uint32_t n, const uint32_t* vperm, const uint32_t* pos, const uint32_t* hn,
+                                uint32_t* vcmap, const uint8_t* pre, uint32_t nc0)
+{
+    DBuf<uint8_t> fl(&c.pool, n);
+    DBuf<uint32_t> minpos(&c.pool, n), id(&c.pool, n);
+    // one zero fill for the four totals and pass 2's three counters
+    DBuf<uint32_t> zeroed(&c.pool, 4 + 3 * (size_t)n);
+    MI_HIP(hipMemsetAsync(zeroed.get(), 0, (4 + 3 * (size_t)n) * 4, c.stream));
+    uint32_t* const totals = zeroed.get();
+    uint32_t* const indeg = totals + 4;
+    uint32_t* const nonact = indeg + n;
+    uint32_t* const hasact = nonact + n;
+    fill_u32(c, minpos.get(), n, INF);
+    k_map_mark<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos, hn, pre, fl.get(), minpos.get(), vcmap);
+    MI_HIP(hipGetLastError());
+    k_map_creators1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm, hn, pre, fl.get(), minpos.get(),
+                                                      id.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 0);
+    k_map_assign1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm, pos, hn, pre, fl.get(),
+                                                    minpos.get(), id.get(), nc0, vcmap);
+    MI_HIP(hipGetLastError());
+
+    // ---- pass 2
+    k_map_r<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, fl.get());
+    MI_HIP(hipGetLastError());
+    k_map_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn, fl.get(), indeg);
+    MI_HIP(hipGetLastError());
+    // Rounds until every unassigned vertex is resolved; the forest formed by
+    // u -> hn(u) has strictly increasing positions, so the lowest unresolved
+    // vertex always resolves and at most n rounds are needed.
+    uint32_t rounds = 0;
+    if (n <= MAP_ONE_WG) {
+        k_map_rounds<<<1, 1024, 0, c.stream>>>(n, hn, fl.get(), indeg, nonact, hasact,
+                                               n + 8, totals + 2);
+        MI_HIP(hipGetLastError());
+    }
+    for (; n > MAP_ONE_WG;) {
+        const int batch = 4;
+        for (int b = 0; b < batch; b++) {
+            MI_HIP(hipMemsetAsync(totals + 2, 0, 4, c.stream));
+            k_map_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, fl.get(), indeg, nonact,
+                                                            hasact, totals + 2);
+            MI_HIP(hipGetLastError());
+            k_map_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn, fl.get(), nonact,
+                                                              hasact);
+            MI_HIP(hipGetLastError());
+        }
+        rounds += batch;
+        uint32_t left = 0;
+        read_words(c, totals + 2, &left, 4);
+        if (left == 0) break;
+        MI_CHECK(rounds <= n + 8, ST_INTERNAL, "map construction did not converge");
+    }
+    fill_u32(c, minpos.get(), n, INF);
+    k_map_min2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos, hn, fl.get(), minpos.get());
+    MI_HIP(hipGetLastError());
+    k_map_creators2<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm, hn, fl.get(), minpos.get(),
+                                                      id.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 1);
+    k_map_assign2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos, hn, fl.get(), minpos.get(),
+                                                    id.get(), totals + 0, nc0, vcmap);
+    MI_HIP(hipGetLastError());
+    k_max_u32<<<red_grid(n), BLOCK, 0, c.stream>>>(vcmap, n, totals + 3);
+    MI_HIP(hipGetLastError());
+    uint32_t tot[4] = {0, 0, 0, 0};
+    read_words(c, totals, tot, 16);
+    MI_CHECK(tot[2] == 0, ST_INTERNAL, "map construction did not converge");
+    MI_CHECK(nc0 + tot[0] + tot[1] >= 1 && tot[3] < nc0 + tot[0] + tot[1], ST_INTERNAL,
+             "map construction left a vertex without a cluster id");
+    return nc0 + tot[0] + tot[1];
+
+}
+
 uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed, uint32_t* vcmap,
                  uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
-    if (c.coarsen == 5 || c.coarsen == 6) {
+    if (c.coarsen >= 5) {
         MI_CHECK(!vperm_out && !hn_out, ST_ARG, "MIS-2 and GOSH coarsening have no permutation or heavy neighbours");
-        return c.coarsen == 5 ? mis2_map(c, g, seed, vcmap, t) : gosh_map(c, g, vcmap, t);
+        return c.coarsen == 5 ? mis2_map(c, g, seed, vcmap, t)
+               : c.coarsen == 6 ? gosh_map(c, g, vcmap, t)
+                                : gosh2_map(c, g, seed, vcmap, t);
     }
     hipEvent_t ev[4];
     if (t)
@@ -1795,77 +2126,9 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
         return nc_v3;
     }
 
-    // ---- map construction, pass 1 (closed form)
-    DBuf<uint8_t> fl(&c.pool, n);
-    DBuf<uint32_t> minpos(&c.pool, n), id(&c.pool, n);
-    // one zero fill for the four totals and pass 2's three counters
-    DBuf<uint32_t> zeroed(&c.pool, 4 + 3 * (size_t)n);
-    MI_HIP(hipMemsetAsync(zeroed.get(), 0, (4 + 3 * (size_t)n) * 4, c.stream));
-    uint32_t* const totals = zeroed.get();
-    uint32_t* const indeg = totals + 4;
-    uint32_t* const nonact = indeg + n;
-    uint32_t* const hasact = nonact + n;
-    fill_u32(c, minpos.get(), n, INF);
-    k_map_mark<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(), vcmap);
-    MI_HIP(hipGetLastError());
-    k_map_creators1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
-                                                      id.get());
-    MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 0);
-    k_map_assign1<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), pos.get(), hn.get(), fl.get(),
-                                                    minpos.get(), id.get(), vcmap);
-    MI_HIP(hipGetLastError());
-
-    // ---- pass 2
-    k_map_r<<<grid1(n), BLOCK, 0, c.stream>>>(n, vcmap, fl.get());
-    MI_HIP(hipGetLastError());
-    k_map_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), indeg);
-    MI_HIP(hipGetLastError());
-    // Rounds until every unassigned vertex is resolved; the forest formed by
-    // u -> hn(u) has strictly increasing positions, so the lowest unresolved
-    // vertex always resolves and at most n rounds are needed.
-    uint32_t rounds = 0;
-    if (n <= MAP_ONE_WG) {
-        k_map_rounds<<<1, 1024, 0, c.stream>>>(n, hn.get(), fl.get(), indeg, nonact, hasact,
-                                               n + 8, totals + 2);
-        MI_HIP(hipGetLastError());
-    }
-    for (; n > MAP_ONE_WG;) {
-        const int batch = 4;
-        for (int b = 0; b < batch; b++) {
-            MI_HIP(hipMemsetAsync(totals + 2, 0, 4, c.stream));
-            k_map_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, fl.get(), indeg, nonact,
-                                                            hasact, totals + 2);
-            MI_HIP(hipGetLastError());
-            k_map_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), fl.get(), nonact,
-                                                              hasact);
-            MI_HIP(hipGetLastError());
-        }
-        rounds += batch;
-        uint32_t left = 0;
-        read_words(c, totals + 2, &left, 4);
-        if (left == 0) break;
-        MI_CHECK(rounds <= n + 8, ST_INTERNAL, "map construction did not converge");
-    }
-    fill_u32(c, minpos.get(), n, INF);
-    k_map_min2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get());
-    MI_HIP(hipGetLastError());
-    k_map_creators2<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), fl.get(), minpos.get(),
-                                                      id.get());
-    MI_HIP(hipGetLastError());
-    scan_exclusive_u32(c, id.get(), id.get(), n, totals + 1);
-    k_map_assign2<<<grid1(n), BLOCK, 0, c.stream>>>(n, pos.get(), hn.get(), fl.get(), minpos.get(),
-                                                    id.get(), totals + 0, vcmap);
-    MI_HIP(hipGetLastError());
+    // ---- map construction (closed form)
+    const uint32_t ncm = map_closed_form(c, n, vperm.get(), pos.get(), hn.get(), vcmap, nullptr, 0);
     if (t) MI_HIP(hipEventRecord(ev[3], c.stream));
-
-    k_max_u32<<<red_grid(n), BLOCK, 0, c.stream>>>(vcmap, n, totals + 3);
-    MI_HIP(hipGetLastError());
-    uint32_t tot[4] = {0, 0, 0, 0};
-    read_words(c, totals, tot, 16);
-    MI_CHECK(tot[2] == 0, ST_INTERNAL, "map construction did not converge");
-    MI_CHECK(tot[0] + tot[1] >= 1 && tot[3] < tot[0] + tot[1], ST_INTERNAL,
-             "map construction left a vertex without a cluster id");
     if (vperm_out)
         MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
     if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
@@ -1880,7 +2143,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
         t->map_ms += d;
         for (auto& e : ev) MI_HIP(hipEventDestroy(e));
     }
-    return tot[0] + tot[1];
+    return ncm;
 }
 
 }  // namespace mi

@@ -38,7 +38,8 @@
 // v3 map, with -DHEC_V2 (the hec2_* binaries) with the v2 map, with -DPUREMATCH (the match_* binaries) with heavy-edge matching,
 // with -DMTMETIS (the mtmetis_* binaries) with that matching and its passes,
 // with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
-// gosh_* binaries) with GOSH centres (mi_ctx_set_coarsening); with -DHASHMAP
+// gosh_* binaries) with GOSH centres, with -DGOSH_V2 (the gosh2_* binaries)
+// with GOSH v2 (mi_ctx_set_coarsening); with -DHASHMAP
 // (hec_spec_hashmap) its coarse rows keep the HASHMAP build's order
 // (mi_ctx_set_builder).
 // Compiled with -DFM (as the reference's *_fm binaries are,
@@ -77,9 +78,12 @@ mi_ctx* context()
             g_ctx = nullptr;
         }
 #endif
-#if defined HEC_V3 || defined HEC_V2 || defined MIS || defined GOSH || defined PUREMATCH || defined MTMETIS
+#if defined HEC_V3 || defined HEC_V2 || defined MIS || defined GOSH || defined GOSH_V2 || defined PUREMATCH || \
+    defined MTMETIS
 #if defined HEC_V2
         const int variant = 4;
+#elif defined GOSH_V2
+        const int variant = 7;
 #elif defined MIS
         const int variant = 5;
 #elif defined PUREMATCH

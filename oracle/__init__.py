@@ -115,6 +115,10 @@ class _Core:
         L.orc_map_construct_v2.argtypes = [C.c_uint32, u32p, u32p, u32p, u32p, C.POINTER(C.c_int)]
         L.orc_coarsen_hec2.restype = C.c_uint32
         L.orc_coarsen_hec2.argtypes = L.orc_coarsen_match.argtypes
+        L.orc_coarsen_gosh2.restype = C.c_uint32
+        L.orc_coarsen_gosh2.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p]
+        L.orc_map_construct_prefilled.restype = C.c_uint32
+        L.orc_map_construct_prefilled.argtypes = [C.c_uint32, u32p, C.c_uint32, u32p, u32p, C.c_uint32]
         L.orc_gosh_clusters.restype = C.c_int
         L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_gosh.restype = C.c_uint32
@@ -459,6 +463,29 @@ def coarsen_hec2(level: int, rowptr, col, w, seed: int):
     return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), r.value
 
 
+def coarsen_gosh2(rowptr, col, w, seed: int):
+    """(vcmap, nc) of one GOSH v2 level (sgp_coarsen_GOSH_v2, heuristics.h:444-601);
+    nc = None when a remaining vertex has no entries."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    nc = L.orc_coarsen_gosh2(n, rowptr, col, wp, seed, vc)
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc))
+
+
+def map_construct_prefilled(vcmap, lst, hn, nc: int):
+    """parallel_map_construct_prefilled (heuristics.h:378-442) on one thread:
+    (new vcmap, nc)."""
+    L = core().L
+    vc = np.ascontiguousarray(vcmap, dtype=np.uint32).copy()
+    lst = np.ascontiguousarray(lst, dtype=np.uint32)
+    hn = np.ascontiguousarray(hn, dtype=np.uint32)
+    out = L.orc_map_construct_prefilled(len(vc), lst if len(lst) else np.zeros(1, np.uint32), len(lst), hn, vc, nc)
+    return vc, int(out)
+
+
 def gosh_clusters(rowptr, col):
     """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
     (heuristics.h:102-217)."""
@@ -496,7 +523,8 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
                     normLap: int = 0, tol: float = 1e-10, variant: int = 0, builder: int = 0):
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
     variant: 0 HEC, 1 heavy-edge matching (match_*), 2 matching with the
-    mt-metis passes (mtmetis_*), 4 HEC with the v2 map (hec2_*), 3 HEC with the v3 map (the
+    mt-metis passes (mtmetis_*), 4 HEC with the v2 map (hec2_*), 7 GOSH v2
+    (gosh2_*), 3 HEC with the v3 map (the
     hec3_* binaries), 5 MIS-2 (mis_*), 6 GOSH (gosh_*); builder 1: the HASHMAP
     build (hec_spec_hashmap)."""
     L = core().L

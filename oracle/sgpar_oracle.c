@@ -955,6 +955,147 @@ vid_t orc_coarsen_gosh(vid_t n, const eid_t *rowptr, const vid_t *col, vid_t *vc
 }
 
 /* ------------------------------------------------------------------ */
+/* parallel_map_construct_prefilled (heuristics.h:378-442) on one      */
+/* thread: HEC's two-pass CAS walk over `list` (in its order) with the  */
+/* ordering condition on vertex ids (u < hn(u)); vertices with a        */
+/* prefilled id (vcmap < n) are taken from the start, so a walker whose */
+/* heavy neighbour holds one joins it.  New ids continue from nc.       */
+/* Returns the new nc.                                                  */
+/* ------------------------------------------------------------------ */
+vid_t orc_map_construct_prefilled(vid_t n, const vid_t *list0, vid_t len, const vid_t *hn, vid_t *vcmap, vid_t nc)
+{
+    vid_t *match = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *list = (vid_t *)malloc((size_t)len * sizeof(vid_t) + 4);
+    vid_t *next = (vid_t *)malloc((size_t)len * sizeof(vid_t) + 4);
+    for (vid_t i = 0; i < n; i++) match[i] = vcmap[i] == ORC_INF ? ORC_INF : n + 1;
+    memcpy(list, list0, (size_t)len * sizeof(vid_t));
+    int swap = 1, passes = 0;
+    while (len > 0 && passes < 64) {
+        passes++;
+        for (vid_t k = 0; k < len; k++) {
+            const vid_t u = list[k], v = hn[u];
+            const int cond = u < v;
+            if (!(cond ^ swap) || match[u] != ORC_INF) continue;
+            match[u] = v;
+            if (u == v || match[v] == ORC_INF) {
+                if (u != v) match[v] = u;
+                vcmap[u] = nc;
+                vcmap[v] = nc;
+                nc++;
+            } else if (vcmap[v] < n) {
+                vcmap[u] = vcmap[v];
+            } else {
+                match[u] = ORC_INF;
+            }
+        }
+        vid_t nl = 0;
+        for (vid_t k = 0; k < len; k++) {
+            const vid_t u = list[k];
+            if (vcmap[u] >= n) {
+                next[nl++] = u;
+                match[u] = ORC_INF;
+            }
+        }
+        swap ^= 1;
+        len = nl;
+        vid_t *t = list;
+        list = next;
+        next = t;
+    }
+    free(match);
+    free(list);
+    free(next);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
+/* GOSH v2 coarsening (the gosh2_* binaries, -DGOSH_V2):                */
+/* sgp_coarsen_GOSH_v2, heuristics.h:444-601, on one thread.  Vertices  */
+/* with more entries than max(nnz/n, 50) get ids in index order; then,  */
+/* in three index-order sweeps that see their own writes, an unmapped   */
+/* vertex takes the id of its mapped neighbour of the largest weight,   */
+/* then of the most entries (the last of equals in its row), then every */
+/* mapped vertex hands its id to its unmapped neighbours; the rest pick */
+/* a heavy neighbour by (weight, draw) with one XorShift64 draw per      */
+/* entry from the pool of `seed` (std::time in the reference, :537),   */
+/* the last of equals, and go through the prefilled CAS map.  Returns   */
+/* nc, or ORC_INF for a remaining vertex without entries (no heavy      */
+/* neighbour: the reference indexes past its arrays).                   */
+/* ------------------------------------------------------------------ */
+vid_t orc_coarsen_gosh2(vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w, uint64_t seed,
+                        vid_t *vcmap)
+{
+    eid_t thr = rowptr[n] / n;
+    if (thr < 50) thr = 50;
+    vid_t nc = 0;
+    for (vid_t i = 0; i < n; i++) vcmap[i] = rowptr[i + 1] - rowptr[i] > thr ? nc++ : ORC_INF;
+    for (vid_t i = 0; i < n; i++) {
+        if (vcmap[i] != ORC_INF) continue;
+        vid_t arg = ORC_INF;
+        wgt_t best = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const vid_t v = col[e];
+            const wgt_t x = w ? w[e] : 1u;
+            if (vcmap[v] != ORC_INF && x >= best) {
+                best = x;
+                arg = v;
+            }
+        }
+        if (arg != ORC_INF) vcmap[i] = vcmap[arg];
+    }
+    for (vid_t i = 0; i < n; i++) {
+        if (vcmap[i] != ORC_INF) continue;
+        vid_t arg = ORC_INF;
+        eid_t best = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const vid_t v = col[e];
+            const eid_t d = rowptr[v + 1] - rowptr[v];
+            if (vcmap[v] != ORC_INF && d >= best) {
+                best = d;
+                arg = v;
+            }
+        }
+        if (arg != ORC_INF) vcmap[i] = vcmap[arg];
+    }
+    for (vid_t i = 0; i < n; i++) {
+        if (vcmap[i] == ORC_INF) continue;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++)
+            if (vcmap[col[e]] == ORC_INF) vcmap[col[e]] = vcmap[i];
+    }
+    vid_t *list = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t len = 0;
+    for (vid_t i = 0; i < n; i++)
+        if (vcmap[i] == ORC_INF) list[len++] = i;
+    uint64_t st = orc_pool_state(seed);
+    for (vid_t k = 0; k < len; k++) {
+        const vid_t i = list[k];
+        vid_t h = ORC_INF;
+        uint64_t best_r = 0;
+        wgt_t best_w = 0;
+        for (eid_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const wgt_t x = w ? w[e] : 1u;
+            const uint64_t r = orc_xs_urand64(&st);
+            if (best_w < x || (best_w == x && best_r <= r)) {
+                best_w = x;
+                best_r = r;
+                h = col[e];
+            }
+        }
+        hn[i] = h;
+        if (h == ORC_INF) {
+            free(list);
+            free(hn);
+            return ORC_INF;
+        }
+    }
+    nc = orc_map_construct_prefilled(n, list, len, hn, vcmap, nc);
+    free(list);
+    free(hn);
+    return nc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Coarse graph, coarsen_kokkos.h:1061-1127 (map/count) with the       */
 /* non-skew builder :799-900 and the RADIX dedupe :750-762, :445-510:  */
 /* rows sorted by column, equal columns merged with summed weights,    */
@@ -1105,7 +1246,8 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
 }
 
 /* The level loop with HEC (variant 0), heavy-edge matching (1), matching
-   with the mt-metis passes (2), HEC with the v3 map (3) or the v2 map (4), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+   with the mt-metis passes (2), HEC with the v3 map (3) or the v2 map (4),
+   GOSH v2 (7), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
    (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
@@ -1134,6 +1276,7 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
         vid_t nc = variant == 3   ? orc_coarsen_hec3(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 5 ? orc_coarsen_mis2(a->n, a->rowptr, a->col, seed, a->vcmap)
                    : variant == 6 ? orc_coarsen_gosh(a->n, a->rowptr, a->col, a->vcmap)
+                   : variant == 7 ? orc_coarsen_gosh2(a->n, a->rowptr, a->col, a->w, seed, a->vcmap)
                    : variant == 1 ? orc_coarsen_match(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 2 ? orc_coarsen_mtmetis(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 4 ? orc_coarsen_hec2(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)

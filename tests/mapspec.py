@@ -79,3 +79,74 @@ def closed_form_map(vperm, hn):
     j = np.where(R & ~act)[0]
     out[j] = cid2[vperm[minpos2[j]]]
     return out.astype(np.uint32), nc1 + int(cf2.sum()), rounds
+
+
+def closed_form_prefilled(vcmap0, hn, nc0):
+    """The same closed form for parallel_map_construct_prefilled
+    (heuristics.h:378-442): positions are vertex ids, the walkers are the
+    vertices without a prefilled id (in id order), a prefilled vertex never
+    walks and a walker whose heavy neighbour holds a prefilled id joins it,
+    in pass 1 through the hn-walk inside E and in pass 2 as any assigned
+    target; new ids continue from nc0."""
+    hn = np.asarray(hn, dtype=np.int64)
+    n = len(hn)
+    ar = np.arange(n)
+    vc0 = np.asarray(vcmap0, dtype=np.int64)
+    pre = vc0 != INF
+    inE = (~pre) & (ar >= hn)
+    minpos = np.full(n, INF, np.int64)
+    w = np.where(inE & (hn != ar))[0]
+    np.minimum.at(minpos, hn[w], w)
+    creator = inE & ((hn == ar) | ((~inE[hn]) & (~pre[hn]) & (minpos[hn] == ar)))
+    cf = creator.astype(np.int64)
+    cid = np.cumsum(cf) - cf + nc0
+    nc1 = int(cf.sum())
+    vc = vc0.copy()
+    cr = np.where(creator)[0]
+    vc[cr] = cid[cr]
+    vc[hn[cr]] = cid[cr]
+    for u in np.where(inE & ~creator)[0]:
+        x = u
+        while (not creator[x]) and inE[hn[x]]:
+            x = hn[x]
+        if creator[x]:
+            vc[u] = cid[x]
+        elif pre[hn[x]]:
+            vc[u] = vc0[hn[x]]
+        else:
+            vc[u] = cid[minpos[hn[x]]]
+    R = vc == INF
+    state = np.zeros(n, np.int64)
+    rw = np.where(R)[0]
+    rpar = rw[R[hn[rw]]]
+    indeg = np.zeros(n, np.int64)
+    np.add.at(indeg, hn[rpar], 1)
+    while True:
+        has_act = np.zeros(n, bool)
+        nonact = np.zeros(n, np.int64)
+        a = rpar[state[rpar] == 1]
+        has_act[hn[a]] = True
+        b = rpar[state[rpar] == 2]
+        np.add.at(nonact, hn[b], 1)
+        new = state.copy()
+        und = R & (state == 0)
+        new[und & has_act] = 2
+        new[und & ~has_act & (nonact == indeg)] = 1
+        if (new == state).all():
+            break
+        state = new
+    act = R & (state == 1)
+    minpos2 = np.full(n, INF, np.int64)
+    a = np.where(act & R[hn])[0]
+    np.minimum.at(minpos2, hn[a], a)
+    cr2 = act & R[hn] & (minpos2[hn] == ar)
+    cf2 = cr2.astype(np.int64)
+    cid2 = np.cumsum(cf2) - cf2 + nc0 + nc1
+    out = vc.copy()
+    j = np.where(act & ~R[hn])[0]
+    out[j] = vc[hn[j]]
+    j = np.where(act & R[hn])[0]
+    out[j] = cid2[minpos2[hn[j]]]
+    j = np.where(R & ~act)[0]
+    out[j] = cid2[minpos2[j]]
+    return out.astype(np.uint32), nc0 + nc1 + int(cf2.sum())
