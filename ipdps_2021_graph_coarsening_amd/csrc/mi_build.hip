@@ -38,21 +38,26 @@ namespace {
 inline unsigned grid1(uint64_t n) { return (unsigned)ceil_div(n ? n : 1, BLOCK); }
 
 // one lane per fine entry; the row comes from the row index.  The value is
-// the entry's weight, or (eidx) its index among the fine entries.
+// the entry's weight, or (eidx) its index among the fine entries; no values
+// (vals null) for unit weights, whose merged weight is the run's length, or
+// when the weight is packed into the key's low WB bits.
 template <class K>
 __global__ void __launch_bounds__(BLOCK) k_emit_edges(uint64_t m, const uint32_t* __restrict__ rowptr,
                                                       const uint32_t* __restrict__ roc,
                                                       const uint32_t* __restrict__ col,
                                                       const uint32_t* __restrict__ w,
                                                       const uint32_t* __restrict__ vcmap, uint32_t nc, int B,
-                                                      bool eidx, K* __restrict__ keys, uint32_t* __restrict__ vals)
+                                                      bool eidx, int WB, K* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals)
 {
     const uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e >= m) return;
     const uint32_t r = entry_row((uint32_t)e, roc, rowptr);
     const uint32_t cu = vcmap[r], cv = vcmap[col[e]];
-    keys[e] = (cu == cv) ? ((K)nc << B) : (((K)cu << B) | (K)cv);
-    vals[e] = eidx ? (uint32_t)e : (w ? w[e] : 1u);
+    K key = (cu == cv) ? ((K)nc << B) : (((K)cu << B) | (K)cv);
+    if (WB) key = (key << WB) | (K)(cu == cv ? 0u : w[e]);  // the weight rides below the key
+    keys[e] = key;
+    if (vals) vals[e] = eidx ? (uint32_t)e : (w ? w[e] : 1u);
 }
 
 __global__ void k_cluster_weights(uint32_t n, const uint32_t* __restrict__ vcmap, const uint32_t* __restrict__ vtx_w,
@@ -84,31 +89,32 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_weights_lds(uint32_t n, uint3
 }
 
 template <class K>
-__global__ void k_heads(uint64_t m, const K* __restrict__ keys, K sentinel, uint32_t* __restrict__ head)
+__global__ void k_heads(uint64_t m, const K* __restrict__ keys, K sentinel, int WB, uint32_t* __restrict__ head)
 {
     uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (k >= m) return;
-    const K key = keys[k];
-    head[k] = (key < sentinel && (k == 0 || keys[k - 1] != key)) ? 1u : 0u;
+    const K key = keys[k] >> WB;
+    head[k] = (key < sentinel && (k == 0 || (keys[k - 1] >> WB) != key)) ? 1u : 0u;
 }
 
 template <class K>
 __global__ void __launch_bounds__(BLOCK) k_write_coarse(uint64_t m, const K* __restrict__ keys,
                                                         const uint32_t* __restrict__ vals,
                                                         const uint32_t* __restrict__ uidx, K sentinel, int B,
-                                                        uint32_t* __restrict__ col_c, uint32_t* __restrict__ w_c,
-                                                        uint32_t* __restrict__ rowcnt)
+                                                        int WB, uint32_t* __restrict__ col_c,
+                                                        uint32_t* __restrict__ w_c, uint32_t* __restrict__ rowcnt)
 {
     const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t grp = INF, row = INF, v = 0, one = 0;
     bool head = false;
     if (k < m) {
-        const K key = keys[k];
+        const K full = keys[k];
+        const K key = full >> WB;
         if (key < sentinel) {
-            head = (k == 0 || keys[k - 1] != key);
+            head = (k == 0 || (keys[k - 1] >> WB) != key);
             grp = head ? uidx[k] : uidx[k] - 1u;
             row = (uint32_t)(key >> B);
-            v = vals[k];
+            v = vals ? vals[k] : WB ? (uint32_t)(full & (((K)1 << WB) - (K)1)) : 1u;
             one = head ? 1u : 0u;
             if (head) col_c[grp] = (uint32_t)(key & (((K)1 << B) - (K)1));
         }
@@ -881,17 +887,18 @@ __global__ void k_copy_rows(uint32_t nc, const uint32_t* __restrict__ rowptr, co
 // ends in order of first occurrence instead.
 template <class K>
 static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m, uint32_t nc, int B, DGraph& out,
-                        bool first_order = false, const uint32_t* fw = nullptr)
+                        bool first_order = false, const uint32_t* fw = nullptr, int WB = 0)
 {
     const K sentinel = (K)nc << B;
     DBuf<K> ktmp(&c.pool, m);
-    DBuf<uint32_t> vtmp(&c.pool, m);
+    DBuf<uint32_t> vtmp;
+    if (vals.get()) vtmp = DBuf<uint32_t>(&c.pool, m);
     {
         K* kp = keys.get();
         uint32_t* vp = vals.get();
         K* kt = ktmp.get();
         uint32_t* vt = vtmp.get();
-        radix_sort_pairs_ping(c, kp, vp, m, 0, 2 * B, kt, vt);
+        radix_sort_pairs_ping(c, kp, vp, m, WB, WB + 2 * B, kt, vt);
         if (kp != keys.get()) {  // the sorted pairs are in the second buffers
             std::swap(keys, ktmp);
             std::swap(vals, vtmp);
@@ -903,7 +910,7 @@ static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m,
     DBuf<uint32_t> uidx(&c.pool, m);
     DBuf<uint32_t> total(&c.pool, 2);
     if (m) {
-        k_heads<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), sentinel, uidx.get());
+        k_heads<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), sentinel, WB, uidx.get());
         MI_HIP(hipGetLastError());
     }
     scan_exclusive_u32(c, uidx.get(), uidx.get(), m, total.get());
@@ -921,7 +928,7 @@ static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m,
     MI_HIP(hipMemsetAsync(rowcnt.get(), 0, ((size_t)nc + 1) * 4, c.stream));
     if (mc) MI_HIP(hipMemsetAsync(out.w.get(), 0, (size_t)mc * 4, c.stream));
     if (m && !first_order) {
-        k_write_coarse<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), vals.get(), uidx.get(), sentinel, B,
+        k_write_coarse<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, keys.get(), vals.get(), uidx.get(), sentinel, B, WB,
                                                             out.col.get(), out.w.get(), rowcnt.get());
         MI_HIP(hipGetLastError());
     }
@@ -951,19 +958,49 @@ static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m,
     scan_exclusive_u32(c, rowcnt.get(), out.rowptr.get(), (uint64_t)nc + 1, nullptr);
 }
 
+// the largest weight of a level (one atomic per wavefront)
+__global__ void k_wmax(uint64_t m, const uint32_t* __restrict__ w, uint32_t* __restrict__ out)
+{
+    uint32_t v = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < m; e += (uint64_t)gridDim.x * BLOCK)
+        v = w[e] > v ? w[e] : v;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(v, o, WAVE);
+        v = x > v ? x : v;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0 && v) atomicMax(out, v);
+}
+
+// The sort moves (key, weight) pairs, or keys alone: for unit weights, and
+// for 64-bit keys with room below the 2B key bits for the level's largest
+// weight, which then rides in the key's low bits (8 bytes per entry and pass
+// instead of 12; equal keys still sum their weights, in any order, exactly).
 template <class K>
 static void build_coarse_impl(Ctx& c, const DGraph& g, const uint32_t* roc, const uint32_t* vcmap, uint32_t nc,
                               int B, DGraph& out)
 {
     const uint64_t m = g.m;
+    int WB = 0;
+    if (sizeof(K) == 8 && g.W() && !c.hashmap && m) {
+        DBuf<uint32_t> mx(&c.pool, 1);
+        fill_u32(c, mx.get(), 1, 0u);
+        k_wmax<<<(unsigned)std::min<uint64_t>(ceil_div(m, BLOCK), 4096), BLOCK, 0, c.stream>>>(m, g.W(), mx.get());
+        MI_HIP(hipGetLastError());
+        uint32_t wmax = 0;
+        read_words(c, mx.get(), &wmax, 4);
+        const int wb = bits_for(wmax);
+        if (2 * B + wb <= 64) WB = wb;
+    }
     DBuf<K> keys(&c.pool, m);
-    DBuf<uint32_t> vals(&c.pool, m);
+    DBuf<uint32_t> vals;  // unit or packed weights: the keys alone
+    if ((g.W() && !WB) || c.hashmap) vals = DBuf<uint32_t>(&c.pool, m);
     if (m) {
-        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, c.hashmap,
+        k_emit_edges<K><<<grid1(m), BLOCK, 0, c.stream>>>(m, g.R(), roc, g.Cc(), g.W(), vcmap, nc, B, c.hashmap, WB,
                                                           keys.get(), vals.get());
         MI_HIP(hipGetLastError());
     }
-    keys_to_csr<K>(c, keys, vals, m, nc, B, out, c.hashmap, g.W());
+    keys_to_csr<K>(c, keys, vals, m, nc, B, out, c.hashmap, g.W(), WB);
 }
 
 namespace {

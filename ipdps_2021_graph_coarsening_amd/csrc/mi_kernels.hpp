@@ -81,7 +81,8 @@ void scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, uint64_t n,
                         uint32_t* total_dev);
 void iota_u32(Ctx& c, uint32_t* out, uint64_t n, uint32_t start);
 void fill_u32(Ctx& c, uint32_t* p, uint64_t n, uint32_t v);
-// Stable LSD radix sort of (key, value) pairs on key bits [begin_bit, end_bit).
+// Stable LSD radix sort of (key, value) pairs on key bits [begin_bit, end_bit);
+// null vals (and vals_tmp) sort the keys alone.
 void radix_sort_pairs(Ctx& c, uint64_t* keys, uint32_t* vals, uint64_t n, int begin_bit,
                       int end_bit, uint64_t* keys_tmp, uint32_t* vals_tmp);
 void radix_sort_pairs(Ctx& c, uint32_t* keys, uint32_t* vals, uint64_t n, int begin_bit,

@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down(const uint32_t* in, uint32_
 }
 
 // ---------------------------------------------------------------- radix
-// Stable LSD radix sort of (key, value) pairs.  A pass sorts on a digit of
+// Stable LSD radix sort of (key, value) pairs (or of keys alone when the
+// values are null).  A pass sorts on a digit of
 // up to DB bits (DB = 8: 256 threads, or 9: 512 threads; one digit per
 // thread in the tile bookkeeping) over tiles of 16 items per thread.  Per
 // pass: a per-tile digit histogram, one scan over (digit, tile), then the
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_hist(const K*
     hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-template <class K, int DB>
+// HV: pairs; without it the keys alone (vals_in / vals_out unused)
+template <class K, int DB, bool HV>
 __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint64_t n, int shift, uint32_t mask, uint32_t num_tiles,
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
     using S = RadixShape<DB>;
     constexpr int RADIX = S::RADIX, WAVES = S::WAVES, TILE = S::TILE;
     __shared__ K skey[TILE];
-    __shared__ uint32_t sval[TILE];
+    __shared__ uint32_t sval[HV ? TILE : 1];
     __shared__ uint32_t wrun[WAVES][RADIX];  // per wavefront: items of each digit so far
     __shared__ uint32_t lstart[RADIX];       // first LDS slot of each digit
     __shared__ uint32_t gofs[RADIX];         // first output position of each digit
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
         const uint64_t i = wbase + (uint64_t)r * WAVE + lane;
         const bool active = i < n;
         kk[r] = active ? keys_in[i] : (K)0;
-        vv[r] = active ? vals_in[i] : 0u;
+        vv[r] = (HV && active) ? vals_in[i] : 0u;
         dd[r] = active ? ((uint32_t)(kk[r] >> shift) & mask) : (uint32_t)RADIX;
     }
 #pragma unroll
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
         if (d < (uint32_t)RADIX) {
             const uint32_t p = lstart[d] + wrun[wid][d] + rk[r];
             skey[p] = kk[r];
-            sval[p] = vv[r];
+            if (HV) sval[p] = vv[r];
         }
     }
     __syncthreads();
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(RadixShape<DB>::THREADS) k_radix_scatter(
         const uint32_t d = (uint32_t)(key >> shift) & mask;
         const uint32_t gp = gofs[d] + (p - lstart[d]);
         keys_out[gp] = key;
-        vals_out[gp] = sval[p];
+        if (HV) vals_out[gp] = sval[p];
     }
 }
 
@@ -351,7 +353,11 @@ static void radix_pass(Ctx& c, const K* ksrc, const uint32_t* vsrc, K* kdst, uin
     k_radix_hist<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, n, shift, mask, (uint32_t)tiles, h);
     MI_HIP(hipGetLastError());
     scan_impl(c, h, h, tiles * S::RADIX, nullptr, ws, ws_words);
-    k_radix_scatter<K, DB><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
+    if (vsrc)
+        k_radix_scatter<K, DB, true><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
+                                                                        (uint32_t)tiles, h);
+    else
+        k_radix_scatter<K, DB, false><<<(unsigned)tiles, S::THREADS, 0, c.stream>>>(ksrc, vsrc, kdst, vdst, n, shift, mask,
                                                                         (uint32_t)tiles, h);
     MI_HIP(hipGetLastError());
 }
@@ -374,7 +380,7 @@ __global__ void __launch_bounds__(1024) k_sort_small(K* __restrict__ keys, uint3
     for (uint32_t i = tid; i < np2; i += blockDim.x) {
         const bool in = i < n;
         sk[i] = in ? keys[i] : ~(K)0;
-        sv[i] = in ? vals[i] : 0u;
+        sv[i] = (in && vals) ? vals[i] : 0u;
         si[i] = in ? i : 0xffffffffu;
     }
     __syncthreads();
@@ -403,7 +409,7 @@ __global__ void __launch_bounds__(1024) k_sort_small(K* __restrict__ keys, uint3
     }
     for (uint32_t i = tid; i < n; i += blockDim.x) {
         keys[i] = sk[i];
-        vals[i] = sv[i];
+        if (vals) vals[i] = sv[i];
     }
 }
 
@@ -425,7 +431,7 @@ __global__ void __launch_bounds__(1024) k_sort_small_packed(K* __restrict__ keys
         if (i < n) {
             const K k = keys[i];
             sk[i] = k;
-            sv[i] = vals[i];
+            sv[i] = vals ? vals[i] : 0u;
             sc[i] = ((((uint64_t)k >> begin_bit) & mask) << 12) | i;
         } else {
             sc[i] = (~(uint64_t)0 << 12) | i;
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(1024) k_sort_small_packed(K* __restrict__ keys
     for (uint32_t i = tid; i < n; i += blockDim.x) {
         const uint32_t src = (uint32_t)(sc[i] & 0xfffu);
         keys[i] = sk[src];
-        vals[i] = sv[src];
+        if (vals) vals[i] = sv[src];
     }
 }
 
@@ -500,7 +506,7 @@ static void radix_sort_impl(Ctx& c, K* keys, uint32_t* vals, uint64_t n, int beg
     radix_sort_ping<K>(c, k, v, n, begin_bit, end_bit, keys_tmp, vals_tmp);
     if (k != keys) {
         MI_HIP(hipMemcpyAsync(keys, k, n * sizeof(K), hipMemcpyDeviceToDevice, c.stream));
-        MI_HIP(hipMemcpyAsync(vals, v, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
+        if (vals) MI_HIP(hipMemcpyAsync(vals, v, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
     }
 }
 
