@@ -107,6 +107,7 @@ __device__ __forceinline__ uint32_t logical_block()
 // read all of them from device memory (boff_all / bwave_all, ncb + 1 each).
 struct Blocks {
     uint32_t ncb, pw, m;
+    uint32_t pws;  // log2(pw): range indices by a shift (pw is a power of two)
     uint32_t boff[KB + 1];
     uint32_t bwave[KB + 1];
     const uint32_t* boff_all;
@@ -119,6 +120,7 @@ Blocks blocks_of(const LevelPlan& p)
     Blocks B;
     B.ncb = p.ncb;
     B.pw = p.pw;
+    B.pws = (uint32_t)__builtin_ctz(p.pw);
     B.m = p.boff[p.ncb > 1 ? p.ncb : 1];
     for (int q = 0; q <= KB; q++) {
         B.boff[q] = p.boff[q];
@@ -149,7 +151,7 @@ __device__ __forceinline__ uint32_t range_of(const Blocks& B, uint32_t e)
 {
     if (B.ncb > KB) {
         const uint32_t q = last_le(B.boff_all, B.ncb, e);
-        return B.bwave_all[q] + (e - B.boff_all[q]) / B.pw;
+        return B.bwave_all[q] + ((e - B.boff_all[q]) >> B.pws);
     }
     uint32_t off = B.boff[0], wb = B.bwave[0];
 #pragma unroll
@@ -159,7 +161,7 @@ __device__ __forceinline__ uint32_t range_of(const Blocks& B, uint32_t e)
             wb = B.bwave[q];
         }
     }
-    return wb + (e - off) / B.pw;
+    return wb + ((e - off) >> B.pws);
 }
 
 // ------------------------------------------------------------------ plan
@@ -1159,8 +1161,8 @@ __device__ __forceinline__ void row_blocks(const StepArgs& A, uint32_t r, uint32
             const uint32_t q = q0 + k;
             const uint32_t bw0 = ALL ? A.B.bwave_all[q] : A.B.bwave[k];
             const uint32_t bo0 = ALL ? A.B.boff_all[q] : A.B.boff[k];
-            w0[k] = bw0 + (s0[k] - bo0) / A.B.pw;
-            w1[k] = bw0 + (s1[k] - 1 - bo0) / A.B.pw;
+            w0[k] = bw0 + ((s0[k] - bo0) >> A.B.pws);
+            w1[k] = bw0 + ((s1[k] - 1 - bo0) >> A.B.pws);
             first[k] = w0[k] == w1[k] ? part[q * A.n + r] : tail[w0[k]];
         }
     }
@@ -1256,8 +1258,8 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
             uint32_t w0[RB], w1[RB];
 #pragma unroll
             for (int k = 0; k < RB; k++) {
-                w0[k] = A.B.bwave[0] + (s0[k] - A.B.boff[0]) / A.B.pw;
-                w1[k] = A.B.bwave[0] + (s1[k] - 1 - A.B.boff[0]) / A.B.pw;
+                w0[k] = A.B.bwave[0] + ((s0[k] - A.B.boff[0]) >> A.B.pws);
+                w1[k] = A.B.bwave[0] + ((s1[k] - 1 - A.B.boff[0]) >> A.B.pws);
                 first[k] = 0;
                 if (s0[k] != s1[k]) first[k] = w0[k] == w1[k] ? part[r + k * stride] : tail[w0[k]];
             }
@@ -1482,8 +1484,8 @@ __device__ __forceinline__ void small_loop(const StepArgs& A, double* buf0, doub
             const uint32_t s0 = A.rowptr[r], s1 = A.rowptr[r + 1];
             has[k] = s0 != s1;
             if (has[k]) {
-                w0[k] = s0 / A.B.pw;
-                w1[k] = (s1 - 1) / A.B.pw;
+                w0[k] = s0 >> A.B.pws;
+                w1[k] = (s1 - 1) >> A.B.pws;
             }
             wd[k] = A.wdeg ? A.wdeg[r] : 0u;
         }
@@ -2193,7 +2195,8 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
     // not depend on the device): two passes on large levels, one on small ones.
     // the streaming kernels keep entry positions in 32 bits
     MI_CHECK(m <= 0xffff0000u, ST_ARG, "level too large for 32-bit entry positions");
-    p.pw = m >= (1u << 22) ? 2 * PASS : PASS;
+    p.pw = m >= (1u << 22) ? 2 * PASS : PASS;  // a power of two (range indices by shifts)
+    static_assert((PASS & (PASS - 1)) == 0, "PASS must be a power of two");
     const uint32_t* rows = g.R();  // CSR the row ids and the finish kernel follow
     uint32_t nrows = n;
     if (ncb >= 2) {
