@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down(const uint32_t* in, uint32_
 // ---------------------------------------------------------------- radix
 // Stable LSD radix sort of (key, value) pairs (or of keys alone when the
 // values are null).  A pass sorts on a digit of
-// up to DB bits (DB = 8: 256 threads, or 9: 512 threads; one digit per
+// up to DB bits (DB = 8: 256 threads; one digit per
 // thread in the tile bookkeeping) over tiles of 16 items per thread.  Per
 // pass: a per-tile digit histogram, one scan over (digit, tile), then the
 // scatter.  In the scatter each wavefront ranks its own contiguous share of
@@ -458,8 +458,8 @@ __global__ void __launch_bounds__(1024) k_sort_small_packed(K* __restrict__ keys
     }
 }
 
-// Sorts on bits [begin_bit, end_bit) in the fewest passes of at most 9 bits
-// (8-bit passes when that takes no more of them), the widths split evenly.
+// Sorts on bits [begin_bit, end_bit) in passes of at most 8 bits, the widths
+// split evenly.
 // The result ends in whichever buffer pair the last pass wrote: on return
 // keys / vals point to it and keys_tmp / vals_tmp to the other pair.
 template <class K>
@@ -480,17 +480,15 @@ static void radix_sort_ping(Ctx& c, K*& keys, uint32_t*& vals, uint64_t n, int b
         MI_HIP(hipGetLastError());
         return;
     }
-    const int p8 = (total + 7) / 8, p9 = (total + 8) / 9;
-    const bool wide = p9 < p8;
-    const int passes = wide ? p9 : p8;
+    // 8-bit passes only: on the skewed keys of coarse builds a 9-bit pass
+    // (512 digits per tile) costs about twice an 8-bit one (R-MAT-24: 5.7 ms
+    // for 436M entries against 3.4 ms for 520M), more than the pass it saves
+    const int passes = (total + 7) / 8;
     DBuf<uint32_t> hist;
     int shift = begin_bit;
     for (int q = 0; q < passes; q++) {
         const int bits = (total - (shift - begin_bit) + (passes - q) - 1) / (passes - q);  // even split
-        if (wide)
-            radix_pass<K, 9>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist, ws, ws_words);
-        else
-            radix_pass<K, 8>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist, ws, ws_words);
+        radix_pass<K, 8>(c, keys, vals, keys_tmp, vals_tmp, n, shift, bits, hist, ws, ws_words);
         std::swap(keys, keys_tmp);
         std::swap(vals, vals_tmp);
         shift += bits;
@@ -544,10 +542,8 @@ void radix_sort_pairs_ws(Ctx& c, uint64_t*& keys, uint32_t*& vals, uint64_t n, i
 uint64_t radix_sort_ws_words(uint64_t n, int nbits)
 {
     if (n <= SMALL_SORT || nbits <= 0) return 0;
-    const int p8 = (nbits + 7) / 8, p9 = (nbits + 8) / 9;
-    const bool wide = p9 < p8;  // as radix_sort_ping chooses
-    const uint64_t h = wide ? ceil_div(n, RadixShape<9>::TILE) * RadixShape<9>::RADIX
-                            : ceil_div(n, RadixShape<8>::TILE) * RadixShape<8>::RADIX;
+    // 8-bit passes, as radix_sort_ping takes them
+    const uint64_t h = ceil_div(n, RadixShape<8>::TILE) * RadixShape<8>::RADIX;
     return h + scan_ws_words(h);
 }
 
