@@ -198,7 +198,15 @@ int mi_ctx_set_refinement(mi_ctx* ctx, int fm);
  * neighbours), then the remaining vertices choose a heavy neighbour by
  * (weight, XorShift64 draw of the level's seed, one per entry) and go
  * through parallel_map_construct_prefilled (:378-442) -- a remaining
- * vertex without entries gives MI_STATUS_ARG.  With variants 5 to 7
+ * vertex without entries gives MI_STATUS_ARG; variant 8 the serial HEC of
+ * hec_spec_serial (-DHEC_SERIAL, cmg/CMakeLists.txt:16,40;
+ * sgp_coarsen_HEC_serial, heuristics.h:828-928): the list is a
+ * Fisher-Yates shuffle by the XorShift64 pool of the level's seed (step i =
+ * n .. 2 swaps positions draw % i and i - 1), the heavy neighbour a random
+ * entry at level 1 (the next n draws) and the first heaviest entry above
+ * it, and the map the greedy walk in list order (an unmapped vertex makes a
+ * new id for its heavy neighbour when that one is unmapped, else joins it)
+ * -- a vertex without entries gives MI_STATUS_ARG.  With variants 5 to 7
  * mi_hec_level leaves vperm and hn untouched.  Other values:
  * MI_STATUS_ARG.
  */

@@ -237,7 +237,8 @@ class Context:
         binaries' (parallel_map_construct_v2),
         3 the hec3_* binaries' (parallel_map_construct_v3), 5 the mis_*
         binaries' (MIS-2 aggregates), 6 the gosh_* binaries' (GOSH centres), 7 the
-        gosh2_* binaries' (GOSH v2)."""
+        gosh2_* binaries' (GOSH v2), 8 hec_spec_serial's (the serial HEC:
+        a Fisher-Yates shuffle and a greedy map in list order)."""
         self.check(self._L.mi_ctx_set_coarsening(self.h, variant), "mi_ctx_set_coarsening")
 
     def set_builder(self, builder: int) -> None:

@@ -70,7 +70,8 @@ DROPIN = os.path.join(PKG, "dropin")
 DROPIN_LIB = os.path.join(PKG, "libsgpar_dropin.so")
 DROPIN_EXE = os.path.join(DROPIN, "sgpar_dropin")
 # the same compiled with -DFM, as the reference's *_fm binaries are, and with
-# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH, -DHEC_V2, -DGOSH_V2, -DHASHMAP, -DPUREMATCH and -DMTMETIS likewise
+# -DHEC_V3 (the hec3_* binaries); -DMIS, -DGOSH, -DHEC_V2, -DGOSH_V2, -DHASHMAP, -DPUREMATCH, -DMTMETIS and
+# -DHEC_SERIAL likewise
 DROPIN_FM_LIB = os.path.join(PKG, "libsgpar_dropin_fm.so")
 DROPIN_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_fm")
 DROPIN_HEC3_LIB = os.path.join(PKG, "libsgpar_dropin_hec3.so")
@@ -95,6 +96,8 @@ DROPIN_HEC2_FM_LIB = os.path.join(PKG, "libsgpar_dropin_hec2_fm.so")
 DROPIN_HEC2_FM_EXE = os.path.join(DROPIN, "sgpar_dropin_hec2_fm")
 DROPIN_GOSH2_LIB = os.path.join(PKG, "libsgpar_dropin_gosh2.so")
 DROPIN_GOSH2_EXE = os.path.join(DROPIN, "sgpar_dropin_gosh2")
+DROPIN_SERIAL_LIB = os.path.join(PKG, "libsgpar_dropin_serial.so")
+DROPIN_SERIAL_EXE = os.path.join(DROPIN, "sgpar_dropin_serial")
 REF_CMG = os.environ.get("SGPAR_REF_CMG", "/root/reference/cmg")
 
 
@@ -120,7 +123,8 @@ def build_dropin(verbose: bool = False, ref_cmg: str = REF_CMG) -> bool:
                             (DROPIN_MTMETIS_LIB, DROPIN_MTMETIS_EXE, ["-DMTMETIS"]),
                             (DROPIN_HEC2_LIB, DROPIN_HEC2_EXE, ["-DHEC_V2"]),
                             (DROPIN_HEC2_FM_LIB, DROPIN_HEC2_FM_EXE, ["-DHEC_V2", "-DFM"]),
-                            (DROPIN_GOSH2_LIB, DROPIN_GOSH2_EXE, ["-DGOSH_V2"])):
+                            (DROPIN_GOSH2_LIB, DROPIN_GOSH2_EXE, ["-DGOSH_V2"]),
+                            (DROPIN_SERIAL_LIB, DROPIN_SERIAL_EXE, ["-DHEC_SERIAL"])):
         tmp = lib + ".tmp"
         # -Bsymbolic: calls inside the shim bind to its own definitions, not
         # to copies a caller's translation unit of sgpar.h may define

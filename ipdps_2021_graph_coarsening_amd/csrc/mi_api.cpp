@@ -480,7 +480,7 @@ int mi_ctx_set_bucketed_build(mi_ctx* ctx, uint32_t min_entries)
 
 int mi_ctx_set_coarsening(mi_ctx* ctx, int variant)
 {
-    if (!ctx || variant < 0 || variant > 7) return ST_ARG;
+    if (!ctx || variant < 0 || variant > 8) return ST_ARG;
     ctx->c.coarsen = variant;
     return ST_OK;
 }
@@ -1289,7 +1289,7 @@ int mi_hec_level(mi_ctx* ctx, const mi_graph* g, int level, uint64_t seed, uint3
         DBuf<uint32_t> roc;
         build_row_index(c, g->g, roc);
         // (MIS-2 and GOSH coarsening have no permutation or heavy neighbours: left untouched)
-        const bool mis = c.coarsen >= 5;
+        const bool mis = c.coarsen >= 5 && c.coarsen <= 7;
         *nc = hec_map(c, g->g, roc.get(), level, seed, dv.get(), mis ? nullptr : dp.get(), mis ? nullptr : dh.get(),
                       nullptr);
         d2h(c, vcmap, dv.get(), (size_t)n * 4);

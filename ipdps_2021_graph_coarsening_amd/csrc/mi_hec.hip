@@ -1184,6 +1184,182 @@ __global__ void k_rel_keep(uint32_t len, const uint32_t* __restrict__ list,
     if (x) hn[u] = (uint32_t)x;
 }
 
+// ---- the serial HEC of hec_spec_serial (sgp_coarsen_HEC_serial,
+// heuristics.h:828-928).  The Fisher-Yates shuffle (step i = n .. 2 swaps
+// position j_i = draw % i with i - 1) without the walk: step i moves to
+// position i - 1 the value at j_i, which the latest earlier step writing j_i
+// (the smallest i' > i with j_i' = j_i) put there as X_i', or j_i itself;
+// X_i, the value at i - 1 before step i, follows the same rule -- chains
+// that only go to larger steps.  The greedy map: u acts (is unmapped at its
+// turn) iff no acting vertex before it names it; an acting u makes a new id
+// when its heavy neighbour is unmapped at its turn, else joins it.
+__global__ void k_hs_steps(uint32_t n, const uint64_t* __restrict__ draws, uint32_t* __restrict__ jpos,
+                           uint32_t* __restrict__ step)
+{
+    const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;  // step i = k + 2
+    if (k + 1 >= n) return;
+    const uint32_t i = k + 2;
+    jpos[k] = (uint32_t)(draws[n - i] % i);
+    step[k] = i;
+}
+
+__global__ void k_hs_count(uint32_t ns, const uint32_t* __restrict__ jsorted, uint32_t* __restrict__ cnt)
+{
+    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < ns) atomicAdd(&cnt[jsorted[t]], 1u);
+}
+
+// the smallest step > i among the steps writing position p (ascending in
+// S[gs[p] .. gs[p+1])), or INF
+__device__ __forceinline__ uint32_t hs_next(const uint32_t* S, const uint32_t* gs, uint32_t p, uint32_t i)
+{
+    uint32_t lo = gs[p], hi = gs[p + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S[mid] <= i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < gs[p + 1] ? S[lo] : INF;
+}
+
+// qx[i]: the step whose value X_i copies (INF: position i - 1 untouched)
+__global__ void k_hs_qx(uint32_t n, const uint32_t* __restrict__ S, const uint32_t* __restrict__ gs,
+                        uint32_t* __restrict__ qx)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x + 2;
+    if (i <= n) qx[i] = hs_next(S, gs, i - 1, i);
+}
+
+__device__ __forceinline__ uint32_t hs_x(uint32_t i, uint32_t n, const uint32_t* qx)
+{
+    for (uint32_t k = 0; k <= n; k++) {
+        const uint32_t q = qx[i];
+        if (q == INF) return i - 1;
+        i = q;
+    }
+    return INF;
+}
+
+__global__ void k_hs_perm(uint32_t n, const uint64_t* __restrict__ draws, const uint32_t* __restrict__ S,
+                          const uint32_t* __restrict__ gs, const uint32_t* __restrict__ qx,
+                          uint32_t* __restrict__ vperm)
+{
+    const uint32_t p = blockIdx.x * BLOCK + threadIdx.x;  // position
+    if (p >= n) return;
+    // position p >= 1 is final after step p + 1, which takes the value at j;
+    // position 0 ends with the value the last step writing it left there
+    const uint32_t i = p >= 1 ? p + 1 : 1;
+    const uint32_t j = p >= 1 ? (uint32_t)(draws[n - i] % i) : 0u;
+    const uint32_t q = hs_next(S, gs, j, i);
+    vperm[p] = q != INF ? hs_x(q, n, qx) : j;
+}
+
+// level 1: the random neighbour from draw n - 1 + i
+__global__ void k_hs_hn_random(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                               const uint64_t* __restrict__ draws, uint32_t* __restrict__ hn)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = rowptr[i], d = rowptr[i + 1] - b;
+    if (d) hn[i] = col[b + (uint32_t)(draws[(uint64_t)n - 1 + i] % d)];
+}
+
+// namers that come first: indeg[u] = #{w != u : hn(w) = u, pos(w) < pos(u)}
+__global__ void k_hs_indeg(uint32_t n, const uint32_t* __restrict__ hn, const uint32_t* __restrict__ pos,
+                           uint32_t* __restrict__ indeg)
+{
+    const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
+    if (w >= n) return;
+    const uint32_t u = hn[w];
+    if (u != w && pos[w] < pos[u]) atomicAdd(&indeg[u], 1u);
+}
+
+// st: 0 undecided, 1 acts, 2 does not (named by an earlier acting vertex)
+__global__ void k_hs_resolve(uint32_t n, uint8_t* st, const uint32_t* __restrict__ indeg,
+                             const uint32_t* __restrict__ nonact, const uint32_t* __restrict__ hasact,
+                             uint32_t* __restrict__ left)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n || st[u] != 0) return;
+    if (hasact[u])
+        st[u] = 2;
+    else if (nonact[u] == indeg[u])
+        st[u] = 1;
+    else
+        atomicAdd(left, 1u);
+}
+
+__global__ void k_hs_propagate(uint32_t n, const uint32_t* __restrict__ hn, const uint32_t* __restrict__ pos,
+                               const uint8_t* __restrict__ st, uint8_t* __restrict__ sent, uint32_t* nonact,
+                               uint32_t* hasact)
+{
+    const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
+    if (w >= n || st[w] == 0 || sent[w]) return;
+    sent[w] = 1;
+    const uint32_t u = hn[w];
+    if (u == w || pos[w] >= pos[u]) return;
+    if (st[w] == 1)
+        hasact[u] = 1u;
+    else
+        atomicAdd(&nonact[u], 1u);
+}
+
+// first[v]: the smallest position of an acting namer of v
+__global__ void k_hs_first(uint32_t n, const uint32_t* __restrict__ hn, const uint32_t* __restrict__ pos,
+                           const uint8_t* __restrict__ st, uint32_t* __restrict__ first)
+{
+    const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
+    if (w < n && st[w] == 1 && hn[w] != w) atomicMin(&first[hn[w]], pos[w]);
+}
+
+__device__ __forceinline__ bool hs_creator(uint32_t u, const uint32_t* hn, const uint32_t* pos, const uint8_t* st,
+                                           const uint32_t* first)
+{
+    if (st[u] != 1) return false;
+    const uint32_t v = hn[u];
+    if (v == u) return true;
+    return first[v] == pos[u] && !(st[v] == 1 && pos[v] < pos[u]);
+}
+
+__global__ void k_hs_cflag(uint32_t n, const uint32_t* __restrict__ vperm, const uint32_t* __restrict__ hn,
+                           const uint32_t* __restrict__ pos, const uint8_t* __restrict__ st,
+                           const uint32_t* __restrict__ first, uint32_t* __restrict__ cflag)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) cflag[i] = hs_creator(vperm[i], hn, pos, st, first) ? 1u : 0u;
+}
+
+__global__ void k_hs_assign(uint32_t n, const uint32_t* __restrict__ vperm, const uint32_t* __restrict__ hn,
+                            const uint32_t* __restrict__ pos, const uint8_t* __restrict__ st,
+                            const uint32_t* __restrict__ first, const uint32_t* __restrict__ cid,
+                            uint32_t* __restrict__ vcmap, uint32_t* __restrict__ err)
+{
+    const uint32_t u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n) return;
+    uint32_t x = u, val = INF;
+    for (uint32_t k = 0; k <= n; k++) {
+        if (st[x] != 1) {  // made an id by its first acting namer
+            val = first[x] != INF ? cid[first[x]] : INF;
+            break;
+        }
+        if (hs_creator(x, hn, pos, st, first)) {
+            val = cid[pos[x]];
+            break;
+        }
+        const uint32_t v = hn[x];
+        if (st[v] == 1 && pos[v] < pos[x]) {
+            x = v;  // v acted first: its id, by the same rules
+            continue;
+        }
+        val = first[v] != INF ? cid[first[v]] : INF;  // v's first acting namer made it
+        break;
+    }
+    if (val == INF) atomicAdd(err, 1u);
+    vcmap[u] = val;
+}
+
 // ---- GOSH v2 coarsening (sgp_coarsen_GOSH_v2, heuristics.h:444-601; the
 // gosh2_* binaries).  Seeds (more entries than max(nnz/n, 50)) take ids in
 // index order; two index-order sweeps that see their own writes hand an
@@ -1732,6 +1908,109 @@ static uint32_t match_rounds(Ctx& c, const DGraph& g, int level, const uint32_t*
 static uint32_t map_closed_form(Ctx& c, uint32_t n, const uint32_t* vperm, const uint32_t* pos, const uint32_t* hn,
                                 uint32_t* vcmap, const uint8_t* pre, uint32_t nc0);
 
+// One level of the serial HEC (hec_spec_serial).
+static uint32_t hec_serial_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64_t seed,
+                               uint32_t* vcmap, uint32_t* vperm_out, uint32_t* hn_out, HecTimes* t)
+{
+    const uint32_t n = g.n;
+    auto t0 = std::chrono::steady_clock::now();
+    DBuf<uint32_t> cnt(&c.pool, 4);
+    {  // a vertex without entries has no heavy neighbour (the reference divides by zero or indexes with INF)
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_match_empty<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), cnt.get());
+        MI_HIP(hipGetLastError());
+        uint32_t ne = 0;
+        read_words(c, cnt.get(), &ne, 4);
+        MI_CHECK(ne == 0, ST_ARG, "serial HEC: a vertex without entries has no heavy neighbour");
+    }
+    const uint64_t ndraw = (uint64_t)(n - 1) + (level == 1 ? n : 0);
+    DBuf<uint64_t> draws(&c.pool, ndraw ? ndraw : 1);
+    xs_draws(c, xs_pool_state(seed), 0, ndraw, draws.get());
+    // the shuffle
+    DBuf<uint32_t> vperm(&c.pool, n), pos(&c.pool, n), hn(&c.pool, n);
+    if (n == 1) {
+        fill_u32(c, vperm.get(), 1, 0u);
+    } else {
+        const uint32_t ns = n - 1;
+        DBuf<uint32_t> jp(&c.pool, ns), st(&c.pool, ns), jt(&c.pool, ns), stt(&c.pool, ns), gc(&c.pool, n + 1),
+            gs(&c.pool, n + 1), qx(&c.pool, n + 1);
+        k_hs_steps<<<grid1(ns), BLOCK, 0, c.stream>>>(n, draws.get(), jp.get(), st.get());
+        MI_HIP(hipGetLastError());
+        int nb = 1;
+        while (nb < 32 && (1ull << nb) <= n) nb++;
+        radix_sort_pairs(c, jp.get(), st.get(), ns, 0, nb, jt.get(), stt.get());  // steps by position, ascending
+        fill_u32(c, gc.get(), n + 1, 0u);
+        k_hs_count<<<grid1(ns), BLOCK, 0, c.stream>>>(ns, jp.get(), gc.get());
+        MI_HIP(hipGetLastError());
+        scan_exclusive_u32(c, gc.get(), gs.get(), n + 1, nullptr);
+        fill_u32(c, qx.get(), n + 1, INF);
+        k_hs_qx<<<grid1(ns), BLOCK, 0, c.stream>>>(n, st.get(), gs.get(), qx.get());
+        MI_HIP(hipGetLastError());
+        k_hs_perm<<<grid1(n), BLOCK, 0, c.stream>>>(n, draws.get(), st.get(), gs.get(), qx.get(), vperm.get());
+        MI_HIP(hipGetLastError());
+    }
+    k_scatter_pos<<<grid1(n), BLOCK, 0, c.stream>>>(vperm.get(), n, pos.get());
+    MI_HIP(hipGetLastError());
+    // heavy neighbours
+    if (level == 1) {
+        k_hs_hn_random<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), draws.get(), hn.get());
+        MI_HIP(hipGetLastError());
+    } else {
+        DBuf<uint32_t> iso(&c.pool, n);
+        DBuf<unsigned long long> best(&c.pool, n);
+        MI_HIP(hipMemsetAsync(best.get(), 0, (size_t)n * 8, c.stream));
+        if (g.m) {
+            k_hn_heavy<<<(unsigned)ceil_div(g.m, BLOCK), BLOCK, 0, c.stream>>>(g.m, g.R(), roc, g.W(), true,
+                                                                              best.get());
+            MI_HIP(hipGetLastError());
+        }
+        k_hn_from_best<<<grid1(n), BLOCK, 0, c.stream>>>(n, g.R(), g.Cc(), best.get(), true, hn.get(), iso.get());
+        MI_HIP(hipGetLastError());
+    }
+    draws.reset();
+    if (vperm_out) MI_HIP(hipMemcpyAsync(vperm_out, vperm.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    if (hn_out) MI_HIP(hipMemcpyAsync(hn_out, hn.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, c.stream));
+    // who acts: rounds over the namers that come first
+    DBuf<uint32_t> z(&c.pool, 3ull * n), first(&c.pool, n), cflag(&c.pool, n), cid(&c.pool, n);
+    DBuf<uint8_t> stt(&c.pool, n), sent(&c.pool, n);
+    fill_u32(c, z.get(), 3ull * n, 0u);
+    uint32_t* const indeg = z.get();
+    uint32_t* const nonact = indeg + n;
+    uint32_t* const hasact = nonact + n;
+    MI_HIP(hipMemsetAsync(stt.get(), 0, n, c.stream));
+    MI_HIP(hipMemsetAsync(sent.get(), 0, n, c.stream));
+    k_hs_indeg<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), pos.get(), indeg);
+    MI_HIP(hipGetLastError());
+    for (uint32_t round = 0;; round++) {
+        MI_CHECK(round <= n + 2, ST_INTERNAL, "serial HEC: acting vertices did not resolve");
+        fill_u32(c, cnt.get(), 1, 0u);
+        k_hs_resolve<<<grid1(n), BLOCK, 0, c.stream>>>(n, stt.get(), indeg, nonact, hasact, cnt.get());
+        MI_HIP(hipGetLastError());
+        k_hs_propagate<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), pos.get(), stt.get(), sent.get(), nonact,
+                                                          hasact);
+        MI_HIP(hipGetLastError());
+        uint32_t left = 0;
+        read_words(c, cnt.get(), &left, 4);
+        if (left == 0) break;
+    }
+    fill_u32(c, first.get(), n, INF);
+    k_hs_first<<<grid1(n), BLOCK, 0, c.stream>>>(n, hn.get(), pos.get(), stt.get(), first.get());
+    MI_HIP(hipGetLastError());
+    k_hs_cflag<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), pos.get(), stt.get(), first.get(),
+                                                 cflag.get());
+    MI_HIP(hipGetLastError());
+    scan_exclusive_u32(c, cflag.get(), cid.get(), n, cnt.get() + 1);
+    fill_u32(c, cnt.get() + 2, 1, 0u);
+    k_hs_assign<<<grid1(n), BLOCK, 0, c.stream>>>(n, vperm.get(), hn.get(), pos.get(), stt.get(), first.get(),
+                                                  cid.get(), vcmap, cnt.get() + 2);
+    MI_HIP(hipGetLastError());
+    uint32_t h[2] = {0, 0};
+    read_words(c, cnt.get() + 1, h, 8);
+    MI_CHECK(h[1] == 0, ST_INTERNAL, "serial HEC: a vertex left without an id");
+    if (t) t->map_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return h[0];
+}
+
 // One joining sweep of GOSH v2 (by_deg: the second one).
 static void g2_join_sweep(Ctx& c, const DGraph& g, int by_deg, uint32_t* vcmap, uint8_t* a0, uint8_t* reach,
                           uint32_t* p, uint32_t* cnt)
@@ -1933,6 +2212,7 @@ uint32_t hec_map(Ctx& c, const DGraph& g, const uint32_t* roc, int level, uint64
 {
     const uint32_t n = g.n;
     MI_CHECK(n > 0, ST_ARG, "empty level");
+    if (c.coarsen == 8) return hec_serial_map(c, g, roc, level, seed, vcmap, vperm_out, hn_out, t);
     if (c.coarsen >= 5) {
         MI_CHECK(!vperm_out && !hn_out, ST_ARG, "MIS-2 and GOSH coarsening have no permutation or heavy neighbours");
         return c.coarsen == 5 ? mis2_map(c, g, seed, vcmap, t)

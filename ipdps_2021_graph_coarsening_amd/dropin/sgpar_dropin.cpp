@@ -39,7 +39,8 @@
 // with -DMTMETIS (the mtmetis_* binaries) with that matching and its passes,
 // with -DMIS (the mis_* binaries) with MIS-2, with -DGOSH (the
 // gosh_* binaries) with GOSH centres, with -DGOSH_V2 (the gosh2_* binaries)
-// with GOSH v2 (mi_ctx_set_coarsening); with -DHASHMAP
+// with GOSH v2, with -DHEC_SERIAL (hec_spec_serial) with the serial HEC
+// (mi_ctx_set_coarsening); with -DHASHMAP
 // (hec_spec_hashmap) its coarse rows keep the HASHMAP build's order
 // (mi_ctx_set_builder).
 // Compiled with -DFM (as the reference's *_fm binaries are,
@@ -79,8 +80,10 @@ mi_ctx* context()
         }
 #endif
 #if defined HEC_V3 || defined HEC_V2 || defined MIS || defined GOSH || defined GOSH_V2 || defined PUREMATCH || \
-    defined MTMETIS
-#if defined HEC_V2
+    defined MTMETIS || defined HEC_SERIAL
+#if defined HEC_SERIAL
+        const int variant = 8;
+#elif defined HEC_V2
         const int variant = 4;
 #elif defined GOSH_V2
         const int variant = 7;

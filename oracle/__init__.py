@@ -119,6 +119,8 @@ class _Core:
         L.orc_coarsen_gosh2.argtypes = [C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p]
         L.orc_map_construct_prefilled.restype = C.c_uint32
         L.orc_map_construct_prefilled.argtypes = [C.c_uint32, u32p, C.c_uint32, u32p, u32p, C.c_uint32]
+        L.orc_coarsen_hec_serial.restype = C.c_uint32
+        L.orc_coarsen_hec_serial.argtypes = [C.c_int, C.c_uint32, u32p, u32p, C.c_void_p, C.c_uint64, u32p, u32p, u32p]
         L.orc_gosh_clusters.restype = C.c_int
         L.orc_gosh_clusters.argtypes = [C.c_uint32, u32p, u32p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]
         L.orc_coarsen_gosh.restype = C.c_uint32
@@ -486,6 +488,21 @@ def map_construct_prefilled(vcmap, lst, hn, nc: int):
     return vc, int(out)
 
 
+def coarsen_hec_serial(level: int, rowptr, col, w, seed: int):
+    """(vcmap, nc, vperm, hn) of one level of hec_spec_serial
+    (sgp_coarsen_HEC_serial, heuristics.h:828-928); nc = None for a vertex
+    without entries."""
+    L = core().L
+    rowptr, col = _csr(rowptr, col)
+    n = len(rowptr) - 1
+    w, wp = _wptr(w)
+    vc = np.empty(n + 1, dtype=np.uint32)
+    vp = np.empty(n + 1, dtype=np.uint32)
+    hn = np.empty(n + 1, dtype=np.uint32)
+    nc = L.orc_coarsen_hec_serial(level, n, rowptr, col, wp, seed, vc, vp, hn)
+    return vc[:n].copy(), (None if nc == 0xFFFFFFFF else int(nc)), vp[:n].copy(), hn[:n].copy()
+
+
 def gosh_clusters(rowptr, col):
     """(states (int32: 1 centre, -1 next to one), rounds) of GOSH_clusters
     (heuristics.h:102-217)."""
@@ -524,7 +541,7 @@ def partition_graph(rowptr, col, w, hec_seed: int, pcg_state: int, pcg_inc: int 
     """Whole bisection; dict(part, edge_cut, levels_n, iters, evec, pcg_state).
     variant: 0 HEC, 1 heavy-edge matching (match_*), 2 matching with the
     mt-metis passes (mtmetis_*), 4 HEC with the v2 map (hec2_*), 7 GOSH v2
-    (gosh2_*), 3 HEC with the v3 map (the
+    (gosh2_*), 8 the serial HEC (hec_spec_serial), 3 HEC with the v3 map (the
     hec3_* binaries), 5 MIS-2 (mis_*), 6 GOSH (gosh_*); builder 1: the HASHMAP
     build (hec_spec_hashmap)."""
     L = core().L

@@ -424,6 +424,67 @@ vid_t orc_coarsen_hec2(int level, vid_t n, const eid_t *rowptr, const vid_t *col
     return nc;
 }
 
+/* ------------------------------------------------------------------ */
+/* The serial HEC of hec_spec_serial (-DHEC_SERIAL,                     */
+/* sgp_coarsen_HEC_serial, heuristics.h:828-928): one generator of the  */
+/* pool of `seed` (std::time in the reference) shuffles the identity by */
+/* Fisher-Yates from the end (swap position urand64() % i with i - 1,   */
+/* i = n .. 2; :850-860), then draws each vertex's random neighbour at  */
+/* level 1 (:866-872); above level 1 the heavy neighbour is the first   */
+/* heaviest entry (strict <, :875-885).  The map walks the shuffled     */
+/* order: an unmapped u makes hn(u) a new id if it has none and joins   */
+/* it (:896-906).  Returns nc, or ORC_INF for a vertex without entries  */
+/* (the reference divides by zero or indexes with SGP_INFTY).           */
+/* ------------------------------------------------------------------ */
+vid_t orc_coarsen_hec_serial(int level, vid_t n, const eid_t *rowptr, const vid_t *col, const wgt_t *w,
+                             uint64_t seed, vid_t *vcmap, vid_t *vperm_out, vid_t *hn_out)
+{
+    uint64_t st = orc_pool_state(seed);
+    vid_t *vperm = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t *hn = (vid_t *)malloc((size_t)n * sizeof(vid_t) + 4);
+    vid_t nc = 0;
+    for (vid_t i = 0; i < n; i++) vperm[i] = i;
+    for (vid_t i = n; i > 1; i--) {
+        const vid_t j = (vid_t)(orc_xs_urand64(&st) % i);
+        const vid_t t = vperm[j];
+        vperm[j] = vperm[i - 1];
+        vperm[i - 1] = t;
+    }
+    for (vid_t i = 0; i < n; i++) {
+        const eid_t b = rowptr[i], deg = rowptr[i + 1] - b;
+        if (deg == 0) {
+            nc = ORC_INF;
+            goto done;
+        }
+        if (level == 1) {
+            hn[i] = col[b + (eid_t)(orc_xs_urand64(&st) % deg)];
+        } else {
+            vid_t h = ORC_INF;
+            wgt_t best = 0;
+            for (eid_t e = b; e < b + deg; e++)
+                if (best < (w ? w[e] : 1u)) {
+                    best = w ? w[e] : 1u;
+                    h = col[e];
+                }
+            hn[i] = h;
+        }
+    }
+    for (vid_t i = 0; i < n; i++) vcmap[i] = ORC_INF;
+    for (vid_t i = 0; i < n; i++) {
+        const vid_t u = vperm[i], v = hn[u];
+        if (vcmap[u] == ORC_INF) {
+            if (vcmap[v] == ORC_INF) vcmap[v] = nc++;
+            vcmap[u] = vcmap[v];
+        }
+    }
+done:
+    if (vperm_out) memcpy(vperm_out, vperm, (size_t)n * sizeof(vid_t));
+    if (hn_out && nc != ORC_INF) memcpy(hn_out, hn, (size_t)n * sizeof(vid_t));
+    free(vperm);
+    free(hn);
+    return nc;
+}
+
 /* One HEC level with the v3 map (sgp_coarsen_HEC with -DHEC_V3,
    heuristics.h:930-1031): the same pool, permutation draws and heavy
    neighbours as orc_coarsen_hec, then orc_map_construct_v3. */
@@ -1247,7 +1308,7 @@ orc_hier_t *orc_hierarchy_build(vid_t n, const eid_t *rowptr, const vid_t *col,
 
 /* The level loop with HEC (variant 0), heavy-edge matching (1), matching
    with the mt-metis passes (2), HEC with the v3 map (3) or the v2 map (4),
-   GOSH v2 (7), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
+   GOSH v2 (7), the serial HEC of hec_spec_serial (8), MIS-2 (5) or GOSH (6) in the low byte of variant; bit 8 set: the HASHMAP build
    (rows in first-occurrence order) in place of the RADIX one. */
 orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_t *col,
                                         const wgt_t *w, uint64_t seed, int variant)
@@ -1280,6 +1341,7 @@ orc_hier_t *orc_hierarchy_build_variant(vid_t n, const eid_t *rowptr, const vid_
                    : variant == 1 ? orc_coarsen_match(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 2 ? orc_coarsen_mtmetis(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
                    : variant == 4 ? orc_coarsen_hec2(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL)
+                   : variant == 8 ? orc_coarsen_hec_serial(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap, NULL, NULL)
                                   : orc_coarsen_hec(level, a->n, a->rowptr, a->col, a->w, seed, a->vcmap,
                                                     NULL, NULL);
         if (nc == ORC_INF) {  /* no map (matching on a row without entries): the levels so far */
