@@ -1119,12 +1119,17 @@ void bisect_many(Ctx& c, const std::vector<const DGraph*>& gs, const mi_config* 
     cur.clear();
     auto t2 = std::chrono::steady_clock::now();
     H.clear();
+    // ev[k] belongs to the pool of the worker that solved graph k, which is
+    // in general not the one that splits it: it is released only once every
+    // worker's stream has drained, from this thread (a block released
+    // earlier could be handed out by its pool to its own stream while the
+    // splitting worker's kernels still read it)
     run_on_workers(ws, K, [&](Ctx& w, size_t i) {
         const size_t k = by_size[i];
         edge_cuts[k] = compute_partition(w, *gs[k], ev[k].get(), config->num_partitions, perc_imbalance, d_parts[k]);
         if (stats) stats[k].edge_cut = edge_cuts[k];
-        ev[k] = DBuf<double>();
     });
+    ev.clear();
     MI_HIP(hipStreamSynchronize(c.stream));
     auto t3 = std::chrono::steady_clock::now();
     if (phase_ms) {

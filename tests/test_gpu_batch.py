@@ -57,3 +57,21 @@ def test_bisect_batch_resumed_launches_and_empty(ctx):
     assert empty == []
     for g in G:
         g.close()
+
+
+def test_bisect_batch_many_graphs_repeated(ctx):
+    """96 RGGs of 2,000 to 60,000 vertices, three batches in a row: every
+    graph's vector is solved on one worker context and usually split on
+    another, with the workers allocating meanwhile; each batch equals the
+    one-graph calls in partition and cut."""
+    rng = np.random.default_rng(12)
+    ns = np.exp(rng.uniform(np.log(2e3), np.log(6e4), 96)).astype(int)
+    G = [ctx.graph(*graphs.rgg(int(n), seed=500 + i)[:2]) for i, n in enumerate(ns)]
+    seeds, states = [1] * len(G), [1] * len(G)
+    single = [mi.partition_graph(ctx, g, hec_seed=1, pcg_state=1) for g in G]
+    for rep in range(3):
+        res, _ = mi.bisect_batch(ctx, G, hec_seeds=seeds, pcg_states=states)
+        for k, (a, b) in enumerate(zip(res, single)):
+            assert a.edge_cut == b.edge_cut and np.array_equal(a.part, b.part), (rep, k)
+    for g in G:
+        g.close()
