@@ -482,6 +482,8 @@ struct StepArgs {
     double* ax;              // FP64 loop: A x of every iteration's input row by row, or null
     const uint32_t* span;    // LDS form: the schedule table (see plan_level), or null (even split)
     uint32_t* sched;         // ... its two counters: next shared chunk, workgroups done
+    uint32_t dense;          // 1: part holds every (virtual) row's whole sum, zero where a row has no
+                             // entries in a block (levels of two or more column blocks; k_fold_rows)
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -1200,6 +1202,33 @@ __device__ __forceinline__ V row_total(const StepArgs& A, uint32_t r, uint32_t n
     return v;
 }
 
+// Rows that cross ranges, folded into part: range w owns the row that
+// leaves it (its partial in tail[w]) when that row began in w, and adds the
+// pieces of the following ranges the row enters (head[h], h = w+1 ..) in
+// range order -- the order in which the finish kernel adds them otherwise.
+// A range's first row began earlier exactly when its metadata has HEADF,
+// which never holds across a block boundary.
+template <class V>
+__global__ void __launch_bounds__(BLOCK) k_fold_rows(StepArgs A)
+{
+    if (A.mode == 0 && A.cnt[CT_DONE]) return;
+    const uint32_t w = blockIdx.x * BLOCK + threadIdx.x;
+    if (w + 1 >= A.nwaves) return;
+    const uint32_t mn = A.wmeta[w + 1];
+    if (!(mn & HEADF)) return;
+    const uint32_t row = mn & ~HEADF;
+    const uint32_t mw = A.wmeta[w];
+    if ((mw & HEADF) && (mw & ~HEADF) == row) return;  // the row began before range w
+    const V* head = static_cast<const V*>(A.head);
+    V v = static_cast<const V*>(A.tail)[w];
+    for (uint32_t h = w + 1; h < A.nwaves; h++) {
+        const uint32_t mh = A.wmeta[h];
+        if (!(mh & HEADF) || (mh & ~HEADF) != row) break;
+        v += head[h];
+    }
+    static_cast<V*>(A.part)[row] = v;
+}
+
 // Row sums over the column blocks, the shift, y and the four sums per
 // workgroup (fixed grid of at most MAXF workgroups, rows strided).
 // V: the value type of the vectors and row partials.  The partials of a
@@ -1240,6 +1269,23 @@ __global__ void __launch_bounds__(BLOCK) k_power_finish(StepArgs A)
     };
     const uint32_t stride = gridDim.x * BLOCK;
     uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (A.dense) {
+        // every row's block sums in block order, eight loads in flight (a
+        // block without entries of the row holds 0)
+        for (; r < A.n; r += stride) {
+            V v = part[r];
+            uint32_t q = 1;
+            for (; q + 8 <= nb; q += 8) {
+                V h[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) h[j] = part[(q + j) * A.n + r];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v += h[j];
+            }
+            for (; q < nb; q++) v += part[q * A.n + r];
+            emit(r, yp[r], A.wdeg ? A.wdeg[r] : 0u, v);
+        }
+    }
     if (!MANY && nb == 1) {
         // one block: four rows at a time, their loads in flight together (a
         // thread's rows stay in the same order, so the sums do not change)
@@ -2045,6 +2091,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.part = p.part.get();
     A.head = p.head.get();
     A.tail = p.tail.get();
+    A.dense = p.ncb >= 2 ? 1u : 0u;
     A.nblk_a = p.nblk_a;
     A.nblk_f = p.nblk_f;
     A.fprev = p.fsum.get();
@@ -2159,6 +2206,14 @@ void launch_finish(Ctx& c, int normLap, const StepArgs& A)
 void launch_step(Ctx& c, int normLap, const StepArgs& A)
 {
     launch_stream(c, A);
+    if (A.dense) {
+        const unsigned g = (unsigned)ceil_div(A.nwaves ? A.nwaves : 1u, BLOCK);
+        if (A.f32)
+            k_fold_rows<float><<<g, BLOCK, 0, c.stream>>>(A);
+        else
+            k_fold_rows<double><<<g, BLOCK, 0, c.stream>>>(A);
+        MI_HIP(hipGetLastError());
+    }
     launch_finish(c, normLap, A);
 }
 
@@ -2420,6 +2475,9 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         }
     }
     p.part = DBuf<double>(&c.pool, (size_t)nrows);
+    // levels of several column blocks: the (row, block) pairs without
+    // entries are never written and read as 0 by the finish kernel
+    if (ncb >= 2) fill_u32(c, reinterpret_cast<uint32_t*>(p.part.get()), 2ull * nrows, 0u);
     p.head = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     p.tail = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     // one row per finish thread where the streaming kernel can reduce that
