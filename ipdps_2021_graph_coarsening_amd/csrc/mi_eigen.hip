@@ -483,7 +483,7 @@ struct StepArgs {
     const uint32_t* span;    // LDS form: the schedule table (see plan_level), or null (even split)
     uint32_t* sched;         // ... its two counters: next shared chunk, workgroups done
     uint32_t dense;          // 1: part holds every (virtual) row's whole sum, zero where a row has no
-                             // entries in a block (levels of two or more column blocks; k_fold_rows)
+                             // entries in a block (large levels of several column blocks; k_fold_rows)
 };
 
 // Input transform of this iteration from the previous one's four sums, and
@@ -2091,7 +2091,7 @@ StepArgs make_args(Ctx& c, const DGraph& g, const LevelPlan& p, const uint32_t* 
     A.part = p.part.get();
     A.head = p.head.get();
     A.tail = p.tail.get();
-    A.dense = p.ncb >= 2 ? 1u : 0u;
+    A.dense = p.dense ? 1u : 0u;
     A.nblk_a = p.nblk_a;
     A.nblk_f = p.nblk_f;
     A.fprev = p.fsum.get();
@@ -2475,9 +2475,12 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         }
     }
     p.part = DBuf<double>(&c.pool, (size_t)nrows);
-    // levels of several column blocks: the (row, block) pairs without
-    // entries are never written and read as 0 by the finish kernel
-    if (ncb >= 2) fill_u32(c, reinterpret_cast<uint32_t*>(p.part.get()), 2ull * nrows, 0u);
+    // levels of several column blocks and at least 2^22 (row, block) pairs:
+    // the finish kernel reads the row sums densely (the pairs without
+    // entries are never written and read as 0), which saves the virtual row
+    // offsets' 4 bytes per pair for the cost of one more launch per step
+    p.dense = ncb >= 2 && nrows >= (1u << 22);
+    if (p.dense) fill_u32(c, reinterpret_cast<uint32_t*>(p.part.get()), 2ull * nrows, 0u);
     p.head = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     p.tail = DBuf<double>(&c.pool, p.nwaves ? p.nwaves : 1);
     // one row per finish thread where the streaming kernel can reduce that

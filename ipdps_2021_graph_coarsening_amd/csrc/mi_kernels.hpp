@@ -145,6 +145,7 @@ struct LevelPlan {
     DBuf<uint32_t> col_al, w_al;  // block layout, or 16-byte aligned copies
     DBuf<double> part, head, tail, fsum;
     uint32_t ncb = 0, bw = 0;
+    bool dense = false;  // part holds whole row sums, 0 for (row, block) pairs without entries
     uint32_t boff[MAX_COLUMN_BLOCKS + 1] = {}, bwave[MAX_COLUMN_BLOCKS + 1] = {};
     DBuf<uint32_t> blk;  // boff then bwave on the device, when more than 8 blocks
     uint32_t pw = 0, nwaves = 0;
