@@ -956,6 +956,7 @@ static void keys_to_csr(Ctx& c, DBuf<K>& keys, DBuf<uint32_t>& vals, uint64_t m,
         }
     }
     scan_exclusive_u32(c, rowcnt.get(), out.rowptr.get(), (uint64_t)nc + 1, nullptr);
+    out.col_sorted = !first_order;  // rows by key: columns ascending
 }
 
 // the largest weight of a level (one atomic per wavefront)

@@ -2263,11 +2263,13 @@ void plan_level(Ctx& c, const DGraph& g, LevelPlan& p, bool f32)
         if (!g.unit) p.w_al = DBuf<uint32_t>(&c.pool, m);
         DBuf<uint32_t> roc0, flag(&c.pool, 1);
         build_row_index(c, g, roc0);
-        MI_HIP(hipMemsetAsync(flag.get(), 0, 4, c.stream));
-        k_unsorted<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc0.get(), g.R(), g.Cc(), flag.get());
-        MI_HIP(hipGetLastError());
         uint32_t unsorted = 0;
-        read_words(c, flag.get(), &unsorted, 4);
+        if (!g.col_sorted) {  // (levels of the radix coarse build are sorted by construction)
+            MI_HIP(hipMemsetAsync(flag.get(), 0, 4, c.stream));
+            k_unsorted<<<grid1(m), BLOCK, 0, c.stream>>>(m, roc0.get(), g.R(), g.Cc(), flag.get());
+            MI_HIP(hipGetLastError());
+            read_words(c, flag.get(), &unsorted, 4);
+        }
         if (!unsorted) {
             // flat over the entries
             DBuf<uint32_t> vstart(&c.pool, nv);

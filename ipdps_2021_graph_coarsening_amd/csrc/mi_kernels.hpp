@@ -16,6 +16,7 @@ struct DGraph {
     uint32_t n = 0;
     uint32_t m = 0;
     bool unit = true;
+    bool col_sorted = false;  // every row's columns ascend (set by the radix coarse build)
     DBuf<uint32_t> rowptr, col, w;
     // borrowed device arrays (caller-owned) when the graph wraps device memory
     const uint32_t* b_rowptr = nullptr;
