@@ -292,6 +292,24 @@ def test_power_iter_many_column_blocks_vs_oracle(ctx):
     assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
 
 
+def test_power_iter_dense_block_sums_vs_oracle(ctx):
+    """A level of 16 column blocks and 300,000 vertices (4.8M row-block
+    pairs, above the 2^22 from which the finish kernel reads the block sums
+    densely after the rows that cross ranges are folded in): the whole power
+    loop twice (identical) and against the oracle."""
+    rp, col, w = _dense_graph(300000, 66, 23, True)
+    g = ctx.graph(rp, col, w)
+    ncb = g.bench_step(1)["column_blocks"]
+    assert ncb == 16 and ncb * (len(rp) - 1) >= 1 << 22
+    u0 = np.random.default_rng(24).random(len(rp) - 1)
+    u, it, _ = g.power_iter(u0, normLap=0, tol=1e-10)
+    u2, it2, _ = g.power_iter(u0, normLap=0, tol=1e-10)
+    assert it == it2 and np.array_equal(u, u2)
+    ou, oit, _, _ = oracle.power_iter(rp, col, w, u0, normLap=0, tol=1e-10)
+    assert abs(it - oit) <= max(2, ITER_SLACK * oit)
+    assert abs(u @ ou) / (np.linalg.norm(u) * np.linalg.norm(ou)) >= EV_COS
+
+
 @pytest.mark.parametrize("normLap", [0, 1])
 def test_power_iter_graph_replay_equals_eager(ctx, normLap):
     """Iterations replayed from the captured HIP graph give exactly what
